@@ -1,0 +1,134 @@
+/*
+ * msgs_hip.h -- C-ABI of libmsgs_hip.so, the MI355X (gfx950) read-mapping hot path
+ * of MSGseqTK.  Plain pointers and sizes only; every call returns 0 or a negative
+ * MSGS_E* code and never throws.  A context is used by one host thread at a time.
+ *
+ * The reference has no FFI: its boundary is the C++ class API in namespace
+ * EGriceLab::MSGseqTK.  Each entry point below names the reference interface
+ * it replaces (paths relative to the reference source tree).
+ */
+#ifndef MSGS_HIP_H
+#define MSGS_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    MSGS_OK = 0,
+    MSGS_E_ARG = -1,        /* bad argument */
+    MSGS_E_HIP = -2,        /* HIP runtime error */
+    MSGS_E_NOMEM = -3,      /* device or host allocation failed */
+    MSGS_E_STATE = -4,      /* call out of order (e.g. no database) */
+    MSGS_E_DB = -5,         /* database refused (not bi-directional / IUPAC-extended, like
+                               FMDIndex::buildCounts, src/FMDIndex.cpp:118-134) */
+    MSGS_E_CAP = -6,        /* caller buffer too small; *needed reports the size */
+    MSGS_E_UNSUPPORTED = -7 /* parameters outside what the device path implements */
+};
+
+typedef struct msgs_ctx msgs_ctx;
+
+/* Mapping parameters; defaults are the reference's (src/SMEM.h:204-207,
+ * src/SMEM.cpp:17, src/ScoreScheme.cpp:14-18, src/PairingScheme.cpp:14-16,
+ * src/msgseqtk-align.cpp:59-62).  mode 0 = GLOBAL (Needleman-Wunsch), 1 = LOCAL. */
+typedef struct {
+    int64_t minLen; double maxEvalue; int64_t maxNSeed;
+    int32_t mode; int32_t maxReport;
+    double match, mismatch, gapO, gapE, clip;
+    double minScoreRate;
+    double insMean, insSD, insMin, insMax;
+    int32_t noMixed, noDiscordant, noTailOver, noContain, noOverlap, pad0;
+    int64_t maxNPair;
+} msgs_params;
+
+void msgs_params_default(msgs_params* p);
+
+/* A batch of reads: nt16 codes (A=1 C=2 G=4 T=8 ...) concatenated, Phred scores
+ * (already minus 33, >= 2) in the same layout (may be NULL: all 30), offsets[n+1],
+ * and names ('\n'-separated, may be NULL). */
+typedef struct {
+    const uint8_t* seq;
+    const uint8_t* qual;
+    const int64_t* offs;
+    const char* names;
+    int64_t n;
+} msgs_reads;
+
+/* ---- context ---- */
+int msgs_ctx_create(int device, msgs_ctx** out);
+void msgs_ctx_destroy(msgs_ctx* ctx);
+const char* msgs_last_error(msgs_ctx* ctx);
+
+/* ---- database: MetaGenome + FMDIndex resident in HBM ----
+ * Replaces MetaGenome::load/loadSeq + FMDIndex::load (src/MetaGenome.cpp:42-73,
+ * src/FMDIndex.cpp:153-161) and, for building, FMDIndex(const DNAseq&, ...)
+ * (src/FMDIndex.cpp:29-116) over MetaGenome::getBDSeq (src/MetaGenome.cpp:104-113).
+ * The suffix array is built on the device.  names: '\n'-separated chrom names. */
+int msgs_db_build(msgs_ctx* ctx, const uint8_t* chromSeqs, const int64_t* chromLens, int32_t nChrom,
+                  const char* names);
+/* FMDIndex::length / getBaseCount / getCumCount (src/FMDIndex.h:152-179) */
+int64_t msgs_db_length(msgs_ctx* ctx);
+int msgs_db_counts(msgs_ctx* ctx, int64_t* B256, int64_t* C256);
+/* Copies of the BD text, BWT and suffix array (any may be NULL), for parity checks. */
+int msgs_db_copy(msgs_ctx* ctx, uint8_t* bd, uint8_t* bwt, int64_t* sa);
+/* FMDIndex::loglik (src/FMDIndex.h:195-197) */
+double msgs_db_loglik(msgs_ctx* ctx, int32_t base);
+
+/* ---- FMD primitives, batched (device) ----
+ * rank(s, i) of WaveletTreeRRR (src/libsds/src/WaveletTreeRRR.cpp:101-125): #s in BWT[0..i]. */
+int msgs_rank_batch(msgs_ctx* ctx, const uint8_t* sym, const int64_t* pos, int64_t* out, int64_t n);
+/* FMDIndex::backExt / fwdExt (src/FMDIndex.cpp:577-610, src/FMDIndex.h:226-228):
+ * pqs is n rows of (p, q, s), updated in place; ret[i] = new s. */
+int msgs_ext_batch(msgs_ctx* ctx, int64_t* pqs, const uint8_t* bases, int32_t forward, int64_t* ret, int64_t n);
+
+/* ---- seeding ----
+ * SMEM_LIST::findAllSMEMS (src/SMEM.cpp:186-215), one read per lane.  Per read i,
+ * counts[i] SMEMs in emission order; rows5 = (from, to, fwdStart, revStart, size).
+ * Returns MSGS_E_CAP with *needed set when cap rows are not enough. */
+int msgs_smem_batch(msgs_ctx* ctx, const msgs_reads* reads, const msgs_params* p, int64_t* counts,
+                    int64_t* rows5, double* logP, int64_t cap, int64_t* needed);
+/* SMEM_LIST::findSeeds (src/SMEM.cpp:271-305) incl. SeedPair::removeRedundant
+ * (src/SeedPair.cpp:71-88).  rows6 = (from, to, start, end, tid, strand). */
+int msgs_seed_batch(msgs_ctx* ctx, const msgs_reads* reads, const msgs_params* p, int64_t* counts,
+                    int64_t* rows6, double* logP, int64_t cap, int64_t* needed);
+
+/* ---- whole path ----
+ * msgseqtk-align main_SE / main_PE (src/msgseqtk-align.cpp:595-758): seeds, chains
+ * (SeedChain::getChains, src/SeedChain.cpp:40-73), Alignment::buildAlignments
+ * (src/Alignment.cpp:420-427), filter/evaluate/calcMapQ/sort and record output.
+ * mates == NULL selects single-end.  SAM text of read (pair) i is
+ * out[lineOffs[i] .. lineOffs[i+1]).  *needed reports the bytes required. */
+int msgs_map_batch(msgs_ctx* ctx, const msgs_reads* reads, const msgs_reads* mates, const msgs_params* p,
+                   char* out, int64_t cap, int64_t* lineOffs, int64_t* needed);
+
+/* Per-chain alignments (for parity checks of the DP): per read counts[i] rows of
+ * (tid, strand, tStart, tEnd, alnFrom, alnTo, alnStart, alnEnd, hazard, cells), score,
+ * and CIGAR strings joined by '\n' into cigars (cigCap bytes). */
+int msgs_align_batch(msgs_ctx* ctx, const msgs_reads* reads, const msgs_params* p, int64_t* counts,
+                     int64_t* rows10, double* scores, char* cigars, int64_t cigCap, int64_t cap,
+                     int64_t* needed);
+
+/* ---- resident batches (bench): reads copied to HBM once, mapped many times ----
+ * slot 0..3.  msgs_batch_run maps the resident batch and keeps the records on the
+ * host side of the context; msgs_batch_stats reports counters of the last run:
+ * stats[0] reads, [1] mapped reads, [2] extension calls, [3] SA lookups, [4] DP cells,
+ * [5] seeding kernel microseconds (HIP events), [6] DP kernel microseconds,
+ * [7] whole-run microseconds, [8] seed-stage microseconds, [9] host finishing microseconds. */
+int msgs_batch_upload(msgs_ctx* ctx, int32_t slot, const msgs_reads* reads, const msgs_reads* mates);
+int msgs_batch_run(msgs_ctx* ctx, int32_t slot, const msgs_params* p, int32_t emitSam);
+int msgs_batch_stats(msgs_ctx* ctx, double* stats, int32_t n);
+int msgs_batch_sam(msgs_ctx* ctx, char* out, int64_t cap, int64_t* needed);
+
+/* ---- utilities ----
+ * The permutation std::sort (libstdc++) gives for n keys compared with '<' -- the
+ * reference's unstable sorts (src/SMEM.cpp:275-276, src/SeedPair.cpp:75,
+ * src/Alignment.h:629-633) as the library restates them for the device; runs on
+ * the host.  perm must hold 0..n-1 on entry. */
+void msgs_util_sort_by_key(const double* keys, int32_t* perm, int64_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MSGS_HIP_H */

@@ -1,0 +1,292 @@
+"""msgseqtk_amd -- MI355X (gfx950) read-mapping hot path of MSGseqTK.
+
+The compute runs in libmsgs_hip.so (HIP kernels behind the C-ABI declared in
+include/msgs_hip.h).  This module is thin ctypes plumbing around that library for
+tests and benchmarks; there is no CPU fallback: importing the product path without
+the built library raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmsgs_hip.so")
+
+P64 = C.POINTER(C.c_int64)
+PD = C.POINTER(C.c_double)
+PU8 = C.POINTER(C.c_uint8)
+
+MSGS_E_CAP = -6
+
+# every extern "C" symbol of include/msgs_hip.h
+EXPORTS = [
+    "msgs_params_default", "msgs_ctx_create", "msgs_ctx_destroy", "msgs_last_error",
+    "msgs_db_build", "msgs_db_length", "msgs_db_counts", "msgs_db_copy", "msgs_db_loglik",
+    "msgs_rank_batch", "msgs_ext_batch", "msgs_smem_batch", "msgs_seed_batch",
+    "msgs_map_batch", "msgs_align_batch", "msgs_batch_upload", "msgs_batch_run",
+    "msgs_batch_stats", "msgs_batch_sam", "msgs_util_sort_by_key",
+]
+
+
+class Params(C.Structure):
+    """msgs_params (include/msgs_hip.h)."""
+    _fields_ = [("minLen", C.c_int64), ("maxEvalue", C.c_double), ("maxNSeed", C.c_int64),
+                ("mode", C.c_int32), ("maxReport", C.c_int32),
+                ("match", C.c_double), ("mismatch", C.c_double), ("gapO", C.c_double),
+                ("gapE", C.c_double), ("clip", C.c_double), ("minScoreRate", C.c_double),
+                ("insMean", C.c_double), ("insSD", C.c_double), ("insMin", C.c_double),
+                ("insMax", C.c_double),
+                ("noMixed", C.c_int32), ("noDiscordant", C.c_int32), ("noTailOver", C.c_int32),
+                ("noContain", C.c_int32), ("noOverlap", C.c_int32), ("pad0", C.c_int32),
+                ("maxNPair", C.c_int64)]
+
+
+class Reads(C.Structure):
+    """msgs_reads (include/msgs_hip.h)."""
+    _fields_ = [("seq", PU8), ("qual", PU8), ("offs", P64), ("names", C.c_char_p), ("n", C.c_int64)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libmsgs_hip.so; raises when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libmsgs_hip.so is not built (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.msgs_params_default.argtypes = [C.POINTER(Params)]
+    L.msgs_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+    L.msgs_ctx_destroy.argtypes = [vp]
+    L.msgs_last_error.argtypes = [vp]
+    L.msgs_last_error.restype = C.c_char_p
+    L.msgs_db_build.argtypes = [vp, PU8, P64, C.c_int32, C.c_char_p]
+    L.msgs_db_length.argtypes = [vp]
+    L.msgs_db_length.restype = C.c_int64
+    L.msgs_db_counts.argtypes = [vp, P64, P64]
+    L.msgs_db_copy.argtypes = [vp, PU8, PU8, P64]
+    L.msgs_db_loglik.argtypes = [vp, C.c_int32]
+    L.msgs_db_loglik.restype = C.c_double
+    L.msgs_rank_batch.argtypes = [vp, PU8, P64, P64, C.c_int64]
+    L.msgs_ext_batch.argtypes = [vp, P64, PU8, C.c_int32, P64, C.c_int64]
+    L.msgs_smem_batch.argtypes = [vp, C.POINTER(Reads), C.POINTER(Params), P64, P64, PD, C.c_int64, P64]
+    L.msgs_seed_batch.argtypes = [vp, C.POINTER(Reads), C.POINTER(Params), P64, P64, PD, C.c_int64, P64]
+    L.msgs_map_batch.argtypes = [vp, C.POINTER(Reads), C.POINTER(Reads), C.POINTER(Params), C.c_char_p,
+                                 C.c_int64, P64, P64]
+    L.msgs_align_batch.argtypes = [vp, C.POINTER(Reads), C.POINTER(Params), P64, P64, PD, C.c_char_p,
+                                   C.c_int64, C.c_int64, P64]
+    L.msgs_batch_upload.argtypes = [vp, C.c_int32, C.POINTER(Reads), C.POINTER(Reads)]
+    L.msgs_batch_run.argtypes = [vp, C.c_int32, C.POINTER(Params), C.c_int32]
+    L.msgs_batch_stats.argtypes = [vp, PD, C.c_int32]
+    L.msgs_batch_sam.argtypes = [vp, C.c_char_p, C.c_int64, P64]
+    L.msgs_util_sort_by_key.argtypes = [PD, C.POINTER(C.c_int32), C.c_int64]
+    _lib = L
+    return L
+
+
+def default_params(**kw):
+    p = Params()
+    lib().msgs_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class ReadBatch:
+    """Reads packed for the C-ABI; keeps the numpy buffers alive."""
+
+    def __init__(self, seqs, quals=None, names=None):
+        self.n = len(seqs)
+        self.offs = np.zeros(self.n + 1, np.int64)
+        if self.n:
+            self.offs[1:] = np.cumsum([len(s) for s in seqs])
+        self.seq = np.ascontiguousarray(np.concatenate(seqs) if self.n else np.zeros(0, np.uint8), np.uint8)
+        if self.seq.size == 0:
+            self.seq = np.zeros(1, np.uint8)
+        self.qual = None
+        if quals is not None:
+            self.qual = np.ascontiguousarray(np.concatenate(quals) if self.n else np.zeros(1, np.uint8), np.uint8)
+            if self.qual.size == 0:
+                self.qual = np.zeros(1, np.uint8)
+        self.names = ("\n".join(names)).encode() if names is not None else None
+        self.st = Reads(_p(self.seq, C.c_uint8), _p(self.qual, C.c_uint8) if self.qual is not None else None,
+                        _p(self.offs, C.c_int64), self.names, self.n)
+
+
+class Mapper:
+    """A device context with a resident database (one GPU)."""
+
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        rc = lib().msgs_ctx_create(device, C.byref(self.h))
+        if rc != 0:
+            raise RuntimeError("msgs_ctx_create failed (%d): no usable GPU" % rc)
+
+    def close(self):
+        if self.h:
+            lib().msgs_ctx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc, what):
+        if rc != 0:
+            raise RuntimeError("%s failed (%d): %s" % (what, rc, lib().msgs_last_error(self.h).decode()))
+
+    def build(self, chroms, names=None):
+        chroms = [np.ascontiguousarray(c, np.uint8) for c in chroms]
+        lens = np.array([len(c) for c in chroms], np.int64)
+        seqs = np.ascontiguousarray(np.concatenate(chroms), np.uint8)
+        nm = "\n".join(names or ["g%04d" % i for i in range(len(chroms))]).encode()
+        self._chk(lib().msgs_db_build(self.h, _p(seqs, C.c_uint8), _p(lens, C.c_int64), len(chroms), nm),
+                  "msgs_db_build")
+        self.n = lib().msgs_db_length(self.h)
+        return self
+
+    def counts(self):
+        B = np.zeros(256, np.int64)
+        Cc = np.zeros(256, np.int64)
+        self._chk(lib().msgs_db_counts(self.h, _p(B, C.c_int64), _p(Cc, C.c_int64)), "msgs_db_counts")
+        return B, Cc
+
+    def arrays(self):
+        bd = np.zeros(self.n, np.uint8)
+        bwt = np.zeros(self.n, np.uint8)
+        sa = np.zeros(self.n, np.int64)
+        self._chk(lib().msgs_db_copy(self.h, _p(bd, C.c_uint8), _p(bwt, C.c_uint8), _p(sa, C.c_int64)),
+                  "msgs_db_copy")
+        return bd, bwt, sa
+
+    def loglik(self, b):
+        return lib().msgs_db_loglik(self.h, b)
+
+    def rank(self, syms, pos):
+        syms = np.ascontiguousarray(syms, np.uint8)
+        pos = np.ascontiguousarray(pos, np.int64)
+        out = np.zeros(len(pos), np.int64)
+        self._chk(lib().msgs_rank_batch(self.h, _p(syms, C.c_uint8), _p(pos, C.c_int64), _p(out, C.c_int64),
+                                        len(pos)), "msgs_rank_batch")
+        return out
+
+    def ext(self, pqs, bases, forward):
+        pqs = np.ascontiguousarray(pqs, np.int64).copy()
+        bases = np.ascontiguousarray(bases, np.uint8)
+        ret = np.zeros(len(bases), np.int64)
+        self._chk(lib().msgs_ext_batch(self.h, _p(pqs, C.c_int64), _p(bases, C.c_uint8), int(forward),
+                                       _p(ret, C.c_int64), len(bases)), "msgs_ext_batch")
+        return pqs, ret
+
+    def smems(self, seqs, params):
+        rb = ReadBatch(seqs)
+        counts = np.zeros(max(rb.n, 1), np.int64)
+        need = C.c_int64(0)
+        cap = max(16, 8 * rb.n)
+        while True:
+            rows = np.zeros((cap, 5), np.int64)
+            lp = np.zeros(cap, np.float64)
+            rc = lib().msgs_smem_batch(self.h, C.byref(rb.st), C.byref(params), _p(counts, C.c_int64),
+                                       _p(rows, C.c_int64), _p(lp, C.c_double), cap, C.byref(need))
+            if rc == MSGS_E_CAP:
+                cap = need.value
+                continue
+            self._chk(rc, "msgs_smem_batch")
+            k = need.value
+            return counts[:rb.n].copy(), rows[:k].copy(), lp[:k].copy()
+
+    def seeds(self, seqs, params):
+        rb = ReadBatch(seqs)
+        counts = np.zeros(max(rb.n, 1), np.int64)
+        need = C.c_int64(0)
+        cap = max(16, 8 * rb.n)
+        while True:
+            rows = np.zeros((cap, 6), np.int64)
+            lp = np.zeros(cap, np.float64)
+            rc = lib().msgs_seed_batch(self.h, C.byref(rb.st), C.byref(params), _p(counts, C.c_int64),
+                                       _p(rows, C.c_int64), _p(lp, C.c_double), cap, C.byref(need))
+            if rc == MSGS_E_CAP:
+                cap = need.value
+                continue
+            self._chk(rc, "msgs_seed_batch")
+            k = need.value
+            return counts[:rb.n].copy(), rows[:k].copy(), lp[:k].copy()
+
+    def alignments(self, seqs, quals, params):
+        rb = ReadBatch(seqs, quals)
+        counts = np.zeros(max(rb.n, 1), np.int64)
+        need = C.c_int64(0)
+        cap = max(16, 4 * rb.n)
+        while True:
+            rows = np.zeros((cap, 10), np.int64)
+            sc = np.zeros(cap, np.float64)
+            cig = C.create_string_buffer(64 * cap + 64)
+            rc = lib().msgs_align_batch(self.h, C.byref(rb.st), C.byref(params), _p(counts, C.c_int64),
+                                        _p(rows, C.c_int64), _p(sc, C.c_double), cig, len(cig), cap,
+                                        C.byref(need))
+            if rc == MSGS_E_CAP:
+                cap = need.value
+                continue
+            self._chk(rc, "msgs_align_batch")
+            k = need.value
+            cigars = cig.value.decode().split("\n")[:k]
+            return counts[:rb.n].copy(), rows[:k].copy(), sc[:k].copy(), cigars
+
+    def map(self, reads, params, mates=None):
+        """reads/mates: (seqs, quals, names).  Returns the SAM text per read (pair)."""
+        rb = ReadBatch(*reads)
+        mb = ReadBatch(*mates) if mates is not None else None
+        need = C.c_int64(0)
+        lo = np.zeros(rb.n + 1, np.int64)
+        cap = 1024 * max(rb.n, 1)
+        while True:
+            buf = C.create_string_buffer(cap)
+            rc = lib().msgs_map_batch(self.h, C.byref(rb.st), C.byref(mb.st) if mb else None, C.byref(params),
+                                      buf, cap, _p(lo, C.c_int64), C.byref(need))
+            if rc == MSGS_E_CAP:
+                cap = need.value + 1
+                continue
+            self._chk(rc, "msgs_map_batch")
+            text = buf.raw[:need.value].decode()
+            return [text[lo[i]:lo[i + 1]] for i in range(rb.n)]
+
+    # resident batches
+    def upload(self, slot, reads, mates=None):
+        rb = ReadBatch(*reads)
+        mb = ReadBatch(*mates) if mates is not None else None
+        self._chk(lib().msgs_batch_upload(self.h, slot, C.byref(rb.st), C.byref(mb.st) if mb else None),
+                  "msgs_batch_upload")
+
+    def run(self, slot, params, emit_sam=True):
+        self._chk(lib().msgs_batch_run(self.h, slot, C.byref(params), int(emit_sam)), "msgs_batch_run")
+
+    def stats(self):
+        s = np.zeros(16, np.float64)
+        self._chk(lib().msgs_batch_stats(self.h, _p(s, C.c_double), 16), "msgs_batch_stats")
+        return s
+
+    def sam(self):
+        need = C.c_int64(0)
+        rc = lib().msgs_batch_sam(self.h, None, 0, C.byref(need))
+        buf = C.create_string_buffer(need.value + 1)
+        self._chk(lib().msgs_batch_sam(self.h, buf, need.value + 1, C.byref(need)), "msgs_batch_sam")
+        return buf.raw[:need.value].decode()
+
+
+def sort_by_key(keys):
+    """Permutation the reference's std::sort produces for keys compared with '<'
+    (the sort restatement of the library, run on the host)."""
+    keys = np.ascontiguousarray(keys, np.float64)
+    perm = np.arange(len(keys), dtype=np.int32)
+    lib().msgs_util_sort_by_key(_p(keys, C.c_double), _p(perm, C.c_int32), len(keys))
+    return perm

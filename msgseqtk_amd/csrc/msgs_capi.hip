@@ -1,0 +1,440 @@
+/*
+ * msgs_capi.hip -- the extern "C" face of libmsgs_hip.so: context, resident
+ * database, FMD primitives and the seeding batch.  See include/msgs_hip.h.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "msgs_capi_impl.h"
+
+/* ------------------------------------------------------------------ state */
+
+void DbState::release() {
+    void* ptrs[] = {dBd, dSa, dBwt, dOcc, dNPos, dChromBD, dTgtOff, dChromLen, dTgt};
+    for (void* p : ptrs) if (p) (void)hipFree(p);
+    dBd = nullptr; dSa = nullptr; dBwt = nullptr; dOcc = nullptr; dNPos = nullptr;
+    dChromBD = nullptr; dTgtOff = nullptr; dChromLen = nullptr; dTgt = nullptr;
+    n = 0;
+}
+
+DevIndex DbState::view() const {
+    DevIndex v;
+    v.occ = dOcc; v.sa = dSa; v.nPos = dNPos; v.nN = nN; v.n = n;
+    for (int i = 0; i < 17; ++i) v.C[i] = C[i];
+    for (int i = 0; i < 16; ++i) v.ll[i] = ll[i];
+    v.chromBD = dChromBD; v.nChrom = nChrom; v.tgt = dTgt; v.tgtOff = dTgtOff; v.chromLen = dChromLen;
+    return v;
+}
+
+int fail(msgs_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+/* ------------------------------------------------------------------ e-value thresholds */
+
+static uint64_t okey(double x) {
+    uint64_t u;
+    std::memcpy(&u, &x, 8);
+    return (u >> 63) ? ~u : (u | (1ull << 63));
+}
+static double fromKey(uint64_t k) {
+    uint64_t u = (k >> 63) ? (k & ~(1ull << 63)) : ~k;
+    double x;
+    std::memcpy(&x, &u, 8);
+    return x;
+}
+
+/* Largest double T with (double)n * exp(T) <= maxE, using the host libm exp that
+ * the reference calls (SMEM::evalue, src/SMEM.h:104-110).  NaN when no logP passes,
+ * +inf when every logP passes.  Returns false if the predicate is not monotone in
+ * a 1000-ulp neighbourhood of T. */
+bool evalueThreshold(double n, double maxE, double* T) {
+    auto ok = [&](double x) { return n * std::exp(x) <= maxE; };
+    const double inf = std::numeric_limits<double>::infinity();
+    if (std::isnan(maxE) || !ok(-inf)) { *T = std::numeric_limits<double>::quiet_NaN(); return true; }
+    if (ok(inf)) { *T = inf; return true; }
+    uint64_t lo = okey(-DBL_MAX), hi = okey(DBL_MAX);
+    if (!ok(-DBL_MAX)) { *T = -inf; return true; }
+    while (hi - lo > 1) {
+        uint64_t mid = lo + (hi - lo) / 2;
+        if (ok(fromKey(mid))) lo = mid; else hi = mid;
+    }
+    for (int d = 1; d <= 1000; ++d) {
+        if (!ok(fromKey(lo - d))) return false;
+        if (ok(fromKey(lo + d))) return false;
+    }
+    *T = fromKey(lo);
+    return true;
+}
+
+/* ------------------------------------------------------------------ small kernels */
+
+__global__ void k_rank(DevIndex ix, const uint8_t* sym, const int64_t* pos, int64_t* out, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int64_t r[5];
+    occ5(ix, pos[i], r);
+    uint8_t s = sym[i];
+    out[i] = s == 0 ? r[0] : s == 1 ? r[1] : s == 2 ? r[2] : s == 4 ? r[3] : s == 8 ? r[4] : INT64_MIN;
+}
+
+__global__ void k_ext(DevIndex ix, int64_t* pqs, const uint8_t* bases, int32_t fwd, int64_t* ret, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int64_t p = pqs[3 * i], q = pqs[3 * i + 1], s = pqs[3 * i + 2];
+    ret[i] = fwd ? dFwdExt(ix, p, q, s, bases[i]) : dBackExt(ix, p, q, s, bases[i]);
+    pqs[3 * i] = p; pqs[3 * i + 1] = q; pqs[3 * i + 2] = s;
+}
+
+static inline unsigned gridFor(int64_t n, int t = 256) { return (unsigned)std::max<int64_t>(1, (n + t - 1) / t); }
+
+/* ------------------------------------------------------------------ context */
+
+extern "C" {
+
+void msgs_params_default(msgs_params* p) {
+    std::memset(p, 0, sizeof(*p));
+    p->minLen = 17; p->maxEvalue = 0.01; p->maxNSeed = 500;
+    p->mode = 0; p->maxReport = 1;
+    p->match = 1; p->mismatch = 3; p->gapO = 5; p->gapE = 2; p->clip = 4;
+    p->minScoreRate = 0.65;
+    p->insMean = 200; p->insSD = 20; p->insMin = 0; p->insMax = 400;
+    p->maxNPair = 0;
+}
+
+int msgs_ctx_create(int device, msgs_ctx** out) {
+    if (!out) return MSGS_E_ARG;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= device || device < 0) return MSGS_E_HIP;
+    if (hipSetDevice(device) != hipSuccess) return MSGS_E_HIP;
+    msgs_ctx* c = new msgs_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->nCU = prop.multiProcessorCount;
+    if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return MSGS_E_HIP; }
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess) {
+        delete c;
+        return MSGS_E_HIP;
+    }
+    *out = c;
+    return MSGS_OK;
+}
+
+void msgs_ctx_destroy(msgs_ctx* c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->db.release();
+    c->bufs.releaseAll();
+    for (auto& s : c->slots) s.release();
+    (void)hipEventDestroy(c->ev0); (void)hipEventDestroy(c->ev1);
+    (void)hipEventDestroy(c->ev2); (void)hipEventDestroy(c->ev3);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* msgs_last_error(msgs_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+/* ------------------------------------------------------------------ database */
+
+static const uint8_t kBasic[16] = {0, 1, 2, 2, 4, 4, 4, 4, 8, 8, 8, 8, 8, 8, 8, 15};
+static inline uint8_t compBase(uint8_t b) {
+    return (uint8_t)(((b & 1) << 3) | ((b & 2) << 1) | ((b & 4) >> 1) | ((b & 8) >> 3));
+}
+
+int msgs_db_build(msgs_ctx* c, const uint8_t* seqs, const int64_t* lens, int32_t nChrom, const char* names) {
+    if (!c || !seqs || !lens || nChrom <= 0) return fail(c, MSGS_E_ARG, "bad arguments");
+    if (hipSetDevice(c->device) != hipSuccess) return fail(c, MSGS_E_HIP, "hipSetDevice");
+    c->db.release();
+    DbState& db = c->db;
+    db.names.clear(); db.chromLen.clear(); db.chromBD.clear(); db.tgtOff.clear();
+    db.nChrom = nChrom;
+    int64_t total = 0, seqTotal = 0;
+    for (int32_t t = 0; t < nChrom; ++t) {
+        if (lens[t] < 0) return fail(c, MSGS_E_ARG, "negative chrom length");
+        total += 2 * (lens[t] + 1);
+        seqTotal += lens[t];
+    }
+    /* names */
+    const char* p = names ? names : "";
+    for (int32_t t = 0; t < nChrom; ++t) {
+        const char* e = p;
+        while (*e && *e != '\n') ++e;
+        std::string nm(p, e);
+        if (nm.empty()) { char buf[32]; std::snprintf(buf, sizeof buf, "g%04d", t); nm = buf; }
+        db.names.push_back(nm);
+        p = *e ? e + 1 : e;
+    }
+    /* BD text (MetaGenome::getBDSeq, src/MetaGenome.cpp:104-113) and targets */
+    std::vector<uint8_t> bd(total);
+    db.tgtHost.assign(seqTotal + nChrom, 0);
+    int64_t off = 0, bdOff = 0, tOff = 0;
+    for (int32_t t = 0; t < nChrom; ++t) {
+        const int64_t L = lens[t];
+        db.chromLen.push_back(L);
+        db.chromBD.push_back(bdOff);
+        db.tgtOff.push_back(tOff);
+        for (int64_t i = 0; i < L; ++i) {
+            uint8_t b = seqs[off + i];
+            if (b > 15) return fail(c, MSGS_E_ARG, "chrom bases must be nt16 codes");
+            bd[bdOff + i] = kBasic[b];
+            bd[bdOff + L + 1 + (L - 1 - i)] = compBase(kBasic[b]);
+            db.tgtHost[tOff + i] = b;
+        }
+        bd[bdOff + L] = 0;
+        bd[bdOff + 2 * L + 1] = 0;
+        off += L;
+        bdOff += 2 * (L + 1);
+        tOff += L + 1;
+    }
+    db.chromBD.push_back(bdOff);
+    db.n = total;
+    /* counts (FMDIndex::buildCounts, src/FMDIndex.cpp:118-134) */
+    std::fill(db.B, db.B + 256, 0);
+    std::fill(db.C, db.C + 256, 0);
+    for (int64_t i = 0; i < total; ++i) db.B[bd[i]]++;
+    int64_t S = 0;
+    for (int i = 0; i <= 16; ++i) { db.C[i] = S; S += db.B[i]; }
+    for (int i = 17; i < 256; ++i) db.C[i] = 0;
+    for (int b = 0; b < 16; ++b)
+        if (db.B[b] != db.B[compBase((uint8_t)b)]) return fail(c, MSGS_E_DB, "input seq is not bi-directional");
+    if (db.C[15] - (db.B[1] + db.B[2] + db.B[4] + db.B[8]) - db.B[0] > 0)
+        return fail(c, MSGS_E_DB, "input seq does not allow IUPAC-extended bases");
+    for (int b = 0; b < 16; ++b) db.ll[b] = std::log((double)db.B[kBasic[b]]) - std::log((double)total);
+    for (int b = 0; b < 16; ++b)
+        if (!(db.ll[b] <= 0)) return fail(c, MSGS_E_DB, "base log-frequency above zero");
+
+    hipStream_t st = c->stream;
+    if (hipMalloc(&db.dBd, total) != hipSuccess) return fail(c, MSGS_E_NOMEM, "BD text");
+    if (hipMemcpyAsync(db.dBd, bd.data(), total, hipMemcpyHostToDevice, st) != hipSuccess)
+        return fail(c, MSGS_E_HIP, "BD upload");
+    std::string err;
+    int rc = buildIndexDevice(db, st, err);
+    if (rc != MSGS_OK) return fail(c, rc, err);
+    /* targets and coordinates */
+    if (hipMalloc(&db.dTgt, db.tgtHost.size()) != hipSuccess ||
+        hipMalloc(&db.dTgtOff, nChrom * sizeof(int64_t)) != hipSuccess ||
+        hipMalloc(&db.dChromLen, nChrom * sizeof(int64_t)) != hipSuccess ||
+        hipMalloc(&db.dChromBD, (nChrom + 1) * sizeof(int64_t)) != hipSuccess)
+        return fail(c, MSGS_E_NOMEM, "targets");
+    if (hipMemcpyAsync(db.dTgt, db.tgtHost.data(), db.tgtHost.size(), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(db.dTgtOff, db.tgtOff.data(), nChrom * sizeof(int64_t), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(db.dChromLen, db.chromLen.data(), nChrom * sizeof(int64_t), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(db.dChromBD, db.chromBD.data(), (nChrom + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st) != hipSuccess)
+        return fail(c, MSGS_E_HIP, "target upload");
+    if (hipStreamSynchronize(st) != hipSuccess) return fail(c, MSGS_E_HIP, "sync");
+    return MSGS_OK;
+}
+
+int64_t msgs_db_length(msgs_ctx* c) { return c ? c->db.n : 0; }
+
+int msgs_db_counts(msgs_ctx* c, int64_t* B, int64_t* C) {
+    if (!c || !c->db.n) return fail(c, MSGS_E_STATE, "no database");
+    if (B) std::memcpy(B, c->db.B, sizeof(c->db.B));
+    if (C) std::memcpy(C, c->db.C, sizeof(c->db.C));
+    return MSGS_OK;
+}
+
+double msgs_db_loglik(msgs_ctx* c, int32_t b) {
+    if (!c || !c->db.n || b < 0 || b > 15) return std::numeric_limits<double>::quiet_NaN();
+    return c->db.ll[b];
+}
+
+int msgs_db_copy(msgs_ctx* c, uint8_t* bd, uint8_t* bwt, int64_t* sa) {
+    if (!c || !c->db.n) return fail(c, MSGS_E_STATE, "no database");
+    DbState& db = c->db;
+    hipStream_t st = c->stream;
+    if (bd && hipMemcpyAsync(bd, db.dBd, db.n, hipMemcpyDeviceToHost, st) != hipSuccess) return fail(c, MSGS_E_HIP, "copy");
+    if (bwt && hipMemcpyAsync(bwt, db.dBwt, db.n, hipMemcpyDeviceToHost, st) != hipSuccess) return fail(c, MSGS_E_HIP, "copy");
+    if (sa) {
+        std::vector<uint32_t> s(db.n);
+        if (hipMemcpyAsync(s.data(), db.dSa, db.n * 4, hipMemcpyDeviceToHost, st) != hipSuccess) return fail(c, MSGS_E_HIP, "copy");
+        if (hipStreamSynchronize(st) != hipSuccess) return fail(c, MSGS_E_HIP, "sync");
+        for (int64_t i = 0; i < db.n; ++i) sa[i] = s[i];
+    }
+    if (hipStreamSynchronize(st) != hipSuccess) return fail(c, MSGS_E_HIP, "sync");
+    return MSGS_OK;
+}
+
+/* ------------------------------------------------------------------ FMD primitives */
+
+int msgs_rank_batch(msgs_ctx* c, const uint8_t* sym, const int64_t* pos, int64_t* out, int64_t n) {
+    if (!c || !c->db.n) return fail(c, MSGS_E_STATE, "no database");
+    if (n <= 0) return MSGS_OK;
+    hipError_t e = hipSuccess;
+    uint8_t* dS = c->bufs.get<uint8_t>("rk.s", n, e);
+    int64_t* dP = c->bufs.get<int64_t>("rk.p", n, e);
+    int64_t* dO = c->bufs.get<int64_t>("rk.o", n, e);
+    if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "rank buffers");
+    hipStream_t st = c->stream;
+    (void)hipMemcpyAsync(dS, sym, n, hipMemcpyHostToDevice, st);
+    (void)hipMemcpyAsync(dP, pos, n * 8, hipMemcpyHostToDevice, st);
+    k_rank<<<gridFor(n), 256, 0, st>>>(c->db.view(), dS, dP, dO, n);
+    (void)hipMemcpyAsync(out, dO, n * 8, hipMemcpyDeviceToHost, st);
+    if (hipStreamSynchronize(st) != hipSuccess) return fail(c, MSGS_E_HIP, "rank kernel");
+    return MSGS_OK;
+}
+
+int msgs_ext_batch(msgs_ctx* c, int64_t* pqs, const uint8_t* bases, int32_t fwd, int64_t* ret, int64_t n) {
+    if (!c || !c->db.n) return fail(c, MSGS_E_STATE, "no database");
+    if (n <= 0) return MSGS_OK;
+    hipError_t e = hipSuccess;
+    int64_t* dQ = c->bufs.get<int64_t>("ex.q", 3 * n, e);
+    uint8_t* dB = c->bufs.get<uint8_t>("ex.b", n, e);
+    int64_t* dR = c->bufs.get<int64_t>("ex.r", n, e);
+    if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "ext buffers");
+    hipStream_t st = c->stream;
+    (void)hipMemcpyAsync(dQ, pqs, 3 * n * 8, hipMemcpyHostToDevice, st);
+    (void)hipMemcpyAsync(dB, bases, n, hipMemcpyHostToDevice, st);
+    k_ext<<<gridFor(n), 256, 0, st>>>(c->db.view(), dQ, dB, fwd, dR, n);
+    (void)hipMemcpyAsync(pqs, dQ, 3 * n * 8, hipMemcpyDeviceToHost, st);
+    if (ret) (void)hipMemcpyAsync(ret, dR, n * 8, hipMemcpyDeviceToHost, st);
+    if (hipStreamSynchronize(st) != hipSuccess) return fail(c, MSGS_E_HIP, "ext kernel");
+    return MSGS_OK;
+}
+
+}  // extern "C"
+
+/* ------------------------------------------------------------------ seeding stage */
+
+/* Upload a read batch into device buffers of the given prefix; returns max length. */
+int uploadReads(msgs_ctx* c, const msgs_reads* r, const char* pfx, DevReads& out) {
+    if (!r || r->n < 0 || (r->n > 0 && (!r->seq || !r->offs))) return fail(c, MSGS_E_ARG, "bad reads");
+    const int64_t n = r->n;
+    const int64_t total = n ? r->offs[n] - r->offs[0] : 0;
+    int64_t maxL = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t L = r->offs[i + 1] - r->offs[i];
+        if (L < 0) return fail(c, MSGS_E_ARG, "read offsets must not decrease");
+        maxL = std::max(maxL, L);
+    }
+    if (maxL > 100000) return fail(c, MSGS_E_UNSUPPORTED, "reads longer than 100000 bases");
+    for (int64_t i = 0; i < total; ++i)
+        if (r->seq[r->offs[0] + i] > 15) return fail(c, MSGS_E_ARG, "read bases must be nt16 codes");
+    std::string p(pfx);
+    hipError_t e = hipSuccess;
+    out.n = n;
+    out.maxL = maxL;
+    out.total = total;
+    out.seq = c->bufs.get<uint8_t>(p + ".seq", total + 1, e);
+    out.offs = c->bufs.get<int64_t>(p + ".offs", n + 1, e);
+    if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "read buffers");
+    std::vector<int64_t> offs(n + 1);
+    for (int64_t i = 0; i <= n; ++i) offs[i] = r->offs[i] - r->offs[0];
+    hipStream_t st = c->stream;
+    if (total) (void)hipMemcpyAsync(out.seq, r->seq + r->offs[0], total, hipMemcpyHostToDevice, st);
+    (void)hipMemcpyAsync(out.offs, offs.data(), (n + 1) * 8, hipMemcpyHostToDevice, st);
+    out.offsHost = offs;
+    return MSGS_OK;
+}
+
+/* Run the SMEM kernel over resident reads; grows the per-read slot count until no
+ * read overflows.  Leaves per-read counts and records in c->bufs ("sm.cnt", "sm.out"). */
+int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out, unsigned long long* dStats) {
+    const int64_t n = r.n;
+    if (p->minLen <= 0) return fail(c, MSGS_E_ARG, "minLen must be positive");
+    double T;
+    if (!evalueThreshold((double)c->db.n, p->maxEvalue, &T))
+        return fail(c, MSGS_E_UNSUPPORTED, "e-value predicate is not monotone near its threshold");
+    SmemArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.seqs = r.seq; a.offs = r.offs; a.nReads = n;
+    a.minLen = p->minLen;
+    a.gate1 = std::isfinite(p->maxEvalue) ? 1 : 0;
+    a.T1 = T; a.T2 = T;
+    const int64_t lanesMax = (int64_t)std::max(1, c->nCU) * c->lanesPerCU;
+    const int64_t lanes = std::max<int64_t>(256, std::min<int64_t>(((n + 255) / 256) * 256, lanesMax));
+    const int64_t candCap = r.maxL + 2;
+    hipError_t e = hipSuccess;
+    a.candPos = c->bufs.get<int32_t>("sm.cp", candCap * lanes, e);
+    a.candF = c->bufs.get<int64_t>("sm.cf", candCap * lanes, e);
+    a.candR = c->bufs.get<int64_t>("sm.cr", candCap * lanes, e);
+    a.candS = c->bufs.get<int64_t>("sm.cs", candCap * lanes, e);
+    a.candLL = c->bufs.get<double>("sm.cl", candCap * lanes, e);
+    a.nLanes = lanes;
+    a.queue = c->bufs.get<unsigned long long>("sm.q", 1, e);
+    a.cnt = c->bufs.get<int32_t>("sm.cnt", n + 1, e);
+    a.stats = dStats;
+    if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "smem scratch");
+    int cap = c->smemCap;
+    hipStream_t st = c->stream;
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        a.cap = cap;
+        a.out = c->bufs.get<SmemRec>("sm.out", (size_t)cap * std::max<int64_t>(n, 1), e);
+        if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "smem output");
+        (void)hipMemsetAsync(a.queue, 0, 8, st);
+        if (n > 0) smem_kernel<<<(unsigned)(lanes / 256), 256, 0, st>>>(c->db.view(), a);
+        if (hipGetLastError() != hipSuccess) return fail(c, MSGS_E_HIP, "smem launch");
+        /* largest count */
+        std::vector<int32_t> cnt(n);
+        if (n) (void)hipMemcpyAsync(cnt.data(), a.cnt, n * 4, hipMemcpyDeviceToHost, st);
+        if (hipStreamSynchronize(st) != hipSuccess) return fail(c, MSGS_E_HIP, "smem kernel");
+        int mx = 0;
+        for (int64_t i = 0; i < n; ++i) mx = std::max(mx, cnt[i]);
+        if (mx <= cap) {
+            out.cap = cap;
+            out.cnt = a.cnt;
+            out.recs = a.out;
+            out.cntHost.swap(cnt);
+            return MSGS_OK;
+        }
+        cap = mx;
+    }
+    return fail(c, MSGS_E_HIP, "SMEM slot growth did not settle");
+}
+
+extern "C" int msgs_smem_batch(msgs_ctx* c, const msgs_reads* reads, const msgs_params* p, int64_t* counts,
+                               int64_t* rows5, double* logP, int64_t cap, int64_t* needed) {
+    if (!c || !c->db.n) return fail(c, MSGS_E_STATE, "no database");
+    if (!reads || !p) return fail(c, MSGS_E_ARG, "bad arguments");
+    DevReads r;
+    int rc = uploadReads(c, reads, "rd", r);
+    if (rc) return rc;
+    SmemStage sm;
+    rc = runSmem(c, r, p, sm, nullptr);
+    if (rc) return rc;
+    const int64_t n = r.n;
+    std::vector<SmemRec> recs((size_t)sm.cap * n);
+    if (n) (void)hipMemcpyAsync(recs.data(), sm.recs, recs.size() * sizeof(SmemRec), hipMemcpyDeviceToHost, c->stream);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, MSGS_E_HIP, "copy");
+    int64_t total = 0;
+    for (int64_t i = 0; i < n; ++i) total += sm.cntHost[i];
+    if (needed) *needed = total;
+    if (total > cap) return fail(c, MSGS_E_CAP, "row capacity");
+    int64_t k = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (counts) counts[i] = sm.cntHost[i];
+        for (int j = 0; j < sm.cntHost[i]; ++j, ++k) {
+            const SmemRec& m = recs[(size_t)i * sm.cap + j];
+            int64_t* o = rows5 + 5 * k;
+            o[0] = m.from; o[1] = m.to; o[2] = m.fwd; o[3] = m.rev; o[4] = m.size;
+            if (logP) logP[k] = m.logP;
+        }
+    }
+    return MSGS_OK;
+}
+
+/* ------------------------------------------------------------------ utilities */
+
+#include "msgs_sort.h"
+
+/* The permutation std::sort gives for keys compared with '<' (host run of the
+ * restated libstdc++ sort, exposed so the restatement can be checked). */
+extern "C" void msgs_util_sort_by_key(const double* keys, int32_t* perm, int64_t n) {
+    struct Less {
+        const double* k;
+        bool operator()(int32_t a, int32_t b) const { return k[a] < k[b]; }
+    };
+    msgs_sort::sort(perm, perm + n, Less{keys});
+}

@@ -1,0 +1,147 @@
+/*
+ * msgs_device.h -- HBM layout of the FMD index and the device-side FMD primitives.
+ *
+ * Occ blocks.  The BWT is cut into 128-position blocks of 64 bytes, one cache
+ * sector each:
+ *   cnt[4]   number of A, C, G, T in BWT[0, 128*k)            (16 B)
+ *   spec[4]  bit j set when BWT[128*k + j] is '\0' or N         (16 B)
+ *   code[8]  2-bit codes (A=0 C=1 G=2 T=3; '\0' and N stored as 0),
+ *            position j in word j/16, bits 2*(j%16)..+1          (32 B)
+ * A rank query at any i touches exactly one block.  '\0' counts come from
+ * positions minus the four base counts minus N positions; N positions (absent in
+ * the synthetic workloads) are kept in a sorted side array.  Counts are 32-bit, so
+ * the BD text must be shorter than 2^32.
+ *
+ * The rank semantics are those of WaveletTreeRRR::rank
+ * (src/libsds/src/WaveletTreeRRR.cpp:101-125): count in BWT[0..i] inclusive, and
+ * size_t(-1) (here -1) for i outside [0, n).
+ */
+#ifndef MSGS_DEVICE_H
+#define MSGS_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+struct __attribute__((aligned(64))) OccBlock {
+    uint32_t cnt[4];
+    uint32_t spec[4];
+    uint32_t code[8];
+};
+
+/* Everything a kernel needs to read the index; passed by value. */
+struct DevIndex {
+    const OccBlock* occ;
+    const uint32_t* sa;        /* full suffix array */
+    const int64_t* nPos;       /* sorted BWT positions holding N */
+    int64_t nN;
+    int64_t n;                 /* BD text length */
+    int64_t C[17];             /* cumulative counts by nt16 code */
+    double ll[16];             /* loglik table (src/FMDIndex.h:195-197) */
+    const int64_t* chromBD;    /* BD start of each chrom, nChrom+1 entries */
+    int32_t nChrom;
+    const uint8_t* tgt;        /* chrom sequences, each followed by a 0 byte */
+    const int64_t* tgtOff;     /* start of chrom t in tgt */
+    const int64_t* chromLen;   /* length of chrom t (without the terminator) */
+};
+
+__device__ __forceinline__ bool amb(uint8_t b) { return !(b == 1 || b == 2 || b == 4 || b == 8); }
+__device__ __forceinline__ int toInt(uint8_t b) { return b == 1 ? 1 : b == 2 ? 2 : b == 4 ? 3 : b == 8 ? 4 : 0; }
+__device__ __forceinline__ uint8_t comp16(uint8_t b) {
+    /* base2comp (src/DNAalphabet.cpp): reverse the 4 bits */
+    return (uint8_t)(((b & 1) << 3) | ((b & 2) << 1) | ((b & 4) >> 1) | ((b & 8) >> 3));
+}
+
+/* # of N in BWT[0..i] */
+__device__ __forceinline__ int64_t countN(const DevIndex& ix, int64_t i) {
+    int64_t lo = 0, hi = ix.nN;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (ix.nPos[mid] <= i) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+/* rank of '\0', A, C, G, T at i from an already-loaded block */
+__device__ __forceinline__ void occFromBlock(const DevIndex& ix, const uint4& h, const uint4& sp, const uint4& c0,
+                                             const uint4& c1, int64_t i, int64_t* r) {
+    const uint32_t j = (uint32_t)(i & 127);
+    const uint32_t wj = j >> 4, bj = j & 15;
+    const uint32_t part = (2u << (2 * bj)) - 1u;
+    const uint32_t w[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+    uint32_t nT = 0, nLo = 0, nHi = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) {
+        uint32_t mk = k < wj ? 0xffffffffu : (k == wj ? part : 0u);
+        uint32_t x = w[k] & mk;
+        uint32_t lo = x & 0x55555555u, hi = (x >> 1) & 0x55555555u;
+        nT += __popc(lo & hi);
+        nLo += __popc(lo);
+        nHi += __popc(hi);
+    }
+    const uint32_t sw = j >> 5, sb = j & 31;
+    const uint32_t smask = (2u << sb) - 1u;
+    const uint32_t s[4] = {sp.x, sp.y, sp.z, sp.w};
+    uint32_t nSp = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        uint32_t mk = k < sw ? 0xffffffffu : (k == sw ? smask : 0u);
+        nSp += __popc(s[k] & mk);
+    }
+    const uint32_t nC = nLo - nT, nG = nHi - nT;
+    const uint32_t nA = (j + 1) - nLo - nHi + nT;
+    r[1] = (int64_t)h.x + nA - nSp;
+    r[2] = (int64_t)h.y + nC;
+    r[3] = (int64_t)h.z + nG;
+    r[4] = (int64_t)h.w + nT;
+    int64_t special = (i & ~(int64_t)127) - ((int64_t)h.x + h.y + h.z + h.w) + nSp;
+    r[0] = ix.nN ? special - countN(ix, i) : special;
+}
+
+__device__ __forceinline__ void occ5(const DevIndex& ix, int64_t i, int64_t* r) {
+    if (i < 0 || i >= ix.n) { r[0] = r[1] = r[2] = r[3] = r[4] = -1; return; }
+    const uint4* b = reinterpret_cast<const uint4*>(ix.occ + (i >> 7));
+    uint4 h = b[0], sp = b[1], c0 = b[2], c1 = b[3];
+    occFromBlock(ix, h, sp, c0, c1, i, r);
+}
+
+/* Both rank rows of one extension; the two block loads are issued together, and
+ * only one when both positions share a block. */
+__device__ __forceinline__ void occPair(const DevIndex& ix, int64_t a, int64_t b, int64_t* ra, int64_t* rb) {
+    const bool va = a >= 0 && a < ix.n, vb = b >= 0 && b < ix.n;
+    const int64_t ba = va ? (a >> 7) : 0, bb = vb ? (b >> 7) : 0;
+    const uint4* pa = reinterpret_cast<const uint4*>(ix.occ + ba);
+    const uint4* pb = reinterpret_cast<const uint4*>(ix.occ + bb);
+    uint4 h0 = pa[0], s0 = pa[1], c00 = pa[2], c01 = pa[3];
+    uint4 h1, s1, c10, c11;
+    if (bb != ba) { h1 = pb[0]; s1 = pb[1]; c10 = pb[2]; c11 = pb[3]; }
+    else { h1 = h0; s1 = s0; c10 = c00; c11 = c01; }
+    if (va) occFromBlock(ix, h0, s0, c00, c01, a, ra); else { ra[0] = ra[1] = ra[2] = ra[3] = ra[4] = -1; }
+    if (vb) occFromBlock(ix, h1, s1, c10, c11, b, rb); else { rb[0] = rb[1] = rb[2] = rb[3] = rb[4] = -1; }
+}
+
+/* FMDIndex::backExt (src/FMDIndex.cpp:577-610): bi-interval [p, q, s] extended by b
+ * on the left.  The q update adds the '\0' count and the counts of bases above b. */
+__device__ __forceinline__ int64_t dBackExt(const DevIndex& ix, int64_t& p, int64_t& q, int64_t& s, uint8_t b) {
+    if (amb(b)) return (s = 0);
+    const int c = toInt(b);
+    int64_t lo[5], hi[5];
+    occPair(ix, p - 1, p + s - 1, lo, hi);
+    const int64_t O = lo[c];
+    const int64_t sc = hi[c] - O;
+    if (sc <= 0) return (s = sc);
+    if (sc != s) {
+        int64_t add = hi[0] - lo[0];
+        for (int j = c + 1; j <= 4; ++j) add += hi[j] - lo[j];
+        q = q + add;
+    }
+    p = ix.C[b] + O;
+    s = sc;
+    return s;
+}
+
+/* FMDIndex::fwdExt (src/FMDIndex.h:226-228) */
+__device__ __forceinline__ int64_t dFwdExt(const DevIndex& ix, int64_t& p, int64_t& q, int64_t& s, uint8_t b) {
+    return dBackExt(ix, q, p, s, comp16(b));
+}
+
+#endif

@@ -1,0 +1,135 @@
+/*
+ * msgs_kernels.h -- kernel argument blocks and launch declarations shared by the
+ * HIP translation units of libmsgs_hip.so.
+ */
+#ifndef MSGS_KERNELS_H
+#define MSGS_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "msgs_device.h"
+
+/* one SMEM: read interval [from, to), bi-interval (fwd, rev, size), loglik */
+struct SmemRec {
+    int32_t from, to;
+    int64_t fwd, rev, size;
+    double logP;
+};
+
+struct SmemArgs {
+    const uint8_t* seqs;
+    const int64_t* offs;
+    int64_t nReads;
+    int64_t minLen;
+    double T1, T2;             /* phase-1 / phase-2 e-value thresholds on logP */
+    int32_t gate1;             /* 0 when maxEvalue is not finite (phase-1 gate skipped) */
+    int32_t cap;               /* SMEM slots per read */
+    int32_t* cnt;              /* SMEMs emitted per read (may exceed cap) */
+    SmemRec* out;              /* [nReads][cap] */
+    int32_t* candPos;          /* candidate scratch, [candCap][nLanes] */
+    int64_t* candF;
+    int64_t* candR;
+    int64_t* candS;
+    double* candLL;
+    int64_t nLanes;
+    unsigned long long* queue; /* next read */
+    unsigned long long* stats; /* [0] extension calls */
+};
+
+__global__ void smem_kernel(DevIndex ix, SmemArgs a);
+
+/* one seed (SeedPair, src/SeedPair.h:120-128) */
+struct SeedRec {
+    int64_t from, to, start, end;
+    int32_t tid, strand;
+    double logP;
+};
+
+struct SeedArgs {
+    const int64_t* offs;      /* read offsets (for L) */
+    int64_t nReads;
+    const int32_t* smemCnt;
+    const SmemRec* smems;      /* [nReads][cap] */
+    int32_t cap;
+    int64_t maxNSeed;
+    int32_t* order;            /* [nReads][cap] SMEM permutation after the loglik sort */
+    int32_t* nUsed;            /* SMEMs whose seeds are taken */
+    int64_t* nRaw;             /* seeds before removeRedundant */
+    const int64_t* seedOff;    /* exclusive scan of nRaw */
+    SeedRec* seeds;
+    int32_t* seedCnt;          /* seeds after removeRedundant */
+    unsigned long long* stats; /* [1] SA lookups */
+};
+
+__global__ void smem_order_kernel(DevIndex ix, SeedArgs a);
+__global__ void seed_kernel(DevIndex ix, SeedArgs a);
+
+/* chains: per read, a list of chains, each a run of seed indices */
+struct ChainArgs {
+    const int64_t* offs;
+    int64_t nReads;
+    const int64_t* seedOff;
+    const int32_t* seedCnt;
+    const SeedRec* seeds;
+    int32_t* stackNode;        /* per read scratch [seedOff .. + seedCnt) */
+    int32_t* stackNext;
+    uint8_t* seen;
+    int32_t* nChains;          /* pass 1 outputs */
+    int64_t* nChainSeeds;
+    const int64_t* chainOff;   /* pass 2 inputs: exclusive scans */
+    const int64_t* chainSeedOff;
+    int32_t* chainStart;       /* [chain] start index into chainSeeds */
+    int32_t* chainLen;
+    int32_t* chainSeeds;       /* seed index (relative to the read's seeds) */
+    int32_t pass;
+};
+__global__ void chain_kernel(ChainArgs a);
+
+/* DP over one chain (one alignment job) */
+struct AlnRec {
+    int32_t rid, tid, strand, hazard;
+    int64_t tStart, tEnd, alnFrom, alnTo, alnStart, alnEnd;
+    int32_t score;             /* integer-valued alnScore */
+    int32_t nCigar, mdLen;
+    int32_t pad;
+    double log10P;
+    int64_t cells;
+};
+
+struct DpArgs {
+    const uint8_t* seqs;       /* reads */
+    const uint8_t* quals;
+    const uint8_t* rcSeqs;     /* reverse complements, same offsets */
+    const uint8_t* rcQuals;
+    const int64_t* offs;
+    const int64_t* seedOff;
+    const SeedRec* seeds;
+    int64_t nJobs;
+    const int32_t* jobRead;    /* read of each job */
+    const int32_t* jobChainStart;
+    const int32_t* jobChainLen;
+    const int32_t* chainSeeds;
+    const int64_t* tileOff;    /* per job offset into the tile pool (cells) */
+    int32_t* tileM;
+    int32_t* tileI;
+    int32_t* tileD;
+    const int64_t* pathOff;    /* per job offset into path/cigar/md pools */
+    uint8_t* path;
+    uint32_t* cigar;
+    char* md;
+    AlnRec* out;
+    int32_t mode;              /* 0 GLOBAL, 1 LOCAL */
+    int32_t match, mismatch, gapO, gapE;   /* integer-valued scores */
+    double clip;
+    const double* qMatch;      /* log10(1 - phredQ2P(q)) by q */
+    const double* qMis;        /* q / PHRED_SCALE by q */
+    const double* qGapOpen;    /* q / PHRED_SCALE - gapO by q */
+    double gapE_d;
+    int32_t sizeOnly;          /* 1: compute tStart/tEnd and pool sizes only */
+    int64_t* tileNeed;
+    int64_t* pathNeed;
+};
+__global__ void dp_kernel(DevIndex ix, DpArgs a);
+
+#endif

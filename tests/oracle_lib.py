@@ -1,0 +1,273 @@
+"""ctypes bindings to the CPU oracle (oracle/liborc.so) and, when built, to the
+reference's own seeding stack (oracle/_ref/libmsgs_ref.so).  Test
+infrastructure only."""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORC_SO = os.path.join(ROOT, "oracle", "liborc.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libmsgs_ref.so")
+
+P64 = C.POINTER(C.c_int64)
+PD = C.POINTER(C.c_double)
+PU8 = C.POINTER(C.c_uint8)
+
+
+class Params(C.Structure):
+    """Parameter block shared by the oracle and the product C-ABI (same layout)."""
+    _fields_ = [("minLen", C.c_int64), ("maxEvalue", C.c_double), ("maxNSeed", C.c_int64),
+                ("mode", C.c_int32), ("maxReport", C.c_int32),
+                ("match", C.c_double), ("mismatch", C.c_double), ("gapO", C.c_double),
+                ("gapE", C.c_double), ("clip", C.c_double), ("minScoreRate", C.c_double),
+                ("insMean", C.c_double), ("insSD", C.c_double), ("insMin", C.c_double),
+                ("insMax", C.c_double),
+                ("noMixed", C.c_int32), ("noDiscordant", C.c_int32), ("noTailOver", C.c_int32),
+                ("noContain", C.c_int32), ("noOverlap", C.c_int32), ("pad0", C.c_int32),
+                ("maxNPair", C.c_int64)]
+
+
+def default_params(**kw):
+    p = Params(minLen=17, maxEvalue=0.01, maxNSeed=500, mode=0, maxReport=1, match=1, mismatch=3,
+               gapO=5, gapE=2, clip=4, minScoreRate=0.65, insMean=200, insSD=20, insMin=0,
+               insMax=400, maxNPair=0)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def pe_params(mean, sd, **kw):
+    """Insert model the way msgseqtk-align sets it from -m/-d (setMean, setSD, updateRange)."""
+    return default_params(insMean=mean, insSD=sd, insMin=mean - 10 * sd, insMax=mean + 10 * sd, **kw)
+
+
+def ptr(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def _load(path):
+    lib = C.CDLL(path)
+    return lib
+
+
+_orc = None
+_ref = None
+
+
+def orc():
+    global _orc
+    if _orc is None:
+        lib = _load(ORC_SO)
+        lib.orc_db_create.restype = C.c_void_p
+        lib.orc_db_create.argtypes = [PU8, P64, C.c_int32, C.c_char_p, P64]
+        lib.orc_db_free.argtypes = [C.c_void_p]
+        lib.orc_db_len.argtypes = [C.c_void_p]
+        lib.orc_db_len.restype = C.c_int64
+        lib.orc_db_counts.argtypes = [C.c_void_p, P64, P64]
+        lib.orc_db_copy.argtypes = [C.c_void_p, PU8, PU8, P64]
+        lib.orc_rank.argtypes = [C.c_void_p, C.c_int32, C.c_int64]
+        lib.orc_rank.restype = C.c_int64
+        lib.orc_backext.argtypes = [C.c_void_p, P64, C.c_int32]
+        lib.orc_backext.restype = C.c_int64
+        lib.orc_fwdext.argtypes = [C.c_void_p, P64, C.c_int32]
+        lib.orc_fwdext.restype = C.c_int64
+        lib.orc_loglik.argtypes = [C.c_void_p, C.c_int32]
+        lib.orc_loglik.restype = C.c_double
+        lib.orc_tid.argtypes = [C.c_void_p, C.c_int64]
+        lib.orc_tid.restype = C.c_int64
+        lib.orc_smems.argtypes = [C.c_void_p, PU8, P64, C.c_int64, C.c_int64, C.c_double, C.c_int32,
+                                  P64, P64, PD, C.c_int64]
+        lib.orc_smems.restype = C.c_int64
+        lib.orc_seeds.argtypes = [C.c_void_p, PU8, P64, C.c_int64, C.c_int64, C.c_double, C.c_int64,
+                                  P64, P64, PD, C.c_int64]
+        lib.orc_seeds.restype = C.c_int64
+        lib.orc_remove_redundant.argtypes = [P64, PD, C.c_int64]
+        lib.orc_remove_redundant.restype = C.c_int64
+        lib.orc_get_chains.argtypes = [P64, C.c_int64, C.c_int64, C.c_int64, P64, C.c_int64]
+        lib.orc_get_chains.restype = C.c_int64
+        lib.orc_align_full.argtypes = [C.POINTER(Params), PU8, C.c_int64, PU8, C.c_int64, C.c_int32,
+                                       P64, PD, C.c_char_p, C.c_char_p, C.c_int64]
+        lib.orc_align_full.restype = C.c_int64
+        lib.orc_alignments.argtypes = [C.c_void_p, C.POINTER(Params), PU8, PU8, P64, C.c_int64, P64, P64,
+                                       PD, C.c_char_p, C.c_int64, C.c_int64]
+        lib.orc_alignments.restype = C.c_int64
+        lib.orc_map.argtypes = [C.c_void_p, C.POINTER(Params), PU8, PU8, P64, C.c_char_p, PU8, PU8, P64,
+                                C.c_char_p, C.c_int64, C.c_int32, C.c_char_p, C.c_int64, P64, PD]
+        lib.orc_map.restype = C.c_int64
+        lib.orc_max_threads.restype = C.c_int32
+        _orc = lib
+    return _orc
+
+
+def ref_available():
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        lib = _load(REF_SO)
+        lib.ref_index_build.restype = C.c_void_p
+        lib.ref_index_build.argtypes = [PU8, C.c_int64]
+        lib.ref_index_free.argtypes = [C.c_void_p]
+        lib.ref_length.argtypes = [C.c_void_p]
+        lib.ref_length.restype = C.c_int64
+        lib.ref_counts.argtypes = [C.c_void_p, P64, P64]
+        lib.ref_access_bwt.argtypes = [C.c_void_p, C.c_int64]
+        lib.ref_access_bwt.restype = C.c_int32
+        lib.ref_rank.argtypes = [C.c_void_p, C.c_int32, C.c_int64]
+        lib.ref_rank.restype = C.c_int64
+        lib.ref_access_sa.argtypes = [C.c_void_p, C.c_int64]
+        lib.ref_access_sa.restype = C.c_int64
+        lib.ref_backext.argtypes = [C.c_void_p, P64, C.c_int32]
+        lib.ref_backext.restype = C.c_int64
+        lib.ref_fwdext.argtypes = [C.c_void_p, P64, C.c_int32]
+        lib.ref_fwdext.restype = C.c_int64
+        lib.ref_loglik.argtypes = [C.c_void_p, C.c_int32]
+        lib.ref_loglik.restype = C.c_double
+        lib.ref_find_smems.argtypes = [C.c_void_p, PU8, C.c_int64, C.c_int64, C.c_double, C.c_int32, P64,
+                                       PD, C.c_int64]
+        lib.ref_find_smems.restype = C.c_int64
+        lib.ref_remove_redundant.argtypes = [P64, PD, C.c_int64]
+        lib.ref_remove_redundant.restype = C.c_int64
+        lib.ref_get_chains.argtypes = [P64, C.c_int64, C.c_int64, C.c_int64, P64, C.c_int64]
+        lib.ref_get_chains.restype = C.c_int64
+        _ref = lib
+    return _ref
+
+
+class OracleDb:
+    """Metagenome + FMD index held by the oracle."""
+
+    def __init__(self, chroms, names=None, sa=None):
+        self.chroms = [np.ascontiguousarray(c, dtype=np.uint8) for c in chroms]
+        self.lens = np.array([len(c) for c in self.chroms], dtype=np.int64)
+        self.names = names or ["g%04d" % i for i in range(len(chroms))]
+        seqs = np.concatenate(self.chroms) if self.chroms else np.zeros(0, np.uint8)
+        sap = ptr(sa, C.c_int64) if sa is not None else None
+        self.h = orc().orc_db_create(ptr(seqs, C.c_uint8), ptr(self.lens, C.c_int64), len(self.chroms),
+                                     "\n".join(self.names).encode(), sap)
+        if not self.h:
+            raise ValueError("oracle refused the database")
+        self.n = orc().orc_db_len(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            orc().orc_db_free(self.h)
+            self.h = None
+
+    def counts(self):
+        B = np.zeros(256, np.int64)
+        Cc = np.zeros(256, np.int64)
+        orc().orc_db_counts(self.h, ptr(B, C.c_int64), ptr(Cc, C.c_int64))
+        return B, Cc
+
+    def arrays(self):
+        bd = np.zeros(self.n, np.uint8)
+        bwt = np.zeros(self.n, np.uint8)
+        sa = np.zeros(self.n, np.int64)
+        orc().orc_db_copy(self.h, ptr(bd, C.c_uint8), ptr(bwt, C.c_uint8), ptr(sa, C.c_int64))
+        return bd, bwt, sa
+
+    def backext(self, p, q, s, b, fwd=False):
+        v = np.array([p, q, s], np.int64)
+        f = orc().orc_fwdext if fwd else orc().orc_backext
+        r = f(self.h, ptr(v, C.c_int64), b)
+        return int(r), tuple(int(x) for x in v)
+
+
+def pack_reads(reads):
+    """list of uint8 arrays -> (concat, offsets)"""
+    offs = np.zeros(len(reads) + 1, np.int64)
+    offs[1:] = np.cumsum([len(r) for r in reads])
+    seq = np.concatenate([np.asarray(r, np.uint8) for r in reads]) if reads else np.zeros(0, np.uint8)
+    if len(seq) == 0:
+        seq = np.zeros(1, np.uint8)
+    return np.ascontiguousarray(seq), offs
+
+
+def orc_smems(db, reads, minLen=17, maxE=0.01, mems=False):
+    seq, offs = pack_reads(reads)
+    n = len(reads)
+    counts = np.zeros(max(n, 1), np.int64)
+    cap = 1 << 16
+    while True:
+        out = np.zeros((cap, 5), np.int64)
+        lp = np.zeros(cap, np.float64)
+        k = orc().orc_smems(db.h, ptr(seq, C.c_uint8), ptr(offs, C.c_int64), n, minLen, maxE, int(mems),
+                            ptr(counts, C.c_int64), ptr(out, C.c_int64), ptr(lp, C.c_double), cap)
+        if k <= cap:
+            return counts[:n].copy(), out[:k].copy(), lp[:k].copy()
+        cap = k
+
+
+def orc_seeds(db, reads, minLen=17, maxE=0.01, maxNSeed=500):
+    seq, offs = pack_reads(reads)
+    n = len(reads)
+    counts = np.zeros(max(n, 1), np.int64)
+    cap = 1 << 16
+    while True:
+        out = np.zeros((cap, 6), np.int64)
+        lp = np.zeros(cap, np.float64)
+        k = orc().orc_seeds(db.h, ptr(seq, C.c_uint8), ptr(offs, C.c_int64), n, minLen, maxE, maxNSeed,
+                            ptr(counts, C.c_int64), ptr(out, C.c_int64), ptr(lp, C.c_double), cap)
+        if k <= cap:
+            return counts[:n].copy(), out[:k].copy(), lp[:k].copy()
+        cap = k
+
+
+def orc_map(db, params, reads, quals, names, mates=None, nthreads=1):
+    """SAM text per read (SE) or per pair (PE), in input order."""
+    seq, offs = pack_reads(reads)
+    qual, _ = pack_reads(quals)
+    n = len(reads)
+    nm = "\n".join(names).encode()
+    if mates is not None:
+        seq2, offs2 = pack_reads(mates[0])
+        qual2, _ = pack_reads(mates[1])
+        nm2 = "\n".join(mates[2]).encode()
+        a2 = (ptr(seq2, C.c_uint8), ptr(qual2, C.c_uint8), ptr(offs2, C.c_int64), nm2)
+    else:
+        a2 = (None, None, None, None)
+    secs = np.zeros(1, np.float64)
+    need = orc().orc_map(db.h, C.byref(params), ptr(seq, C.c_uint8), ptr(qual, C.c_uint8),
+                         ptr(offs, C.c_int64), nm, *a2, n, nthreads, None, 0, None, ptr(secs, C.c_double))
+    buf = C.create_string_buffer(int(need) + 1)
+    lo = np.zeros(n + 1, np.int64)
+    orc().orc_map(db.h, C.byref(params), ptr(seq, C.c_uint8), ptr(qual, C.c_uint8), ptr(offs, C.c_int64),
+                  nm, *a2, n, nthreads, buf, need + 1, ptr(lo, C.c_int64), ptr(secs, C.c_double))
+    text = buf.raw[:need].decode()
+    return [text[lo[i]:lo[i + 1]] for i in range(n)], float(secs[0])
+
+
+class RefIndex:
+    """The reference's own FMDIndex built by its own divsufsort from a BD text."""
+
+    def __init__(self, bd):
+        bd = np.ascontiguousarray(bd, dtype=np.uint8)
+        self.h = ref().ref_index_build(ptr(bd, C.c_uint8), len(bd))
+        if not self.h:
+            raise ValueError("reference FMDIndex refused the text")
+        self.n = len(bd)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            ref().ref_index_free(self.h)
+            self.h = None
+
+    def counts(self):
+        B = np.zeros(256, np.int64)
+        Cc = np.zeros(256, np.int64)
+        ref().ref_counts(self.h, ptr(B, C.c_int64), ptr(Cc, C.c_int64))
+        return B, Cc
+
+    def smems(self, read, minLen=17, maxE=0.01, mems=False):
+        r = np.ascontiguousarray(read, np.uint8)
+        cap = 4096
+        out = np.zeros((cap, 5), np.int64)
+        lp = np.zeros(cap, np.float64)
+        k = ref().ref_find_smems(self.h, ptr(r, C.c_uint8), len(r), minLen, maxE, int(mems),
+                                 ptr(out, C.c_int64), ptr(lp, C.c_double), cap)
+        assert 0 <= k <= cap
+        return out[:k].copy(), lp[:k].copy()

@@ -1,0 +1,133 @@
+"""The CPU oracle against the reference's own seeding code (oracle/_ref, compiled from
+the reference sources by oracle/Makefile) on the same texts and reads.  Pins the
+oracle's SA/BWT/rank/backExt (A4-A8), SMEM search (A10-A13), removeRedundant (A17) and
+getChains (A18) to the reference bit for bit."""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from msgseqtk_amd import synth
+
+pytestmark = pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (reference absent)")
+
+
+def _db(seed, n=4, length=20000, strains=0.25):
+    chroms, names = synth.metagenome(n, length, gc=(0.3, 0.7), seed=seed, strain_frac=strains)
+    return chroms, names, O.OracleDb(chroms, names)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_index_matches_reference(seed):
+    chroms, names, db = _db(seed)
+    bd, bwt, sa = db.arrays()
+    ri = O.RefIndex(bd)
+    B1, C1 = db.counts()
+    B2, C2 = ri.counts()
+    assert (B1 == B2).all() and (C1 == C2).all()
+    L = O.ref()
+    idx = np.random.default_rng(seed).integers(0, db.n, 3000)
+    for i in idx:
+        i = int(i)
+        assert L.ref_access_sa(ri.h, i) == sa[i]
+        assert L.ref_access_bwt(ri.h, i) == bwt[i]
+        for b in (0, 1, 2, 4, 8):
+            assert L.ref_rank(ri.h, b, i) == O.orc().orc_rank(db.h, b, i)
+    for b in range(16):
+        assert L.ref_loglik(ri.h, b) == O.orc().orc_loglik(db.h, b) or (
+            np.isinf(L.ref_loglik(ri.h, b)) and np.isinf(O.orc().orc_loglik(db.h, b)))
+
+
+def test_backext_matches_reference():
+    chroms, names, db = _db(3)
+    bd, _, _ = db.arrays()
+    ri = O.RefIndex(bd)
+    B, Cc = db.counts()
+    rng = np.random.default_rng(7)
+    for trial in range(400):
+        b0 = int(rng.choice([1, 2, 4, 8]))
+        comp = {1: 8, 2: 4, 4: 2, 8: 1}[b0]
+        v1 = np.array([Cc[b0], Cc[comp], Cc[b0 + 1] - Cc[b0]], np.int64)
+        v2 = v1.copy()
+        for step in range(int(rng.integers(1, 25))):
+            b = int(rng.choice([0, 1, 2, 4, 8, 15, 3]))
+            fwd = bool(rng.integers(0, 2))
+            f1 = O.orc().orc_fwdext if fwd else O.orc().orc_backext
+            f2 = O.ref().ref_fwdext if fwd else O.ref().ref_backext
+            r1 = f1(db.h, O.ptr(v1, O.C.c_int64), b)
+            r2 = f2(ri.h, O.ptr(v2, O.C.c_int64), b)
+            assert r1 == r2 and (v1 == v2).all()
+            if v1[2] <= 0:
+                break
+
+
+@pytest.mark.parametrize("minLen,maxE", [(17, 0.01), (12, 1.0), (5, float("inf")), (25, 1e-6)])
+def test_smems_match_reference(minLen, maxE):
+    chroms, names, db = _db(4)
+    bd, _, _ = db.arrays()
+    ri = O.RefIndex(bd)
+    seqs, _, _, _ = synth.simulate_reads(chroms, 120, (40, 200), sub=0.02, indel=0.002, seed=minLen)
+    seqs += [synth.encode("ACGTNNACGTAGGATTACA" * 3), synth.encode("A"), synth.encode("ACGTACGTAC"),
+             synth.encode("NNNN")]
+    cnt, rows, lps = O.orc_smems(db, seqs, minLen, maxE)
+    off = 0
+    for i, s in enumerate(seqs):
+        r2, l2 = ri.smems(s, minLen, maxE)
+        r1, l1 = rows[off:off + cnt[i]], lps[off:off + cnt[i]]
+        off += cnt[i]
+        assert np.array_equal(r1, r2), i
+        assert np.array_equal(l1.view(np.int64), l2.view(np.int64)), i
+
+
+def test_mems_match_reference():
+    chroms, names, db = _db(5)
+    bd, _, _ = db.arrays()
+    ri = O.RefIndex(bd)
+    seqs, _, _, _ = synth.simulate_reads(chroms, 60, 150, sub=0.02, indel=0.002, seed=9)
+    cnt, rows, lps = O.orc_smems(db, seqs, 1, float("inf"), mems=True)
+    off = 0
+    for i, s in enumerate(seqs):
+        r2, l2 = ri.smems(s, 1, float("inf"), mems=True)
+        assert np.array_equal(rows[off:off + cnt[i]], r2)
+        assert np.array_equal(lps[off:off + cnt[i]].view(np.int64), l2.view(np.int64))
+        off += cnt[i]
+
+
+def _random_seeds(rng, n):
+    rows = []
+    for _ in range(n):
+        f = int(rng.integers(0, 60))
+        ln = int(rng.integers(1, 30))
+        st = int(rng.integers(0, 80))
+        rows.append([f, f + ln, st, st + ln, int(rng.integers(0, 2)), int(rng.integers(1, 3))])
+    return np.array(rows, np.int64).reshape(-1, 6)
+
+
+def test_remove_redundant_matches_reference():
+    rng = np.random.default_rng(11)
+    for trial in range(200):
+        n = int(rng.integers(0, 40))
+        rows = _random_seeds(rng, n)
+        if n and trial % 3 == 0:   # duplicates with distinct logP exercise the unstable sort
+            rows = np.concatenate([rows, rows[: n // 2]])
+        lp = -rng.random(len(rows)) * 30
+        a, la = rows.copy(), lp.copy()
+        b, lb = rows.copy(), lp.copy()
+        ka = O.orc().orc_remove_redundant(O.ptr(a, O.C.c_int64), O.ptr(la, O.C.c_double), len(a))
+        kb = O.ref().ref_remove_redundant(O.ptr(b, O.C.c_int64), O.ptr(lb, O.C.c_double), len(b))
+        assert ka == kb
+        assert np.array_equal(a[:ka], b[:kb]) and np.array_equal(la[:ka], lb[:kb])
+
+
+def test_chains_match_reference():
+    rng = np.random.default_rng(12)
+    for trial in range(200):
+        n = int(rng.integers(0, 30))
+        rows = _random_seeds(rng, n)
+        order = np.lexsort((rows[:, 3], rows[:, 2], rows[:, 5], rows[:, 4], rows[:, 1], rows[:, 0]))
+        rows = np.ascontiguousarray(rows[order])
+        mm, ind = int(rng.integers(0, 20)), int(rng.integers(0, 10))
+        o1 = np.zeros(4096, np.int64)
+        o2 = np.zeros(4096, np.int64)
+        k1 = O.orc().orc_get_chains(O.ptr(rows, O.C.c_int64), len(rows), mm, ind, O.ptr(o1, O.C.c_int64), 4096)
+        k2 = O.ref().ref_get_chains(O.ptr(rows, O.C.c_int64), len(rows), mm, ind, O.ptr(o2, O.C.c_int64), 4096)
+        assert k1 == k2 and np.array_equal(o1[:k1], o2[:k2])
