@@ -1,0 +1,147 @@
+#!/usr/bin/env python3
+"""Mapped reads/s of the MI355X mapping path on the paired-end workload of 64 genomes of
+4 Mb (256 Mbp, GC ~ U(0.3, 0.7)), 2x150 bp pairs with insert N(350, 50), 0.5%
+substitutions, 0.05% indels and 5% off-target pairs, in batches of 256k reads
+(128k pairs) resident in HBM.  A step maps one batch end to end (SMEMs, seeds,
+chains, DP + backtrace on the GPU, pairing/MAPQ/SAM records on the host).
+
+Prints one JSON line (see --help).  The CPU baseline is the oracle restatement of the
+reference's path timed on the host cores over a bounded sample of the same pairs.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "mapped reads/s on 1 MI355X, learned-index vs plain FMD, bit-exact SAM; DP GCUPS"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--genomes", type=int, default=64)
+    ap.add_argument("--genome-len", type=int, default=4_000_000)
+    ap.add_argument("--pairs", type=int, default=1_000_000)
+    ap.add_argument("--batch-reads", type=int, default=262144)
+    ap.add_argument("--cpu-pairs", type=int, default=4000, help="pairs in the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py runs on one GPU only: the mapping path shards by read batch with a replicated "
+              "index, and this build measures the single-GPU kernel; use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    import msgseqtk_amd as M
+    from msgseqtk_amd import synth
+
+    t0 = time.time()
+    chroms, names = synth.metagenome(a.genomes, a.genome_len, gc=(0.3, 0.7), seed=0xC0FFEE + 2)
+    pairs_per_batch = a.batch_reads // 2
+    n_batches = max(1, min((a.pairs + pairs_per_batch - 1) // pairs_per_batch, 16))
+    batches = []
+    for b in range(n_batches):
+        batches.append(synth.simulate_pairs_fast(chroms, pairs_per_batch, 150, insert=(350, 50), sub=0.005,
+                                                 indel=0.0005, seed=0xC0FFEE + 1002 + b, offtarget=0.05,
+                                                 prefix="b%dp" % b))
+    log("data: %d genomes, %d batches of %d pairs in %.1fs" % (len(chroms), n_batches, pairs_per_batch,
+                                                              time.time() - t0))
+    t1 = time.time()
+    m = M.Mapper(0).build(chroms, names)
+    build_s = time.time() - t1
+    log("index built on the GPU in %.1fs (BD length %d)" % (build_s, m.n))
+    for b, (r1, r2) in enumerate(batches):
+        m.upload(b, r1, r2)
+    p = M.default_params(insMean=350, insSD=50, insMin=350 - 10 * 50, insMax=350 + 10 * 50)
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:
+        sync = lambda: None
+
+    for w in range(a.warmup):
+        m.run(w % n_batches, p, True)
+        log("warmup step %d done" % w)
+    sync()
+    stats = []
+    t2 = time.time()
+    for k in range(a.steps):
+        m.run(k % n_batches, p, True)
+        stats.append(m.stats())
+    sync()
+    elapsed = time.time() - t2
+    S = np.array(stats)
+    reads, mapped = S[:, 0].sum(), S[:, 1].sum()
+    value = mapped / elapsed
+    ext_calls, sa_lookups, cells = S[:, 2].sum(), S[:, 3].sum(), S[:, 4].sum()
+    smem_us, dp_us = S[:, 5].sum(), S[:, 6].sum()
+    # dominant kernel: SMEM search; algorithmic bytes = one 64-B occ block per rank side
+    # (2 per extension with an unambiguous base)
+    launches = 2 * a.steps
+    bytes_per_launch = 64.0 * 2.0 * ext_calls / launches
+    avg_us = smem_us / launches
+    achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9
+    log("timed %d steps in %.2fs: %.0f mapped reads/s (%.1f%% mapped)" % (a.steps, elapsed, value,
+                                                                          100 * mapped / reads))
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "mapped reads/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(1000 * elapsed / a.steps, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int64",
+        "data": "synthetic",
+        "config": {"workload": "PE 2x150: 64 genomes x 4 Mb (256 Mbp, GC U(0.3,0.7)), insert N(350,50), "
+                               "0.5% sub, 0.05% indel, 5% off-target pairs; 256k-read batches in HBM",
+                   "batch_reads": 2 * pairs_per_batch, "bd_length": int(m.n), "form": "plain FMD"},
+        "roofline": {"bound": "hbm", "kernel": "smem_kernel", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "bytes_per_launch": bytes_per_launch, "avg_launch_us": round(avg_us, 1)},
+        "dp_gcups": round(cells / (dp_us * 1e-6) / 1e9, 3) if dp_us > 0 else None,
+        "per_read": {"ext_calls": ext_calls / reads, "sa_lookups": sa_lookups / reads, "dp_cells": cells / reads},
+        "stage_ms_per_step": {"smem_kernel": smem_us / a.steps / 1000, "seed_stage": S[:, 8].sum() / a.steps / 1000,
+                              "dp_kernel": dp_us / a.steps / 1000, "host_finish": S[:, 9].sum() / a.steps / 1000,
+                              "whole": S[:, 7].sum() / a.steps / 1000},
+        "index_build_s": round(build_s, 2),
+        "mapped_frac": round(mapped / reads, 4),
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(m, chroms, names, batches[0], p, a.cpu_pairs)
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(m, chroms, names, batch, p, npairs):
+    """The oracle (CPU restatement of the reference path) over the first npairs pairs of
+    batch 0, OpenMP over all host cores; its SAM text is also compared with the GPU's."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    _, _, sa = m.arrays()
+    odb = O.OracleDb(chroms, names, sa=sa)
+    del sa
+    (s1, q1, n1), (s2, q2, n2) = batch
+    k = min(npairs, len(s1))
+    po = O.pe_params(350, 50)
+    sam_cpu, secs = O.orc_map(odb, po, s1[:k], q1[:k], n1[:k], mates=(s2[:k], q2[:k], n2[:k]), nthreads=cores)
+    sam_gpu = m.map((s1[:k], q1[:k], n1[:k]), p, mates=(s2[:k], q2[:k], n2[:k]))
+    mapped = sum(1 for rec in sam_cpu for line in rec.splitlines() if not (int(line.split("\t")[1]) & 4))
+    log("cpu baseline: %d pairs in %.2fs on %d threads" % (k, secs, cores))
+    return {"value": round(mapped / secs, 1), "unit": "mapped reads/s", "cores": cores, "kind": "port",
+            "sample": "first %d pairs (%d reads) of batch 0, oracle restatement with full SA, OpenMP per pair" % (k, 2 * k),
+            "seconds": round(secs, 2), "sam_identical_to_gpu": sam_cpu == sam_gpu}
+
+
+if __name__ == "__main__":
+    main()

@@ -68,6 +68,13 @@ const char* msgs_last_error(msgs_ctx* ctx);
  * The suffix array is built on the device.  names: '\n'-separated chrom names. */
 int msgs_db_build(msgs_ctx* ctx, const uint8_t* chromSeqs, const int64_t* chromLens, int32_t nChrom,
                   const char* names);
+/* The reference's database files DB.mtg + DB.mgs (MetaGenome::load / loadSeq,
+ * src/MetaGenome.cpp:42-73, src/Genome.cpp:29-97, header src/util/ProgEnv.cpp:24-64);
+ * the FMD index is rebuilt on the device from the sequences. */
+int msgs_db_load(msgs_ctx* ctx, const char* prefix);
+/* MetaGenome::numChroms / getChromName / getChrom(i).size() (src/MetaGenome.h:64-112) */
+int32_t msgs_db_nchrom(msgs_ctx* ctx);
+int msgs_db_chrom(msgs_ctx* ctx, int32_t tid, char* name, int64_t cap, int64_t* len);
 /* FMDIndex::length / getBaseCount / getCumCount (src/FMDIndex.h:152-179) */
 int64_t msgs_db_length(msgs_ctx* ctx);
 int msgs_db_counts(msgs_ctx* ctx, int64_t* B256, int64_t* C256);
@@ -111,7 +118,7 @@ int msgs_align_batch(msgs_ctx* ctx, const msgs_reads* reads, const msgs_params* 
                      int64_t* needed);
 
 /* ---- resident batches (bench): reads copied to HBM once, mapped many times ----
- * slot 0..3.  msgs_batch_run maps the resident batch and keeps the records on the
+ * slot 0..15.  msgs_batch_run maps the resident batch and keeps the records on the
  * host side of the context; msgs_batch_stats reports counters of the last run:
  * stats[0] reads, [1] mapped reads, [2] extension calls, [3] SA lookups, [4] DP cells,
  * [5] seeding kernel microseconds (HIP events), [6] DP kernel microseconds,

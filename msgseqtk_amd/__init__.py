@@ -22,7 +22,7 @@ MSGS_E_CAP = -6
 # every extern "C" symbol of include/msgs_hip.h
 EXPORTS = [
     "msgs_params_default", "msgs_ctx_create", "msgs_ctx_destroy", "msgs_last_error",
-    "msgs_db_build", "msgs_db_length", "msgs_db_counts", "msgs_db_copy", "msgs_db_loglik",
+    "msgs_db_build", "msgs_db_load", "msgs_db_nchrom", "msgs_db_chrom", "msgs_db_length", "msgs_db_counts", "msgs_db_copy", "msgs_db_loglik",
     "msgs_rank_batch", "msgs_ext_batch", "msgs_smem_batch", "msgs_seed_batch",
     "msgs_map_batch", "msgs_align_batch", "msgs_batch_upload", "msgs_batch_run",
     "msgs_batch_stats", "msgs_batch_sam", "msgs_util_sort_by_key",
@@ -65,6 +65,10 @@ def lib():
     L.msgs_last_error.argtypes = [vp]
     L.msgs_last_error.restype = C.c_char_p
     L.msgs_db_build.argtypes = [vp, PU8, P64, C.c_int32, C.c_char_p]
+    L.msgs_db_load.argtypes = [vp, C.c_char_p]
+    L.msgs_db_nchrom.argtypes = [vp]
+    L.msgs_db_nchrom.restype = C.c_int32
+    L.msgs_db_chrom.argtypes = [vp, C.c_int32, C.c_char_p, C.c_int64, P64]
     L.msgs_db_length.argtypes = [vp]
     L.msgs_db_length.restype = C.c_int64
     L.msgs_db_counts.argtypes = [vp, P64, P64]
@@ -152,6 +156,12 @@ class Mapper:
         nm = "\n".join(names or ["g%04d" % i for i in range(len(chroms))]).encode()
         self._chk(lib().msgs_db_build(self.h, _p(seqs, C.c_uint8), _p(lens, C.c_int64), len(chroms), nm),
                   "msgs_db_build")
+        self.n = lib().msgs_db_length(self.h)
+        return self
+
+    def load(self, prefix):
+        """the reference's database files prefix.mtg + prefix.mgs"""
+        self._chk(lib().msgs_db_load(self.h, prefix.encode()), "msgs_db_load")
         self.n = lib().msgs_db_length(self.h)
         return self
 

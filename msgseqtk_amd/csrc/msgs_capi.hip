@@ -11,6 +11,7 @@
 #include <cstring>
 #include <limits>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "msgs_capi_impl.h"
@@ -122,6 +123,7 @@ int msgs_ctx_create(int device, msgs_ctx** out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->nCU = prop.multiProcessorCount;
     if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return MSGS_E_HIP; }
+    c->hostThreads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
         hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess) {
         delete c;
@@ -374,7 +376,9 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
         a.out = c->bufs.get<SmemRec>("sm.out", (size_t)cap * std::max<int64_t>(n, 1), e);
         if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "smem output");
         (void)hipMemsetAsync(a.queue, 0, 8, st);
+        (void)hipEventRecord(c->ev0, st);
         if (n > 0) smem_kernel<<<(unsigned)(lanes / 256), 256, 0, st>>>(c->db.view(), a);
+        (void)hipEventRecord(c->ev1, st);
         if (hipGetLastError() != hipSuccess) return fail(c, MSGS_E_HIP, "smem launch");
         /* largest count */
         std::vector<int32_t> cnt(n);
@@ -383,6 +387,9 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
         int mx = 0;
         for (int64_t i = 0; i < n; ++i) mx = std::max(mx, cnt[i]);
         if (mx <= cap) {
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+            c->lastSmemMicros = ms * 1000.0;
             out.cap = cap;
             out.cnt = a.cnt;
             out.recs = a.out;
@@ -438,3 +445,84 @@ extern "C" void msgs_util_sort_by_key(const double* keys, int32_t* perm, int64_t
     };
     msgs_sort::sort(perm, perm + n, Less{keys});
 }
+
+/* ------------------------------------------------------------------ reference database files */
+
+#include <fstream>
+
+/* MetaGenome + its sequences from the reference's database files: DB.mtg
+ * ("MSGseqTK" + 3 int32 version numbers, then size_t #genomes and per genome the
+ * strings id and name, size_t #chroms and per chrom the string name and int64 length;
+ * strings are size_t length + bytes: src/util/ProgEnv.cpp:24-64,
+ * src/MetaGenome.cpp:42-73, src/Genome.cpp:66-97) and DB.mgs (per chrom len+1 nt16
+ * bytes incl. the 0 terminator: src/Genome.cpp:29-44).  The FMD index is rebuilt on
+ * the device from the sequences (the .fmd file is not needed). */
+extern "C" int msgs_db_load(msgs_ctx* c, const char* prefix) {
+    if (!c || !prefix) return MSGS_E_ARG;
+    std::string pre(prefix);
+    std::ifstream mtg(pre + ".mtg", std::ios::binary), mgs(pre + ".mgs", std::ios::binary);
+    if (!mtg) return fail(c, MSGS_E_ARG, "cannot open " + pre + ".mtg");
+    if (!mgs) return fail(c, MSGS_E_ARG, "cannot open " + pre + ".mgs");
+    char magic[8];
+    int32_t ver[3];
+    mtg.read(magic, 8);
+    mtg.read((char*)ver, sizeof(ver));
+    if (!mtg || std::string(magic, 8) != "MSGseqTK") return fail(c, MSGS_E_DB, "not a MSGseqTK database file");
+    auto readStr = [&](std::string& s) -> bool {
+        uint64_t len = 0;
+        mtg.read((char*)&len, 8);
+        if (!mtg || len > (1u << 30)) return false;
+        s.resize(len);
+        mtg.read(&s[0], len);
+        return (bool)mtg;
+    };
+    uint64_t NG = 0;
+    mtg.read((char*)&NG, 8);
+    if (!mtg || NG > (1u << 24)) return fail(c, MSGS_E_DB, "bad genome count");
+    std::vector<std::string> names;
+    std::vector<int64_t> lens;
+    for (uint64_t g = 0; g < NG; ++g) {
+        std::string id, nm;
+        uint64_t NC = 0;
+        if (!readStr(id) || !readStr(nm)) return fail(c, MSGS_E_DB, "truncated genome record");
+        mtg.read((char*)&NC, 8);
+        if (!mtg || NC > (1u << 24)) return fail(c, MSGS_E_DB, "bad chrom count");
+        for (uint64_t k = 0; k < NC; ++k) {
+            std::string cn;
+            int64_t len = 0;
+            if (!readStr(cn)) return fail(c, MSGS_E_DB, "truncated chrom record");
+            mtg.read((char*)&len, 8);
+            if (!mtg || len < 0) return fail(c, MSGS_E_DB, "bad chrom length");
+            names.push_back(cn);
+            lens.push_back(len);
+        }
+    }
+    int64_t total = 0;
+    for (int64_t L : lens) total += L;
+    std::vector<uint8_t> seqs(total);
+    int64_t off = 0;
+    for (size_t t = 0; t < lens.size(); ++t) {
+        mgs.read((char*)seqs.data() + off, lens[t]);
+        char term = 1;
+        mgs.read(&term, 1);
+        if (!mgs || term != 0) return fail(c, MSGS_E_DB, "truncated or malformed sequence file");
+        off += lens[t];
+    }
+    std::string joined;
+    for (size_t t = 0; t < names.size(); ++t) { joined += names[t]; joined += '\n'; }
+    return msgs_db_build(c, seqs.data(), lens.data(), (int32_t)lens.size(), joined.c_str());
+}
+
+/* chrom names and lengths of the resident database (for record headers) */
+extern "C" int msgs_db_chrom(msgs_ctx* c, int32_t tid, char* name, int64_t cap, int64_t* len) {
+    if (!c || !c->db.n) return fail(c, MSGS_E_STATE, "no database");
+    if (tid < 0 || tid >= c->db.nChrom) return fail(c, MSGS_E_ARG, "bad tid");
+    if (len) *len = c->db.chromLen[tid];
+    if (name) {
+        const std::string& s = c->db.names[tid];
+        if ((int64_t)s.size() + 1 > cap) return fail(c, MSGS_E_CAP, "name capacity");
+        std::memcpy(name, s.c_str(), s.size() + 1);
+    }
+    return MSGS_OK;
+}
+extern "C" int32_t msgs_db_nchrom(msgs_ctx* c) { return c ? c->db.nChrom : 0; }

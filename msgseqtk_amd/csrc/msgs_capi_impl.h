@@ -50,10 +50,21 @@ struct SmemStage {
     std::vector<int32_t> cntHost;
 };
 
+/* host copy of a batch (names, bases and qualities are needed for the records) */
+struct HostBatch {
+    std::vector<uint8_t> seq, qual;
+    std::vector<int64_t> offs;
+    std::string names;
+    msgs_reads view{};
+};
+
 /* a resident batch (reads kept in HBM between runs) */
 struct Slot {
     bool used = false;
     bool paired = false;
+    HostBatch host[2];
+    DevReads dev[2];
+    uint8_t* dQual[2] = {nullptr, nullptr};
     void release() { used = false; }
 };
 
@@ -67,8 +78,12 @@ struct msgs_ctx {
     std::string err;
     DbState db;
     BufPool bufs;
-    Slot slots[4];
+    Slot slots[16];
     double stats[16] = {};
+    double lastSmemMicros = 0;
+    int hostThreads = 16;
+    std::string lastSam;
+    std::vector<int64_t> lastLineOffs;
 };
 
 int fail(msgs_ctx* c, int code, const std::string& msg);

@@ -110,11 +110,13 @@ struct DpArgs {
     const int32_t* jobChainStart;
     const int32_t* jobChainLen;
     const int32_t* chainSeeds;
+    int32_t* auxD0;            /* per chain seed: first diagonal value */
+    int64_t* auxGap;           /* per chain seed: tile offset of the following gap rect */
     const int64_t* tileOff;    /* per job offset into the tile pool (cells) */
     int32_t* tileM;
     int32_t* tileI;
     int32_t* tileD;
-    const int64_t* pathOff;    /* per job offset into path/cigar/md pools */
+    const int64_t* pathOff;    /* per job offset into path pool; cigar at +2*job, md at 3x+8*job */
     uint8_t* path;
     uint32_t* cigar;
     char* md;

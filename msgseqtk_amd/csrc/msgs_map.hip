@@ -1,0 +1,858 @@
+/*
+ * msgs_map.hip -- the whole mapping path over one batch: SMEMs -> seeds -> chains ->
+ * DP/backtrace on the device, then the host finishing step of msgseqtk-align
+ * (src/msgseqtk-align.cpp:538-758): Alignment::filter, calcMapQ, sort, PE pairing
+ * and record output.
+ *
+ * Host finishing keeps the libm-dependent decisions (exp, log10, round) on the host,
+ * where the reference computes them, and reorders with the restated libstdc++ sort.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <limits>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "msgs_capi_impl.h"
+#include "msgs_sort.h"
+
+namespace {
+
+constexpr int32_t NEG = -1073741824;
+
+__global__ void k_revcom(const uint8_t* seq, const uint8_t* qual, const int64_t* offs, int64_t n, uint8_t* rs,
+                         uint8_t* rq) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t o = offs[r], L = offs[r + 1] - o;
+        for (int64_t i = 0; i < L; ++i) {
+            uint8_t b = seq[o + L - 1 - i];
+            rs[o + i] = (uint8_t)(((b & 1) << 3) | ((b & 2) << 1) | ((b & 4) >> 1) | ((b & 8) >> 3));
+            rq[o + i] = qual[o + L - 1 - i];
+        }
+    }
+}
+
+__global__ void k_fill_u8(uint8_t* p, int64_t n, uint8_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+inline unsigned gridFor(int64_t n, int t = 256) {
+    return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + t - 1) / t, 1 << 16));
+}
+
+template <class T> int64_t exclusiveScan(const std::vector<T>& v, std::vector<int64_t>& off) {
+    off.resize(v.size() + 1);
+    int64_t s = 0;
+    for (size_t i = 0; i < v.size(); ++i) { off[i] = s; s += (int64_t)v[i]; }
+    off[v.size()] = s;
+    return s;
+}
+
+#define CK(x, msg)                                                       \
+    do {                                                                 \
+        hipError_t e_ = (x);                                             \
+        if (e_ != hipSuccess) return fail(c, MSGS_E_HIP, std::string(msg) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+}  // namespace
+
+/* ------------------------------------------------------------------ device stages */
+
+struct SeedStage {
+    std::vector<int64_t> seedOff;   /* host copy, n+1 */
+    std::vector<int32_t> seedCnt;
+    int64_t* dSeedOff = nullptr;
+    int32_t* dSeedCnt = nullptr;
+    SeedRec* dSeeds = nullptr;
+};
+
+struct ChainStage {
+    int64_t nJobs = 0;
+    std::vector<int64_t> chainOff;  /* per read, n+1 */
+    int32_t* dJobRead = nullptr;
+    int32_t* dChainStart = nullptr;
+    int32_t* dChainLen = nullptr;
+    int32_t* dChainSeeds = nullptr;
+    int64_t nChainSeeds = 0;
+};
+
+struct DpStage {
+    std::vector<AlnRec> recs;
+    std::vector<int64_t> pathOff;
+    std::vector<uint32_t> cigar;
+    std::vector<char> md;
+    std::vector<uint8_t> path;
+};
+
+static int runSeeds(msgs_ctx* c, const DevReads& r, const msgs_params* p, const SmemStage& sm, SeedStage& out,
+                    unsigned long long* dStats) {
+    const int64_t n = r.n;
+    hipStream_t st = c->stream;
+    hipError_t e = hipSuccess;
+    SeedArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.offs = r.offs; a.nReads = n; a.smemCnt = sm.cnt; a.smems = sm.recs; a.cap = sm.cap;
+    a.maxNSeed = p->maxNSeed;
+    a.order = c->bufs.get<int32_t>("sd.ord", (size_t)sm.cap * std::max<int64_t>(n, 1), e);
+    a.nUsed = c->bufs.get<int32_t>("sd.used", n + 1, e);
+    a.nRaw = c->bufs.get<int64_t>("sd.raw", n + 1, e);
+    a.stats = dStats;
+    if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "seed scratch");
+    if (n) smem_order_kernel<<<gridFor(n), 256, 0, st>>>(c->db.view(), a);
+    std::vector<int64_t> raw(n);
+    if (n) CK(hipMemcpyAsync(raw.data(), a.nRaw, n * 8, hipMemcpyDeviceToHost, st), "nRaw");
+    CK(hipStreamSynchronize(st), "order kernel");
+    const int64_t total = exclusiveScan(raw, out.seedOff);
+    out.dSeedOff = c->bufs.get<int64_t>("sd.off", n + 1, e);
+    out.dSeeds = c->bufs.get<SeedRec>("sd.seeds", total + 1, e);
+    out.dSeedCnt = c->bufs.get<int32_t>("sd.cnt", n + 1, e);
+    if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "seed pool");
+    CK(hipMemcpyAsync(out.dSeedOff, out.seedOff.data(), (n + 1) * 8, hipMemcpyHostToDevice, st), "seedOff");
+    a.seedOff = out.dSeedOff; a.seeds = out.dSeeds; a.seedCnt = out.dSeedCnt;
+    if (n) seed_kernel<<<gridFor(n), 256, 0, st>>>(c->db.view(), a);
+    out.seedCnt.resize(n);
+    if (n) CK(hipMemcpyAsync(out.seedCnt.data(), out.dSeedCnt, n * 4, hipMemcpyDeviceToHost, st), "seedCnt");
+    CK(hipStreamSynchronize(st), "seed kernel");
+    return MSGS_OK;
+}
+
+static int runChains(msgs_ctx* c, const DevReads& r, const SeedStage& sd, ChainStage& out) {
+    const int64_t n = r.n;
+    hipStream_t st = c->stream;
+    hipError_t e = hipSuccess;
+    const int64_t seedTotal = sd.seedOff[n];
+    ChainArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.offs = r.offs; a.nReads = n; a.seedOff = sd.dSeedOff; a.seedCnt = sd.dSeedCnt; a.seeds = sd.dSeeds;
+    a.stackNode = c->bufs.get<int32_t>("ch.node", seedTotal + 1, e);
+    a.stackNext = c->bufs.get<int32_t>("ch.next", seedTotal + 1, e);
+    a.seen = c->bufs.get<uint8_t>("ch.seen", seedTotal + 1, e);
+    a.nChains = c->bufs.get<int32_t>("ch.n", n + 1, e);
+    a.nChainSeeds = c->bufs.get<int64_t>("ch.ns", n + 1, e);
+    if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "chain scratch");
+    a.pass = 0;
+    if (n) chain_kernel<<<gridFor(n), 256, 0, st>>>(a);
+    std::vector<int32_t> nch(n);
+    std::vector<int64_t> ncs(n);
+    if (n) {
+        CK(hipMemcpyAsync(nch.data(), a.nChains, n * 4, hipMemcpyDeviceToHost, st), "nChains");
+        CK(hipMemcpyAsync(ncs.data(), a.nChainSeeds, n * 8, hipMemcpyDeviceToHost, st), "nChainSeeds");
+    }
+    CK(hipStreamSynchronize(st), "chain pass 0");
+    std::vector<int64_t> csOff;
+    out.nJobs = exclusiveScan(nch, out.chainOff);
+    out.nChainSeeds = exclusiveScan(ncs, csOff);
+    int64_t* dChainOff = c->bufs.get<int64_t>("ch.coff", n + 1, e);
+    int64_t* dCsOff = c->bufs.get<int64_t>("ch.soff", n + 1, e);
+    out.dChainStart = c->bufs.get<int32_t>("ch.start", out.nJobs + 1, e);
+    out.dChainLen = c->bufs.get<int32_t>("ch.len", out.nJobs + 1, e);
+    out.dChainSeeds = c->bufs.get<int32_t>("ch.seeds", out.nChainSeeds + 1, e);
+    out.dJobRead = c->bufs.get<int32_t>("ch.read", out.nJobs + 1, e);
+    if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "chain pool");
+    std::vector<int32_t> jobRead(out.nJobs);
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t j = out.chainOff[i]; j < out.chainOff[i + 1]; ++j) jobRead[j] = (int32_t)i;
+    CK(hipMemcpyAsync(dChainOff, out.chainOff.data(), (n + 1) * 8, hipMemcpyHostToDevice, st), "chainOff");
+    CK(hipMemcpyAsync(dCsOff, csOff.data(), (n + 1) * 8, hipMemcpyHostToDevice, st), "csOff");
+    if (out.nJobs)
+        CK(hipMemcpyAsync(out.dJobRead, jobRead.data(), out.nJobs * 4, hipMemcpyHostToDevice, st), "jobRead");
+    a.chainOff = dChainOff; a.chainSeedOff = dCsOff;
+    a.chainStart = out.dChainStart; a.chainLen = out.dChainLen; a.chainSeeds = out.dChainSeeds;
+    a.pass = 1;
+    if (n) chain_kernel<<<gridFor(n), 256, 0, st>>>(a);
+    CK(hipStreamSynchronize(st), "chain pass 1");
+    return MSGS_OK;
+}
+
+static bool integral(double x) { return std::isfinite(x) && x == std::floor(x) && std::fabs(x) <= 10000; }
+
+static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msgs_params* p, const SeedStage& sd,
+                 const ChainStage& ch, DpStage& out, double* dpMicros) {
+    const int64_t n = r.n, nJobs = ch.nJobs;
+    hipStream_t st = c->stream;
+    hipError_t e = hipSuccess;
+    if (!integral(p->match) || !integral(p->mismatch) || !integral(p->gapO) || !integral(p->gapE))
+        return fail(c, MSGS_E_UNSUPPORTED, "the device DP needs integer-valued match/mismatch/gap scores");
+    out.recs.clear();
+    if (nJobs == 0) return MSGS_OK;
+    DpArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.seqs = r.seq; a.quals = dQual; a.offs = r.offs;
+    uint8_t* rcS = c->bufs.get<uint8_t>("dp.rs", r.total + 1, e);
+    uint8_t* rcQ = c->bufs.get<uint8_t>("dp.rq", r.total + 1, e);
+    a.rcSeqs = rcS;
+    a.rcQuals = rcQ;
+    a.seedOff = sd.dSeedOff; a.seeds = sd.dSeeds; a.nJobs = nJobs;
+    a.jobRead = ch.dJobRead; a.jobChainStart = ch.dChainStart; a.jobChainLen = ch.dChainLen;
+    a.chainSeeds = ch.dChainSeeds;
+    a.auxD0 = c->bufs.get<int32_t>("dp.d0", ch.nChainSeeds + 1, e);
+    a.auxGap = c->bufs.get<int64_t>("dp.gap", ch.nChainSeeds + 1, e);
+    a.mode = p->mode;
+    a.match = (int32_t)p->match; a.mismatch = (int32_t)p->mismatch; a.gapO = (int32_t)p->gapO; a.gapE = (int32_t)p->gapE;
+    a.clip = p->clip; a.gapE_d = p->gapE;
+    /* quality terms of Alignment::evaluate with the host libm (src/Alignment.cpp:460-477,
+     * src/QualStr.h:109-117) */
+    double hq[3 * 256];
+    for (int q = 0; q < 256; ++q) {
+        hq[q] = ::log10(1 - ::exp(std::log(10.0) * q / -10.0));
+        hq[256 + q] = q / -10.0;
+        hq[512 + q] = q / -10.0 - p->gapO;
+    }
+    double* dq = c->bufs.get<double>("dp.qtab", 3 * 256, e);
+    a.tileNeed = c->bufs.get<int64_t>("dp.tneed", nJobs + 1, e);
+    a.pathNeed = c->bufs.get<int64_t>("dp.pneed", nJobs + 1, e);
+    if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "dp scratch");
+    CK(hipMemcpyAsync(dq, hq, sizeof(hq), hipMemcpyHostToDevice, st), "qtab");
+    a.qMatch = dq; a.qMis = dq + 256; a.qGapOpen = dq + 512;
+    if (n) k_revcom<<<gridFor(n), 256, 0, st>>>(r.seq, dQual, r.offs, n, rcS, rcQ);
+    a.sizeOnly = 1;
+    dp_kernel<<<gridFor(nJobs), 256, 0, st>>>(c->db.view(), a);
+    std::vector<int64_t> tneed(nJobs), pneed(nJobs);
+    CK(hipMemcpyAsync(tneed.data(), a.tileNeed, nJobs * 8, hipMemcpyDeviceToHost, st), "tneed");
+    CK(hipMemcpyAsync(pneed.data(), a.pathNeed, nJobs * 8, hipMemcpyDeviceToHost, st), "pneed");
+    CK(hipStreamSynchronize(st), "dp sizing");
+    std::vector<int64_t> tileOff;
+    const int64_t tiles = exclusiveScan(tneed, tileOff);
+    const int64_t paths = exclusiveScan(pneed, out.pathOff);
+    int64_t* dTileOff = c->bufs.get<int64_t>("dp.toff", nJobs + 1, e);
+    int64_t* dPathOff = c->bufs.get<int64_t>("dp.poff", nJobs + 1, e);
+    a.tileM = c->bufs.get<int32_t>("dp.tm", tiles + 1, e);
+    a.tileI = c->bufs.get<int32_t>("dp.ti", tiles + 1, e);
+    a.tileD = c->bufs.get<int32_t>("dp.td", tiles + 1, e);
+    a.path = c->bufs.get<uint8_t>("dp.path", paths + 1, e);
+    a.cigar = c->bufs.get<uint32_t>("dp.cig", paths + 2 * nJobs + 1, e);
+    a.md = c->bufs.get<char>("dp.md", 3 * paths + 8 * nJobs + 1, e);
+    a.out = c->bufs.get<AlnRec>("dp.out", nJobs + 1, e);
+    if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "dp pools");
+    CK(hipMemcpyAsync(dTileOff, tileOff.data(), (nJobs + 1) * 8, hipMemcpyHostToDevice, st), "toff");
+    CK(hipMemcpyAsync(dPathOff, out.pathOff.data(), (nJobs + 1) * 8, hipMemcpyHostToDevice, st), "poff");
+    a.tileOff = dTileOff; a.pathOff = dPathOff;
+    a.sizeOnly = 0;
+    CK(hipEventRecord(c->ev2, st), "event");
+    dp_kernel<<<gridFor(nJobs), 256, 0, st>>>(c->db.view(), a);
+    CK(hipEventRecord(c->ev3, st), "event");
+    out.recs.resize(nJobs);
+    out.cigar.resize(paths + 2 * nJobs + 1);
+    out.md.resize(3 * paths + 8 * nJobs + 1);
+    CK(hipMemcpyAsync(out.recs.data(), a.out, nJobs * sizeof(AlnRec), hipMemcpyDeviceToHost, st), "recs");
+    CK(hipMemcpyAsync(out.cigar.data(), a.cigar, out.cigar.size() * 4, hipMemcpyDeviceToHost, st), "cigar");
+    CK(hipMemcpyAsync(out.md.data(), a.md, out.md.size(), hipMemcpyDeviceToHost, st), "md");
+    CK(hipStreamSynchronize(st), "dp kernel");
+    if (dpMicros) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, c->ev2, c->ev3);
+        *dpMicros = ms * 1000.0;
+    }
+    return MSGS_OK;
+}
+
+/* ------------------------------------------------------------------ host finishing */
+
+namespace {
+
+enum { FWD = 1, REV = 2 };
+
+struct HostRead {
+    const uint8_t* seq;
+    const uint8_t* qual;
+    int64_t L;
+    std::string name;
+};
+
+struct HAln {
+    const AlnRec* r;
+    double log10P;
+    double postP = 0;
+    uint8_t mapQ = 0xff;
+};
+
+/* Eigen's VectorXd::sum for SSE2 2-wide packets: two packet accumulators over
+ * multiples of 4, one more packet, horizontal add, then the scalar tail. */
+double packetSum(const std::vector<double>& v) {
+    const size_t n = v.size();
+    if (n == 0) return 0;
+    const size_t a2 = (n / 4) * 4, a1 = (n / 2) * 2;
+    if (a1 == 0) return v[0];
+    double p0 = v[0], p1 = v[1];
+    if (a1 > 2) {
+        double q0 = v[2], q1 = v[3];
+        for (size_t k = 4; k < a2; k += 4) { p0 += v[k]; p1 += v[k + 1]; q0 += v[k + 2]; q1 += v[k + 3]; }
+        p0 += q0; p1 += q1;
+        if (a1 > a2) { p0 += v[a2]; p1 += v[a2 + 1]; }
+    }
+    double s = p0 + p1;
+    for (size_t k = a1; k < n; ++k) s += v[k];
+    return s;
+}
+
+/* calcMapQ (src/Alignment.cpp:491-512 and :514-539) on log-likelihoods */
+void mapq(const std::vector<double>& ll, std::vector<double>& postP, std::vector<uint8_t>& mq, bool pe) {
+    const size_t n = ll.size();
+    std::vector<double> pr(ll);
+    double maxV = *std::max_element(pr.begin(), pr.end());
+    const double inf = std::numeric_limits<double>::infinity();
+    double scale = (maxV != -inf && maxV < -256) ? -256 - maxV : 0;
+    for (size_t i = 0; i < n; ++i) pr[i] = std::exp(pr[i] + scale);
+    double sum = packetSum(pr);
+    postP.resize(n);
+    mq.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+        double pp = (!pe || sum > 0) ? pr[i] / sum : 1.0 / (double)n;
+        postP[i] = pp;
+        double q = -10 * ::log10(1 - pp);
+        mq[i] = (uint8_t)std::min(::round(q), 250.0);
+    }
+}
+
+struct Finisher {
+    const DbState& db;
+    const msgs_params& P;
+    const DpStage& dp;
+    bool emit = true;
+    int64_t mapped = 0;
+    const double log10e = ::log10(M_E);
+
+    std::string cigarOf(int64_t job) const {
+        static const char kOps[] = "MIDNSHP=XB";
+        const AlnRec& r = dp.recs[job];
+        const uint32_t* cg = dp.cigar.data() + dp.pathOff[job] + 2 * job;
+        std::string s;
+        for (int k = 0; k < r.nCigar; ++k) { s += std::to_string(cg[k] >> 4); s += kOps[cg[k] & 15]; }
+        return s;
+    }
+    std::string mdOf(int64_t job) const {
+        const AlnRec& r = dp.recs[job];
+        return std::string(dp.md.data() + 3 * dp.pathOff[job] + 8 * job, r.mdLen);
+    }
+
+    static void fmtFloat(std::string& s, float f) {
+        char buf[48];
+        std::snprintf(buf, sizeof buf, "%g", (double)f);
+        s += buf;
+    }
+
+    /* one SAM text record (field layout of htslib's sam_format1) */
+    void line(std::string& s, const HostRead& rd, bool rc, int flag, int32_t tid, int64_t pos, int mapq,
+              const std::string& cigar, int32_t isize, const int64_t* tags, const std::string* md, float xh,
+              float xp) const {
+        static const char kNt16[17] = "=ACMGRSVTWYHKDBN";
+        if (!emit) return;
+        s += rd.name;
+        s += '\t'; s += std::to_string(flag);
+        s += '\t'; s += tid >= 0 ? db.names[tid] : std::string("*");
+        s += '\t'; s += std::to_string(pos + 1);
+        s += '\t'; s += std::to_string(mapq);
+        s += '\t'; s += cigar.empty() ? std::string("*") : cigar;
+        s += "\t*\t0\t"; s += std::to_string(isize);
+        s += '\t';
+        if (rd.L == 0) s += '*';
+        for (int64_t i = 0; i < rd.L; ++i) {
+            uint8_t b = rc ? (uint8_t)(((rd.seq[rd.L - 1 - i] & 1) << 3) | ((rd.seq[rd.L - 1 - i] & 2) << 1) |
+                                       ((rd.seq[rd.L - 1 - i] & 4) >> 1) | ((rd.seq[rd.L - 1 - i] & 8) >> 3))
+                           : rd.seq[i];
+            s += kNt16[b & 15];
+        }
+        s += '\t';
+        if (rd.L == 0 || (rc ? rd.qual[rd.L - 1] : rd.qual[0]) == 0xff) s += '*';
+        else for (int64_t i = 0; i < rd.L; ++i) s += (char)((rc ? rd.qual[rd.L - 1 - i] : rd.qual[i]) + 33);
+        if (tags) {
+            s += "\tAS:i:"; s += std::to_string(tags[0]);
+            s += "\tMD:Z:"; s += *md;
+            s += "\tXH:f:"; fmtFloat(s, xh);
+            s += "\tNH:i:"; s += std::to_string(tags[1]);
+            s += "\tXN:i:"; s += std::to_string(tags[2]);
+            s += "\tXP:f:"; fmtFloat(s, xp);
+        }
+        s += '\n';
+    }
+
+    void unmapped(std::string& s, const HostRead& rd, int flag) const {
+        line(s, rd, false, flag, -1, 0, 0, std::string(), 0, nullptr, nullptr, 0, 0);
+    }
+
+    /* exportBAM + output() (src/Alignment.cpp:429-444, src/msgseqtk-align.cpp:538-557) */
+    void alnRecord(std::string& s, const HostRead& rd, const HAln& a, int flagOr, int mapq, int32_t isize,
+                   int64_t nh, int64_t xn, double postP) const {
+        const AlnRec& r = *a.r;
+        const int64_t job = &r - dp.recs.data();
+        const bool rev = r.strand == REV;
+        int flag = (rev ? 16 : 0) | flagOr;
+        int64_t tags[3] = {(int64_t)(int)std::round((double)r.score), nh, xn};
+        std::string md = mdOf(job);
+        line(s, rd, rev, flag, r.tid, r.alnStart, mapq, cigarOf(job), isize, tags, &md, (float)a.log10P,
+             (float)postP);
+    }
+
+    /* filter (src/Alignment.cpp:548-555): drop alnScore < L * minScoreRate */
+    std::vector<HAln> alnList(const HostRead& rd, int64_t j0, int64_t j1) const {
+        std::vector<HAln> v;
+        const double thr = (double)rd.L * P.minScoreRate;
+        for (int64_t j = j0; j < j1; ++j) {
+            const AlnRec& r = dp.recs[j];
+            double sc = r.score == NEG ? -std::numeric_limits<double>::infinity() : (double)r.score;
+            if (!(sc < thr)) v.push_back(HAln{&r, r.log10P});
+        }
+        return v;
+    }
+
+    void calcMapQ(std::vector<HAln>& v) const {
+        std::vector<double> ll(v.size()), pp;
+        std::vector<uint8_t> mq;
+        for (size_t i = 0; i < v.size(); ++i) ll[i] = v[i].log10P / log10e;
+        mapq(ll, pp, mq, false);
+        for (size_t i = 0; i < v.size(); ++i) { v[i].postP = pp[i]; v[i].mapQ = mq[i]; }
+    }
+
+    static void sortAlns(std::vector<HAln>& v) {
+        msgs_sort::sort(v.data(), v.data() + v.size(), [](const HAln& l, const HAln& r) { return l.log10P > r.log10P; });
+    }
+
+    void outputAlns(std::string& s, const HostRead& rd, const std::vector<HAln>& v, int extra) const {
+        size_t num = P.maxReport == 0 ? v.size() : std::min((size_t)(uint32_t)P.maxReport, v.size());
+        for (size_t i = 0; i < num; ++i)
+            alnRecord(s, rd, v[i], extra | (i > 0 ? 256 : 0), v[i].mapQ, 0, (int64_t)num, (int64_t)v.size(), v[i].postP);
+    }
+
+    /* main_SE task (src/msgseqtk-align.cpp:607-649) */
+    void se(std::string& s, const HostRead& rd, int32_t nSeeds, int64_t j0, int64_t j1) {
+        if (nSeeds == 0) { unmapped(s, rd, 4); return; }
+        std::vector<HAln> v = alnList(rd, j0, j1);
+        if (v.empty()) { unmapped(s, rd, 4); return; }
+        mapped++;
+        calcMapQ(v);
+        sortAlns(v);
+        outputAlns(s, rd, v, 0);
+    }
+
+    /* PairingScheme::pr with boost::math::pdf(normal) evaluated as boost does it */
+    double pairPr(double x) const {
+        if (P.insMean == 0) return 1;
+        if (P.insMin <= x && x <= P.insMax) {
+            double ex = x - P.insMean;
+            ex *= -ex;
+            ex /= 2 * P.insSD * P.insSD;
+            double r = std::exp(ex);
+            r /= P.insSD * 2.506628274631000502415765284811045253e+00;
+            return r;
+        }
+        return 0;
+    }
+
+    /* main_PE task (src/msgseqtk-align.cpp:679-754) */
+    void pe(std::string& s, const HostRead& r1, const HostRead& r2, int32_t ns1, int32_t ns2, int64_t a0, int64_t a1,
+            int64_t b0, int64_t b1) {
+        if (ns1 == 0 && ns2 == 0) { unmapped(s, r1, 4 | 8 | 1 | 64); unmapped(s, r2, 4 | 8 | 1 | 128); return; }
+        std::vector<HAln> f = alnList(r1, a0, a1), r = alnList(r2, b0, b1);
+        if (f.empty() && r.empty()) { unmapped(s, r1, 4 | 8 | 1 | 64); unmapped(s, r2, 4 | 8 | 1 | 128); return; }
+        struct Pr { const HAln* f; const HAln* r; int32_t isize; double l10; double ll; double postP; uint8_t mq; };
+        std::vector<Pr> pairs;
+        for (const HAln& x : f)
+            for (const HAln& y : r)
+                if ((x.r->strand & y.r->strand) == 0) {
+                    int32_t isize = x.r->tid != y.r->tid ? -1 :
+                        (int32_t)(std::max(x.r->alnEnd, y.r->alnEnd) - std::min(x.r->alnStart, y.r->alnStart));
+                    double pr = pairPr(isize);
+                    pairs.push_back(Pr{&x, &y, isize, x.log10P + y.log10P + std::log10(pr),
+                                       x.log10P / log10e + y.log10P / log10e + std::log(pr), 0, 0xff});
+                }
+        msgs_sort::sort(pairs.data(), pairs.data() + pairs.size(), [](const Pr& l, const Pr& r) { return l.l10 > r.l10; });
+        auto rm = [&](std::function<bool(const Pr&)> g) { pairs.erase(std::remove_if(pairs.begin(), pairs.end(), g), pairs.end()); };
+        rm([&](const Pr& q) { return !(P.insMin <= q.isize && q.isize <= P.insMax); });
+        if (P.noDiscordant) rm([](const Pr& q) { return !(q.f->r->tid == q.r->r->tid && (q.f->r->strand & q.r->r->strand) == 0); });
+        if (P.noTailOver) rm([](const Pr& q) {
+            const AlnRec &a = *q.f->r, &b = *q.r->r;
+            return (a.alnStart <= b.alnStart && a.alnEnd > b.alnStart) || (b.alnStart <= a.alnStart && b.alnEnd > a.alnStart); });
+        if (P.noContain) rm([](const Pr& q) {
+            const AlnRec &a = *q.f->r, &b = *q.r->r;
+            return (a.alnStart <= b.alnStart && a.alnEnd >= b.alnEnd) || (b.alnStart <= a.alnStart && b.alnEnd >= a.alnEnd); });
+        if (P.noOverlap) rm([](const Pr& q) { return q.f->r->alnStart < q.r->r->alnEnd && q.f->r->alnEnd > q.r->r->alnStart; });
+        if (P.maxNPair > 0 && (int64_t)pairs.size() > P.maxNPair) pairs.erase(pairs.begin() + P.maxNPair, pairs.end());
+        if (!pairs.empty()) {
+            mapped += 2;
+            std::vector<double> ll(pairs.size()), pp;
+            std::vector<uint8_t> mq;
+            for (size_t i = 0; i < pairs.size(); ++i) ll[i] = pairs[i].ll;
+            mapq(ll, pp, mq, true);
+            size_t num = P.maxReport == 0 ? pairs.size() : std::min((size_t)(uint32_t)P.maxReport, pairs.size());
+            for (size_t i = 0; i < num; ++i) {
+                const Pr& q = pairs[i];
+                const AlnRec &a = *q.f->r, &b = *q.r->r;
+                int32_t fi = a.alnStart < b.alnStart ? q.isize : -q.isize;
+                int sec = i > 0 ? 256 : 0;
+                alnRecord(s, r1, *q.f, 1 | 2 | 64 | sec, mq[i], fi, (int64_t)num, (int64_t)pairs.size(), pp[i]);
+                alnRecord(s, r2, *q.r, 1 | 2 | 128 | sec, mq[i], -fi, (int64_t)num, (int64_t)pairs.size(), pp[i]);
+            }
+        } else if (!P.noMixed) {
+            mapped += (f.empty() ? 0 : 1) + (r.empty() ? 0 : 1);
+            if (!f.empty()) { calcMapQ(f); sortAlns(f); outputAlns(s, r1, f, 1 | 64); }
+            if (!r.empty()) { calcMapQ(r); sortAlns(r); outputAlns(s, r2, r, 1 | 128); }
+        }
+    }
+};
+
+std::vector<std::string> splitNames(const char* names, int64_t n) {
+    std::vector<std::string> v;
+    v.reserve(n);
+    const char* p = names ? names : "";
+    for (int64_t i = 0; i < n; ++i) {
+        const char* e = p;
+        while (*e && *e != '\n') ++e;
+        v.emplace_back(p, e);
+        p = *e ? e + 1 : e;
+    }
+    return v;
+}
+
+}  // namespace
+
+/* ------------------------------------------------------------------ one batch, all stages */
+
+struct BatchResult {
+    SmemStage sm;
+    SeedStage sd;
+    ChainStage ch;
+    DpStage dp;
+};
+
+static int uploadQual(msgs_ctx* c, const msgs_reads* r, const DevReads& d, const char* name, uint8_t** out) {
+    hipError_t e = hipSuccess;
+    uint8_t* q = c->bufs.get<uint8_t>(name, d.total + 1, e);
+    if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "quality buffer");
+    if (d.total) {
+        if (r->qual) CK(hipMemcpyAsync(q, r->qual + r->offs[0], d.total, hipMemcpyHostToDevice, c->stream), "qual");
+        else k_fill_u8<<<gridFor(d.total), 256, 0, c->stream>>>(q, d.total, 30);
+    }
+    *out = q;
+    return MSGS_OK;
+}
+
+static int runAll(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msgs_params* p, BatchResult& b,
+                  unsigned long long* dStats, double* dpMicros) {
+    int rc = runSmem(c, r, p, b.sm, dStats);
+    if (rc) return rc;
+    rc = runSeeds(c, r, p, b.sm, b.sd, dStats);
+    if (rc) return rc;
+    rc = runChains(c, r, b.sd, b.ch);
+    if (rc) return rc;
+    return runDp(c, r, dQual, p, b.sd, b.ch, b.dp, dpMicros);
+}
+
+static int checkParams(msgs_ctx* c, const msgs_params* p) {
+    if (!p) return fail(c, MSGS_E_ARG, "no parameters");
+    if (p->minLen <= 0) return fail(c, MSGS_E_ARG, "minLen must be positive");
+    if (p->maxNSeed <= 0) return fail(c, MSGS_E_ARG, "maxNSeed must be positive");
+    if (p->mode != 0 && p->mode != 1) return fail(c, MSGS_E_ARG, "mode must be 0 (GLOBAL) or 1 (LOCAL)");
+    return MSGS_OK;
+}
+
+extern "C" int msgs_seed_batch(msgs_ctx* c, const msgs_reads* reads, const msgs_params* p, int64_t* counts,
+                               int64_t* rows6, double* logP, int64_t cap, int64_t* needed) {
+    if (!c || !c->db.n) return fail(c, MSGS_E_STATE, "no database");
+    int rc = checkParams(c, p);
+    if (rc) return rc;
+    DevReads r;
+    rc = uploadReads(c, reads, "rd", r);
+    if (rc) return rc;
+    SmemStage sm;
+    SeedStage sd;
+    rc = runSmem(c, r, p, sm, nullptr);
+    if (rc) return rc;
+    rc = runSeeds(c, r, p, sm, sd, nullptr);
+    if (rc) return rc;
+    const int64_t n = r.n;
+    std::vector<SeedRec> seeds(sd.seedOff[n] + 1);
+    CK(hipMemcpyAsync(seeds.data(), sd.dSeeds, seeds.size() * sizeof(SeedRec), hipMemcpyDeviceToHost, c->stream), "seeds");
+    CK(hipStreamSynchronize(c->stream), "sync");
+    int64_t total = 0;
+    for (int64_t i = 0; i < n; ++i) total += sd.seedCnt[i];
+    if (needed) *needed = total;
+    if (total > cap) return fail(c, MSGS_E_CAP, "row capacity");
+    int64_t k = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (counts) counts[i] = sd.seedCnt[i];
+        for (int32_t j = 0; j < sd.seedCnt[i]; ++j, ++k) {
+            const SeedRec& s = seeds[sd.seedOff[i] + j];
+            int64_t* o = rows6 + 6 * k;
+            o[0] = s.from; o[1] = s.to; o[2] = s.start; o[3] = s.end; o[4] = s.tid; o[5] = s.strand;
+            if (logP) logP[k] = s.logP;
+        }
+    }
+    return MSGS_OK;
+}
+
+extern "C" int msgs_align_batch(msgs_ctx* c, const msgs_reads* reads, const msgs_params* p, int64_t* counts,
+                                int64_t* rows10, double* scores, char* cigars, int64_t cigCap, int64_t cap,
+                                int64_t* needed) {
+    if (!c || !c->db.n) return fail(c, MSGS_E_STATE, "no database");
+    int rc = checkParams(c, p);
+    if (rc) return rc;
+    DevReads r;
+    rc = uploadReads(c, reads, "rd", r);
+    if (rc) return rc;
+    uint8_t* dQual = nullptr;
+    rc = uploadQual(c, reads, r, "rd.qual", &dQual);
+    if (rc) return rc;
+    BatchResult b;
+    rc = runAll(c, r, dQual, p, b, nullptr, nullptr);
+    if (rc) return rc;
+    const int64_t nJobs = b.ch.nJobs;
+    if (needed) *needed = nJobs;
+    if (nJobs > cap) return fail(c, MSGS_E_CAP, "row capacity");
+    Finisher F{c->db, *p, b.dp};
+    std::string cig;
+    for (int64_t i = 0; i < r.n; ++i)
+        if (counts) counts[i] = b.ch.chainOff[i + 1] - b.ch.chainOff[i];
+    for (int64_t j = 0; j < nJobs; ++j) {
+        const AlnRec& a = b.dp.recs[j];
+        int64_t* o = rows10 + 10 * j;
+        o[0] = a.tid; o[1] = a.strand; o[2] = a.tStart; o[3] = a.tEnd; o[4] = a.alnFrom; o[5] = a.alnTo;
+        o[6] = a.alnStart; o[7] = a.alnEnd; o[8] = a.hazard; o[9] = a.cells;
+        if (scores) scores[j] = a.score == NEG ? -std::numeric_limits<double>::infinity() : (double)a.score;
+        cig += F.cigarOf(j);
+        cig += '\n';
+    }
+    if (cigars) {
+        if ((int64_t)cig.size() + 1 > cigCap) return fail(c, MSGS_E_CAP, "cigar capacity");
+        std::memcpy(cigars, cig.c_str(), cig.size() + 1);
+    }
+    return MSGS_OK;
+}
+
+/* SAM text of a whole batch (SE when mates is NULL), finished on host threads in
+ * read-order chunks; returns the number of mapped reads. */
+static int64_t samText(msgs_ctx* c, const msgs_params* p, const msgs_reads* reads, const msgs_reads* mates,
+                       const BatchResult& b1, const BatchResult* b2, bool emit, std::string& out,
+                       std::vector<int64_t>& lineOffs) {
+    const int64_t n = reads->n;
+    std::vector<std::string> nm1 = splitNames(reads->names, n);
+    std::vector<std::string> nm2 = mates ? splitNames(mates->names, n) : std::vector<std::string>();
+    lineOffs.assign(n + 1, 0);
+    const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(c->hostThreads, (n + 255) / 256));
+    std::vector<std::string> part(nt);
+    std::vector<std::vector<int64_t>> pOffs(nt);
+    std::vector<int64_t> mappedPart(nt, 0);
+    auto work = [&](int t) {
+        const int64_t i0 = n * t / nt, i1 = n * (t + 1) / nt;
+        Finisher F{c->db, *p, b1.dp};
+        F.emit = emit;
+        std::vector<uint8_t> q30;
+        auto host = [&](const msgs_reads* r, const std::vector<std::string>& nm, int64_t i) {
+            HostRead h;
+            h.seq = r->seq + r->offs[i];
+            h.L = r->offs[i + 1] - r->offs[i];
+            if (r->qual) h.qual = r->qual + r->offs[i];
+            else { if ((int64_t)q30.size() < h.L) q30.assign(h.L, 30); h.qual = q30.data(); }
+            h.name = nm[i].empty() ? std::string("r") : nm[i];
+            return h;
+        };
+        std::string& o = part[t];
+        pOffs[t].resize(i1 - i0);
+        for (int64_t i = i0; i < i1; ++i) {
+            pOffs[t][i - i0] = (int64_t)o.size();
+            HostRead h1 = host(reads, nm1, i);
+            if (!mates) {
+                F.se(o, h1, b1.sd.seedCnt[i], b1.ch.chainOff[i], b1.ch.chainOff[i + 1]);
+            } else {
+                HostRead h2 = host(mates, nm2, i);
+                F.pe(o, h1, h2, b1.sd.seedCnt[i], b2->sd.seedCnt[i], b1.ch.chainOff[i], b1.ch.chainOff[i + 1],
+                     b1.ch.nJobs + b2->ch.chainOff[i], b1.ch.nJobs + b2->ch.chainOff[i + 1]);
+            }
+        }
+        mappedPart[t] = F.mapped;
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back(work, t);
+    work(0);
+    for (auto& x : th) x.join();
+    int64_t total = 0, mapped = 0;
+    for (int t = 0; t < nt; ++t) { total += (int64_t)part[t].size(); mapped += mappedPart[t]; }
+    out.clear();
+    out.reserve(total);
+    for (int t = 0; t < nt; ++t) {
+        const int64_t i0 = n * t / nt;
+        for (size_t k = 0; k < pOffs[t].size(); ++k) lineOffs[i0 + k] = (int64_t)out.size() + pOffs[t][k];
+        out += part[t];
+    }
+    lineOffs[n] = (int64_t)out.size();
+    return mapped;
+}
+
+/* merge the mate batch's DP results behind the first batch's so one Finisher sees both */
+static void mergeDp(DpStage& a, const DpStage& b) {
+    const int64_t nA = (int64_t)a.recs.size();
+    const int64_t pA = a.pathOff.empty() ? 0 : a.pathOff.back();
+    /* layouts: cigar at pathOff[j] + 2j, md at 3*pathOff[j] + 8j */
+    std::vector<uint32_t> cig;
+    std::vector<char> md;
+    const int64_t nB = (int64_t)b.recs.size();
+    const int64_t pB = b.pathOff.empty() ? 0 : b.pathOff.back();
+    cig.assign(pA + pB + 2 * (nA + nB) + 1, 0);
+    md.assign(3 * (pA + pB) + 8 * (nA + nB) + 1, 0);
+    if (nA) {
+        std::memcpy(cig.data(), a.cigar.data(), (pA + 2 * nA) * 4);
+        std::memcpy(md.data(), a.md.data(), 3 * pA + 8 * nA);
+    }
+    std::vector<int64_t> off(nA + nB + 1);
+    for (int64_t j = 0; j < nA; ++j) off[j] = a.pathOff[j];
+    for (int64_t j = 0; j <= nB; ++j) off[nA + j] = pA + (nB ? b.pathOff[j] : 0);
+    for (int64_t j = 0; j < nB; ++j) {
+        const int64_t src = b.pathOff[j] + 2 * j, dst = off[nA + j] + 2 * (nA + j);
+        std::memcpy(cig.data() + dst, b.cigar.data() + src, b.recs[j].nCigar * 4);
+        const int64_t ms = 3 * b.pathOff[j] + 8 * j, mdst = 3 * off[nA + j] + 8 * (nA + j);
+        std::memcpy(md.data() + mdst, b.md.data() + ms, b.recs[j].mdLen);
+    }
+    a.recs.insert(a.recs.end(), b.recs.begin(), b.recs.end());
+    a.pathOff.swap(off);
+    a.cigar.swap(cig);
+    a.md.swap(md);
+}
+
+extern "C" int msgs_map_batch(msgs_ctx* c, const msgs_reads* reads, const msgs_reads* mates, const msgs_params* p,
+                              char* out, int64_t cap, int64_t* lineOffs, int64_t* needed) {
+    if (!c || !c->db.n) return fail(c, MSGS_E_STATE, "no database");
+    int rc = checkParams(c, p);
+    if (rc) return rc;
+    if (!reads) return fail(c, MSGS_E_ARG, "no reads");
+    if (mates && mates->n != reads->n) return fail(c, MSGS_E_ARG, "mate batches differ in size");
+    BatchResult b1, b2;
+    {
+        DevReads r;
+        rc = uploadReads(c, reads, "rd", r);
+        if (rc) return rc;
+        uint8_t* dQual = nullptr;
+        rc = uploadQual(c, reads, r, "rd.qual", &dQual);
+        if (rc) return rc;
+        rc = runAll(c, r, dQual, p, b1, nullptr, nullptr);
+        if (rc) return rc;
+    }
+    if (mates) {
+        DevReads r;
+        rc = uploadReads(c, mates, "rd", r);
+        if (rc) return rc;
+        uint8_t* dQual = nullptr;
+        rc = uploadQual(c, mates, r, "rd.qual", &dQual);
+        if (rc) return rc;
+        rc = runAll(c, r, dQual, p, b2, nullptr, nullptr);
+        if (rc) return rc;
+        mergeDp(b1.dp, b2.dp);
+    }
+    std::string text;
+    std::vector<int64_t> lo;
+    samText(c, p, reads, mates, b1, mates ? &b2 : nullptr, true, text, lo);
+    if (needed) *needed = (int64_t)text.size();
+    if ((int64_t)text.size() > cap || !out) return fail(c, MSGS_E_CAP, "output capacity");
+    std::memcpy(out, text.data(), text.size());
+    if (lineOffs) std::memcpy(lineOffs, lo.data(), lo.size() * 8);
+    return MSGS_OK;
+}
+
+/* ------------------------------------------------------------------ resident batches */
+
+static std::vector<uint8_t> hostCopy(const uint8_t* p, int64_t n) { return p ? std::vector<uint8_t>(p, p + n) : std::vector<uint8_t>(); }
+
+extern "C" int msgs_batch_upload(msgs_ctx* c, int32_t slot, const msgs_reads* reads, const msgs_reads* mates) {
+    if (!c) return MSGS_E_ARG;
+    if (slot < 0 || slot >= 16 || !reads) return fail(c, MSGS_E_ARG, "bad slot or reads");
+    if (mates && mates->n != reads->n) return fail(c, MSGS_E_ARG, "mate batches differ in size");
+    Slot& S = c->slots[slot];
+    S.release();
+    const msgs_reads* rs[2] = {reads, mates};
+    for (int m = 0; m < (mates ? 2 : 1); ++m) {
+        const msgs_reads* r = rs[m];
+        HostBatch& hb = S.host[m];
+        const int64_t n = r->n, total = n ? r->offs[n] - r->offs[0] : 0;
+        hb.seq = hostCopy(r->seq + (n ? r->offs[0] : 0), total);
+        hb.qual = r->qual ? hostCopy(r->qual + (n ? r->offs[0] : 0), total) : std::vector<uint8_t>(total, 30);
+        hb.offs.assign(n + 1, 0);
+        for (int64_t i = 0; i <= n; ++i) hb.offs[i] = n ? r->offs[i] - r->offs[0] : 0;
+        hb.names = r->names ? std::string(r->names) : std::string();
+        hb.view.seq = hb.seq.empty() ? nullptr : hb.seq.data();
+        hb.view.qual = hb.qual.empty() ? nullptr : hb.qual.data();
+        hb.view.offs = hb.offs.data();
+        hb.view.names = hb.names.empty() ? nullptr : hb.names.c_str();
+        hb.view.n = n;
+        char pfx[32];
+        std::snprintf(pfx, sizeof pfx, "slot%d.%d", slot, m);
+        int rc = uploadReads(c, &hb.view, pfx, S.dev[m]);
+        if (rc) return rc;
+        std::snprintf(pfx, sizeof pfx, "slot%d.%d.q", slot, m);
+        rc = uploadQual(c, &hb.view, S.dev[m], pfx, &S.dQual[m]);
+        if (rc) return rc;
+    }
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, MSGS_E_HIP, "upload");
+    S.paired = mates != nullptr;
+    S.used = true;
+    return MSGS_OK;
+}
+
+extern "C" int msgs_batch_run(msgs_ctx* c, int32_t slot, const msgs_params* p, int32_t emitSam) {
+    if (!c || !c->db.n) return fail(c, MSGS_E_STATE, "no database");
+    if (slot < 0 || slot >= 16 || !c->slots[slot].used) return fail(c, MSGS_E_STATE, "slot not uploaded");
+    int rc = checkParams(c, p);
+    if (rc) return rc;
+    Slot& S = c->slots[slot];
+    hipError_t e = hipSuccess;
+    unsigned long long* dStats = c->bufs.get<unsigned long long>("stats", 8, e);
+    if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "stats");
+    CK(hipMemsetAsync(dStats, 0, 8 * 8, c->stream), "stats");
+    auto t0 = std::chrono::steady_clock::now();
+    BatchResult b[2];
+    double seedUs = 0, smemUs = 0, dpUs = 0;
+    for (int m = 0; m < (S.paired ? 2 : 1); ++m) {
+        auto s0 = std::chrono::steady_clock::now();
+        rc = runSmem(c, S.dev[m], p, b[m].sm, dStats);
+        if (rc) return rc;
+        smemUs += c->lastSmemMicros;
+        rc = runSeeds(c, S.dev[m], p, b[m].sm, b[m].sd, dStats);
+        if (rc) return rc;
+        rc = runChains(c, S.dev[m], b[m].sd, b[m].ch);
+        if (rc) return rc;
+        seedUs += std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - s0).count();
+        double us = 0;
+        rc = runDp(c, S.dev[m], S.dQual[m], p, b[m].sd, b[m].ch, b[m].dp, &us);
+        if (rc) return rc;
+        dpUs += us;
+    }
+    if (S.paired) mergeDp(b[0].dp, b[1].dp);
+    auto h0 = std::chrono::steady_clock::now();
+    int64_t mapped = samText(c, p, &S.host[0].view, S.paired ? &S.host[1].view : nullptr, b[0], S.paired ? &b[1] : nullptr,
+                             emitSam != 0, c->lastSam, c->lastLineOffs);
+    auto t1 = std::chrono::steady_clock::now();
+    unsigned long long hs[8];
+    CK(hipMemcpy(hs, dStats, sizeof(hs), hipMemcpyDeviceToHost), "stats");
+    int64_t cells = 0;
+    for (const AlnRec& a : b[0].dp.recs) cells += a.cells;
+    std::memset(c->stats, 0, sizeof(c->stats));
+    c->stats[0] = (double)(S.host[0].view.n * (S.paired ? 2 : 1));
+    c->stats[1] = (double)mapped;
+    c->stats[2] = (double)hs[0];
+    c->stats[3] = (double)hs[1];
+    c->stats[4] = (double)cells;
+    c->stats[5] = smemUs;
+    c->stats[6] = dpUs;
+    c->stats[7] = std::chrono::duration<double, std::micro>(t1 - t0).count();
+    c->stats[8] = seedUs;
+    c->stats[9] = std::chrono::duration<double, std::micro>(t1 - h0).count();
+    return MSGS_OK;
+}
+
+extern "C" int msgs_batch_stats(msgs_ctx* c, double* stats, int32_t n) {
+    if (!c || !stats) return MSGS_E_ARG;
+    for (int32_t i = 0; i < n && i < 16; ++i) stats[i] = c->stats[i];
+    return MSGS_OK;
+}
+
+extern "C" int msgs_batch_sam(msgs_ctx* c, char* out, int64_t cap, int64_t* needed) {
+    if (!c) return MSGS_E_ARG;
+    if (needed) *needed = (int64_t)c->lastSam.size();
+    if (!out || (int64_t)c->lastSam.size() + 1 > cap) return fail(c, MSGS_E_CAP, "output capacity");
+    std::memcpy(out, c->lastSam.c_str(), c->lastSam.size() + 1);
+    return MSGS_OK;
+}

@@ -140,3 +140,57 @@ def encode(s):
 
 def decode(seq):
     return bytes(np.frombuffer(b"$ACMGRSVTWYHKDBN", np.uint8)[np.asarray(seq) & 15]).decode()
+
+
+def simulate_pairs_fast(chroms, n, length=150, insert=(350, 50), sub=0.005, indel=0.0005, qual=(20, 40),
+                        seed=1, offtarget=0.0, prefix="p"):
+    """Vectorized paired-end simulation for large batches (same model as simulate_pairs:
+    fragment ~ N(insert), mate 1 = fragment start, mate 2 = revcom of fragment end,
+    uniform genome / position / strand, substitutions and 1-3 bp indels).  Returns
+    ((seqs1, quals1, names), (seqs2, quals2, names)) with fixed-length numpy rows."""
+    rng = np.random.default_rng(seed)
+    lens = np.array([len(c) for c in chroms], np.int64)
+    starts = np.concatenate([[0], np.cumsum(lens)[:-1]])
+    allseq = np.concatenate(chroms)
+    span = length + 8
+    isz = np.maximum(length, np.round(rng.normal(insert[0], insert[1], n)).astype(np.int64))
+    g = rng.integers(0, len(chroms), n)
+    pos = (rng.random(n) * np.maximum(1, lens[g] - isz - span)).astype(np.int64)
+    strand = rng.integers(0, 2, n)
+    off = rng.random(n) < offtarget
+    mates = []
+    for m in range(2):
+        if m == 0:
+            a = starts[g] + pos                         # fragment start
+        else:
+            a = starts[g] + pos + isz - span            # window ending at the fragment end
+        a = np.clip(a, 0, len(allseq) - span)
+        idx = a[:, None] + np.arange(span)[None, :]
+        w = allseq[idx]
+        if m == 1:
+            w = COMP[w[:, ::-1]]                        # mate 2 reads the reverse strand
+        mates.append(w)
+    # a fragment drawn from the reverse strand swaps the roles of the two windows
+    w1 = np.where(strand[:, None] == 0, mates[0], mates[1])
+    w2 = np.where(strand[:, None] == 0, mates[1], mates[0])
+    rnd = NT[rng.integers(0, 4, (n, span))]
+    w1 = np.where(off[:, None], rnd, w1)
+    rnd2 = NT[rng.integers(0, 4, (n, span))]
+    w2 = np.where(off[:, None], rnd2, w2)
+    out = []
+    for w in (w1, w2):
+        w = w.copy()
+        subm = rng.random(w.shape) < sub
+        cur = np.searchsorted(NT, w[subm]) % 4
+        w[subm] = NT[(cur + rng.integers(1, 4, int(subm.sum()))) % 4]
+        seqs = [row[:length] for row in w]
+        k = rng.binomial(span, indel, n) if indel > 0 else np.zeros(n, np.int64)
+        for i in np.nonzero(k)[0]:
+            seqs[i] = mutate(rng, w[i], 0.0, indel)[:length]
+            if len(seqs[i]) < length:
+                seqs[i] = np.concatenate([seqs[i], w[i][len(seqs[i]):length]])
+        seqs = [np.ascontiguousarray(s, np.uint8) for s in seqs]
+        quals = list(rng.integers(qual[0], qual[1] + 1, (n, length)).astype(np.uint8))
+        out.append((seqs, quals))
+    names = ["%s%d" % (prefix, i) for i in range(n)]
+    return (out[0][0], out[0][1], names), (out[1][0], out[1][1], names)
