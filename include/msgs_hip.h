@@ -60,6 +60,11 @@ typedef struct {
 int msgs_ctx_create(int device, msgs_ctx** out);
 void msgs_ctx_destroy(msgs_ctx* ctx);
 const char* msgs_last_error(msgs_ctx* ctx);
+/* Context options (no reference counterpart; tuning knobs of the device path):
+ * "form" 0 = plain FMD extensions, 1 = unique intervals followed through the text (default);
+ * "lanes_per_cu" persistent lanes of the SMEM kernel per CU; "smem_cap" SMEM slots per
+ * read before a regrow; "host_threads" threads of the host finishing step. */
+int msgs_ctx_set(msgs_ctx* ctx, const char* key, int64_t value);
 
 /* ---- database: MetaGenome + FMDIndex resident in HBM ----
  * Replaces MetaGenome::load/loadSeq + FMDIndex::load (src/MetaGenome.cpp:42-73,

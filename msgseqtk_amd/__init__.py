@@ -21,7 +21,7 @@ MSGS_E_CAP = -6
 
 # every extern "C" symbol of include/msgs_hip.h
 EXPORTS = [
-    "msgs_params_default", "msgs_ctx_create", "msgs_ctx_destroy", "msgs_last_error",
+    "msgs_params_default", "msgs_ctx_create", "msgs_ctx_destroy", "msgs_last_error", "msgs_ctx_set",
     "msgs_db_build", "msgs_db_load", "msgs_db_nchrom", "msgs_db_chrom", "msgs_db_length", "msgs_db_counts", "msgs_db_copy", "msgs_db_loglik",
     "msgs_rank_batch", "msgs_ext_batch", "msgs_smem_batch", "msgs_seed_batch",
     "msgs_map_batch", "msgs_align_batch", "msgs_batch_upload", "msgs_batch_run",
@@ -64,6 +64,7 @@ def lib():
     L.msgs_ctx_destroy.argtypes = [vp]
     L.msgs_last_error.argtypes = [vp]
     L.msgs_last_error.restype = C.c_char_p
+    L.msgs_ctx_set.argtypes = [vp, C.c_char_p, C.c_int64]
     L.msgs_db_build.argtypes = [vp, PU8, P64, C.c_int32, C.c_char_p]
     L.msgs_db_load.argtypes = [vp, C.c_char_p]
     L.msgs_db_nchrom.argtypes = [vp]
@@ -148,6 +149,10 @@ class Mapper:
     def _chk(self, rc, what):
         if rc != 0:
             raise RuntimeError("%s failed (%d): %s" % (what, rc, lib().msgs_last_error(self.h).decode()))
+
+    def set(self, key, value):
+        self._chk(lib().msgs_ctx_set(self.h, key.encode(), int(value)), "msgs_ctx_set")
+        return self
 
     def build(self, chroms, names=None):
         chroms = [np.ascontiguousarray(c, np.uint8) for c in chroms]

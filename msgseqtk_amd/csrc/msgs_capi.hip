@@ -19,16 +19,16 @@
 /* ------------------------------------------------------------------ state */
 
 void DbState::release() {
-    void* ptrs[] = {dBd, dSa, dBwt, dOcc, dNPos, dChromBD, dTgtOff, dChromLen, dTgt};
+    void* ptrs[] = {dBd, dSa, dIsa, dBwt, dOcc, dNPos, dChromBD, dTgtOff, dChromLen, dTgt};
     for (void* p : ptrs) if (p) (void)hipFree(p);
-    dBd = nullptr; dSa = nullptr; dBwt = nullptr; dOcc = nullptr; dNPos = nullptr;
+    dBd = nullptr; dSa = nullptr; dIsa = nullptr; dBwt = nullptr; dOcc = nullptr; dNPos = nullptr;
     dChromBD = nullptr; dTgtOff = nullptr; dChromLen = nullptr; dTgt = nullptr;
     n = 0;
 }
 
 DevIndex DbState::view() const {
     DevIndex v;
-    v.occ = dOcc; v.sa = dSa; v.nPos = dNPos; v.nN = nN; v.n = n;
+    v.occ = dOcc; v.sa = dSa; v.isa = dIsa; v.bd = dBd; v.nPos = dNPos; v.nN = nN; v.n = n;
     for (int i = 0; i < 17; ++i) v.C[i] = C[i];
     for (int i = 0; i < 16; ++i) v.ll[i] = ll[i];
     v.chromBD = dChromBD; v.nChrom = nChrom; v.tgt = dTgt; v.tgtOff = dTgtOff; v.chromLen = dChromLen;
@@ -364,6 +364,7 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
     a.candR = c->bufs.get<int64_t>("sm.cr", candCap * lanes, e);
     a.candS = c->bufs.get<int64_t>("sm.cs", candCap * lanes, e);
     a.candLL = c->bufs.get<double>("sm.cl", candCap * lanes, e);
+    a.candX = c->bufs.get<int64_t>("sm.cx", candCap * lanes, e);
     a.nLanes = lanes;
     a.queue = c->bufs.get<unsigned long long>("sm.q", 1, e);
     a.cnt = c->bufs.get<int32_t>("sm.cnt", n + 1, e);
@@ -377,7 +378,10 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
         if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "smem output");
         (void)hipMemsetAsync(a.queue, 0, 8, st);
         (void)hipEventRecord(c->ev0, st);
-        if (n > 0) smem_kernel<<<(unsigned)(lanes / 256), 256, 0, st>>>(c->db.view(), a);
+        if (n > 0) {
+            if (c->form == 0) smem_kernel_t<false><<<(unsigned)(lanes / 256), 256, 0, st>>>(c->db.view(), a);
+            else smem_kernel_t<true><<<(unsigned)(lanes / 256), 256, 0, st>>>(c->db.view(), a);
+        }
         (void)hipEventRecord(c->ev1, st);
         if (hipGetLastError() != hipSuccess) return fail(c, MSGS_E_HIP, "smem launch");
         /* largest count */
@@ -526,3 +530,16 @@ extern "C" int msgs_db_chrom(msgs_ctx* c, int32_t tid, char* name, int64_t cap, 
     return MSGS_OK;
 }
 extern "C" int32_t msgs_db_nchrom(msgs_ctx* c) { return c ? c->db.nChrom : 0; }
+
+/* context options: "form" (0 plain FMD, 1 unique-interval text path), "lanes_per_cu",
+ * "smem_cap", "host_threads" */
+extern "C" int msgs_ctx_set(msgs_ctx* c, const char* key, int64_t value) {
+    if (!c || !key) return MSGS_E_ARG;
+    std::string k(key);
+    if (k == "form") { if (value < 0 || value > 1) return fail(c, MSGS_E_ARG, "form 0 or 1"); c->form = (int)value; }
+    else if (k == "lanes_per_cu") { if (value < 64 || value > 8192 || value % 256) return fail(c, MSGS_E_ARG, "lanes_per_cu: multiple of 256 in [256, 8192]"); c->lanesPerCU = (int)value; }
+    else if (k == "smem_cap") { if (value < 1 || value > 4096) return fail(c, MSGS_E_ARG, "smem_cap in [1, 4096]"); c->smemCap = (int)value; }
+    else if (k == "host_threads") { if (value < 1 || value > 256) return fail(c, MSGS_E_ARG, "host_threads in [1, 256]"); c->hostThreads = (int)value; }
+    else return fail(c, MSGS_E_ARG, "unknown option " + k);
+    return MSGS_OK;
+}

@@ -73,6 +73,7 @@ struct msgs_ctx {
     int nCU = 256;
     int lanesPerCU = 1024;
     int smemCap = 32;
+    int form = 1;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     std::string err;

@@ -32,6 +32,8 @@ struct __attribute__((aligned(64))) OccBlock {
 struct DevIndex {
     const OccBlock* occ;
     const uint32_t* sa;        /* full suffix array */
+    const uint32_t* isa;       /* inverse suffix array (ISA[SA[i]] = i) */
+    const uint8_t* bd;         /* BD text */
     const int64_t* nPos;       /* sorted BWT positions holding N */
     int64_t nN;
     int64_t n;                 /* BD text length */
@@ -66,7 +68,7 @@ __device__ __forceinline__ void occFromBlock(const DevIndex& ix, const uint4& h,
                                              const uint4& c1, int64_t i, int64_t* r) {
     const uint32_t j = (uint32_t)(i & 127);
     const uint32_t wj = j >> 4, bj = j & 15;
-    const uint32_t part = (2u << (2 * bj)) - 1u;
+    const uint32_t part = (2u << (2 * bj + 1)) - 1u;   /* bits 0 .. 2*bj+1 */
     const uint32_t w[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
     uint32_t nT = 0, nLo = 0, nHi = 0;
 #pragma unroll

@@ -24,6 +24,7 @@ struct DbState {
     /* device */
     uint8_t* dBd = nullptr;
     uint32_t* dSa = nullptr;
+    uint32_t* dIsa = nullptr;
     uint8_t* dBwt = nullptr;
     OccBlock* dOcc = nullptr;
     int64_t* dNPos = nullptr;

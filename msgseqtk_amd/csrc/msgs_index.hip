@@ -70,6 +70,12 @@ __global__ void k_count_distinct(const uint32_t* gstart, int64_t n, unsigned lon
     if ((threadIdx.x & 63) == 0 && c) atomicAdd(cnt, c);
 }
 
+__global__ void k_isa(const uint32_t* sa, int64_t n, uint32_t* isa) {
+    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    isa[sa[k]] = (uint32_t)k;
+}
+
 __global__ void k_bwt(const uint8_t* bd, const uint32_t* sa, int64_t n, uint8_t* bwt) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
@@ -189,6 +195,9 @@ int buildIndexDevice(DbState& db, hipStream_t st, std::string& err) {
     /* v0 is the suffix array */
     HCHK(hipMalloc(&db.dSa, n * sizeof(uint32_t)));
     HCHK(hipMemcpyAsync(db.dSa, v0, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
+    HCHK(hipMalloc(&db.dIsa, n * sizeof(uint32_t)));
+    k_isa<<<grid(n), 256, 0, st>>>(db.dSa, n, db.dIsa);
+    HCHK(hipGetLastError());
     HCHK(hipMalloc(&db.dBwt, n));
     k_bwt<<<grid(n), 256, 0, st>>>(db.dBd, db.dSa, n, db.dBwt);
     HCHK(hipGetLastError());

@@ -32,12 +32,13 @@ struct SmemArgs {
     int64_t* candR;
     int64_t* candS;
     double* candLL;
+    int64_t* candX;            /* moving side as a text position (-1: row values exact) */
     int64_t nLanes;
     unsigned long long* queue; /* next read */
-    unsigned long long* stats; /* [0] extension calls */
+    unsigned long long* stats; /* [0] occ-block extensions, [2] text steps, [3] SA/ISA lookups */
 };
 
-__global__ void smem_kernel(DevIndex ix, SmemArgs a);
+template <bool FAST> __global__ void smem_kernel_t(DevIndex ix, SmemArgs a);
 
 /* one seed (SeedPair, src/SeedPair.h:120-128) */
 struct SeedRec {

@@ -36,7 +36,7 @@ struct Lane {
     int state;
     /* phase-1 SMEM */
     int cFrom, cTo;
-    int64_t cF, cR, cS;
+    int64_t cF, cR, cS, cX;
     double cLL;
     /* candidate list */
     int nCand, lo, k, w, nValid;
@@ -44,7 +44,7 @@ struct Lane {
     /* output */
     int nOut;
     /* counters */
-    int64_t nExt;
+    int64_t nExt, nFast, nLook;
 };
 
 __device__ __forceinline__ uint8_t base(const Lane& l, int i) { return i < l.L ? l.s[i] : (uint8_t)0; }
@@ -65,9 +65,11 @@ __device__ __forceinline__ void emit(const Ctx& c, Lane& l, int from, int to, in
 
 __device__ __forceinline__ size_t cidx(const Ctx& c, int k) { return (size_t)k * c.a.nLanes + c.lane; }
 
-__device__ __forceinline__ void candStore(const Ctx& c, int k, int pos, int64_t F, int64_t R, int64_t S, double ll) {
+__device__ __forceinline__ void candStore(const Ctx& c, int k, int pos, int64_t F, int64_t R, int64_t S, double ll,
+                                          int64_t X) {
     size_t i = cidx(c, k);
     c.a.candPos[i] = pos; c.a.candF[i] = F; c.a.candR[i] = R; c.a.candS[i] = S; c.a.candLL[i] = ll;
+    c.a.candX[i] = X;
 }
 
 /* start findFwdBackSMEMS / findBackFwdSMEMS at f; false if it ends at once (ambiguous base) */
@@ -78,6 +80,7 @@ __device__ __forceinline__ bool startSearch(const Ctx& c, Lane& l, int f, bool f
     if (amb(b)) return false;
     l.cFrom = f; l.cTo = f + 1;
     l.cF = c.ix.C[b]; l.cR = c.ix.C[comp16(b)]; l.cS = c.ix.C[b + 1] - c.ix.C[b];
+    l.cX = -1;
     double ll = 0;
     ll += c.ix.ll[b];
     l.cLL = ll;
@@ -168,9 +171,19 @@ __device__ bool nextRead(const Ctx& c, Lane& l) {
 
 }  // namespace
 
-__global__ void __launch_bounds__(256) smem_kernel(DevIndex ix, SmemArgs a) {
+/*
+ * FAST selects the unique-interval path: while an interval has size 1, a backward step
+ * by b succeeds iff the text byte before the suffix is b (BWT[p] = BD[SA[p]-1]), and
+ * then p becomes LF(p) = ISA[SA[p]-1] while q is unchanged (src/FMDIndex.cpp:577-610
+ * with s = 1: no split, so no q update); forward steps act on q the same way
+ * (FMDIndex::fwdExt).  The moving side is carried as its text position X and turned
+ * back into a row (ISA[X]) only when the SMEM is pushed or emitted.  Values are
+ * identical to the occ-block path; only the memory traffic differs.
+ */
+template <bool FAST>
+__global__ void __launch_bounds__(256) smem_kernel_t(DevIndex ix, SmemArgs a) {
     Lane l;
-    l.nExt = 0;
+    l.nExt = 0; l.nFast = 0; l.nLook = 0;
     l.state = ST_IDLE;
     Ctx c{ix, a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x};
     const int64_t minLen = a.minLen;
@@ -182,41 +195,63 @@ __global__ void __launch_bounds__(256) smem_kernel(DevIndex ix, SmemArgs a) {
         const int st = l.state;
         const bool phase1 = (st == ST_FB1 || st == ST_BF1);
         int pos = 0;
-        int64_t P, Q, S;
+        int64_t P, Q, S, X;
         double LL;
         int mFrom, mTo;
         if (phase1) {
-            P = l.cF; Q = l.cR; S = l.cS; LL = l.cLL; mFrom = l.cFrom; mTo = l.cTo;
+            P = l.cF; Q = l.cR; S = l.cS; X = l.cX; LL = l.cLL; mFrom = l.cFrom; mTo = l.cTo;
         } else {
             size_t i = cidx(c, l.k);
             pos = a.candPos[i]; P = a.candF[i]; Q = a.candR[i]; S = a.candS[i]; LL = a.candLL[i];
+            X = FAST ? a.candX[i] : -1;
             if (st == ST_FB2) { mFrom = l.sFrom; mTo = pos; } else { mFrom = pos; mTo = l.sTo; }
         }
         const bool fwd = (st == ST_FB1 || st == ST_BF2);
         const uint8_t b = fwd ? base(l, mTo) : (mFrom > 0 ? base(l, mFrom - 1) : (uint8_t)0);
-        const int64_t F0 = P, R0 = Q, S0 = S;
+        int64_t F0 = P, R0 = Q;
+        const int64_t S0 = S, X0 = X;
         const double LL0 = LL;
         /* ---- the extension (SMEM::fwdExt / SMEM::backExt, src/SMEM.h:140-177) ---- */
-        l.nExt += amb(b) ? 0 : 1;
-        if (fwd) dBackExt(ix, Q, P, S, comp16(b));
-        else dBackExt(ix, P, Q, S, b);
+        if (FAST && S == 1) {
+            if (X < 0) { X = ix.sa[fwd ? Q : P]; l.nLook++; }
+            const uint8_t bb = fwd ? comp16(b) : b;
+            if (!amb(b) && X > 0 && ix.bd[X - 1] == bb) { X--; l.nFast++; }
+            else S = 0;
+        } else {
+            l.nExt += amb(b) ? 0 : 1;
+            if (fwd) dBackExt(ix, Q, P, S, comp16(b));
+            else dBackExt(ix, P, Q, S, b);
+            X = -1;
+        }
         if (fwd) { if (mTo < l.L) LL += ix.ll[b]; mTo++; }
         else { if (mFrom > 0) LL += ix.ll[b]; mFrom--; }
+        /* the pre-step SMEM with its moving side turned back into a row */
+        auto exact0 = [&]() {
+            if (FAST && X0 >= 0) {
+                const int64_t v = ix.isa[X0];
+                l.nLook++;
+                if (fwd) R0 = v; else F0 = v;
+            }
+        };
 
         /* ---- control ---- */
         if (st == ST_FB1) {
-            if (S != S0 && l.cTo >= minLen && gateFwd(c, l, LL0, l.cFrom))
-                candStore(c, l.nCand++, l.cTo, F0, R0, S0, LL0);
-            l.cTo = mTo; l.cF = P; l.cR = Q; l.cS = S; l.cLL = LL;
+            if (S != S0 && l.cTo >= minLen && gateFwd(c, l, LL0, l.cFrom)) {
+                exact0();
+                candStore(c, l.nCand++, l.cTo, F0, R0, S0, LL0, -1);
+            }
+            l.cTo = mTo; l.cF = P; l.cR = Q; l.cS = S; l.cLL = LL; l.cX = X;
             if (!(S >= 1)) {
                 l.sTo = mTo;
                 if (l.nCand == 0) { l.sFrom--; driverAfter(c, l); }
                 else { l.state = ST_FB2; l.lo = 0; startIteration(l); }
             }
         } else if (st == ST_BF1) {
-            if (S != S0 && (int64_t)(l.L - l.cFrom) >= minLen && gateBack(c, l, LL0, l.cTo))
-                candStore(c, l.nCand++, l.cFrom, F0, R0, S0, LL0);
-            l.cFrom = mFrom; l.cF = P; l.cR = Q; l.cS = S; l.cLL = LL;
+            if (S != S0 && (int64_t)(l.L - l.cFrom) >= minLen && gateBack(c, l, LL0, l.cTo)) {
+                exact0();
+                candStore(c, l.nCand++, l.cFrom, F0, R0, S0, LL0, -1);
+            }
+            l.cFrom = mFrom; l.cF = P; l.cR = Q; l.cS = S; l.cLL = LL; l.cX = X;
             if (!(S >= 1)) {
                 l.sFrom = mFrom;
                 if (l.nCand == 0) { l.sTo++; driverAfter(c, l); }
@@ -226,13 +261,14 @@ __global__ void __launch_bounds__(256) smem_kernel(DevIndex ix, SmemArgs a) {
             const bool fb = (st == ST_FB2);
             const int len0 = fb ? (pos - l.sFrom) : (l.sTo - pos);
             if (S != S0 && S0 >= 1 && l.nValid == 0 && len0 >= minLen && LL0 <= a.T2) {
+                exact0();
                 if (fb) emit(c, l, l.sFrom, pos, F0, R0, S0, LL0);
                 else emit(c, l, pos, l.sTo, F0, R0, S0, LL0);
             }
             if (S >= 1 && S != l.s0) {
                 l.s0 = S;
                 l.nValid++;
-                candStore(c, l.w--, pos, P, Q, S, LL);
+                candStore(c, l.w--, pos, P, Q, S, LL, X);
             }
             l.k--;
             if (l.k < l.lo) {
@@ -247,5 +283,13 @@ __global__ void __launch_bounds__(256) smem_kernel(DevIndex ix, SmemArgs a) {
         }
         if (l.state == ST_IDLE) a.cnt[l.rid] = l.nOut;
     }
-    if (a.stats) atomicAdd((unsigned long long*)&a.stats[0], (unsigned long long)l.nExt);
+    if (a.stats) {
+        atomicAdd(&a.stats[0], (unsigned long long)l.nExt);
+        atomicAdd(&a.stats[2], (unsigned long long)l.nFast);
+        atomicAdd(&a.stats[3], (unsigned long long)l.nLook);
+    }
 }
+
+
+template __global__ void smem_kernel_t<false>(DevIndex, SmemArgs);
+template __global__ void smem_kernel_t<true>(DevIndex, SmemArgs);
