@@ -36,8 +36,11 @@ def parse():
     ap.add_argument("--genome-len", type=int, default=4_000_000)
     ap.add_argument("--pairs", type=int, default=1_000_000)
     ap.add_argument("--batch-reads", type=int, default=262144)
-    ap.add_argument("--cpu-pairs", type=int, default=4000, help="pairs in the CPU-baseline sample")
+    ap.add_argument("--cpu-pairs", type=int, default=40000, help="pairs in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--form", type=int, default=1, help="0 plain FMD extensions, 1 unique-interval text path")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run a 1 GiB device copy (counter calibration under rocprofv3 --pmc)")
     return ap.parse_args()
 
 
@@ -63,6 +66,7 @@ def main():
                                                               time.time() - t0))
     t1 = time.time()
     m = M.Mapper(0).build(chroms, names)
+    m.set("form", a.form)
     build_s = time.time() - t1
     log("index built on the GPU in %.1fs (BD length %d)" % (build_s, m.n))
     for b, (r1, r2) in enumerate(batches):
@@ -93,7 +97,8 @@ def main():
     # dominant kernel: SMEM search; algorithmic bytes = one 64-B occ block per rank side
     # (2 per extension with an unambiguous base)
     launches = 2 * a.steps
-    bytes_per_launch = 64.0 * 2.0 * ext_calls / launches
+    fast_steps, lookups = S[:, 10].sum(), S[:, 11].sum()
+    bytes_per_launch = (64.0 * 2.0 * ext_calls + 64.0 * lookups + fast_steps) / launches
     avg_us = smem_us / launches
     achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9
     log("timed %d steps in %.2fs: %.0f mapped reads/s (%.1f%% mapped)" % (a.steps, elapsed, value,
@@ -105,12 +110,15 @@ def main():
         "data": "synthetic",
         "config": {"workload": "PE 2x150: 64 genomes x 4 Mb (256 Mbp, GC U(0.3,0.7)), insert N(350,50), "
                                "0.5% sub, 0.05% indel, 5% off-target pairs; 256k-read batches in HBM",
-                   "batch_reads": 2 * pairs_per_batch, "bd_length": int(m.n), "form": "plain FMD"},
+                   "batch_reads": 2 * pairs_per_batch, "bd_length": int(m.n),
+                   "form": ["plain FMD", "unique-interval text path"][a.form]},
         "roofline": {"bound": "hbm", "kernel": "smem_kernel", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "bytes_per_launch": bytes_per_launch, "avg_launch_us": round(avg_us, 1)},
         "dp_gcups": round(cells / (dp_us * 1e-6) / 1e9, 3) if dp_us > 0 else None,
-        "per_read": {"ext_calls": ext_calls / reads, "sa_lookups": sa_lookups / reads, "dp_cells": cells / reads},
+        "per_read": {"occ_extensions": ext_calls / reads, "text_steps": fast_steps / reads,
+                     "smem_sa_isa_lookups": lookups / reads, "seed_sa_lookups": sa_lookups / reads,
+                     "dp_cells": cells / reads},
         "stage_ms_per_step": {"smem_kernel": smem_us / a.steps / 1000, "seed_stage": S[:, 8].sum() / a.steps / 1000,
                               "dp_kernel": dp_us / a.steps / 1000, "host_finish": S[:, 9].sum() / a.steps / 1000,
                               "whole": S[:, 7].sum() / a.steps / 1000},
@@ -119,6 +127,13 @@ def main():
     }
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(m, chroms, names, batches[0], p, a.cpu_pairs)
+    if a.calib_copy:
+        import torch
+        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        y.copy_(x)
+        torch.cuda.synchronize()
+        out["calib_copy_bytes"] = 4 << 28
     print(json.dumps(out), flush=True)
 
 

@@ -127,7 +127,8 @@ int msgs_align_batch(msgs_ctx* ctx, const msgs_reads* reads, const msgs_params* 
  * host side of the context; msgs_batch_stats reports counters of the last run:
  * stats[0] reads, [1] mapped reads, [2] extension calls, [3] SA lookups, [4] DP cells,
  * [5] seeding kernel microseconds (HIP events), [6] DP kernel microseconds,
- * [7] whole-run microseconds, [8] seed-stage microseconds, [9] host finishing microseconds. */
+ * [7] whole-run microseconds, [8] seed-stage microseconds, [9] host finishing microseconds,
+ * [10] unique-interval text steps, [11] SA/ISA lookups of the SMEM kernel. */
 int msgs_batch_upload(msgs_ctx* ctx, int32_t slot, const msgs_reads* reads, const msgs_reads* mates);
 int msgs_batch_run(msgs_ctx* ctx, int32_t slot, const msgs_params* p, int32_t emitSam);
 int msgs_batch_stats(msgs_ctx* ctx, double* stats, int32_t n);

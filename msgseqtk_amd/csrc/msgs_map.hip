@@ -39,6 +39,19 @@ __global__ void k_revcom(const uint8_t* seq, const uint8_t* qual, const int64_t*
     }
 }
 
+/* gather each job's CIGAR ops and MD bytes from their worst-case slots into
+ * contiguous arrays (offsets from the host's scan of nCigar / mdLen) */
+__global__ void k_compact(const AlnRec* recs, int64_t nJobs, const int64_t* pathOff, const uint32_t* cig,
+                          const char* md, const int64_t* cOff, const int64_t* mOff, uint32_t* cOut, char* mOut) {
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < nJobs; j += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t* cs = cig + pathOff[j] + 2 * j;
+        const char* ms = md + 3 * pathOff[j] + 8 * j;
+        const int32_t nc = recs[j].nCigar, nm = recs[j].mdLen;
+        for (int32_t k = 0; k < nc; ++k) cOut[cOff[j] + k] = cs[k];
+        for (int32_t k = 0; k < nm; ++k) mOut[mOff[j] + k] = ms[k];
+    }
+}
+
 __global__ void k_fill_u8(uint8_t* p, int64_t n, uint8_t v) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
 }
@@ -83,12 +96,13 @@ struct ChainStage {
     int64_t nChainSeeds = 0;
 };
 
+/* DP results on the host: records plus compacted CIGAR ops / MD bytes of each job
+ * at cigOff[j] / mdOff[j] */
 struct DpStage {
     std::vector<AlnRec> recs;
-    std::vector<int64_t> pathOff;
+    std::vector<int64_t> cigOff, mdOff;
     std::vector<uint32_t> cigar;
     std::vector<char> md;
-    std::vector<uint8_t> path;
 };
 
 static int runSeeds(msgs_ctx* c, const DevReads& r, const msgs_params* p, const SmemStage& sm, SeedStage& out,
@@ -220,7 +234,8 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     CK(hipStreamSynchronize(st), "dp sizing");
     std::vector<int64_t> tileOff;
     const int64_t tiles = exclusiveScan(tneed, tileOff);
-    const int64_t paths = exclusiveScan(pneed, out.pathOff);
+    std::vector<int64_t> pathOff;
+    const int64_t paths = exclusiveScan(pneed, pathOff);
     int64_t* dTileOff = c->bufs.get<int64_t>("dp.toff", nJobs + 1, e);
     int64_t* dPathOff = c->bufs.get<int64_t>("dp.poff", nJobs + 1, e);
     a.tileM = c->bufs.get<int32_t>("dp.tm", tiles + 1, e);
@@ -232,19 +247,31 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     a.out = c->bufs.get<AlnRec>("dp.out", nJobs + 1, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "dp pools");
     CK(hipMemcpyAsync(dTileOff, tileOff.data(), (nJobs + 1) * 8, hipMemcpyHostToDevice, st), "toff");
-    CK(hipMemcpyAsync(dPathOff, out.pathOff.data(), (nJobs + 1) * 8, hipMemcpyHostToDevice, st), "poff");
+    CK(hipMemcpyAsync(dPathOff, pathOff.data(), (nJobs + 1) * 8, hipMemcpyHostToDevice, st), "poff");
     a.tileOff = dTileOff; a.pathOff = dPathOff;
     a.sizeOnly = 0;
     CK(hipEventRecord(c->ev2, st), "event");
     dp_kernel<<<gridFor(nJobs), 256, 0, st>>>(c->db.view(), a);
     CK(hipEventRecord(c->ev3, st), "event");
     out.recs.resize(nJobs);
-    out.cigar.resize(paths + 2 * nJobs + 1);
-    out.md.resize(3 * paths + 8 * nJobs + 1);
     CK(hipMemcpyAsync(out.recs.data(), a.out, nJobs * sizeof(AlnRec), hipMemcpyDeviceToHost, st), "recs");
-    CK(hipMemcpyAsync(out.cigar.data(), a.cigar, out.cigar.size() * 4, hipMemcpyDeviceToHost, st), "cigar");
-    CK(hipMemcpyAsync(out.md.data(), a.md, out.md.size(), hipMemcpyDeviceToHost, st), "md");
     CK(hipStreamSynchronize(st), "dp kernel");
+    std::vector<int32_t> nc(nJobs), nm(nJobs);
+    for (int64_t j = 0; j < nJobs; ++j) { nc[j] = out.recs[j].nCigar; nm[j] = out.recs[j].mdLen; }
+    const int64_t totC = exclusiveScan(nc, out.cigOff), totM = exclusiveScan(nm, out.mdOff);
+    int64_t* dCOff = c->bufs.get<int64_t>("dp.coff", nJobs + 1, e);
+    int64_t* dMOff = c->bufs.get<int64_t>("dp.moff", nJobs + 1, e);
+    uint32_t* dC = c->bufs.get<uint32_t>("dp.cc", totC + 1, e);
+    char* dM = c->bufs.get<char>("dp.mc", totM + 1, e);
+    if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "dp compaction");
+    CK(hipMemcpyAsync(dCOff, out.cigOff.data(), (nJobs + 1) * 8, hipMemcpyHostToDevice, st), "coff");
+    CK(hipMemcpyAsync(dMOff, out.mdOff.data(), (nJobs + 1) * 8, hipMemcpyHostToDevice, st), "moff");
+    k_compact<<<gridFor(nJobs), 256, 0, st>>>(a.out, nJobs, dPathOff, a.cigar, a.md, dCOff, dMOff, dC, dM);
+    out.cigar.resize(totC + 1);
+    out.md.resize(totM + 1);
+    CK(hipMemcpyAsync(out.cigar.data(), dC, (totC + 1) * 4, hipMemcpyDeviceToHost, st), "cigar");
+    CK(hipMemcpyAsync(out.md.data(), dM, totM + 1, hipMemcpyDeviceToHost, st), "md");
+    CK(hipStreamSynchronize(st), "dp compaction");
     if (dpMicros) {
         float ms = 0;
         (void)hipEventElapsedTime(&ms, c->ev2, c->ev3);
@@ -322,14 +349,14 @@ struct Finisher {
     std::string cigarOf(int64_t job) const {
         static const char kOps[] = "MIDNSHP=XB";
         const AlnRec& r = dp.recs[job];
-        const uint32_t* cg = dp.cigar.data() + dp.pathOff[job] + 2 * job;
+        const uint32_t* cg = dp.cigar.data() + dp.cigOff[job];
         std::string s;
         for (int k = 0; k < r.nCigar; ++k) { s += std::to_string(cg[k] >> 4); s += kOps[cg[k] & 15]; }
         return s;
     }
     std::string mdOf(int64_t job) const {
         const AlnRec& r = dp.recs[job];
-        return std::string(dp.md.data() + 3 * dp.pathOff[job] + 8 * job, r.mdLen);
+        return std::string(dp.md.data() + dp.mdOff[job], r.mdLen);
     }
 
     static void fmtFloat(std::string& s, float f) {
@@ -686,32 +713,25 @@ static int64_t samText(msgs_ctx* c, const msgs_params* p, const msgs_reads* read
 
 /* merge the mate batch's DP results behind the first batch's so one Finisher sees both */
 static void mergeDp(DpStage& a, const DpStage& b) {
-    const int64_t nA = (int64_t)a.recs.size();
-    const int64_t pA = a.pathOff.empty() ? 0 : a.pathOff.back();
-    /* layouts: cigar at pathOff[j] + 2j, md at 3*pathOff[j] + 8j */
-    std::vector<uint32_t> cig;
-    std::vector<char> md;
-    const int64_t nB = (int64_t)b.recs.size();
-    const int64_t pB = b.pathOff.empty() ? 0 : b.pathOff.back();
-    cig.assign(pA + pB + 2 * (nA + nB) + 1, 0);
-    md.assign(3 * (pA + pB) + 8 * (nA + nB) + 1, 0);
-    if (nA) {
-        std::memcpy(cig.data(), a.cigar.data(), (pA + 2 * nA) * 4);
-        std::memcpy(md.data(), a.md.data(), 3 * pA + 8 * nA);
+    const int64_t nA = (int64_t)a.recs.size(), nB = (int64_t)b.recs.size();
+    const int64_t cA = nA ? a.cigOff[nA] : 0, mA = nA ? a.mdOff[nA] : 0;
+    const int64_t cB = nB ? b.cigOff[nB] : 0, mB = nB ? b.mdOff[nB] : 0;
+    a.cigar.resize(cA);
+    a.md.resize(mA);
+    if (nB) {
+        a.cigar.insert(a.cigar.end(), b.cigar.begin(), b.cigar.begin() + cB);
+        a.md.insert(a.md.end(), b.md.begin(), b.md.begin() + mB);
     }
-    std::vector<int64_t> off(nA + nB + 1);
-    for (int64_t j = 0; j < nA; ++j) off[j] = a.pathOff[j];
-    for (int64_t j = 0; j <= nB; ++j) off[nA + j] = pA + (nB ? b.pathOff[j] : 0);
-    for (int64_t j = 0; j < nB; ++j) {
-        const int64_t src = b.pathOff[j] + 2 * j, dst = off[nA + j] + 2 * (nA + j);
-        std::memcpy(cig.data() + dst, b.cigar.data() + src, b.recs[j].nCigar * 4);
-        const int64_t ms = 3 * b.pathOff[j] + 8 * j, mdst = 3 * off[nA + j] + 8 * (nA + j);
-        std::memcpy(md.data() + mdst, b.md.data() + ms, b.recs[j].mdLen);
+    a.cigar.push_back(0);
+    a.md.push_back(0);
+    if (a.cigOff.empty()) { a.cigOff.assign(1, 0); a.mdOff.assign(1, 0); }
+    a.cigOff.resize(nA);
+    a.mdOff.resize(nA);
+    for (int64_t j = 0; j <= nB; ++j) {
+        a.cigOff.push_back(cA + (nB ? b.cigOff[j] : 0));
+        a.mdOff.push_back(mA + (nB ? b.mdOff[j] : 0));
     }
     a.recs.insert(a.recs.end(), b.recs.begin(), b.recs.end());
-    a.pathOff.swap(off);
-    a.cigar.swap(cig);
-    a.md.swap(md);
 }
 
 extern "C" int msgs_map_batch(msgs_ctx* c, const msgs_reads* reads, const msgs_reads* mates, const msgs_params* p,
@@ -840,6 +860,8 @@ extern "C" int msgs_batch_run(msgs_ctx* c, int32_t slot, const msgs_params* p, i
     c->stats[7] = std::chrono::duration<double, std::micro>(t1 - t0).count();
     c->stats[8] = seedUs;
     c->stats[9] = std::chrono::duration<double, std::micro>(t1 - h0).count();
+    c->stats[10] = (double)hs[2];
+    c->stats[11] = (double)hs[3];
     return MSGS_OK;
 }
 
