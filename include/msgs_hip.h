@@ -61,7 +61,8 @@ int msgs_ctx_create(int device, msgs_ctx** out);
 void msgs_ctx_destroy(msgs_ctx* ctx);
 const char* msgs_last_error(msgs_ctx* ctx);
 /* Context options (no reference counterpart; tuning knobs of the device path):
- * "form" 0 = plain FMD extensions, 1 = unique intervals followed through the text (default);
+ * "form" 0 = plain FMD extensions, 1 = unique intervals followed through the text (default),
+ *   2 = form 1 plus the learned-index read filter (needs msgs_lix_train);
  * "lanes_per_cu" persistent lanes of the SMEM kernel per CU; "smem_cap" SMEM slots per
  * read before a regrow; "host_threads" threads of the host finishing step. */
 int msgs_ctx_set(msgs_ctx* ctx, const char* key, int64_t value);
@@ -133,6 +134,34 @@ int msgs_batch_upload(msgs_ctx* ctx, int32_t slot, const msgs_reads* reads, cons
 int msgs_batch_run(msgs_ctx* ctx, int32_t slot, const msgs_params* p, int32_t emitSam);
 int msgs_batch_stats(msgs_ctx* ctx, double* stats, int32_t n);
 int msgs_batch_sam(msgs_ctx* ctx, char* out, int64_t cap, int64_t* needed);
+
+/* ---- learned index (no reference counterpart) ----
+ * A model of the SA rank of 16-mers: piecewise-linear root over `segments` rank
+ * quantiles + a 2-layer MLP (16 -> 64 -> 1), trained on the device (fp32, Adam) on
+ * `samples` (key, rank) pairs drawn from the index; inference in bf16 MFMA; exact
+ * lower bounds by a search bounded by the measured max error with a full-search
+ * fallback.  Answers FMDIndex::count for 16-mers (src/FMDIndex.cpp:163-176) and, with
+ * the context option "form" = 2, lets the SMEM stage skip reads that have no 16-mer
+ * in the index (exact when minLen >= 16: every SMEM that long contains one).
+ * info: [0] max error (MLP), [1] max error (root alone), [2] first loss, [3] last loss,
+ * [4] training milliseconds. */
+int msgs_lix_train(msgs_ctx* ctx, int32_t segments, int64_t samples, int32_t steps, int32_t batch, double lr,
+                   uint64_t seed, double* info);
+/* exact lower bounds lb[i] of keys (16 symbols, 3 bits each: '\0'=1 A=2 C=3 G=4 T=5 N=6,
+ * first symbol most significant); useMlp 1 = model + bound, 0 = root alone, -1 = plain
+ * binary search; probes = key-table reads, fell = 1 where the bound missed. */
+int msgs_lix_query(msgs_ctx* ctx, const uint64_t* keys, int64_t n, int32_t useMlp, int64_t* lb, int32_t* probes,
+                   int32_t* fell);
+/* raw predictions: mode 0 bf16 MFMA, mode 1 the same arithmetic on VALU */
+int msgs_lix_predict(msgs_ctx* ctx, const uint64_t* keys, int64_t n, int32_t mode, double* out);
+/* gradient of the mean squared error at weights W (1153 floats: W1[16][64], b1, W2, b2)
+ * over B rows of features X (B x 16) and targets T, by the training kernels */
+int msgs_lix_grad(msgs_ctx* ctx, const float* W, const float* X, const float* T, int32_t B, float* grad,
+                  float* loss);
+/* read (get) and/or replace (set) the 1153 MLP weights */
+int msgs_lix_weights(msgs_ctx* ctx, float* get, const float* set);
+/* knot table: S+1 keys and lower bounds */
+int msgs_lix_knots(msgs_ctx* ctx, uint64_t* keys, int64_t* ranks, int32_t* S);
 
 /* ---- utilities ----
  * The permutation std::sort (libstdc++) gives for n keys compared with '<' -- the

@@ -26,7 +26,9 @@ EXPORTS = [
     "msgs_rank_batch", "msgs_ext_batch", "msgs_smem_batch", "msgs_seed_batch",
     "msgs_map_batch", "msgs_align_batch", "msgs_batch_upload", "msgs_batch_run",
     "msgs_batch_stats", "msgs_batch_sam", "msgs_util_sort_by_key",
+    "msgs_lix_train", "msgs_lix_query", "msgs_lix_predict", "msgs_lix_grad", "msgs_lix_weights", "msgs_lix_knots",
 ]
+N_LIX_PARAMS = 16 * 64 + 64 + 64 + 1
 
 
 class Params(C.Structure):
@@ -89,6 +91,14 @@ def lib():
     L.msgs_batch_stats.argtypes = [vp, PD, C.c_int32]
     L.msgs_batch_sam.argtypes = [vp, C.c_char_p, C.c_int64, P64]
     L.msgs_util_sort_by_key.argtypes = [PD, C.POINTER(C.c_int32), C.c_int64]
+    PF = C.POINTER(C.c_float)
+    PU64 = C.POINTER(C.c_uint64)
+    L.msgs_lix_train.argtypes = [vp, C.c_int32, C.c_int64, C.c_int32, C.c_int32, C.c_double, C.c_uint64, PD]
+    L.msgs_lix_query.argtypes = [vp, PU64, C.c_int64, C.c_int32, P64, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    L.msgs_lix_predict.argtypes = [vp, PU64, C.c_int64, C.c_int32, PD]
+    L.msgs_lix_grad.argtypes = [vp, PF, PF, PF, C.c_int32, PF, PF]
+    L.msgs_lix_weights.argtypes = [vp, PF, PF]
+    L.msgs_lix_knots.argtypes = [vp, PU64, P64, C.POINTER(C.c_int32)]
     _lib = L
     return L
 
@@ -274,6 +284,49 @@ class Mapper:
             self._chk(rc, "msgs_map_batch")
             text = buf.raw[:need.value].decode()
             return [text[lo[i]:lo[i + 1]] for i in range(rb.n)]
+
+    # learned index
+    def lix_train(self, segments=4096, samples=1 << 20, steps=2000, batch=4096, lr=1e-3, seed=7):
+        info = np.zeros(5, np.float64)
+        self._chk(lib().msgs_lix_train(self.h, segments, samples, steps, batch, lr, seed, _p(info, C.c_double)),
+                  "msgs_lix_train")
+        return {"max_err": int(info[0]), "max_err_root": int(info[1]), "loss0": info[2], "loss": info[3],
+                "train_ms": info[4]}
+
+    def lix_query(self, keys, use_mlp=1):
+        keys = np.ascontiguousarray(keys, np.uint64)
+        n = len(keys)
+        lb = np.zeros(n, np.int64)
+        pr = np.zeros(n, np.int32)
+        fb = np.zeros(n, np.int32)
+        self._chk(lib().msgs_lix_query(self.h, _p(keys, C.c_uint64), n, use_mlp, _p(lb, C.c_int64),
+                                       _p(pr, C.c_int32), _p(fb, C.c_int32)), "msgs_lix_query")
+        return lb, pr, fb
+
+    def lix_predict(self, keys, mode=0):
+        keys = np.ascontiguousarray(keys, np.uint64)
+        out = np.zeros(len(keys), np.float64)
+        self._chk(lib().msgs_lix_predict(self.h, _p(keys, C.c_uint64), len(keys), mode, _p(out, C.c_double)),
+                  "msgs_lix_predict")
+        return out
+
+    def lix_grad(self, W, X, T):
+        W = np.ascontiguousarray(W, np.float32)
+        X = np.ascontiguousarray(X, np.float32)
+        T = np.ascontiguousarray(T, np.float32)
+        g = np.zeros(N_LIX_PARAMS, np.float32)
+        loss = np.zeros(1, np.float32)
+        self._chk(lib().msgs_lix_grad(self.h, _p(W, C.c_float), _p(X, C.c_float), _p(T, C.c_float), len(T),
+                                      _p(g, C.c_float), _p(loss, C.c_float)), "msgs_lix_grad")
+        return g, float(loss[0])
+
+    def lix_weights(self, set_to=None):
+        w = np.zeros(N_LIX_PARAMS, np.float32)
+        if set_to is not None:
+            s = np.ascontiguousarray(set_to, np.float32)
+            self._chk(lib().msgs_lix_weights(self.h, None, _p(s, C.c_float)), "msgs_lix_weights")
+        self._chk(lib().msgs_lix_weights(self.h, _p(w, C.c_float), None), "msgs_lix_weights")
+        return w
 
     # resident batches
     def upload(self, slot, reads, mates=None):

@@ -138,6 +138,7 @@ void msgs_ctx_destroy(msgs_ctx* c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     c->db.release();
+    c->lix.release();
     c->bufs.releaseAll();
     for (auto& s : c->slots) s.release();
     (void)hipEventDestroy(c->ev0); (void)hipEventDestroy(c->ev1);
@@ -159,6 +160,7 @@ int msgs_db_build(msgs_ctx* c, const uint8_t* seqs, const int64_t* lens, int32_t
     if (!c || !seqs || !lens || nChrom <= 0) return fail(c, MSGS_E_ARG, "bad arguments");
     if (hipSetDevice(c->device) != hipSuccess) return fail(c, MSGS_E_HIP, "hipSetDevice");
     c->db.release();
+    c->lix.release();
     DbState& db = c->db;
     db.names.clear(); db.chromLen.clear(); db.chromBD.clear(); db.tgtOff.clear();
     db.nChrom = nChrom;
@@ -355,6 +357,14 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
     a.minLen = p->minLen;
     a.gate1 = std::isfinite(p->maxEvalue) ? 1 : 0;
     a.T1 = T; a.T2 = T;
+    /* form 2: reads without any 16-mer in the index cannot have an SMEM of length >= 16 */
+    a.need = nullptr;
+    if (c->form == 2 && c->lix.trained && p->minLen >= 16 && n > 0) {
+        uint8_t* need = nullptr;
+        int rc = runLixFilter(c, r, &need, dStats);
+        if (rc) return rc;
+        a.need = need;
+    }
     const int64_t lanesMax = (int64_t)std::max(1, c->nCU) * c->lanesPerCU;
     const int64_t lanes = std::max<int64_t>(256, std::min<int64_t>(((n + 255) / 256) * 256, lanesMax));
     const int64_t candCap = r.maxL + 2;
@@ -536,7 +546,7 @@ extern "C" int32_t msgs_db_nchrom(msgs_ctx* c) { return c ? c->db.nChrom : 0; }
 extern "C" int msgs_ctx_set(msgs_ctx* c, const char* key, int64_t value) {
     if (!c || !key) return MSGS_E_ARG;
     std::string k(key);
-    if (k == "form") { if (value < 0 || value > 1) return fail(c, MSGS_E_ARG, "form 0 or 1"); c->form = (int)value; }
+    if (k == "form") { if (value < 0 || value > 2) return fail(c, MSGS_E_ARG, "form 0, 1 or 2"); c->form = (int)value; }
     else if (k == "lanes_per_cu") { if (value < 64 || value > 8192 || value % 256) return fail(c, MSGS_E_ARG, "lanes_per_cu: multiple of 256 in [256, 8192]"); c->lanesPerCU = (int)value; }
     else if (k == "smem_cap") { if (value < 1 || value > 4096) return fail(c, MSGS_E_ARG, "smem_cap in [1, 4096]"); c->smemCap = (int)value; }
     else if (k == "host_threads") { if (value < 1 || value > 256) return fail(c, MSGS_E_ARG, "host_threads in [1, 256]"); c->hostThreads = (int)value; }

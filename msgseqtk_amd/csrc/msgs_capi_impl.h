@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "msgs_host.h"
+#include "msgs_lix.h"
 
 /* grow-only named device buffers */
 struct BufPool {
@@ -78,6 +79,7 @@ struct msgs_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     std::string err;
     DbState db;
+    LixState lix;
     BufPool bufs;
     Slot slots[16];
     double stats[16] = {};
@@ -91,5 +93,6 @@ int fail(msgs_ctx* c, int code, const std::string& msg);
 bool evalueThreshold(double n, double maxE, double* T);
 int uploadReads(msgs_ctx* c, const msgs_reads* r, const char* pfx, DevReads& out);
 int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out, unsigned long long* dStats);
+int runLixFilter(msgs_ctx* c, const DevReads& r, uint8_t** needOut, unsigned long long* dStats);
 
 #endif

@@ -34,6 +34,7 @@ struct SmemArgs {
     double* candLL;
     int64_t* candX;            /* moving side as a text position (-1: row values exact) */
     int64_t nLanes;
+    const uint8_t* need;       /* per read: 0 = no 16-mer in the index, skip (may be NULL) */
     unsigned long long* queue; /* next read */
     unsigned long long* stats; /* [0] occ-block extensions, [2] text steps, [3] SA/ISA lookups */
 };

@@ -157,7 +157,7 @@ __device__ bool nextRead(const Ctx& c, Lane& l) {
         l.s = c.a.seqs + off;
         l.L = (int)(c.a.offs[rid + 1] - off);
         l.nOut = 0;
-        if (l.L > 0) {
+        if (l.L > 0 && (!c.a.need || c.a.need[rid])) {
             l.dFrom = l.L / 2;
             l.dTo = l.dFrom + 1;
             l.stage = STG_A;
