@@ -320,6 +320,14 @@ class Mapper:
                                       _p(g, C.c_float), _p(loss, C.c_float)), "msgs_lix_grad")
         return g, float(loss[0])
 
+    def lix_knots(self):
+        S = C.c_int32(0)
+        self._chk(lib().msgs_lix_knots(self.h, None, None, C.byref(S)), "msgs_lix_knots")
+        keys = np.zeros(S.value + 1, np.uint64)
+        ranks = np.zeros(S.value + 1, np.int64)
+        self._chk(lib().msgs_lix_knots(self.h, _p(keys, C.c_uint64), _p(ranks, C.c_int64), None), "msgs_lix_knots")
+        return keys, ranks
+
     def lix_weights(self, set_to=None):
         w = np.zeros(N_LIX_PARAMS, np.float32)
         if set_to is not None:

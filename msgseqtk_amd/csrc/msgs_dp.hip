@@ -1,26 +1,39 @@
 /*
- * msgs_dp.hip -- chain-restricted affine DP and backtrace, one lane per chain.
+ * msgs_dp.hip -- chain-restricted affine DP and backtrace.
  *
  * The reference (Alignment(read, rcRead, mtg, chain) + calculateScores(chain) +
  * backTrace, src/Alignment.h:84-94, src/Alignment.cpp:46-276) fills three dense
  * (L+1) x (tLen+1) double matrices with -inf and then writes, in this order:
- *   init      M row 0 = 0; NW: I(i,0) = -gapPenalty(i); SW: M col 0 = 0
- *   R5        the 5' rectangle before the first seed (if both sides are non-empty)
- *   seed k    M block := -inf, the seed diagonal (first cell floored at 0), the
- *             "step c" cell one past the seed, I row to_k+1 := M row to_k + o and
- *             D col end_k+1 := M col end_k + o (whole row / whole column)
- *   gap k     the rectangle between seed k and k+1
- *   R3        the 3' rectangle after the last seed
+ *   event 1       R5, the 5' rectangle before the first seed (if both sides are non-empty)
+ *   event 2+2k    seed k: M block := -inf, the seed diagonal (first cell floored at 0),
+ *                 the "step c" cell one past the seed, I row to_k+1 := M row to_k + o and
+ *                 D col end_k+1 := M col end_k + o (whole row / whole column)
+ *   event 3+2k    gap k, the rectangle between seed k and k+1
+ *   event 2ns+1   R3, the 3' rectangle after the last seed
+ * on top of the initial values (M row 0 = 0; NW: I(i,0) = -gapPenalty(i); SW: M col 0 = 0).
  * Only the rectangles hold O(area) data.  Here each rectangle is a tile of int32
- * M/I/D values in HBM; every other cell is resolved by walking that write history
- * backwards (vM/vI/vD below), which yields the value the dense matrix would hold at
- * any point in time.  Rectangle recurrences read their outside neighbours that way,
- * and the backtrace reads final values that way, so the result (score, end cell,
- * path, tie-breaks, even out-of-matrix reads) is the dense reference's.
+ * (M, I, D) cells in HBM, and every other cell is resolved from the write history:
+ * seeds are ordered and compatible, so rows (and columns) split into alternating
+ * rectangle and seed bands, and the latest writer of a cell at any event is found from
+ * the band of its row (vMk / vIk / vDk).  Rectangle borders, seed diagonals and the
+ * backtrace read values exactly as the dense reference would, including the order of
+ * ties and the out-of-matrix reads of the backtrace (flagged as hazard, read as -inf).
  *
- * Scores are integer-valued (the reference's defaults 1/3/5/2 and any integer
- * setting); -inf is the sentinel NEG, which absorbs additions.  Non-integer score
- * settings are refused by the host (MSGS_E_UNSUPPORTED).
+ * Three kernels:
+ *   dp_setup_kernel  one lane per job: tStart/tEnd, seeds in matrix coordinates,
+ *                    rectangle geometry and pool sizes, and the fill class;
+ *   dp_fill_kernel   one wavefront per job with rectangles: each rectangle is swept in
+ *                    strips of 64 rows along anti-diagonals (lane = row, up/diagonal
+ *                    neighbours by lane shuffle, the strip's top row in LDS), seeds'
+ *                    first diagonal values in between, rectangle summaries at the end;
+ *   dp_trace_kernel  one lane per job: the serial fill for jobs the wavefront kernel does
+ *                    not take, the NW last-row scan / SW maximum, the backtrace (seed
+ *                    diagonal interiors are crossed in one step: their neighbours are
+ *                    provably -inf), CIGAR, MD and the quality-based log10 likelihood.
+ *
+ * Scores are integer-valued (the reference's defaults 1/3/5/2 and any integer setting);
+ * -inf is the sentinel NEG, which absorbs additions.  Non-integer score settings are
+ * refused by the host (MSGS_E_UNSUPPORTED).
  */
 #include "msgs_kernels.h"
 
@@ -29,143 +42,170 @@ namespace {
 constexpr int32_t NEG = -1073741824;
 enum : int32_t { OP_M = 0, OP_I = 1, OP_D = 2, OP_S = 4, OP_EQ = 7, OP_X = 8 };
 enum : int32_t { FWD = 1, REV = 2 };
+enum : int32_t { CLS_NONE = 0, CLS_WAVE = 1, CLS_SERIAL = 2 };
+/* limits of the wavefront fill: seeds per job (LDS copy) and rectangle columns */
+constexpr int FILL_NS = 256;
+constexpr int FILL_COLS = 512;
 
 __device__ __forceinline__ int32_t addn(int32_t a, int32_t b) { return a == NEG ? NEG : a + b; }
+__device__ __forceinline__ int32_t subn(int32_t a, int32_t b) { return a == NEG ? NEG : a - b; }
 __device__ __forceinline__ int32_t max3(int32_t a, int32_t b, int32_t c) { return max(a, max(b, c)); }
 
-struct Rect {
-    int64_t r0, r1, c0, c1, off;   /* rows r0..r1, cols c0..c1 (matrix coords), tile offset */
-    __device__ bool has(int64_t i, int64_t j) const { return i >= r0 && i <= r1 && j >= c0 && j <= c1; }
-    __device__ int64_t idx(int64_t i, int64_t j) const { return off + (j - c0) * (r1 - r0 + 1) + (i - r0); }
-    __device__ bool empty() const { return r0 > r1 || c0 > c1; }
-    __device__ int64_t area() const { return empty() ? 0 : (r1 - r0 + 1) * (c1 - c0 + 1); }
+/* a rectangle with its tile offset and event index */
+struct Rc {
+    int32_t r0, r1, c0, c1, ev;
+    int64_t off;
+    bool ok;
+    __device__ int64_t idx(int32_t i, int32_t j) const { return off + (int64_t)(j - c0) * (r1 - r0 + 1) + (i - r0); }
+    __device__ bool hasCol(int32_t j) const { return j >= c0 && j <= c1; }
+    __device__ int64_t area() const { return ok ? (int64_t)(r1 - r0 + 1) * (c1 - c0 + 1) : 0; }
 };
 
-struct Job {
-    const SeedRec* seeds;     /* the read's seeds */
-    const int32_t* idx;       /* chain: seed indices */
-    int32_t ns;
-    int64_t L, tStart, tEnd, tLen;
-    const uint8_t* q;
-    const uint8_t* qual;
-    const uint8_t* tg;        /* target chrom (followed by its 0 terminator) */
-    int64_t chromLen;
-    int32_t* tM;
-    int32_t* tI;
-    int32_t* tD;
-    int32_t* d0;              /* first diagonal value of each seed */
-    int64_t* gapOff;          /* tile offset of gap rect k */
-    Rect r5, r3;
-    int32_t mode, match, mis, go, ge;
+struct Geo {
+    const int4* sg;          /* bi, bj, len, d0 per seed */
+    const int64_t* gapOff;   /* tile offset of gap rect k */
+    int4* tile;              /* M, I, D per rectangle cell */
+    int32_t L, tLen, ns, mode, match, mis, go, ge, o;
+    int32_t r5r1, r5c1, r3r0, r3c0;
+    int64_t r3off;
     int32_t hazard;
-
-    __device__ const SeedRec& S(int k) const { return seeds[idx[k]]; }
-    __device__ Rect gap(int k) const {
-        const SeedRec& a = S(k);
-        const SeedRec& b = S(k + 1);
-        Rect r;
-        r.r0 = a.to + 1; r.r1 = b.from; r.c0 = a.end - tStart + 1; r.c1 = b.start - tStart;
-        r.off = gapOff[k];
-        return r;
-    }
-    __device__ uint8_t tb(int64_t t) const { return (t >= 0 && t <= chromLen) ? tg[t] : (uint8_t)0; }
-    __device__ int32_t score(uint8_t a, uint8_t b) const {
-        return ((a & b) && a != 15 && b != 15) ? match : -mis;
-    }
 };
 
-/* M at (i, j) after events 1..e (0 = init only) */
-__device__ int32_t vM(Job& J, int64_t i, int64_t j, int e) {
-    if (i < 0 || j < 0 || i > J.L || j > J.tLen) { J.hazard = 1; return NEG; }
-    const int last = 2 * J.ns + 1;
-    for (; e >= 1; --e) {
-        if (e == 1) {
-            if (!J.r5.empty() && J.r5.has(i, j)) return J.tM[J.r5.idx(i, j)];
-        } else if (e == last) {
-            if (!J.r3.empty() && J.r3.has(i, j)) return J.tM[J.r3.idx(i, j)];
-        } else if ((e & 1) == 0) {
-            const int k = (e - 2) >> 1;
-            const SeedRec& s = J.S(k);
-            const int64_t bi = s.from + 1, bj = s.start - J.tStart + 1, len = s.to - s.from;
-            if (i >= bi && i < bi + len && j >= bj && j < bj + len)
-                return (i - bi == j - bj) ? J.d0[k] + (int32_t)(i - bi) * J.match : NEG;
-            if (i == bi + len && j == bj + len && i <= J.L && j <= J.tLen) return J.d0[k] + (int32_t)len * J.match;
-        } else {
-            const Rect g = J.gap((e - 3) >> 1);
-            if (!g.empty() && g.has(i, j)) return J.tM[g.idx(i, j)];
-        }
-    }
-    return (i == 0 || (J.mode == 1 && j == 0)) ? 0 : NEG;
+__device__ __forceinline__ int32_t initM(const Geo& G, int32_t i, int32_t j) {
+    return (i == 0 || (G.mode == 1 && j == 0)) ? 0 : NEG;
+}
+__device__ __forceinline__ int32_t initI(const Geo& G, int32_t i, int32_t j) {
+    return (G.mode == 0 && j == 0) ? (i == 0 ? 0 : -(G.go + G.ge * i)) : NEG;
 }
 
-__device__ int32_t vI(Job& J, int64_t i, int64_t j, int e) {
-    if (i < 0 || j < 0 || i > J.L || j > J.tLen) { J.hazard = 1; return NEG; }
-    const int last = 2 * J.ns + 1;
-    const int32_t o = -(J.go + J.ge);
-    for (; e >= 1; --e) {
-        if (e == 1) {
-            if (!J.r5.empty() && J.r5.has(i, j)) return J.tI[J.r5.idx(i, j)];
-        } else if (e == last) {
-            if (!J.r3.empty() && J.r3.has(i, j)) return J.tI[J.r3.idx(i, j)];
-        } else if ((e & 1) == 0) {
-            const int k = (e - 2) >> 1;
-            const int64_t r = J.S(k).to + 1;
-            if (i == r && r <= J.L) return addn(vM(J, i - 1, j, e), o);
-        } else {
-            const Rect g = J.gap((e - 3) >> 1);
-            if (!g.empty() && g.has(i, j)) return J.tI[g.idx(i, j)];
+/* largest k with bi_k <= i (-1 if none) */
+__device__ int rowSeg(const Geo& G, int32_t i) {
+    int lo = -1, hi = G.ns - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (G.sg[mid].x <= i) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+/* largest k with bj_k <= j (-1 if none) */
+__device__ int colSeg(const Geo& G, int32_t j) {
+    int lo = -1, hi = G.ns - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (G.sg[mid].y <= j) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ Rc rectR5(const Geo& G) {
+    Rc R;
+    R.r0 = 1; R.r1 = G.r5r1; R.c0 = 1; R.c1 = G.r5c1; R.ev = 1; R.off = 0;
+    R.ok = G.r5r1 >= 1 && G.r5c1 >= 1;
+    return R;
+}
+__device__ __forceinline__ Rc rectR3(const Geo& G) {
+    Rc R;
+    R.r0 = G.r3r0; R.r1 = G.L; R.c0 = G.r3c0; R.c1 = G.tLen; R.ev = 2 * G.ns + 1; R.off = G.r3off;
+    R.ok = R.r0 <= R.r1 && R.c0 <= R.c1;
+    return R;
+}
+/* gap k between seed s = k and t = k+1 */
+__device__ __forceinline__ Rc rectGap(const Geo& G, int k, const int4& s, const int4& t) {
+    Rc R;
+    R.r0 = s.x + s.z; R.r1 = t.x - 1; R.c0 = s.y + s.z; R.c1 = t.y - 1; R.ev = 3 + 2 * k; R.off = G.gapOff[k];
+    R.ok = R.r0 <= R.r1 && R.c0 <= R.c1;
+    return R;
+}
+/* rectangle of the band after seed k (k = rowSeg(i) and row i past seed k) */
+__device__ __forceinline__ Rc rectAfter(const Geo& G, int k, const int4& s) {
+    if (k + 1 < G.ns) return rectGap(G, k, s, G.sg[k + 1]);
+    return rectR3(G);
+}
+
+/* M at (i, j) after events 1..e; (i, j) inside the matrix, k = rowSeg(i) */
+__device__ int32_t vMk(const Geo& G, int32_t i, int32_t j, int e, int k) {
+    if (k < 0) {
+        const Rc R = rectR5(G);
+        if (i >= 1 && R.ok && e >= 1 && R.hasCol(j)) return G.tile[R.idx(i, j)].x;
+        return initM(G, i, j);
+    }
+    const int4 s = G.sg[k];
+    if (i < s.x + s.z) {   /* seed k's rows */
+        if (e >= 2 + 2 * k && j >= s.y && j < s.y + s.z) return (i - s.x == j - s.y) ? s.w + (i - s.x) * G.match : NEG;
+        if (k >= 1 && e >= 2 * k) {   /* step c of seed k-1 on the first row of seed k */
+            const int4 p = G.sg[k - 1];
+            if (i == p.x + p.z && j == p.y + p.z) return p.w + p.z * G.match;
+        }
+        return initM(G, i, j);
+    }
+    const Rc R = rectAfter(G, k, s);
+    if (R.ok && e >= R.ev && R.hasCol(j)) return G.tile[R.idx(i, j)].x;
+    if (e >= 2 + 2 * k && i == s.x + s.z && j == s.y + s.z) return s.w + s.z * G.match;   /* step c */
+    return initM(G, i, j);
+}
+
+__device__ int32_t vIk(const Geo& G, int32_t i, int32_t j, int e, int k) {
+    if (k >= 0) {
+        const int4 s = G.sg[k];
+        if (i >= s.x + s.z) {
+            const Rc R = rectAfter(G, k, s);
+            if (R.ok && e >= R.ev && R.hasCol(j)) return G.tile[R.idx(i, j)].y;
+            /* I row to_k + 1 := M row to_k + o at seed k's event */
+            if (i == s.x + s.z && e >= 2 + 2 * k) return addn(vMk(G, i - 1, j, 2 + 2 * k, k), G.o);
+        } else if (k >= 1 && e >= 2 * k) {
+            const int4 p = G.sg[k - 1];
+            if (i == p.x + p.z) return addn(vMk(G, i - 1, j, 2 * k, k - 1), G.o);
+        }
+    } else {
+        const Rc R = rectR5(G);
+        if (i >= 1 && R.ok && e >= 1 && R.hasCol(j)) return G.tile[R.idx(i, j)].y;
+    }
+    return initI(G, i, j);
+}
+
+__device__ int32_t vDk(const Geo& G, int32_t i, int32_t j, int e, int k) {
+    if (k < 0) {
+        const Rc R = rectR5(G);
+        if (i >= 1 && R.ok && e >= 1 && R.hasCol(j)) return G.tile[R.idx(i, j)].z;
+    } else {
+        const int4 s = G.sg[k];
+        if (i >= s.x + s.z) {
+            const Rc R = rectAfter(G, k, s);
+            if (R.ok && e >= R.ev && R.hasCol(j)) return G.tile[R.idx(i, j)].z;
         }
     }
-    if (J.mode == 0 && j == 0) return i == 0 ? 0 : -(J.go + J.ge * (int32_t)i);
+    /* D col end_m + 1 := M col end_m + o at seed m's event; m is colSeg(j) or the one before */
+    const int m = colSeg(G, j);
+    if (m >= 0) {
+        const int4 s = G.sg[m];
+        if (j == s.y + s.z && e >= 2 + 2 * m) return addn(vMk(G, i, j - 1, 2 + 2 * m, k), G.o);
+        if (m >= 1) {
+            const int4 p = G.sg[m - 1];
+            if (j == p.y + p.z && e >= 2 * m) return addn(vMk(G, i, j - 1, 2 * m, k), G.o);
+        }
+    }
     return NEG;
 }
 
-__device__ int32_t vD(Job& J, int64_t i, int64_t j, int e) {
-    if (i < 0 || j < 0 || i > J.L || j > J.tLen) { J.hazard = 1; return NEG; }
-    const int last = 2 * J.ns + 1;
-    const int32_t o = -(J.go + J.ge);
-    for (; e >= 1; --e) {
-        if (e == 1) {
-            if (!J.r5.empty() && J.r5.has(i, j)) return J.tD[J.r5.idx(i, j)];
-        } else if (e == last) {
-            if (!J.r3.empty() && J.r3.has(i, j)) return J.tD[J.r3.idx(i, j)];
-        } else if ((e & 1) == 0) {
-            const int k = (e - 2) >> 1;
-            const int64_t c = J.S(k).end - J.tStart + 1;
-            if (j == c && c <= J.tLen) return addn(vM(J, i, j - 1, e), o);
-        } else {
-            const Rect g = J.gap((e - 3) >> 1);
-            if (!g.empty() && g.has(i, j)) return J.tD[g.idx(i, j)];
-        }
-    }
-    return NEG;
+__device__ __forceinline__ bool inside(Geo& G, int32_t i, int32_t j) {
+    if (i < 0 || j < 0 || i > G.L || j > G.tLen) { G.hazard = 1; return false; }
+    return true;
+}
+__device__ int32_t vM(Geo& G, int32_t i, int32_t j, int e) { return inside(G, i, j) ? vMk(G, i, j, e, rowSeg(G, i)) : NEG; }
+__device__ int32_t vI(Geo& G, int32_t i, int32_t j, int e) { return inside(G, i, j) ? vIk(G, i, j, e, rowSeg(G, i)) : NEG; }
+__device__ int32_t vD(Geo& G, int32_t i, int32_t j, int e) { return inside(G, i, j) ? vDk(G, i, j, e, rowSeg(G, i)) : NEG; }
+
+/* first diagonal value of seed k (src/Alignment.cpp:146-150), from its corner at event 1+2k */
+__device__ int32_t seedD0(Geo& G, int k) {
+    const int4 s = G.sg[k];
+    const int e = 1 + 2 * k;
+    const int32_t sm = G.match;
+    return max(max3(addn(vM(G, s.x - 1, s.y - 1, e), sm), addn(vD(G, s.x - 1, s.y - 1, e), sm),
+                    addn(vI(G, s.x - 1, s.y - 1, e), sm)), 0);
 }
 
-/* calculateScoresNW / calculateScoresSW over a rectangle written at event e */
-__device__ int64_t fillRect(Job& J, const Rect& R, int e) {
-    if (R.empty()) return 0;
-    const int32_t o = -(J.go + J.ge), ex = -J.ge;
-    for (int64_t j = R.c0; j <= R.c1; ++j) {
-        const uint8_t t = J.tb(J.tStart + j - 1);
-        for (int64_t i = R.r0; i <= R.r1; ++i) {
-            const int32_t sc = J.score(J.q[i - 1], t);
-            const bool up = i - 1 >= R.r0, left = j - 1 >= R.c0;
-            int32_t mD, iD, dD, mU, iU, mL, dL;
-            if (up && left) { int64_t x = R.idx(i - 1, j - 1); mD = J.tM[x]; iD = J.tI[x]; dD = J.tD[x]; }
-            else { mD = vM(J, i - 1, j - 1, e - 1); iD = vI(J, i - 1, j - 1, e - 1); dD = vD(J, i - 1, j - 1, e - 1); }
-            if (up) { int64_t x = R.idx(i - 1, j); mU = J.tM[x]; iU = J.tI[x]; }
-            else { mU = vM(J, i - 1, j, e - 1); iU = vI(J, i - 1, j, e - 1); }
-            if (left) { int64_t x = R.idx(i, j - 1); mL = J.tM[x]; dL = J.tD[x]; }
-            else { mL = vM(J, i, j - 1, e - 1); dL = vD(J, i, j - 1, e - 1); }
-            int32_t m = max3(addn(mD, sc), addn(dD, sc), addn(iD, sc));
-            if (J.mode == 1) m = max(m, 0);
-            const int64_t x = R.idx(i, j);
-            J.tM[x] = m;
-            J.tI[x] = max(addn(mU, o), addn(iU, ex));
-            J.tD[x] = max(addn(mL, o), addn(dL, ex));
-        }
-    }
-    return R.area();
+__device__ __forceinline__ bool swBetter(int32_t v, int32_t i, int32_t j, int32_t bv, int32_t bi, int32_t bj) {
+    return v > bv || (v == bv && (j < bj || (j == bj && i < bi)));
 }
 
 /* matchMax / insMax / delMax (src/Alignment.h:577-597), matchMax quirk included */
@@ -175,7 +215,6 @@ __device__ __forceinline__ int32_t matchMax(int32_t match, int32_t ins, int32_t 
     if (del > mx) s = OP_D;
     return s;
 }
-__device__ __forceinline__ int32_t subn(int32_t a, int32_t b) { return a == NEG ? NEG : a - b; }
 
 __device__ __forceinline__ char decodeBase(uint8_t b) {
     const char* sym = "$ACMGRSVTWYHKDBN";
@@ -190,153 +229,337 @@ __device__ __forceinline__ int putNum(char* md, int64_t pos, uint32_t v) {
     return n;
 }
 
-__device__ void setupJob(const DevIndex& ix, const DpArgs& a, int64_t job, Job& J) {
-    const int32_t rid = a.jobRead[job];
-    J.seeds = a.seeds + a.seedOff[rid];
-    J.idx = a.chainSeeds + a.jobChainStart[job];
-    J.ns = a.jobChainLen[job];
-    J.L = a.offs[rid + 1] - a.offs[rid];
-    const SeedRec& f = J.S(0);
-    const SeedRec& l = J.S(J.ns - 1);
-    const int32_t tid = f.tid;
-    J.chromLen = ix.chromLen[tid];
-    J.tg = ix.tgt + ix.tgtOff[tid];
-    /* initTStart / initTEnd (src/Alignment.cpp:46-52) */
-    const double kIndel = 1 + 0.1;
-    J.tStart = max((int64_t)0, (int64_t)((double)f.start - kIndel * (double)(f.from - 0)));
-    J.tEnd = min(J.chromLen + 1, (int64_t)((double)l.end + kIndel * (double)(J.L - l.to)));
-    J.tLen = J.tEnd - J.tStart;
-    const bool fwd = f.strand == FWD;
-    J.q = (fwd ? a.seqs : a.rcSeqs) + a.offs[rid];
-    J.qual = (fwd ? a.quals : a.rcQuals) + a.offs[rid];
-    J.mode = a.mode; J.match = a.match; J.mis = a.mismatch; J.go = a.gapO; J.ge = a.gapE;
-    J.hazard = 0;
-    J.r5.r0 = 1; J.r5.r1 = f.from; J.r5.c0 = 1; J.r5.c1 = f.start - J.tStart;
-    if (!(0 < f.from && J.tStart < f.start)) { J.r5.r0 = 1; J.r5.r1 = 0; }
-    J.r3.r0 = l.to + 1; J.r3.r1 = J.L; J.r3.c0 = l.end - J.tStart + 1; J.r3.c1 = J.tLen;
-    if (!(l.to < J.L && l.end < J.tEnd)) { J.r3.r0 = 1; J.r3.r1 = 0; }
+__device__ __forceinline__ void makeGeo(Geo& G, const DpArgs& a, const JobGeo& g, const int4* sg, int64_t job) {
+    G.sg = sg;
+    G.gapOff = a.auxGap + g.cs;
+    G.tile = a.tile + a.tileOff[job];
+    G.L = g.L; G.tLen = g.tLen; G.ns = g.ns; G.mode = a.mode;
+    G.match = a.match; G.mis = a.mismatch; G.go = a.gapO; G.ge = a.gapE; G.o = -(a.gapO + a.gapE);
+    G.r5r1 = g.r5r1; G.r5c1 = g.r5c1; G.r3r0 = g.r3r0; G.r3c0 = g.r3c0; G.r3off = g.r3off;
+    G.hazard = 0;
 }
 
 }  // namespace
 
-__global__ void __launch_bounds__(256) dp_kernel(DevIndex ix, DpArgs a) {
+/* ------------------------------------------------------------------ setup */
+
+__global__ void __launch_bounds__(256) dp_setup_kernel(DevIndex ix, DpArgs a) {
     for (int64_t job = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; job < a.nJobs;
          job += (int64_t)gridDim.x * blockDim.x) {
-        Job J;
-        setupJob(ix, a, job, J);
-        const int64_t pathCap = J.L + J.tLen + 3;
-        if (a.sizeOnly) {
-            int64_t area = J.r5.area() + J.r3.area();
-            for (int k = 0; k + 1 < J.ns; ++k) {
-                const SeedRec& s0 = J.S(k);
-                const SeedRec& s1 = J.S(k + 1);
-                int64_t rows = s1.from - s0.to, cols = s1.start - s0.end;
-                if (rows > 0 && cols > 0) area += rows * cols;
+        const int32_t rid = a.jobRead[job];
+        const SeedRec* seeds = a.seeds + a.seedOff[rid];
+        const int32_t cs = a.jobChainStart[job];
+        const int32_t* idx = a.chainSeeds + cs;
+        const int32_t ns = a.jobChainLen[job];
+        const int64_t L = a.offs[rid + 1] - a.offs[rid];
+        const SeedRec& f = seeds[idx[0]];
+        const SeedRec& l = seeds[idx[ns - 1]];
+        const int64_t chromLen = ix.chromLen[f.tid];
+        /* initTStart / initTEnd (src/Alignment.cpp:46-52) */
+        const double kIndel = 1 + 0.1;
+        const int64_t tStart = max((int64_t)0, (int64_t)((double)f.start - kIndel * (double)(f.from - 0)));
+        const int64_t tEnd = min(chromLen + 1, (int64_t)((double)l.end + kIndel * (double)(L - l.to)));
+        JobGeo g;
+        g.tStart = tStart; g.tEnd = tEnd;
+        g.L = (int32_t)L; g.tLen = (int32_t)(tEnd - tStart); g.ns = ns; g.cs = cs;
+        g.tid = f.tid; g.strand = f.strand; g.rid = rid;
+        /* R5 only when both sides are non-empty (src/Alignment.cpp:57-58) */
+        g.r5r1 = (int32_t)f.from; g.r5c1 = (int32_t)(f.start - tStart);
+        if (!(0 < f.from && tStart < f.start)) { g.r5r1 = 0; g.r5c1 = 0; }
+        int64_t area = (int64_t)g.r5r1 * g.r5c1;
+        int32_t maxCols = g.r5r1 >= 1 ? g.r5c1 : 0;
+        for (int k = 0; k < ns; ++k) {
+            const SeedRec& s = seeds[idx[k]];
+            a.auxSG[cs + k] = make_int4((int32_t)(s.from + 1), (int32_t)(s.start - tStart + 1), (int32_t)(s.to - s.from), 0);
+            if (k + 1 < ns) {
+                const SeedRec& t = seeds[idx[k + 1]];
+                const int64_t rows = t.from - s.to, cols = t.start - s.end;
+                a.auxGap[cs + k] = area;
+                if (rows > 0 && cols > 0) { area += rows * cols; maxCols = max(maxCols, (int32_t)cols); }
             }
-            a.tileNeed[job] = area;
-            a.pathNeed[job] = pathCap;
-            continue;
         }
-        const int64_t base = a.tileOff[job];
-        J.tM = a.tileM + base; J.tI = a.tileI + base; J.tD = a.tileD + base;
-        /* per-seed scratch, addressed by the chain's range in the chain-seed pool */
-        J.d0 = a.auxD0 + a.jobChainStart[job];
-        J.gapOff = a.auxGap + a.jobChainStart[job];
-        AlnRec rec;
-        rec.rid = a.jobRead[job];
-        /* tiles are laid out R5, gap 0 .. gap ns-2, R3 */
-        int64_t cur = J.r5.area();
-        J.r5.off = 0;
-        for (int k = 0; k + 1 < J.ns; ++k) {
-            const SeedRec& s0 = J.S(k);
-            const SeedRec& s1 = J.S(k + 1);
-            int64_t rows = s1.from - s0.to, cols = s1.start - s0.end;
-            J.gapOff[k] = cur;
-            if (rows > 0 && cols > 0) cur += rows * cols;
+        g.r3off = area;
+        g.r3r0 = (int32_t)(l.to + 1); g.r3c0 = (int32_t)(l.end - tStart + 1);
+        if (!(l.to < L && l.end < tEnd)) { g.r3r0 = g.L + 1; g.r3c0 = g.tLen + 1; }
+        else { area += (int64_t)(g.L - g.r3r0 + 1) * (g.tLen - g.r3c0 + 1); maxCols = max(maxCols, g.tLen - g.r3c0 + 1); }
+        g.cls = area == 0 ? CLS_NONE : ((ns <= FILL_NS && maxCols <= FILL_COLS) ? CLS_WAVE : CLS_SERIAL);
+        a.geo[job] = g;
+        a.jobCls[job] = g.cls;
+        a.tileNeed[job] = area;
+        a.pathNeed[job] = L + g.tLen + 3;
+    }
+}
+
+/* ------------------------------------------------------------------ wavefront fill */
+
+__global__ void __launch_bounds__(64) dp_fill_kernel(DevIndex ix, DpArgs a) {
+    __shared__ int4 sSG[FILL_NS];
+    __shared__ int32_t sTM[FILL_COLS + 1], sTI[FILL_COLS + 1], sTD[FILL_COLS + 1];
+    __shared__ uint8_t sTg[FILL_COLS];
+    const int lane = threadIdx.x;
+    for (int64_t w = blockIdx.x; w < a.nWaveJobs; w += gridDim.x) {
+        const int64_t job = a.waveJobs[w];
+        const JobGeo g = a.geo[job];
+        for (int k = lane; k < g.ns; k += 64) sSG[k] = a.auxSG[g.cs + k];
+        __syncthreads();
+        Geo G;
+        makeGeo(G, a, g, sSG, job);
+        const uint8_t* q = ((g.strand == FWD) ? a.seqs : a.rcSeqs) + a.offs[g.rid];
+        const uint8_t* tg = ix.tgt + ix.tgtOff[g.tid];
+        const int64_t chromLen = ix.chromLen[g.tid];
+        int32_t bv = NEG, bi = 0, bj = 0;               /* SW: best rectangle cell of this lane */
+        int32_t nv = NEG, nc = 0, nst = OP_M;           /* NW: scan of R3's last row (its lane) */
+        int32_t cm = NEG, ci = NEG, cd = NEG;           /* corner of the last filled rectangle */
+
+        /* sweep rectangle R (event R.ev), reading its borders at event R.ev - 1 */
+        auto fill = [&](const Rc& R) {
+            const int32_t Rn = R.r1 - R.r0 + 1, C = R.c1 - R.c0 + 1;
+            const int e = R.ev;
+            for (int t = lane; t <= C; t += 64) {
+                const int32_t j = R.c0 - 1 + t;
+                sTM[t] = vM(G, R.r0 - 1, j, e - 1);
+                sTI[t] = vI(G, R.r0 - 1, j, e - 1);
+                sTD[t] = vD(G, R.r0 - 1, j, e - 1);
+                if (t < C) {
+                    const int64_t tp = g.tStart + (R.c0 + t) - 1;
+                    sTg[t] = (tp >= 0 && tp <= chromLen) ? tg[tp] : (uint8_t)0;
+                }
+            }
+            __syncthreads();
+            for (int32_t s0 = 0; s0 < Rn; s0 += 64) {
+                const int32_t rows = min(64, Rn - s0);
+                const int32_t i = R.r0 + s0 + lane;
+                const bool act = lane < rows;
+                const bool more = s0 + rows < Rn;
+                int32_t cM = NEG, cI = NEG, cD = NEG;
+                if (act) {
+                    cM = vM(G, i, R.c0 - 1, e - 1);
+                    cI = vI(G, i, R.c0 - 1, e - 1);
+                    cD = vD(G, i, R.c0 - 1, e - 1);
+                }
+                const int32_t bM0 = cM, bI0 = cI, bD0 = cD;
+                const uint8_t qb = act ? q[i - 1] : (uint8_t)0;
+                int32_t pM = NEG, pI = NEG, pD = NEG;
+                const int32_t steps = C + rows - 1;
+                for (int32_t t = 0; t < steps; ++t) {
+                    /* up = (i-1, j): row above, one step ago; diagonal = (i-1, j-1): two steps ago */
+                    int32_t uM = __shfl_up(cM, 1, 64), uI = __shfl_up(cI, 1, 64), uD = __shfl_up(cD, 1, 64);
+                    int32_t dM = pM, dI = pI, dD = pD;
+                    const int32_t jj = t - lane;
+                    if (lane == 0) {
+                        const int32_t x = min(jj, C - 1);
+                        uM = sTM[x + 1]; uI = sTI[x + 1]; uD = sTD[x + 1];
+                        dM = sTM[x]; dI = sTI[x]; dD = sTD[x];
+                    }
+                    pM = uM; pI = uI; pD = uD;
+                    if (act && jj >= 0 && jj < C) {
+                        const int32_t j = R.c0 + jj;
+                        const uint8_t tb = sTg[jj];
+                        const int32_t sc = ((qb & tb) && qb != 15 && tb != 15) ? G.match : -G.mis;
+                        int32_t m = max3(addn(dM, sc), addn(dD, sc), addn(dI, sc));
+                        if (G.mode == 1) m = max(m, 0);
+                        const int32_t nI = max(addn(uM, G.o), addn(uI, -G.ge));
+                        const int32_t nD = max(addn(cM, G.o), addn(cD, -G.ge));
+                        cM = m; cI = nI; cD = nD;
+                        G.tile[R.off + (int64_t)jj * Rn + (i - R.r0)] = make_int4(m, nI, nD, 0);
+                        if (more && lane == rows - 1) { sTM[jj + 1] = m; sTI[jj + 1] = nI; sTD[jj + 1] = nD; }
+                        if (G.mode == 1) {
+                            if (swBetter(m, i, j, bv, bi, bj)) { bv = m; bi = i; bj = j; }
+                        } else if (i == G.L) {
+                            if (m > nv) { nv = m; nc = j; nst = OP_M; }
+                            if (nI > nv) { nv = nI; nc = j; nst = OP_I; }
+                        }
+                    }
+                }
+                __syncthreads();
+                if (more && lane == rows - 1) { sTM[0] = bM0; sTI[0] = bI0; sTD[0] = bD0; }
+                if (!more) {
+                    cm = __shfl(cM, rows - 1, 64);
+                    ci = __shfl(cI, rows - 1, 64);
+                    cd = __shfl(cD, rows - 1, 64);
+                }
+                __syncthreads();
+            }
+        };
+
+        Rc R = rectR5(G);
+        bool prevOk = R.ok;
+        if (R.ok) fill(R);
+        for (int k = 0; k < g.ns; ++k) {
+            int32_t d0;
+            if (prevOk) d0 = max(max3(addn(cm, G.match), addn(cd, G.match), addn(ci, G.match)), 0);
+            else d0 = seedD0(G, k);
+            __syncthreads();
+            if (lane == 0) sSG[k].w = d0;
+            __syncthreads();
+            if (k + 1 < g.ns) {
+                R = rectGap(G, k, sSG[k], sSG[k + 1]);
+                prevOk = R.ok;
+                if (R.ok) fill(R);
+            }
         }
-        J.r3.off = cur;
-        int32_t* d0p = J.d0;
-        /* ---- calculateScores(chain) (src/Alignment.cpp:54-70) ---- */
-        int64_t cells = fillRect(J, J.r5, 1);
-        for (int k = 0; k < J.ns; ++k) {
-            const int e = 2 + 2 * k;
-            const SeedRec& s = J.S(k);
-            const int64_t bi = s.from + 1, bj = s.start - J.tStart + 1, len = s.to - s.from;
-            const int32_t sm = J.match;
-            d0p[k] = max(max3(addn(vM(J, bi - 1, bj - 1, e - 1), sm), addn(vD(J, bi - 1, bj - 1, e - 1), sm),
-                              addn(vI(J, bi - 1, bj - 1, e - 1), sm)), 0);
-            cells += len;
-            if (bi + len <= J.L && bj + len <= J.tLen) cells += 1;
-            if (k + 1 < J.ns) cells += fillRect(J, J.gap(k), e + 1);
+        R = rectR3(G);
+        if (R.ok) fill(R);
+        for (int k = lane; k < g.ns; k += 64) a.auxSG[g.cs + k].w = sSG[k].w;
+        for (int off = 32; off >= 1; off >>= 1) {
+            const int32_t ov = __shfl_xor(bv, off, 64), oi = __shfl_xor(bi, off, 64), oj = __shfl_xor(bj, off, 64);
+            if (swBetter(ov, oi, oj, bv, bi, bj)) { bv = ov; bi = oi; bj = oj; }
         }
-        cells += fillRect(J, J.r3, 2 * J.ns + 1);
-        const int T = 2 * J.ns + 1;
+        const int nwLane = R.ok ? (G.L - R.r0) % 64 : 0;
+        nv = __shfl(nv, nwLane, 64);
+        nc = __shfl(nc, nwLane, 64);
+        nst = __shfl(nst, nwLane, 64);
+        if (lane == 0) {
+            JobSum S;
+            S.swV = bv; S.swI = bi; S.swJ = bj; S.nwV = nv; S.nwC = nc; S.nwS = nst; S.pad0 = 0; S.pad1 = 0;
+            a.sum[job] = S;
+        }
+        __syncthreads();
+    }
+}
+
+/* ------------------------------------------------------------------ trace */
+
+__global__ void __launch_bounds__(256) dp_trace_kernel(DevIndex ix, DpArgs a) {
+    for (int64_t job = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; job < a.nJobs;
+         job += (int64_t)gridDim.x * blockDim.x) {
+        const JobGeo g = a.geo[job];
+        int4* sgw = a.auxSG + g.cs;
+        Geo G;
+        makeGeo(G, a, g, sgw, job);
+        const bool fwd = g.strand == FWD;
+        const uint8_t* q = (fwd ? a.seqs : a.rcSeqs) + a.offs[g.rid];
+        const uint8_t* qual = (fwd ? a.quals : a.rcQuals) + a.offs[g.rid];
+        const uint8_t* tg = ix.tgt + ix.tgtOff[g.tid];
+        const int64_t chromLen = ix.chromLen[g.tid];
+        auto tb = [&](int64_t t) -> uint8_t { return (t >= 0 && t <= chromLen) ? tg[t] : (uint8_t)0; };
+        const int32_t L = g.L, ns = g.ns;
+        JobSum S;
+        S.swV = NEG; S.swI = 0; S.swJ = 0; S.nwV = NEG; S.nwC = 0; S.nwS = OP_M;
+        if (g.cls == CLS_WAVE) {
+            S = a.sum[job];
+        } else {
+            /* calculateScores(chain) (src/Alignment.cpp:54-70) in event order, one lane */
+            auto fillSerial = [&](const Rc& R) {
+                if (!R.ok) return;
+                const int e = R.ev;
+                for (int32_t j = R.c0; j <= R.c1; ++j) {
+                    const uint8_t t = tb(g.tStart + j - 1);
+                    int32_t uM = vM(G, R.r0 - 1, j, e - 1), uI = vI(G, R.r0 - 1, j, e - 1);
+                    for (int32_t i = R.r0; i <= R.r1; ++i) {
+                        const uint8_t qb = q[i - 1];
+                        const int32_t sc = ((qb & t) && qb != 15 && t != 15) ? G.match : -G.mis;
+                        int32_t mD, iD, dD, mL, dL;
+                        if (j > R.c0 && i > R.r0) { const int4 v = G.tile[R.idx(i - 1, j - 1)]; mD = v.x; iD = v.y; dD = v.z; }
+                        else { mD = vM(G, i - 1, j - 1, e - 1); iD = vI(G, i - 1, j - 1, e - 1); dD = vD(G, i - 1, j - 1, e - 1); }
+                        if (j > R.c0) { const int4 v = G.tile[R.idx(i, j - 1)]; mL = v.x; dL = v.z; }
+                        else { mL = vM(G, i, j - 1, e - 1); dL = vD(G, i, j - 1, e - 1); }
+                        int32_t m = max3(addn(mD, sc), addn(dD, sc), addn(iD, sc));
+                        if (G.mode == 1) m = max(m, 0);
+                        const int32_t nI = max(addn(uM, G.o), addn(uI, -G.ge));
+                        const int32_t nD = max(addn(mL, G.o), addn(dL, -G.ge));
+                        G.tile[R.idx(i, j)] = make_int4(m, nI, nD, 0);
+                        uM = m; uI = nI;
+                        if (G.mode == 1) {
+                            if (swBetter(m, i, j, S.swV, S.swI, S.swJ)) { S.swV = m; S.swI = i; S.swJ = j; }
+                        } else if (i == L) {
+                            if (m > S.nwV) { S.nwV = m; S.nwC = j; S.nwS = OP_M; }
+                            if (nI > S.nwV) { S.nwV = nI; S.nwC = j; S.nwS = OP_I; }
+                        }
+                    }
+                }
+            };
+            if (g.cls == CLS_SERIAL) fillSerial(rectR5(G));
+            for (int k = 0; k < ns; ++k) {
+                sgw[k].w = seedD0(G, k);
+                if (g.cls == CLS_SERIAL && k + 1 < ns) fillSerial(rectGap(G, k, sgw[k], sgw[k + 1]));
+            }
+            if (g.cls == CLS_SERIAL) fillSerial(rectR3(G));
+        }
+        /* cells the reference's calculateScores touches */
+        int64_t cells = rectR5(G).area() + rectR3(G).area();
+        for (int k = 0; k < ns; ++k) {
+            const int4 s = G.sg[k];
+            cells += s.z;
+            if (s.x + s.z <= L && s.y + s.z <= g.tLen) cells += 1;
+            if (k + 1 < ns) cells += rectGap(G, k, s, G.sg[k + 1]).area();
+        }
+        const int T = 2 * ns + 1;
+        const int64_t pathCap = (int64_t)L + g.tLen + 3;
         /* ---- backtrace (src/Alignment.cpp:173-276) ---- */
         uint8_t* path = a.path + a.pathOff[job];
         int64_t np = 0;
-        int64_t alnTo, alnEnd, i, j;
+        int32_t alnTo, alnEnd, i, j;
         int32_t best;
         int32_t s = OP_M;
-        const int32_t go = J.go, ge = J.ge;
-        if (J.mode == 0) {
-            best = NEG; alnEnd = 0; alnTo = J.L;
-            for (int64_t c = 1; c <= J.tLen; ++c) {
-                int32_t m = vM(J, alnTo, c, T);
+        const int32_t go = G.go, ge = G.ge;
+        if (G.mode == 0) {
+            /* scan of the last row, M before I, strict '>' */
+            best = NEG; alnEnd = 0; alnTo = L;
+            const Rc R3 = rectR3(G);
+            const int32_t cEnd = R3.ok ? R3.c0 - 1 : g.tLen;
+            const int kL = rowSeg(G, L);
+            for (int32_t c = 1; c <= cEnd; ++c) {
+                const int32_t m = vMk(G, L, c, T, kL);
                 if (m > best) { best = m; alnEnd = c; s = OP_M; }
-                int32_t x = vI(J, alnTo, c, T);
+                const int32_t x = vIk(G, L, c, T, kL);
                 if (x > best) { best = x; alnEnd = c; s = OP_I; }
             }
-            i = alnTo; j = alnEnd;
-            while (i > 0 && np < pathCap) {
-                path[np++] = (uint8_t)s;
-                if (s == OP_M) { s = matchMax(vM(J, i - 1, j - 1, T), vI(J, i - 1, j - 1, T), vD(J, i - 1, j - 1, T)); i--; j--; }
-                else if (s == OP_I) { s = subn(subn(vM(J, i - 1, j, T), go), ge) > subn(vI(J, i - 1, j, T), ge) ? OP_M : OP_I; i--; }
-                else { s = subn(subn(vM(J, i, j - 1, T), go), ge) > subn(vD(J, i, j - 1, T), ge) ? OP_M : OP_D; j--; }
-                if (J.hazard && j < -1) break;
-            }
+            if (R3.ok && S.nwV > best) { best = S.nwV; alnEnd = S.nwC; s = S.nwS; }
         } else {
             /* M.maxCoeff: first maximum in column-major order */
-            best = 0; int64_t bi = 0, bj = 0;
-            auto visit = [&](int64_t ci, int64_t cj) {
-                int32_t v = vM(J, ci, cj, T);
-                if (v > best || (v == best && (cj < bj || (cj == bj && ci < bi)))) { best = v; bi = ci; bj = cj; }
+            best = 0; int32_t bi = 0, bj = 0;
+            auto visit = [&](int32_t v, int32_t ci, int32_t cj) {
+                if (swBetter(v, ci, cj, best, bi, bj)) { best = v; bi = ci; bj = cj; }
             };
-            auto visitRect = [&](const Rect& R) {
-                if (R.empty()) return;
-                for (int64_t cj = R.c0; cj <= R.c1; ++cj)
-                    for (int64_t ci = R.r0; ci <= R.r1; ++ci) visit(ci, cj);
-            };
-            visitRect(J.r5);
-            for (int k = 0; k < J.ns; ++k) {
-                const SeedRec& sd = J.S(k);
-                const int64_t b0 = sd.from + 1, b1 = sd.start - J.tStart + 1, len = sd.to - sd.from;
-                for (int64_t d = 0; d < len; ++d) visit(b0 + d, b1 + d);
-                if (b0 + len <= J.L && b1 + len <= J.tLen) visit(b0 + len, b1 + len);
-                if (k + 1 < J.ns) visitRect(J.gap(k));
+            if (S.swV != NEG) visit(S.swV, S.swI, S.swJ);
+            for (int k = 0; k < ns; ++k) {
+                const int4 sd = G.sg[k];
+                const int32_t t = G.match > 0 ? sd.z - 1 : 0;   /* the diagonal is monotone in its index */
+                visit(sd.w + t * G.match, sd.x + t, sd.y + t);
+                if (sd.x + sd.z <= L && sd.y + sd.z <= g.tLen) visit(vM(G, sd.x + sd.z, sd.y + sd.z, T), sd.x + sd.z, sd.y + sd.z);
             }
-            visitRect(J.r3);
             alnTo = bi; alnEnd = bj;
-            i = alnTo; j = alnEnd;
-            while ((s != OP_M || vM(J, i, j, T) > 0) && np < pathCap) {
-                path[np++] = (uint8_t)s;
-                if (s == OP_M) { s = matchMax(vM(J, i - 1, j - 1, T), vI(J, i - 1, j - 1, T), vD(J, i - 1, j - 1, T)); i--; j--; }
-                else if (s == OP_I) { s = subn(subn(vM(J, i - 1, j, T), go), ge) > subn(vI(J, i - 1, j, T), ge) ? OP_M : OP_I; i--; }
-                else { s = subn(subn(vM(J, i, j - 1, T), go), ge) > subn(vD(J, i, j - 1, T), ge) ? OP_M : OP_D; j--; }
-                if (J.hazard && (i < -1 || j < -1)) break;
+        }
+        i = alnTo; j = alnEnd;
+        while (true) {
+            const bool go_on = G.mode == 0 ? i > 0 : (s != OP_M || vM(G, i, j, T) > 0);
+            if (!go_on || np >= pathCap) break;
+            if (s == OP_M && G.match > 0 && i >= 0) {
+                /* inside a seed diagonal the step is M with -inf I/D neighbours: cross to its second cell */
+                const int k = rowSeg(G, i);
+                if (k >= 0) {
+                    const int4 sd = G.sg[k];
+                    const int32_t t = i - sd.x;
+                    if (i < sd.x + sd.z && j - sd.y == t && t >= 2) {
+                        const int64_t jmp = min((int64_t)(t - 1), pathCap - np);
+                        for (int64_t x = 0; x < jmp; ++x) path[np + x] = (uint8_t)OP_M;
+                        np += jmp; i -= (int32_t)jmp; j -= (int32_t)jmp;
+                        continue;
+                    }
+                }
             }
+            path[np++] = (uint8_t)s;
+            if (s == OP_M) { s = matchMax(vM(G, i - 1, j - 1, T), vI(G, i - 1, j - 1, T), vD(G, i - 1, j - 1, T)); i--; j--; }
+            else if (s == OP_I) { s = subn(subn(vM(G, i - 1, j, T), go), ge) > subn(vI(G, i - 1, j, T), ge) ? OP_M : OP_I; i--; }
+            else { s = subn(subn(vM(G, i, j - 1, T), go), ge) > subn(vD(G, i, j - 1, T), ge) ? OP_M : OP_D; j--; }
+            if (G.mode == 0) { if (G.hazard && j < -1) break; }
+            else if (G.hazard && (i < -1 || j < -1)) break;
         }
         /* path was collected end -> start; reverse it */
         for (int64_t x = 0, y = np - 1; x < y; ++x, --y) { uint8_t t = path[x]; path[x] = path[y]; path[y] = t; }
-        const int64_t alnFrom = i, alnStart = J.tStart + j;
-        rec.tid = J.S(0).tid; rec.strand = J.S(0).strand; rec.hazard = J.hazard;
-        rec.tStart = J.tStart; rec.tEnd = J.tEnd;
-        rec.alnFrom = alnFrom; rec.alnTo = alnTo; rec.alnStart = alnStart; rec.alnEnd = alnEnd + J.tStart;
+        const int64_t alnFrom = i, alnStart = g.tStart + j;
+        AlnRec rec;
+        rec.rid = g.rid;
+        rec.tid = g.tid; rec.strand = g.strand; rec.hazard = G.hazard;
+        rec.tStart = g.tStart; rec.tEnd = g.tEnd;
+        rec.alnFrom = alnFrom; rec.alnTo = alnTo; rec.alnStart = alnStart; rec.alnEnd = alnEnd + g.tStart;
         rec.score = best;
         rec.cells = cells;
         /* ---- getAlnCigar (src/Alignment.cpp:278-303) ---- */
         uint32_t* cg = a.cigar + a.pathOff[job] + 2 * job;
         int32_t nc = 0;
-        const int32_t clip5 = (int32_t)(alnFrom - 0), clip3 = (int32_t)(J.L - alnTo);
+        const int32_t clip5 = (int32_t)(alnFrom - 0), clip3 = (int32_t)(L - alnTo);
         if (clip5 > 0) cg[nc++] = ((uint32_t)clip5 << 4) | OP_S;
         if (np > 0) {
             uint32_t op = path[0], len = 1;
@@ -357,9 +580,9 @@ __global__ void __launch_bounds__(256) dp_kernel(DevIndex ix, DpArgs a) {
         for (int64_t c = 0; c < alnFrom; ++c) x += -a.clip;
         int64_t qi = alnFrom, tj = alnStart;
         for (int64_t k = 0; k < np; ++k) {
-            const uint8_t qb = (qi >= 0 && qi < J.L) ? J.q[qi] : (uint8_t)0;
-            const uint8_t tbv = J.tb(tj);
-            const int qv = (qi >= 0 && qi < J.L) ? J.qual[qi] : 0;
+            const uint8_t qb = (qi >= 0 && qi < L) ? q[qi] : (uint8_t)0;
+            const uint8_t tbv = tb(tj);
+            const int qv = (qi >= 0 && qi < L) ? qual[qi] : 0;
             uint32_t op = path[k];
             if (op == OP_M) {
                 x += (qb & tbv) ? a.qMatch[qv] : a.qMis[qv];
@@ -381,7 +604,7 @@ __global__ void __launch_bounds__(256) dp_kernel(DevIndex ix, DpArgs a) {
             prev = op;
         }
         ml += putNum(md, ml, run);
-        for (int64_t c = alnTo; c < J.L; ++c) x += -a.clip;
+        for (int64_t c = alnTo; c < L; ++c) x += -a.clip;
         rec.mdLen = (int32_t)ml;
         rec.log10P = x;
         rec.pad = (int32_t)np;

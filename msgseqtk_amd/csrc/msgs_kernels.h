@@ -99,6 +99,21 @@ struct AlnRec {
     int64_t cells;
 };
 
+/* per-job geometry in matrix coordinates (row i = query base i-1 + 1, column j =
+ * target base tStart + j - 1); R5 = rows 1..r5r1 x cols 1..r5c1, R3 = rows r3r0..L x
+ * cols r3c0..tLen (empty when a side is empty) */
+struct JobGeo {
+    int64_t tStart, tEnd, r3off;
+    int32_t L, tLen, ns, cs;        /* cs: first chain-seed slot of the job */
+    int32_t tid, strand, rid, cls;  /* cls: 0 no rectangle, 1 wavefront fill, 2 serial fill */
+    int32_t r5r1, r5c1, r3r0, r3c0;
+};
+/* rectangle summaries left by the fill: the column-major first maximum of M over all
+ * rectangle cells (LOCAL), and the in-order scan of M/I over R3's last row (GLOBAL) */
+struct JobSum {
+    int32_t swV, swI, swJ, nwV, nwC, nwS, pad0, pad1;
+};
+
 struct DpArgs {
     const uint8_t* seqs;       /* reads */
     const uint8_t* quals;
@@ -112,12 +127,15 @@ struct DpArgs {
     const int32_t* jobChainStart;
     const int32_t* jobChainLen;
     const int32_t* chainSeeds;
-    int32_t* auxD0;            /* per chain seed: first diagonal value */
-    int64_t* auxGap;           /* per chain seed: tile offset of the following gap rect */
+    int4* auxSG;               /* per chain seed: bi, bj, len, d0 (first diagonal value) */
+    int64_t* auxGap;           /* per chain seed: tile offset of the following gap rect (job-relative) */
+    JobGeo* geo;
+    JobSum* sum;
+    int32_t* jobCls;
+    const int64_t* waveJobs;   /* jobs filled by the wavefront kernel, largest first */
+    int64_t nWaveJobs;
     const int64_t* tileOff;    /* per job offset into the tile pool (cells) */
-    int32_t* tileM;
-    int32_t* tileI;
-    int32_t* tileD;
+    int4* tile;                /* M, I, D per rectangle cell */
     const int64_t* pathOff;    /* per job offset into path pool; cigar at +2*job, md at 3x+8*job */
     uint8_t* path;
     uint32_t* cigar;
@@ -130,10 +148,14 @@ struct DpArgs {
     const double* qMis;        /* q / PHRED_SCALE by q */
     const double* qGapOpen;    /* q / PHRED_SCALE - gapO by q */
     double gapE_d;
-    int32_t sizeOnly;          /* 1: compute tStart/tEnd and pool sizes only */
     int64_t* tileNeed;
     int64_t* pathNeed;
 };
-__global__ void dp_kernel(DevIndex ix, DpArgs a);
+/* seeds, rectangles and sizes of each job */
+__global__ void dp_setup_kernel(DevIndex ix, DpArgs a);
+/* rectangle fills, one wavefront per listed job */
+__global__ void dp_fill_kernel(DevIndex ix, DpArgs a);
+/* remaining fills, backtrace, CIGAR, MD and evaluate, one lane per job */
+__global__ void dp_trace_kernel(DevIndex ix, DpArgs a);
 
 #endif

@@ -51,12 +51,12 @@ __device__ __forceinline__ short toBf16(float f) {
 __device__ __forceinline__ float fromBf16(short h) { return __uint_as_float(((uint32_t)(uint16_t)h) << 16); }
 
 __global__ void k_ct(const uint8_t* bd, const uint32_t* sa, int64_t n, uint64_t* ct) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int64_t p = sa[i];
-    uint64_t k = 0;
-    for (int t = 0; t < 16; ++t) k = (k << 3) | (p + t < n ? sym3(bd[p + t]) : 0);
-    ct[i] = k;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = sa[i];
+        uint64_t k = 0;
+        for (int t = 0; t < 16; ++t) k = (k << 3) | (p + t < n ? sym3(bd[p + t]) : 0);
+        ct[i] = k;
+    }
 }
 
 __device__ __forceinline__ int64_t lbFull(const uint64_t* ct, int64_t n, uint64_t x) {
@@ -69,12 +69,12 @@ __device__ __forceinline__ int64_t lbFull(const uint64_t* ct, int64_t n, uint64_
 }
 
 __global__ void k_knots(const uint64_t* ct, int64_t n, int32_t S, uint64_t* kKey, int64_t* kRank) {
-    int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j > S) return;
-    if (j == S) { kKey[S] = ~0ull; kRank[S] = n; return; }
-    const uint64_t x = ct[(int64_t)j * n / S];
-    kKey[j] = x;
-    kRank[j] = lbFull(ct, n, x);
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j <= S; j += (int64_t)gridDim.x * blockDim.x) {
+        if (j == S) { kKey[S] = ~0ull; kRank[S] = n; continue; }
+        const uint64_t x = ct[j * n / S];
+        kKey[j] = x;
+        kRank[j] = lbFull(ct, n, x);
+    }
 }
 
 /* segment of x: largest j with kKey[j] <= x (0 when x is below every knot) */
@@ -373,7 +373,8 @@ __device__ __forceinline__ uint64_t readKey(const uint8_t* s) {
 }
 
 /* need[r] = 1 when some 16-mer of read r occurs in the index (windows tried from the
- * middle outwards), else 0: such a read cannot have an SMEM of length >= 16. */
+ * middle outwards) or some window holds a base other than A/C/G/T, else 0: such a read
+ * cannot have an SMEM of length >= 16. */
 __global__ void __launch_bounds__(256) k_filter(LixView m, int64_t E, const uint8_t* seqs, const int64_t* offs,
                                                 int64_t nReads, uint8_t* need, unsigned long long* stats) {
     __shared__ short W1b[16 * 64];
@@ -396,7 +397,12 @@ __global__ void __launch_bounds__(256) k_filter(LixView m, int64_t E, const uint
             while (!found && t < nw && x == ~0ull) {
                 const int64_t w = (t & 1) ? mid - (t + 1) / 2 : mid + t / 2;
                 ++t;
-                if (w >= 0 && w < nw) x = readKey(seqs + o + w);
+                if (w >= 0 && w < nw) {
+                    x = readKey(seqs + o + w);
+                    /* a window with another base (N, IUPAC, gap) keeps the read: SMEMs may
+                     * extend through such bases */
+                    if (x == ~0ull) found = true;
+                }
             }
             const bool active = !found && x != ~0ull;
             if (!__any(active)) break;

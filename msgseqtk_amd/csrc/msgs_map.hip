@@ -206,8 +206,11 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     a.seedOff = sd.dSeedOff; a.seeds = sd.dSeeds; a.nJobs = nJobs;
     a.jobRead = ch.dJobRead; a.jobChainStart = ch.dChainStart; a.jobChainLen = ch.dChainLen;
     a.chainSeeds = ch.dChainSeeds;
-    a.auxD0 = c->bufs.get<int32_t>("dp.d0", ch.nChainSeeds + 1, e);
+    a.auxSG = c->bufs.get<int4>("dp.sg", ch.nChainSeeds + 1, e);
     a.auxGap = c->bufs.get<int64_t>("dp.gap", ch.nChainSeeds + 1, e);
+    a.geo = c->bufs.get<JobGeo>("dp.geo", nJobs + 1, e);
+    a.sum = c->bufs.get<JobSum>("dp.sum", nJobs + 1, e);
+    a.jobCls = c->bufs.get<int32_t>("dp.cls", nJobs + 1, e);
     a.mode = p->mode;
     a.match = (int32_t)p->match; a.mismatch = (int32_t)p->mismatch; a.gapO = (int32_t)p->gapO; a.gapE = (int32_t)p->gapE;
     a.clip = p->clip; a.gapE_d = p->gapE;
@@ -226,21 +229,26 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     CK(hipMemcpyAsync(dq, hq, sizeof(hq), hipMemcpyHostToDevice, st), "qtab");
     a.qMatch = dq; a.qMis = dq + 256; a.qGapOpen = dq + 512;
     if (n) k_revcom<<<gridFor(n), 256, 0, st>>>(r.seq, dQual, r.offs, n, rcS, rcQ);
-    a.sizeOnly = 1;
-    dp_kernel<<<gridFor(nJobs), 256, 0, st>>>(c->db.view(), a);
+    dp_setup_kernel<<<gridFor(nJobs), 256, 0, st>>>(c->db.view(), a);
     std::vector<int64_t> tneed(nJobs), pneed(nJobs);
+    std::vector<int32_t> cls(nJobs);
     CK(hipMemcpyAsync(tneed.data(), a.tileNeed, nJobs * 8, hipMemcpyDeviceToHost, st), "tneed");
     CK(hipMemcpyAsync(pneed.data(), a.pathNeed, nJobs * 8, hipMemcpyDeviceToHost, st), "pneed");
-    CK(hipStreamSynchronize(st), "dp sizing");
+    CK(hipMemcpyAsync(cls.data(), a.jobCls, nJobs * 4, hipMemcpyDeviceToHost, st), "cls");
+    CK(hipStreamSynchronize(st), "dp setup");
     std::vector<int64_t> tileOff;
     const int64_t tiles = exclusiveScan(tneed, tileOff);
     std::vector<int64_t> pathOff;
     const int64_t paths = exclusiveScan(pneed, pathOff);
+    /* wavefront jobs, largest first so that the long ones start early */
+    std::vector<int64_t> wave;
+    for (int64_t j = 0; j < nJobs; ++j)
+        if (cls[j] == 1) wave.push_back(j);
+    std::sort(wave.begin(), wave.end(), [&](int64_t x, int64_t y) { return tneed[x] > tneed[y] || (tneed[x] == tneed[y] && x < y); });
     int64_t* dTileOff = c->bufs.get<int64_t>("dp.toff", nJobs + 1, e);
     int64_t* dPathOff = c->bufs.get<int64_t>("dp.poff", nJobs + 1, e);
-    a.tileM = c->bufs.get<int32_t>("dp.tm", tiles + 1, e);
-    a.tileI = c->bufs.get<int32_t>("dp.ti", tiles + 1, e);
-    a.tileD = c->bufs.get<int32_t>("dp.td", tiles + 1, e);
+    int64_t* dWave = c->bufs.get<int64_t>("dp.wave", wave.size() + 1, e);
+    a.tile = c->bufs.get<int4>("dp.tile", tiles + 1, e);
     a.path = c->bufs.get<uint8_t>("dp.path", paths + 1, e);
     a.cigar = c->bufs.get<uint32_t>("dp.cig", paths + 2 * nJobs + 1, e);
     a.md = c->bufs.get<char>("dp.md", 3 * paths + 8 * nJobs + 1, e);
@@ -248,10 +256,15 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "dp pools");
     CK(hipMemcpyAsync(dTileOff, tileOff.data(), (nJobs + 1) * 8, hipMemcpyHostToDevice, st), "toff");
     CK(hipMemcpyAsync(dPathOff, pathOff.data(), (nJobs + 1) * 8, hipMemcpyHostToDevice, st), "poff");
+    if (!wave.empty()) CK(hipMemcpyAsync(dWave, wave.data(), wave.size() * 8, hipMemcpyHostToDevice, st), "wave");
     a.tileOff = dTileOff; a.pathOff = dPathOff;
-    a.sizeOnly = 0;
+    a.waveJobs = dWave; a.nWaveJobs = (int64_t)wave.size();
     CK(hipEventRecord(c->ev2, st), "event");
-    dp_kernel<<<gridFor(nJobs), 256, 0, st>>>(c->db.view(), a);
+    if (!wave.empty()) {
+        const int64_t blocks = std::min<int64_t>((int64_t)wave.size(), (int64_t)std::max(1, c->nCU) * 16);
+        dp_fill_kernel<<<(unsigned)blocks, 64, 0, st>>>(c->db.view(), a);
+    }
+    dp_trace_kernel<<<gridFor(nJobs), 256, 0, st>>>(c->db.view(), a);
     CK(hipEventRecord(c->ev3, st), "event");
     out.recs.resize(nJobs);
     CK(hipMemcpyAsync(out.recs.data(), a.out, nJobs * sizeof(AlnRec), hipMemcpyDeviceToHost, st), "recs");
