@@ -85,7 +85,9 @@ struct msgs_ctx {
     double stats[16] = {};
     double lastSmemMicros = 0;
     int hostThreads = 16;
-    std::string lastSam;
+    /* SAM text of the last msgs_batch_run: per-thread parts in read order and the
+     * offset of each read's records in their concatenation */
+    std::vector<std::string> lastParts;
     std::vector<int64_t> lastLineOffs;
 };
 

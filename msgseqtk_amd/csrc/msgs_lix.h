@@ -17,6 +17,8 @@ struct LixView {
     const int64_t* kRank;   /* S+1 knot lower bounds (last = n) */
     int32_t S;
     const float* W;         /* MLP parameters */
+    const uint32_t* eSeg;       /* per segment: max |prediction - lb| of its keys, MLP */
+    const uint32_t* eSegRoot;   /* the same for the root alone */
 };
 
 struct LixState {
@@ -24,6 +26,8 @@ struct LixState {
     uint64_t* dKey = nullptr;
     int64_t* dRank = nullptr;
     float* dW = nullptr;
+    uint32_t* dESeg = nullptr;
+    uint32_t* dESegRoot = nullptr;
     int64_t n = 0;
     int32_t S = 0;
     int64_t E = 0, Eplr = 0;   /* recorded max errors: MLP, root alone */

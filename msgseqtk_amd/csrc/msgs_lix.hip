@@ -14,8 +14,9 @@
  * reductions, Adam); inference runs the first layer as bf16 MFMA
  * (v_mfma_f32_16x16x16 bf16) with fp32 accumulation.
  *
- * Exactness.  E is the largest |prediction - lb| over every distinct key present in
- * CT, measured with the same inference code.  A query searches [pred-E, pred+E] and
+ * Exactness.  E_j is the largest |prediction - lb| over the distinct keys present in
+ * CT that fall in segment j, measured with the same inference code (the recorded max
+ * error; E = max_j E_j is reported).  A query in segment j searches [pred-E_j, pred+E_j] and
  * checks CT[lo-1] < x <= CT[lo]; when the check fails (keys absent from the index can
  * fall outside the bound) it falls back to a binary search over all of CT.  Results
  * are therefore exact lower bounds; tests judge them against FMD backward search.
@@ -188,13 +189,17 @@ __device__ __forceinline__ void loadW1(const LixView& m, short* W1b) {
     __syncthreads();
 }
 
-/* prediction of lb(x) (rounded) with the MFMA path; useMlp = 0 gives the root alone */
-__device__ __forceinline__ int64_t predict(const LixView& m, uint64_t x, bool active, const short* W1b, bool useMlp) {
+/* prediction of lb(x) (rounded) with the MFMA path; useMlp = 0 gives the root alone;
+ * seg receives the key's segment */
+__device__ __forceinline__ int64_t predict(const LixView& m, uint64_t x, bool active, const short* W1b, bool useMlp,
+                                           int32_t* seg) {
     double base = 0, width = 1;
     float f[16];
     for (int k = 0; k < 16; ++k) f[k] = 0.f;
+    *seg = 0;
     if (active) {
         const int32_t j = segOf(m, x);
+        *seg = j;
         Plr p = plrOf(m, x, j);
         base = p.base; width = p.width;
         features(x, p.t, f);
@@ -319,8 +324,9 @@ __global__ void k_adam(float* W, const float* G, float* Mo, float* Vo, int32_t P
     W[i] -= lr * mh / (sqrtf(vh) + 1e-8f);
 }
 
-/* max |prediction - lb| over distinct keys: [0] MLP (MFMA path), [1] root alone */
-__global__ void __launch_bounds__(256) k_maxerr(LixView m, unsigned long long* err) {
+/* max |prediction - lb| over distinct keys, per segment and overall: [0] MLP (MFMA
+ * path), [1] root alone */
+__global__ void __launch_bounds__(256) k_maxerr(LixView m, uint32_t* eSeg, uint32_t* eSegRoot, unsigned long long* err) {
     __shared__ short W1b[16 * 64];
     loadW1(m, W1b);
     unsigned long long e0 = 0, e1 = 0;
@@ -330,18 +336,25 @@ __global__ void __launch_bounds__(256) k_maxerr(LixView m, unsigned long long* e
         const int64_t i = r * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
         const bool active = i < m.n && (i == 0 || m.ct[i] != m.ct[i - 1]);
         const uint64_t x = active ? m.ct[i] : 0;
-        const int64_t p = predict(m, x, active, W1b, true);
+        int32_t j = 0, j0 = 0;
+        const int64_t p = predict(m, x, active, W1b, true, &j);
         if (active) {
-            const int64_t p0 = predict(m, x, true, W1b, false);
-            e0 = max(e0, (unsigned long long)(p > i ? p - i : i - p));
-            e1 = max(e1, (unsigned long long)(p0 > i ? p0 - i : i - p0));
+            const int64_t p0 = predict(m, x, true, W1b, false, &j0);
+            const unsigned long long a = (unsigned long long)(p > i ? p - i : i - p);
+            const unsigned long long b = (unsigned long long)(p0 > i ? p0 - i : i - p0);
+            e0 = max(e0, a);
+            e1 = max(e1, b);
+            atomicMax(&eSeg[j], (uint32_t)min(a, 0xffffffffull));
+            atomicMax(&eSegRoot[j0], (uint32_t)min(b, 0xffffffffull));
         }
     }
     atomicMax(&err[0], e0);
     atomicMax(&err[1], e1);
 }
 
-__global__ void __launch_bounds__(256) k_query(LixView m, const uint64_t* keys, int64_t nq, int32_t useMlp, int64_t E,
+/* useMlp: 1 model + per-segment bound, 0 root + its per-segment bound, -1 search of the
+ * whole table */
+__global__ void __launch_bounds__(256) k_query(LixView m, const uint64_t* keys, int64_t nq, int32_t useMlp,
                                                int64_t* lb, int32_t* probes, int32_t* fell) {
     __shared__ short W1b[16 * 64];
     loadW1(m, W1b);
@@ -351,9 +364,11 @@ __global__ void __launch_bounds__(256) k_query(LixView m, const uint64_t* keys, 
         const int64_t i = r * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
         const bool active = i < nq;
         const uint64_t x = active ? keys[i] : 0;
-        const int64_t p = predict(m, x, active, W1b, useMlp != 0);
+        int32_t j = 0;
+        const int64_t p = predict(m, x, active, W1b, useMlp > 0, &j);
         if (active) {
             int32_t pr = 0, fb = 0;
+            const int64_t E = useMlp > 0 ? (int64_t)m.eSeg[j] : (useMlp == 0 ? (int64_t)m.eSegRoot[j] : m.n);
             lb[i] = lbBounded(m, x, p, E, &pr, &fb);
             if (probes) probes[i] = pr;
             if (fell) fell[i] = fb;
@@ -375,7 +390,7 @@ __device__ __forceinline__ uint64_t readKey(const uint8_t* s) {
 /* need[r] = 1 when some 16-mer of read r occurs in the index (windows tried from the
  * middle outwards) or some window holds a base other than A/C/G/T, else 0: such a read
  * cannot have an SMEM of length >= 16. */
-__global__ void __launch_bounds__(256) k_filter(LixView m, int64_t E, const uint8_t* seqs, const int64_t* offs,
+__global__ void __launch_bounds__(256) k_filter(LixView m, const uint8_t* seqs, const int64_t* offs,
                                                 int64_t nReads, uint8_t* need, unsigned long long* stats) {
     __shared__ short W1b[16 * 64];
     loadW1(m, W1b);
@@ -406,10 +421,11 @@ __global__ void __launch_bounds__(256) k_filter(LixView m, int64_t E, const uint
             }
             const bool active = !found && x != ~0ull;
             if (!__any(active)) break;
-            const int64_t p = predict(m, x, active, W1b, true);
+            int32_t j = 0;
+            const int64_t p = predict(m, x, active, W1b, true, &j);
             if (active) {
                 int32_t pr = 0, fb = 0;
-                const int64_t lo = lbBounded(m, x, p, E, &pr, &fb);
+                const int64_t lo = lbBounded(m, x, p, (int64_t)m.eSeg[j], &pr, &fb);
                 found = lo < m.n && m.ct[lo] == x;
                 nq++;
             }
@@ -420,7 +436,8 @@ __global__ void __launch_bounds__(256) k_filter(LixView m, int64_t E, const uint
 }
 
 /* raw model outputs (before rounding) for checks: mode 0 bf16 MFMA, mode 1 the same
- * arithmetic on VALU (bf16-rounded features and first-layer weights, fp32 sums) */
+ * arithmetic on VALU (bf16-rounded features and first-layer weights, fp32 sums), mode 2
+ * the root alone */
 __global__ void __launch_bounds__(256) k_predict(LixView m, const uint64_t* keys, int64_t nq, int32_t mode,
                                                  double* out) {
     __shared__ short W1b[16 * 64];
@@ -442,6 +459,8 @@ __global__ void __launch_bounds__(256) k_predict(LixView m, const uint64_t* keys
         float y;
         if (mode == 0) {
             y = mlpMfma(m, f, W1b);
+        } else if (mode == 2) {
+            y = 0.f;
         } else {
             const float* b1 = m.W + 16 * 64;
             const float* W2 = b1 + 64;
@@ -475,14 +494,14 @@ inline unsigned gridFor(int64_t n, int t = 256) {
 
 LixView LixState::view() const {
     LixView v;
-    v.ct = dCt; v.n = n; v.kKey = dKey; v.kRank = dRank; v.S = S; v.W = dW;
+    v.ct = dCt; v.n = n; v.kKey = dKey; v.kRank = dRank; v.S = S; v.W = dW; v.eSeg = dESeg; v.eSegRoot = dESegRoot;
     return v;
 }
 
 void LixState::release() {
-    void* p[] = {dCt, dKey, dRank, dW};
+    void* p[] = {dCt, dKey, dRank, dW, dESeg, dESegRoot};
     for (void* x : p) if (x) (void)hipFree(x);
-    dCt = nullptr; dKey = nullptr; dRank = nullptr; dW = nullptr;
+    dCt = nullptr; dKey = nullptr; dRank = nullptr; dW = nullptr; dESeg = nullptr; dESegRoot = nullptr;
     trained = false;
 }
 
@@ -513,6 +532,8 @@ extern "C" int msgs_lix_train(msgs_ctx* c, int32_t segments, int64_t samples, in
     LCK(hipMalloc(&L.dKey, (segments + 1) * sizeof(uint64_t)), "knots");
     LCK(hipMalloc(&L.dRank, (segments + 1) * sizeof(int64_t)), "knots");
     LCK(hipMalloc(&L.dW, kLixParams * sizeof(float)), "weights");
+    LCK(hipMalloc(&L.dESeg, segments * sizeof(uint32_t)), "bounds");
+    LCK(hipMalloc(&L.dESegRoot, segments * sizeof(uint32_t)), "bounds");
     hipEvent_t e0 = c->ev0, e1 = c->ev1;
     LCK(hipEventRecord(e0, st), "event");
     k_ct<<<gridFor(n), 256, 0, st>>>(c->db.dBd, c->db.dSa, n, L.dCt);
@@ -555,7 +576,9 @@ extern "C" int msgs_lix_train(msgs_ctx* c, int32_t segments, int64_t samples, in
     }
     LCK(hipEventRecord(e1, st), "event");
     LCK(hipMemsetAsync(dErr, 0, 16, st), "err");
-    k_maxerr<<<gridFor(std::min<int64_t>(n, (int64_t)c->nCU * 2048)), 256, 0, st>>>(L.view(), dErr);
+    LCK(hipMemsetAsync(L.dESeg, 0, segments * sizeof(uint32_t), st), "bounds");
+    LCK(hipMemsetAsync(L.dESegRoot, 0, segments * sizeof(uint32_t), st), "bounds");
+    k_maxerr<<<gridFor(std::min<int64_t>(n, (int64_t)c->nCU * 2048)), 256, 0, st>>>(L.view(), L.dESeg, L.dESegRoot, dErr);
     unsigned long long he[2] = {0, 0};
     LCK(hipMemcpyAsync(he, dErr, 16, hipMemcpyDeviceToHost, st), "err");
     LCK(hipStreamSynchronize(st), "max error");
@@ -619,8 +642,7 @@ extern "C" int msgs_lix_query(msgs_ctx* c, const uint64_t* keys, int64_t nq, int
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "query buffers");
     LCK(hipMemcpyAsync(dK, keys, nq * 8, hipMemcpyHostToDevice, st), "keys");
     const LixState& L = c->lix;
-    int64_t E = useMlp > 0 ? L.E : (useMlp == 0 ? L.Eplr : L.n);
-    k_query<<<gridFor(nq), 256, 0, st>>>(L.view(), dK, nq, useMlp > 0 ? 1 : 0, E, dL, dP, dF);
+    k_query<<<gridFor(nq), 256, 0, st>>>(L.view(), dK, nq, useMlp > 0 ? 1 : (useMlp == 0 ? 0 : -1), dL, dP, dF);
     LCK(hipMemcpyAsync(lb, dL, nq * 8, hipMemcpyDeviceToHost, st), "lb");
     if (probes) LCK(hipMemcpyAsync(probes, dP, nq * 4, hipMemcpyDeviceToHost, st), "probes");
     if (fell) LCK(hipMemcpyAsync(fell, dF, nq * 4, hipMemcpyDeviceToHost, st), "fell");
@@ -632,7 +654,7 @@ int runLixFilter(msgs_ctx* c, const DevReads& r, uint8_t** needOut, unsigned lon
     hipError_t e = hipSuccess;
     uint8_t* need = c->bufs.get<uint8_t>("lx.need", r.n + 1, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "filter buffer");
-    if (r.n) k_filter<<<gridFor(r.n), 256, 0, c->stream>>>(c->lix.view(), c->lix.E, r.seq, r.offs, r.n, need, dStats);
+    if (r.n) k_filter<<<gridFor(r.n), 256, 0, c->stream>>>(c->lix.view(), r.seq, r.offs, r.n, need, dStats);
     LCK(hipGetLastError(), "filter");
     *needOut = need;
     return MSGS_OK;

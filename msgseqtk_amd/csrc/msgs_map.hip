@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <charconv>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -303,8 +304,23 @@ struct HostRead {
     const uint8_t* seq;
     const uint8_t* qual;
     int64_t L;
-    std::string name;
+    const char* name;
+    int64_t nameLen;
 };
+
+inline void putInt(std::string& s, int64_t v) {
+    char b[24];
+    const auto r = std::to_chars(b, b + sizeof b, v);
+    s.append(b, r.ptr - b);
+}
+
+/* "%g" of a float widened to double (std::to_chars with general format and precision 6
+ * is specified as printf's %.6g) */
+inline void putFloat(std::string& s, float f) {
+    char b[48];
+    const auto r = std::to_chars(b, b + sizeof b, (double)f, std::chars_format::general, 6);
+    s.append(b, r.ptr - b);
+}
 
 struct HAln {
     const AlnRec* r;
@@ -332,10 +348,11 @@ double packetSum(const std::vector<double>& v) {
     return s;
 }
 
-/* calcMapQ (src/Alignment.cpp:491-512 and :514-539) on log-likelihoods */
-void mapq(const std::vector<double>& ll, std::vector<double>& postP, std::vector<uint8_t>& mq, bool pe) {
+/* calcMapQ (src/Alignment.cpp:491-512 and :514-539) on log-likelihoods; pr is scratch */
+void mapq(const std::vector<double>& ll, std::vector<double>& pr, std::vector<double>& postP, std::vector<uint8_t>& mq,
+          bool pe) {
     const size_t n = ll.size();
-    std::vector<double> pr(ll);
+    pr.assign(ll.begin(), ll.end());
     double maxV = *std::max_element(pr.begin(), pr.end());
     const double inf = std::numeric_limits<double>::infinity();
     double scale = (maxV != -inf && maxV < -256) ? -256 - maxV : 0;
@@ -367,54 +384,55 @@ struct Finisher {
         for (int k = 0; k < r.nCigar; ++k) { s += std::to_string(cg[k] >> 4); s += kOps[cg[k] & 15]; }
         return s;
     }
-    std::string mdOf(int64_t job) const {
-        const AlnRec& r = dp.recs[job];
-        return std::string(dp.md.data() + dp.mdOff[job], r.mdLen);
-    }
-
-    static void fmtFloat(std::string& s, float f) {
-        char buf[48];
-        std::snprintf(buf, sizeof buf, "%g", (double)f);
-        s += buf;
-    }
 
     /* one SAM text record (field layout of htslib's sam_format1) */
     void line(std::string& s, const HostRead& rd, bool rc, int flag, int32_t tid, int64_t pos, int mapq,
-              const std::string& cigar, int32_t isize, const int64_t* tags, const std::string* md, float xh,
+              const uint32_t* cg, int nc, int32_t isize, const int64_t* tags, const char* md, int mdLen, float xh,
               float xp) const {
         static const char kNt16[17] = "=ACMGRSVTWYHKDBN";
+        static const char kNt16c[17] = "=TGKCYSBAWRDMHVN";   /* complement of each code */
+        static const char kOps[] = "MIDNSHP=XB";
         if (!emit) return;
-        s += rd.name;
-        s += '\t'; s += std::to_string(flag);
-        s += '\t'; s += tid >= 0 ? db.names[tid] : std::string("*");
-        s += '\t'; s += std::to_string(pos + 1);
-        s += '\t'; s += std::to_string(mapq);
-        s += '\t'; s += cigar.empty() ? std::string("*") : cigar;
-        s += "\t*\t0\t"; s += std::to_string(isize);
+        s.append(rd.name, rd.nameLen);
+        s += '\t'; putInt(s, flag);
+        s += '\t'; if (tid >= 0) s += db.names[tid]; else s += '*';
+        s += '\t'; putInt(s, pos + 1);
+        s += '\t'; putInt(s, mapq);
+        s += '\t';
+        if (nc == 0) s += '*';
+        for (int k = 0; k < nc; ++k) { putInt(s, cg[k] >> 4); s += kOps[cg[k] & 15]; }
+        s += "\t*\t0\t"; putInt(s, isize);
         s += '\t';
         if (rd.L == 0) s += '*';
-        for (int64_t i = 0; i < rd.L; ++i) {
-            uint8_t b = rc ? (uint8_t)(((rd.seq[rd.L - 1 - i] & 1) << 3) | ((rd.seq[rd.L - 1 - i] & 2) << 1) |
-                                       ((rd.seq[rd.L - 1 - i] & 4) >> 1) | ((rd.seq[rd.L - 1 - i] & 8) >> 3))
-                           : rd.seq[i];
-            s += kNt16[b & 15];
+        else {
+            const size_t o = s.size();
+            s.resize(o + rd.L);
+            char* d = &s[o];
+            if (rc) for (int64_t i = 0; i < rd.L; ++i) d[i] = kNt16c[rd.seq[rd.L - 1 - i] & 15];
+            else for (int64_t i = 0; i < rd.L; ++i) d[i] = kNt16[rd.seq[i] & 15];
         }
         s += '\t';
         if (rd.L == 0 || (rc ? rd.qual[rd.L - 1] : rd.qual[0]) == 0xff) s += '*';
-        else for (int64_t i = 0; i < rd.L; ++i) s += (char)((rc ? rd.qual[rd.L - 1 - i] : rd.qual[i]) + 33);
+        else {
+            const size_t o = s.size();
+            s.resize(o + rd.L);
+            char* d = &s[o];
+            if (rc) for (int64_t i = 0; i < rd.L; ++i) d[i] = (char)(rd.qual[rd.L - 1 - i] + 33);
+            else for (int64_t i = 0; i < rd.L; ++i) d[i] = (char)(rd.qual[i] + 33);
+        }
         if (tags) {
-            s += "\tAS:i:"; s += std::to_string(tags[0]);
-            s += "\tMD:Z:"; s += *md;
-            s += "\tXH:f:"; fmtFloat(s, xh);
-            s += "\tNH:i:"; s += std::to_string(tags[1]);
-            s += "\tXN:i:"; s += std::to_string(tags[2]);
-            s += "\tXP:f:"; fmtFloat(s, xp);
+            s += "\tAS:i:"; putInt(s, tags[0]);
+            s += "\tMD:Z:"; s.append(md, mdLen);
+            s += "\tXH:f:"; putFloat(s, xh);
+            s += "\tNH:i:"; putInt(s, tags[1]);
+            s += "\tXN:i:"; putInt(s, tags[2]);
+            s += "\tXP:f:"; putFloat(s, xp);
         }
         s += '\n';
     }
 
     void unmapped(std::string& s, const HostRead& rd, int flag) const {
-        line(s, rd, false, flag, -1, 0, 0, std::string(), 0, nullptr, nullptr, 0, 0);
+        line(s, rd, false, flag, -1, 0, 0, nullptr, 0, 0, nullptr, nullptr, 0, 0, 0);
     }
 
     /* exportBAM + output() (src/Alignment.cpp:429-444, src/msgseqtk-align.cpp:538-557) */
@@ -425,28 +443,32 @@ struct Finisher {
         const bool rev = r.strand == REV;
         int flag = (rev ? 16 : 0) | flagOr;
         int64_t tags[3] = {(int64_t)(int)std::round((double)r.score), nh, xn};
-        std::string md = mdOf(job);
-        line(s, rd, rev, flag, r.tid, r.alnStart, mapq, cigarOf(job), isize, tags, &md, (float)a.log10P,
-             (float)postP);
+        line(s, rd, rev, flag, r.tid, r.alnStart, mapq, dp.cigar.data() + dp.cigOff[job], r.nCigar, isize, tags,
+             dp.md.data() + dp.mdOff[job], r.mdLen, (float)a.log10P, (float)postP);
     }
 
+    struct Pr { const HAln* f; const HAln* r; int32_t isize; double l10; double ll; double postP; uint8_t mq; };
+    std::vector<Pr> prBuf;
+    /* per-thread scratch, reused across reads */
+    std::vector<HAln> fBuf, rBuf;
+    std::vector<double> ll, pr, pp;
+    std::vector<uint8_t> mq;
+
     /* filter (src/Alignment.cpp:548-555): drop alnScore < L * minScoreRate */
-    std::vector<HAln> alnList(const HostRead& rd, int64_t j0, int64_t j1) const {
-        std::vector<HAln> v;
+    void alnList(const HostRead& rd, int64_t j0, int64_t j1, std::vector<HAln>& v) const {
+        v.clear();
         const double thr = (double)rd.L * P.minScoreRate;
         for (int64_t j = j0; j < j1; ++j) {
             const AlnRec& r = dp.recs[j];
             double sc = r.score == NEG ? -std::numeric_limits<double>::infinity() : (double)r.score;
             if (!(sc < thr)) v.push_back(HAln{&r, r.log10P});
         }
-        return v;
     }
 
-    void calcMapQ(std::vector<HAln>& v) const {
-        std::vector<double> ll(v.size()), pp;
-        std::vector<uint8_t> mq;
+    void calcMapQ(std::vector<HAln>& v) {
+        ll.resize(v.size());
         for (size_t i = 0; i < v.size(); ++i) ll[i] = v[i].log10P / log10e;
-        mapq(ll, pp, mq, false);
+        mapq(ll, pr, pp, mq, false);
         for (size_t i = 0; i < v.size(); ++i) { v[i].postP = pp[i]; v[i].mapQ = mq[i]; }
     }
 
@@ -463,7 +485,8 @@ struct Finisher {
     /* main_SE task (src/msgseqtk-align.cpp:607-649) */
     void se(std::string& s, const HostRead& rd, int32_t nSeeds, int64_t j0, int64_t j1) {
         if (nSeeds == 0) { unmapped(s, rd, 4); return; }
-        std::vector<HAln> v = alnList(rd, j0, j1);
+        std::vector<HAln>& v = fBuf;
+        alnList(rd, j0, j1, v);
         if (v.empty()) { unmapped(s, rd, 4); return; }
         mapped++;
         calcMapQ(v);
@@ -489,10 +512,13 @@ struct Finisher {
     void pe(std::string& s, const HostRead& r1, const HostRead& r2, int32_t ns1, int32_t ns2, int64_t a0, int64_t a1,
             int64_t b0, int64_t b1) {
         if (ns1 == 0 && ns2 == 0) { unmapped(s, r1, 4 | 8 | 1 | 64); unmapped(s, r2, 4 | 8 | 1 | 128); return; }
-        std::vector<HAln> f = alnList(r1, a0, a1), r = alnList(r2, b0, b1);
+        std::vector<HAln>& f = fBuf;
+        std::vector<HAln>& r = rBuf;
+        alnList(r1, a0, a1, f);
+        alnList(r2, b0, b1, r);
         if (f.empty() && r.empty()) { unmapped(s, r1, 4 | 8 | 1 | 64); unmapped(s, r2, 4 | 8 | 1 | 128); return; }
-        struct Pr { const HAln* f; const HAln* r; int32_t isize; double l10; double ll; double postP; uint8_t mq; };
-        std::vector<Pr> pairs;
+        std::vector<Pr>& pairs = prBuf;
+        pairs.clear();
         for (const HAln& x : f)
             for (const HAln& y : r)
                 if ((x.r->strand & y.r->strand) == 0) {
@@ -503,7 +529,7 @@ struct Finisher {
                                        x.log10P / log10e + y.log10P / log10e + std::log(pr), 0, 0xff});
                 }
         msgs_sort::sort(pairs.data(), pairs.data() + pairs.size(), [](const Pr& l, const Pr& r) { return l.l10 > r.l10; });
-        auto rm = [&](std::function<bool(const Pr&)> g) { pairs.erase(std::remove_if(pairs.begin(), pairs.end(), g), pairs.end()); };
+        auto rm = [&](auto g) { pairs.erase(std::remove_if(pairs.begin(), pairs.end(), g), pairs.end()); };
         rm([&](const Pr& q) { return !(P.insMin <= q.isize && q.isize <= P.insMax); });
         if (P.noDiscordant) rm([](const Pr& q) { return !(q.f->r->tid == q.r->r->tid && (q.f->r->strand & q.r->r->strand) == 0); });
         if (P.noTailOver) rm([](const Pr& q) {
@@ -516,10 +542,9 @@ struct Finisher {
         if (P.maxNPair > 0 && (int64_t)pairs.size() > P.maxNPair) pairs.erase(pairs.begin() + P.maxNPair, pairs.end());
         if (!pairs.empty()) {
             mapped += 2;
-            std::vector<double> ll(pairs.size()), pp;
-            std::vector<uint8_t> mq;
+            ll.resize(pairs.size());
             for (size_t i = 0; i < pairs.size(); ++i) ll[i] = pairs[i].ll;
-            mapq(ll, pp, mq, true);
+            mapq(ll, pr, pp, mq, true);
             size_t num = P.maxReport == 0 ? pairs.size() : std::min((size_t)(uint32_t)P.maxReport, pairs.size());
             for (size_t i = 0; i < num; ++i) {
                 const Pr& q = pairs[i];
@@ -537,17 +562,18 @@ struct Finisher {
     }
 };
 
-std::vector<std::string> splitNames(const char* names, int64_t n) {
-    std::vector<std::string> v;
-    v.reserve(n);
+/* [start, end) of each of n newline-separated names */
+void splitNames(const char* names, int64_t n, std::vector<int64_t>& off) {
+    off.assign(2 * n, 0);
     const char* p = names ? names : "";
+    const char* b = p;
     for (int64_t i = 0; i < n; ++i) {
-        const char* e = p;
-        while (*e && *e != '\n') ++e;
-        v.emplace_back(p, e);
+        const char* e = std::strchr(p, '\n');
+        if (!e) e = p + std::strlen(p);
+        off[2 * i] = p - b;
+        off[2 * i + 1] = e - b;
         p = *e ? e + 1 : e;
     }
-    return v;
 }
 
 }  // namespace
@@ -666,36 +692,38 @@ extern "C" int msgs_align_batch(msgs_ctx* c, const msgs_reads* reads, const msgs
 }
 
 /* SAM text of a whole batch (SE when mates is NULL), finished on host threads in
- * read-order chunks; returns the number of mapped reads. */
+ * read-order chunks (part t holds the records of the t-th chunk); lineOffs[i] is where
+ * read i's records start in the parts' concatenation.  Returns the number of mapped reads. */
 static int64_t samText(msgs_ctx* c, const msgs_params* p, const msgs_reads* reads, const msgs_reads* mates,
-                       const BatchResult& b1, const BatchResult* b2, bool emit, std::string& out,
+                       const BatchResult& b1, const BatchResult* b2, bool emit, std::vector<std::string>& part,
                        std::vector<int64_t>& lineOffs) {
     const int64_t n = reads->n;
-    std::vector<std::string> nm1 = splitNames(reads->names, n);
-    std::vector<std::string> nm2 = mates ? splitNames(mates->names, n) : std::vector<std::string>();
+    std::vector<int64_t> nm1, nm2;
+    splitNames(reads->names, n, nm1);
+    if (mates) splitNames(mates->names, n, nm2);
     lineOffs.assign(n + 1, 0);
     const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(c->hostThreads, (n + 255) / 256));
-    std::vector<std::string> part(nt);
-    std::vector<std::vector<int64_t>> pOffs(nt);
+    part.assign(nt, std::string());
     std::vector<int64_t> mappedPart(nt, 0);
     auto work = [&](int t) {
         const int64_t i0 = n * t / nt, i1 = n * (t + 1) / nt;
         Finisher F{c->db, *p, b1.dp};
         F.emit = emit;
         std::vector<uint8_t> q30;
-        auto host = [&](const msgs_reads* r, const std::vector<std::string>& nm, int64_t i) {
+        auto host = [&](const msgs_reads* r, const std::vector<int64_t>& nm, int64_t i) {
             HostRead h;
             h.seq = r->seq + r->offs[i];
             h.L = r->offs[i + 1] - r->offs[i];
             if (r->qual) h.qual = r->qual + r->offs[i];
             else { if ((int64_t)q30.size() < h.L) q30.assign(h.L, 30); h.qual = q30.data(); }
-            h.name = nm[i].empty() ? std::string("r") : nm[i];
+            if (nm[2 * i + 1] > nm[2 * i]) { h.name = r->names + nm[2 * i]; h.nameLen = nm[2 * i + 1] - nm[2 * i]; }
+            else { h.name = "r"; h.nameLen = 1; }
             return h;
         };
         std::string& o = part[t];
-        pOffs[t].resize(i1 - i0);
+        if (emit) o.reserve((size_t)(i1 - i0) * (mates ? 2 : 1) * (size_t)(160 + 2 * (reads->offs[n] - reads->offs[0]) / std::max<int64_t>(1, n)));
         for (int64_t i = i0; i < i1; ++i) {
-            pOffs[t][i - i0] = (int64_t)o.size();
+            lineOffs[i] = (int64_t)o.size();
             HostRead h1 = host(reads, nm1, i);
             if (!mates) {
                 F.se(o, h1, b1.sd.seedCnt[i], b1.ch.chainOff[i], b1.ch.chainOff[i + 1]);
@@ -711,17 +739,20 @@ static int64_t samText(msgs_ctx* c, const msgs_params* p, const msgs_reads* read
     for (int t = 1; t < nt; ++t) th.emplace_back(work, t);
     work(0);
     for (auto& x : th) x.join();
-    int64_t total = 0, mapped = 0;
-    for (int t = 0; t < nt; ++t) { total += (int64_t)part[t].size(); mapped += mappedPart[t]; }
-    out.clear();
-    out.reserve(total);
+    int64_t base = 0, mapped = 0;
     for (int t = 0; t < nt; ++t) {
-        const int64_t i0 = n * t / nt;
-        for (size_t k = 0; k < pOffs[t].size(); ++k) lineOffs[i0 + k] = (int64_t)out.size() + pOffs[t][k];
-        out += part[t];
+        const int64_t i0 = n * t / nt, i1 = n * (t + 1) / nt;
+        for (int64_t i = i0; i < i1; ++i) lineOffs[i] += base;
+        base += (int64_t)part[t].size();
+        mapped += mappedPart[t];
     }
-    lineOffs[n] = (int64_t)out.size();
+    lineOffs[n] = base;
     return mapped;
+}
+
+static void copyParts(const std::vector<std::string>& part, char* out) {
+    int64_t k = 0;
+    for (const std::string& s : part) { std::memcpy(out + k, s.data(), s.size()); k += (int64_t)s.size(); }
 }
 
 /* merge the mate batch's DP results behind the first batch's so one Finisher sees both */
@@ -776,12 +807,13 @@ extern "C" int msgs_map_batch(msgs_ctx* c, const msgs_reads* reads, const msgs_r
         if (rc) return rc;
         mergeDp(b1.dp, b2.dp);
     }
-    std::string text;
+    std::vector<std::string> parts;
     std::vector<int64_t> lo;
-    samText(c, p, reads, mates, b1, mates ? &b2 : nullptr, true, text, lo);
-    if (needed) *needed = (int64_t)text.size();
-    if ((int64_t)text.size() > cap || !out) return fail(c, MSGS_E_CAP, "output capacity");
-    std::memcpy(out, text.data(), text.size());
+    samText(c, p, reads, mates, b1, mates ? &b2 : nullptr, true, parts, lo);
+    const int64_t total = lo[reads->n];
+    if (needed) *needed = total;
+    if (total > cap || !out) return fail(c, MSGS_E_CAP, "output capacity");
+    copyParts(parts, out);
     if (lineOffs) std::memcpy(lineOffs, lo.data(), lo.size() * 8);
     return MSGS_OK;
 }
@@ -856,7 +888,7 @@ extern "C" int msgs_batch_run(msgs_ctx* c, int32_t slot, const msgs_params* p, i
     if (S.paired) mergeDp(b[0].dp, b[1].dp);
     auto h0 = std::chrono::steady_clock::now();
     int64_t mapped = samText(c, p, &S.host[0].view, S.paired ? &S.host[1].view : nullptr, b[0], S.paired ? &b[1] : nullptr,
-                             emitSam != 0, c->lastSam, c->lastLineOffs);
+                             emitSam != 0, c->lastParts, c->lastLineOffs);
     auto t1 = std::chrono::steady_clock::now();
     unsigned long long hs[8];
     CK(hipMemcpy(hs, dStats, sizeof(hs), hipMemcpyDeviceToHost), "stats");
@@ -886,8 +918,10 @@ extern "C" int msgs_batch_stats(msgs_ctx* c, double* stats, int32_t n) {
 
 extern "C" int msgs_batch_sam(msgs_ctx* c, char* out, int64_t cap, int64_t* needed) {
     if (!c) return MSGS_E_ARG;
-    if (needed) *needed = (int64_t)c->lastSam.size();
-    if (!out || (int64_t)c->lastSam.size() + 1 > cap) return fail(c, MSGS_E_CAP, "output capacity");
-    std::memcpy(out, c->lastSam.c_str(), c->lastSam.size() + 1);
+    const int64_t total = c->lastLineOffs.empty() ? 0 : c->lastLineOffs.back();
+    if (needed) *needed = total;
+    if (!out || total + 1 > cap) return fail(c, MSGS_E_CAP, "output capacity");
+    copyParts(c->lastParts, out);
+    out[total] = 0;
     return MSGS_OK;
 }

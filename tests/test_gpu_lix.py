@@ -126,15 +126,23 @@ def test_counts_match_fmd_backward_search(env):
 
 
 def test_trained_network_gains_over_root_and_binary_search(env):
+    """what the network adds to the method without it: smaller prediction errors than the
+    piecewise-linear root alone, hence narrower per-segment bounds and fewer key-table
+    probes than the root alone or a plain binary search"""
     chroms, names, odb, m, info, ct = env
     rng = np.random.default_rng(10)
-    keys = ct[rng.integers(0, len(ct), 20000)]
+    rows = rng.integers(0, len(ct), 20000)
+    keys = ct[rows]
+    lb = LX.lower_bounds(ct, keys)
+    err_mlp = np.abs(m.lix_predict(keys, 0) - lb)
+    err_root = np.abs(m.lix_predict(keys, 2) - lb)
     _, pr_mlp, _ = m.lix_query(keys, 1)
     _, pr_root, _ = m.lix_query(keys, 0)
     _, pr_bin, _ = m.lix_query(keys, -1)
-    print("max error mlp %d root %d; mean probes mlp %.2f root %.2f binary %.2f" % (
-        info["max_err"], info["max_err_root"], pr_mlp.mean(), pr_root.mean(), pr_bin.mean()), flush=True)
-    assert info["max_err"] < info["max_err_root"]
+    print("mean |error| mlp %.1f root %.1f; max error mlp %d root %d; mean probes mlp %.2f root %.2f binary %.2f" % (
+        err_mlp.mean(), err_root.mean(), info["max_err"], info["max_err_root"], pr_mlp.mean(), pr_root.mean(),
+        pr_bin.mean()), flush=True)
+    assert err_mlp.mean() < 0.8 * err_root.mean()
     assert pr_mlp.mean() < pr_root.mean() < pr_bin.mean()
 
 
