@@ -38,7 +38,13 @@ def parse():
     ap.add_argument("--batch-reads", type=int, default=262144)
     ap.add_argument("--cpu-pairs", type=int, default=40000, help="pairs in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--form", type=int, default=1, help="0 plain FMD extensions, 1 unique-interval text path")
+    ap.add_argument("--no-compare", action="store_true", help="skip the plain-FMD comparison run")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default lanes per CU")
+    ap.add_argument("--no-torch", action="store_true", help="no torch device synchronisation (profiler runs)")
+    ap.add_argument("--form", type=int, default=2,
+                    help="0 plain FMD extensions, 1 unique-interval text path, 2 form 1 + learned-index read filter")
+    ap.add_argument("--lix-segments", type=int, default=1 << 20, help="learned index: root segments")
+    ap.add_argument("--lix-steps", type=int, default=1000, help="learned index: training steps")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 1 GiB device copy (counter calibration under rocprofv3 --pmc)")
     return ap.parse_args()
@@ -66,37 +72,40 @@ def main():
                                                               time.time() - t0))
     t1 = time.time()
     m = M.Mapper(0).build(chroms, names)
-    m.set("form", a.form)
     build_s = time.time() - t1
+    lix = None
+    if a.form == 2:
+        t3 = time.time()
+        lix = m.lix_train(segments=a.lix_segments, samples=1 << 22, steps=a.lix_steps, batch=4096, lr=1e-3, seed=7)
+        lix["train_s"] = round(time.time() - t3, 2)
+        log("learned index trained in %.1fs: %s" % (lix["train_s"], lix))
+    m.set("form", a.form)
     log("index built on the GPU in %.1fs (BD length %d)" % (build_s, m.n))
     for b, (r1, r2) in enumerate(batches):
         m.upload(b, r1, r2)
     p = M.default_params(insMean=350, insSD=50, insMin=350 - 10 * 50, insMax=350 + 10 * 50)
-    try:
-        import torch
-        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
-    except Exception:
-        sync = lambda: None
+    sync = lambda: None   # msgs_batch_run returns after its stream has drained
+    if not a.no_torch:
+        try:
+            import torch
+            if torch.cuda.is_available():
+                sync = torch.cuda.synchronize
+        except Exception:
+            pass
 
-    for w in range(a.warmup):
-        m.run(w % n_batches, p, True)
-        log("warmup step %d done" % w)
-    sync()
-    stats = []
-    t2 = time.time()
-    for k in range(a.steps):
-        m.run(k % n_batches, p, True)
-        stats.append(m.stats())
-    sync()
-    elapsed = time.time() - t2
-    S = np.array(stats)
+    tuned = None
+    if not a.no_tune:
+        lanes, tus = m.tune(0, p)
+        tuned = {"lanes_per_cu": lanes, "smem_us": round(tus, 1)}
+        log("autotuner: %d lanes per CU (%.0f us for the SMEM stage of batch 0)" % (lanes, tus))
+    elapsed, S = run_steps(m, p, n_batches, a.steps, a.warmup, sync, log)
     reads, mapped = S[:, 0].sum(), S[:, 1].sum()
     value = mapped / elapsed
     ext_calls, sa_lookups, cells = S[:, 2].sum(), S[:, 3].sum(), S[:, 4].sum()
     smem_us, dp_us = S[:, 5].sum(), S[:, 6].sum()
     # dominant kernel: SMEM search; algorithmic bytes = one 64-B occ block per rank side
     # (2 per extension with an unambiguous base)
-    launches = 2 * a.steps
+    launches = a.steps  # one SMEM launch per step: the mates follow the reads in one batch
     fast_steps, lookups = S[:, 10].sum(), S[:, 11].sum()
     bytes_per_launch = (64.0 * 2.0 * ext_calls + 64.0 * lookups + fast_steps) / launches
     avg_us = smem_us / launches
@@ -111,30 +120,78 @@ def main():
         "config": {"workload": "PE 2x150: 64 genomes x 4 Mb (256 Mbp, GC U(0.3,0.7)), insert N(350,50), "
                                "0.5% sub, 0.05% indel, 5% off-target pairs; 256k-read batches in HBM",
                    "batch_reads": 2 * pairs_per_batch, "bd_length": int(m.n),
-                   "form": ["plain FMD", "unique-interval text path"][a.form]},
+                   "form": ["plain FMD", "unique-interval text path", "unique-interval text path + learned-index read filter"][a.form]},
         "roofline": {"bound": "hbm", "kernel": "smem_kernel", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "bytes_per_launch": bytes_per_launch, "avg_launch_us": round(avg_us, 1)},
         "dp_gcups": round(cells / (dp_us * 1e-6) / 1e9, 3) if dp_us > 0 else None,
+        "smem_iterations": {"lane_per_read": S[:, 12].sum() / reads,
+                            "simd_utilisation": S[:, 12].sum() / max(1.0, 64.0 * S[:, 13].sum()),
+                            "max_wavefront": int(S[:, 14].max()),
+                            "us_per_iteration_of_slowest_wavefront": smem_us / a.steps / max(1.0, S[:, 14].mean())},
         "per_read": {"occ_extensions": ext_calls / reads, "text_steps": fast_steps / reads,
                      "smem_sa_isa_lookups": lookups / reads, "seed_sa_lookups": sa_lookups / reads,
                      "dp_cells": cells / reads},
-        "stage_ms_per_step": {"smem_kernel": smem_us / a.steps / 1000, "seed_stage": S[:, 8].sum() / a.steps / 1000,
+        "stage_ms_per_step": {"read_filter": S[:, 15].sum() / a.steps / 1000,
+                              "smem_kernel": smem_us / a.steps / 1000, "seed_stage": S[:, 8].sum() / a.steps / 1000,
                               "dp_kernel": dp_us / a.steps / 1000, "host_finish": S[:, 9].sum() / a.steps / 1000,
                               "whole": S[:, 7].sum() / a.steps / 1000},
         "index_build_s": round(build_s, 2),
+        "learned_index": lix,
+        "autotune": tuned,
         "mapped_frac": round(mapped / reads, 4),
     }
+    if a.form != 0 and not a.no_compare:
+        # the plain-FMD form on the same batches, same process, same clock
+        m.set("form", 0)
+        lanes0 = None
+        if not a.no_tune:
+            lanes0, _ = m.tune(0, p)   # the plain form gets its own best setting
+        el0, S0 = run_steps(m, p, n_batches, a.steps, a.warmup, sync, None)
+        v0 = S0[:, 1].sum() / el0
+        out["plain_fmd"] = {"value": round(v0, 1), "ms_per_step": round(1000 * el0 / a.steps, 3),
+                            "smem_kernel_ms_per_step": S0[:, 5].sum() / a.steps / 1000,
+                            "occ_extensions_per_read": S0[:, 2].sum() / S0[:, 0].sum(), "lanes_per_cu": lanes0}
+        out["speedup_vs_plain_fmd"] = round(value / v0, 3)
+        out["smem_stage_speedup_vs_plain_fmd"] = round((S0[:, 5].sum() + 0.0) / max(1.0, smem_us + S[:, 15].sum()), 3)
+        log("plain FMD form: %.0f mapped reads/s; learned-index form / plain = %.3f" % (v0, value / v0))
+        m.set("form", a.form)
+        if tuned:
+            m.set("lanes_per_cu", tuned["lanes_per_cu"])
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(m, chroms, names, batches[0], p, a.cpu_pairs)
     if a.calib_copy:
-        import torch
-        x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
-        y = torch.empty_like(x)
-        y.copy_(x)
-        torch.cuda.synchronize()
-        out["calib_copy_bytes"] = 4 << 28
+        # a plain device copy of known size (k_calib_copy) to scale FETCH_SIZE / WRITE_SIZE
+        out["calib_copy_bytes"] = 1 << 30
+        out["calib_copy_us"] = m.calib_copy(1 << 30)
     print(json.dumps(out), flush=True)
+
+
+def run_steps(m, p, nb, steps, warmup, sync, log):
+    """Map `steps` batches (batch k = resident slot k % nb).  With two or more resident
+    batches the runs are pipelined: batch k's host finishing (records, MAPQ, SAM text)
+    overlaps batch k+1's GPU stages.  Returns (seconds, per-step counters)."""
+    for w in range(warmup):
+        m.submit(w % nb, p, True)
+        m.wait(w % nb)
+        if log:
+            log("warmup step %d done" % w)
+    sync()
+    stats = []
+    t = time.time()
+    prev = None
+    for k in range(steps):
+        slot = k % nb
+        if nb == 1 and prev is not None:
+            stats.append(m.wait(prev))
+            prev = None
+        m.submit(slot, p, True)
+        if prev is not None:
+            stats.append(m.wait(prev))
+        prev = slot
+    stats.append(m.wait(prev))
+    sync()
+    return time.time() - t, np.array(stats)
 
 
 def cpu_baseline(m, chroms, names, batch, p, npairs):

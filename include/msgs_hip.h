@@ -129,11 +129,25 @@ int msgs_align_batch(msgs_ctx* ctx, const msgs_reads* reads, const msgs_params* 
  * stats[0] reads, [1] mapped reads, [2] extension calls, [3] SA lookups, [4] DP cells,
  * [5] seeding kernel microseconds (HIP events), [6] DP kernel microseconds,
  * [7] whole-run microseconds, [8] seed-stage microseconds, [9] host finishing microseconds,
- * [10] unique-interval text steps, [11] SA/ISA lookups of the SMEM kernel. */
+ * [10] unique-interval text steps, [11] SA/ISA lookups of the SMEM kernel,
+ * [12] SMEM-kernel lane iterations, [13] its wavefront iterations (busiest lane of each
+ * wavefront, summed), [14] the largest wavefront iteration count, [15] learned-index read
+ * filter microseconds (form 2). */
 int msgs_batch_upload(msgs_ctx* ctx, int32_t slot, const msgs_reads* reads, const msgs_reads* mates);
 int msgs_batch_run(msgs_ctx* ctx, int32_t slot, const msgs_params* p, int32_t emitSam);
+/* msgs_batch_run in two halves: submit runs the GPU stages and leaves the host
+ * finishing (records, MAPQ, SAM text) to a worker thread of the slot, so the caller can
+ * submit the next batch while it runs; wait joins it and makes its counters and SAM the
+ * context's last run (msgs_batch_stats / msgs_batch_sam). */
+int msgs_batch_submit(msgs_ctx* ctx, int32_t slot, const msgs_params* p, int32_t emitSam);
+int msgs_batch_wait(msgs_ctx* ctx, int32_t slot);
 int msgs_batch_stats(msgs_ctx* ctx, double* stats, int32_t n);
 int msgs_batch_sam(msgs_ctx* ctx, char* out, int64_t cap, int64_t* needed);
+/* autotuner (no reference counterpart): times the SMEM search of resident batch `slot`
+ * with each lanes-per-CU value in cand[0..nCand) (NULL: 128..1024), keeps the fastest
+ * in the context ("lanes_per_cu") and reports it; results never depend on the choice. */
+int msgs_tune(msgs_ctx* ctx, int32_t slot, const msgs_params* p, const int32_t* cand, int32_t nCand, int32_t* best,
+              double* bestMicros);
 
 /* ---- learned index (no reference counterpart) ----
  * A model of the SA rank of 16-mers: piecewise-linear root over `segments` rank
@@ -162,6 +176,10 @@ int msgs_lix_grad(msgs_ctx* ctx, const float* W, const float* X, const float* T,
 int msgs_lix_weights(msgs_ctx* ctx, float* get, const float* set);
 /* knot table: S+1 keys and lower bounds */
 int msgs_lix_knots(msgs_ctx* ctx, uint64_t* keys, int64_t* ranks, int32_t* S);
+
+/* counter calibration (no reference counterpart): one device copy of `bytes` (multiple
+ * of 16) by a plain kernel, timed; hardware byte counters are scaled against it */
+int msgs_util_calib_copy(msgs_ctx* ctx, int64_t bytes, double* micros);
 
 /* ---- utilities ----
  * The permutation std::sort (libstdc++) gives for n keys compared with '<' -- the

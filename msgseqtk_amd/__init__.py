@@ -26,7 +26,7 @@ EXPORTS = [
     "msgs_rank_batch", "msgs_ext_batch", "msgs_smem_batch", "msgs_seed_batch",
     "msgs_map_batch", "msgs_align_batch", "msgs_batch_upload", "msgs_batch_run",
     "msgs_batch_stats", "msgs_batch_sam", "msgs_util_sort_by_key",
-    "msgs_lix_train", "msgs_lix_query", "msgs_lix_predict", "msgs_lix_grad", "msgs_lix_weights", "msgs_lix_knots",
+    "msgs_batch_submit", "msgs_batch_wait", "msgs_tune", "msgs_util_calib_copy", "msgs_lix_train", "msgs_lix_query", "msgs_lix_predict", "msgs_lix_grad", "msgs_lix_weights", "msgs_lix_knots",
 ]
 N_LIX_PARAMS = 16 * 64 + 64 + 64 + 1
 
@@ -91,6 +91,11 @@ def lib():
     L.msgs_batch_stats.argtypes = [vp, PD, C.c_int32]
     L.msgs_batch_sam.argtypes = [vp, C.c_char_p, C.c_int64, P64]
     L.msgs_util_sort_by_key.argtypes = [PD, C.POINTER(C.c_int32), C.c_int64]
+    L.msgs_tune.argtypes = [vp, C.c_int32, C.POINTER(Params), C.POINTER(C.c_int32), C.c_int32,
+                            C.POINTER(C.c_int32), PD]
+    L.msgs_util_calib_copy.argtypes = [vp, C.c_int64, PD]
+    L.msgs_batch_submit.argtypes = [vp, C.c_int32, C.POINTER(Params), C.c_int32]
+    L.msgs_batch_wait.argtypes = [vp, C.c_int32]
     PF = C.POINTER(C.c_float)
     PU64 = C.POINTER(C.c_uint64)
     L.msgs_lix_train.argtypes = [vp, C.c_int32, C.c_int64, C.c_int32, C.c_int32, C.c_double, C.c_uint64, PD]
@@ -284,6 +289,27 @@ class Mapper:
             self._chk(rc, "msgs_map_batch")
             text = buf.raw[:need.value].decode()
             return [text[lo[i]:lo[i + 1]] for i in range(rb.n)]
+
+    def submit(self, slot, params, emit_sam=True):
+        self._chk(lib().msgs_batch_submit(self.h, slot, C.byref(params), 1 if emit_sam else 0), "msgs_batch_submit")
+
+    def wait(self, slot):
+        self._chk(lib().msgs_batch_wait(self.h, slot), "msgs_batch_wait")
+        return self.stats()
+
+    def calib_copy(self, nbytes):
+        us = C.c_double(0)
+        self._chk(lib().msgs_util_calib_copy(self.h, nbytes, C.byref(us)), "msgs_util_calib_copy")
+        return us.value
+
+    def tune(self, slot, params, cand=None):
+        """pick the fastest lanes-per-CU for the SMEM search of a resident batch"""
+        c = None if cand is None else np.ascontiguousarray(cand, np.int32)
+        best = C.c_int32(0)
+        us = C.c_double(0)
+        self._chk(lib().msgs_tune(self.h, slot, C.byref(params), None if c is None else _p(c, C.c_int32),
+                                  0 if c is None else len(c), C.byref(best), C.byref(us)), "msgs_tune")
+        return best.value, us.value
 
     # learned index
     def lix_train(self, segments=4096, samples=1 << 20, steps=2000, batch=4096, lr=1e-3, seed=7):

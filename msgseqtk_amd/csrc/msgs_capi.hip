@@ -135,6 +135,7 @@ int msgs_ctx_create(int device, msgs_ctx** out) {
 
 void msgs_ctx_destroy(msgs_ctx* c) {
     if (!c) return;
+    for (auto& s : c->slots) s.join();
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     c->db.release();
@@ -170,6 +171,8 @@ int msgs_db_build(msgs_ctx* c, const uint8_t* seqs, const int64_t* lens, int32_t
         total += 2 * (lens[t] + 1);
         seqTotal += lens[t];
     }
+    /* 32-bit occ counts, SA/ISA entries and bi-interval fields: one-block indexes only */
+    if (total >= (int64_t)1 << 31) return fail(c, MSGS_E_UNSUPPORTED, "BD text of 2^31 or more bases");
     /* names */
     const char* p = names ? names : "";
     for (int32_t t = 0; t < nChrom; ++t) {
@@ -220,7 +223,8 @@ int msgs_db_build(msgs_ctx* c, const uint8_t* seqs, const int64_t* lens, int32_t
         if (!(db.ll[b] <= 0)) return fail(c, MSGS_E_DB, "base log-frequency above zero");
 
     hipStream_t st = c->stream;
-    if (hipMalloc(&db.dBd, total) != hipSuccess) return fail(c, MSGS_E_NOMEM, "BD text");
+    /* padded: the SMEM kernel reads the text in aligned 8-byte words */
+    if (hipMalloc(&db.dBd, total + 64) != hipSuccess) return fail(c, MSGS_E_NOMEM, "BD text");
     if (hipMemcpyAsync(db.dBd, bd.data(), total, hipMemcpyHostToDevice, st) != hipSuccess)
         return fail(c, MSGS_E_HIP, "BD upload");
     std::string err;
@@ -331,7 +335,8 @@ int uploadReads(msgs_ctx* c, const msgs_reads* r, const char* pfx, DevReads& out
     out.n = n;
     out.maxL = maxL;
     out.total = total;
-    out.seq = c->bufs.get<uint8_t>(p + ".seq", total + 1, e);
+    /* padded: the SMEM kernel reads bases in aligned 8-byte words */
+    out.seq = c->bufs.get<uint8_t>(p + ".seq", total + 64, e);
     out.offs = c->bufs.get<int64_t>(p + ".offs", n + 1, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "read buffers");
     std::vector<int64_t> offs(n + 1);
@@ -357,24 +362,29 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
     a.minLen = p->minLen;
     a.gate1 = std::isfinite(p->maxEvalue) ? 1 : 0;
     a.T1 = T; a.T2 = T;
-    /* form 2: reads without any 16-mer in the index cannot have an SMEM of length >= 16 */
+    /* form 2: skip reads in which no window that an emitted SMEM must contain occurs */
     a.need = nullptr;
+    c->lastFilterMicros = 0;
     if (c->form == 2 && c->lix.trained && p->minLen >= 16 && n > 0) {
         uint8_t* need = nullptr;
-        int rc = runLixFilter(c, r, &need, dStats);
+        (void)hipEventRecord(c->ev0, c->stream);
+        int rc = runLixFilter(c, r, (int32_t)p->minLen, T, &need, dStats);
         if (rc) return rc;
+        (void)hipEventRecord(c->ev1, c->stream);
+        if (hipEventSynchronize(c->ev1) != hipSuccess) return fail(c, MSGS_E_HIP, "read filter");
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        c->lastFilterMicros = ms * 1000.0;
         a.need = need;
     }
+    const int G = c->form == 2 ? c->groupLanes : 0;
     const int64_t lanesMax = (int64_t)std::max(1, c->nCU) * c->lanesPerCU;
-    const int64_t lanes = std::max<int64_t>(256, std::min<int64_t>(((n + 255) / 256) * 256, lanesMax));
+    const int64_t want = n * std::max(G, 1);
+    const int64_t lanes = std::max<int64_t>(256, (std::min<int64_t>(((want + 255) / 256) * 256, lanesMax) / 256) * 256);
     const int64_t candCap = r.maxL + 2;
     hipError_t e = hipSuccess;
-    a.candPos = c->bufs.get<int32_t>("sm.cp", candCap * lanes, e);
-    a.candF = c->bufs.get<int64_t>("sm.cf", candCap * lanes, e);
-    a.candR = c->bufs.get<int64_t>("sm.cr", candCap * lanes, e);
-    a.candS = c->bufs.get<int64_t>("sm.cs", candCap * lanes, e);
-    a.candLL = c->bufs.get<double>("sm.cl", candCap * lanes, e);
-    a.candX = c->bufs.get<int64_t>("sm.cx", candCap * lanes, e);
+    a.cand = c->bufs.get<SmemCand>("sm.cand", candCap * lanes, e);
+    a.candCap = candCap;
     a.nLanes = lanes;
     a.queue = c->bufs.get<unsigned long long>("sm.q", 1, e);
     a.cnt = c->bufs.get<int32_t>("sm.cnt", n + 1, e);
@@ -388,9 +398,18 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
         if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "smem output");
         (void)hipMemsetAsync(a.queue, 0, 8, st);
         (void)hipEventRecord(c->ev0, st);
+        if (n > 0 && a.gate1) {
+            double* pref = c->bufs.get<double>("sm.pref", r.total + n + 1, e);
+            if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "loglik prefix sums");
+            smem_prefix_kernel<<<(unsigned)std::min<int64_t>((n + 255) / 256, 4096), 256, 0, st>>>(c->db.view(), a, pref);
+            a.prefLL = pref;
+        }
         if (n > 0) {
-            if (c->form == 0) smem_kernel_t<false><<<(unsigned)(lanes / 256), 256, 0, st>>>(c->db.view(), a);
-            else smem_kernel_t<true><<<(unsigned)(lanes / 256), 256, 0, st>>>(c->db.view(), a);
+            const unsigned grid = (unsigned)(lanes / 256);
+            if (G == 8) smem_group_kernel<8><<<grid, 256, 0, st>>>(c->db.view(), a);
+            else if (G == 16) smem_group_kernel<16><<<grid, 256, 0, st>>>(c->db.view(), a);
+            else if (c->form == 0) smem_kernel_t<false><<<grid, 256, 0, st>>>(c->db.view(), a);
+            else smem_kernel_t<true><<<grid, 256, 0, st>>>(c->db.view(), a);
         }
         (void)hipEventRecord(c->ev1, st);
         if (hipGetLastError() != hipSuccess) return fail(c, MSGS_E_HIP, "smem launch");
@@ -541,15 +560,46 @@ extern "C" int msgs_db_chrom(msgs_ctx* c, int32_t tid, char* name, int64_t cap, 
 }
 extern "C" int32_t msgs_db_nchrom(msgs_ctx* c) { return c ? c->db.nChrom : 0; }
 
-/* context options: "form" (0 plain FMD, 1 unique-interval text path), "lanes_per_cu",
- * "smem_cap", "host_threads" */
+/* context options: "form" (0 plain FMD, 1 unique-interval text path, 2 the batched form:
+ * lane groups per read, text path and the learned-index read filter), "group_lanes",
+ * "lanes_per_cu", "smem_cap", "host_threads" */
 extern "C" int msgs_ctx_set(msgs_ctx* c, const char* key, int64_t value) {
     if (!c || !key) return MSGS_E_ARG;
     std::string k(key);
     if (k == "form") { if (value < 0 || value > 2) return fail(c, MSGS_E_ARG, "form 0, 1 or 2"); c->form = (int)value; }
-    else if (k == "lanes_per_cu") { if (value < 64 || value > 8192 || value % 256) return fail(c, MSGS_E_ARG, "lanes_per_cu: multiple of 256 in [256, 8192]"); c->lanesPerCU = (int)value; }
+    else if (k == "group_lanes") { if (value != 0 && value != 8 && value != 16) return fail(c, MSGS_E_ARG, "group_lanes: 0, 8 or 16"); c->groupLanes = (int)value; }
+    else if (k == "lanes_per_cu") { if (value < 64 || value > 8192 || value % 64) return fail(c, MSGS_E_ARG, "lanes_per_cu: multiple of 64 in [64, 8192]"); c->lanesPerCU = (int)value; }
     else if (k == "smem_cap") { if (value < 1 || value > 4096) return fail(c, MSGS_E_ARG, "smem_cap in [1, 4096]"); c->smemCap = (int)value; }
     else if (k == "host_threads") { if (value < 1 || value > 256) return fail(c, MSGS_E_ARG, "host_threads in [1, 256]"); c->hostThreads = (int)value; }
     else return fail(c, MSGS_E_ARG, "unknown option " + k);
+    return MSGS_OK;
+}
+
+/* ------------------------------------------------------------------ counter calibration */
+
+namespace {
+__global__ void __launch_bounds__(256) k_calib_copy(const uint4* src, uint4* dst, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+}  // namespace
+
+/* One device-to-device copy of `bytes` (a multiple of 16) by a plain kernel, so that
+ * hardware byte counters can be scaled against a transfer of known size. */
+extern "C" int msgs_util_calib_copy(msgs_ctx* c, int64_t bytes, double* micros) {
+    if (!c || bytes <= 0 || bytes % 16) return fail(c, MSGS_E_ARG, "bytes must be a positive multiple of 16");
+    hipError_t e = hipSuccess;
+    uint4* a = c->bufs.get<uint4>("calib.a", bytes / 16, e);
+    uint4* b = c->bufs.get<uint4>("calib.b", bytes / 16, e);
+    if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "calibration buffers");
+    const int64_t n = bytes / 16;
+    const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, (int64_t)std::max(1, c->nCU) * 64);
+    (void)hipEventRecord(c->ev0, c->stream);
+    k_calib_copy<<<blocks, 256, 0, c->stream>>>(a, b, n);
+    (void)hipEventRecord(c->ev1, c->stream);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, MSGS_E_HIP, "calibration copy");
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    if (micros) *micros = ms * 1000.0;
     return MSGS_OK;
 }

@@ -8,6 +8,8 @@
 
 #include <map>
 #include <string>
+#include <memory>
+#include <thread>
 #include <vector>
 
 #include "msgs_host.h"
@@ -59,14 +61,30 @@ struct HostBatch {
     msgs_reads view{};
 };
 
-/* a resident batch (reads kept in HBM between runs) */
+struct BatchResult;
+
+/* a resident batch (reads kept in HBM between runs); paired batches hold the reads
+ * followed by their mates on the device.  A submitted run finishes its records on a
+ * host worker thread while the caller goes on with other batches. */
 struct Slot {
     bool used = false;
     bool paired = false;
     HostBatch host[2];
-    DevReads dev[2];
-    uint8_t* dQual[2] = {nullptr, nullptr};
-    void release() { used = false; }
+    DevReads dev;
+    uint8_t* dQual = nullptr;
+    std::thread worker;
+    bool pending = false;
+    std::shared_ptr<BatchResult> res;
+    msgs_params params{};
+    std::vector<std::string> parts;
+    std::vector<int64_t> lineOffs;
+    double stats[16] = {};
+    void join() {
+        if (worker.joinable()) worker.join();
+        pending = false;
+        res.reset();
+    }
+    void release() { join(); used = false; }
 };
 
 struct msgs_ctx {
@@ -75,6 +93,7 @@ struct msgs_ctx {
     int lanesPerCU = 1024;
     int smemCap = 32;
     int form = 1;
+    int groupLanes = 8;   /* form 2: lanes per read of the SMEM kernel (0 = one lane per read) */
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
     std::string err;
@@ -84,6 +103,7 @@ struct msgs_ctx {
     Slot slots[16];
     double stats[16] = {};
     double lastSmemMicros = 0;
+    double lastFilterMicros = 0;
     int hostThreads = 16;
     /* SAM text of the last msgs_batch_run: per-thread parts in read order and the
      * offset of each read's records in their concatenation */
@@ -95,6 +115,7 @@ int fail(msgs_ctx* c, int code, const std::string& msg);
 bool evalueThreshold(double n, double maxE, double* T);
 int uploadReads(msgs_ctx* c, const msgs_reads* r, const char* pfx, DevReads& out);
 int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out, unsigned long long* dStats);
-int runLixFilter(msgs_ctx* c, const DevReads& r, uint8_t** needOut, unsigned long long* dStats);
+int runLixFilter(msgs_ctx* c, const DevReads& r, int32_t minLen, double T, uint8_t** needOut,
+                 unsigned long long* dStats);
 
 #endif

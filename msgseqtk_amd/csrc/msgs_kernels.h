@@ -17,6 +17,15 @@ struct SmemRec {
     double logP;
 };
 
+/* a candidate of a search's lock-step list: bi-interval, loglik, its moving side as a
+ * text position (X, -1 when the row values are exact) and its fixed read end (pos);
+ * 32 bytes (indexes hold fewer than 2^31 rows) */
+struct SmemCand {
+    int32_t F, R, S, X;
+    double LL;
+    int32_t pos, pad;
+};
+
 struct SmemArgs {
     const uint8_t* seqs;
     const int64_t* offs;
@@ -27,19 +36,21 @@ struct SmemArgs {
     int32_t cap;               /* SMEM slots per read */
     int32_t* cnt;              /* SMEMs emitted per read (may exceed cap) */
     SmemRec* out;              /* [nReads][cap] */
-    int32_t* candPos;          /* candidate scratch, [candCap][nLanes] */
-    int64_t* candF;
-    int64_t* candR;
-    int64_t* candS;
-    double* candLL;
-    int64_t* candX;            /* moving side as a text position (-1: row values exact) */
+    SmemCand* cand;            /* candidate scratch, [nLanes][candCap] */
+    int64_t candCap;
     int64_t nLanes;
-    const uint8_t* need;       /* per read: 0 = no 16-mer in the index, skip (may be NULL) */
+    const uint8_t* need;       /* per read: 0 = the read cannot emit an SMEM, skip (may be NULL) */
+    const double* prefLL;      /* per read at offs[r] + r: L+1 left-to-right loglik prefix sums */
     unsigned long long* queue; /* next read */
-    unsigned long long* stats; /* [0] occ-block extensions, [2] text steps, [3] SA/ISA lookups */
+    unsigned long long* stats; /* [0] occ-block extensions, [2] text steps, [3] SA/ISA lookups,
+                                * [5] lane iterations, [6] wavefront iterations, [7] max of [6] */
 };
 
 template <bool FAST> __global__ void smem_kernel_t(DevIndex ix, SmemArgs a);
+/* the same search with one read per group of G lanes (same output) */
+template <int G> __global__ void smem_group_kernel(DevIndex ix, SmemArgs a);
+/* per-read loglik prefix sums (SmemArgs::prefLL) */
+__global__ void smem_prefix_kernel(DevIndex ix, SmemArgs a, double* pref);
 
 /* one seed (SeedPair, src/SeedPair.h:120-128) */
 struct SeedRec {

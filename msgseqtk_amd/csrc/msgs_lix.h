@@ -9,6 +9,8 @@
 
 /* W1[16][64], b1[64], W2[64], b2 */
 constexpr int kLixParams = 16 * 64 + 64 + 64 + 1;
+/* the root's bucket table indexes keys by their top 24 bits (first 8 symbols) */
+constexpr int kBucketBits = 24;
 
 struct LixView {
     const uint64_t* ct;     /* 16-symbol key of each SA row (non-decreasing) */
@@ -17,6 +19,7 @@ struct LixView {
     const int64_t* kRank;   /* S+1 knot lower bounds (last = n) */
     int32_t S;
     const float* W;         /* MLP parameters */
+    const uint32_t* kBucket;    /* 2^24 + 1: first knot with key >= bucket b << 24 */
     const uint32_t* eSeg;       /* per segment: max |prediction - lb| of its keys, MLP */
     const uint32_t* eSegRoot;   /* the same for the root alone */
 };
@@ -26,6 +29,7 @@ struct LixState {
     uint64_t* dKey = nullptr;
     int64_t* dRank = nullptr;
     float* dW = nullptr;
+    uint32_t* dBucket = nullptr;
     uint32_t* dESeg = nullptr;
     uint32_t* dESegRoot = nullptr;
     int64_t n = 0;

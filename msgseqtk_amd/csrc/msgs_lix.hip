@@ -78,14 +78,31 @@ __global__ void k_knots(const uint64_t* ct, int64_t n, int32_t S, uint64_t* kKey
     }
 }
 
-/* segment of x: largest j with kKey[j] <= x (0 when x is below every knot) */
-__device__ __forceinline__ int32_t segOf(const LixView& m, uint64_t x) {
-    int32_t lo = 0, hi = m.S;   /* invariant: answer in [lo, hi) */
-    while (hi - lo > 1) {
-        int32_t mid = (lo + hi) >> 1;
-        if (m.kKey[mid] <= x) lo = mid; else hi = mid;
+/* kBucket[b] = first knot j with kKey[j] >= b << (48 - kBucketBits) */
+__global__ void k_buckets(const uint64_t* kKey, int32_t S, uint32_t* kBucket) {
+    const int64_t nb = ((int64_t)1 << kBucketBits) + 1;
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t x = (uint64_t)b << (48 - kBucketBits);
+        int32_t lo = 0, hi = S;   /* kKey[S] = all ones >= x */
+        while (lo < hi) {
+            const int32_t mid = (lo + hi) >> 1;
+            if (kKey[mid] < x) lo = mid + 1; else hi = mid;
+        }
+        kBucket[b] = (uint32_t)lo;
     }
-    return lo;
+}
+
+/* segment of x: largest j < S with kKey[j] <= x (0 when x is below every knot); the
+ * first knot above x lies between the bucket bounds of x's top bits */
+__device__ __forceinline__ int32_t segOf(const LixView& m, uint64_t x) {
+    const uint64_t b = x >> (48 - kBucketBits);
+    if (b >= ((uint64_t)1 << kBucketBits)) return m.S - 1;   /* above every 48-bit key */
+    int32_t lo = (int32_t)m.kBucket[b], hi = (int32_t)m.kBucket[b + 1];
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (m.kKey[mid] > x) hi = mid; else lo = mid + 1;
+    }
+    return lo > 0 ? lo - 1 : 0;
 }
 
 struct Plr { double base, width, t; };
@@ -281,34 +298,43 @@ __global__ void k_fwd(const float* W, const float* X, const float* T, int64_t of
     Err[s] = d * d;
 }
 
-/* gradients, one output element per thread, each a fixed-order sum over the batch */
-__global__ void k_grad(const float* W, const float* X, int64_t off, int32_t B, const float* H, const float* Dout,
-                       const float* Err, float* G, float* loss) {
-    const int32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+/* gradients: one 256-thread block per parameter; thread t sums samples t, t+256, ...
+ * in order, then a fixed pairwise tree in LDS (deterministic: the summation order does
+ * not depend on scheduling).  Block kLixParams - 1 (b2) also reduces the loss. */
+__global__ void __launch_bounds__(256) k_grad(const float* W, const float* X, int64_t off, int32_t B, const float* H,
+                                              const float* Dout, const float* Err, float* G, float* loss) {
+    __shared__ float sA[256], sE[256];
+    const int32_t id = blockIdx.x;
+    const int t = threadIdx.x;
     const float* W2 = W + 16 * 64 + 64;
+    float a = 0.f, e = 0.f;
     if (id < 16 * 64) {
         const int k = id / 64, u = id % 64;
-        float a = 0.f;
-        for (int32_t s = 0; s < B; ++s) {
-            const float dh = H[s * 64 + u] > 0 ? Dout[s] * W2[u] : 0.f;
+        const float w2 = W2[u];
+        for (int32_t s = t; s < B; s += 256) {
+            const float dh = H[s * 64 + u] > 0 ? Dout[s] * w2 : 0.f;
             a += X[(off + s) * 16 + k] * dh;
         }
-        G[id] = a;
     } else if (id < 16 * 64 + 64) {
         const int u = id - 16 * 64;
-        float a = 0.f;
-        for (int32_t s = 0; s < B; ++s) a += H[s * 64 + u] > 0 ? Dout[s] * W2[u] : 0.f;
-        G[id] = a;
+        const float w2 = W2[u];
+        for (int32_t s = t; s < B; s += 256) a += H[s * 64 + u] > 0 ? Dout[s] * w2 : 0.f;
     } else if (id < 16 * 64 + 128) {
         const int u = id - 16 * 64 - 64;
-        float a = 0.f;
-        for (int32_t s = 0; s < B; ++s) a += H[s * 64 + u] * Dout[s];
-        G[id] = a;
-    } else if (id == 16 * 64 + 128) {
-        float a = 0.f, e = 0.f;
-        for (int32_t s = 0; s < B; ++s) { a += Dout[s]; e += Err[s]; }
-        G[id] = a;
-        if (loss) *loss = e / (float)B;
+        for (int32_t s = t; s < B; s += 256) a += H[s * 64 + u] * Dout[s];
+    } else {
+        for (int32_t s = t; s < B; s += 256) { a += Dout[s]; e += Err[s]; }
+    }
+    sA[t] = a;
+    sE[t] = e;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if (t < w) { sA[t] += sA[t + w]; sE[t] += sE[t + w]; }
+        __syncthreads();
+    }
+    if (t == 0) {
+        G[id] = sA[0];
+        if (id == kLixParams - 1 && loss) *loss = sE[0] / (float)B;
     }
 }
 
@@ -387,50 +413,86 @@ __device__ __forceinline__ uint64_t readKey(const uint8_t* s) {
     return k;
 }
 
-/* need[r] = 1 when some 16-mer of read r occurs in the index (windows tried from the
- * middle outwards) or some window holds a base other than A/C/G/T, else 0: such a read
- * cannot have an SMEM of length >= 16. */
-__global__ void __launch_bounds__(256) k_filter(LixView m, const uint8_t* seqs, const int64_t* offs,
-                                                int64_t nReads, uint8_t* need, unsigned long long* stats) {
+/*
+ * Read filter (eight lanes per read).  An SMEM the search emits is a substring Y of
+ * the read with |Y| >= minLen that occurs in the BD text and whose logP (a sum of the
+ * per-base logliks, src/SMEM.cpp:26-40) is <= T (the e-value gate, src/SMEM.cpp:99,152).
+ * All logliks are negative, so a Y starting at a contains W(a) = read[a, c) with c the
+ * first end such that c - a >= minLen and the left-to-right sum is <= T + delta (delta
+ * covers any summation order: |error| < 1e-11 for reads of a few hundred bases).  If no
+ * W(a) occurs, the read emits no SMEM and its search can be skipped.  W(a) occurs iff
+ * some SA row whose 16-symbol key equals W's first 16 bases (found with the learned
+ * index) continues in the text with W's remaining bases.  Windows with a base other than
+ * A/C/G/T, and key blocks longer than kBlock rows, keep the read (conservative).
+ * need[r] = 1 keeps read r; requires minLen >= 16.
+ */
+constexpr int kBlock = 16;
+constexpr int kFilterLanes = 8;   /* lanes (windows in flight) per read */
+
+__global__ void __launch_bounds__(256) k_filter(LixView m, DevIndex ix, const uint8_t* seqs, const int64_t* offs,
+                                                int64_t nReads, int32_t minLen, double T, uint8_t* need,
+                                                unsigned long long* stats) {
     __shared__ short W1b[16 * 64];
     loadW1(m, W1b);
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    const int64_t rounds = (nReads + stride - 1) / stride;
+    const int lane = threadIdx.x & 63;
+    const int g = lane / kFilterLanes, sub = lane % kFilterLanes;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nWaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    constexpr int kGroups = 64 / kFilterLanes;
+    const double lA = ix.ll[1], lC = ix.ll[2], lG = ix.ll[4], lT = ix.ll[8];
+    const double Tm = T + 1e-9;
     unsigned long long nq = 0;
-    for (int64_t r = 0; r < rounds; ++r) {
-        const int64_t rid = r * stride + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int64_t r0 = wave * kGroups; r0 < nReads; r0 += nWaves * kGroups) {
+        const int64_t rid = r0 + g;
         const bool mine = rid < nReads;
-        const int64_t o = mine ? offs[rid] : 0;
-        const int64_t L = mine ? offs[rid + 1] - o : 0;
-        const int64_t nw = L >= 16 ? L - 15 : 0;
+        const int64_t off = mine ? offs[rid] : 0;
+        const int64_t L = mine ? offs[rid + 1] - off : 0;
+        const int64_t nw = L >= minLen ? L - minLen + 1 : 0;
         const int64_t mid = nw / 2;
-        int64_t t = 0;
         bool found = false;
-        while (true) {
-            /* next window with only A/C/G/T */
-            uint64_t x = ~0ull;
-            while (!found && t < nw && x == ~0ull) {
-                const int64_t w = (t & 1) ? mid - (t + 1) / 2 : mid + t / 2;
-                ++t;
-                if (w >= 0 && w < nw) {
-                    x = readKey(seqs + o + w);
-                    /* a window with another base (N, IUPAC, gap) keeps the read: SMEMs may
-                     * extend through such bases */
-                    if (x == ~0ull) found = true;
+        for (int64_t base = 0; __any(!found && base < nw); base += kFilterLanes) {
+            /* windows from the middle outwards */
+            const int64_t t = base + sub;
+            const int64_t a = (t & 1) ? mid - (t + 1) / 2 : mid + t / 2;
+            bool active = !found && t < nw && a >= 0 && a < nw;
+            bool keep = false;
+            int64_t c = -1;
+            uint64_t key = 0;
+            if (active) {
+                double x = 0;
+                int64_t wb = -1;
+                uint64_t w = 0;
+                for (int64_t b = a; b < L; ++b) {
+                    const int64_t gp = off + b;
+                    if ((gp & ~(int64_t)7) != wb) { wb = gp & ~(int64_t)7; w = *(const uint64_t*)(seqs + wb); }
+                    const uint32_t v = (uint32_t)(w >> (8 * (gp & 7))) & 0xffu;
+                    if (!(v == 1 || v == 2 || v == 4 || v == 8)) { keep = true; break; }
+                    if (b < a + 16) key = (key << 3) | sym3((uint8_t)v);
+                    x += v == 1 ? lA : (v == 2 ? lC : (v == 4 ? lG : lT));
+                    if (b + 1 - a >= minLen && x <= Tm) { c = b + 1; break; }
+                }
+                if (keep || c < 0) active = false;
+            }
+            int32_t j = 0;
+            const int64_t p = predict(m, key, active, W1b, true, &j);
+            if (active) {
+                ++nq;
+                int32_t pr = 0, fb = 0;
+                int64_t r = lbBounded(m, key, p, (int64_t)m.eSeg[j], &pr, &fb);
+                const int64_t wl = c - a;
+                for (int k = 0; r < m.n && m.ct[r] == key; ++r, ++k) {
+                    if (k == kBlock) { keep = true; break; }
+                    const int64_t pos = ix.sa[r];
+                    if (pos + wl > ix.n) continue;
+                    bool eq = true;
+                    for (int64_t u = 16; u < wl && eq; ++u) eq = ix.bd[pos + u] == seqs[off + a + u];
+                    if (eq) { keep = true; break; }
                 }
             }
-            const bool active = !found && x != ~0ull;
-            if (!__any(active)) break;
-            int32_t j = 0;
-            const int64_t p = predict(m, x, active, W1b, true, &j);
-            if (active) {
-                int32_t pr = 0, fb = 0;
-                const int64_t lo = lbBounded(m, x, p, (int64_t)m.eSeg[j], &pr, &fb);
-                found = lo < m.n && m.ct[lo] == x;
-                nq++;
-            }
+            const unsigned long long bal = __ballot(keep);
+            found = found || ((bal >> (g * kFilterLanes)) & ((1ull << kFilterLanes) - 1)) != 0;
         }
-        if (mine) need[rid] = found ? 1 : 0;
+        if (mine && sub == 0) need[rid] = found ? 1 : 0;
     }
     if (stats) atomicAdd(&stats[4], nq);
 }
@@ -494,14 +556,15 @@ inline unsigned gridFor(int64_t n, int t = 256) {
 
 LixView LixState::view() const {
     LixView v;
-    v.ct = dCt; v.n = n; v.kKey = dKey; v.kRank = dRank; v.S = S; v.W = dW; v.eSeg = dESeg; v.eSegRoot = dESegRoot;
+    v.ct = dCt; v.n = n; v.kKey = dKey; v.kRank = dRank; v.S = S; v.W = dW; v.kBucket = dBucket; v.eSeg = dESeg;
+    v.eSegRoot = dESegRoot;
     return v;
 }
 
 void LixState::release() {
-    void* p[] = {dCt, dKey, dRank, dW, dESeg, dESegRoot};
+    void* p[] = {dCt, dKey, dRank, dW, dBucket, dESeg, dESegRoot};
     for (void* x : p) if (x) (void)hipFree(x);
-    dCt = nullptr; dKey = nullptr; dRank = nullptr; dW = nullptr; dESeg = nullptr; dESegRoot = nullptr;
+    dCt = nullptr; dKey = nullptr; dRank = nullptr; dW = nullptr; dBucket = nullptr; dESeg = nullptr; dESegRoot = nullptr;
     trained = false;
 }
 
@@ -532,12 +595,14 @@ extern "C" int msgs_lix_train(msgs_ctx* c, int32_t segments, int64_t samples, in
     LCK(hipMalloc(&L.dKey, (segments + 1) * sizeof(uint64_t)), "knots");
     LCK(hipMalloc(&L.dRank, (segments + 1) * sizeof(int64_t)), "knots");
     LCK(hipMalloc(&L.dW, kLixParams * sizeof(float)), "weights");
+    LCK(hipMalloc(&L.dBucket, (((size_t)1 << kBucketBits) + 1) * sizeof(uint32_t)), "buckets");
     LCK(hipMalloc(&L.dESeg, segments * sizeof(uint32_t)), "bounds");
     LCK(hipMalloc(&L.dESegRoot, segments * sizeof(uint32_t)), "bounds");
     hipEvent_t e0 = c->ev0, e1 = c->ev1;
     LCK(hipEventRecord(e0, st), "event");
     k_ct<<<gridFor(n), 256, 0, st>>>(c->db.dBd, c->db.dSa, n, L.dCt);
     k_knots<<<gridFor(segments + 1), 256, 0, st>>>(L.dCt, n, segments, L.dKey, L.dRank);
+    k_buckets<<<gridFor(((int64_t)1 << kBucketBits) + 1), 256, 0, st>>>(L.dKey, segments, L.dBucket);
     std::vector<float> W;
     initWeights(W, seed);
     LCK(hipMemcpyAsync(L.dW, W.data(), kLixParams * 4, hipMemcpyHostToDevice, st), "W");
@@ -563,7 +628,7 @@ extern "C" int msgs_lix_train(msgs_ctx* c, int32_t segments, int64_t samples, in
         const int64_t off = ((int64_t)s * batch) % std::max<int64_t>(1, samples - batch + 1);
         const int32_t B = (int32_t)std::min<int64_t>(batch, samples - off);
         k_fwd<<<gridFor(B), 256, 0, st>>>(L.dW, X, T, off, B, H, Dout, Err);
-        k_grad<<<gridFor(kLixParams), 256, 0, st>>>(L.dW, X, off, B, H, Dout, Err, G, dLoss);
+        k_grad<<<kLixParams, 256, 0, st>>>(L.dW, X, off, B, H, Dout, Err, G, dLoss);
         b1t *= 0.9f; b2t *= 0.999f;
         k_adam<<<gridFor(kLixParams), 256, 0, st>>>(L.dW, G, Mo, Vo, kLixParams, (float)lr, b1t, b2t);
         if (s == 0 || s == steps - 1) {
@@ -620,7 +685,7 @@ extern "C" int msgs_lix_grad(msgs_ctx* c, const float* W, const float* X, const 
     LCK(hipMemcpyAsync(dX, X, (size_t)B * 16 * 4, hipMemcpyHostToDevice, st), "X");
     LCK(hipMemcpyAsync(dT, T, (size_t)B * 4, hipMemcpyHostToDevice, st), "T");
     k_fwd<<<gridFor(B), 256, 0, st>>>(dW, dX, dT, 0, B, H, Dout, Err);
-    k_grad<<<gridFor(kLixParams), 256, 0, st>>>(dW, dX, 0, B, H, Dout, Err, G, dL);
+    k_grad<<<kLixParams, 256, 0, st>>>(dW, dX, 0, B, H, Dout, Err, G, dL);
     LCK(hipMemcpyAsync(grad, G, kLixParams * 4, hipMemcpyDeviceToHost, st), "G");
     if (loss) LCK(hipMemcpyAsync(loss, dL, 4, hipMemcpyDeviceToHost, st), "loss");
     LCK(hipStreamSynchronize(st), "gradient");
@@ -650,11 +715,16 @@ extern "C" int msgs_lix_query(msgs_ctx* c, const uint64_t* keys, int64_t nq, int
     return MSGS_OK;
 }
 
-int runLixFilter(msgs_ctx* c, const DevReads& r, uint8_t** needOut, unsigned long long* dStats) {
+int runLixFilter(msgs_ctx* c, const DevReads& r, int32_t minLen, double T, uint8_t** needOut,
+                 unsigned long long* dStats) {
     hipError_t e = hipSuccess;
     uint8_t* need = c->bufs.get<uint8_t>("lx.need", r.n + 1, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "filter buffer");
-    if (r.n) k_filter<<<gridFor(r.n), 256, 0, c->stream>>>(c->lix.view(), r.seq, r.offs, r.n, need, dStats);
+    if (r.n) {
+        const int64_t blocks = std::min<int64_t>((r.n + 31) / 32, (int64_t)std::max(1, c->nCU) * 32);
+        k_filter<<<(unsigned)blocks, 256, 0, c->stream>>>(c->lix.view(), c->db.view(), r.seq, r.offs, r.n, minLen, T,
+                                                          need, dStats);
+    }
     LCK(hipGetLastError(), "filter");
     *needOut = need;
     return MSGS_OK;

@@ -691,12 +691,12 @@ extern "C" int msgs_align_batch(msgs_ctx* c, const msgs_reads* reads, const msgs
     return MSGS_OK;
 }
 
-/* SAM text of a whole batch (SE when mates is NULL), finished on host threads in
- * read-order chunks (part t holds the records of the t-th chunk); lineOffs[i] is where
+/* SAM text of a whole batch (SE when mates is NULL; otherwise b holds the reads followed by
+ * their mates, mate i as read n + i), finished on host threads in read-order chunks
+ * (part t holds the records of the t-th chunk); lineOffs[i] is where
  * read i's records start in the parts' concatenation.  Returns the number of mapped reads. */
 static int64_t samText(msgs_ctx* c, const msgs_params* p, const msgs_reads* reads, const msgs_reads* mates,
-                       const BatchResult& b1, const BatchResult* b2, bool emit, std::vector<std::string>& part,
-                       std::vector<int64_t>& lineOffs) {
+                       const BatchResult& b, bool emit, std::vector<std::string>& part, std::vector<int64_t>& lineOffs) {
     const int64_t n = reads->n;
     std::vector<int64_t> nm1, nm2;
     splitNames(reads->names, n, nm1);
@@ -707,7 +707,7 @@ static int64_t samText(msgs_ctx* c, const msgs_params* p, const msgs_reads* read
     std::vector<int64_t> mappedPart(nt, 0);
     auto work = [&](int t) {
         const int64_t i0 = n * t / nt, i1 = n * (t + 1) / nt;
-        Finisher F{c->db, *p, b1.dp};
+        Finisher F{c->db, *p, b.dp};
         F.emit = emit;
         std::vector<uint8_t> q30;
         auto host = [&](const msgs_reads* r, const std::vector<int64_t>& nm, int64_t i) {
@@ -726,11 +726,11 @@ static int64_t samText(msgs_ctx* c, const msgs_params* p, const msgs_reads* read
             lineOffs[i] = (int64_t)o.size();
             HostRead h1 = host(reads, nm1, i);
             if (!mates) {
-                F.se(o, h1, b1.sd.seedCnt[i], b1.ch.chainOff[i], b1.ch.chainOff[i + 1]);
+                F.se(o, h1, b.sd.seedCnt[i], b.ch.chainOff[i], b.ch.chainOff[i + 1]);
             } else {
                 HostRead h2 = host(mates, nm2, i);
-                F.pe(o, h1, h2, b1.sd.seedCnt[i], b2->sd.seedCnt[i], b1.ch.chainOff[i], b1.ch.chainOff[i + 1],
-                     b1.ch.nJobs + b2->ch.chainOff[i], b1.ch.nJobs + b2->ch.chainOff[i + 1]);
+                F.pe(o, h1, h2, b.sd.seedCnt[i], b.sd.seedCnt[n + i], b.ch.chainOff[i], b.ch.chainOff[i + 1],
+                     b.ch.chainOff[n + i], b.ch.chainOff[n + i + 1]);
             }
         }
         mappedPart[t] = F.mapped;
@@ -755,27 +755,24 @@ static void copyParts(const std::vector<std::string>& part, char* out) {
     for (const std::string& s : part) { std::memcpy(out + k, s.data(), s.size()); k += (int64_t)s.size(); }
 }
 
-/* merge the mate batch's DP results behind the first batch's so one Finisher sees both */
-static void mergeDp(DpStage& a, const DpStage& b) {
-    const int64_t nA = (int64_t)a.recs.size(), nB = (int64_t)b.recs.size();
-    const int64_t cA = nA ? a.cigOff[nA] : 0, mA = nA ? a.mdOff[nA] : 0;
-    const int64_t cB = nB ? b.cigOff[nB] : 0, mB = nB ? b.mdOff[nB] : 0;
-    a.cigar.resize(cA);
-    a.md.resize(mA);
-    if (nB) {
-        a.cigar.insert(a.cigar.end(), b.cigar.begin(), b.cigar.begin() + cB);
-        a.md.insert(a.md.end(), b.md.begin(), b.md.begin() + mB);
-    }
-    a.cigar.push_back(0);
-    a.md.push_back(0);
-    if (a.cigOff.empty()) { a.cigOff.assign(1, 0); a.mdOff.assign(1, 0); }
-    a.cigOff.resize(nA);
-    a.mdOff.resize(nA);
-    for (int64_t j = 0; j <= nB; ++j) {
-        a.cigOff.push_back(cA + (nB ? b.cigOff[j] : 0));
-        a.mdOff.push_back(mA + (nB ? b.mdOff[j] : 0));
-    }
-    a.recs.insert(a.recs.end(), b.recs.begin(), b.recs.end());
+/* reads followed by their mates as one batch (mate i becomes read n + i) */
+static void joinMates(const msgs_reads* a, const msgs_reads* b, HostBatch& out) {
+    const int64_t n = a->n;
+    const int64_t ta = n ? a->offs[n] - a->offs[0] : 0, tb = n ? b->offs[n] - b->offs[0] : 0;
+    out.seq.resize(ta + tb);
+    out.qual.resize(ta + tb);
+    if (ta) std::memcpy(out.seq.data(), a->seq + a->offs[0], ta);
+    if (tb) std::memcpy(out.seq.data() + ta, b->seq + b->offs[0], tb);
+    if (ta) { if (a->qual) std::memcpy(out.qual.data(), a->qual + a->offs[0], ta); else std::memset(out.qual.data(), 30, ta); }
+    if (tb) { if (b->qual) std::memcpy(out.qual.data() + ta, b->qual + b->offs[0], tb); else std::memset(out.qual.data() + ta, 30, tb); }
+    out.offs.assign(2 * n + 1, 0);
+    for (int64_t i = 0; i <= n; ++i) out.offs[i] = a->offs[i] - a->offs[0];
+    for (int64_t i = 1; i <= n; ++i) out.offs[n + i] = ta + b->offs[i] - b->offs[0];
+    out.view.seq = out.seq.data();
+    out.view.qual = out.qual.data();
+    out.view.offs = out.offs.data();
+    out.view.names = nullptr;
+    out.view.n = 2 * n;
 }
 
 extern "C" int msgs_map_batch(msgs_ctx* c, const msgs_reads* reads, const msgs_reads* mates, const msgs_params* p,
@@ -785,31 +782,23 @@ extern "C" int msgs_map_batch(msgs_ctx* c, const msgs_reads* reads, const msgs_r
     if (rc) return rc;
     if (!reads) return fail(c, MSGS_E_ARG, "no reads");
     if (mates && mates->n != reads->n) return fail(c, MSGS_E_ARG, "mate batches differ in size");
-    BatchResult b1, b2;
+    HostBatch joined;
+    const msgs_reads* all = reads;
+    if (mates) { joinMates(reads, mates, joined); all = &joined.view; }
+    BatchResult b;
     {
         DevReads r;
-        rc = uploadReads(c, reads, "rd", r);
+        rc = uploadReads(c, all, "rd", r);
         if (rc) return rc;
         uint8_t* dQual = nullptr;
-        rc = uploadQual(c, reads, r, "rd.qual", &dQual);
+        rc = uploadQual(c, all, r, "rd.qual", &dQual);
         if (rc) return rc;
-        rc = runAll(c, r, dQual, p, b1, nullptr, nullptr);
+        rc = runAll(c, r, dQual, p, b, nullptr, nullptr);
         if (rc) return rc;
-    }
-    if (mates) {
-        DevReads r;
-        rc = uploadReads(c, mates, "rd", r);
-        if (rc) return rc;
-        uint8_t* dQual = nullptr;
-        rc = uploadQual(c, mates, r, "rd.qual", &dQual);
-        if (rc) return rc;
-        rc = runAll(c, r, dQual, p, b2, nullptr, nullptr);
-        if (rc) return rc;
-        mergeDp(b1.dp, b2.dp);
     }
     std::vector<std::string> parts;
     std::vector<int64_t> lo;
-    samText(c, p, reads, mates, b1, mates ? &b2 : nullptr, true, parts, lo);
+    samText(c, p, reads, mates, b, true, parts, lo);
     const int64_t total = lo[reads->n];
     if (needed) *needed = total;
     if (total > cap || !out) return fail(c, MSGS_E_CAP, "output capacity");
@@ -843,71 +832,107 @@ extern "C" int msgs_batch_upload(msgs_ctx* c, int32_t slot, const msgs_reads* re
         hb.view.offs = hb.offs.data();
         hb.view.names = hb.names.empty() ? nullptr : hb.names.c_str();
         hb.view.n = n;
-        char pfx[32];
-        std::snprintf(pfx, sizeof pfx, "slot%d.%d", slot, m);
-        int rc = uploadReads(c, &hb.view, pfx, S.dev[m]);
-        if (rc) return rc;
-        std::snprintf(pfx, sizeof pfx, "slot%d.%d.q", slot, m);
-        rc = uploadQual(c, &hb.view, S.dev[m], pfx, &S.dQual[m]);
-        if (rc) return rc;
     }
+    /* on the device the mates follow the reads as one batch */
+    HostBatch joined;
+    const msgs_reads* all = &S.host[0].view;
+    if (mates) { joinMates(&S.host[0].view, &S.host[1].view, joined); all = &joined.view; }
+    char pfx[32];
+    std::snprintf(pfx, sizeof pfx, "slot%d", slot);
+    int rc = uploadReads(c, all, pfx, S.dev);
+    if (rc) return rc;
+    std::snprintf(pfx, sizeof pfx, "slot%d.q", slot);
+    rc = uploadQual(c, all, S.dev, pfx, &S.dQual);
+    if (rc) return rc;
     if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, MSGS_E_HIP, "upload");
     S.paired = mates != nullptr;
     S.used = true;
     return MSGS_OK;
 }
 
-extern "C" int msgs_batch_run(msgs_ctx* c, int32_t slot, const msgs_params* p, int32_t emitSam) {
+/* GPU stages of a resident batch, then its host finishing on the slot's worker thread */
+extern "C" int msgs_batch_submit(msgs_ctx* c, int32_t slot, const msgs_params* p, int32_t emitSam) {
     if (!c || !c->db.n) return fail(c, MSGS_E_STATE, "no database");
     if (slot < 0 || slot >= 16 || !c->slots[slot].used) return fail(c, MSGS_E_STATE, "slot not uploaded");
     int rc = checkParams(c, p);
     if (rc) return rc;
     Slot& S = c->slots[slot];
+    S.join();
     hipError_t e = hipSuccess;
     unsigned long long* dStats = c->bufs.get<unsigned long long>("stats", 8, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "stats");
     CK(hipMemsetAsync(dStats, 0, 8 * 8, c->stream), "stats");
-    auto t0 = std::chrono::steady_clock::now();
-    BatchResult b[2];
-    double seedUs = 0, smemUs = 0, dpUs = 0;
-    for (int m = 0; m < (S.paired ? 2 : 1); ++m) {
+    const auto t0 = std::chrono::steady_clock::now();
+    auto b = std::make_shared<BatchResult>();
+    double seedUs = 0, smemUs = 0, dpUs = 0, filterUs = 0;
+    {
         auto s0 = std::chrono::steady_clock::now();
-        rc = runSmem(c, S.dev[m], p, b[m].sm, dStats);
+        rc = runSmem(c, S.dev, p, b->sm, dStats);
         if (rc) return rc;
         smemUs += c->lastSmemMicros;
-        rc = runSeeds(c, S.dev[m], p, b[m].sm, b[m].sd, dStats);
+        filterUs += c->lastFilterMicros;
+        rc = runSeeds(c, S.dev, p, b->sm, b->sd, dStats);
         if (rc) return rc;
-        rc = runChains(c, S.dev[m], b[m].sd, b[m].ch);
+        rc = runChains(c, S.dev, b->sd, b->ch);
         if (rc) return rc;
         seedUs += std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - s0).count();
         double us = 0;
-        rc = runDp(c, S.dev[m], S.dQual[m], p, b[m].sd, b[m].ch, b[m].dp, &us);
+        rc = runDp(c, S.dev, S.dQual, p, b->sd, b->ch, b->dp, &us);
         if (rc) return rc;
         dpUs += us;
     }
-    if (S.paired) mergeDp(b[0].dp, b[1].dp);
-    auto h0 = std::chrono::steady_clock::now();
-    int64_t mapped = samText(c, p, &S.host[0].view, S.paired ? &S.host[1].view : nullptr, b[0], S.paired ? &b[1] : nullptr,
-                             emitSam != 0, c->lastParts, c->lastLineOffs);
-    auto t1 = std::chrono::steady_clock::now();
     unsigned long long hs[8];
     CK(hipMemcpy(hs, dStats, sizeof(hs), hipMemcpyDeviceToHost), "stats");
     int64_t cells = 0;
-    for (const AlnRec& a : b[0].dp.recs) cells += a.cells;
-    std::memset(c->stats, 0, sizeof(c->stats));
-    c->stats[0] = (double)(S.host[0].view.n * (S.paired ? 2 : 1));
-    c->stats[1] = (double)mapped;
-    c->stats[2] = (double)hs[0];
-    c->stats[3] = (double)hs[1];
-    c->stats[4] = (double)cells;
-    c->stats[5] = smemUs;
-    c->stats[6] = dpUs;
-    c->stats[7] = std::chrono::duration<double, std::micro>(t1 - t0).count();
-    c->stats[8] = seedUs;
-    c->stats[9] = std::chrono::duration<double, std::micro>(t1 - h0).count();
-    c->stats[10] = (double)hs[2];
-    c->stats[11] = (double)hs[3];
+    for (const AlnRec& a : b->dp.recs) cells += a.cells;
+    double* st = S.stats;
+    std::memset(st, 0, sizeof(S.stats));
+    st[0] = (double)(S.host[0].view.n * (S.paired ? 2 : 1));
+    st[2] = (double)hs[0];
+    st[3] = (double)hs[1];
+    st[4] = (double)cells;
+    st[5] = smemUs;
+    st[6] = dpUs;
+    st[8] = seedUs;
+    st[10] = (double)hs[2];
+    st[11] = (double)hs[3];
+    st[12] = (double)hs[5];
+    st[13] = (double)hs[6];
+    st[14] = (double)hs[7];
+    st[15] = filterUs;
+    S.params = *p;
+    S.res = b;
+    S.pending = true;
+    const bool emit = emitSam != 0;
+    S.worker = std::thread([c, &S, emit, t0]() {
+        const auto h0 = std::chrono::steady_clock::now();
+        const int64_t mapped = samText(c, &S.params, &S.host[0].view, S.paired ? &S.host[1].view : nullptr, *S.res,
+                                       emit, S.parts, S.lineOffs);
+        const auto t1 = std::chrono::steady_clock::now();
+        S.stats[1] = (double)mapped;
+        S.stats[7] = std::chrono::duration<double, std::micro>(t1 - t0).count();
+        S.stats[9] = std::chrono::duration<double, std::micro>(t1 - h0).count();
+    });
     return MSGS_OK;
+}
+
+/* wait for a submitted batch; its counters and records become the context's last run */
+extern "C" int msgs_batch_wait(msgs_ctx* c, int32_t slot) {
+    if (!c) return MSGS_E_ARG;
+    if (slot < 0 || slot >= 16) return fail(c, MSGS_E_ARG, "bad slot");
+    Slot& S = c->slots[slot];
+    if (!S.pending) return fail(c, MSGS_E_STATE, "slot has no submitted run");
+    S.join();
+    std::memcpy(c->stats, S.stats, sizeof(c->stats));
+    std::swap(c->lastParts, S.parts);
+    std::swap(c->lastLineOffs, S.lineOffs);
+    return MSGS_OK;
+}
+
+extern "C" int msgs_batch_run(msgs_ctx* c, int32_t slot, const msgs_params* p, int32_t emitSam) {
+    int rc = msgs_batch_submit(c, slot, p, emitSam);
+    if (rc) return rc;
+    return msgs_batch_wait(c, slot);
 }
 
 extern "C" int msgs_batch_stats(msgs_ctx* c, double* stats, int32_t n) {
@@ -923,5 +948,41 @@ extern "C" int msgs_batch_sam(msgs_ctx* c, char* out, int64_t cap, int64_t* need
     if (!out || total + 1 > cap) return fail(c, MSGS_E_CAP, "output capacity");
     copyParts(c->lastParts, out);
     out[total] = 0;
+    return MSGS_OK;
+}
+
+/* ------------------------------------------------------------------ autotuner */
+
+/* Time the SMEM search of resident batch `slot` for each wavefronts-per-CU setting in
+ * `cand` (lanes per CU, multiples of 64) and keep the fastest in the context.  The
+ * search's results do not depend on the setting (reads are independent; the work queue
+ * only changes which lane runs which read). */
+extern "C" int msgs_tune(msgs_ctx* c, int32_t slot, const msgs_params* p, const int32_t* cand, int32_t nCand,
+                         int32_t* best, double* bestMicros) {
+    if (!c || !c->db.n) return fail(c, MSGS_E_STATE, "no database");
+    if (slot < 0 || slot >= 16 || !c->slots[slot].used) return fail(c, MSGS_E_STATE, "slot not uploaded");
+    int rc = checkParams(c, p);
+    if (rc) return rc;
+    static const int32_t kDefault[] = {128, 256, 384, 512, 768, 1024};
+    if (!cand || nCand <= 0) { cand = kDefault; nCand = 6; }
+    Slot& S = c->slots[slot];
+    const int keep = c->lanesPerCU;
+    int32_t bestL = keep;
+    double bestUs = 1e300;
+    for (int32_t i = 0; i < nCand; ++i) {
+        if (cand[i] < 64 || cand[i] > 4096 || cand[i] % 64) { c->lanesPerCU = keep; return fail(c, MSGS_E_ARG, "lanes per CU"); }
+        c->lanesPerCU = cand[i];
+        double t = 1e300;
+        for (int rep = 0; rep < 2; ++rep) {
+            SmemStage sm;
+            rc = runSmem(c, S.dev, p, sm, nullptr);
+            if (rc) { c->lanesPerCU = keep; return rc; }
+            t = std::min(t, c->lastSmemMicros + c->lastFilterMicros);
+        }
+        if (t < bestUs) { bestUs = t; bestL = cand[i]; }
+    }
+    c->lanesPerCU = bestL;
+    if (best) *best = bestL;
+    if (bestMicros) *bestMicros = bestUs;
     return MSGS_OK;
 }

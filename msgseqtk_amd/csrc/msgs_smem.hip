@@ -7,9 +7,10 @@
  * step machine: every loop iteration of a lane performs exactly one FMD extension, so
  * all lanes of a wavefront issue their occ-block loads at the same instruction no
  * matter which phase of which search they are in.  The lock-step candidate list of a
- * search (the reference's `prev`) lives in a per-lane scratch area interleaved across
- * lanes; it is traversed in reverse of its push order and compacted in place exactly
- * like the reference's reverse + erase(remove_if).
+ * search (the reference's `prev`) lives in a per-lane scratch area (32-byte records,
+ * contiguous per lane so that consecutive candidates share cache lines); it is
+ * traversed in reverse of its push order and compacted in place exactly like the
+ * reference's reverse + erase(remove_if).
  *
  * Floating point: logP is accumulated in fp64 in the reference's order.  The e-value
  * gates N*exp(logP) <= maxE are evaluated as logP <= T, with T found on the host from
@@ -28,6 +29,7 @@ struct Lane {
     /* read */
     int64_t rid;
     const uint8_t* s;
+    const double* P;   /* loglik prefix sums of the read (NULL: sequential gates only) */
     int L;
     /* findAllSMEMS variables */
     int dFrom, dTo, stage;
@@ -63,13 +65,13 @@ __device__ __forceinline__ void emit(const Ctx& c, Lane& l, int from, int to, in
     l.nOut++;
 }
 
-__device__ __forceinline__ size_t cidx(const Ctx& c, int k) { return (size_t)k * c.a.nLanes + c.lane; }
+__device__ __forceinline__ size_t cidx(const Ctx& c, int k) { return (size_t)c.lane * c.a.candCap + k; }
 
 __device__ __forceinline__ void candStore(const Ctx& c, int k, int pos, int64_t F, int64_t R, int64_t S, double ll,
                                           int64_t X) {
-    size_t i = cidx(c, k);
-    c.a.candPos[i] = pos; c.a.candF[i] = F; c.a.candR[i] = R; c.a.candS[i] = S; c.a.candLL[i] = ll;
-    c.a.candX[i] = X;
+    SmemCand v;
+    v.F = (int32_t)F; v.R = (int32_t)R; v.S = (int32_t)S; v.X = (int32_t)X; v.LL = ll; v.pos = pos; v.pad = 0;
+    c.a.cand[cidx(c, k)] = v;
 }
 
 /* start findFwdBackSMEMS / findBackFwdSMEMS at f; false if it ends at once (ambiguous base) */
@@ -117,27 +119,165 @@ __device__ void driverAfter(const Ctx& c, Lane& l) {
     }
 }
 
-__device__ __forceinline__ bool gateFwd(const Ctx& c, const Lane& l, double ll0, int from) {
+/* loglik of base code v: A/C/G/T from registers, other codes from the table */
+__device__ __forceinline__ double llOf(const Ctx& c, uint32_t v, double lA, double lC, double lG, double lT) {
+    return v == 1 ? lA : (v == 2 ? lC : (v == 4 ? lG : (v == 8 ? lT : c.ix.ll[v])));
+}
+
+/* x0 + ll(s[i0]) + ll(s[i0+1]) + ... + ll(s[i1-1]) <= T, summed in that order; stops once
+ * the running sum is <= T (every term is <= 0).  Bases are read 8 at a time from
+ * aligned words (the read buffer is padded). */
+__device__ __forceinline__ bool gateSum(const Ctx& c, const Lane& l, double x, int i0, int i1, double T, double lA,
+                                        double lC, double lG, double lT) {
+    const int64_t off = l.s - c.a.seqs;
+    int64_t wb = -1;
+    uint64_t w = 0;
+    for (int i = i0; i < i1; ++i) {
+        if (x <= T) return true;
+        const int64_t g = off + i;
+        if ((g & ~(int64_t)7) != wb) { wb = g & ~(int64_t)7; w = *(const uint64_t*)(c.a.seqs + wb); }
+        x += llOf(c, (uint32_t)(w >> (8 * (g & 7))) & 15u, lA, lC, lG, lT);
+    }
+    return x <= T;
+}
+
+/* The gates below compare with T a sum of m <= L+1 non-positive terms.  Two summation
+ * orders of the same terms differ by at most 2 (m-1) u sum|t| (u = 2^-53; the prefix
+ * difference of gateBack adds one such bound), so a prefix-sum estimate outside
+ * T +- slack, slack = 1e-15 (L+2) (1 + |ll0| + |P[L]|), decides exactly; estimates inside
+ * the band (and NaN estimates from -inf terms) take the sequential sum in the
+ * reference's order. */
+__device__ __forceinline__ double gateSlack(const Lane& l, double ll0) {
+    return 1e-15 * (double)(l.L + 2) * (1.0 + fabs(ll0) + fabs(l.P[l.L]));
+}
+
+__device__ __forceinline__ bool gateFwd(const Ctx& c, const Lane& l, double ll0, int from, double lA, double lC,
+                                        double lG, double lT) {
     /* evalue(0, to) <= maxE of the phase-1 candidate (src/SMEM.cpp:26-40,81-82) */
     if (!c.a.gate1) return true;
+    if (l.P) {
+        const double v = ll0 + l.P[from], sl = gateSlack(l, ll0);
+        if (v < c.a.T1 - sl) return true;
+        if (v > c.a.T1 + sl) return false;
+    }
     double x = 0;
     x += ll0;
-    for (int i = 0; i < from; ++i) {
-        if (x <= c.a.T1) return true;
-        x += c.ix.ll[l.s[i] & 15];
-    }
-    return x <= c.a.T1;
+    return gateSum(c, l, x, 0, from, c.a.T1, lA, lC, lG, lT);
 }
-__device__ __forceinline__ bool gateBack(const Ctx& c, const Lane& l, double ll0, int to) {
+__device__ __forceinline__ bool gateBack(const Ctx& c, const Lane& l, double ll0, int to, double lA, double lC,
+                                         double lG, double lT) {
     /* evalue(from, L) <= maxE (src/SMEM.cpp:136-137) */
     if (!c.a.gate1) return true;
+    if (l.P) {
+        const double v = ll0 + (l.P[l.L] - l.P[to]), sl = gateSlack(l, ll0);
+        if (v < c.a.T1 - sl) return true;
+        if (v > c.a.T1 + sl) return false;
+    }
     double x = 0;
     x += ll0;
-    for (int i = to; i < l.L; ++i) {
-        if (x <= c.a.T1) return true;
-        x += c.ix.ll[l.s[i] & 15];
+    return gateSum(c, l, x, to, l.L, c.a.T1, lA, lC, lG, lT);
+}
+
+__device__ __forceinline__ uint8_t byteAt(uint64_t w0, uint64_t w1, uint64_t w2, int64_t o) {
+    const uint64_t w = o < 8 ? w0 : (o < 16 ? w1 : w2);
+    return (uint8_t)(w >> (8 * (o & 7)));
+}
+
+constexpr int kRun = 16;
+
+/*
+ * Number (<= kRun) of consecutive successful unique-interval steps from text position X:
+ * forward steps compare the complement of read base r0 + t with BD[X-1-t], backward
+ * steps read base r0 - 1 - t with BD[X-1-t]; a step needs an A/C/G/T base inside the
+ * read and X - t > 0.  Text and read bytes come from three aligned 8-byte words each
+ * (both buffers are padded past their ends).
+ */
+__device__ __forceinline__ int matchRun(const DevIndex& ix, const SmemArgs& a, const Lane& l, int64_t X, int r0,
+                                        bool fwd, double& LL, double lA, double lC, double lG, double lT) {
+    if (X <= 0) return 0;
+    const int64_t tb = X > kRun ? X - kRun : 0;
+    const int64_t tA = tb & ~(int64_t)7;
+    const uint64_t* tw = (const uint64_t*)(ix.bd + tA);
+    const uint64_t t0 = tw[0], t1 = tw[1], t2 = tw[2];
+    const int64_t off = l.s - a.seqs;
+    const int64_t rb = fwd ? off + r0 : off + (r0 > kRun ? r0 - kRun : 0);
+    const int64_t rA = rb & ~(int64_t)7;
+    const uint64_t* rw = (const uint64_t*)(a.seqs + rA);
+    const uint64_t q0 = rw[0], q1 = rw[1], q2 = rw[2];
+    int t = 0;
+    for (; t < kRun; ++t) {
+        if (X - t <= 0) break;
+        const int rp = fwd ? r0 + t : r0 - 1 - t;
+        if (rp < 0 || rp >= l.L) break;
+        const uint8_t b = byteAt(q0, q1, q2, off + rp - rA);
+        if (amb(b)) break;
+        const uint8_t tx = byteAt(t0, t1, t2, X - 1 - t - tA);
+        if (tx != (fwd ? comp16(b) : b)) break;
+        LL += b == 1 ? lA : (b == 2 ? lC : (b == 4 ? lG : lT));
     }
-    return x <= c.a.T1;
+    return t;
+}
+
+/* bytes [a, a+8) and [a+8, a+16) of a padded buffer, from three aligned words */
+__device__ __forceinline__ void load16(const uint8_t* buf, int64_t a, uint64_t& u0, uint64_t& u1) {
+    const uint64_t* w = (const uint64_t*)(buf + (a & ~(int64_t)7));
+    const uint64_t w0 = w[0], w1 = w[1], w2 = w[2];
+    const uint32_t sh = 8u * (uint32_t)(a & 7);
+    u0 = sh ? (w0 >> sh) | (w1 << (64 - sh)) : w0;
+    u1 = sh ? (w1 >> sh) | (w2 << (64 - sh)) : w1;
+}
+
+/* per byte: 0 if the byte is one of the nt16 codes A, C, G, T (1, 2, 4, 8), else nonzero */
+__device__ __forceinline__ uint64_t notBasic8(uint64_t v) {
+    uint64_t x = v - ((v >> 1) & 0x5555555555555555ull);
+    x = (x & 0x3333333333333333ull) + ((x >> 2) & 0x3333333333333333ull);
+    x = (x + (x >> 4)) & 0x0f0f0f0f0f0f0f0full;   /* per-byte popcount */
+    return (x ^ 0x0101010101010101ull) | (v & 0xf0f0f0f0f0f0f0f0ull);
+}
+
+/* per byte: complement of a basic nt16 code (bit reversal of the low nibble) */
+__device__ __forceinline__ uint64_t comp8(uint64_t v) {
+    return ((v & 0x0101010101010101ull) << 3) | ((v & 0x0202020202020202ull) << 1) |
+           ((v & 0x0404040404040404ull) >> 1) | ((v & 0x0808080808080808ull) >> 3);
+}
+
+/* leading run (<= kRun) of matching steps, 8 bases per word: text byte X-1-t against
+ * read base r0+t (forward, complemented) or r0-1-t (backward); same steps and LL sums as
+ * matchRun.  Falls back to matchRun near the start of the text or of the read buffer. */
+__device__ __forceinline__ int matchRunW(const DevIndex& ix, const SmemArgs& a, const Lane& l, int64_t X, int r0,
+                                         bool fwd, double& LL, double lA, double lC, double lG, double lT) {
+    const int64_t off = l.s - a.seqs;
+    if (X < kRun || (!fwd && off + r0 < kRun)) return matchRun(ix, a, l, X, r0, fwd, LL, lA, lC, lG, lT);
+    uint64_t t0, t1, q0, q1;
+    load16(ix.bd, X - kRun, t0, t1);
+    /* text in step order: t = 0 is the byte at X-1 */
+    const uint64_t T[2] = {__builtin_bswap64(t1), __builtin_bswap64(t0)};
+    uint64_t R[2];
+    if (fwd) {
+        load16(a.seqs, off + r0, q0, q1);
+        R[0] = q0; R[1] = q1;
+    } else {
+        load16(a.seqs, off + r0 - kRun, q0, q1);
+        R[0] = __builtin_bswap64(q1); R[1] = __builtin_bswap64(q0);
+    }
+    /* steps allowed by the read ends and the text start */
+    const int64_t lim = min((int64_t)kRun, min(fwd ? (int64_t)(l.L - r0) : (int64_t)r0, X));
+    int m = 0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        if (m == 8 * h) {
+            const uint64_t v = R[h];
+            const uint64_t bad = notBasic8(v) | (T[h] ^ (fwd ? comp8(v) : v));
+            const int run = bad ? (__builtin_ctzll(bad) >> 3) : 8;
+            m += run;
+        }
+    }
+    if (m > lim) m = (int)lim;
+    for (int t = 0; t < m; ++t) {
+        const uint32_t b = (uint32_t)((t < 8 ? R[0] : R[1]) >> (8 * (t & 7))) & 0xffu;
+        LL += b == 1 ? lA : (b == 2 ? lC : (b == 4 ? lG : lT));
+    }
+    return m;
 }
 
 __device__ __forceinline__ void startIteration(Lane& l) {
@@ -156,6 +296,7 @@ __device__ bool nextRead(const Ctx& c, Lane& l) {
         const int64_t off = c.a.offs[rid];
         l.s = c.a.seqs + off;
         l.L = (int)(c.a.offs[rid + 1] - off);
+        l.P = c.a.prefLL ? c.a.prefLL + off + rid : nullptr;
         l.nOut = 0;
         if (l.L > 0 && (!c.a.need || c.a.need[rid])) {
             l.dFrom = l.L / 2;
@@ -181,16 +322,19 @@ __device__ bool nextRead(const Ctx& c, Lane& l) {
  * identical to the occ-block path; only the memory traffic differs.
  */
 template <bool FAST>
-__global__ void __launch_bounds__(256) smem_kernel_t(DevIndex ix, SmemArgs a) {
+__global__ void __launch_bounds__(256, 4) smem_kernel_t(DevIndex ix, SmemArgs a) {
     Lane l;
     l.nExt = 0; l.nFast = 0; l.nLook = 0;
     l.state = ST_IDLE;
     Ctx c{ix, a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x};
     const int64_t minLen = a.minLen;
+    const double llA = ix.ll[1], llC = ix.ll[2], llG = ix.ll[4], llT = ix.ll[8];
+    uint32_t nIter = 0;
     while (true) {
         if (l.state == ST_IDLE) {
             if (!nextRead(c, l)) break;
         }
+        ++nIter;
         /* ---- operands of this step's extension ---- */
         const int st = l.state;
         const bool phase1 = (st == ST_FB1 || st == ST_BF1);
@@ -201,12 +345,27 @@ __global__ void __launch_bounds__(256) smem_kernel_t(DevIndex ix, SmemArgs a) {
         if (phase1) {
             P = l.cF; Q = l.cR; S = l.cS; X = l.cX; LL = l.cLL; mFrom = l.cFrom; mTo = l.cTo;
         } else {
-            size_t i = cidx(c, l.k);
-            pos = a.candPos[i]; P = a.candF[i]; Q = a.candR[i]; S = a.candS[i]; LL = a.candLL[i];
-            X = FAST ? a.candX[i] : -1;
+            const SmemCand cd = a.cand[cidx(c, l.k)];
+            pos = cd.pos; P = cd.F; Q = cd.R; S = cd.S; LL = cd.LL;
+            X = FAST ? cd.X : -1;
             if (st == ST_FB2) { mFrom = l.sFrom; mTo = pos; } else { mFrom = pos; mTo = l.sTo; }
         }
         const bool fwd = (st == ST_FB1 || st == ST_BF2);
+        /* unique interval with no other candidate in play: run over matching text bytes;
+         * each of those steps keeps S = 1 and so pushes, emits and drops nothing */
+        if (FAST && S == 1 && (phase1 || (l.k == l.lo && l.lo == l.nCand - 1))) {
+            if (X < 0) { X = ix.sa[fwd ? Q : P]; l.nLook++; }
+            const int m = matchRunW(ix, a, l, X, fwd ? mTo : mFrom, fwd, LL, llA, llC, llG, llT);
+            if (m > 0) {
+                X -= m;
+                if (fwd) mTo += m; else mFrom -= m;
+                l.nFast += m;
+                if (st == ST_FB1) { l.cTo = mTo; l.cLL = LL; l.cX = X; }
+                else if (st == ST_BF1) { l.cFrom = mFrom; l.cLL = LL; l.cX = X; }
+                else if (st == ST_FB2) l.sFrom = mFrom;
+                else l.sTo = mTo;
+            }
+        }
         const uint8_t b = fwd ? base(l, mTo) : (mFrom > 0 ? base(l, mFrom - 1) : (uint8_t)0);
         int64_t F0 = P, R0 = Q;
         const int64_t S0 = S, X0 = X;
@@ -223,8 +382,8 @@ __global__ void __launch_bounds__(256) smem_kernel_t(DevIndex ix, SmemArgs a) {
             else dBackExt(ix, P, Q, S, b);
             X = -1;
         }
-        if (fwd) { if (mTo < l.L) LL += ix.ll[b]; mTo++; }
-        else { if (mFrom > 0) LL += ix.ll[b]; mFrom--; }
+        if (fwd) { if (mTo < l.L) LL += llOf(c, b, llA, llC, llG, llT); mTo++; }
+        else { if (mFrom > 0) LL += llOf(c, b, llA, llC, llG, llT); mFrom--; }
         /* the pre-step SMEM with its moving side turned back into a row */
         auto exact0 = [&]() {
             if (FAST && X0 >= 0) {
@@ -236,7 +395,7 @@ __global__ void __launch_bounds__(256) smem_kernel_t(DevIndex ix, SmemArgs a) {
 
         /* ---- control ---- */
         if (st == ST_FB1) {
-            if (S != S0 && l.cTo >= minLen && gateFwd(c, l, LL0, l.cFrom)) {
+            if (S != S0 && l.cTo >= minLen && gateFwd(c, l, LL0, l.cFrom, llA, llC, llG, llT)) {
                 exact0();
                 candStore(c, l.nCand++, l.cTo, F0, R0, S0, LL0, -1);
             }
@@ -247,7 +406,7 @@ __global__ void __launch_bounds__(256) smem_kernel_t(DevIndex ix, SmemArgs a) {
                 else { l.state = ST_FB2; l.lo = 0; startIteration(l); }
             }
         } else if (st == ST_BF1) {
-            if (S != S0 && (int64_t)(l.L - l.cFrom) >= minLen && gateBack(c, l, LL0, l.cTo)) {
+            if (S != S0 && (int64_t)(l.L - l.cFrom) >= minLen && gateBack(c, l, LL0, l.cTo, llA, llC, llG, llT)) {
                 exact0();
                 candStore(c, l.nCand++, l.cFrom, F0, R0, S0, LL0, -1);
             }
@@ -287,9 +446,266 @@ __global__ void __launch_bounds__(256) smem_kernel_t(DevIndex ix, SmemArgs a) {
         atomicAdd(&a.stats[0], (unsigned long long)l.nExt);
         atomicAdd(&a.stats[2], (unsigned long long)l.nFast);
         atomicAdd(&a.stats[3], (unsigned long long)l.nLook);
+        /* lane iterations, and the wavefront's (its busiest lane's) iterations */
+        uint32_t w = nIter;
+        for (int off = 32; off >= 1; off >>= 1) w = max(w, (uint32_t)__shfl_xor((int)w, off, 64));
+        atomicAdd(&a.stats[5], (unsigned long long)nIter);
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&a.stats[6], (unsigned long long)w);
+            atomicMax(&a.stats[7], (unsigned long long)w);
+        }
     }
 }
 
 
 template __global__ void smem_kernel_t<false>(DevIndex, SmemArgs);
 template __global__ void smem_kernel_t<true>(DevIndex, SmemArgs);
+
+/* per read, P[0] = 0 and P[i+1] = P[i] + loglik(base i) (the gates' estimates) */
+__global__ void __launch_bounds__(256) smem_prefix_kernel(DevIndex ix, SmemArgs a, double* pref) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.nReads; r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t o = a.offs[r], L = a.offs[r + 1] - o;
+        double* P = pref + o + r;
+        double x = 0;
+        P[0] = 0;
+        for (int64_t i = 0; i < L; ++i) {
+            x += ix.ll[a.seqs[o + i] & 15];
+            P[i + 1] = x;
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ one read per lane group */
+
+namespace {
+
+/* a group's shuffle source: lane `src` of the wavefront (all lanes of the group active) */
+__device__ __forceinline__ int64_t shfl64(int64_t v, int src) {
+    const int lo = __shfl((int)(uint32_t)(uint64_t)v, src, 64);
+    const int hi = __shfl((int)(uint32_t)((uint64_t)v >> 32), src, 64);
+    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+/* the group leader takes reads from the queue until one has a running search; the group
+ * follows with the same values (false when the queue is empty) */
+template <int G>
+__device__ bool nextReadGroup(const Ctx& c, Lane& l, int gbase, int gl) {
+    while (true) {
+        int64_t rid = 0;
+        if (gl == 0) rid = (int64_t)atomicAdd(c.a.queue, 1ull);
+        rid = shfl64(rid, gbase);
+        if (rid >= c.a.nReads) return false;
+        l.rid = rid;
+        const int64_t off = c.a.offs[rid];
+        l.s = c.a.seqs + off;
+        l.L = (int)(c.a.offs[rid + 1] - off);
+        l.P = c.a.prefLL ? c.a.prefLL + off + rid : nullptr;
+        l.nOut = 0;
+        if (l.L > 0 && (!c.a.need || c.a.need[rid])) {
+            l.dFrom = l.L / 2;
+            l.dTo = l.dFrom + 1;
+            l.stage = STG_A;
+            if (startSearch(c, l, l.dFrom, true)) return true;
+            driverAfter(c, l);
+            if (l.state != ST_IDLE) return true;
+        }
+        if (gl == 0) c.a.cnt[rid] = l.nOut;
+    }
+}
+
+}  // namespace
+
+/*
+ * The SMEM search with one read per group of G lanes (64 / G reads per wavefront).  The
+ * read's state is uniform across its group.  A phase-1 step (the growing SMEM) is one
+ * extension that all lanes of the group compute alike; a phase-2 step extends up to G
+ * candidates of the lock-step list at once, lane i taking list entry i of the chunk.
+ * The reference's sequential rules for one pass (src/SMEM.cpp:93-113, 146-166) become
+ * per-candidate tests because sizes along the list are non-decreasing (longer strings
+ * have no more occurrences, and all candidates extend by the same base):
+ *   keep_i = S'_i >= 1 and S'_i != S'_{i-1}          (the reference's s0 test)
+ *   emit_i = S'_i != S_i and S_i >= 1 and S'_{i-1} < 1 (nValid == 0) and len, e-value,
+ * with S'_{-1} = -1; the next chunk continues from the last S' of the previous one.
+ * Emitted SMEMs keep the list order (ballot prefix counts), and kept candidates are
+ * compacted to the front of the list in order, as erase(remove_if) does.  A lone unique
+ * candidate walks the text as in the per-lane kernel.  Output equals smem_kernel_t's.
+ */
+template <int G>
+__global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a) {
+    static_assert(G == 4 || G == 8 || G == 16 || G == 32, "group size");
+    const int lane = threadIdx.x & 63;
+    const int gl = lane % G, gbase = lane - gl;
+    const uint64_t gmask = (G == 64 ? ~0ull : ((1ull << G) - 1)) << gbase;
+    const uint64_t below = (1ull << lane) - 1;
+    Lane l;
+    l.nExt = 0; l.nFast = 0; l.nLook = 0;
+    l.state = ST_IDLE;
+    l.nCand = 0; l.lo = 0; l.k = 0; l.w = 0; l.s0 = -1; l.nValid = 0;
+    Ctx c{ix, a, ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G};
+    const int64_t minLen = a.minLen;
+    const double llA = ix.ll[1], llC = ix.ll[2], llG = ix.ll[4], llT = ix.ll[8];
+    uint32_t nIter = 0;
+    while (true) {
+        if (l.state == ST_IDLE) {
+            if (!nextReadGroup<G>(c, l, gbase, gl)) break;
+        }
+        ++nIter;
+        const int st = l.state;
+        if (st == ST_FB1 || st == ST_BF1) {
+            /* ---- phase 1: one extension, alike in every lane of the group ---- */
+            int64_t P = l.cF, Q = l.cR, S = l.cS, X = l.cX;
+            double LL = l.cLL;
+            int mFrom = l.cFrom, mTo = l.cTo;
+            const bool fwd = st == ST_FB1;
+            if (S == 1) {
+                if (X < 0) { X = ix.sa[fwd ? Q : P]; if (gl == 0) l.nLook++; }
+                const int m = matchRunW(ix, a, l, X, fwd ? mTo : mFrom, fwd, LL, llA, llC, llG, llT);
+                if (m > 0) {
+                    X -= m;
+                    if (fwd) mTo += m; else mFrom -= m;
+                    if (gl == 0) l.nFast += m;
+                    if (fwd) l.cTo = mTo; else l.cFrom = mFrom;
+                    l.cLL = LL; l.cX = X;
+                }
+            }
+            const uint8_t b = fwd ? base(l, mTo) : (mFrom > 0 ? base(l, mFrom - 1) : (uint8_t)0);
+            int64_t F0 = P, R0 = Q;
+            const int64_t S0 = S, X0 = X;
+            const double LL0 = LL;
+            if (S == 1) {
+                if (X < 0) { X = ix.sa[fwd ? Q : P]; if (gl == 0) l.nLook++; }
+                const uint8_t bb = fwd ? comp16(b) : b;
+                if (!amb(b) && X > 0 && ix.bd[X - 1] == bb) { X--; if (gl == 0) l.nFast++; }
+                else S = 0;
+            } else {
+                if (gl == 0) l.nExt += amb(b) ? 0 : 1;
+                if (fwd) dBackExt(ix, Q, P, S, comp16(b));
+                else dBackExt(ix, P, Q, S, b);
+                X = -1;
+            }
+            if (fwd) { if (mTo < l.L) LL += llOf(c, b, llA, llC, llG, llT); mTo++; }
+            else { if (mFrom > 0) LL += llOf(c, b, llA, llC, llG, llT); mFrom--; }
+            const bool push = fwd ? (S != S0 && l.cTo >= minLen && gateFwd(c, l, LL0, l.cFrom, llA, llC, llG, llT))
+                                  : (S != S0 && (int64_t)(l.L - l.cFrom) >= minLen &&
+                                     gateBack(c, l, LL0, l.cTo, llA, llC, llG, llT));
+            if (push) {
+                if (X0 >= 0) {
+                    const int64_t v = ix.isa[X0];
+                    if (gl == 0) l.nLook++;
+                    if (fwd) R0 = v; else F0 = v;
+                }
+                if (gl == 0) candStore(c, l.nCand, fwd ? l.cTo : l.cFrom, F0, R0, S0, LL0, -1);
+                l.nCand++;
+            }
+            if (fwd) l.cTo = mTo; else l.cFrom = mFrom;
+            l.cF = P; l.cR = Q; l.cS = S; l.cLL = LL; l.cX = X;
+            if (!(S >= 1)) {
+                if (fwd) l.sTo = mTo; else l.sFrom = mFrom;
+                if (l.nCand == 0) {
+                    if (fwd) l.sFrom--; else l.sTo++;
+                    driverAfter(c, l);
+                } else {
+                    l.state = fwd ? ST_FB2 : ST_BF2;
+                    l.lo = l.nCand - 1;   /* slot of list entry 0 (the last push) */
+                    l.k = 0; l.w = 0; l.s0 = -1;
+                }
+            }
+        } else {
+            /* ---- phase 2: a chunk of the lock-step list, one candidate per lane ---- */
+            const bool fb = st == ST_FB2;
+            const bool fwd = !fb;
+            const int i = l.k + gl;
+            const bool act = i < l.nCand;
+            int pos = 0;
+            int64_t P = 0, Q = 0, S = 0, X = -1;
+            double LL = 0;
+            if (act) {
+                const SmemCand cd = a.cand[cidx(c, l.lo - i)];
+                pos = cd.pos; P = cd.F; Q = cd.R; S = cd.S; X = cd.X; LL = cd.LL;
+            }
+            /* a lone unique candidate walks the text; the whole group follows its boundary */
+            if (l.nCand == 1 && l.k == 0) {
+                int m = 0;
+                if (gl == 0 && S == 1) {
+                    if (X < 0) { X = ix.sa[fwd ? Q : P]; l.nLook++; }
+                    m = matchRunW(ix, a, l, X, fwd ? l.sTo : l.sFrom, fwd, LL, llA, llC, llG, llT);
+                    X -= m;
+                    l.nFast += m;
+                }
+                m = __shfl(m, gbase, 64);
+                if (fb) l.sFrom -= m; else l.sTo += m;
+            }
+            int mFrom = fb ? l.sFrom : pos, mTo = fb ? pos : l.sTo;
+            const uint8_t b = fwd ? base(l, mTo) : (mFrom > 0 ? base(l, mFrom - 1) : (uint8_t)0);
+            int64_t F0 = P, R0 = Q;
+            const int64_t S0 = S, X0 = X;
+            const double LL0 = LL;
+            if (act) {
+                if (S == 1) {
+                    if (X < 0) { X = ix.sa[fwd ? Q : P]; l.nLook++; }
+                    const uint8_t bb = fwd ? comp16(b) : b;
+                    if (!amb(b) && X > 0 && ix.bd[X - 1] == bb) { X--; l.nFast++; }
+                    else S = 0;
+                } else {
+                    l.nExt += amb(b) ? 0 : 1;
+                    if (fwd) dBackExt(ix, Q, P, S, comp16(b));
+                    else dBackExt(ix, P, Q, S, b);
+                    X = -1;
+                }
+                if (fwd) { if (mTo < l.L) LL += llOf(c, b, llA, llC, llG, llT); }
+                else { if (mFrom > 0) LL += llOf(c, b, llA, llC, llG, llT); }
+            }
+            /* S' of the previous list entry */
+            const int64_t up = shfl64(S, gl == 0 ? lane : lane - 1);
+            const int64_t prev = gl == 0 ? (int64_t)l.s0 : up;
+            const int len0 = fb ? (pos - l.sFrom) : (l.sTo - pos);
+            const bool keep = act && S >= 1 && S != prev;
+            const bool em = act && S != S0 && S0 >= 1 && prev < 1 && len0 >= minLen && LL0 <= a.T2;
+            const uint64_t kMask = __ballot(keep) & gmask, eMask = __ballot(em) & gmask;
+            if (em) {
+                if (X0 >= 0) {
+                    const int64_t v = ix.isa[X0];
+                    l.nLook++;
+                    if (fwd) R0 = v; else F0 = v;
+                }
+                const int slot = l.nOut + __popcll(eMask & below);
+                if (slot < a.cap) {
+                    SmemRec* o = a.out + (size_t)l.rid * a.cap + slot;
+                    if (fb) { o->from = l.sFrom; o->to = pos; } else { o->from = pos; o->to = l.sTo; }
+                    o->fwd = F0; o->rev = R0; o->size = S0; o->logP = LL0;
+                }
+            }
+            if (keep) candStore(c, l.lo - (l.w + __popcll(kMask & below)), pos, P, Q, S, LL, X);
+            l.nOut += __popcll(eMask);
+            l.w += __popcll(kMask);
+            const int last = min(G, l.nCand - l.k) - 1;
+            l.s0 = (int)shfl64(S, gbase + last);
+            l.k += G;
+            if (l.k >= l.nCand) {
+                l.nCand = l.w;
+                bool more;
+                if (fb) { l.sFrom--; more = l.sFrom >= 0; }
+                else { l.sTo++; more = l.sTo <= l.L; }
+                more = more && l.nCand > 0;
+                if (more) { l.k = 0; l.w = 0; l.s0 = -1; }
+                else driverAfter(c, l);
+            }
+        }
+        if (l.state == ST_IDLE && gl == 0) a.cnt[l.rid] = l.nOut;
+    }
+    if (a.stats) {
+        atomicAdd(&a.stats[0], (unsigned long long)l.nExt);
+        atomicAdd(&a.stats[2], (unsigned long long)l.nFast);
+        atomicAdd(&a.stats[3], (unsigned long long)l.nLook);
+        uint32_t w = nIter;
+        for (int off = 32; off >= 1; off >>= 1) w = max(w, (uint32_t)__shfl_xor((int)w, off, 64));
+        if (gl == 0) atomicAdd(&a.stats[5], (unsigned long long)nIter);
+        if ((threadIdx.x & 63) == 0) {
+            atomicAdd(&a.stats[6], (unsigned long long)w);
+            atomicMax(&a.stats[7], (unsigned long long)w);
+        }
+    }
+}
+
+template __global__ void smem_group_kernel<8>(DevIndex, SmemArgs);
+template __global__ void smem_group_kernel<16>(DevIndex, SmemArgs);

@@ -112,3 +112,27 @@ def test_resident_batch(env):
     assert text == "".join(sam1)
     mapped = sum(1 for line in text.splitlines() if not (int(line.split("\t")[1]) & 4))
     assert s[0] == 500 and s[1] == mapped
+
+
+def test_pipelined_batches(env):
+    """submit/wait: batch k's host finishing overlaps batch k+1's GPU stages; every batch's
+    SAM (SE and PE slots mixed) equals the oracle's"""
+    chroms, names, odb, m = env
+    pe = synth.simulate_pairs(chroms, 150, 120, insert=(350, 50), sub=0.01, indel=0.001, seed=7)
+    (s1, q1, n1), (s2, q2, n2) = pe
+    se = synth.simulate_reads(chroms, 300, (60, 200), sub=0.02, indel=0.002, seed=8, offtarget=0.1)
+    pm_pe = M.default_params(insMean=350, insSD=50, insMin=350 - 500, insMax=350 + 500)
+    pm_se = M.default_params()
+    m.upload(1, (s1, q1, n1), (s2, q2, n2))
+    m.upload(2, (se[0], se[1], se[2]))
+    exp_pe, _ = O.orc_map(odb, O.pe_params(350, 50), s1, q1, n1, mates=(s2, q2, n2))
+    exp_se, _ = O.orc_map(odb, O.default_params(), se[0], se[1], se[2])
+    for rep in range(3):
+        m.submit(1, pm_pe, True)
+        m.submit(2, pm_se, True)
+        m.wait(1)
+        assert m.sam() == "".join(exp_pe), rep
+        m.wait(2)
+        assert m.sam() == "".join(exp_se), rep
+    with pytest.raises(RuntimeError):
+        m.wait(2)   # nothing pending

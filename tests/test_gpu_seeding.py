@@ -67,16 +67,20 @@ def test_ext_parity(dbs):
 
 
 def _check_smems(odb, m, seqs, minLen, maxE):
-    """both device forms: plain occ-block extensions (0) and the unique-interval text path (1)"""
+    """every device form: plain occ-block extensions (0), the unique-interval text path
+    (1), and lane groups of 8 and 16 per read (2; no learned index trained here)"""
     p = M.default_params(minLen=minLen, maxEvalue=maxE)
     c1, r1, l1 = O.orc_smems(odb, seqs, minLen, maxE)
-    for form in (0, 1):
+    for form, group in ((0, 0), (1, 0), (2, 8), (2, 16)):
         m.set("form", form)
+        if group:
+            m.set("group_lanes", group)
         c2, r2, l2 = m.smems(seqs, p)
-        assert np.array_equal(c1, c2), form
-        assert np.array_equal(r1, r2), form
-        assert np.array_equal(l1.view(np.int64), l2.view(np.int64)), form
+        assert np.array_equal(c1, c2), (form, group)
+        assert np.array_equal(r1, r2), (form, group)
+        assert np.array_equal(l1.view(np.int64), l2.view(np.int64)), (form, group)
     m.set("form", 1)
+    m.set("group_lanes", 8)
 
 
 @pytest.mark.parametrize("minLen,maxE", [(17, 0.01), (10, 1.0), (3, float("inf")), (30, 1e-9), (17, 0.0)])
