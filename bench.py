@@ -41,10 +41,13 @@ def parse():
     ap.add_argument("--no-compare", action="store_true", help="skip the plain-FMD comparison run")
     ap.add_argument("--no-tune", action="store_true", help="keep the default lanes per CU")
     ap.add_argument("--no-torch", action="store_true", help="no torch device synchronisation (profiler runs)")
+    ap.add_argument("--no-pipeline", action="store_true", help="finish each batch before submitting the next")
+    ap.add_argument("--host-threads", type=int, default=0, help="host finishing threads (0: the library default)")
     ap.add_argument("--form", type=int, default=2,
                     help="0 plain FMD extensions, 1 unique-interval text path, 2 form 1 + learned-index read filter")
     ap.add_argument("--lix-segments", type=int, default=1 << 20, help="learned index: root segments")
     ap.add_argument("--lix-steps", type=int, default=1000, help="learned index: training steps")
+    ap.add_argument("--no-gather", action="store_true", help="skip the random-gather ceiling measurement")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 1 GiB device copy (counter calibration under rocprofv3 --pmc)")
     return ap.parse_args()
@@ -96,8 +99,12 @@ def main():
     tuned = None
     if not a.no_tune:
         lanes, tus = m.tune(0, p)
-        tuned = {"lanes_per_cu": lanes, "smem_us": round(tus, 1)}
-        log("autotuner: %d lanes per CU (%.0f us for the SMEM stage of batch 0)" % (lanes, tus))
+        tuned = {"lanes_per_cu": lanes, "group_lanes": m.get("group_lanes"), "smem_us": round(tus, 1)}
+        log("autotuner: %d lanes per CU, %d lanes per read (%.0f us for the SMEM stage of batch 0)" % (
+            lanes, tuned["group_lanes"], tus))
+    run_steps.serial = a.no_pipeline
+    if a.host_threads:
+        m.set("host_threads", a.host_threads)
     elapsed, S = run_steps(m, p, n_batches, a.steps, a.warmup, sync, log)
     reads, mapped = S[:, 0].sum(), S[:, 1].sum()
     value = mapped / elapsed
@@ -136,8 +143,17 @@ def main():
                               "smem_kernel": smem_us / a.steps / 1000, "seed_stage": S[:, 8].sum() / a.steps / 1000,
                               "dp_kernel": dp_us / a.steps / 1000, "host_finish": S[:, 9].sum() / a.steps / 1000,
                               "whole": S[:, 7].sum() / a.steps / 1000},
+        "host_side_ms_per_step": dict(run_steps.split, **{
+            "gpu_stages_wall": S[:, 16].sum() / a.steps / 1000, "smem_wall": S[:, 17].sum() / a.steps / 1000,
+            "seeds_wall": S[:, 18].sum() / a.steps / 1000, "chains_wall": S[:, 19].sum() / a.steps / 1000,
+            "dp_wall": S[:, 20].sum() / a.steps / 1000,
+            "dp_pools_ready": S[:, 23].sum() / a.steps / 1000, "dp_records_back": S[:, 24].sum() / a.steps / 1000,
+            "dp_cigar_md_back": S[:, 25].sum() / a.steps / 1000}),
+        "dp_jobs_per_step": {"all": float(S[:, 21].mean()), "wavefront_fill": float(S[:, 22].mean())},
         "index_build_s": round(build_s, 2),
-        "learned_index": lix,
+        "learned_index": dict(lix or {}, **({"exact_path_reads_frac": float(S[:, 26].sum() / max(1.0, S[:, 0].sum())),
+                                              "filtered_reads_frac": float(S[:, 27].sum() / max(1.0, S[:, 0].sum()))}
+                                             if lix else {})),
         "autotune": tuned,
         "mapped_frac": round(mapped / reads, 4),
     }
@@ -157,7 +173,19 @@ def main():
         log("plain FMD form: %.0f mapped reads/s; learned-index form / plain = %.3f" % (v0, value / v0))
         m.set("form", a.form)
         if tuned:
-            m.set("lanes_per_cu", tuned["lanes_per_cu"])
+            m.set("lanes_per_cu", tuned["lanes_per_cu"]).set("group_lanes", tuned["group_lanes"])
+    if not a.no_gather:
+        # random 64-B line gathers over buffers the size of the occ table and of occ + SA + ISA:
+        # the random-access ceiling the SMEM kernel's lookups are held against
+        occ_bytes = 64 * (int(m.n) // 128 + 1)
+        g = {}
+        for name, nb in (("occ_table", occ_bytes), ("occ_sa_isa", occ_bytes + 8 * int(m.n))):
+            nb = (nb + 63) // 64 * 64
+            us = m.random_gather(nb, 1 << 27)
+            g[name] = {"buffer_bytes": nb, "gbs": round(64.0 * (1 << 27) / us / 1e3, 1)}
+        out["roofline"]["random_gather_ceiling"] = g
+        out["roofline"]["frac_of_gather_ceiling"] = round(achieved / g["occ_sa_isa"]["gbs"], 4)
+        log("random 64-B gather ceiling: %s" % g)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(m, chroms, names, batches[0], p, a.cpu_pairs)
     if a.calib_copy:
@@ -168,6 +196,7 @@ def main():
 
 
 def run_steps(m, p, nb, steps, warmup, sync, log):
+    # (run_steps.serial: submit each batch only after the previous one is finished)
     """Map `steps` batches (batch k = resident slot k % nb).  With two or more resident
     batches the runs are pipelined: batch k's host finishing (records, MAPQ, SAM text)
     overlaps batch k+1's GPU stages.  Returns (seconds, per-step counters)."""
@@ -180,17 +209,25 @@ def run_steps(m, p, nb, steps, warmup, sync, log):
     stats = []
     t = time.time()
     prev = None
+    tsub = twait = 0.0
     for k in range(steps):
         slot = k % nb
-        if nb == 1 and prev is not None:
+        if (nb == 1 or run_steps.serial) and prev is not None:
             stats.append(m.wait(prev))
             prev = None
+        t0 = time.time()
         m.submit(slot, p, True)
+        t1 = time.time()
         if prev is not None:
             stats.append(m.wait(prev))
+        twait += time.time() - t1
+        tsub += t1 - t0
         prev = slot
+    t1 = time.time()
     stats.append(m.wait(prev))
+    twait += time.time() - t1
     sync()
+    run_steps.split = {"submit_ms_per_step": 1000 * tsub / steps, "wait_ms_per_step": 1000 * twait / steps}
     return time.time() - t, np.array(stats)
 
 

@@ -62,10 +62,16 @@ void msgs_ctx_destroy(msgs_ctx* ctx);
 const char* msgs_last_error(msgs_ctx* ctx);
 /* Context options (no reference counterpart; tuning knobs of the device path):
  * "form" 0 = plain FMD extensions, 1 = unique intervals followed through the text (default),
- *   2 = form 1 plus the learned-index read filter (needs msgs_lix_train);
+ *   2 = form 1 plus the learned index in front of the search (needs msgs_lix_train): reads
+ *   that occur exactly once with unique halves get their SMEM list from two learned-index
+ *   lower bounds, and (with "read_filter" 1, the default) reads in which no SMEM can occur
+ *   are skipped;
+ * "group_lanes" lanes per read of the form-2 search kernel (0 = one lane per read, 8, 16);
  * "lanes_per_cu" persistent lanes of the SMEM kernel per CU; "smem_cap" SMEM slots per
- * read before a regrow; "host_threads" threads of the host finishing step. */
+ * read before a regrow; "host_threads" threads of the host finishing step.  Results never
+ * depend on these settings. */
 int msgs_ctx_set(msgs_ctx* ctx, const char* key, int64_t value);
+int msgs_ctx_get(msgs_ctx* ctx, const char* key, int64_t* value);
 
 /* ---- database: MetaGenome + FMDIndex resident in HBM ----
  * Replaces MetaGenome::load/loadSeq + FMDIndex::load (src/MetaGenome.cpp:42-73,
@@ -132,7 +138,12 @@ int msgs_align_batch(msgs_ctx* ctx, const msgs_reads* reads, const msgs_params* 
  * [10] unique-interval text steps, [11] SA/ISA lookups of the SMEM kernel,
  * [12] SMEM-kernel lane iterations, [13] its wavefront iterations (busiest lane of each
  * wavefront, summed), [14] the largest wavefront iteration count, [15] learned-index read
- * filter microseconds (form 2). */
+ * filter microseconds (form 2), [16] microseconds until the GPU stages were done,
+ * [17]..[20] host wall microseconds of the SMEM, seed, chain and DP stages, [21] DP jobs,
+ * [22] wavefront-fill DP jobs, [23]..[25] microseconds into the DP stage when its pool
+ * sizes, its records and its CIGAR/MD bytes were back on the host, [26] reads whose SMEM
+ * list came from the learned index's exact-read path (form 2), [27] reads the form-2
+ * filter skipped (n <= 32). */
 int msgs_batch_upload(msgs_ctx* ctx, int32_t slot, const msgs_reads* reads, const msgs_reads* mates);
 int msgs_batch_run(msgs_ctx* ctx, int32_t slot, const msgs_params* p, int32_t emitSam);
 /* msgs_batch_run in two halves: submit runs the GPU stages and leaves the host
@@ -144,8 +155,9 @@ int msgs_batch_wait(msgs_ctx* ctx, int32_t slot);
 int msgs_batch_stats(msgs_ctx* ctx, double* stats, int32_t n);
 int msgs_batch_sam(msgs_ctx* ctx, char* out, int64_t cap, int64_t* needed);
 /* autotuner (no reference counterpart): times the SMEM search of resident batch `slot`
- * with each lanes-per-CU value in cand[0..nCand) (NULL: 128..1024), keeps the fastest
- * in the context ("lanes_per_cu") and reports it; results never depend on the choice. */
+ * with each lanes-per-CU value in cand[0..nCand) (NULL: 128..1024), in form 2 for each
+ * lanes-per-read choice (0, 8, 16) too, keeps the fastest in the context ("lanes_per_cu",
+ * "group_lanes") and reports the lanes per CU; results never depend on the choice. */
 int msgs_tune(msgs_ctx* ctx, int32_t slot, const msgs_params* p, const int32_t* cand, int32_t nCand, int32_t* best,
               double* bestMicros);
 
@@ -180,6 +192,9 @@ int msgs_lix_knots(msgs_ctx* ctx, uint64_t* keys, int64_t* ranks, int32_t* S);
 /* counter calibration (no reference counterpart): one device copy of `bytes` (multiple
  * of 16) by a plain kernel, timed; hardware byte counters are scaled against it */
 int msgs_util_calib_copy(msgs_ctx* ctx, int64_t bytes, double* micros);
+/* random-access ceiling (no reference counterpart): nLoads reads of one 64-byte line each
+ * at hashed positions of a device buffer of `bytes` (multiple of 64); *micros = kernel time */
+int msgs_util_random_gather(msgs_ctx* ctx, int64_t bytes, int64_t nLoads, double* micros);
 
 /* ---- utilities ----
  * The permutation std::sort (libstdc++) gives for n keys compared with '<' -- the

@@ -26,7 +26,7 @@ EXPORTS = [
     "msgs_rank_batch", "msgs_ext_batch", "msgs_smem_batch", "msgs_seed_batch",
     "msgs_map_batch", "msgs_align_batch", "msgs_batch_upload", "msgs_batch_run",
     "msgs_batch_stats", "msgs_batch_sam", "msgs_util_sort_by_key",
-    "msgs_batch_submit", "msgs_batch_wait", "msgs_tune", "msgs_util_calib_copy", "msgs_lix_train", "msgs_lix_query", "msgs_lix_predict", "msgs_lix_grad", "msgs_lix_weights", "msgs_lix_knots",
+    "msgs_batch_submit", "msgs_batch_wait", "msgs_tune", "msgs_util_calib_copy", "msgs_util_random_gather", "msgs_ctx_get", "msgs_lix_train", "msgs_lix_query", "msgs_lix_predict", "msgs_lix_grad", "msgs_lix_weights", "msgs_lix_knots",
 ]
 N_LIX_PARAMS = 16 * 64 + 64 + 64 + 1
 
@@ -94,6 +94,8 @@ def lib():
     L.msgs_tune.argtypes = [vp, C.c_int32, C.POINTER(Params), C.POINTER(C.c_int32), C.c_int32,
                             C.POINTER(C.c_int32), PD]
     L.msgs_util_calib_copy.argtypes = [vp, C.c_int64, PD]
+    L.msgs_ctx_get.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64)]
+    L.msgs_util_random_gather.argtypes = [vp, C.c_int64, C.c_int64, PD]
     L.msgs_batch_submit.argtypes = [vp, C.c_int32, C.POINTER(Params), C.c_int32]
     L.msgs_batch_wait.argtypes = [vp, C.c_int32]
     PF = C.POINTER(C.c_float)
@@ -168,6 +170,11 @@ class Mapper:
     def set(self, key, value):
         self._chk(lib().msgs_ctx_set(self.h, key.encode(), int(value)), "msgs_ctx_set")
         return self
+
+    def get(self, key):
+        v = C.c_int64(0)
+        self._chk(lib().msgs_ctx_get(self.h, key.encode(), C.byref(v)), "msgs_ctx_get")
+        return v.value
 
     def build(self, chroms, names=None):
         chroms = [np.ascontiguousarray(c, np.uint8) for c in chroms]
@@ -302,6 +309,12 @@ class Mapper:
         self._chk(lib().msgs_util_calib_copy(self.h, nbytes, C.byref(us)), "msgs_util_calib_copy")
         return us.value
 
+    def random_gather(self, nbytes, nloads):
+        """microseconds for nloads random 64-byte line reads over a device buffer of nbytes"""
+        us = C.c_double(0)
+        self._chk(lib().msgs_util_random_gather(self.h, nbytes, nloads, C.byref(us)), "msgs_util_random_gather")
+        return us.value
+
     def tune(self, slot, params, cand=None):
         """pick the fastest lanes-per-CU for the SMEM search of a resident batch"""
         c = None if cand is None else np.ascontiguousarray(cand, np.int32)
@@ -373,8 +386,8 @@ class Mapper:
         self._chk(lib().msgs_batch_run(self.h, slot, C.byref(params), int(emit_sam)), "msgs_batch_run")
 
     def stats(self):
-        s = np.zeros(16, np.float64)
-        self._chk(lib().msgs_batch_stats(self.h, _p(s, C.c_double), 16), "msgs_batch_stats")
+        s = np.zeros(32, np.float64)
+        self._chk(lib().msgs_batch_stats(self.h, _p(s, C.c_double), 32), "msgs_batch_stats")
         return s
 
     def sam(self):

@@ -362,13 +362,15 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
     a.minLen = p->minLen;
     a.gate1 = std::isfinite(p->maxEvalue) ? 1 : 0;
     a.T1 = T; a.T2 = T;
-    /* form 2: skip reads in which no window that an emitted SMEM must contain occurs */
+    /* form 2: skip reads in which no window that an emitted SMEM must contain occurs, and
+     * take the SMEM list of exactly matching reads from the learned index */
     a.need = nullptr;
     c->lastFilterMicros = 0;
     if (c->form == 2 && c->lix.trained && p->minLen >= 16 && n > 0) {
         uint8_t* need = nullptr;
+        SmemRec* exact = nullptr;
         (void)hipEventRecord(c->ev0, c->stream);
-        int rc = runLixFilter(c, r, (int32_t)p->minLen, T, &need, dStats);
+        int rc = runLixFilter(c, r, (int32_t)p->minLen, T, a.gate1, &need, &exact, dStats);
         if (rc) return rc;
         (void)hipEventRecord(c->ev1, c->stream);
         if (hipEventSynchronize(c->ev1) != hipSuccess) return fail(c, MSGS_E_HIP, "read filter");
@@ -376,6 +378,7 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
         (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
         c->lastFilterMicros = ms * 1000.0;
         a.need = need;
+        a.exact = exact;
     }
     const int G = c->form == 2 ? c->groupLanes : 0;
     const int64_t lanesMax = (int64_t)std::max(1, c->nCU) * c->lanesPerCU;
@@ -571,6 +574,20 @@ extern "C" int msgs_ctx_set(msgs_ctx* c, const char* key, int64_t value) {
     else if (k == "lanes_per_cu") { if (value < 64 || value > 8192 || value % 64) return fail(c, MSGS_E_ARG, "lanes_per_cu: multiple of 64 in [64, 8192]"); c->lanesPerCU = (int)value; }
     else if (k == "smem_cap") { if (value < 1 || value > 4096) return fail(c, MSGS_E_ARG, "smem_cap in [1, 4096]"); c->smemCap = (int)value; }
     else if (k == "host_threads") { if (value < 1 || value > 256) return fail(c, MSGS_E_ARG, "host_threads in [1, 256]"); c->hostThreads = (int)value; }
+    else if (k == "read_filter") { if (value != 0 && value != 1) return fail(c, MSGS_E_ARG, "read_filter: 0 or 1"); c->readFilter = (int)value; }
+    else return fail(c, MSGS_E_ARG, "unknown option " + k);
+    return MSGS_OK;
+}
+
+extern "C" int msgs_ctx_get(msgs_ctx* c, const char* key, int64_t* value) {
+    if (!c || !key || !value) return MSGS_E_ARG;
+    std::string k(key);
+    if (k == "form") *value = c->form;
+    else if (k == "group_lanes") *value = c->groupLanes;
+    else if (k == "lanes_per_cu") *value = c->lanesPerCU;
+    else if (k == "smem_cap") *value = c->smemCap;
+    else if (k == "host_threads") *value = c->hostThreads;
+    else if (k == "read_filter") *value = c->readFilter;
     else return fail(c, MSGS_E_ARG, "unknown option " + k);
     return MSGS_OK;
 }
@@ -581,6 +598,23 @@ namespace {
 __global__ void __launch_bounds__(256) k_calib_copy(const uint4* src, uint4* dst, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         dst[i] = src[i];
+}
+
+/* nLoads gathers of one 64-byte line each at hashed line indexes of a buffer of nLines
+ * lines (the access pattern of occ-block / SA lookups with no locality) */
+__global__ void __launch_bounds__(256) k_gather(const uint4* buf, int64_t nLines, int64_t nLoads, uint64_t seed,
+                                                unsigned long long* sink) {
+    uint32_t acc = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nLoads; i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t z = seed + (uint64_t)i * 0x9e3779b97f4a7c15ull;
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        z ^= z >> 31;
+        const uint4* l = buf + 4 * (int64_t)(z % (uint64_t)nLines);
+        const uint4 a = l[0], b = l[1], c = l[2], d = l[3];
+        acc ^= a.x ^ a.y ^ a.z ^ a.w ^ b.x ^ b.y ^ b.z ^ b.w ^ c.x ^ c.y ^ c.z ^ c.w ^ d.x ^ d.y ^ d.z ^ d.w;
+    }
+    if (acc == 0x9e3779b9u) atomicAdd(sink, 1ull);
 }
 }  // namespace
 
@@ -600,6 +634,29 @@ extern "C" int msgs_util_calib_copy(msgs_ctx* c, int64_t bytes, double* micros) 
     if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, MSGS_E_HIP, "calibration copy");
     float ms = 0;
     (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    if (micros) *micros = ms * 1000.0;
+    return MSGS_OK;
+}
+
+/* Random-gather ceiling: nLoads 64-byte line reads at uniformly hashed positions of a
+ * device buffer of `bytes` (a multiple of 64), timed with HIP events (second of two
+ * launches). */
+extern "C" int msgs_util_random_gather(msgs_ctx* c, int64_t bytes, int64_t nLoads, double* micros) {
+    if (!c || bytes <= 0 || bytes % 64 || nLoads <= 0) return fail(c, MSGS_E_ARG, "bytes: positive multiple of 64; nLoads > 0");
+    hipError_t e = hipSuccess;
+    uint4* buf = c->bufs.get<uint4>("gather.buf", bytes / 16, e);
+    unsigned long long* sink = c->bufs.get<unsigned long long>("gather.sink", 1, e);
+    if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "gather buffer");
+    if (hipMemsetAsync(buf, 0x5a, bytes, c->stream) != hipSuccess) return fail(c, MSGS_E_HIP, "gather fill");
+    const unsigned blocks = (unsigned)std::max(1, c->nCU) * 32;
+    float ms = 0;
+    for (int k = 0; k < 2; ++k) {
+        (void)hipEventRecord(c->ev0, c->stream);
+        k_gather<<<blocks, 256, 0, c->stream>>>(buf, bytes / 64, nLoads, 0x1234567ull + k, sink);
+        (void)hipEventRecord(c->ev1, c->stream);
+        if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, MSGS_E_HIP, "gather");
+        (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    }
     if (micros) *micros = ms * 1000.0;
     return MSGS_OK;
 }

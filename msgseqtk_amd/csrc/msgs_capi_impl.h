@@ -6,7 +6,10 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <map>
+#include <new>
+#include <utility>
 #include <string>
 #include <memory>
 #include <thread>
@@ -58,8 +61,30 @@ struct HostBatch {
     std::vector<uint8_t> seq, qual;
     std::vector<int64_t> offs;
     std::string names;
+    std::vector<int64_t> nameOff;   /* [start, end) of each name in `names` */
     msgs_reads view{};
 };
+
+/* page-locked host memory for device-to-host results (fast DMA), non-coherent so that
+ * the host reads it through its caches (results are only moved by explicit copies);
+ * resize leaves new elements uninitialised (they are overwritten by the copy) */
+template <class T> struct PinnedAlloc {
+    using value_type = T;
+    PinnedAlloc() = default;
+    template <class U> PinnedAlloc(const PinnedAlloc<U>&) {}
+    T* allocate(size_t n) {
+        void* p = nullptr;
+        if (hipHostMalloc(&p, std::max<size_t>(1, n) * sizeof(T), hipHostMallocNonCoherent) != hipSuccess)
+            throw std::bad_alloc();
+        return static_cast<T*>(p);
+    }
+    void deallocate(T* p, size_t) { (void)hipHostFree(p); }
+    template <class U> void construct(U*) noexcept {}
+    template <class U, class... A> void construct(U* p, A&&... a) { ::new ((void*)p) U(std::forward<A>(a)...); }
+    template <class U> bool operator==(const PinnedAlloc<U>&) const { return true; }
+    template <class U> bool operator!=(const PinnedAlloc<U>&) const { return false; }
+};
+template <class T> using PVec = std::vector<T, PinnedAlloc<T>>;
 
 struct BatchResult;
 
@@ -78,11 +103,11 @@ struct Slot {
     msgs_params params{};
     std::vector<std::string> parts;
     std::vector<int64_t> lineOffs;
-    double stats[16] = {};
+    double stats[32] = {};
+    int rc = 0;
     void join() {
         if (worker.joinable()) worker.join();
-        pending = false;
-        res.reset();
+        pending = false;   /* res is kept: its pinned buffers serve the next run */
     }
     void release() { join(); used = false; }
 };
@@ -92,6 +117,7 @@ struct msgs_ctx {
     int nCU = 256;
     int lanesPerCU = 1024;
     int smemCap = 32;
+    int readFilter = 1;        /* form 2: also skip reads that cannot emit an SMEM */
     int form = 1;
     int groupLanes = 8;   /* form 2: lanes per read of the SMEM kernel (0 = one lane per read) */
     hipStream_t stream = nullptr;
@@ -101,7 +127,7 @@ struct msgs_ctx {
     LixState lix;
     BufPool bufs;
     Slot slots[16];
-    double stats[16] = {};
+    double stats[32] = {};
     double lastSmemMicros = 0;
     double lastFilterMicros = 0;
     int hostThreads = 16;
@@ -109,13 +135,15 @@ struct msgs_ctx {
      * offset of each read's records in their concatenation */
     std::vector<std::string> lastParts;
     std::vector<int64_t> lastLineOffs;
+    /* finished batch results, recycled so that their pinned buffers are reused */
+    std::vector<std::shared_ptr<BatchResult>> spareResults;
 };
 
 int fail(msgs_ctx* c, int code, const std::string& msg);
 bool evalueThreshold(double n, double maxE, double* T);
 int uploadReads(msgs_ctx* c, const msgs_reads* r, const char* pfx, DevReads& out);
 int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out, unsigned long long* dStats);
-int runLixFilter(msgs_ctx* c, const DevReads& r, int32_t minLen, double T, uint8_t** needOut,
-                 unsigned long long* dStats);
+int runLixFilter(msgs_ctx* c, const DevReads& r, int32_t minLen, double T, int32_t gate1, uint8_t** needOut,
+                 SmemRec** exactOut, unsigned long long* dStats);
 
 #endif

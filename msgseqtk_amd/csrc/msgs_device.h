@@ -53,6 +53,33 @@ __device__ __forceinline__ uint8_t comp16(uint8_t b) {
     return (uint8_t)(((b & 1) << 3) | ((b & 2) << 1) | ((b & 4) >> 1) | ((b & 8) >> 3));
 }
 
+/* per byte: 0 if the byte is one of the nt16 codes A, C, G, T (1, 2, 4, 8), else nonzero */
+__device__ __forceinline__ uint64_t notBasic8(uint64_t v) {
+    uint64_t x = v - ((v >> 1) & 0x5555555555555555ull);
+    x = (x & 0x3333333333333333ull) + ((x >> 2) & 0x3333333333333333ull);
+    x = (x + (x >> 4)) & 0x0f0f0f0f0f0f0f0full;   /* per-byte popcount */
+    return (x ^ 0x0101010101010101ull) | (v & 0xf0f0f0f0f0f0f0f0ull);
+}
+
+/* the 8 bytes at buf[a .. a+8) from two aligned words (buf padded by 8 bytes past its end) */
+__device__ __forceinline__ uint64_t load8(const uint8_t* buf, int64_t a) {
+    const uint64_t* w = (const uint64_t*)(buf + (a & ~(int64_t)7));
+    const uint32_t sh = 8u * (uint32_t)(a & 7);
+    const uint64_t w0 = w[0];
+    return sh ? (w0 >> sh) | (w[1] << (64 - sh)) : w0;
+}
+
+/* number of leading positions u < len at which read byte rd[u] is A/C/G/T and equals
+ * text byte tx[u] (both buffers padded by 8 bytes) */
+__device__ __forceinline__ int64_t fwdMatch(const uint8_t* tx, int64_t tp, const uint8_t* rd, int64_t rp, int64_t len) {
+    for (int64_t u = 0; u < len; u += 8) {
+        const uint64_t v = load8(rd, rp + u);
+        const uint64_t bad = notBasic8(v) | (v ^ load8(tx, tp + u));
+        if (bad) return min(len, u + (int64_t)(__builtin_ctzll(bad) >> 3));
+    }
+    return len;
+}
+
 /* # of N in BWT[0..i] */
 __device__ __forceinline__ int64_t countN(const DevIndex& ix, int64_t i) {
     int64_t lo = 0, hi = ix.nN;

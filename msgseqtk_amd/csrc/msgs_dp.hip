@@ -42,7 +42,6 @@ namespace {
 constexpr int32_t NEG = -1073741824;
 enum : int32_t { OP_M = 0, OP_I = 1, OP_D = 2, OP_S = 4, OP_EQ = 7, OP_X = 8 };
 enum : int32_t { FWD = 1, REV = 2 };
-enum : int32_t { CLS_NONE = 0, CLS_WAVE = 1, CLS_SERIAL = 2 };
 /* limits of the wavefront fill: seeds per job (LDS copy) and rectangle columns */
 constexpr int FILL_NS = 256;
 constexpr int FILL_COLS = 512;
@@ -244,6 +243,7 @@ __device__ __forceinline__ void makeGeo(Geo& G, const DpArgs& a, const JobGeo& g
 /* ------------------------------------------------------------------ setup */
 
 __global__ void __launch_bounds__(256) dp_setup_kernel(DevIndex ix, DpArgs a) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) { a.tileNeed[a.nJobs] = 0; a.pathNeed[a.nJobs] = 0; }
     for (int64_t job = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; job < a.nJobs;
          job += (int64_t)gridDim.x * blockDim.x) {
         const int32_t rid = a.jobRead[job];

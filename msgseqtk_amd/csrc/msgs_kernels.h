@@ -39,7 +39,9 @@ struct SmemArgs {
     SmemCand* cand;            /* candidate scratch, [nLanes][candCap] */
     int64_t candCap;
     int64_t nLanes;
-    const uint8_t* need;       /* per read: 0 = the read cannot emit an SMEM, skip (may be NULL) */
+    const uint8_t* need;       /* per read (may be NULL): 0 = the read cannot emit an SMEM, skip it;
+                                * 2 = its whole SMEM list is exact[r] alone; 1 or 3 = search */
+    const SmemRec* exact;      /* per read, read where need[r] == 2 */
     const double* prefLL;      /* per read at offs[r] + r: L+1 left-to-right loglik prefix sums */
     unsigned long long* queue; /* next read */
     unsigned long long* stats; /* [0] occ-block extensions, [2] text steps, [3] SA/ISA lookups,
@@ -109,6 +111,9 @@ struct AlnRec {
     double log10P;
     int64_t cells;
 };
+
+/* DP job classes: no rectangle, wavefront fill (one wave per job), serial fill */
+enum : int32_t { CLS_NONE = 0, CLS_WAVE = 1, CLS_SERIAL = 2 };
 
 /* per-job geometry in matrix coordinates (row i = query base i-1 + 1, column j =
  * target base tStart + j - 1); R5 = rows 1..r5r1 x cols 1..r5c1, R3 = rows r3r0..L x

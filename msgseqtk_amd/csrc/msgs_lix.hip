@@ -414,8 +414,29 @@ __device__ __forceinline__ uint64_t readKey(const uint8_t* s) {
 }
 
 /*
- * Read filter (eight lanes per read).  An SMEM the search emits is a substring Y of
- * the read with |Y| >= minLen that occurs in the BD text and whose logP (a sum of the
+ * Form-2 front end: the learned index decides, before the SMEM search, what the search
+ * would output for two kinds of reads.  need[r] codes: 0 = the read emits no SMEM (skip
+ * it), 1 = search it, 2 = its SMEM list is exact[r] alone, 3 = undecided (searched unless
+ * a later pass decides it).
+ *
+ * Exact reads (k_exact, two lanes per read).  Let h = L/2 and suppose every base is
+ * A/C/G/T, the read occurs exactly once in the BD text, and so do read[0, h+1) and
+ * read[h, L).  Then SMEM_LIST::findAllSMEMS (src/SMEM.cpp:186-215) runs a single
+ * findFwdBackSMEMS from h (src/SMEM.cpp:64-117): the forward phase reaches L with size 1
+ * and pushes [h, L) (its gate: the logP of [h, L) plus the logliks of read[0, h) <= T);
+ * the backward phase keeps [x, L) first with size 1 down to x = 0 (every longer candidate
+ * list entry [x, k), k > h, has size 1 by x = 1 and is dropped as a repeat of that size),
+ * and the step past the read start ('\0' is ambiguous, so size 0) emits [0, L) if its logP
+ * is <= T and L >= minLen; then from = -1 and to = L+1 end the driver loop.  All
+ * occurrences of read[0, h+1) and of the read lie in the key block of read[0, 16), all
+ * occurrences of read[h, L) in the key block of read[h, h+16): lane 0 counts the first
+ * two by comparing the text at each row of its block, lane 1 the third.  The reverse row
+ * is ISA of the mirrored occurrence (forward and reverse halves of a chrom's 2(len+1) BD
+ * symbols).  Blocks longer than kBlock rows, any other base and any failed gate leave
+ * the read to the search.
+ *
+ * Reads without SMEMs (k_window1 / k_window2).  An SMEM the search emits is a substring
+ * Y of the read with |Y| >= minLen that occurs in the BD text and whose logP (a sum of the
  * per-base logliks, src/SMEM.cpp:26-40) is <= T (the e-value gate, src/SMEM.cpp:99,152).
  * All logliks are negative, so a Y starting at a contains W(a) = read[a, c) with c the
  * first end such that c - a >= minLen and the left-to-right sum is <= T + delta (delta
@@ -424,14 +445,143 @@ __device__ __forceinline__ uint64_t readKey(const uint8_t* s) {
  * some SA row whose 16-symbol key equals W's first 16 bases (found with the learned
  * index) continues in the text with W's remaining bases.  Windows with a base other than
  * A/C/G/T, and key blocks longer than kBlock rows, keep the read (conservative).
- * need[r] = 1 keeps read r; requires minLen >= 16.
+ * k_window1 tries eight windows from the middle of each undecided read (eight lanes per
+ * read); reads it leaves open go to a list that k_window2 finishes with a wavefront per
+ * read.  Requires minLen >= 16.
  */
 constexpr int kBlock = 16;
-constexpr int kFilterLanes = 8;   /* lanes (windows in flight) per read */
+constexpr int kFilterLanes = 8;   /* lanes (windows in flight) per read in k_window1 */
 
-__global__ void __launch_bounds__(256) k_filter(LixView m, DevIndex ix, const uint8_t* seqs, const int64_t* offs,
-                                                int64_t nReads, int32_t minLen, double T, uint8_t* need,
-                                                unsigned long long* stats) {
+__global__ void __launch_bounds__(256) k_exact(LixView m, DevIndex ix, const uint8_t* seqs, const int64_t* offs,
+                                               int64_t nReads, int32_t minLen, double T, int32_t gate1,
+                                               uint8_t* need, SmemRec* exact, unsigned long long* stats) {
+    __shared__ short W1b[16 * 64];
+    loadW1(m, W1b);
+    const int lane = threadIdx.x & 63;
+    const int sub = lane & 1;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nWaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const double lA = ix.ll[1], lC = ix.ll[2], lG = ix.ll[4], lT = ix.ll[8];
+    unsigned long long nq = 0, nExact = 0;
+    for (int64_t r0 = wave * 32; r0 < nReads; r0 += nWaves * 32) {
+        const int64_t rid = r0 + (lane >> 1);
+        const bool mine = rid < nReads;
+        const int64_t off = mine ? offs[rid] : 0;
+        const int64_t L = mine ? offs[rid + 1] - off : 0;
+        const int64_t h = L / 2;
+        const bool xr = mine && L >= 32 && L >= minLen;
+        const uint64_t xkey = xr ? readKey(seqs + off + (sub == 0 ? 0 : h)) : ~0ull;
+        const bool xact = xr && xkey != ~0ull;
+        int32_t xj = 0;
+        const int64_t xp = predict(m, xkey, xact, W1b, true, &xj);
+        int32_t nPart = -1, nFull = 0;
+        int64_t fRow = -1, fPos = -1;
+        if (xact) {
+            ++nq;
+            int32_t pr = 0, fb = 0;
+            int64_t r = lbBounded(m, xkey, xp, (int64_t)m.eSeg[xj], &pr, &fb);
+            const int64_t s0 = sub == 0 ? 0 : h, len = L - s0, part = sub == 0 ? h + 1 : L - h;
+            nPart = 0;
+            for (int k = 0; r < m.n && m.ct[r] == xkey; ++r, ++k) {
+                if (k == kBlock) { nPart = -1; break; }
+                const int64_t pos = ix.sa[r];
+                const int64_t run = fwdMatch(ix.bd, pos, seqs, off + s0, len);
+                if (run >= part) nPart++;
+                if (run == len) { nFull++; fRow = r; fPos = pos; }
+            }
+        }
+        const int32_t nSuffix = __shfl(nPart, lane + 1);
+        bool xok = xact && sub == 0 && nPart == 1 && nFull == 1 && nSuffix == 1;
+        if (xok) {
+            /* logP in the search's order: the base at h, the forward steps to L-1, then the
+             * backward steps down to 0; the forward push gate adds read[0, h) in increasing
+             * order to the forward logP (src/SMEM.cpp:26-40, src/SMEM.h:140-177) */
+            auto llb = [&](int64_t i) {
+                const uint8_t v = seqs[off + i];
+                return v == 1 ? lA : (v == 2 ? lC : (v == 4 ? lG : lT));
+            };
+            double lf = 0;
+            lf += llb(h);
+            for (int64_t i = h + 1; i < L; ++i) lf += llb(i);
+            if (gate1) {
+                double ext = lf;
+                for (int64_t i = 0; i < h; ++i) ext += llb(i);
+                xok = ext <= T;
+            }
+            double lb = lf;
+            for (int64_t i = h - 1; i >= 0; --i) lb += llb(i);
+            xok = xok && lb <= T;
+            if (xok) {
+                int32_t lo = 0, hi = ix.nChrom;
+                while (hi - lo > 1) {
+                    const int32_t md = (lo + hi) >> 1;
+                    if (ix.chromBD[md] <= fPos) lo = md; else hi = md;
+                }
+                const int64_t cs = ix.chromBD[lo], cl = ix.chromBD[lo + 1] - cs;
+                const int64_t mir = 2 * cs + cl - 1 - fPos - L;
+                SmemRec x;
+                x.from = 0; x.to = (int32_t)L;
+                x.fwd = fRow; x.rev = (int64_t)ix.isa[mir]; x.size = 1; x.logP = lb;
+                exact[rid] = x;
+            }
+        }
+        if (mine && sub == 0) {
+            need[rid] = xok ? 2 : 3;
+            nExact += xok ? 1 : 0;
+        }
+    }
+    if (stats) {
+        atomicAdd(&stats[4], nq);
+        if (nExact) atomicAdd(&stats[8], nExact);
+    }
+}
+
+/* window W(a) of a read: its end c (-1 when no such window) and the key of its first 16
+ * bases; `keep` when a base other than A/C/G/T lies in it (conservative) */
+__device__ __forceinline__ void windowOf(const uint8_t* seqs, int64_t off, int64_t L, int64_t a, int32_t minLen,
+                                         double Tm, double lA, double lC, double lG, double lT, int64_t& c,
+                                         uint64_t& key, bool& keep) {
+    double x = 0;
+    int64_t wb = -1;
+    uint64_t w = 0;
+    c = -1;
+    key = 0;
+    for (int64_t b = a; b < L; ++b) {
+        const int64_t gp = off + b;
+        if ((gp & ~(int64_t)7) != wb) { wb = gp & ~(int64_t)7; w = *(const uint64_t*)(seqs + wb); }
+        const uint32_t v = (uint32_t)(w >> (8 * (gp & 7))) & 0xffu;
+        if (!(v == 1 || v == 2 || v == 4 || v == 8)) { keep = true; return; }
+        if (b < a + 16) key = (key << 3) | sym3((uint8_t)v);
+        x += v == 1 ? lA : (v == 2 ? lC : (v == 4 ? lG : lT));
+        if (b + 1 - a >= minLen && x <= Tm) { c = b + 1; return; }
+    }
+}
+
+/* whether window [a, c) with key `key` occurs (or the check gives up: block too long) */
+__device__ __forceinline__ bool windowOccurs(const LixView& m, const DevIndex& ix, const uint8_t* seqs, int64_t off,
+                                             int64_t a, int64_t c, uint64_t key, int64_t p, int32_t seg) {
+    int32_t pr = 0, fb = 0;
+    int64_t r = lbBounded(m, key, p, (int64_t)m.eSeg[seg], &pr, &fb);
+    const int64_t wl = c - a;
+    for (int k = 0; r < m.n && m.ct[r] == key; ++r, ++k) {
+        if (k == kBlock) return true;
+        const int64_t pos = ix.sa[r];
+        if (pos + wl > ix.n) continue;
+        if (fwdMatch(ix.bd, pos + 16, seqs, off + a + 16, wl - 16) == wl - 16) return true;
+    }
+    return false;
+}
+
+/* window a of round `base` + sub: the windows of a read from the middle outwards */
+__device__ __forceinline__ int64_t windowStart(int64_t t, int64_t nw) {
+    const int64_t mid = nw / 2;
+    return (t & 1) ? mid - (t + 1) / 2 : mid + t / 2;
+}
+
+__global__ void __launch_bounds__(256) k_window1(LixView m, DevIndex ix, const uint8_t* seqs, const int64_t* offs,
+                                                 int64_t nReads, int32_t minLen, double T, uint8_t* need,
+                                                 int64_t* open, unsigned long long* nOpen,
+                                                 unsigned long long* stats) {
     __shared__ short W1b[16 * 64];
     loadW1(m, W1b);
     const int lane = threadIdx.x & 63;
@@ -441,60 +591,87 @@ __global__ void __launch_bounds__(256) k_filter(LixView m, DevIndex ix, const ui
     constexpr int kGroups = 64 / kFilterLanes;
     const double lA = ix.ll[1], lC = ix.ll[2], lG = ix.ll[4], lT = ix.ll[8];
     const double Tm = T + 1e-9;
-    unsigned long long nq = 0;
+    unsigned long long nq = 0, nSkip = 0;
     for (int64_t r0 = wave * kGroups; r0 < nReads; r0 += nWaves * kGroups) {
         const int64_t rid = r0 + g;
-        const bool mine = rid < nReads;
+        const bool mine = rid < nReads && need[rid] == 3;
         const int64_t off = mine ? offs[rid] : 0;
         const int64_t L = mine ? offs[rid + 1] - off : 0;
         const int64_t nw = L >= minLen ? L - minLen + 1 : 0;
-        const int64_t mid = nw / 2;
+        bool keep = false, active = false;
+        int64_t a = 0, c = -1;
+        uint64_t key = 0;
+        if (mine && sub < nw) {
+            a = windowStart(sub, nw);
+            windowOf(seqs, off, L, a, minLen, Tm, lA, lC, lG, lT, c, key, keep);
+            active = !keep && c >= 0;
+        }
+        int32_t j = 0;
+        const int64_t p = predict(m, key, active, W1b, true, &j);
+        if (active) {
+            ++nq;
+            keep = windowOccurs(m, ix, seqs, off, a, c, key, p, j);
+        }
+        const unsigned long long bal = __ballot(keep);
+        const bool found = ((bal >> (g * kFilterLanes)) & ((1ull << kFilterLanes) - 1)) != 0;
+        if (mine && sub == 0) {
+            if (found) need[rid] = 1;
+            else if (nw <= kFilterLanes) { need[rid] = 0; ++nSkip; }
+            else open[atomicAdd(nOpen, 1ull)] = rid;
+        }
+    }
+    if (stats) {
+        atomicAdd(&stats[4], nq);
+        if (nSkip) atomicAdd(&stats[9], nSkip);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_window2(LixView m, DevIndex ix, const uint8_t* seqs, const int64_t* offs,
+                                                 int32_t minLen, double T, uint8_t* need, const int64_t* open,
+                                                 const unsigned long long* nOpen, unsigned long long* stats) {
+    __shared__ short W1b[16 * 64];
+    loadW1(m, W1b);
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nWaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const double lA = ix.ll[1], lC = ix.ll[2], lG = ix.ll[4], lT = ix.ll[8];
+    const double Tm = T + 1e-9;
+    const int64_t nList = (int64_t)*nOpen;
+    unsigned long long nq = 0, nSkip = 0;
+    for (int64_t w = wave; w < nList; w += nWaves) {
+        const int64_t rid = open[w];
+        const int64_t off = offs[rid];
+        const int64_t L = offs[rid + 1] - off;
+        const int64_t nw = L - minLen + 1;   /* > kFilterLanes: k_window1 listed it */
         bool found = false;
-        for (int64_t base = 0; __any(!found && base < nw); base += kFilterLanes) {
-            /* windows from the middle outwards */
-            const int64_t t = base + sub;
-            const int64_t a = (t & 1) ? mid - (t + 1) / 2 : mid + t / 2;
-            bool active = !found && t < nw && a >= 0 && a < nw;
-            bool keep = false;
-            int64_t c = -1;
+        /* rounds of 64 windows after the eight k_window1 tried */
+        for (int64_t base = kFilterLanes; !found && base < nw; base += 64) {
+            const int64_t t = base + lane;
+            bool keep = false, active = false;
+            int64_t a = 0, c = -1;
             uint64_t key = 0;
-            if (active) {
-                double x = 0;
-                int64_t wb = -1;
-                uint64_t w = 0;
-                for (int64_t b = a; b < L; ++b) {
-                    const int64_t gp = off + b;
-                    if ((gp & ~(int64_t)7) != wb) { wb = gp & ~(int64_t)7; w = *(const uint64_t*)(seqs + wb); }
-                    const uint32_t v = (uint32_t)(w >> (8 * (gp & 7))) & 0xffu;
-                    if (!(v == 1 || v == 2 || v == 4 || v == 8)) { keep = true; break; }
-                    if (b < a + 16) key = (key << 3) | sym3((uint8_t)v);
-                    x += v == 1 ? lA : (v == 2 ? lC : (v == 4 ? lG : lT));
-                    if (b + 1 - a >= minLen && x <= Tm) { c = b + 1; break; }
-                }
-                if (keep || c < 0) active = false;
+            if (t < nw) {
+                a = windowStart(t, nw);
+                windowOf(seqs, off, L, a, minLen, Tm, lA, lC, lG, lT, c, key, keep);
+                active = !keep && c >= 0;
             }
             int32_t j = 0;
             const int64_t p = predict(m, key, active, W1b, true, &j);
             if (active) {
                 ++nq;
-                int32_t pr = 0, fb = 0;
-                int64_t r = lbBounded(m, key, p, (int64_t)m.eSeg[j], &pr, &fb);
-                const int64_t wl = c - a;
-                for (int k = 0; r < m.n && m.ct[r] == key; ++r, ++k) {
-                    if (k == kBlock) { keep = true; break; }
-                    const int64_t pos = ix.sa[r];
-                    if (pos + wl > ix.n) continue;
-                    bool eq = true;
-                    for (int64_t u = 16; u < wl && eq; ++u) eq = ix.bd[pos + u] == seqs[off + a + u];
-                    if (eq) { keep = true; break; }
-                }
+                keep = windowOccurs(m, ix, seqs, off, a, c, key, p, j);
             }
-            const unsigned long long bal = __ballot(keep);
-            found = found || ((bal >> (g * kFilterLanes)) & ((1ull << kFilterLanes) - 1)) != 0;
+            found = __ballot(keep) != 0;
         }
-        if (mine && sub == 0) need[rid] = found ? 1 : 0;
+        if (lane == 0) {
+            need[rid] = found ? 1 : 0;
+            nSkip += found ? 0 : 1;
+        }
     }
-    if (stats) atomicAdd(&stats[4], nq);
+    if (stats) {
+        atomicAdd(&stats[4], nq);
+        if (nSkip) atomicAdd(&stats[9], nSkip);
+    }
 }
 
 /* raw model outputs (before rounding) for checks: mode 0 bf16 MFMA, mode 1 the same
@@ -715,18 +892,29 @@ extern "C" int msgs_lix_query(msgs_ctx* c, const uint64_t* keys, int64_t nq, int
     return MSGS_OK;
 }
 
-int runLixFilter(msgs_ctx* c, const DevReads& r, int32_t minLen, double T, uint8_t** needOut,
-                 unsigned long long* dStats) {
+int runLixFilter(msgs_ctx* c, const DevReads& r, int32_t minLen, double T, int32_t gate1, uint8_t** needOut,
+                 SmemRec** exactOut, unsigned long long* dStats) {
     hipError_t e = hipSuccess;
     uint8_t* need = c->bufs.get<uint8_t>("lx.need", r.n + 1, e);
+    SmemRec* exact = c->bufs.get<SmemRec>("lx.exact", r.n + 1, e);
+    int64_t* open = c->bufs.get<int64_t>("lx.open", r.n + 1, e);
+    unsigned long long* nOpen = c->bufs.get<unsigned long long>("lx.nopen", 1, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "filter buffer");
     if (r.n) {
-        const int64_t blocks = std::min<int64_t>((r.n + 31) / 32, (int64_t)std::max(1, c->nCU) * 32);
-        k_filter<<<(unsigned)blocks, 256, 0, c->stream>>>(c->lix.view(), c->db.view(), r.seq, r.offs, r.n, minLen, T,
-                                                          need, dStats);
+        const int64_t cap = (int64_t)std::max(1, c->nCU) * 32;
+        k_exact<<<(unsigned)std::min<int64_t>((r.n + 127) / 128, cap), 256, 0, c->stream>>>(
+            c->lix.view(), c->db.view(), r.seq, r.offs, r.n, minLen, T, gate1, need, exact, dStats);
+        if (c->readFilter) {
+            LCK(hipMemsetAsync(nOpen, 0, 8, c->stream), "filter list");
+            k_window1<<<(unsigned)std::min<int64_t>((r.n + 31) / 32, cap), 256, 0, c->stream>>>(
+                c->lix.view(), c->db.view(), r.seq, r.offs, r.n, minLen, T, need, open, nOpen, dStats);
+            k_window2<<<(unsigned)cap, 256, 0, c->stream>>>(c->lix.view(), c->db.view(), r.seq, r.offs, minLen, T,
+                                                              need, open, nOpen, dStats);
+        }
     }
     LCK(hipGetLastError(), "filter");
     *needOut = need;
+    *exactOut = exact;
     return MSGS_OK;
 }
 

@@ -8,6 +8,7 @@
  * where the reference computes them, and reorders with the restated libstdc++ sort.
  */
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <charconv>
@@ -28,12 +29,16 @@ namespace {
 
 constexpr int32_t NEG = -1073741824;
 
+/* reverse complement of every read and its qualities: one wavefront per read, lanes
+ * on consecutive bytes so that loads and stores coalesce */
 __global__ void k_revcom(const uint8_t* seq, const uint8_t* qual, const int64_t* offs, int64_t n, uint8_t* rs,
                          uint8_t* rq) {
-    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t waves = (int64_t)gridDim.x * (blockDim.x >> 6);
+    const int lane = threadIdx.x & 63;
+    for (int64_t r = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); r < n; r += waves) {
         const int64_t o = offs[r], L = offs[r + 1] - o;
-        for (int64_t i = 0; i < L; ++i) {
-            uint8_t b = seq[o + L - 1 - i];
+        for (int64_t i = lane; i < L; i += 64) {
+            const uint8_t b = seq[o + L - 1 - i];
             rs[o + i] = (uint8_t)(((b & 1) << 3) | ((b & 2) << 1) | ((b & 4) >> 1) | ((b & 8) >> 3));
             rq[o + i] = qual[o + L - 1 - i];
         }
@@ -52,6 +57,57 @@ __global__ void k_compact(const AlnRec* recs, int64_t nJobs, const int64_t* path
         for (int32_t k = 0; k < nm; ++k) mOut[mOff[j] + k] = ms[k];
     }
 }
+
+/* CIGAR op and MD byte counts of every job, plus a zero at nJobs (scan input) */
+__global__ void k_rec_counts(const AlnRec* recs, int64_t nJobs, int64_t* nc, int64_t* nm) {
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j <= nJobs; j += (int64_t)gridDim.x * blockDim.x) {
+        nc[j] = j < nJobs ? recs[j].nCigar : 0;
+        nm[j] = j < nJobs ? recs[j].mdLen : 0;
+    }
+}
+
+/* wavefront-fill job list, largest tile area first: jobs are bucketed by the bit
+ * length of their area (64 buckets, largest bucket first); the order inside a bucket
+ * is whatever the atomics give, which changes no result (jobs write disjoint
+ * outputs).  hist[64] counts, cur[64] cursors, tot[1] the list length. */
+__device__ __forceinline__ int areaBucket(int64_t area) { return area > 0 ? 64 - __clzll((unsigned long long)area) : 0; }
+
+__global__ void __launch_bounds__(256) k_wave_hist(const int32_t* cls, const int64_t* area, int64_t nJobs,
+                                                    unsigned long long* hist) {
+    __shared__ unsigned int h[64];
+    if (threadIdx.x < 64) h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < nJobs && cls[j] == CLS_WAVE) atomicAdd(&h[areaBucket(area[j]) & 63], 1u);
+    __syncthreads();
+    if (threadIdx.x < 64 && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], (unsigned long long)h[threadIdx.x]);
+}
+
+__global__ void k_wave_base(const unsigned long long* hist, unsigned long long* cur, int64_t* tot) {
+    if (threadIdx.x == 0) {
+        unsigned long long s = 0;
+        for (int b = 63; b >= 0; --b) { cur[b] = s; s += hist[b]; }
+        tot[0] = (int64_t)s;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_wave_scatter(const int32_t* cls, const int64_t* area, int64_t nJobs,
+                                                       unsigned long long* cur, int64_t* wave) {
+    __shared__ unsigned int h[64];
+    __shared__ unsigned long long base[64];
+    if (threadIdx.x < 64) h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int b = -1;
+    unsigned int r = 0;
+    if (j < nJobs && cls[j] == CLS_WAVE) { b = areaBucket(area[j]) & 63; r = atomicAdd(&h[b], 1u); }
+    __syncthreads();
+    if (threadIdx.x < 64 && h[threadIdx.x]) base[threadIdx.x] = atomicAdd(&cur[threadIdx.x], (unsigned long long)h[threadIdx.x]);
+    __syncthreads();
+    if (b >= 0) wave[base[b] + r] = j;
+}
+
+__global__ void k_copy_i64(const int64_t* src, int64_t* dst) { if (threadIdx.x == 0) *dst = *src; }
 
 __global__ void k_fill_u8(uint8_t* p, int64_t n, uint8_t v) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
@@ -100,10 +156,11 @@ struct ChainStage {
 /* DP results on the host: records plus compacted CIGAR ops / MD bytes of each job
  * at cigOff[j] / mdOff[j] */
 struct DpStage {
-    std::vector<AlnRec> recs;
-    std::vector<int64_t> cigOff, mdOff;
-    std::vector<uint32_t> cigar;
-    std::vector<char> md;
+    PVec<AlnRec> recs;
+    PVec<int64_t> cigOff, mdOff;
+    PVec<uint32_t> cigar;
+    PVec<char> md;
+    PVec<int64_t> tot;   /* device totals read back: tiles, paths, wave jobs, CIGAR ops, MD bytes */
 };
 
 static int runSeeds(msgs_ctx* c, const DevReads& r, const msgs_params* p, const SmemStage& sm, SeedStage& out,
@@ -189,8 +246,10 @@ static int runChains(msgs_ctx* c, const DevReads& r, const SeedStage& sd, ChainS
 static bool integral(double x) { return std::isfinite(x) && x == std::floor(x) && std::fabs(x) <= 10000; }
 
 static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msgs_params* p, const SeedStage& sd,
-                 const ChainStage& ch, DpStage& out, double* dpMicros) {
+                 const ChainStage& ch, DpStage& out, double* dpMicros, double* phase = nullptr) {
     const int64_t n = r.n, nJobs = ch.nJobs;
+    const auto t0 = std::chrono::steady_clock::now();
+    auto since = [&t0]() { return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count(); };
     hipStream_t st = c->stream;
     hipError_t e = hipSuccess;
     if (!integral(p->match) || !integral(p->mismatch) || !integral(p->gapO) || !integral(p->gapE))
@@ -229,63 +288,75 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "dp scratch");
     CK(hipMemcpyAsync(dq, hq, sizeof(hq), hipMemcpyHostToDevice, st), "qtab");
     a.qMatch = dq; a.qMis = dq + 256; a.qGapOpen = dq + 512;
-    if (n) k_revcom<<<gridFor(n), 256, 0, st>>>(r.seq, dQual, r.offs, n, rcS, rcQ);
+    if (n) k_revcom<<<(unsigned)std::min<int64_t>((n + 3) / 4, 65536), 256, 0, st>>>(r.seq, dQual, r.offs, n, rcS, rcQ);
     dp_setup_kernel<<<gridFor(nJobs), 256, 0, st>>>(c->db.view(), a);
-    std::vector<int64_t> tneed(nJobs), pneed(nJobs);
-    std::vector<int32_t> cls(nJobs);
-    CK(hipMemcpyAsync(tneed.data(), a.tileNeed, nJobs * 8, hipMemcpyDeviceToHost, st), "tneed");
-    CK(hipMemcpyAsync(pneed.data(), a.pathNeed, nJobs * 8, hipMemcpyDeviceToHost, st), "pneed");
-    CK(hipMemcpyAsync(cls.data(), a.jobCls, nJobs * 4, hipMemcpyDeviceToHost, st), "cls");
-    CK(hipStreamSynchronize(st), "dp setup");
-    std::vector<int64_t> tileOff;
-    const int64_t tiles = exclusiveScan(tneed, tileOff);
-    std::vector<int64_t> pathOff;
-    const int64_t paths = exclusiveScan(pneed, pathOff);
-    /* wavefront jobs, largest first so that the long ones start early */
-    std::vector<int64_t> wave;
-    for (int64_t j = 0; j < nJobs; ++j)
-        if (cls[j] == 1) wave.push_back(j);
-    std::sort(wave.begin(), wave.end(), [&](int64_t x, int64_t y) { return tneed[x] > tneed[y] || (tneed[x] == tneed[y] && x < y); });
+    /* tile and path offsets (exclusive scans with the totals at nJobs) and the wavefront
+     * job list, all on the device; the host reads back three totals */
+    const int nb = (int)((nJobs + 255) / 256);
+    size_t scanBytes = 0;
+    if (hipcub::DeviceScan::ExclusiveSum(nullptr, scanBytes, a.tileNeed, a.tileNeed, (int)(nJobs + 1), st) != hipSuccess)
+        return fail(c, MSGS_E_HIP, "scan size");
+    void* scanTmp = c->bufs.get<uint8_t>("dp.scantmp", scanBytes + 1, e);
     int64_t* dTileOff = c->bufs.get<int64_t>("dp.toff", nJobs + 1, e);
     int64_t* dPathOff = c->bufs.get<int64_t>("dp.poff", nJobs + 1, e);
-    int64_t* dWave = c->bufs.get<int64_t>("dp.wave", wave.size() + 1, e);
+    int64_t* dWave = c->bufs.get<int64_t>("dp.wave", nJobs + 1, e);
+    unsigned long long* dHist = c->bufs.get<unsigned long long>("dp.whist", 128, e);
+    int64_t* dTot = c->bufs.get<int64_t>("dp.tot", 8, e);
+    if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "dp scans");
+    CK(hipMemsetAsync(dHist, 0, 128 * 8, st), "hist");
+    CK(hipcub::DeviceScan::ExclusiveSum(scanTmp, scanBytes, a.tileNeed, dTileOff, (int)(nJobs + 1), st), "tile scan");
+    CK(hipcub::DeviceScan::ExclusiveSum(scanTmp, scanBytes, a.pathNeed, dPathOff, (int)(nJobs + 1), st), "path scan");
+    k_wave_hist<<<nb, 256, 0, st>>>(a.jobCls, a.tileNeed, nJobs, dHist);
+    k_wave_base<<<1, 64, 0, st>>>(dHist, dHist + 64, dTot + 2);
+    k_wave_scatter<<<nb, 256, 0, st>>>(a.jobCls, a.tileNeed, nJobs, dHist + 64, dWave);
+    k_copy_i64<<<1, 64, 0, st>>>(dTileOff + nJobs, dTot + 0);
+    k_copy_i64<<<1, 64, 0, st>>>(dPathOff + nJobs, dTot + 1);
+    out.tot.resize(8);
+    CK(hipMemcpyAsync(out.tot.data(), dTot, 3 * 8, hipMemcpyDeviceToHost, st), "dp totals");
+    CK(hipStreamSynchronize(st), "dp setup");
+    const int64_t tiles = out.tot[0], paths = out.tot[1], nWave = out.tot[2];
+    if (phase) { phase[0] = (double)nJobs; phase[1] = (double)nWave; phase[2] = since(); }
     a.tile = c->bufs.get<int4>("dp.tile", tiles + 1, e);
     a.path = c->bufs.get<uint8_t>("dp.path", paths + 1, e);
     a.cigar = c->bufs.get<uint32_t>("dp.cig", paths + 2 * nJobs + 1, e);
     a.md = c->bufs.get<char>("dp.md", 3 * paths + 8 * nJobs + 1, e);
     a.out = c->bufs.get<AlnRec>("dp.out", nJobs + 1, e);
+    int64_t* dCOff = c->bufs.get<int64_t>("dp.coff", nJobs + 1, e);
+    int64_t* dMOff = c->bufs.get<int64_t>("dp.moff", nJobs + 1, e);
+    int64_t* dNc = c->bufs.get<int64_t>("dp.nc", nJobs + 1, e);
+    int64_t* dNm = c->bufs.get<int64_t>("dp.nm", nJobs + 1, e);
+    uint32_t* dC = c->bufs.get<uint32_t>("dp.cc", paths + 2 * nJobs + 1, e);
+    char* dM = c->bufs.get<char>("dp.mc", 3 * paths + 8 * nJobs + 1, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "dp pools");
-    CK(hipMemcpyAsync(dTileOff, tileOff.data(), (nJobs + 1) * 8, hipMemcpyHostToDevice, st), "toff");
-    CK(hipMemcpyAsync(dPathOff, pathOff.data(), (nJobs + 1) * 8, hipMemcpyHostToDevice, st), "poff");
-    if (!wave.empty()) CK(hipMemcpyAsync(dWave, wave.data(), wave.size() * 8, hipMemcpyHostToDevice, st), "wave");
     a.tileOff = dTileOff; a.pathOff = dPathOff;
-    a.waveJobs = dWave; a.nWaveJobs = (int64_t)wave.size();
+    a.waveJobs = dWave; a.nWaveJobs = nWave;
     CK(hipEventRecord(c->ev2, st), "event");
-    if (!wave.empty()) {
-        const int64_t blocks = std::min<int64_t>((int64_t)wave.size(), (int64_t)std::max(1, c->nCU) * 16);
+    if (nWave) {
+        const int64_t blocks = std::min<int64_t>(nWave, (int64_t)std::max(1, c->nCU) * 16);
         dp_fill_kernel<<<(unsigned)blocks, 64, 0, st>>>(c->db.view(), a);
     }
     dp_trace_kernel<<<gridFor(nJobs), 256, 0, st>>>(c->db.view(), a);
     CK(hipEventRecord(c->ev3, st), "event");
-    out.recs.resize(nJobs);
-    CK(hipMemcpyAsync(out.recs.data(), a.out, nJobs * sizeof(AlnRec), hipMemcpyDeviceToHost, st), "recs");
-    CK(hipStreamSynchronize(st), "dp kernel");
-    std::vector<int32_t> nc(nJobs), nm(nJobs);
-    for (int64_t j = 0; j < nJobs; ++j) { nc[j] = out.recs[j].nCigar; nm[j] = out.recs[j].mdLen; }
-    const int64_t totC = exclusiveScan(nc, out.cigOff), totM = exclusiveScan(nm, out.mdOff);
-    int64_t* dCOff = c->bufs.get<int64_t>("dp.coff", nJobs + 1, e);
-    int64_t* dMOff = c->bufs.get<int64_t>("dp.moff", nJobs + 1, e);
-    uint32_t* dC = c->bufs.get<uint32_t>("dp.cc", totC + 1, e);
-    char* dM = c->bufs.get<char>("dp.mc", totM + 1, e);
-    if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "dp compaction");
-    CK(hipMemcpyAsync(dCOff, out.cigOff.data(), (nJobs + 1) * 8, hipMemcpyHostToDevice, st), "coff");
-    CK(hipMemcpyAsync(dMOff, out.mdOff.data(), (nJobs + 1) * 8, hipMemcpyHostToDevice, st), "moff");
+    /* compaction of CIGAR ops / MD bytes, offsets scanned on the device */
+    k_rec_counts<<<gridFor(nJobs + 1), 256, 0, st>>>(a.out, nJobs, dNc, dNm);
+    CK(hipcub::DeviceScan::ExclusiveSum(scanTmp, scanBytes, dNc, dCOff, (int)(nJobs + 1), st), "cigar scan");
+    CK(hipcub::DeviceScan::ExclusiveSum(scanTmp, scanBytes, dNm, dMOff, (int)(nJobs + 1), st), "md scan");
     k_compact<<<gridFor(nJobs), 256, 0, st>>>(a.out, nJobs, dPathOff, a.cigar, a.md, dCOff, dMOff, dC, dM);
+    out.recs.resize(nJobs);
+    out.cigOff.resize(nJobs + 1);
+    out.mdOff.resize(nJobs + 1);
+    CK(hipMemcpyAsync(out.recs.data(), a.out, nJobs * sizeof(AlnRec), hipMemcpyDeviceToHost, st), "recs");
+    CK(hipMemcpyAsync(out.cigOff.data(), dCOff, (nJobs + 1) * 8, hipMemcpyDeviceToHost, st), "coff");
+    CK(hipMemcpyAsync(out.mdOff.data(), dMOff, (nJobs + 1) * 8, hipMemcpyDeviceToHost, st), "moff");
+    CK(hipStreamSynchronize(st), "dp kernel");
+    if (phase) phase[3] = since();
+    const int64_t totC = out.cigOff[nJobs], totM = out.mdOff[nJobs];
     out.cigar.resize(totC + 1);
     out.md.resize(totM + 1);
     CK(hipMemcpyAsync(out.cigar.data(), dC, (totC + 1) * 4, hipMemcpyDeviceToHost, st), "cigar");
     CK(hipMemcpyAsync(out.md.data(), dM, totM + 1, hipMemcpyDeviceToHost, st), "md");
     CK(hipStreamSynchronize(st), "dp compaction");
+    if (phase) phase[4] = since();
     if (dpMicros) {
         float ms = 0;
         (void)hipEventElapsedTime(&ms, c->ev2, c->ev3);
@@ -618,7 +689,7 @@ static int checkParams(msgs_ctx* c, const msgs_params* p) {
     return MSGS_OK;
 }
 
-extern "C" int msgs_seed_batch(msgs_ctx* c, const msgs_reads* reads, const msgs_params* p, int64_t* counts,
+static int impl_seed_batch(msgs_ctx* c, const msgs_reads* reads, const msgs_params* p, int64_t* counts,
                                int64_t* rows6, double* logP, int64_t cap, int64_t* needed) {
     if (!c || !c->db.n) return fail(c, MSGS_E_STATE, "no database");
     int rc = checkParams(c, p);
@@ -653,7 +724,16 @@ extern "C" int msgs_seed_batch(msgs_ctx* c, const msgs_reads* reads, const msgs_
     return MSGS_OK;
 }
 
-extern "C" int msgs_align_batch(msgs_ctx* c, const msgs_reads* reads, const msgs_params* p, int64_t* counts,
+extern "C" int msgs_seed_batch(msgs_ctx* c, const msgs_reads* reads, const msgs_params* p, int64_t* counts,
+                               int64_t* rows6, double* logP, int64_t cap, int64_t* needed) {
+    try {
+        return impl_seed_batch(c, reads, p, counts, rows6, logP, cap, needed);
+    } catch (const std::bad_alloc&) {
+        return fail(c, MSGS_E_NOMEM, "host memory");
+    }
+}
+
+static int impl_align_batch(msgs_ctx* c, const msgs_reads* reads, const msgs_params* p, int64_t* counts,
                                 int64_t* rows10, double* scores, char* cigars, int64_t cigCap, int64_t cap,
                                 int64_t* needed) {
     if (!c || !c->db.n) return fail(c, MSGS_E_STATE, "no database");
@@ -691,19 +771,34 @@ extern "C" int msgs_align_batch(msgs_ctx* c, const msgs_reads* reads, const msgs
     return MSGS_OK;
 }
 
+extern "C" int msgs_align_batch(msgs_ctx* c, const msgs_reads* reads, const msgs_params* p, int64_t* counts,
+                                int64_t* rows10, double* scores, char* cigars, int64_t cigCap, int64_t cap,
+                                int64_t* needed) {
+    try {
+        return impl_align_batch(c, reads, p, counts, rows10, scores, cigars, cigCap, cap, needed);
+    } catch (const std::bad_alloc&) {
+        return fail(c, MSGS_E_NOMEM, "host memory");
+    }
+}
+
 /* SAM text of a whole batch (SE when mates is NULL; otherwise b holds the reads followed by
  * their mates, mate i as read n + i), finished on host threads in read-order chunks
  * (part t holds the records of the t-th chunk); lineOffs[i] is where
  * read i's records start in the parts' concatenation.  Returns the number of mapped reads. */
 static int64_t samText(msgs_ctx* c, const msgs_params* p, const msgs_reads* reads, const msgs_reads* mates,
-                       const BatchResult& b, bool emit, std::vector<std::string>& part, std::vector<int64_t>& lineOffs) {
+                       const BatchResult& b, bool emit, std::vector<std::string>& part, std::vector<int64_t>& lineOffs,
+                       const std::vector<int64_t>* names1 = nullptr, const std::vector<int64_t>* names2 = nullptr,
+                       int threads = 0) {
     const int64_t n = reads->n;
-    std::vector<int64_t> nm1, nm2;
-    splitNames(reads->names, n, nm1);
-    if (mates) splitNames(mates->names, n, nm2);
+    std::vector<int64_t> own1, own2;
+    if (!names1) { splitNames(reads->names, n, own1); names1 = &own1; }
+    if (mates && !names2) { splitNames(mates->names, n, own2); names2 = &own2; }
+    const std::vector<int64_t>& nm1 = *names1;
+    const std::vector<int64_t>& nm2 = mates ? *names2 : own2;
     lineOffs.assign(n + 1, 0);
-    const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(c->hostThreads, (n + 255) / 256));
-    part.assign(nt, std::string());
+    const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(threads > 0 ? threads : c->hostThreads, (n + 255) / 256));
+    part.resize(nt);
+    for (std::string& x : part) x.clear();   /* keeps the capacity of recycled parts */
     std::vector<int64_t> mappedPart(nt, 0);
     auto work = [&](int t) {
         const int64_t i0 = n * t / nt, i1 = n * (t + 1) / nt;
@@ -775,7 +870,7 @@ static void joinMates(const msgs_reads* a, const msgs_reads* b, HostBatch& out) 
     out.view.n = 2 * n;
 }
 
-extern "C" int msgs_map_batch(msgs_ctx* c, const msgs_reads* reads, const msgs_reads* mates, const msgs_params* p,
+static int impl_map_batch(msgs_ctx* c, const msgs_reads* reads, const msgs_reads* mates, const msgs_params* p,
                               char* out, int64_t cap, int64_t* lineOffs, int64_t* needed) {
     if (!c || !c->db.n) return fail(c, MSGS_E_STATE, "no database");
     int rc = checkParams(c, p);
@@ -807,6 +902,15 @@ extern "C" int msgs_map_batch(msgs_ctx* c, const msgs_reads* reads, const msgs_r
     return MSGS_OK;
 }
 
+extern "C" int msgs_map_batch(msgs_ctx* c, const msgs_reads* reads, const msgs_reads* mates, const msgs_params* p,
+                              char* out, int64_t cap, int64_t* lineOffs, int64_t* needed) {
+    try {
+        return impl_map_batch(c, reads, mates, p, out, cap, lineOffs, needed);
+    } catch (const std::bad_alloc&) {
+        return fail(c, MSGS_E_NOMEM, "host memory");
+    }
+}
+
 /* ------------------------------------------------------------------ resident batches */
 
 static std::vector<uint8_t> hostCopy(const uint8_t* p, int64_t n) { return p ? std::vector<uint8_t>(p, p + n) : std::vector<uint8_t>(); }
@@ -832,6 +936,7 @@ extern "C" int msgs_batch_upload(msgs_ctx* c, int32_t slot, const msgs_reads* re
         hb.view.offs = hb.offs.data();
         hb.view.names = hb.names.empty() ? nullptr : hb.names.c_str();
         hb.view.n = n;
+        splitNames(hb.view.names, n, hb.nameOff);
     }
     /* on the device the mates follow the reads as one batch */
     HostBatch joined;
@@ -851,7 +956,17 @@ extern "C" int msgs_batch_upload(msgs_ctx* c, int32_t slot, const msgs_reads* re
 }
 
 /* GPU stages of a resident batch, then its host finishing on the slot's worker thread */
+static int batchSubmit(msgs_ctx* c, int32_t slot, const msgs_params* p, int32_t emitSam);
+
 extern "C" int msgs_batch_submit(msgs_ctx* c, int32_t slot, const msgs_params* p, int32_t emitSam) {
+    try {
+        return batchSubmit(c, slot, p, emitSam);
+    } catch (const std::bad_alloc&) {
+        return fail(c, MSGS_E_NOMEM, "host memory");
+    }
+}
+
+static int batchSubmit(msgs_ctx* c, int32_t slot, const msgs_params* p, int32_t emitSam) {
     if (!c || !c->db.n) return fail(c, MSGS_E_STATE, "no database");
     if (slot < 0 || slot >= 16 || !c->slots[slot].used) return fail(c, MSGS_E_STATE, "slot not uploaded");
     int rc = checkParams(c, p);
@@ -859,38 +974,49 @@ extern "C" int msgs_batch_submit(msgs_ctx* c, int32_t slot, const msgs_params* p
     Slot& S = c->slots[slot];
     S.join();
     hipError_t e = hipSuccess;
-    unsigned long long* dStats = c->bufs.get<unsigned long long>("stats", 8, e);
+    unsigned long long* dStats = c->bufs.get<unsigned long long>("stats", 16, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "stats");
-    CK(hipMemsetAsync(dStats, 0, 8 * 8, c->stream), "stats");
+    CK(hipMemsetAsync(dStats, 0, 16 * 8, c->stream), "stats");
     const auto t0 = std::chrono::steady_clock::now();
-    auto b = std::make_shared<BatchResult>();
+    if (!S.res) {
+        if (!c->spareResults.empty()) { S.res = c->spareResults.back(); c->spareResults.pop_back(); }
+        else S.res = std::make_shared<BatchResult>();
+    }
+    std::shared_ptr<BatchResult> b = S.res;
     double seedUs = 0, smemUs = 0, dpUs = 0, filterUs = 0;
+    double wall[4] = {0, 0, 0, 0};   /* host wall time of the SMEM, seed, chain and DP stages */
+    double dpPhase[5] = {0, 0, 0, 0, 0};
     {
+        auto us = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+            return std::chrono::duration<double, std::micro>(b - a).count();
+        };
         auto s0 = std::chrono::steady_clock::now();
         rc = runSmem(c, S.dev, p, b->sm, dStats);
         if (rc) return rc;
+        auto s1 = std::chrono::steady_clock::now();
         smemUs += c->lastSmemMicros;
         filterUs += c->lastFilterMicros;
         rc = runSeeds(c, S.dev, p, b->sm, b->sd, dStats);
         if (rc) return rc;
+        auto s2 = std::chrono::steady_clock::now();
         rc = runChains(c, S.dev, b->sd, b->ch);
         if (rc) return rc;
-        seedUs += std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - s0).count();
-        double us = 0;
-        rc = runDp(c, S.dev, S.dQual, p, b->sd, b->ch, b->dp, &us);
+        auto s3 = std::chrono::steady_clock::now();
+        seedUs += us(s0, s3);
+        double k = 0;
+        rc = runDp(c, S.dev, S.dQual, p, b->sd, b->ch, b->dp, &k, dpPhase);
         if (rc) return rc;
-        dpUs += us;
+        auto s4 = std::chrono::steady_clock::now();
+        dpUs += k;
+        wall[0] = us(s0, s1); wall[1] = us(s1, s2); wall[2] = us(s2, s3); wall[3] = us(s3, s4);
     }
-    unsigned long long hs[8];
+    unsigned long long hs[16];
     CK(hipMemcpy(hs, dStats, sizeof(hs), hipMemcpyDeviceToHost), "stats");
-    int64_t cells = 0;
-    for (const AlnRec& a : b->dp.recs) cells += a.cells;
     double* st = S.stats;
     std::memset(st, 0, sizeof(S.stats));
     st[0] = (double)(S.host[0].view.n * (S.paired ? 2 : 1));
     st[2] = (double)hs[0];
     st[3] = (double)hs[1];
-    st[4] = (double)cells;
     st[5] = smemUs;
     st[6] = dpUs;
     st[8] = seedUs;
@@ -900,14 +1026,29 @@ extern "C" int msgs_batch_submit(msgs_ctx* c, int32_t slot, const msgs_params* p
     st[13] = (double)hs[6];
     st[14] = (double)hs[7];
     st[15] = filterUs;
+    st[16] = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count();
+    for (int k = 0; k < 4; ++k) st[17 + k] = wall[k];
+    for (int k = 0; k < 5; ++k) st[21 + k] = dpPhase[k];
+    st[26] = (double)hs[8];
+    st[27] = (double)hs[9];
     S.params = *p;
     S.res = b;
     S.pending = true;
     const bool emit = emitSam != 0;
+    S.rc = MSGS_OK;
     S.worker = std::thread([c, &S, emit, t0]() {
         const auto h0 = std::chrono::steady_clock::now();
-        const int64_t mapped = samText(c, &S.params, &S.host[0].view, S.paired ? &S.host[1].view : nullptr, *S.res,
-                                       emit, S.parts, S.lineOffs);
+        int64_t mapped = 0;
+        try {
+            mapped = samText(c, &S.params, &S.host[0].view, S.paired ? &S.host[1].view : nullptr, *S.res, emit,
+                             S.parts, S.lineOffs, &S.host[0].nameOff, S.paired ? &S.host[1].nameOff : nullptr,
+                             std::max(1, c->hostThreads - 1));   /* one core stays with the submitting thread */
+        } catch (const std::bad_alloc&) {
+            S.rc = MSGS_E_NOMEM;
+        }
+        int64_t cells = 0;
+        for (const AlnRec& a : S.res->dp.recs) cells += a.cells;
+        S.stats[4] = (double)cells;
         const auto t1 = std::chrono::steady_clock::now();
         S.stats[1] = (double)mapped;
         S.stats[7] = std::chrono::duration<double, std::micro>(t1 - t0).count();
@@ -923,6 +1064,8 @@ extern "C" int msgs_batch_wait(msgs_ctx* c, int32_t slot) {
     Slot& S = c->slots[slot];
     if (!S.pending) return fail(c, MSGS_E_STATE, "slot has no submitted run");
     S.join();
+    if (S.res) { c->spareResults.push_back(S.res); S.res.reset(); }
+    if (S.rc) return fail(c, S.rc, "host memory while finishing records");
     std::memcpy(c->stats, S.stats, sizeof(c->stats));
     std::swap(c->lastParts, S.parts);
     std::swap(c->lastLineOffs, S.lineOffs);
@@ -937,7 +1080,7 @@ extern "C" int msgs_batch_run(msgs_ctx* c, int32_t slot, const msgs_params* p, i
 
 extern "C" int msgs_batch_stats(msgs_ctx* c, double* stats, int32_t n) {
     if (!c || !stats) return MSGS_E_ARG;
-    for (int32_t i = 0; i < n && i < 16; ++i) stats[i] = c->stats[i];
+    for (int32_t i = 0; i < n && i < 32; ++i) stats[i] = c->stats[i];
     return MSGS_OK;
 }
 
@@ -966,22 +1109,30 @@ extern "C" int msgs_tune(msgs_ctx* c, int32_t slot, const msgs_params* p, const 
     static const int32_t kDefault[] = {128, 256, 384, 512, 768, 1024};
     if (!cand || nCand <= 0) { cand = kDefault; nCand = 6; }
     Slot& S = c->slots[slot];
-    const int keep = c->lanesPerCU;
+    for (int32_t i = 0; i < nCand; ++i)
+        if (cand[i] < 64 || cand[i] > 4096 || cand[i] % 64) return fail(c, MSGS_E_ARG, "lanes per CU");
+    const int keep = c->lanesPerCU, keepG = c->groupLanes;
+    static const int kGroups[] = {0, 8, 16};
+    const int nG = c->form == 2 ? 3 : 1;
     int32_t bestL = keep;
+    int bestG = keepG;
     double bestUs = 1e300;
-    for (int32_t i = 0; i < nCand; ++i) {
-        if (cand[i] < 64 || cand[i] > 4096 || cand[i] % 64) { c->lanesPerCU = keep; return fail(c, MSGS_E_ARG, "lanes per CU"); }
-        c->lanesPerCU = cand[i];
-        double t = 1e300;
-        for (int rep = 0; rep < 2; ++rep) {
-            SmemStage sm;
-            rc = runSmem(c, S.dev, p, sm, nullptr);
-            if (rc) { c->lanesPerCU = keep; return rc; }
-            t = std::min(t, c->lastSmemMicros + c->lastFilterMicros);
+    for (int gi = 0; gi < nG; ++gi) {
+        if (c->form == 2) c->groupLanes = kGroups[gi];
+        for (int32_t i = 0; i < nCand; ++i) {
+            c->lanesPerCU = cand[i];
+            double t = 1e300;
+            for (int rep = 0; rep < 2; ++rep) {
+                SmemStage sm;
+                rc = runSmem(c, S.dev, p, sm, nullptr);
+                if (rc) { c->lanesPerCU = keep; c->groupLanes = keepG; return rc; }
+                t = std::min(t, c->lastSmemMicros + c->lastFilterMicros);
+            }
+            if (t < bestUs) { bestUs = t; bestL = cand[i]; bestG = c->groupLanes; }
         }
-        if (t < bestUs) { bestUs = t; bestL = cand[i]; }
     }
     c->lanesPerCU = bestL;
+    c->groupLanes = bestG;
     if (best) *best = bestL;
     if (bestMicros) *bestMicros = bestUs;
     return MSGS_OK;

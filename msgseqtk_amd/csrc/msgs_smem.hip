@@ -227,14 +227,6 @@ __device__ __forceinline__ void load16(const uint8_t* buf, int64_t a, uint64_t& 
     u1 = sh ? (w1 >> sh) | (w2 << (64 - sh)) : w1;
 }
 
-/* per byte: 0 if the byte is one of the nt16 codes A, C, G, T (1, 2, 4, 8), else nonzero */
-__device__ __forceinline__ uint64_t notBasic8(uint64_t v) {
-    uint64_t x = v - ((v >> 1) & 0x5555555555555555ull);
-    x = (x & 0x3333333333333333ull) + ((x >> 2) & 0x3333333333333333ull);
-    x = (x + (x >> 4)) & 0x0f0f0f0f0f0f0f0full;   /* per-byte popcount */
-    return (x ^ 0x0101010101010101ull) | (v & 0xf0f0f0f0f0f0f0f0ull);
-}
-
 /* per byte: complement of a basic nt16 code (bit reversal of the low nibble) */
 __device__ __forceinline__ uint64_t comp8(uint64_t v) {
     return ((v & 0x0101010101010101ull) << 3) | ((v & 0x0202020202020202ull) << 1) |
@@ -298,7 +290,8 @@ __device__ bool nextRead(const Ctx& c, Lane& l) {
         l.L = (int)(c.a.offs[rid + 1] - off);
         l.P = c.a.prefLL ? c.a.prefLL + off + rid : nullptr;
         l.nOut = 0;
-        if (l.L > 0 && (!c.a.need || c.a.need[rid])) {
+        const uint8_t nd = c.a.need ? c.a.need[rid] : 1;
+        if (l.L > 0 && (nd & 1)) {
             l.dFrom = l.L / 2;
             l.dTo = l.dFrom + 1;
             l.stage = STG_A;
@@ -306,6 +299,7 @@ __device__ bool nextRead(const Ctx& c, Lane& l) {
             driverAfter(c, l);
             if (l.state != ST_IDLE) return true;
         }
+        if (nd == 2) { c.a.out[rid * c.a.cap] = c.a.exact[rid]; l.nOut = 1; }
         c.a.cnt[rid] = l.nOut;
     }
 }
@@ -501,7 +495,8 @@ __device__ bool nextReadGroup(const Ctx& c, Lane& l, int gbase, int gl) {
         l.L = (int)(c.a.offs[rid + 1] - off);
         l.P = c.a.prefLL ? c.a.prefLL + off + rid : nullptr;
         l.nOut = 0;
-        if (l.L > 0 && (!c.a.need || c.a.need[rid])) {
+        const uint8_t nd = c.a.need ? c.a.need[rid] : 1;
+        if (l.L > 0 && (nd & 1)) {
             l.dFrom = l.L / 2;
             l.dTo = l.dFrom + 1;
             l.stage = STG_A;
@@ -509,25 +504,42 @@ __device__ bool nextReadGroup(const Ctx& c, Lane& l, int gbase, int gl) {
             driverAfter(c, l);
             if (l.state != ST_IDLE) return true;
         }
+        if (nd == 2) {
+            if (gl == 0) c.a.out[rid * c.a.cap] = c.a.exact[rid];
+            l.nOut = 1;
+        }
         if (gl == 0) c.a.cnt[rid] = l.nOut;
     }
 }
 
 }  // namespace
 
+/* lane `dst` of the wavefront receives v (a permutation within the active lanes) */
+__device__ __forceinline__ int permute32(int dst, int v) { return __builtin_amdgcn_ds_permute(dst << 2, v); }
+__device__ __forceinline__ int64_t permute64(int dst, int64_t v) {
+    const int lo = permute32(dst, (int)(uint32_t)(uint64_t)v);
+    const int hi = permute32(dst, (int)(uint32_t)((uint64_t)v >> 32));
+    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+__device__ __forceinline__ double permuteF64(int dst, double v) {
+    return __longlong_as_double(permute64(dst, __double_as_longlong(v)));
+}
+
 /*
  * The SMEM search with one read per group of G lanes (64 / G reads per wavefront).  The
  * read's state is uniform across its group.  A phase-1 step (the growing SMEM) is one
  * extension that all lanes of the group compute alike; a phase-2 step extends up to G
- * candidates of the lock-step list at once, lane i taking list entry i of the chunk.
- * The reference's sequential rules for one pass (src/SMEM.cpp:93-113, 146-166) become
- * per-candidate tests because sizes along the list are non-decreasing (longer strings
- * have no more occurrences, and all candidates extend by the same base):
+ * candidates of the lock-step list at once, lane i taking list entry i.  The reference's
+ * sequential rules for one pass (src/SMEM.cpp:93-113, 146-166) become per-candidate tests
+ * because sizes along the list are non-decreasing (longer strings have no more
+ * occurrences, and all candidates extend by the same base):
  *   keep_i = S'_i >= 1 and S'_i != S'_{i-1}          (the reference's s0 test)
  *   emit_i = S'_i != S_i and S_i >= 1 and S'_{i-1} < 1 (nValid == 0) and len, e-value,
- * with S'_{-1} = -1; the next chunk continues from the last S' of the previous one.
- * Emitted SMEMs keep the list order (ballot prefix counts), and kept candidates are
- * compacted to the front of the list in order, as erase(remove_if) does.  A lone unique
+ * with S'_{-1} = -1.  Emitted SMEMs keep the list order (ballot prefix counts) and kept
+ * candidates move to the front in order, as erase(remove_if) does.  Lists of at most G
+ * entries live in the group's registers (lane i holds entry i; compaction is a lane
+ * permutation); longer lists live in the per-group scratch and are processed G entries
+ * per step, each chunk continuing from the last S' of the previous one.  A lone unique
  * candidate walks the text as in the per-lane kernel.  Output equals smem_kernel_t's.
  */
 template <int G>
@@ -535,12 +547,16 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
     static_assert(G == 4 || G == 8 || G == 16 || G == 32, "group size");
     const int lane = threadIdx.x & 63;
     const int gl = lane % G, gbase = lane - gl;
-    const uint64_t gmask = (G == 64 ? ~0ull : ((1ull << G) - 1)) << gbase;
+    const uint64_t gmask = ((1ull << G) - 1) << gbase;
     const uint64_t below = (1ull << lane) - 1;
     Lane l;
     l.nExt = 0; l.nFast = 0; l.nLook = 0;
     l.state = ST_IDLE;
     l.nCand = 0; l.lo = 0; l.k = 0; l.w = 0; l.s0 = -1; l.nValid = 0;
+    /* this lane's list entry in register mode (l.nValid == 1) or its phase-1 push */
+    int rPos = 0;
+    int64_t rP = 0, rQ = 0, rS = 0, rX = -1;
+    double rLL = 0;
     Ctx c{ix, a, ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G};
     const int64_t minLen = a.minLen;
     const double llA = ix.ll[1], llC = ix.ll[2], llG = ix.ll[4], llT = ix.ll[8];
@@ -594,7 +610,12 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
                     if (gl == 0) l.nLook++;
                     if (fwd) R0 = v; else F0 = v;
                 }
-                if (gl == 0) candStore(c, l.nCand, fwd ? l.cTo : l.cFrom, F0, R0, S0, LL0, -1);
+                const int pp = fwd ? l.cTo : l.cFrom;
+                if (l.nCand < G) {
+                    if (gl == l.nCand) { rPos = pp; rP = F0; rQ = R0; rS = S0; rX = -1; rLL = LL0; }
+                } else if (gl == 0) {
+                    candStore(c, l.nCand, pp, F0, R0, S0, LL0, -1);
+                }
                 l.nCand++;
             }
             if (fwd) l.cTo = mTo; else l.cFrom = mFrom;
@@ -608,18 +629,33 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
                     l.state = fwd ? ST_FB2 : ST_BF2;
                     l.lo = l.nCand - 1;   /* slot of list entry 0 (the last push) */
                     l.k = 0; l.w = 0; l.s0 = -1;
+                    if (l.nCand <= G) {
+                        /* registers: lane i takes push nCand-1-i (list order) */
+                        const int src = gbase + max(0, l.nCand - 1 - gl);
+                        rPos = __shfl(rPos, src, 64);
+                        rP = shfl64(rP, src); rQ = shfl64(rQ, src); rS = shfl64(rS, src); rX = shfl64(rX, src);
+                        rLL = __longlong_as_double(shfl64(__double_as_longlong(rLL), src));
+                        l.nValid = 1;
+                    } else {
+                        /* scratch: the first G pushes join the rest */
+                        candStore(c, gl, rPos, rP, rQ, rS, rLL, rX);
+                        l.nValid = 0;
+                    }
                 }
             }
         } else {
-            /* ---- phase 2: a chunk of the lock-step list, one candidate per lane ---- */
+            /* ---- phase 2: up to G candidates of the lock-step list, one per lane ---- */
             const bool fb = st == ST_FB2;
             const bool fwd = !fb;
+            const bool regs = l.nValid == 1;
             const int i = l.k + gl;
             const bool act = i < l.nCand;
             int pos = 0;
             int64_t P = 0, Q = 0, S = 0, X = -1;
             double LL = 0;
-            if (act) {
+            if (regs) {
+                if (act) { pos = rPos; P = rP; Q = rQ; S = rS; X = rX; LL = rLL; }
+            } else if (act) {
                 const SmemCand cd = a.cand[cidx(c, l.lo - i)];
                 pos = cd.pos; P = cd.F; Q = cd.R; S = cd.S; X = cd.X; LL = cd.LL;
             }
@@ -675,9 +711,19 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
                     o->fwd = F0; o->rev = R0; o->size = S0; o->logP = LL0;
                 }
             }
-            if (keep) candStore(c, l.lo - (l.w + __popcll(kMask & below)), pos, P, Q, S, LL, X);
+            const int nKeep = __popcll(kMask);
+            if (regs) {
+                /* compaction: kept entries to the front in order, the others behind them */
+                const int rank = keep ? __popcll(kMask & below) : nKeep + __popcll(~kMask & gmask & below);
+                const int dst = gbase + rank;
+                rPos = permute32(dst, pos);
+                rP = permute64(dst, P); rQ = permute64(dst, Q); rS = permute64(dst, S); rX = permute64(dst, X);
+                rLL = permuteF64(dst, LL);
+            } else if (keep) {
+                candStore(c, l.lo - (l.w + __popcll(kMask & below)), pos, P, Q, S, LL, X);
+            }
             l.nOut += __popcll(eMask);
-            l.w += __popcll(kMask);
+            l.w += nKeep;
             const int last = min(G, l.nCand - l.k) - 1;
             l.s0 = (int)shfl64(S, gbase + last);
             l.k += G;
@@ -687,8 +733,19 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
                 if (fb) { l.sFrom--; more = l.sFrom >= 0; }
                 else { l.sTo++; more = l.sTo <= l.L; }
                 more = more && l.nCand > 0;
-                if (more) { l.k = 0; l.w = 0; l.s0 = -1; }
-                else driverAfter(c, l);
+                if (more) {
+                    l.k = 0; l.w = 0; l.s0 = -1;
+                    if (!regs && l.nCand <= G) {
+                        /* the list now fits the group: load it once */
+                        if (gl < l.nCand) {
+                            const SmemCand cd = a.cand[cidx(c, l.lo - gl)];
+                            rPos = cd.pos; rP = cd.F; rQ = cd.R; rS = cd.S; rX = cd.X; rLL = cd.LL;
+                        }
+                        l.nValid = 1;
+                    }
+                } else {
+                    driverAfter(c, l);
+                }
             }
         }
         if (l.state == ST_IDLE && gl == 0) a.cnt[l.rid] = l.nOut;
