@@ -9,6 +9,7 @@ Prints one JSON line (see --help).  The CPU baseline is the oracle restatement o
 reference's path timed on the host cores over a bounded sample of the same pairs.
 """
 import argparse
+import collections
 import json
 import os
 import sys
@@ -48,6 +49,9 @@ def parse():
     ap.add_argument("--lix-segments", type=int, default=1 << 20, help="learned index: root segments")
     ap.add_argument("--lix-steps", type=int, default=1000, help="learned index: training steps")
     ap.add_argument("--no-gather", action="store_true", help="skip the random-gather ceiling measurement")
+    ap.add_argument("--depth", type=int, default=3, help="batches in flight (device stages of two overlap)")
+    ap.add_argument("--lanes-per-cu", type=int, default=0, help="fix the SMEM kernel's lanes per CU (no autotuning)")
+    ap.add_argument("--group-lanes", type=int, default=-1, help="fix the form-2 lanes per read: 0, 8 or 16")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 1 GiB device copy (counter calibration under rocprofv3 --pmc)")
     return ap.parse_args()
@@ -97,12 +101,17 @@ def main():
             pass
 
     tuned = None
-    if not a.no_tune:
+    if a.lanes_per_cu:
+        m.set("lanes_per_cu", a.lanes_per_cu)
+    if a.group_lanes >= 0:
+        m.set("group_lanes", a.group_lanes)
+    if not a.no_tune and not a.lanes_per_cu:
         lanes, tus = m.tune(0, p)
         tuned = {"lanes_per_cu": lanes, "group_lanes": m.get("group_lanes"), "smem_us": round(tus, 1)}
         log("autotuner: %d lanes per CU, %d lanes per read (%.0f us for the SMEM stage of batch 0)" % (
             lanes, tuned["group_lanes"], tus))
     run_steps.serial = a.no_pipeline
+    run_steps.depth = a.depth
     if a.host_threads:
         m.set("host_threads", a.host_threads)
     elapsed, S = run_steps(m, p, n_batches, a.steps, a.warmup, sync, log)
@@ -127,8 +136,9 @@ def main():
         "config": {"workload": "PE 2x150: 64 genomes x 4 Mb (256 Mbp, GC U(0.3,0.7)), insert N(350,50), "
                                "0.5% sub, 0.05% indel, 5% off-target pairs; 256k-read batches in HBM",
                    "batch_reads": 2 * pairs_per_batch, "bd_length": int(m.n),
-                   "form": ["plain FMD", "unique-interval text path", "unique-interval text path + learned-index read filter"][a.form]},
-        "roofline": {"bound": "hbm", "kernel": "smem_kernel", "achieved": round(achieved, 1),
+                   "form": ["plain FMD", "unique-interval text path",
+                            "learned-index exact-read path and read filter + unique-interval text path"][a.form]},
+        "roofline": {"bound": "hbm", "kernel": smem_kernel_name(m, a.form), "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "bytes_per_launch": bytes_per_launch, "avg_launch_us": round(avg_us, 1)},
         "dp_gcups": round(cells / (dp_us * 1e-6) / 1e9, 3) if dp_us > 0 else None,
@@ -150,6 +160,8 @@ def main():
             "dp_pools_ready": S[:, 23].sum() / a.steps / 1000, "dp_records_back": S[:, 24].sum() / a.steps / 1000,
             "dp_cigar_md_back": S[:, 25].sum() / a.steps / 1000}),
         "dp_jobs_per_step": {"all": float(S[:, 21].mean()), "wavefront_fill": float(S[:, 22].mean())},
+        "host_finish_thread_ms_per_step": {k: S[:, 28 + i].sum() / a.steps / 1000 for i, k in
+                                           enumerate(("sam_text", "pairing", "mapq", "alignment_lists"))},
         "index_build_s": round(build_s, 2),
         "learned_index": dict(lix or {}, **({"exact_path_reads_frac": float(S[:, 26].sum() / max(1.0, S[:, 0].sum())),
                                               "filtered_reads_frac": float(S[:, 27].sum() / max(1.0, S[:, 0].sum()))}
@@ -195,37 +207,44 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def smem_kernel_name(m, form):
+    g = m.get("group_lanes") if form == 2 else 0
+    if g:
+        return "smem_group_kernel<%d>" % g
+    return "smem_kernel_t<%s>" % ("false" if form == 0 else "true")
+
+
 def run_steps(m, p, nb, steps, warmup, sync, log):
-    # (run_steps.serial: submit each batch only after the previous one is finished)
+    # (run_steps.serial: submit each batch only after the previous one is finished;
+    #  run_steps.depth: batches in flight at once)
     """Map `steps` batches (batch k = resident slot k % nb).  With two or more resident
-    batches the runs are pipelined: batch k's host finishing (records, MAPQ, SAM text)
-    overlaps batch k+1's GPU stages.  Returns (seconds, per-step counters)."""
+    batches the runs are pipelined: up to `depth` submitted batches are in flight, so that
+    one batch's device stages overlap another's (two worker streams) and the host finishing
+    of a third.  Returns (seconds, per-step counters)."""
     for w in range(warmup):
         m.submit(w % nb, p, True)
         m.wait(w % nb)
         if log:
             log("warmup step %d done" % w)
+    depth = 1 if (nb == 1 or run_steps.serial) else max(1, min(nb, run_steps.depth))
     sync()
     stats = []
     t = time.time()
-    prev = None
+    inflight = collections.deque()
     tsub = twait = 0.0
     for k in range(steps):
-        slot = k % nb
-        if (nb == 1 or run_steps.serial) and prev is not None:
-            stats.append(m.wait(prev))
-            prev = None
+        if len(inflight) >= depth:
+            t1 = time.time()
+            stats.append(m.wait(inflight.popleft()))
+            twait += time.time() - t1
         t0 = time.time()
-        m.submit(slot, p, True)
+        m.submit(k % nb, p, True)
+        tsub += time.time() - t0
+        inflight.append(k % nb)
+    while inflight:
         t1 = time.time()
-        if prev is not None:
-            stats.append(m.wait(prev))
+        stats.append(m.wait(inflight.popleft()))
         twait += time.time() - t1
-        tsub += t1 - t0
-        prev = slot
-    t1 = time.time()
-    stats.append(m.wait(prev))
-    twait += time.time() - t1
     sync()
     run_steps.split = {"submit_ms_per_step": 1000 * tsub / steps, "wait_ms_per_step": 1000 * twait / steps}
     return time.time() - t, np.array(stats)

@@ -143,7 +143,8 @@ int msgs_align_batch(msgs_ctx* ctx, const msgs_reads* reads, const msgs_params* 
  * [22] wavefront-fill DP jobs, [23]..[25] microseconds into the DP stage when its pool
  * sizes, its records and its CIGAR/MD bytes were back on the host, [26] reads whose SMEM
  * list came from the learned index's exact-read path (form 2), [27] reads the form-2
- * filter skipped (n <= 32). */
+ * filter skipped, [28]..[31] host finishing thread-microseconds (summed over threads) in
+ * SAM text, PE pairing, MAPQ and alignment lists (n <= 32). */
 int msgs_batch_upload(msgs_ctx* ctx, int32_t slot, const msgs_reads* reads, const msgs_reads* mates);
 int msgs_batch_run(msgs_ctx* ctx, int32_t slot, const msgs_params* p, int32_t emitSam);
 /* msgs_batch_run in two halves: submit runs the GPU stages and leaves the host
@@ -195,6 +196,9 @@ int msgs_util_calib_copy(msgs_ctx* ctx, int64_t bytes, double* micros);
 /* random-access ceiling (no reference counterpart): nLoads reads of one 64-byte line each
  * at hashed positions of a device buffer of `bytes` (multiple of 64); *micros = kernel time */
 int msgs_util_random_gather(msgs_ctx* ctx, int64_t bytes, int64_t nLoads, double* micros);
+/* test hook of the SAM writer (no reference counterpart): the "%g" text of each float as
+ * printed in XH:f / XP:f tags, NUL-terminated in out[16 * i .. 16 * i + 16) */
+void msgs_util_format_g6(const float* v, int64_t n, char* out);
 
 /* ---- utilities ----
  * The permutation std::sort (libstdc++) gives for n keys compared with '<' -- the

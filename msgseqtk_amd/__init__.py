@@ -26,7 +26,7 @@ EXPORTS = [
     "msgs_rank_batch", "msgs_ext_batch", "msgs_smem_batch", "msgs_seed_batch",
     "msgs_map_batch", "msgs_align_batch", "msgs_batch_upload", "msgs_batch_run",
     "msgs_batch_stats", "msgs_batch_sam", "msgs_util_sort_by_key",
-    "msgs_batch_submit", "msgs_batch_wait", "msgs_tune", "msgs_util_calib_copy", "msgs_util_random_gather", "msgs_ctx_get", "msgs_lix_train", "msgs_lix_query", "msgs_lix_predict", "msgs_lix_grad", "msgs_lix_weights", "msgs_lix_knots",
+    "msgs_batch_submit", "msgs_batch_wait", "msgs_tune", "msgs_util_calib_copy", "msgs_util_random_gather", "msgs_util_format_g6", "msgs_ctx_get", "msgs_lix_train", "msgs_lix_query", "msgs_lix_predict", "msgs_lix_grad", "msgs_lix_weights", "msgs_lix_knots",
 ]
 N_LIX_PARAMS = 16 * 64 + 64 + 64 + 1
 
@@ -95,6 +95,8 @@ def lib():
                             C.POINTER(C.c_int32), PD]
     L.msgs_util_calib_copy.argtypes = [vp, C.c_int64, PD]
     L.msgs_ctx_get.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int64)]
+    L.msgs_util_format_g6.argtypes = [C.POINTER(C.c_float), C.c_int64, C.c_char_p]
+    L.msgs_util_format_g6.restype = None
     L.msgs_util_random_gather.argtypes = [vp, C.c_int64, C.c_int64, PD]
     L.msgs_batch_submit.argtypes = [vp, C.c_int32, C.POINTER(Params), C.c_int32]
     L.msgs_batch_wait.argtypes = [vp, C.c_int32]

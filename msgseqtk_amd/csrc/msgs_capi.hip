@@ -36,8 +36,53 @@ DevIndex DbState::view() const {
 }
 
 int fail(msgs_ctx* c, int code, const std::string& msg) {
-    if (c) c->err = msg;
+    if (c) c->x().err = msg;
     return code;
+}
+
+/* ------------------------------------------------------------------ executors */
+
+namespace {
+thread_local Exec* tlsExec = nullptr;   /* set by a batch worker for its run */
+}
+
+Exec& msgs_ctx::x() { return tlsExec ? *tlsExec : own; }
+
+bool Exec::create(unsigned streamFlags) {
+    return hipStreamCreateWithFlags(&stream, streamFlags) == hipSuccess && hipEventCreate(&ev0) == hipSuccess &&
+           hipEventCreate(&ev1) == hipSuccess && hipEventCreate(&ev2) == hipSuccess &&
+           hipEventCreate(&ev3) == hipSuccess;
+}
+
+void Exec::destroy() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    bufs.releaseAll();
+    for (hipEvent_t* e : {&ev0, &ev1, &ev2, &ev3})
+        if (*e) { (void)hipEventDestroy(*e); *e = nullptr; }
+    if (stream) { (void)hipStreamDestroy(stream); stream = nullptr; }
+}
+
+/* a free worker executor, bound to the calling thread until releaseWork */
+Exec* msgs_ctx::acquireWork() {
+    std::unique_lock<std::mutex> lk(workMu);
+    int k = -1;
+    workCv.wait(lk, [&]() {
+        for (int i = 0; i < kWorkExecs; ++i)
+            if (!workBusy[i]) { k = i; return true; }
+        return false;
+    });
+    workBusy[k] = true;
+    tlsExec = &work[k];
+    return &work[k];
+}
+
+void msgs_ctx::releaseWork(Exec* e) {
+    tlsExec = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(workMu);
+        workBusy[e - work] = false;
+    }
+    workCv.notify_one();
 }
 
 /* ------------------------------------------------------------------ e-value thresholds */
@@ -122,10 +167,13 @@ int msgs_ctx_create(int device, msgs_ctx** out) {
     c->device = device;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->nCU = prop.multiProcessorCount;
-    if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return MSGS_E_HIP; }
     c->hostThreads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
-        hipEventCreate(&c->ev2) != hipSuccess || hipEventCreate(&c->ev3) != hipSuccess) {
+    /* the worker streams do not synchronize with the null stream */
+    bool ok = c->own.create(hipStreamDefault);
+    for (Exec& e : c->work) ok = ok && e.create(hipStreamNonBlocking);
+    if (!ok) {
+        c->own.destroy();
+        for (Exec& e : c->work) e.destroy();
         delete c;
         return MSGS_E_HIP;
     }
@@ -137,18 +185,16 @@ void msgs_ctx_destroy(msgs_ctx* c) {
     if (!c) return;
     for (auto& s : c->slots) s.join();
     (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(c->own.stream);
     c->db.release();
     c->lix.release();
-    c->bufs.releaseAll();
     for (auto& s : c->slots) s.release();
-    (void)hipEventDestroy(c->ev0); (void)hipEventDestroy(c->ev1);
-    (void)hipEventDestroy(c->ev2); (void)hipEventDestroy(c->ev3);
-    (void)hipStreamDestroy(c->stream);
+    for (Exec& e : c->work) e.destroy();
+    c->own.destroy();
     delete c;
 }
 
-const char* msgs_last_error(msgs_ctx* c) { return c ? c->err.c_str() : "no context"; }
+const char* msgs_last_error(msgs_ctx* c) { return c ? c->own.err.c_str() : "no context"; }
 
 /* ------------------------------------------------------------------ database */
 
@@ -222,7 +268,7 @@ int msgs_db_build(msgs_ctx* c, const uint8_t* seqs, const int64_t* lens, int32_t
     for (int b = 0; b < 16; ++b)
         if (!(db.ll[b] <= 0)) return fail(c, MSGS_E_DB, "base log-frequency above zero");
 
-    hipStream_t st = c->stream;
+    hipStream_t st = c->x().stream;
     /* padded: the SMEM kernel reads the text in aligned 8-byte words */
     if (hipMalloc(&db.dBd, total + 64) != hipSuccess) return fail(c, MSGS_E_NOMEM, "BD text");
     if (hipMemcpyAsync(db.dBd, bd.data(), total, hipMemcpyHostToDevice, st) != hipSuccess)
@@ -262,7 +308,7 @@ double msgs_db_loglik(msgs_ctx* c, int32_t b) {
 int msgs_db_copy(msgs_ctx* c, uint8_t* bd, uint8_t* bwt, int64_t* sa) {
     if (!c || !c->db.n) return fail(c, MSGS_E_STATE, "no database");
     DbState& db = c->db;
-    hipStream_t st = c->stream;
+    hipStream_t st = c->x().stream;
     if (bd && hipMemcpyAsync(bd, db.dBd, db.n, hipMemcpyDeviceToHost, st) != hipSuccess) return fail(c, MSGS_E_HIP, "copy");
     if (bwt && hipMemcpyAsync(bwt, db.dBwt, db.n, hipMemcpyDeviceToHost, st) != hipSuccess) return fail(c, MSGS_E_HIP, "copy");
     if (sa) {
@@ -281,11 +327,11 @@ int msgs_rank_batch(msgs_ctx* c, const uint8_t* sym, const int64_t* pos, int64_t
     if (!c || !c->db.n) return fail(c, MSGS_E_STATE, "no database");
     if (n <= 0) return MSGS_OK;
     hipError_t e = hipSuccess;
-    uint8_t* dS = c->bufs.get<uint8_t>("rk.s", n, e);
-    int64_t* dP = c->bufs.get<int64_t>("rk.p", n, e);
-    int64_t* dO = c->bufs.get<int64_t>("rk.o", n, e);
+    uint8_t* dS = c->x().bufs.get<uint8_t>("rk.s", n, e);
+    int64_t* dP = c->x().bufs.get<int64_t>("rk.p", n, e);
+    int64_t* dO = c->x().bufs.get<int64_t>("rk.o", n, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "rank buffers");
-    hipStream_t st = c->stream;
+    hipStream_t st = c->x().stream;
     (void)hipMemcpyAsync(dS, sym, n, hipMemcpyHostToDevice, st);
     (void)hipMemcpyAsync(dP, pos, n * 8, hipMemcpyHostToDevice, st);
     k_rank<<<gridFor(n), 256, 0, st>>>(c->db.view(), dS, dP, dO, n);
@@ -298,11 +344,11 @@ int msgs_ext_batch(msgs_ctx* c, int64_t* pqs, const uint8_t* bases, int32_t fwd,
     if (!c || !c->db.n) return fail(c, MSGS_E_STATE, "no database");
     if (n <= 0) return MSGS_OK;
     hipError_t e = hipSuccess;
-    int64_t* dQ = c->bufs.get<int64_t>("ex.q", 3 * n, e);
-    uint8_t* dB = c->bufs.get<uint8_t>("ex.b", n, e);
-    int64_t* dR = c->bufs.get<int64_t>("ex.r", n, e);
+    int64_t* dQ = c->x().bufs.get<int64_t>("ex.q", 3 * n, e);
+    uint8_t* dB = c->x().bufs.get<uint8_t>("ex.b", n, e);
+    int64_t* dR = c->x().bufs.get<int64_t>("ex.r", n, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "ext buffers");
-    hipStream_t st = c->stream;
+    hipStream_t st = c->x().stream;
     (void)hipMemcpyAsync(dQ, pqs, 3 * n * 8, hipMemcpyHostToDevice, st);
     (void)hipMemcpyAsync(dB, bases, n, hipMemcpyHostToDevice, st);
     k_ext<<<gridFor(n), 256, 0, st>>>(c->db.view(), dQ, dB, fwd, dR, n);
@@ -336,12 +382,12 @@ int uploadReads(msgs_ctx* c, const msgs_reads* r, const char* pfx, DevReads& out
     out.maxL = maxL;
     out.total = total;
     /* padded: the SMEM kernel reads bases in aligned 8-byte words */
-    out.seq = c->bufs.get<uint8_t>(p + ".seq", total + 64, e);
-    out.offs = c->bufs.get<int64_t>(p + ".offs", n + 1, e);
+    out.seq = c->x().bufs.get<uint8_t>(p + ".seq", total + 64, e);
+    out.offs = c->x().bufs.get<int64_t>(p + ".offs", n + 1, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "read buffers");
     std::vector<int64_t> offs(n + 1);
     for (int64_t i = 0; i <= n; ++i) offs[i] = r->offs[i] - r->offs[0];
-    hipStream_t st = c->stream;
+    hipStream_t st = c->x().stream;
     if (total) (void)hipMemcpyAsync(out.seq, r->seq + r->offs[0], total, hipMemcpyHostToDevice, st);
     (void)hipMemcpyAsync(out.offs, offs.data(), (n + 1) * 8, hipMemcpyHostToDevice, st);
     out.offsHost = offs;
@@ -349,7 +395,7 @@ int uploadReads(msgs_ctx* c, const msgs_reads* r, const char* pfx, DevReads& out
 }
 
 /* Run the SMEM kernel over resident reads; grows the per-read slot count until no
- * read overflows.  Leaves per-read counts and records in c->bufs ("sm.cnt", "sm.out"). */
+ * read overflows.  Leaves per-read counts and records in c->x().bufs ("sm.cnt", "sm.out"). */
 int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out, unsigned long long* dStats) {
     const int64_t n = r.n;
     if (p->minLen <= 0) return fail(c, MSGS_E_ARG, "minLen must be positive");
@@ -365,18 +411,18 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
     /* form 2: skip reads in which no window that an emitted SMEM must contain occurs, and
      * take the SMEM list of exactly matching reads from the learned index */
     a.need = nullptr;
-    c->lastFilterMicros = 0;
+    c->x().lastFilterMicros = 0;
     if (c->form == 2 && c->lix.trained && p->minLen >= 16 && n > 0) {
         uint8_t* need = nullptr;
         SmemRec* exact = nullptr;
-        (void)hipEventRecord(c->ev0, c->stream);
+        (void)hipEventRecord(c->x().ev0, c->x().stream);
         int rc = runLixFilter(c, r, (int32_t)p->minLen, T, a.gate1, &need, &exact, dStats);
         if (rc) return rc;
-        (void)hipEventRecord(c->ev1, c->stream);
-        if (hipEventSynchronize(c->ev1) != hipSuccess) return fail(c, MSGS_E_HIP, "read filter");
+        (void)hipEventRecord(c->x().ev1, c->x().stream);
+        if (hipEventSynchronize(c->x().ev1) != hipSuccess) return fail(c, MSGS_E_HIP, "read filter");
         float ms = 0;
-        (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
-        c->lastFilterMicros = ms * 1000.0;
+        (void)hipEventElapsedTime(&ms, c->x().ev0, c->x().ev1);
+        c->x().lastFilterMicros = ms * 1000.0;
         a.need = need;
         a.exact = exact;
     }
@@ -386,23 +432,23 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
     const int64_t lanes = std::max<int64_t>(256, (std::min<int64_t>(((want + 255) / 256) * 256, lanesMax) / 256) * 256);
     const int64_t candCap = r.maxL + 2;
     hipError_t e = hipSuccess;
-    a.cand = c->bufs.get<SmemCand>("sm.cand", candCap * lanes, e);
+    a.cand = c->x().bufs.get<SmemCand>("sm.cand", candCap * lanes, e);
     a.candCap = candCap;
     a.nLanes = lanes;
-    a.queue = c->bufs.get<unsigned long long>("sm.q", 1, e);
-    a.cnt = c->bufs.get<int32_t>("sm.cnt", n + 1, e);
+    a.queue = c->x().bufs.get<unsigned long long>("sm.q", 1, e);
+    a.cnt = c->x().bufs.get<int32_t>("sm.cnt", n + 1, e);
     a.stats = dStats;
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "smem scratch");
     int cap = c->smemCap;
-    hipStream_t st = c->stream;
+    hipStream_t st = c->x().stream;
     for (int attempt = 0; attempt < 3; ++attempt) {
         a.cap = cap;
-        a.out = c->bufs.get<SmemRec>("sm.out", (size_t)cap * std::max<int64_t>(n, 1), e);
+        a.out = c->x().bufs.get<SmemRec>("sm.out", (size_t)cap * std::max<int64_t>(n, 1), e);
         if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "smem output");
         (void)hipMemsetAsync(a.queue, 0, 8, st);
-        (void)hipEventRecord(c->ev0, st);
+        (void)hipEventRecord(c->x().ev0, st);
         if (n > 0 && a.gate1) {
-            double* pref = c->bufs.get<double>("sm.pref", r.total + n + 1, e);
+            double* pref = c->x().bufs.get<double>("sm.pref", r.total + n + 1, e);
             if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "loglik prefix sums");
             smem_prefix_kernel<<<(unsigned)std::min<int64_t>((n + 255) / 256, 4096), 256, 0, st>>>(c->db.view(), a, pref);
             a.prefLL = pref;
@@ -414,7 +460,7 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
             else if (c->form == 0) smem_kernel_t<false><<<grid, 256, 0, st>>>(c->db.view(), a);
             else smem_kernel_t<true><<<grid, 256, 0, st>>>(c->db.view(), a);
         }
-        (void)hipEventRecord(c->ev1, st);
+        (void)hipEventRecord(c->x().ev1, st);
         if (hipGetLastError() != hipSuccess) return fail(c, MSGS_E_HIP, "smem launch");
         /* largest count */
         std::vector<int32_t> cnt(n);
@@ -424,8 +470,8 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
         for (int64_t i = 0; i < n; ++i) mx = std::max(mx, cnt[i]);
         if (mx <= cap) {
             float ms = 0;
-            (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
-            c->lastSmemMicros = ms * 1000.0;
+            (void)hipEventElapsedTime(&ms, c->x().ev0, c->x().ev1);
+            c->x().lastSmemMicros = ms * 1000.0;
             out.cap = cap;
             out.cnt = a.cnt;
             out.recs = a.out;
@@ -449,8 +495,8 @@ extern "C" int msgs_smem_batch(msgs_ctx* c, const msgs_reads* reads, const msgs_
     if (rc) return rc;
     const int64_t n = r.n;
     std::vector<SmemRec> recs((size_t)sm.cap * n);
-    if (n) (void)hipMemcpyAsync(recs.data(), sm.recs, recs.size() * sizeof(SmemRec), hipMemcpyDeviceToHost, c->stream);
-    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, MSGS_E_HIP, "copy");
+    if (n) (void)hipMemcpyAsync(recs.data(), sm.recs, recs.size() * sizeof(SmemRec), hipMemcpyDeviceToHost, c->x().stream);
+    if (hipStreamSynchronize(c->x().stream) != hipSuccess) return fail(c, MSGS_E_HIP, "copy");
     int64_t total = 0;
     for (int64_t i = 0; i < n; ++i) total += sm.cntHost[i];
     if (needed) *needed = total;
@@ -623,17 +669,17 @@ __global__ void __launch_bounds__(256) k_gather(const uint4* buf, int64_t nLines
 extern "C" int msgs_util_calib_copy(msgs_ctx* c, int64_t bytes, double* micros) {
     if (!c || bytes <= 0 || bytes % 16) return fail(c, MSGS_E_ARG, "bytes must be a positive multiple of 16");
     hipError_t e = hipSuccess;
-    uint4* a = c->bufs.get<uint4>("calib.a", bytes / 16, e);
-    uint4* b = c->bufs.get<uint4>("calib.b", bytes / 16, e);
+    uint4* a = c->x().bufs.get<uint4>("calib.a", bytes / 16, e);
+    uint4* b = c->x().bufs.get<uint4>("calib.b", bytes / 16, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "calibration buffers");
     const int64_t n = bytes / 16;
     const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, (int64_t)std::max(1, c->nCU) * 64);
-    (void)hipEventRecord(c->ev0, c->stream);
-    k_calib_copy<<<blocks, 256, 0, c->stream>>>(a, b, n);
-    (void)hipEventRecord(c->ev1, c->stream);
-    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, MSGS_E_HIP, "calibration copy");
+    (void)hipEventRecord(c->x().ev0, c->x().stream);
+    k_calib_copy<<<blocks, 256, 0, c->x().stream>>>(a, b, n);
+    (void)hipEventRecord(c->x().ev1, c->x().stream);
+    if (hipStreamSynchronize(c->x().stream) != hipSuccess) return fail(c, MSGS_E_HIP, "calibration copy");
     float ms = 0;
-    (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+    (void)hipEventElapsedTime(&ms, c->x().ev0, c->x().ev1);
     if (micros) *micros = ms * 1000.0;
     return MSGS_OK;
 }
@@ -644,18 +690,18 @@ extern "C" int msgs_util_calib_copy(msgs_ctx* c, int64_t bytes, double* micros) 
 extern "C" int msgs_util_random_gather(msgs_ctx* c, int64_t bytes, int64_t nLoads, double* micros) {
     if (!c || bytes <= 0 || bytes % 64 || nLoads <= 0) return fail(c, MSGS_E_ARG, "bytes: positive multiple of 64; nLoads > 0");
     hipError_t e = hipSuccess;
-    uint4* buf = c->bufs.get<uint4>("gather.buf", bytes / 16, e);
-    unsigned long long* sink = c->bufs.get<unsigned long long>("gather.sink", 1, e);
+    uint4* buf = c->x().bufs.get<uint4>("gather.buf", bytes / 16, e);
+    unsigned long long* sink = c->x().bufs.get<unsigned long long>("gather.sink", 1, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "gather buffer");
-    if (hipMemsetAsync(buf, 0x5a, bytes, c->stream) != hipSuccess) return fail(c, MSGS_E_HIP, "gather fill");
+    if (hipMemsetAsync(buf, 0x5a, bytes, c->x().stream) != hipSuccess) return fail(c, MSGS_E_HIP, "gather fill");
     const unsigned blocks = (unsigned)std::max(1, c->nCU) * 32;
     float ms = 0;
     for (int k = 0; k < 2; ++k) {
-        (void)hipEventRecord(c->ev0, c->stream);
-        k_gather<<<blocks, 256, 0, c->stream>>>(buf, bytes / 64, nLoads, 0x1234567ull + k, sink);
-        (void)hipEventRecord(c->ev1, c->stream);
-        if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, MSGS_E_HIP, "gather");
-        (void)hipEventElapsedTime(&ms, c->ev0, c->ev1);
+        (void)hipEventRecord(c->x().ev0, c->x().stream);
+        k_gather<<<blocks, 256, 0, c->x().stream>>>(buf, bytes / 64, nLoads, 0x1234567ull + k, sink);
+        (void)hipEventRecord(c->x().ev1, c->x().stream);
+        if (hipStreamSynchronize(c->x().stream) != hipSuccess) return fail(c, MSGS_E_HIP, "gather");
+        (void)hipEventElapsedTime(&ms, c->x().ev0, c->x().ev1);
     }
     if (micros) *micros = ms * 1000.0;
     return MSGS_OK;

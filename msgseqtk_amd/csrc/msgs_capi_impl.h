@@ -12,6 +12,8 @@
 #include <utility>
 #include <string>
 #include <memory>
+#include <condition_variable>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -105,12 +107,29 @@ struct Slot {
     std::vector<int64_t> lineOffs;
     double stats[32] = {};
     int rc = 0;
+    std::string err;
     void join() {
         if (worker.joinable()) worker.join();
         pending = false;   /* res is kept: its pinned buffers serve the next run */
     }
     void release() { join(); used = false; }
 };
+
+/* what one host thread needs to drive the device path: a stream, timing events, device
+ * pools and the last stage timings.  The caller's thread uses the context's own; each
+ * submitted batch runs its device stages on one of the worker executors, so that two
+ * batches' kernels can overlap on the GPU. */
+struct Exec {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+    BufPool bufs;
+    double lastSmemMicros = 0;
+    double lastFilterMicros = 0;
+    std::string err;
+    bool create(unsigned streamFlags);
+    void destroy();
+};
+constexpr int kWorkExecs = 2;
 
 struct msgs_ctx {
     int device = 0;
@@ -120,17 +139,20 @@ struct msgs_ctx {
     int readFilter = 1;        /* form 2: also skip reads that cannot emit an SMEM */
     int form = 1;
     int groupLanes = 8;   /* form 2: lanes per read of the SMEM kernel (0 = one lane per read) */
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
-    std::string err;
+    Exec own;                      /* the caller's thread */
+    Exec work[kWorkExecs];         /* submitted batches' device stages */
+    bool workBusy[kWorkExecs] = {};
+    std::mutex workMu;
+    std::condition_variable workCv;
     DbState db;
     LixState lix;
-    BufPool bufs;
     Slot slots[16];
     double stats[32] = {};
-    double lastSmemMicros = 0;
-    double lastFilterMicros = 0;
     int hostThreads = 16;
+    /* the executor of the calling thread */
+    Exec& x();
+    Exec* acquireWork();
+    void releaseWork(Exec* e);
     /* SAM text of the last msgs_batch_run: per-thread parts in read order and the
      * offset of each read's records in their concatenation */
     std::vector<std::string> lastParts;

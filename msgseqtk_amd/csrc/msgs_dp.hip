@@ -486,8 +486,18 @@ __global__ void __launch_bounds__(256) dp_trace_kernel(DevIndex ix, DpArgs a) {
         }
         const int T = 2 * ns + 1;
         const int64_t pathCap = (int64_t)L + g.tLen + 3;
-        /* ---- backtrace (src/Alignment.cpp:173-276) ---- */
-        uint8_t* path = a.path + a.pathOff[job];
+        /* ---- backtrace (src/Alignment.cpp:173-276) ----
+         * The path is kept as runs of equal ops (BAM CIGAR words), produced end -> start
+         * into the tail of the job's CIGAR slot: slot words [0, cgCap) */
+        uint32_t* cg = a.cigar + a.pathOff[job] + 2 * job;
+        const int64_t cgCap = pathCap + 2;
+        int64_t nRun = 0;           /* runs written at cg[cgCap - 1], cg[cgCap - 2], ... */
+        uint32_t runOp = 0xffu, runLen = 0;
+        auto push = [&](uint32_t op, uint32_t n) {
+            if (op == runOp) { runLen += n; return; }
+            if (runLen) cg[cgCap - 1 - nRun++] = (runLen << 4) | runOp;
+            runOp = op; runLen = n;
+        };
         int64_t np = 0;
         int32_t alnTo, alnEnd, i, j;
         int32_t best;
@@ -533,21 +543,22 @@ __global__ void __launch_bounds__(256) dp_trace_kernel(DevIndex ix, DpArgs a) {
                     const int32_t t = i - sd.x;
                     if (i < sd.x + sd.z && j - sd.y == t && t >= 2) {
                         const int64_t jmp = min((int64_t)(t - 1), pathCap - np);
-                        for (int64_t x = 0; x < jmp; ++x) path[np + x] = (uint8_t)OP_M;
+                        push(OP_M, (uint32_t)jmp);
                         np += jmp; i -= (int32_t)jmp; j -= (int32_t)jmp;
                         continue;
                     }
                 }
             }
-            path[np++] = (uint8_t)s;
+            push((uint32_t)s, 1u);
+            np++;
             if (s == OP_M) { s = matchMax(vM(G, i - 1, j - 1, T), vI(G, i - 1, j - 1, T), vD(G, i - 1, j - 1, T)); i--; j--; }
             else if (s == OP_I) { s = subn(subn(vM(G, i - 1, j, T), go), ge) > subn(vI(G, i - 1, j, T), ge) ? OP_M : OP_I; i--; }
             else { s = subn(subn(vM(G, i, j - 1, T), go), ge) > subn(vD(G, i, j - 1, T), ge) ? OP_M : OP_D; j--; }
             if (G.mode == 0) { if (G.hazard && j < -1) break; }
             else if (G.hazard && (i < -1 || j < -1)) break;
         }
-        /* path was collected end -> start; reverse it */
-        for (int64_t x = 0, y = np - 1; x < y; ++x, --y) { uint8_t t = path[x]; path[x] = path[y]; path[y] = t; }
+        if (runLen) cg[cgCap - 1 - nRun++] = (runLen << 4) | runOp;
+        /* the runs sit at cg[cgCap - nRun, cgCap) in start -> end order */
         const int64_t alnFrom = i, alnStart = g.tStart + j;
         AlnRec rec;
         rec.rid = g.rid;
@@ -556,55 +567,49 @@ __global__ void __launch_bounds__(256) dp_trace_kernel(DevIndex ix, DpArgs a) {
         rec.alnFrom = alnFrom; rec.alnTo = alnTo; rec.alnStart = alnStart; rec.alnEnd = alnEnd + g.tStart;
         rec.score = best;
         rec.cells = cells;
-        /* ---- getAlnCigar (src/Alignment.cpp:278-303) ---- */
-        uint32_t* cg = a.cigar + a.pathOff[job] + 2 * job;
-        int32_t nc = 0;
-        const int32_t clip5 = (int32_t)(alnFrom - 0), clip3 = (int32_t)(L - alnTo);
-        if (clip5 > 0) cg[nc++] = ((uint32_t)clip5 << 4) | OP_S;
-        if (np > 0) {
-            uint32_t op = path[0], len = 1;
-            for (int64_t x = 1; x < np; ++x) {
-                if (path[x] != op) { cg[nc++] = (len << 4) | op; len = 1; }
-                else len++;
-                op = path[x];
-            }
-            cg[nc++] = (len << 4) | op;
-        }
-        if (clip3 > 0) cg[nc++] = ((uint32_t)clip3 << 4) | OP_S;
-        rec.nCigar = nc;
-        /* ---- getAlnMDTag (src/Alignment.cpp:305-353) and evaluate (:446-489) ---- */
+        /* ---- getAlnMDTag (src/Alignment.cpp:305-353) and evaluate (:446-489), over the runs ---- */
         char* md = a.md + 3 * a.pathOff[job] + 8 * job;
         int64_t ml = 0;
         uint32_t prev = 0xffffffffu, run = 0;
         double x = 0;
         for (int64_t c = 0; c < alnFrom; ++c) x += -a.clip;
         int64_t qi = alnFrom, tj = alnStart;
-        for (int64_t k = 0; k < np; ++k) {
-            const uint8_t qb = (qi >= 0 && qi < L) ? q[qi] : (uint8_t)0;
-            const uint8_t tbv = tb(tj);
-            const int qv = (qi >= 0 && qi < L) ? qual[qi] : 0;
-            uint32_t op = path[k];
-            if (op == OP_M) {
-                x += (qb & tbv) ? a.qMatch[qv] : a.qMis[qv];
-                op = (qb & tbv) ? OP_EQ : OP_X;
-            } else {
-                const bool open = (k == 0 || path[k - 1] != path[k]);
-                if (open) x += a.qGapOpen[qv];
-                x += -a.gapE_d;
+        for (int64_t r = cgCap - nRun; r < cgCap; ++r) {
+            const uint32_t rop = cg[r] & 15u, rlen = cg[r] >> 4;
+            for (uint32_t u = 0; u < rlen; ++u) {
+                const uint8_t qb = (qi >= 0 && qi < L) ? q[qi] : (uint8_t)0;
+                const uint8_t tbv = tb(tj);
+                const int qv = (qi >= 0 && qi < L) ? qual[qi] : 0;
+                uint32_t op = rop;
+                if (op == OP_M) {
+                    x += (qb & tbv) ? a.qMatch[qv] : a.qMis[qv];
+                    op = (qb & tbv) ? OP_EQ : OP_X;
+                } else {
+                    if (u == 0) x += a.qGapOpen[qv];   /* the first op of a run opens a gap */
+                    x += -a.gapE_d;
+                }
+                if (op == OP_EQ) { qi++; tj++; run++; }
+                else if (op == OP_X) { ml += putNum(md, ml, run); run = 0; md[ml++] = decodeBase(tbv); qi++; tj++; }
+                else if (op == OP_I) {
+                    if (prev != OP_I) ml += putNum(md, ml, run);
+                    run = 0;
+                    if (op != prev) md[ml++] = '^';
+                    md[ml++] = decodeBase(qb);
+                    qi++;
+                } else if (op == OP_D) { tj++; }
+                prev = op;
             }
-            if (op == OP_EQ) { qi++; tj++; run++; }
-            else if (op == OP_X) { ml += putNum(md, ml, run); run = 0; md[ml++] = decodeBase(tbv); qi++; tj++; }
-            else if (op == OP_I) {
-                if (prev != OP_I) ml += putNum(md, ml, run);
-                run = 0;
-                if (op != prev) md[ml++] = '^';
-                md[ml++] = decodeBase(qb);
-                qi++;
-            } else if (op == OP_D) { tj++; }
-            prev = op;
         }
         ml += putNum(md, ml, run);
         for (int64_t c = alnTo; c < L; ++c) x += -a.clip;
+        /* ---- getAlnCigar (src/Alignment.cpp:278-303): clips around the runs, moved to the
+         * head of the slot (the head is written upwards from below the runs' start) ---- */
+        int32_t nc = 0;
+        const int32_t clip5 = (int32_t)(alnFrom - 0), clip3 = (int32_t)(L - alnTo);
+        if (clip5 > 0) cg[nc++] = ((uint32_t)clip5 << 4) | OP_S;
+        for (int64_t r = cgCap - nRun; r < cgCap; ++r) cg[nc++] = cg[r];
+        if (clip3 > 0) cg[nc++] = ((uint32_t)clip3 << 4) | OP_S;
+        rec.nCigar = nc;
         rec.mdLen = (int32_t)ml;
         rec.log10P = x;
         rec.pad = (int32_t)np;

@@ -152,8 +152,8 @@ struct DpArgs {
     int64_t nWaveJobs;
     const int64_t* tileOff;    /* per job offset into the tile pool (cells) */
     int4* tile;                /* M, I, D per rectangle cell */
-    const int64_t* pathOff;    /* per job offset into path pool; cigar at +2*job, md at 3x+8*job */
-    uint8_t* path;
+    const int64_t* pathOff;    /* per job: scan of path bounds (L + tLen + 3); the CIGAR slot is at
+                                * pathOff + 2*job, the MD slot at 3*pathOff + 8*job */
     uint32_t* cigar;
     char* md;
     AlnRec* out;

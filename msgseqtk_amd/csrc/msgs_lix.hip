@@ -765,7 +765,7 @@ extern "C" int msgs_lix_train(msgs_ctx* c, int32_t segments, int64_t samples, in
     LixState& L = c->lix;
     L.release();
     const int64_t n = c->db.n;
-    hipStream_t st = c->stream;
+    hipStream_t st = c->x().stream;
     L.n = n;
     L.S = segments;
     LCK(hipMalloc(&L.dCt, n * sizeof(uint64_t)), "CT");
@@ -775,7 +775,7 @@ extern "C" int msgs_lix_train(msgs_ctx* c, int32_t segments, int64_t samples, in
     LCK(hipMalloc(&L.dBucket, (((size_t)1 << kBucketBits) + 1) * sizeof(uint32_t)), "buckets");
     LCK(hipMalloc(&L.dESeg, segments * sizeof(uint32_t)), "bounds");
     LCK(hipMalloc(&L.dESegRoot, segments * sizeof(uint32_t)), "bounds");
-    hipEvent_t e0 = c->ev0, e1 = c->ev1;
+    hipEvent_t e0 = c->x().ev0, e1 = c->x().ev1;
     LCK(hipEventRecord(e0, st), "event");
     k_ct<<<gridFor(n), 256, 0, st>>>(c->db.dBd, c->db.dSa, n, L.dCt);
     k_knots<<<gridFor(segments + 1), 256, 0, st>>>(L.dCt, n, segments, L.dKey, L.dRank);
@@ -784,16 +784,16 @@ extern "C" int msgs_lix_train(msgs_ctx* c, int32_t segments, int64_t samples, in
     initWeights(W, seed);
     LCK(hipMemcpyAsync(L.dW, W.data(), kLixParams * 4, hipMemcpyHostToDevice, st), "W");
     hipError_t e = hipSuccess;
-    float* X = c->bufs.get<float>("lx.X", samples * 16, e);
-    float* T = c->bufs.get<float>("lx.T", samples, e);
-    float* H = c->bufs.get<float>("lx.H", (size_t)batch * 64, e);
-    float* Dout = c->bufs.get<float>("lx.D", batch, e);
-    float* Err = c->bufs.get<float>("lx.E", batch, e);
-    float* G = c->bufs.get<float>("lx.G", kLixParams, e);
-    float* Mo = c->bufs.get<float>("lx.M", kLixParams, e);
-    float* Vo = c->bufs.get<float>("lx.V", kLixParams, e);
-    float* dLoss = c->bufs.get<float>("lx.loss", 1, e);
-    unsigned long long* dErr = c->bufs.get<unsigned long long>("lx.err", 2, e);
+    float* X = c->x().bufs.get<float>("lx.X", samples * 16, e);
+    float* T = c->x().bufs.get<float>("lx.T", samples, e);
+    float* H = c->x().bufs.get<float>("lx.H", (size_t)batch * 64, e);
+    float* Dout = c->x().bufs.get<float>("lx.D", batch, e);
+    float* Err = c->x().bufs.get<float>("lx.E", batch, e);
+    float* G = c->x().bufs.get<float>("lx.G", kLixParams, e);
+    float* Mo = c->x().bufs.get<float>("lx.M", kLixParams, e);
+    float* Vo = c->x().bufs.get<float>("lx.V", kLixParams, e);
+    float* dLoss = c->x().bufs.get<float>("lx.loss", 1, e);
+    unsigned long long* dErr = c->x().bufs.get<unsigned long long>("lx.err", 2, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "training buffers");
     LCK(hipMemsetAsync(Mo, 0, kLixParams * 4, st), "adam");
     LCK(hipMemsetAsync(Vo, 0, kLixParams * 4, st), "adam");
@@ -848,15 +848,15 @@ extern "C" int msgs_lix_grad(msgs_ctx* c, const float* W, const float* X, const 
                              float* loss) {
     if (!c || !W || !X || !T || !grad || B < 1 || B > 65536) return fail(c, MSGS_E_ARG, "bad gradient arguments");
     hipError_t e = hipSuccess;
-    hipStream_t st = c->stream;
-    float* dW = c->bufs.get<float>("gc.W", kLixParams, e);
-    float* dX = c->bufs.get<float>("gc.X", (size_t)B * 16, e);
-    float* dT = c->bufs.get<float>("gc.T", B, e);
-    float* H = c->bufs.get<float>("gc.H", (size_t)B * 64, e);
-    float* Dout = c->bufs.get<float>("gc.D", B, e);
-    float* Err = c->bufs.get<float>("gc.E", B, e);
-    float* G = c->bufs.get<float>("gc.G", kLixParams, e);
-    float* dL = c->bufs.get<float>("gc.L", 1, e);
+    hipStream_t st = c->x().stream;
+    float* dW = c->x().bufs.get<float>("gc.W", kLixParams, e);
+    float* dX = c->x().bufs.get<float>("gc.X", (size_t)B * 16, e);
+    float* dT = c->x().bufs.get<float>("gc.T", B, e);
+    float* H = c->x().bufs.get<float>("gc.H", (size_t)B * 64, e);
+    float* Dout = c->x().bufs.get<float>("gc.D", B, e);
+    float* Err = c->x().bufs.get<float>("gc.E", B, e);
+    float* G = c->x().bufs.get<float>("gc.G", kLixParams, e);
+    float* dL = c->x().bufs.get<float>("gc.L", 1, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "gradient buffers");
     LCK(hipMemcpyAsync(dW, W, kLixParams * 4, hipMemcpyHostToDevice, st), "W");
     LCK(hipMemcpyAsync(dX, X, (size_t)B * 16 * 4, hipMemcpyHostToDevice, st), "X");
@@ -876,11 +876,11 @@ extern "C" int msgs_lix_query(msgs_ctx* c, const uint64_t* keys, int64_t nq, int
     if (!c || !c->lix.trained) return fail(c, MSGS_E_STATE, "learned index not trained");
     if (nq <= 0) return MSGS_OK;
     hipError_t e = hipSuccess;
-    hipStream_t st = c->stream;
-    uint64_t* dK = c->bufs.get<uint64_t>("lq.k", nq, e);
-    int64_t* dL = c->bufs.get<int64_t>("lq.l", nq, e);
-    int32_t* dP = c->bufs.get<int32_t>("lq.p", nq, e);
-    int32_t* dF = c->bufs.get<int32_t>("lq.f", nq, e);
+    hipStream_t st = c->x().stream;
+    uint64_t* dK = c->x().bufs.get<uint64_t>("lq.k", nq, e);
+    int64_t* dL = c->x().bufs.get<int64_t>("lq.l", nq, e);
+    int32_t* dP = c->x().bufs.get<int32_t>("lq.p", nq, e);
+    int32_t* dF = c->x().bufs.get<int32_t>("lq.f", nq, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "query buffers");
     LCK(hipMemcpyAsync(dK, keys, nq * 8, hipMemcpyHostToDevice, st), "keys");
     const LixState& L = c->lix;
@@ -895,20 +895,20 @@ extern "C" int msgs_lix_query(msgs_ctx* c, const uint64_t* keys, int64_t nq, int
 int runLixFilter(msgs_ctx* c, const DevReads& r, int32_t minLen, double T, int32_t gate1, uint8_t** needOut,
                  SmemRec** exactOut, unsigned long long* dStats) {
     hipError_t e = hipSuccess;
-    uint8_t* need = c->bufs.get<uint8_t>("lx.need", r.n + 1, e);
-    SmemRec* exact = c->bufs.get<SmemRec>("lx.exact", r.n + 1, e);
-    int64_t* open = c->bufs.get<int64_t>("lx.open", r.n + 1, e);
-    unsigned long long* nOpen = c->bufs.get<unsigned long long>("lx.nopen", 1, e);
+    uint8_t* need = c->x().bufs.get<uint8_t>("lx.need", r.n + 1, e);
+    SmemRec* exact = c->x().bufs.get<SmemRec>("lx.exact", r.n + 1, e);
+    int64_t* open = c->x().bufs.get<int64_t>("lx.open", r.n + 1, e);
+    unsigned long long* nOpen = c->x().bufs.get<unsigned long long>("lx.nopen", 1, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "filter buffer");
     if (r.n) {
         const int64_t cap = (int64_t)std::max(1, c->nCU) * 32;
-        k_exact<<<(unsigned)std::min<int64_t>((r.n + 127) / 128, cap), 256, 0, c->stream>>>(
+        k_exact<<<(unsigned)std::min<int64_t>((r.n + 127) / 128, cap), 256, 0, c->x().stream>>>(
             c->lix.view(), c->db.view(), r.seq, r.offs, r.n, minLen, T, gate1, need, exact, dStats);
         if (c->readFilter) {
-            LCK(hipMemsetAsync(nOpen, 0, 8, c->stream), "filter list");
-            k_window1<<<(unsigned)std::min<int64_t>((r.n + 31) / 32, cap), 256, 0, c->stream>>>(
+            LCK(hipMemsetAsync(nOpen, 0, 8, c->x().stream), "filter list");
+            k_window1<<<(unsigned)std::min<int64_t>((r.n + 31) / 32, cap), 256, 0, c->x().stream>>>(
                 c->lix.view(), c->db.view(), r.seq, r.offs, r.n, minLen, T, need, open, nOpen, dStats);
-            k_window2<<<(unsigned)cap, 256, 0, c->stream>>>(c->lix.view(), c->db.view(), r.seq, r.offs, minLen, T,
+            k_window2<<<(unsigned)cap, 256, 0, c->x().stream>>>(c->lix.view(), c->db.view(), r.seq, r.offs, minLen, T,
                                                               need, open, nOpen, dStats);
         }
     }
@@ -922,13 +922,13 @@ extern "C" int msgs_lix_predict(msgs_ctx* c, const uint64_t* keys, int64_t nq, i
     if (!c || !c->lix.trained) return fail(c, MSGS_E_STATE, "learned index not trained");
     if (nq <= 0) return MSGS_OK;
     hipError_t e = hipSuccess;
-    uint64_t* dK = c->bufs.get<uint64_t>("lp.k", nq, e);
-    double* dO = c->bufs.get<double>("lp.o", nq, e);
+    uint64_t* dK = c->x().bufs.get<uint64_t>("lp.k", nq, e);
+    double* dO = c->x().bufs.get<double>("lp.o", nq, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "predict buffers");
-    LCK(hipMemcpyAsync(dK, keys, nq * 8, hipMemcpyHostToDevice, c->stream), "keys");
-    k_predict<<<gridFor(nq), 256, 0, c->stream>>>(c->lix.view(), dK, nq, mode, dO);
-    LCK(hipMemcpyAsync(out, dO, nq * 8, hipMemcpyDeviceToHost, c->stream), "out");
-    LCK(hipStreamSynchronize(c->stream), "predict");
+    LCK(hipMemcpyAsync(dK, keys, nq * 8, hipMemcpyHostToDevice, c->x().stream), "keys");
+    k_predict<<<gridFor(nq), 256, 0, c->x().stream>>>(c->lix.view(), dK, nq, mode, dO);
+    LCK(hipMemcpyAsync(out, dO, nq * 8, hipMemcpyDeviceToHost, c->x().stream), "out");
+    LCK(hipStreamSynchronize(c->x().stream), "predict");
     return MSGS_OK;
 }
 

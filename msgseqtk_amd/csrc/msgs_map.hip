@@ -9,8 +9,10 @@
  */
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
+#include <x86intrin.h>
 
 #include <algorithm>
+#include <array>
 #include <charconv>
 #include <chrono>
 #include <cmath>
@@ -23,6 +25,7 @@
 #include <vector>
 
 #include "msgs_capi_impl.h"
+#include "msgs_samfmt.h"
 #include "msgs_sort.h"
 
 namespace {
@@ -166,15 +169,15 @@ struct DpStage {
 static int runSeeds(msgs_ctx* c, const DevReads& r, const msgs_params* p, const SmemStage& sm, SeedStage& out,
                     unsigned long long* dStats) {
     const int64_t n = r.n;
-    hipStream_t st = c->stream;
+    hipStream_t st = c->x().stream;
     hipError_t e = hipSuccess;
     SeedArgs a;
     std::memset(&a, 0, sizeof(a));
     a.offs = r.offs; a.nReads = n; a.smemCnt = sm.cnt; a.smems = sm.recs; a.cap = sm.cap;
     a.maxNSeed = p->maxNSeed;
-    a.order = c->bufs.get<int32_t>("sd.ord", (size_t)sm.cap * std::max<int64_t>(n, 1), e);
-    a.nUsed = c->bufs.get<int32_t>("sd.used", n + 1, e);
-    a.nRaw = c->bufs.get<int64_t>("sd.raw", n + 1, e);
+    a.order = c->x().bufs.get<int32_t>("sd.ord", (size_t)sm.cap * std::max<int64_t>(n, 1), e);
+    a.nUsed = c->x().bufs.get<int32_t>("sd.used", n + 1, e);
+    a.nRaw = c->x().bufs.get<int64_t>("sd.raw", n + 1, e);
     a.stats = dStats;
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "seed scratch");
     if (n) smem_order_kernel<<<gridFor(n), 256, 0, st>>>(c->db.view(), a);
@@ -182,9 +185,9 @@ static int runSeeds(msgs_ctx* c, const DevReads& r, const msgs_params* p, const 
     if (n) CK(hipMemcpyAsync(raw.data(), a.nRaw, n * 8, hipMemcpyDeviceToHost, st), "nRaw");
     CK(hipStreamSynchronize(st), "order kernel");
     const int64_t total = exclusiveScan(raw, out.seedOff);
-    out.dSeedOff = c->bufs.get<int64_t>("sd.off", n + 1, e);
-    out.dSeeds = c->bufs.get<SeedRec>("sd.seeds", total + 1, e);
-    out.dSeedCnt = c->bufs.get<int32_t>("sd.cnt", n + 1, e);
+    out.dSeedOff = c->x().bufs.get<int64_t>("sd.off", n + 1, e);
+    out.dSeeds = c->x().bufs.get<SeedRec>("sd.seeds", total + 1, e);
+    out.dSeedCnt = c->x().bufs.get<int32_t>("sd.cnt", n + 1, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "seed pool");
     CK(hipMemcpyAsync(out.dSeedOff, out.seedOff.data(), (n + 1) * 8, hipMemcpyHostToDevice, st), "seedOff");
     a.seedOff = out.dSeedOff; a.seeds = out.dSeeds; a.seedCnt = out.dSeedCnt;
@@ -197,17 +200,17 @@ static int runSeeds(msgs_ctx* c, const DevReads& r, const msgs_params* p, const 
 
 static int runChains(msgs_ctx* c, const DevReads& r, const SeedStage& sd, ChainStage& out) {
     const int64_t n = r.n;
-    hipStream_t st = c->stream;
+    hipStream_t st = c->x().stream;
     hipError_t e = hipSuccess;
     const int64_t seedTotal = sd.seedOff[n];
     ChainArgs a;
     std::memset(&a, 0, sizeof(a));
     a.offs = r.offs; a.nReads = n; a.seedOff = sd.dSeedOff; a.seedCnt = sd.dSeedCnt; a.seeds = sd.dSeeds;
-    a.stackNode = c->bufs.get<int32_t>("ch.node", seedTotal + 1, e);
-    a.stackNext = c->bufs.get<int32_t>("ch.next", seedTotal + 1, e);
-    a.seen = c->bufs.get<uint8_t>("ch.seen", seedTotal + 1, e);
-    a.nChains = c->bufs.get<int32_t>("ch.n", n + 1, e);
-    a.nChainSeeds = c->bufs.get<int64_t>("ch.ns", n + 1, e);
+    a.stackNode = c->x().bufs.get<int32_t>("ch.node", seedTotal + 1, e);
+    a.stackNext = c->x().bufs.get<int32_t>("ch.next", seedTotal + 1, e);
+    a.seen = c->x().bufs.get<uint8_t>("ch.seen", seedTotal + 1, e);
+    a.nChains = c->x().bufs.get<int32_t>("ch.n", n + 1, e);
+    a.nChainSeeds = c->x().bufs.get<int64_t>("ch.ns", n + 1, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "chain scratch");
     a.pass = 0;
     if (n) chain_kernel<<<gridFor(n), 256, 0, st>>>(a);
@@ -221,12 +224,12 @@ static int runChains(msgs_ctx* c, const DevReads& r, const SeedStage& sd, ChainS
     std::vector<int64_t> csOff;
     out.nJobs = exclusiveScan(nch, out.chainOff);
     out.nChainSeeds = exclusiveScan(ncs, csOff);
-    int64_t* dChainOff = c->bufs.get<int64_t>("ch.coff", n + 1, e);
-    int64_t* dCsOff = c->bufs.get<int64_t>("ch.soff", n + 1, e);
-    out.dChainStart = c->bufs.get<int32_t>("ch.start", out.nJobs + 1, e);
-    out.dChainLen = c->bufs.get<int32_t>("ch.len", out.nJobs + 1, e);
-    out.dChainSeeds = c->bufs.get<int32_t>("ch.seeds", out.nChainSeeds + 1, e);
-    out.dJobRead = c->bufs.get<int32_t>("ch.read", out.nJobs + 1, e);
+    int64_t* dChainOff = c->x().bufs.get<int64_t>("ch.coff", n + 1, e);
+    int64_t* dCsOff = c->x().bufs.get<int64_t>("ch.soff", n + 1, e);
+    out.dChainStart = c->x().bufs.get<int32_t>("ch.start", out.nJobs + 1, e);
+    out.dChainLen = c->x().bufs.get<int32_t>("ch.len", out.nJobs + 1, e);
+    out.dChainSeeds = c->x().bufs.get<int32_t>("ch.seeds", out.nChainSeeds + 1, e);
+    out.dJobRead = c->x().bufs.get<int32_t>("ch.read", out.nJobs + 1, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "chain pool");
     std::vector<int32_t> jobRead(out.nJobs);
     for (int64_t i = 0; i < n; ++i)
@@ -250,7 +253,7 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     const int64_t n = r.n, nJobs = ch.nJobs;
     const auto t0 = std::chrono::steady_clock::now();
     auto since = [&t0]() { return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count(); };
-    hipStream_t st = c->stream;
+    hipStream_t st = c->x().stream;
     hipError_t e = hipSuccess;
     if (!integral(p->match) || !integral(p->mismatch) || !integral(p->gapO) || !integral(p->gapE))
         return fail(c, MSGS_E_UNSUPPORTED, "the device DP needs integer-valued match/mismatch/gap scores");
@@ -259,18 +262,18 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     DpArgs a;
     std::memset(&a, 0, sizeof(a));
     a.seqs = r.seq; a.quals = dQual; a.offs = r.offs;
-    uint8_t* rcS = c->bufs.get<uint8_t>("dp.rs", r.total + 1, e);
-    uint8_t* rcQ = c->bufs.get<uint8_t>("dp.rq", r.total + 1, e);
+    uint8_t* rcS = c->x().bufs.get<uint8_t>("dp.rs", r.total + 1, e);
+    uint8_t* rcQ = c->x().bufs.get<uint8_t>("dp.rq", r.total + 1, e);
     a.rcSeqs = rcS;
     a.rcQuals = rcQ;
     a.seedOff = sd.dSeedOff; a.seeds = sd.dSeeds; a.nJobs = nJobs;
     a.jobRead = ch.dJobRead; a.jobChainStart = ch.dChainStart; a.jobChainLen = ch.dChainLen;
     a.chainSeeds = ch.dChainSeeds;
-    a.auxSG = c->bufs.get<int4>("dp.sg", ch.nChainSeeds + 1, e);
-    a.auxGap = c->bufs.get<int64_t>("dp.gap", ch.nChainSeeds + 1, e);
-    a.geo = c->bufs.get<JobGeo>("dp.geo", nJobs + 1, e);
-    a.sum = c->bufs.get<JobSum>("dp.sum", nJobs + 1, e);
-    a.jobCls = c->bufs.get<int32_t>("dp.cls", nJobs + 1, e);
+    a.auxSG = c->x().bufs.get<int4>("dp.sg", ch.nChainSeeds + 1, e);
+    a.auxGap = c->x().bufs.get<int64_t>("dp.gap", ch.nChainSeeds + 1, e);
+    a.geo = c->x().bufs.get<JobGeo>("dp.geo", nJobs + 1, e);
+    a.sum = c->x().bufs.get<JobSum>("dp.sum", nJobs + 1, e);
+    a.jobCls = c->x().bufs.get<int32_t>("dp.cls", nJobs + 1, e);
     a.mode = p->mode;
     a.match = (int32_t)p->match; a.mismatch = (int32_t)p->mismatch; a.gapO = (int32_t)p->gapO; a.gapE = (int32_t)p->gapE;
     a.clip = p->clip; a.gapE_d = p->gapE;
@@ -282,9 +285,9 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
         hq[256 + q] = q / -10.0;
         hq[512 + q] = q / -10.0 - p->gapO;
     }
-    double* dq = c->bufs.get<double>("dp.qtab", 3 * 256, e);
-    a.tileNeed = c->bufs.get<int64_t>("dp.tneed", nJobs + 1, e);
-    a.pathNeed = c->bufs.get<int64_t>("dp.pneed", nJobs + 1, e);
+    double* dq = c->x().bufs.get<double>("dp.qtab", 3 * 256, e);
+    a.tileNeed = c->x().bufs.get<int64_t>("dp.tneed", nJobs + 1, e);
+    a.pathNeed = c->x().bufs.get<int64_t>("dp.pneed", nJobs + 1, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "dp scratch");
     CK(hipMemcpyAsync(dq, hq, sizeof(hq), hipMemcpyHostToDevice, st), "qtab");
     a.qMatch = dq; a.qMis = dq + 256; a.qGapOpen = dq + 512;
@@ -296,12 +299,12 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     size_t scanBytes = 0;
     if (hipcub::DeviceScan::ExclusiveSum(nullptr, scanBytes, a.tileNeed, a.tileNeed, (int)(nJobs + 1), st) != hipSuccess)
         return fail(c, MSGS_E_HIP, "scan size");
-    void* scanTmp = c->bufs.get<uint8_t>("dp.scantmp", scanBytes + 1, e);
-    int64_t* dTileOff = c->bufs.get<int64_t>("dp.toff", nJobs + 1, e);
-    int64_t* dPathOff = c->bufs.get<int64_t>("dp.poff", nJobs + 1, e);
-    int64_t* dWave = c->bufs.get<int64_t>("dp.wave", nJobs + 1, e);
-    unsigned long long* dHist = c->bufs.get<unsigned long long>("dp.whist", 128, e);
-    int64_t* dTot = c->bufs.get<int64_t>("dp.tot", 8, e);
+    void* scanTmp = c->x().bufs.get<uint8_t>("dp.scantmp", scanBytes + 1, e);
+    int64_t* dTileOff = c->x().bufs.get<int64_t>("dp.toff", nJobs + 1, e);
+    int64_t* dPathOff = c->x().bufs.get<int64_t>("dp.poff", nJobs + 1, e);
+    int64_t* dWave = c->x().bufs.get<int64_t>("dp.wave", nJobs + 1, e);
+    unsigned long long* dHist = c->x().bufs.get<unsigned long long>("dp.whist", 128, e);
+    int64_t* dTot = c->x().bufs.get<int64_t>("dp.tot", 8, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "dp scans");
     CK(hipMemsetAsync(dHist, 0, 128 * 8, st), "hist");
     CK(hipcub::DeviceScan::ExclusiveSum(scanTmp, scanBytes, a.tileNeed, dTileOff, (int)(nJobs + 1), st), "tile scan");
@@ -316,27 +319,26 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     CK(hipStreamSynchronize(st), "dp setup");
     const int64_t tiles = out.tot[0], paths = out.tot[1], nWave = out.tot[2];
     if (phase) { phase[0] = (double)nJobs; phase[1] = (double)nWave; phase[2] = since(); }
-    a.tile = c->bufs.get<int4>("dp.tile", tiles + 1, e);
-    a.path = c->bufs.get<uint8_t>("dp.path", paths + 1, e);
-    a.cigar = c->bufs.get<uint32_t>("dp.cig", paths + 2 * nJobs + 1, e);
-    a.md = c->bufs.get<char>("dp.md", 3 * paths + 8 * nJobs + 1, e);
-    a.out = c->bufs.get<AlnRec>("dp.out", nJobs + 1, e);
-    int64_t* dCOff = c->bufs.get<int64_t>("dp.coff", nJobs + 1, e);
-    int64_t* dMOff = c->bufs.get<int64_t>("dp.moff", nJobs + 1, e);
-    int64_t* dNc = c->bufs.get<int64_t>("dp.nc", nJobs + 1, e);
-    int64_t* dNm = c->bufs.get<int64_t>("dp.nm", nJobs + 1, e);
-    uint32_t* dC = c->bufs.get<uint32_t>("dp.cc", paths + 2 * nJobs + 1, e);
-    char* dM = c->bufs.get<char>("dp.mc", 3 * paths + 8 * nJobs + 1, e);
+    a.tile = c->x().bufs.get<int4>("dp.tile", tiles + 1, e);
+    a.cigar = c->x().bufs.get<uint32_t>("dp.cig", paths + 2 * nJobs + 1, e);
+    a.md = c->x().bufs.get<char>("dp.md", 3 * paths + 8 * nJobs + 1, e);
+    a.out = c->x().bufs.get<AlnRec>("dp.out", nJobs + 1, e);
+    int64_t* dCOff = c->x().bufs.get<int64_t>("dp.coff", nJobs + 1, e);
+    int64_t* dMOff = c->x().bufs.get<int64_t>("dp.moff", nJobs + 1, e);
+    int64_t* dNc = c->x().bufs.get<int64_t>("dp.nc", nJobs + 1, e);
+    int64_t* dNm = c->x().bufs.get<int64_t>("dp.nm", nJobs + 1, e);
+    uint32_t* dC = c->x().bufs.get<uint32_t>("dp.cc", paths + 2 * nJobs + 1, e);
+    char* dM = c->x().bufs.get<char>("dp.mc", 3 * paths + 8 * nJobs + 1, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "dp pools");
     a.tileOff = dTileOff; a.pathOff = dPathOff;
     a.waveJobs = dWave; a.nWaveJobs = nWave;
-    CK(hipEventRecord(c->ev2, st), "event");
+    CK(hipEventRecord(c->x().ev2, st), "event");
     if (nWave) {
         const int64_t blocks = std::min<int64_t>(nWave, (int64_t)std::max(1, c->nCU) * 16);
         dp_fill_kernel<<<(unsigned)blocks, 64, 0, st>>>(c->db.view(), a);
     }
     dp_trace_kernel<<<gridFor(nJobs), 256, 0, st>>>(c->db.view(), a);
-    CK(hipEventRecord(c->ev3, st), "event");
+    CK(hipEventRecord(c->x().ev3, st), "event");
     /* compaction of CIGAR ops / MD bytes, offsets scanned on the device */
     k_rec_counts<<<gridFor(nJobs + 1), 256, 0, st>>>(a.out, nJobs, dNc, dNm);
     CK(hipcub::DeviceScan::ExclusiveSum(scanTmp, scanBytes, dNc, dCOff, (int)(nJobs + 1), st), "cigar scan");
@@ -359,7 +361,7 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     if (phase) phase[4] = since();
     if (dpMicros) {
         float ms = 0;
-        (void)hipEventElapsedTime(&ms, c->ev2, c->ev3);
+        (void)hipEventElapsedTime(&ms, c->x().ev2, c->x().ev3);
         *dpMicros = ms * 1000.0;
     }
     return MSGS_OK;
@@ -378,20 +380,6 @@ struct HostRead {
     const char* name;
     int64_t nameLen;
 };
-
-inline void putInt(std::string& s, int64_t v) {
-    char b[24];
-    const auto r = std::to_chars(b, b + sizeof b, v);
-    s.append(b, r.ptr - b);
-}
-
-/* "%g" of a float widened to double (std::to_chars with general format and precision 6
- * is specified as printf's %.6g) */
-inline void putFloat(std::string& s, float f) {
-    char b[48];
-    const auto r = std::to_chars(b, b + sizeof b, (double)f, std::chars_format::general, 6);
-    s.append(b, r.ptr - b);
-}
 
 struct HAln {
     const AlnRec* r;
@@ -446,6 +434,8 @@ struct Finisher {
     bool emit = true;
     int64_t mapped = 0;
     const double log10e = ::log10(M_E);
+    /* time-stamp-counter ticks spent in: [0] SAM text, [1] pairing, [2] MAPQ, [3] alignment lists */
+    mutable uint64_t tk[4] = {0, 0, 0, 0};
 
     std::string cigarOf(int64_t job) const {
         static const char kOps[] = "MIDNSHP=XB";
@@ -457,57 +447,53 @@ struct Finisher {
     }
 
     /* one SAM text record (field layout of htslib's sam_format1) */
-    void line(std::string& s, const HostRead& rd, bool rc, int flag, int32_t tid, int64_t pos, int mapq,
+    void line(msgs_sam::Out& out, const HostRead& rd, bool rc, int flag, int32_t tid, int64_t pos, int mapq,
               const uint32_t* cg, int nc, int32_t isize, const int64_t* tags, const char* md, int mdLen, float xh,
               float xp) const {
-        static const char kNt16[17] = "=ACMGRSVTWYHKDBN";
-        static const char kNt16c[17] = "=TGKCYSBAWRDMHVN";   /* complement of each code */
         static const char kOps[] = "MIDNSHP=XB";
         if (!emit) return;
-        s.append(rd.name, rd.nameLen);
-        s += '\t'; putInt(s, flag);
-        s += '\t'; if (tid >= 0) s += db.names[tid]; else s += '*';
-        s += '\t'; putInt(s, pos + 1);
-        s += '\t'; putInt(s, mapq);
-        s += '\t';
-        if (nc == 0) s += '*';
-        for (int k = 0; k < nc; ++k) { putInt(s, cg[k] >> 4); s += kOps[cg[k] & 15]; }
-        s += "\t*\t0\t"; putInt(s, isize);
-        s += '\t';
-        if (rd.L == 0) s += '*';
-        else {
-            const size_t o = s.size();
-            s.resize(o + rd.L);
-            char* d = &s[o];
-            if (rc) for (int64_t i = 0; i < rd.L; ++i) d[i] = kNt16c[rd.seq[rd.L - 1 - i] & 15];
-            else for (int64_t i = 0; i < rd.L; ++i) d[i] = kNt16[rd.seq[i] & 15];
-        }
-        s += '\t';
-        if (rd.L == 0 || (rc ? rd.qual[rd.L - 1] : rd.qual[0]) == 0xff) s += '*';
-        else {
-            const size_t o = s.size();
-            s.resize(o + rd.L);
-            char* d = &s[o];
-            if (rc) for (int64_t i = 0; i < rd.L; ++i) d[i] = (char)(rd.qual[rd.L - 1 - i] + 33);
-            else for (int64_t i = 0; i < rd.L; ++i) d[i] = (char)(rd.qual[i] + 33);
-        }
+        const uint64_t t0 = __rdtsc();
+        const std::string* cn = tid >= 0 ? &db.names[tid] : nullptr;
+        const size_t bound = (size_t)rd.nameLen + (cn ? cn->size() : 1) + 2 * (size_t)rd.L + 16 * (size_t)nc +
+                             (size_t)mdLen + 256;
+        char* p = out.reserve(bound);
+        std::memcpy(p, rd.name, rd.nameLen); p += rd.nameLen;
+        *p++ = '\t'; p = msgs_sam::putInt(p, flag);
+        *p++ = '\t';
+        if (cn) { std::memcpy(p, cn->data(), cn->size()); p += cn->size(); } else *p++ = '*';
+        *p++ = '\t'; p = msgs_sam::putInt(p, pos + 1);
+        *p++ = '\t'; p = msgs_sam::putInt(p, mapq);
+        *p++ = '\t';
+        if (nc == 0) *p++ = '*';
+        for (int k = 0; k < nc; ++k) { p = msgs_sam::putInt(p, cg[k] >> 4); *p++ = kOps[cg[k] & 15]; }
+        std::memcpy(p, "\t*\t0\t", 5); p += 5;
+        p = msgs_sam::putInt(p, isize);
+        *p++ = '\t';
+        if (rd.L == 0) *p++ = '*';
+        else p = msgs_sam::putSeq(p, rd.seq, rd.L, rc);
+        *p++ = '\t';
+        if (rd.L == 0 || (rc ? rd.qual[rd.L - 1] : rd.qual[0]) == 0xff) *p++ = '*';
+        else p = msgs_sam::putQual(p, rd.qual, rd.L, rc);
         if (tags) {
-            s += "\tAS:i:"; putInt(s, tags[0]);
-            s += "\tMD:Z:"; s.append(md, mdLen);
-            s += "\tXH:f:"; putFloat(s, xh);
-            s += "\tNH:i:"; putInt(s, tags[1]);
-            s += "\tXN:i:"; putInt(s, tags[2]);
-            s += "\tXP:f:"; putFloat(s, xp);
+            std::memcpy(p, "\tAS:i:", 6); p = msgs_sam::putInt(p + 6, tags[0]);
+            std::memcpy(p, "\tMD:Z:", 6); p += 6;
+            std::memcpy(p, md, mdLen); p += mdLen;
+            std::memcpy(p, "\tXH:f:", 6); p = msgs_sam::putG6(p + 6, xh);
+            std::memcpy(p, "\tNH:i:", 6); p = msgs_sam::putInt(p + 6, tags[1]);
+            std::memcpy(p, "\tXN:i:", 6); p = msgs_sam::putInt(p + 6, tags[2]);
+            std::memcpy(p, "\tXP:f:", 6); p = msgs_sam::putG6(p + 6, xp);
         }
-        s += '\n';
+        *p++ = '\n';
+        out.commit(p);
+        tk[0] += __rdtsc() - t0;
     }
 
-    void unmapped(std::string& s, const HostRead& rd, int flag) const {
+    void unmapped(msgs_sam::Out& s, const HostRead& rd, int flag) const {
         line(s, rd, false, flag, -1, 0, 0, nullptr, 0, 0, nullptr, nullptr, 0, 0, 0);
     }
 
     /* exportBAM + output() (src/Alignment.cpp:429-444, src/msgseqtk-align.cpp:538-557) */
-    void alnRecord(std::string& s, const HostRead& rd, const HAln& a, int flagOr, int mapq, int32_t isize,
+    void alnRecord(msgs_sam::Out& s, const HostRead& rd, const HAln& a, int flagOr, int mapq, int32_t isize,
                    int64_t nh, int64_t xn, double postP) const {
         const AlnRec& r = *a.r;
         const int64_t job = &r - dp.recs.data();
@@ -527,6 +513,7 @@ struct Finisher {
 
     /* filter (src/Alignment.cpp:548-555): drop alnScore < L * minScoreRate */
     void alnList(const HostRead& rd, int64_t j0, int64_t j1, std::vector<HAln>& v) const {
+        const uint64_t t0 = __rdtsc();
         v.clear();
         const double thr = (double)rd.L * P.minScoreRate;
         for (int64_t j = j0; j < j1; ++j) {
@@ -534,27 +521,30 @@ struct Finisher {
             double sc = r.score == NEG ? -std::numeric_limits<double>::infinity() : (double)r.score;
             if (!(sc < thr)) v.push_back(HAln{&r, r.log10P});
         }
+        tk[3] += __rdtsc() - t0;
     }
 
     void calcMapQ(std::vector<HAln>& v) {
+        const uint64_t t0 = __rdtsc();
         ll.resize(v.size());
         for (size_t i = 0; i < v.size(); ++i) ll[i] = v[i].log10P / log10e;
         mapq(ll, pr, pp, mq, false);
         for (size_t i = 0; i < v.size(); ++i) { v[i].postP = pp[i]; v[i].mapQ = mq[i]; }
+        tk[2] += __rdtsc() - t0;
     }
 
     static void sortAlns(std::vector<HAln>& v) {
         msgs_sort::sort(v.data(), v.data() + v.size(), [](const HAln& l, const HAln& r) { return l.log10P > r.log10P; });
     }
 
-    void outputAlns(std::string& s, const HostRead& rd, const std::vector<HAln>& v, int extra) const {
+    void outputAlns(msgs_sam::Out& s, const HostRead& rd, const std::vector<HAln>& v, int extra) const {
         size_t num = P.maxReport == 0 ? v.size() : std::min((size_t)(uint32_t)P.maxReport, v.size());
         for (size_t i = 0; i < num; ++i)
             alnRecord(s, rd, v[i], extra | (i > 0 ? 256 : 0), v[i].mapQ, 0, (int64_t)num, (int64_t)v.size(), v[i].postP);
     }
 
     /* main_SE task (src/msgseqtk-align.cpp:607-649) */
-    void se(std::string& s, const HostRead& rd, int32_t nSeeds, int64_t j0, int64_t j1) {
+    void se(msgs_sam::Out& s, const HostRead& rd, int32_t nSeeds, int64_t j0, int64_t j1) {
         if (nSeeds == 0) { unmapped(s, rd, 4); return; }
         std::vector<HAln>& v = fBuf;
         alnList(rd, j0, j1, v);
@@ -580,7 +570,7 @@ struct Finisher {
     }
 
     /* main_PE task (src/msgseqtk-align.cpp:679-754) */
-    void pe(std::string& s, const HostRead& r1, const HostRead& r2, int32_t ns1, int32_t ns2, int64_t a0, int64_t a1,
+    void pe(msgs_sam::Out& s, const HostRead& r1, const HostRead& r2, int32_t ns1, int32_t ns2, int64_t a0, int64_t a1,
             int64_t b0, int64_t b1) {
         if (ns1 == 0 && ns2 == 0) { unmapped(s, r1, 4 | 8 | 1 | 64); unmapped(s, r2, 4 | 8 | 1 | 128); return; }
         std::vector<HAln>& f = fBuf;
@@ -588,6 +578,7 @@ struct Finisher {
         alnList(r1, a0, a1, f);
         alnList(r2, b0, b1, r);
         if (f.empty() && r.empty()) { unmapped(s, r1, 4 | 8 | 1 | 64); unmapped(s, r2, 4 | 8 | 1 | 128); return; }
+        const uint64_t tp0 = __rdtsc();
         std::vector<Pr>& pairs = prBuf;
         pairs.clear();
         for (const HAln& x : f)
@@ -611,11 +602,14 @@ struct Finisher {
             return (a.alnStart <= b.alnStart && a.alnEnd >= b.alnEnd) || (b.alnStart <= a.alnStart && b.alnEnd >= a.alnEnd); });
         if (P.noOverlap) rm([](const Pr& q) { return q.f->r->alnStart < q.r->r->alnEnd && q.f->r->alnEnd > q.r->r->alnStart; });
         if (P.maxNPair > 0 && (int64_t)pairs.size() > P.maxNPair) pairs.erase(pairs.begin() + P.maxNPair, pairs.end());
+        const uint64_t tp1 = __rdtsc();
+        tk[1] += tp1 - tp0;
         if (!pairs.empty()) {
             mapped += 2;
             ll.resize(pairs.size());
             for (size_t i = 0; i < pairs.size(); ++i) ll[i] = pairs[i].ll;
             mapq(ll, pr, pp, mq, true);
+            tk[2] += __rdtsc() - tp1;
             size_t num = P.maxReport == 0 ? pairs.size() : std::min((size_t)(uint32_t)P.maxReport, pairs.size());
             for (size_t i = 0; i < num; ++i) {
                 const Pr& q = pairs[i];
@@ -660,11 +654,11 @@ struct BatchResult {
 
 static int uploadQual(msgs_ctx* c, const msgs_reads* r, const DevReads& d, const char* name, uint8_t** out) {
     hipError_t e = hipSuccess;
-    uint8_t* q = c->bufs.get<uint8_t>(name, d.total + 1, e);
+    uint8_t* q = c->x().bufs.get<uint8_t>(name, d.total + 1, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "quality buffer");
     if (d.total) {
-        if (r->qual) CK(hipMemcpyAsync(q, r->qual + r->offs[0], d.total, hipMemcpyHostToDevice, c->stream), "qual");
-        else k_fill_u8<<<gridFor(d.total), 256, 0, c->stream>>>(q, d.total, 30);
+        if (r->qual) CK(hipMemcpyAsync(q, r->qual + r->offs[0], d.total, hipMemcpyHostToDevice, c->x().stream), "qual");
+        else k_fill_u8<<<gridFor(d.total), 256, 0, c->x().stream>>>(q, d.total, 30);
     }
     *out = q;
     return MSGS_OK;
@@ -705,8 +699,8 @@ static int impl_seed_batch(msgs_ctx* c, const msgs_reads* reads, const msgs_para
     if (rc) return rc;
     const int64_t n = r.n;
     std::vector<SeedRec> seeds(sd.seedOff[n] + 1);
-    CK(hipMemcpyAsync(seeds.data(), sd.dSeeds, seeds.size() * sizeof(SeedRec), hipMemcpyDeviceToHost, c->stream), "seeds");
-    CK(hipStreamSynchronize(c->stream), "sync");
+    CK(hipMemcpyAsync(seeds.data(), sd.dSeeds, seeds.size() * sizeof(SeedRec), hipMemcpyDeviceToHost, c->x().stream), "seeds");
+    CK(hipStreamSynchronize(c->x().stream), "sync");
     int64_t total = 0;
     for (int64_t i = 0; i < n; ++i) total += sd.seedCnt[i];
     if (needed) *needed = total;
@@ -788,7 +782,7 @@ extern "C" int msgs_align_batch(msgs_ctx* c, const msgs_reads* reads, const msgs
 static int64_t samText(msgs_ctx* c, const msgs_params* p, const msgs_reads* reads, const msgs_reads* mates,
                        const BatchResult& b, bool emit, std::vector<std::string>& part, std::vector<int64_t>& lineOffs,
                        const std::vector<int64_t>* names1 = nullptr, const std::vector<int64_t>* names2 = nullptr,
-                       int threads = 0) {
+                       int threads = 0, double* phaseUs = nullptr) {
     const int64_t n = reads->n;
     std::vector<int64_t> own1, own2;
     if (!names1) { splitNames(reads->names, n, own1); names1 = &own1; }
@@ -800,6 +794,9 @@ static int64_t samText(msgs_ctx* c, const msgs_params* p, const msgs_reads* read
     part.resize(nt);
     for (std::string& x : part) x.clear();   /* keeps the capacity of recycled parts */
     std::vector<int64_t> mappedPart(nt, 0);
+    std::vector<std::array<uint64_t, 4>> tkPart(nt);
+    const uint64_t tsc0 = __rdtsc();
+    const auto w0 = std::chrono::steady_clock::now();
     auto work = [&](int t) {
         const int64_t i0 = n * t / nt, i1 = n * (t + 1) / nt;
         Finisher F{c->db, *p, b.dp};
@@ -815,8 +812,8 @@ static int64_t samText(msgs_ctx* c, const msgs_params* p, const msgs_reads* read
             else { h.name = "r"; h.nameLen = 1; }
             return h;
         };
-        std::string& o = part[t];
-        if (emit) o.reserve((size_t)(i1 - i0) * (mates ? 2 : 1) * (size_t)(160 + 2 * (reads->offs[n] - reads->offs[0]) / std::max<int64_t>(1, n)));
+        msgs_sam::Out o;
+        o.buf = &part[t];
         for (int64_t i = i0; i < i1; ++i) {
             lineOffs[i] = (int64_t)o.size();
             HostRead h1 = host(reads, nm1, i);
@@ -828,7 +825,9 @@ static int64_t samText(msgs_ctx* c, const msgs_params* p, const msgs_reads* read
                      b.ch.chainOff[n + i], b.ch.chainOff[n + i + 1]);
             }
         }
+        o.finish();
         mappedPart[t] = F.mapped;
+        for (int k = 0; k < 4; ++k) tkPart[t][k] = F.tk[k];
     };
     std::vector<std::thread> th;
     for (int t = 1; t < nt; ++t) th.emplace_back(work, t);
@@ -842,6 +841,16 @@ static int64_t samText(msgs_ctx* c, const msgs_params* p, const msgs_reads* read
         mapped += mappedPart[t];
     }
     lineOffs[n] = base;
+    if (phaseUs) {
+        /* ticks to microseconds with the counter's rate over this call */
+        const double us = std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - w0).count();
+        const double perUs = (double)(__rdtsc() - tsc0) / std::max(1.0, us);
+        for (int k = 0; k < 4; ++k) {
+            uint64_t sum = 0;
+            for (int t = 0; t < nt; ++t) sum += tkPart[t][k];
+            phaseUs[k] = perUs > 0 ? (double)sum / perUs : 0;
+        }
+    }
     return mapped;
 }
 
@@ -949,7 +958,7 @@ extern "C" int msgs_batch_upload(msgs_ctx* c, int32_t slot, const msgs_reads* re
     std::snprintf(pfx, sizeof pfx, "slot%d.q", slot);
     rc = uploadQual(c, all, S.dev, pfx, &S.dQual);
     if (rc) return rc;
-    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(c, MSGS_E_HIP, "upload");
+    if (hipStreamSynchronize(c->x().stream) != hipSuccess) return fail(c, MSGS_E_HIP, "upload");
     S.paired = mates != nullptr;
     S.used = true;
     return MSGS_OK;
@@ -966,26 +975,18 @@ extern "C" int msgs_batch_submit(msgs_ctx* c, int32_t slot, const msgs_params* p
     }
 }
 
-static int batchSubmit(msgs_ctx* c, int32_t slot, const msgs_params* p, int32_t emitSam) {
-    if (!c || !c->db.n) return fail(c, MSGS_E_STATE, "no database");
-    if (slot < 0 || slot >= 16 || !c->slots[slot].used) return fail(c, MSGS_E_STATE, "slot not uploaded");
-    int rc = checkParams(c, p);
-    if (rc) return rc;
-    Slot& S = c->slots[slot];
-    S.join();
+/* the device stages of a submitted batch, on the calling worker's executor; fills S.stats */
+static int batchGpu(msgs_ctx* c, Slot& S, std::chrono::steady_clock::time_point t0) {
+    const msgs_params* p = &S.params;
     hipError_t e = hipSuccess;
-    unsigned long long* dStats = c->bufs.get<unsigned long long>("stats", 16, e);
+    unsigned long long* dStats = c->x().bufs.get<unsigned long long>("stats", 16, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "stats");
-    CK(hipMemsetAsync(dStats, 0, 16 * 8, c->stream), "stats");
-    const auto t0 = std::chrono::steady_clock::now();
-    if (!S.res) {
-        if (!c->spareResults.empty()) { S.res = c->spareResults.back(); c->spareResults.pop_back(); }
-        else S.res = std::make_shared<BatchResult>();
-    }
+    CK(hipMemsetAsync(dStats, 0, 16 * 8, c->x().stream), "stats");
     std::shared_ptr<BatchResult> b = S.res;
     double seedUs = 0, smemUs = 0, dpUs = 0, filterUs = 0;
     double wall[4] = {0, 0, 0, 0};   /* host wall time of the SMEM, seed, chain and DP stages */
     double dpPhase[5] = {0, 0, 0, 0, 0};
+    int rc;
     {
         auto us = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
             return std::chrono::duration<double, std::micro>(b - a).count();
@@ -994,8 +995,8 @@ static int batchSubmit(msgs_ctx* c, int32_t slot, const msgs_params* p, int32_t 
         rc = runSmem(c, S.dev, p, b->sm, dStats);
         if (rc) return rc;
         auto s1 = std::chrono::steady_clock::now();
-        smemUs += c->lastSmemMicros;
-        filterUs += c->lastFilterMicros;
+        smemUs += c->x().lastSmemMicros;
+        filterUs += c->x().lastFilterMicros;
         rc = runSeeds(c, S.dev, p, b->sm, b->sd, dStats);
         if (rc) return rc;
         auto s2 = std::chrono::steady_clock::now();
@@ -1011,7 +1012,8 @@ static int batchSubmit(msgs_ctx* c, int32_t slot, const msgs_params* p, int32_t 
         wall[0] = us(s0, s1); wall[1] = us(s1, s2); wall[2] = us(s2, s3); wall[3] = us(s3, s4);
     }
     unsigned long long hs[16];
-    CK(hipMemcpy(hs, dStats, sizeof(hs), hipMemcpyDeviceToHost), "stats");
+    CK(hipMemcpyAsync(hs, dStats, sizeof(hs), hipMemcpyDeviceToHost, c->x().stream), "stats");
+    CK(hipStreamSynchronize(c->x().stream), "stats");
     double* st = S.stats;
     std::memset(st, 0, sizeof(S.stats));
     st[0] = (double)(S.host[0].view.n * (S.paired ? 2 : 1));
@@ -1031,20 +1033,50 @@ static int batchSubmit(msgs_ctx* c, int32_t slot, const msgs_params* p, int32_t 
     for (int k = 0; k < 5; ++k) st[21 + k] = dpPhase[k];
     st[26] = (double)hs[8];
     st[27] = (double)hs[9];
+    return MSGS_OK;
+}
+
+static int batchSubmit(msgs_ctx* c, int32_t slot, const msgs_params* p, int32_t emitSam) {
+    if (!c || !c->db.n) return fail(c, MSGS_E_STATE, "no database");
+    if (slot < 0 || slot >= 16 || !c->slots[slot].used) return fail(c, MSGS_E_STATE, "slot not uploaded");
+    int rc = checkParams(c, p);
+    if (rc) return rc;
+    Slot& S = c->slots[slot];
+    S.join();
+    if (!S.res) {
+        if (!c->spareResults.empty()) { S.res = c->spareResults.back(); c->spareResults.pop_back(); }
+        else S.res = std::make_shared<BatchResult>();
+    }
     S.params = *p;
-    S.res = b;
     S.pending = true;
-    const bool emit = emitSam != 0;
     S.rc = MSGS_OK;
+    S.err.clear();
+    const bool emit = emitSam != 0;
+    const auto t0 = std::chrono::steady_clock::now();
     S.worker = std::thread([c, &S, emit, t0]() {
+        (void)hipSetDevice(c->device);
+        /* device stages on a free worker executor (two batches can be in them at once) */
+        Exec* X = c->acquireWork();
+        int rc;
+        try {
+            rc = batchGpu(c, S, t0);
+        } catch (const std::bad_alloc&) {
+            rc = fail(c, MSGS_E_NOMEM, "host memory");
+        }
+        if (rc) S.err = X->err;
+        c->releaseWork(X);
+        if (rc) { S.rc = rc; return; }
+        /* host finishing */
         const auto h0 = std::chrono::steady_clock::now();
         int64_t mapped = 0;
         try {
             mapped = samText(c, &S.params, &S.host[0].view, S.paired ? &S.host[1].view : nullptr, *S.res, emit,
                              S.parts, S.lineOffs, &S.host[0].nameOff, S.paired ? &S.host[1].nameOff : nullptr,
-                             std::max(1, c->hostThreads - 1));   /* one core stays with the submitting thread */
+                             std::max(1, c->hostThreads - 1),   /* one core stays with the device stages */
+                             S.stats + 28);
         } catch (const std::bad_alloc&) {
             S.rc = MSGS_E_NOMEM;
+            S.err = "host memory while finishing records";
         }
         int64_t cells = 0;
         for (const AlnRec& a : S.res->dp.recs) cells += a.cells;
@@ -1065,7 +1097,7 @@ extern "C" int msgs_batch_wait(msgs_ctx* c, int32_t slot) {
     if (!S.pending) return fail(c, MSGS_E_STATE, "slot has no submitted run");
     S.join();
     if (S.res) { c->spareResults.push_back(S.res); S.res.reset(); }
-    if (S.rc) return fail(c, S.rc, "host memory while finishing records");
+    if (S.rc) return fail(c, S.rc, S.err);
     std::memcpy(c->stats, S.stats, sizeof(c->stats));
     std::swap(c->lastParts, S.parts);
     std::swap(c->lastLineOffs, S.lineOffs);
@@ -1126,7 +1158,7 @@ extern "C" int msgs_tune(msgs_ctx* c, int32_t slot, const msgs_params* p, const 
                 SmemStage sm;
                 rc = runSmem(c, S.dev, p, sm, nullptr);
                 if (rc) { c->lanesPerCU = keep; c->groupLanes = keepG; return rc; }
-                t = std::min(t, c->lastSmemMicros + c->lastFilterMicros);
+                t = std::min(t, c->x().lastSmemMicros + c->x().lastFilterMicros);
             }
             if (t < bestUs) { bestUs = t; bestL = cand[i]; bestG = c->groupLanes; }
         }

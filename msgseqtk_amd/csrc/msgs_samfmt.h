@@ -1,0 +1,37 @@
+/*
+ * msgs_samfmt.h -- SAM text fields written through a raw cursor (host code).
+ *
+ * The record layout is htslib's sam_format1 as called by the reference's output path
+ * (src/msgseqtk-align.cpp:538-593, src/SAMtools/BAM.cpp:20-56); float tags are set with
+ * BAM::setAux(double) -> float (src/SAMtools/BAM.h:618-620) and printed by htslib as
+ * "%g" of the float widened to double.
+ */
+#pragma once
+#include <cstdint>
+#include <string>
+
+namespace msgs_sam {
+
+/* decimal integer */
+char* putInt(char* p, int64_t v);
+/* printf("%g", (double)f): six significant digits, rounded half-to-even on the exact
+ * binary value, trailing zeros removed */
+char* putG6(char* p, float f);
+/* SEQ of an nt16 read (reverse complement when rc) and QUAL (+33, reversed when rc) */
+char* putSeq(char* p, const uint8_t* s, int64_t L, bool rc);
+char* putQual(char* p, const uint8_t* q, int64_t L, bool rc);
+
+/* an output text with a write cursor: grows its string in large steps; finish() trims it */
+struct Out {
+    std::string* buf = nullptr;
+    size_t n = 0;
+    size_t size() const { return n; }
+    char* reserve(size_t k) {
+        if (n + k > buf->size()) buf->resize(std::max(n + k, 2 * buf->size() + ((size_t)1 << 16)));
+        return &(*buf)[n];
+    }
+    void commit(const char* end) { n = (size_t)(end - buf->data()); }
+    void finish() { buf->resize(n); }
+};
+
+}  // namespace msgs_sam

@@ -457,14 +457,40 @@ template __global__ void smem_kernel_t<true>(DevIndex, SmemArgs);
 
 /* per read, P[0] = 0 and P[i+1] = P[i] + loglik(base i) (the gates' estimates) */
 __global__ void __launch_bounds__(256) smem_prefix_kernel(DevIndex ix, SmemArgs a, double* pref) {
-    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.nReads; r += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t o = a.offs[r], L = a.offs[r + 1] - o;
-        double* P = pref + o + r;
+    /* one read per lane, summed in read order; each chunk of 16 sums per read is staged in
+     * LDS and stored by 16 lanes at a time, so that stores fill whole 128-byte lines */
+    constexpr int kChunk = 16;
+    __shared__ double sv[4][64][kChunk + 1];
+    __shared__ int64_t sBase[4][64];
+    __shared__ int32_t sLen[4][64];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int64_t r0 = ((int64_t)blockIdx.x * 4 + w) * 64; r0 < a.nReads; r0 += (int64_t)gridDim.x * 4 * 64) {
+        const int64_t r = r0 + lane;
+        const bool mine = r < a.nReads;
+        const int64_t o = mine ? a.offs[r] : 0;
+        const int32_t L = mine ? (int32_t)(a.offs[r + 1] - o) : -1;
+        sBase[w][lane] = o + r;
+        sLen[w][lane] = L;
+        int32_t Lmax = L;
+        for (int d = 32; d >= 1; d >>= 1) Lmax = max(Lmax, __shfl_xor(Lmax, d, 64));
         double x = 0;
-        P[0] = 0;
-        for (int64_t i = 0; i < L; ++i) {
-            x += ix.ll[a.seqs[o + i] & 15];
-            P[i + 1] = x;
+        for (int32_t k0 = 0; k0 <= Lmax; k0 += kChunk) {
+            for (int t = 0; t < kChunk; ++t) {
+                const int32_t k = k0 + t;
+                if (k >= 1 && k <= L) x += ix.ll[a.seqs[o + k - 1] & 15];
+                sv[w][lane][t] = x;   /* P[k] */
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            for (int j = lane; j < 64 * kChunk; j += 64) {
+                const int rr = j / kChunk, t = j % kChunk;
+                const int32_t k = k0 + t;
+                if (k <= sLen[w][rr]) pref[sBase[w][rr] + k] = sv[w][rr][t];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
     }
 }

@@ -127,12 +127,20 @@ def test_pipelined_batches(env):
     m.upload(2, (se[0], se[1], se[2]))
     exp_pe, _ = O.orc_map(odb, O.pe_params(350, 50), s1, q1, n1, mates=(s2, q2, n2))
     exp_se, _ = O.orc_map(odb, O.default_params(), se[0], se[1], se[2])
+    m.upload(3, (se[0], se[1], se[2]))
+    exp_local, _ = O.orc_map(odb, O.default_params(mode=1), se[0], se[1], se[2])
+    pm_local = M.default_params(mode=1)
     for rep in range(3):
+        # three batches in flight: two run their device stages at once on the two worker
+        # streams, the third waits for a free one
         m.submit(1, pm_pe, True)
         m.submit(2, pm_se, True)
+        m.submit(3, pm_local, True)
         m.wait(1)
         assert m.sam() == "".join(exp_pe), rep
         m.wait(2)
         assert m.sam() == "".join(exp_se), rep
+        m.wait(3)
+        assert m.sam() == "".join(exp_local), rep
     with pytest.raises(RuntimeError):
         m.wait(2)   # nothing pending
