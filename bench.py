@@ -52,6 +52,7 @@ def parse():
     ap.add_argument("--depth", type=int, default=3, help="batches in flight (device stages of two overlap)")
     ap.add_argument("--lanes-per-cu", type=int, default=0, help="fix the SMEM kernel's lanes per CU (no autotuning)")
     ap.add_argument("--group-lanes", type=int, default=-1, help="fix the form-2 lanes per read: 0, 8 or 16")
+    ap.add_argument("--text-path", type=int, default=-1, help="fix the form-2 one-lane kernel's text path: 0 or 1")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 1 GiB device copy (counter calibration under rocprofv3 --pmc)")
     return ap.parse_args()
@@ -105,9 +106,12 @@ def main():
         m.set("lanes_per_cu", a.lanes_per_cu)
     if a.group_lanes >= 0:
         m.set("group_lanes", a.group_lanes)
+    if a.text_path >= 0:
+        m.set("text_path", a.text_path)
     if not a.no_tune and not a.lanes_per_cu:
         lanes, tus = m.tune(0, p)
-        tuned = {"lanes_per_cu": lanes, "group_lanes": m.get("group_lanes"), "smem_us": round(tus, 1)}
+        tuned = {"lanes_per_cu": lanes, "group_lanes": m.get("group_lanes"), "text_path": m.get("text_path"),
+                 "read_filter": m.get("read_filter"), "smem_us": round(tus, 1)}
         log("autotuner: %d lanes per CU, %d lanes per read (%.0f us for the SMEM stage of batch 0)" % (
             lanes, tuned["group_lanes"], tus))
     run_steps.serial = a.no_pipeline
@@ -186,6 +190,7 @@ def main():
         m.set("form", a.form)
         if tuned:
             m.set("lanes_per_cu", tuned["lanes_per_cu"]).set("group_lanes", tuned["group_lanes"])
+            m.set("text_path", tuned["text_path"]).set("read_filter", tuned["read_filter"])
     if not a.no_gather:
         # random 64-B line gathers over buffers the size of the occ table and of occ + SA + ISA:
         # the random-access ceiling the SMEM kernel's lookups are held against
@@ -211,7 +216,8 @@ def smem_kernel_name(m, form):
     g = m.get("group_lanes") if form == 2 else 0
     if g:
         return "smem_group_kernel<%d>" % g
-    return "smem_kernel_t<%s>" % ("false" if form == 0 else "true")
+    fast = form == 1 or (form == 2 and m.get("text_path") == 1)
+    return "smem_kernel_t<%s>" % ("true" if fast else "false")
 
 
 def run_steps(m, p, nb, steps, warmup, sync, log):

@@ -67,6 +67,8 @@ const char* msgs_last_error(msgs_ctx* ctx);
  *   lower bounds, and (with "read_filter" 1, the default) reads in which no SMEM can occur
  *   are skipped;
  * "group_lanes" lanes per read of the form-2 search kernel (0 = one lane per read, 8, 16);
+ * "text_path" form 2 with one lane per read: 1 follows unique intervals through the text
+ *   (as form 1), 0 extends with occ blocks only (as form 0);
  * "lanes_per_cu" persistent lanes of the SMEM kernel per CU; "smem_cap" SMEM slots per
  * read before a regrow; "host_threads" threads of the host finishing step.  Results never
  * depend on these settings. */
@@ -157,8 +159,9 @@ int msgs_batch_stats(msgs_ctx* ctx, double* stats, int32_t n);
 int msgs_batch_sam(msgs_ctx* ctx, char* out, int64_t cap, int64_t* needed);
 /* autotuner (no reference counterpart): times the SMEM search of resident batch `slot`
  * with each lanes-per-CU value in cand[0..nCand) (NULL: 128..1024), in form 2 for each
- * lanes-per-read choice (0, 8, 16) too, keeps the fastest in the context ("lanes_per_cu",
- * "group_lanes") and reports the lanes per CU; results never depend on the choice. */
+ * search kernel too ("group_lanes" 8 / 16, or one lane per read with "text_path" 1 / 0)
+ * with and without the "read_filter", keeps the fastest in the context and reports the
+ * lanes per CU; results never depend on the choice. */
 int msgs_tune(msgs_ctx* ctx, int32_t slot, const msgs_params* p, const int32_t* cand, int32_t nCand, int32_t* best,
               double* bestMicros);
 

@@ -415,14 +415,12 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
     if (c->form == 2 && c->lix.trained && p->minLen >= 16 && n > 0) {
         uint8_t* need = nullptr;
         SmemRec* exact = nullptr;
-        (void)hipEventRecord(c->x().ev0, c->x().stream);
+        /* timed with the second event pair, read after the search has finished (no host
+         * wait between the two) */
+        (void)hipEventRecord(c->x().ev2, c->x().stream);
         int rc = runLixFilter(c, r, (int32_t)p->minLen, T, a.gate1, &need, &exact, dStats);
         if (rc) return rc;
-        (void)hipEventRecord(c->x().ev1, c->x().stream);
-        if (hipEventSynchronize(c->x().ev1) != hipSuccess) return fail(c, MSGS_E_HIP, "read filter");
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, c->x().ev0, c->x().ev1);
-        c->x().lastFilterMicros = ms * 1000.0;
+        (void)hipEventRecord(c->x().ev3, c->x().stream);
         a.need = need;
         a.exact = exact;
     }
@@ -457,7 +455,7 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
             const unsigned grid = (unsigned)(lanes / 256);
             if (G == 8) smem_group_kernel<8><<<grid, 256, 0, st>>>(c->db.view(), a);
             else if (G == 16) smem_group_kernel<16><<<grid, 256, 0, st>>>(c->db.view(), a);
-            else if (c->form == 0) smem_kernel_t<false><<<grid, 256, 0, st>>>(c->db.view(), a);
+            else if (c->form == 0 || (c->form == 2 && !c->textPath)) smem_kernel_t<false><<<grid, 256, 0, st>>>(c->db.view(), a);
             else smem_kernel_t<true><<<grid, 256, 0, st>>>(c->db.view(), a);
         }
         (void)hipEventRecord(c->x().ev1, st);
@@ -472,6 +470,11 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
             float ms = 0;
             (void)hipEventElapsedTime(&ms, c->x().ev0, c->x().ev1);
             c->x().lastSmemMicros = ms * 1000.0;
+            if (a.need) {
+                float fm = 0;
+                (void)hipEventElapsedTime(&fm, c->x().ev2, c->x().ev3);
+                c->x().lastFilterMicros = fm * 1000.0;
+            }
             out.cap = cap;
             out.cnt = a.cnt;
             out.recs = a.out;
@@ -621,6 +624,7 @@ extern "C" int msgs_ctx_set(msgs_ctx* c, const char* key, int64_t value) {
     else if (k == "smem_cap") { if (value < 1 || value > 4096) return fail(c, MSGS_E_ARG, "smem_cap in [1, 4096]"); c->smemCap = (int)value; }
     else if (k == "host_threads") { if (value < 1 || value > 256) return fail(c, MSGS_E_ARG, "host_threads in [1, 256]"); c->hostThreads = (int)value; }
     else if (k == "read_filter") { if (value != 0 && value != 1) return fail(c, MSGS_E_ARG, "read_filter: 0 or 1"); c->readFilter = (int)value; }
+    else if (k == "text_path") { if (value != 0 && value != 1) return fail(c, MSGS_E_ARG, "text_path: 0 or 1"); c->textPath = (int)value; }
     else return fail(c, MSGS_E_ARG, "unknown option " + k);
     return MSGS_OK;
 }
@@ -634,6 +638,7 @@ extern "C" int msgs_ctx_get(msgs_ctx* c, const char* key, int64_t* value) {
     else if (k == "smem_cap") *value = c->smemCap;
     else if (k == "host_threads") *value = c->hostThreads;
     else if (k == "read_filter") *value = c->readFilter;
+    else if (k == "text_path") *value = c->textPath;
     else return fail(c, MSGS_E_ARG, "unknown option " + k);
     return MSGS_OK;
 }

@@ -137,6 +137,7 @@ struct msgs_ctx {
     int lanesPerCU = 1024;
     int smemCap = 32;
     int readFilter = 1;        /* form 2: also skip reads that cannot emit an SMEM */
+    int textPath = 1;          /* form 2 with one lane per read: follow unique intervals through the text */
     int form = 1;
     int groupLanes = 8;   /* form 2: lanes per read of the SMEM kernel (0 = one lane per read) */
     Exec own;                      /* the caller's thread */

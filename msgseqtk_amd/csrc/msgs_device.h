@@ -149,7 +149,10 @@ __device__ __forceinline__ void occPair(const DevIndex& ix, int64_t a, int64_t b
 }
 
 /* FMDIndex::backExt (src/FMDIndex.cpp:577-610): bi-interval [p, q, s] extended by b
- * on the left.  The q update adds the '\0' count and the counts of bases above b. */
+ * on the left.  The q update adds the '\0' count and the counts of bases above b.
+ * All five ranks are counted at both positions in one uniform sequence: lanes of a
+ * wavefront extend different intervals, and a conditional cheaper count would run
+ * both sequences under divergence. */
 __device__ __forceinline__ int64_t dBackExt(const DevIndex& ix, int64_t& p, int64_t& q, int64_t& s, uint8_t b) {
     if (amb(b)) return (s = 0);
     const int c = toInt(b);

@@ -53,7 +53,8 @@ __device__ __forceinline__ int32_t max3(int32_t a, int32_t b, int32_t c) { retur
 /* a rectangle with its tile offset and event index */
 struct Rc {
     int32_t r0, r1, c0, c1, ev;
-    int64_t off;
+    int64_t off;               /* first cell in the job's decision (or tile) region */
+    int64_t boff;              /* first cell in the job's border region */
     bool ok;
     __device__ int64_t idx(int32_t i, int32_t j) const { return off + (int64_t)(j - c0) * (r1 - r0 + 1) + (i - r0); }
     __device__ bool hasCol(int32_t j) const { return j >= c0 && j <= c1; }
@@ -62,12 +63,17 @@ struct Rc {
 
 struct Geo {
     const int4* sg;          /* bi, bj, len, d0 per seed */
-    const int64_t* gapOff;   /* tile offset of gap rect k */
-    int4* tile;              /* M, I, D per rectangle cell */
+    const int64_t* gapOff;   /* cell offset of gap rect k */
+    const int64_t* gapBOff;  /* border offset of gap rect k */
+    int4* tile;              /* serially filled jobs: M, I, D per rectangle cell */
+    uint8_t* bits;           /* wavefront-filled jobs: backtrace decisions per rectangle cell */
+    int4* border;            /* wavefront-filled jobs: last row and last column of each rectangle */
     int32_t L, tLen, ns, mode, match, mis, go, ge, o;
     int32_t r5r1, r5c1, r3r0, r3c0;
-    int64_t r3off;
+    int64_t r3off, r3boff;
     int32_t hazard;
+    bool serial;             /* rectangle values kept in full (tile) */
+    int32_t interior;        /* set if an interior value of a border-only rectangle was asked for */
 };
 
 __device__ __forceinline__ int32_t initM(const Geo& G, int32_t i, int32_t j) {
@@ -98,13 +104,13 @@ __device__ int colSeg(const Geo& G, int32_t j) {
 
 __device__ __forceinline__ Rc rectR5(const Geo& G) {
     Rc R;
-    R.r0 = 1; R.r1 = G.r5r1; R.c0 = 1; R.c1 = G.r5c1; R.ev = 1; R.off = 0;
+    R.r0 = 1; R.r1 = G.r5r1; R.c0 = 1; R.c1 = G.r5c1; R.ev = 1; R.off = 0; R.boff = 0;
     R.ok = G.r5r1 >= 1 && G.r5c1 >= 1;
     return R;
 }
 __device__ __forceinline__ Rc rectR3(const Geo& G) {
     Rc R;
-    R.r0 = G.r3r0; R.r1 = G.L; R.c0 = G.r3c0; R.c1 = G.tLen; R.ev = 2 * G.ns + 1; R.off = G.r3off;
+    R.r0 = G.r3r0; R.r1 = G.L; R.c0 = G.r3c0; R.c1 = G.tLen; R.ev = 2 * G.ns + 1; R.off = G.r3off; R.boff = G.r3boff;
     R.ok = R.r0 <= R.r1 && R.c0 <= R.c1;
     return R;
 }
@@ -112,6 +118,7 @@ __device__ __forceinline__ Rc rectR3(const Geo& G) {
 __device__ __forceinline__ Rc rectGap(const Geo& G, int k, const int4& s, const int4& t) {
     Rc R;
     R.r0 = s.x + s.z; R.r1 = t.x - 1; R.c0 = s.y + s.z; R.c1 = t.y - 1; R.ev = 3 + 2 * k; R.off = G.gapOff[k];
+    R.boff = G.gapBOff[k];
     R.ok = R.r0 <= R.r1 && R.c0 <= R.c1;
     return R;
 }
@@ -121,11 +128,23 @@ __device__ __forceinline__ Rc rectAfter(const Geo& G, int k, const int4& s) {
     return rectR3(G);
 }
 
+/* M, I, D of a cell of rectangle R after its fill.  Wavefront-filled jobs keep only each
+ * rectangle's last row and last column: the backtrace takes decisions inside a rectangle
+ * from the decision bytes, and a cell outside every rectangle has its neighbours inside
+ * one only on that rectangle's last row or column.  Any other request is flagged. */
+__device__ __forceinline__ int4 rectVal(const Geo& G, const Rc& R, int32_t i, int32_t j) {
+    if (G.serial) return G.tile[R.idx(i, j)];
+    if (i == R.r1) return G.border[R.boff + (j - R.c0)];
+    if (j == R.c1) return G.border[R.boff + (R.c1 - R.c0 + 1) + (i - R.r0)];
+    const_cast<Geo&>(G).interior = 1;
+    return make_int4(NEG, NEG, NEG, 0);
+}
+
 /* M at (i, j) after events 1..e; (i, j) inside the matrix, k = rowSeg(i) */
 __device__ int32_t vMk(const Geo& G, int32_t i, int32_t j, int e, int k) {
     if (k < 0) {
         const Rc R = rectR5(G);
-        if (i >= 1 && R.ok && e >= 1 && R.hasCol(j)) return G.tile[R.idx(i, j)].x;
+        if (i >= 1 && R.ok && e >= 1 && R.hasCol(j)) return rectVal(G, R, i, j).x;
         return initM(G, i, j);
     }
     const int4 s = G.sg[k];
@@ -138,7 +157,7 @@ __device__ int32_t vMk(const Geo& G, int32_t i, int32_t j, int e, int k) {
         return initM(G, i, j);
     }
     const Rc R = rectAfter(G, k, s);
-    if (R.ok && e >= R.ev && R.hasCol(j)) return G.tile[R.idx(i, j)].x;
+    if (R.ok && e >= R.ev && R.hasCol(j)) return rectVal(G, R, i, j).x;
     if (e >= 2 + 2 * k && i == s.x + s.z && j == s.y + s.z) return s.w + s.z * G.match;   /* step c */
     return initM(G, i, j);
 }
@@ -148,7 +167,7 @@ __device__ int32_t vIk(const Geo& G, int32_t i, int32_t j, int e, int k) {
         const int4 s = G.sg[k];
         if (i >= s.x + s.z) {
             const Rc R = rectAfter(G, k, s);
-            if (R.ok && e >= R.ev && R.hasCol(j)) return G.tile[R.idx(i, j)].y;
+            if (R.ok && e >= R.ev && R.hasCol(j)) return rectVal(G, R, i, j).y;
             /* I row to_k + 1 := M row to_k + o at seed k's event */
             if (i == s.x + s.z && e >= 2 + 2 * k) return addn(vMk(G, i - 1, j, 2 + 2 * k, k), G.o);
         } else if (k >= 1 && e >= 2 * k) {
@@ -157,7 +176,7 @@ __device__ int32_t vIk(const Geo& G, int32_t i, int32_t j, int e, int k) {
         }
     } else {
         const Rc R = rectR5(G);
-        if (i >= 1 && R.ok && e >= 1 && R.hasCol(j)) return G.tile[R.idx(i, j)].y;
+        if (i >= 1 && R.ok && e >= 1 && R.hasCol(j)) return rectVal(G, R, i, j).y;
     }
     return initI(G, i, j);
 }
@@ -165,12 +184,12 @@ __device__ int32_t vIk(const Geo& G, int32_t i, int32_t j, int e, int k) {
 __device__ int32_t vDk(const Geo& G, int32_t i, int32_t j, int e, int k) {
     if (k < 0) {
         const Rc R = rectR5(G);
-        if (i >= 1 && R.ok && e >= 1 && R.hasCol(j)) return G.tile[R.idx(i, j)].z;
+        if (i >= 1 && R.ok && e >= 1 && R.hasCol(j)) return rectVal(G, R, i, j).z;
     } else {
         const int4 s = G.sg[k];
         if (i >= s.x + s.z) {
             const Rc R = rectAfter(G, k, s);
-            if (R.ok && e >= R.ev && R.hasCol(j)) return G.tile[R.idx(i, j)].z;
+            if (R.ok && e >= R.ev && R.hasCol(j)) return rectVal(G, R, i, j).z;
         }
     }
     /* D col end_m + 1 := M col end_m + o at seed m's event; m is colSeg(j) or the one before */
@@ -193,6 +212,19 @@ __device__ __forceinline__ bool inside(Geo& G, int32_t i, int32_t j) {
 __device__ int32_t vM(Geo& G, int32_t i, int32_t j, int e) { return inside(G, i, j) ? vMk(G, i, j, e, rowSeg(G, i)) : NEG; }
 __device__ int32_t vI(Geo& G, int32_t i, int32_t j, int e) { return inside(G, i, j) ? vIk(G, i, j, e, rowSeg(G, i)) : NEG; }
 __device__ int32_t vD(Geo& G, int32_t i, int32_t j, int e) { return inside(G, i, j) ? vDk(G, i, j, e, rowSeg(G, i)) : NEG; }
+
+/* the rectangle holding cell (i, j), if any (the band rules of vMk) */
+__device__ bool cellRect(const Geo& G, int32_t i, int32_t j, Rc& R) {
+    if (i < 1 || j < 1 || i > G.L || j > G.tLen) return false;
+    const int k = rowSeg(G, i);
+    if (k < 0) R = rectR5(G);
+    else {
+        const int4 s = G.sg[k];
+        if (i < s.x + s.z) return false;
+        R = rectAfter(G, k, s);
+    }
+    return R.ok && i >= R.r0 && i <= R.r1 && R.hasCol(j);
+}
 
 /* first diagonal value of seed k (src/Alignment.cpp:146-150), from its corner at event 1+2k */
 __device__ int32_t seedD0(Geo& G, int k) {
@@ -231,10 +263,15 @@ __device__ __forceinline__ int putNum(char* md, int64_t pos, uint32_t v) {
 __device__ __forceinline__ void makeGeo(Geo& G, const DpArgs& a, const JobGeo& g, const int4* sg, int64_t job) {
     G.sg = sg;
     G.gapOff = a.auxGap + g.cs;
-    G.tile = a.tile + a.tileOff[job];
+    G.gapBOff = a.auxGapB + g.cs;
+    G.serial = g.cls == CLS_SERIAL;
+    G.tile = a.tile + a.serialOff[job];
+    G.bits = a.bits + a.tileOff[job];
+    G.border = a.border + a.borderOff[job];
+    G.interior = 0;
     G.L = g.L; G.tLen = g.tLen; G.ns = g.ns; G.mode = a.mode;
     G.match = a.match; G.mis = a.mismatch; G.go = a.gapO; G.ge = a.gapE; G.o = -(a.gapO + a.gapE);
-    G.r5r1 = g.r5r1; G.r5c1 = g.r5c1; G.r3r0 = g.r3r0; G.r3c0 = g.r3c0; G.r3off = g.r3off;
+    G.r5r1 = g.r5r1; G.r5c1 = g.r5c1; G.r3r0 = g.r3r0; G.r3c0 = g.r3c0; G.r3off = g.r3off; G.r3boff = g.r3boff;
     G.hazard = 0;
 }
 
@@ -243,7 +280,9 @@ __device__ __forceinline__ void makeGeo(Geo& G, const DpArgs& a, const JobGeo& g
 /* ------------------------------------------------------------------ setup */
 
 __global__ void __launch_bounds__(256) dp_setup_kernel(DevIndex ix, DpArgs a) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) { a.tileNeed[a.nJobs] = 0; a.pathNeed[a.nJobs] = 0; }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.tileNeed[a.nJobs] = 0; a.pathNeed[a.nJobs] = 0; a.borderNeed[a.nJobs] = 0; a.serialNeed[a.nJobs] = 0;
+    }
     for (int64_t job = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; job < a.nJobs;
          job += (int64_t)gridDim.x * blockDim.x) {
         const int32_t rid = a.jobRead[job];
@@ -267,6 +306,7 @@ __global__ void __launch_bounds__(256) dp_setup_kernel(DevIndex ix, DpArgs a) {
         g.r5r1 = (int32_t)f.from; g.r5c1 = (int32_t)(f.start - tStart);
         if (!(0 < f.from && tStart < f.start)) { g.r5r1 = 0; g.r5c1 = 0; }
         int64_t area = (int64_t)g.r5r1 * g.r5c1;
+        int64_t bord = (g.r5r1 >= 1 && g.r5c1 >= 1) ? (int64_t)g.r5r1 + g.r5c1 : 0;
         int32_t maxCols = g.r5r1 >= 1 ? g.r5c1 : 0;
         for (int k = 0; k < ns; ++k) {
             const SeedRec& s = seeds[idx[k]];
@@ -275,18 +315,27 @@ __global__ void __launch_bounds__(256) dp_setup_kernel(DevIndex ix, DpArgs a) {
                 const SeedRec& t = seeds[idx[k + 1]];
                 const int64_t rows = t.from - s.to, cols = t.start - s.end;
                 a.auxGap[cs + k] = area;
-                if (rows > 0 && cols > 0) { area += rows * cols; maxCols = max(maxCols, (int32_t)cols); }
+                a.auxGapB[cs + k] = bord;
+                if (rows > 0 && cols > 0) { area += rows * cols; bord += rows + cols; maxCols = max(maxCols, (int32_t)cols); }
             }
         }
         g.r3off = area;
+        g.r3boff = bord;
         g.r3r0 = (int32_t)(l.to + 1); g.r3c0 = (int32_t)(l.end - tStart + 1);
         if (!(l.to < L && l.end < tEnd)) { g.r3r0 = g.L + 1; g.r3c0 = g.tLen + 1; }
-        else { area += (int64_t)(g.L - g.r3r0 + 1) * (g.tLen - g.r3c0 + 1); maxCols = max(maxCols, g.tLen - g.r3c0 + 1); }
+        else {
+            const int64_t rows = g.L - g.r3r0 + 1, cols = g.tLen - g.r3c0 + 1;
+            area += rows * cols;
+            bord += rows + cols;
+            maxCols = max(maxCols, (int32_t)cols);
+        }
         g.cls = area == 0 ? CLS_NONE : ((ns <= FILL_NS && maxCols <= FILL_COLS) ? CLS_WAVE : CLS_SERIAL);
         a.geo[job] = g;
         a.jobCls[job] = g.cls;
         a.tileNeed[job] = area;
         a.pathNeed[job] = L + g.tLen + 3;
+        a.borderNeed[job] = bord;
+        a.serialNeed[job] = g.cls == CLS_SERIAL ? area : 0;
     }
 }
 
@@ -360,8 +409,16 @@ __global__ void __launch_bounds__(64) dp_fill_kernel(DevIndex ix, DpArgs a) {
                         if (G.mode == 1) m = max(m, 0);
                         const int32_t nI = max(addn(uM, G.o), addn(uI, -G.ge));
                         const int32_t nD = max(addn(cM, G.o), addn(cD, -G.ge));
+                        /* the backtrace's decisions at (i, j), from the values the recurrences
+                         * read (src/Alignment.cpp:188-205, src/Alignment.h:577-597) */
+                        const uint32_t dec = (uint32_t)matchMax(dM, dI, dD) |
+                                             ((subn(subn(uM, G.go), G.ge) > subn(uI, G.ge)) ? 0u : 4u) |
+                                             ((subn(subn(cM, G.go), G.ge) > subn(cD, G.ge)) ? 0u : 8u) |
+                                             (m > 0 ? 16u : 0u);
                         cM = m; cI = nI; cD = nD;
-                        G.tile[R.off + (int64_t)jj * Rn + (i - R.r0)] = make_int4(m, nI, nD, 0);
+                        G.bits[R.off + (int64_t)jj * Rn + (i - R.r0)] = (uint8_t)dec;
+                        if (i == R.r1) G.border[R.boff + jj] = make_int4(m, nI, nD, 0);
+                        if (jj == C - 1) G.border[R.boff + C + (i - R.r0)] = make_int4(m, nI, nD, 0);
                         if (more && lane == rows - 1) { sTM[jj + 1] = m; sTI[jj + 1] = nI; sTD[jj + 1] = nD; }
                         if (G.mode == 1) {
                             if (swBetter(m, i, j, bv, bi, bj)) { bv = m; bi = i; bj = j; }
@@ -527,13 +584,20 @@ __global__ void __launch_bounds__(256) dp_trace_kernel(DevIndex ix, DpArgs a) {
                 const int4 sd = G.sg[k];
                 const int32_t t = G.match > 0 ? sd.z - 1 : 0;   /* the diagonal is monotone in its index */
                 visit(sd.w + t * G.match, sd.x + t, sd.y + t);
-                if (sd.x + sd.z <= L && sd.y + sd.z <= g.tLen) visit(vM(G, sd.x + sd.z, sd.y + sd.z, T), sd.x + sd.z, sd.y + sd.z);
+                /* the step-c cell; inside a rectangle it is already in the rectangle summary */
+                Rc Rs;
+                if (sd.x + sd.z <= L && sd.y + sd.z <= g.tLen && !cellRect(G, sd.x + sd.z, sd.y + sd.z, Rs))
+                    visit(vM(G, sd.x + sd.z, sd.y + sd.z, T), sd.x + sd.z, sd.y + sd.z);
             }
             alnTo = bi; alnEnd = bj;
         }
         i = alnTo; j = alnEnd;
         while (true) {
-            const bool go_on = G.mode == 0 ? i > 0 : (s != OP_M || vM(G, i, j, T) > 0);
+            /* inside a rectangle of a wavefront-filled job, decisions come from its bytes */
+            Rc Rb;
+            const bool inR = !G.serial && cellRect(G, i, j, Rb);
+            const uint32_t dec = inR ? G.bits[Rb.off + (int64_t)(j - Rb.c0) * (Rb.r1 - Rb.r0 + 1) + (i - Rb.r0)] : 0u;
+            const bool go_on = G.mode == 0 ? i > 0 : (s != OP_M || (inR ? (dec & 16u) != 0 : vM(G, i, j, T) > 0));
             if (!go_on || np >= pathCap) break;
             if (s == OP_M && G.match > 0 && i >= 0) {
                 /* inside a seed diagonal the step is M with -inf I/D neighbours: cross to its second cell */
@@ -551,7 +615,11 @@ __global__ void __launch_bounds__(256) dp_trace_kernel(DevIndex ix, DpArgs a) {
             }
             push((uint32_t)s, 1u);
             np++;
-            if (s == OP_M) { s = matchMax(vM(G, i - 1, j - 1, T), vI(G, i - 1, j - 1, T), vD(G, i - 1, j - 1, T)); i--; j--; }
+            if (inR) {
+                if (s == OP_M) { s = (int32_t)(dec & 3u); i--; j--; }
+                else if (s == OP_I) { s = (dec & 4u) ? OP_I : OP_M; i--; }
+                else { s = (dec & 8u) ? OP_D : OP_M; j--; }
+            } else if (s == OP_M) { s = matchMax(vM(G, i - 1, j - 1, T), vI(G, i - 1, j - 1, T), vD(G, i - 1, j - 1, T)); i--; j--; }
             else if (s == OP_I) { s = subn(subn(vM(G, i - 1, j, T), go), ge) > subn(vI(G, i - 1, j, T), ge) ? OP_M : OP_I; i--; }
             else { s = subn(subn(vM(G, i, j - 1, T), go), ge) > subn(vD(G, i, j - 1, T), ge) ? OP_M : OP_D; j--; }
             if (G.mode == 0) { if (G.hazard && j < -1) break; }
@@ -562,7 +630,7 @@ __global__ void __launch_bounds__(256) dp_trace_kernel(DevIndex ix, DpArgs a) {
         const int64_t alnFrom = i, alnStart = g.tStart + j;
         AlnRec rec;
         rec.rid = g.rid;
-        rec.tid = g.tid; rec.strand = g.strand; rec.hazard = G.hazard;
+        rec.tid = g.tid; rec.strand = g.strand; rec.hazard = G.hazard | (G.interior ? 2 : 0);
         rec.tStart = g.tStart; rec.tEnd = g.tEnd;
         rec.alnFrom = alnFrom; rec.alnTo = alnTo; rec.alnStart = alnStart; rec.alnEnd = alnEnd + g.tStart;
         rec.score = best;

@@ -119,7 +119,7 @@ enum : int32_t { CLS_NONE = 0, CLS_WAVE = 1, CLS_SERIAL = 2 };
  * target base tStart + j - 1); R5 = rows 1..r5r1 x cols 1..r5c1, R3 = rows r3r0..L x
  * cols r3c0..tLen (empty when a side is empty) */
 struct JobGeo {
-    int64_t tStart, tEnd, r3off;
+    int64_t tStart, tEnd, r3off, r3boff;   /* R3's cell and border offsets (job-relative) */
     int32_t L, tLen, ns, cs;        /* cs: first chain-seed slot of the job */
     int32_t tid, strand, rid, cls;  /* cls: 0 no rectangle, 1 wavefront fill, 2 serial fill */
     int32_t r5r1, r5c1, r3r0, r3c0;
@@ -144,14 +144,21 @@ struct DpArgs {
     const int32_t* jobChainLen;
     const int32_t* chainSeeds;
     int4* auxSG;               /* per chain seed: bi, bj, len, d0 (first diagonal value) */
-    int64_t* auxGap;           /* per chain seed: tile offset of the following gap rect (job-relative) */
+    int64_t* auxGap;           /* per chain seed: cell offset of the following gap rect (job-relative) */
+    int64_t* auxGapB;          /* per chain seed: border offset of the following gap rect (job-relative) */
     JobGeo* geo;
     JobSum* sum;
     int32_t* jobCls;
     const int64_t* waveJobs;   /* jobs filled by the wavefront kernel, largest first */
     int64_t nWaveJobs;
-    const int64_t* tileOff;    /* per job offset into the tile pool (cells) */
-    int4* tile;                /* M, I, D per rectangle cell */
+    const int64_t* tileOff;    /* per job offset into the decision pool (one byte per rectangle cell) */
+    uint8_t* bits;             /* per rectangle cell of a wavefront-filled job, the backtrace's next
+                                * state from M (bits 0-1), I (bit 2: 1 = stay), D (bit 3: 1 = stay)
+                                * and M > 0 (bit 4) */
+    const int64_t* borderOff;  /* per job offset into the border pool */
+    int4* border;              /* M, I, D of each rectangle's last row, then its last column */
+    const int64_t* serialOff;  /* per job offset into the tile pool (serially filled jobs only) */
+    int4* tile;                /* M, I, D per rectangle cell of serially filled jobs */
     const int64_t* pathOff;    /* per job: scan of path bounds (L + tLen + 3); the CIGAR slot is at
                                 * pathOff + 2*job, the MD slot at 3*pathOff + 8*job */
     uint32_t* cigar;
@@ -164,8 +171,10 @@ struct DpArgs {
     const double* qMis;        /* q / PHRED_SCALE by q */
     const double* qGapOpen;    /* q / PHRED_SCALE - gapO by q */
     double gapE_d;
-    int64_t* tileNeed;
+    int64_t* tileNeed;         /* rectangle cells */
     int64_t* pathNeed;
+    int64_t* borderNeed;       /* border cells */
+    int64_t* serialNeed;       /* rectangle cells of serially filled jobs */
 };
 /* seeds, rectangles and sizes of each job */
 __global__ void dp_setup_kernel(DevIndex ix, DpArgs a);

@@ -271,6 +271,7 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     a.chainSeeds = ch.dChainSeeds;
     a.auxSG = c->x().bufs.get<int4>("dp.sg", ch.nChainSeeds + 1, e);
     a.auxGap = c->x().bufs.get<int64_t>("dp.gap", ch.nChainSeeds + 1, e);
+    a.auxGapB = c->x().bufs.get<int64_t>("dp.gapb", ch.nChainSeeds + 1, e);
     a.geo = c->x().bufs.get<JobGeo>("dp.geo", nJobs + 1, e);
     a.sum = c->x().bufs.get<JobSum>("dp.sum", nJobs + 1, e);
     a.jobCls = c->x().bufs.get<int32_t>("dp.cls", nJobs + 1, e);
@@ -288,6 +289,8 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     double* dq = c->x().bufs.get<double>("dp.qtab", 3 * 256, e);
     a.tileNeed = c->x().bufs.get<int64_t>("dp.tneed", nJobs + 1, e);
     a.pathNeed = c->x().bufs.get<int64_t>("dp.pneed", nJobs + 1, e);
+    a.borderNeed = c->x().bufs.get<int64_t>("dp.bneed", nJobs + 1, e);
+    a.serialNeed = c->x().bufs.get<int64_t>("dp.sneed", nJobs + 1, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "dp scratch");
     CK(hipMemcpyAsync(dq, hq, sizeof(hq), hipMemcpyHostToDevice, st), "qtab");
     a.qMatch = dq; a.qMis = dq + 256; a.qGapOpen = dq + 512;
@@ -302,6 +305,8 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     void* scanTmp = c->x().bufs.get<uint8_t>("dp.scantmp", scanBytes + 1, e);
     int64_t* dTileOff = c->x().bufs.get<int64_t>("dp.toff", nJobs + 1, e);
     int64_t* dPathOff = c->x().bufs.get<int64_t>("dp.poff", nJobs + 1, e);
+    int64_t* dBorderOff = c->x().bufs.get<int64_t>("dp.boff", nJobs + 1, e);
+    int64_t* dSerialOff = c->x().bufs.get<int64_t>("dp.soff", nJobs + 1, e);
     int64_t* dWave = c->x().bufs.get<int64_t>("dp.wave", nJobs + 1, e);
     unsigned long long* dHist = c->x().bufs.get<unsigned long long>("dp.whist", 128, e);
     int64_t* dTot = c->x().bufs.get<int64_t>("dp.tot", 8, e);
@@ -309,17 +314,24 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     CK(hipMemsetAsync(dHist, 0, 128 * 8, st), "hist");
     CK(hipcub::DeviceScan::ExclusiveSum(scanTmp, scanBytes, a.tileNeed, dTileOff, (int)(nJobs + 1), st), "tile scan");
     CK(hipcub::DeviceScan::ExclusiveSum(scanTmp, scanBytes, a.pathNeed, dPathOff, (int)(nJobs + 1), st), "path scan");
+    CK(hipcub::DeviceScan::ExclusiveSum(scanTmp, scanBytes, a.borderNeed, dBorderOff, (int)(nJobs + 1), st), "border scan");
+    CK(hipcub::DeviceScan::ExclusiveSum(scanTmp, scanBytes, a.serialNeed, dSerialOff, (int)(nJobs + 1), st), "serial scan");
     k_wave_hist<<<nb, 256, 0, st>>>(a.jobCls, a.tileNeed, nJobs, dHist);
     k_wave_base<<<1, 64, 0, st>>>(dHist, dHist + 64, dTot + 2);
     k_wave_scatter<<<nb, 256, 0, st>>>(a.jobCls, a.tileNeed, nJobs, dHist + 64, dWave);
     k_copy_i64<<<1, 64, 0, st>>>(dTileOff + nJobs, dTot + 0);
     k_copy_i64<<<1, 64, 0, st>>>(dPathOff + nJobs, dTot + 1);
+    k_copy_i64<<<1, 64, 0, st>>>(dBorderOff + nJobs, dTot + 3);
+    k_copy_i64<<<1, 64, 0, st>>>(dSerialOff + nJobs, dTot + 4);
     out.tot.resize(8);
-    CK(hipMemcpyAsync(out.tot.data(), dTot, 3 * 8, hipMemcpyDeviceToHost, st), "dp totals");
+    CK(hipMemcpyAsync(out.tot.data(), dTot, 5 * 8, hipMemcpyDeviceToHost, st), "dp totals");
     CK(hipStreamSynchronize(st), "dp setup");
-    const int64_t tiles = out.tot[0], paths = out.tot[1], nWave = out.tot[2];
+    const int64_t tiles = out.tot[0], paths = out.tot[1], nWave = out.tot[2], borders = out.tot[3],
+                  serialCells = out.tot[4];
     if (phase) { phase[0] = (double)nJobs; phase[1] = (double)nWave; phase[2] = since(); }
-    a.tile = c->x().bufs.get<int4>("dp.tile", tiles + 1, e);
+    a.bits = c->x().bufs.get<uint8_t>("dp.bits", tiles + 1, e);
+    a.border = c->x().bufs.get<int4>("dp.border", borders + 1, e);
+    a.tile = c->x().bufs.get<int4>("dp.tile", serialCells + 1, e);
     a.cigar = c->x().bufs.get<uint32_t>("dp.cig", paths + 2 * nJobs + 1, e);
     a.md = c->x().bufs.get<char>("dp.md", 3 * paths + 8 * nJobs + 1, e);
     a.out = c->x().bufs.get<AlnRec>("dp.out", nJobs + 1, e);
@@ -330,7 +342,7 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     uint32_t* dC = c->x().bufs.get<uint32_t>("dp.cc", paths + 2 * nJobs + 1, e);
     char* dM = c->x().bufs.get<char>("dp.mc", 3 * paths + 8 * nJobs + 1, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "dp pools");
-    a.tileOff = dTileOff; a.pathOff = dPathOff;
+    a.tileOff = dTileOff; a.pathOff = dPathOff; a.borderOff = dBorderOff; a.serialOff = dSerialOff;
     a.waveJobs = dWave; a.nWaveJobs = nWave;
     CK(hipEventRecord(c->x().ev2, st), "event");
     if (nWave) {
@@ -1143,28 +1155,34 @@ extern "C" int msgs_tune(msgs_ctx* c, int32_t slot, const msgs_params* p, const 
     Slot& S = c->slots[slot];
     for (int32_t i = 0; i < nCand; ++i)
         if (cand[i] < 64 || cand[i] > 4096 || cand[i] % 64) return fail(c, MSGS_E_ARG, "lanes per CU");
-    const int keep = c->lanesPerCU, keepG = c->groupLanes;
-    static const int kGroups[] = {0, 8, 16};
-    const int nG = c->form == 2 ? 3 : 1;
+    const int keep = c->lanesPerCU, keepG = c->groupLanes, keepT = c->textPath, keepF = c->readFilter;
+    /* form 2 search front ends and kernels: (lanes per read, text path, read filter) */
+    static const int kKernels[][3] = {{8, 1, 0}, {16, 1, 0}, {0, 1, 0}, {0, 0, 0}, {8, 1, 1}, {0, 0, 1}};
+    const int nK = c->form == 2 ? 6 : 1;
     int32_t bestL = keep;
-    int bestG = keepG;
+    int bestG = keepG, bestT = keepT, bestF = keepF;
     double bestUs = 1e300;
-    for (int gi = 0; gi < nG; ++gi) {
-        if (c->form == 2) c->groupLanes = kGroups[gi];
+    auto restore = [&]() { c->lanesPerCU = keep; c->groupLanes = keepG; c->textPath = keepT; c->readFilter = keepF; };
+    for (int ki = 0; ki < nK; ++ki) {
+        if (c->form == 2) { c->groupLanes = kKernels[ki][0]; c->textPath = kKernels[ki][1]; c->readFilter = kKernels[ki][2]; }
         for (int32_t i = 0; i < nCand; ++i) {
             c->lanesPerCU = cand[i];
             double t = 1e300;
             for (int rep = 0; rep < 2; ++rep) {
                 SmemStage sm;
                 rc = runSmem(c, S.dev, p, sm, nullptr);
-                if (rc) { c->lanesPerCU = keep; c->groupLanes = keepG; return rc; }
+                if (rc) { restore(); return rc; }
                 t = std::min(t, c->x().lastSmemMicros + c->x().lastFilterMicros);
             }
-            if (t < bestUs) { bestUs = t; bestL = cand[i]; bestG = c->groupLanes; }
+            if (t < bestUs) {
+                bestUs = t; bestL = cand[i]; bestG = c->groupLanes; bestT = c->textPath; bestF = c->readFilter;
+            }
         }
     }
     c->lanesPerCU = bestL;
     c->groupLanes = bestG;
+    c->textPath = bestT;
+    c->readFilter = bestF;
     if (best) *best = bestL;
     if (bestMicros) *bestMicros = bestUs;
     return MSGS_OK;

@@ -279,6 +279,17 @@ __device__ __forceinline__ void startIteration(Lane& l) {
     l.nValid = 0;
 }
 
+/* SMEM::fwdExt / SMEM::backExt on a bi-interval: forward steps are backward steps of the
+ * reverse complement with the roles of p and q swapped (src/FMDIndex.h:226-228).  One
+ * inlined extension serves both directions, so a wavefront whose lanes go different
+ * ways runs the rank code once. */
+__device__ __forceinline__ void extendOne(const DevIndex& ix, bool fwd, int64_t& P, int64_t& Q, int64_t& S, uint8_t b) {
+    int64_t x = fwd ? Q : P, y = fwd ? P : Q;
+    dBackExt(ix, x, y, S, fwd ? comp16(b) : b);
+    P = fwd ? y : x;
+    Q = fwd ? x : y;
+}
+
 /* fetch reads until one has a running search; false when the queue is empty */
 __device__ bool nextRead(const Ctx& c, Lane& l) {
     while (true) {
@@ -372,8 +383,7 @@ __global__ void __launch_bounds__(256, 4) smem_kernel_t(DevIndex ix, SmemArgs a)
             else S = 0;
         } else {
             l.nExt += amb(b) ? 0 : 1;
-            if (fwd) dBackExt(ix, Q, P, S, comp16(b));
-            else dBackExt(ix, P, Q, S, b);
+            extendOne(ix, fwd, P, Q, S, b);
             X = -1;
         }
         if (fwd) { if (mTo < l.L) LL += llOf(c, b, llA, llC, llG, llT); mTo++; }
@@ -621,8 +631,7 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
                 else S = 0;
             } else {
                 if (gl == 0) l.nExt += amb(b) ? 0 : 1;
-                if (fwd) dBackExt(ix, Q, P, S, comp16(b));
-                else dBackExt(ix, P, Q, S, b);
+                extendOne(ix, fwd, P, Q, S, b);
                 X = -1;
             }
             if (fwd) { if (mTo < l.L) LL += llOf(c, b, llA, llC, llG, llT); mTo++; }
@@ -710,8 +719,7 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
                     else S = 0;
                 } else {
                     l.nExt += amb(b) ? 0 : 1;
-                    if (fwd) dBackExt(ix, Q, P, S, comp16(b));
-                    else dBackExt(ix, P, Q, S, b);
+                    extendOne(ix, fwd, P, Q, S, b);
                     X = -1;
                 }
                 if (fwd) { if (mTo < l.L) LL += llOf(c, b, llA, llC, llG, llT); }

@@ -94,13 +94,13 @@ def _reads(chroms):
     return seqs
 
 
-@pytest.mark.parametrize("group,read_filter", [(8, 1), (0, 1), (16, 1), (8, 0)])
-def test_exact_read_path_parity(env, group, read_filter):
-    """every search kernel behind the exact-read path (one lane per read, lane groups of
-    8 and 16), with and without the read filter"""
+@pytest.mark.parametrize("group,read_filter,text_path", [(8, 1, 1), (0, 1, 1), (0, 1, 0), (16, 1, 1), (8, 0, 1)])
+def test_exact_read_path_parity(env, group, read_filter, text_path):
+    """every search kernel behind the exact-read path (one lane per read with and without
+    the text path, lane groups of 8 and 16), with and without the read filter"""
     chroms, names, odb, m = env
     seqs = _reads(chroms)
-    m.set("group_lanes", group).set("read_filter", read_filter)
+    m.set("group_lanes", group).set("read_filter", read_filter).set("text_path", text_path)
     try:
         for minLen, maxE in ((17, 0.01), (16, 1.0), (20, float("inf")), (40, 1e-30), (151, 0.01)):
             p = M.default_params(minLen=minLen, maxEvalue=maxE)
@@ -112,7 +112,7 @@ def test_exact_read_path_parity(env, group, read_filter):
             assert np.array_equal(r1, r2), (minLen, maxE)
             assert np.array_equal(l1.view(np.int64), l2.view(np.int64)), (minLen, maxE)
     finally:
-        m.set("form", 1).set("group_lanes", 8).set("read_filter", 1)
+        m.set("form", 1).set("group_lanes", 8).set("read_filter", 1).set("text_path", 1)
 
 
 def test_exact_read_path_sam_and_count(env):
