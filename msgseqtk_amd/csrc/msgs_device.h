@@ -171,6 +171,71 @@ __device__ __forceinline__ int64_t dBackExt(const DevIndex& ix, int64_t& p, int6
     return s;
 }
 
+/* 32-bit forms of the ranks and of backExt for the search kernels (indexes hold fewer
+ * than 2^31 rows, so every rank, row and size fits; same values as the 64-bit forms) */
+__device__ __forceinline__ void occFromBlock32(const DevIndex& ix, const uint4& h, const uint4& sp, const uint4& c0,
+                                               const uint4& c1, int32_t i, int32_t* r) {
+    const uint32_t j = (uint32_t)i & 127u;
+    const uint32_t wj = j >> 4, bj = j & 15;
+    const uint32_t part = (2u << (2 * bj + 1)) - 1u;
+    const uint32_t w[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+    uint32_t nT = 0, nLo = 0, nHi = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) {
+        uint32_t mk = k < wj ? 0xffffffffu : (k == wj ? part : 0u);
+        uint32_t x = w[k] & mk;
+        uint32_t lo = x & 0x55555555u, hi = (x >> 1) & 0x55555555u;
+        nT += __popc(lo & hi);
+        nLo += __popc(lo);
+        nHi += __popc(hi);
+    }
+    const uint32_t sw = j >> 5, sb = j & 31;
+    const uint32_t smask = (2u << sb) - 1u;
+    const uint32_t s[4] = {sp.x, sp.y, sp.z, sp.w};
+    uint32_t nSp = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        uint32_t mk = k < sw ? 0xffffffffu : (k == sw ? smask : 0u);
+        nSp += __popc(s[k] & mk);
+    }
+    const uint32_t nC = nLo - nT, nG = nHi - nT;
+    const uint32_t nA = (j + 1) - nLo - nHi + nT;
+    r[1] = (int32_t)(h.x + nA - nSp);
+    r[2] = (int32_t)(h.y + nC);
+    r[3] = (int32_t)(h.z + nG);
+    r[4] = (int32_t)(h.w + nT);
+    const int32_t special = (int32_t)(((uint32_t)i & ~127u) - (h.x + h.y + h.z + h.w) + nSp);
+    r[0] = ix.nN ? special - (int32_t)countN(ix, i) : special;
+}
+
+__device__ __forceinline__ int32_t dBackExt32(const DevIndex& ix, int32_t& p, int32_t& q, int32_t& s, uint8_t b) {
+    if (amb(b)) return (s = 0);
+    const int c = toInt(b);
+    const int32_t a = p - 1, e = p + s - 1;
+    const bool va = a >= 0 && (int64_t)a < ix.n, ve = e >= 0 && (int64_t)e < ix.n;
+    const int32_t ba = va ? (a >> 7) : 0, be = ve ? (e >> 7) : 0;
+    const uint4* pa = reinterpret_cast<const uint4*>(ix.occ + ba);
+    const uint4* pe = reinterpret_cast<const uint4*>(ix.occ + be);
+    const uint4 h0 = pa[0], s0 = pa[1], c00 = pa[2], c01 = pa[3];
+    uint4 h1, s1, c10, c11;
+    if (be != ba) { h1 = pe[0]; s1 = pe[1]; c10 = pe[2]; c11 = pe[3]; }
+    else { h1 = h0; s1 = s0; c10 = c00; c11 = c01; }
+    int32_t lo[5], hi[5];
+    if (va) occFromBlock32(ix, h0, s0, c00, c01, a, lo); else { lo[0] = lo[1] = lo[2] = lo[3] = lo[4] = -1; }
+    if (ve) occFromBlock32(ix, h1, s1, c10, c11, e, hi); else { hi[0] = hi[1] = hi[2] = hi[3] = hi[4] = -1; }
+    const int32_t O = lo[c];
+    const int32_t sc = hi[c] - O;
+    if (sc <= 0) return (s = sc);
+    if (sc != s) {
+        int32_t add = hi[0] - lo[0];
+        for (int j = c + 1; j <= 4; ++j) add += hi[j] - lo[j];
+        q = q + add;
+    }
+    p = (int32_t)ix.C[b] + O;
+    s = sc;
+    return s;
+}
+
 /* FMDIndex::fwdExt (src/FMDIndex.h:226-228) */
 __device__ __forceinline__ int64_t dFwdExt(const DevIndex& ix, int64_t& p, int64_t& q, int64_t& s, uint8_t b) {
     return dBackExt(ix, q, p, s, comp16(b));

@@ -38,11 +38,11 @@ struct Lane {
     int state;
     /* phase-1 SMEM */
     int cFrom, cTo;
-    int64_t cF, cR, cS, cX;
+    int32_t cF, cR, cS, cX;   /* rows and sizes (indexes hold fewer than 2^31 rows) */
     double cLL;
     /* candidate list */
     int nCand, lo, k, w, nValid;
-    int64_t s0;
+    int32_t s0;
     /* output */
     int nOut;
     /* counters */
@@ -57,7 +57,7 @@ struct Ctx {
     int64_t lane;
 };
 
-__device__ __forceinline__ void emit(const Ctx& c, Lane& l, int from, int to, int64_t F, int64_t R, int64_t S, double ll) {
+__device__ __forceinline__ void emit(const Ctx& c, Lane& l, int from, int to, int32_t F, int32_t R, int32_t S, double ll) {
     if (l.nOut < c.a.cap) {
         SmemRec* o = c.a.out + (size_t)l.rid * c.a.cap + l.nOut;
         o->from = from; o->to = to; o->fwd = F; o->rev = R; o->size = S; o->logP = ll;
@@ -65,12 +65,14 @@ __device__ __forceinline__ void emit(const Ctx& c, Lane& l, int from, int to, in
     l.nOut++;
 }
 
-__device__ __forceinline__ size_t cidx(const Ctx& c, int k) { return (size_t)c.lane * c.a.candCap + k; }
+/* candidate k of a lane (or group): interleaved over lanes, so that lanes at the same
+ * list position touch neighbouring entries */
+__device__ __forceinline__ size_t cidx(const Ctx& c, int k) { return (size_t)k * c.a.nLanes + c.lane; }
 
-__device__ __forceinline__ void candStore(const Ctx& c, int k, int pos, int64_t F, int64_t R, int64_t S, double ll,
-                                          int64_t X) {
+__device__ __forceinline__ void candStore(const Ctx& c, int k, int pos, int32_t F, int32_t R, int32_t S, double ll,
+                                          int32_t X) {
     SmemCand v;
-    v.F = (int32_t)F; v.R = (int32_t)R; v.S = (int32_t)S; v.X = (int32_t)X; v.LL = ll; v.pos = pos; v.pad = 0;
+    v.F = F; v.R = R; v.S = S; v.X = X; v.LL = ll; v.pos = pos; v.pad = 0;
     c.a.cand[cidx(c, k)] = v;
 }
 
@@ -151,31 +153,28 @@ __device__ __forceinline__ double gateSlack(const Lane& l, double ll0) {
     return 1e-15 * (double)(l.L + 2) * (1.0 + fabs(ll0) + fabs(l.P[l.L]));
 }
 
-__device__ __forceinline__ bool gateFwd(const Ctx& c, const Lane& l, double ll0, int from, double lA, double lC,
-                                        double lG, double lT) {
-    /* evalue(0, to) <= maxE of the phase-1 candidate (src/SMEM.cpp:26-40,81-82) */
+/* the phase-1 push gate: logP of the candidate plus the logliks of read[i0, i1) <= T1,
+ * i.e. evalue(0, to) for a forward search ([0, from)) and evalue(from, L) for a backward
+ * one ([to, L)) (src/SMEM.cpp:26-40, :81-82, :136-137) */
+__device__ __forceinline__ bool gatePush(const Ctx& c, const Lane& l, double ll0, int i0, int i1, double lA,
+                                         double lC, double lG, double lT) {
     if (!c.a.gate1) return true;
     if (l.P) {
-        const double v = ll0 + l.P[from], sl = gateSlack(l, ll0);
+        const double v = ll0 + (l.P[i1] - l.P[i0]), sl = gateSlack(l, ll0);
         if (v < c.a.T1 - sl) return true;
         if (v > c.a.T1 + sl) return false;
     }
     double x = 0;
     x += ll0;
-    return gateSum(c, l, x, 0, from, c.a.T1, lA, lC, lG, lT);
+    return gateSum(c, l, x, i0, i1, c.a.T1, lA, lC, lG, lT);
+}
+__device__ __forceinline__ bool gateFwd(const Ctx& c, const Lane& l, double ll0, int from, double lA, double lC,
+                                        double lG, double lT) {
+    return gatePush(c, l, ll0, 0, from, lA, lC, lG, lT);
 }
 __device__ __forceinline__ bool gateBack(const Ctx& c, const Lane& l, double ll0, int to, double lA, double lC,
                                          double lG, double lT) {
-    /* evalue(from, L) <= maxE (src/SMEM.cpp:136-137) */
-    if (!c.a.gate1) return true;
-    if (l.P) {
-        const double v = ll0 + (l.P[l.L] - l.P[to]), sl = gateSlack(l, ll0);
-        if (v < c.a.T1 - sl) return true;
-        if (v > c.a.T1 + sl) return false;
-    }
-    double x = 0;
-    x += ll0;
-    return gateSum(c, l, x, to, l.L, c.a.T1, lA, lC, lG, lT);
+    return gatePush(c, l, ll0, to, l.L, lA, lC, lG, lT);
 }
 
 __device__ __forceinline__ uint8_t byteAt(uint64_t w0, uint64_t w1, uint64_t w2, int64_t o) {
@@ -283,9 +282,9 @@ __device__ __forceinline__ void startIteration(Lane& l) {
  * reverse complement with the roles of p and q swapped (src/FMDIndex.h:226-228).  One
  * inlined extension serves both directions, so a wavefront whose lanes go different
  * ways runs the rank code once. */
-__device__ __forceinline__ void extendOne(const DevIndex& ix, bool fwd, int64_t& P, int64_t& Q, int64_t& S, uint8_t b) {
-    int64_t x = fwd ? Q : P, y = fwd ? P : Q;
-    dBackExt(ix, x, y, S, fwd ? comp16(b) : b);
+__device__ __forceinline__ void extendOne(const DevIndex& ix, bool fwd, int32_t& P, int32_t& Q, int32_t& S, uint8_t b) {
+    int32_t x = fwd ? Q : P, y = fwd ? P : Q;
+    dBackExt32(ix, x, y, S, fwd ? comp16(b) : b);
     P = fwd ? y : x;
     Q = fwd ? x : y;
 }
@@ -344,7 +343,7 @@ __global__ void __launch_bounds__(256, 4) smem_kernel_t(DevIndex ix, SmemArgs a)
         const int st = l.state;
         const bool phase1 = (st == ST_FB1 || st == ST_BF1);
         int pos = 0;
-        int64_t P, Q, S, X;
+        int32_t P, Q, S, X;
         double LL;
         int mFrom, mTo;
         if (phase1) {
@@ -372,8 +371,8 @@ __global__ void __launch_bounds__(256, 4) smem_kernel_t(DevIndex ix, SmemArgs a)
             }
         }
         const uint8_t b = fwd ? base(l, mTo) : (mFrom > 0 ? base(l, mFrom - 1) : (uint8_t)0);
-        int64_t F0 = P, R0 = Q;
-        const int64_t S0 = S, X0 = X;
+        int32_t F0 = P, R0 = Q;
+        const int32_t S0 = S, X0 = X;
         const double LL0 = LL;
         /* ---- the extension (SMEM::fwdExt / SMEM::backExt, src/SMEM.h:140-177) ---- */
         if (FAST && S == 1) {
@@ -391,42 +390,43 @@ __global__ void __launch_bounds__(256, 4) smem_kernel_t(DevIndex ix, SmemArgs a)
         /* the pre-step SMEM with its moving side turned back into a row */
         auto exact0 = [&]() {
             if (FAST && X0 >= 0) {
-                const int64_t v = ix.isa[X0];
+                const int32_t v = (int32_t)ix.isa[X0];
                 l.nLook++;
                 if (fwd) R0 = v; else F0 = v;
             }
         };
 
-        /* ---- control ---- */
-        if (st == ST_FB1) {
-            if (S != S0 && l.cTo >= minLen && gateFwd(c, l, LL0, l.cFrom, llA, llC, llG, llT)) {
+        /* ---- control ----
+         * (one copy of each step for both directions: lanes of a wavefront in different
+         * states then share the code that runs) */
+        bool after = false;   /* the search returned: findAllSMEMS goes on */
+        if (phase1) {
+            /* push the pre-step SMEM when its size changes (src/SMEM.cpp:77-83, :130-138) */
+            const int len = fwd ? l.cTo : (l.L - l.cFrom);
+            if (S != S0 && len >= minLen &&
+                gatePush(c, l, LL0, fwd ? 0 : l.cTo, fwd ? l.cFrom : l.L, llA, llC, llG, llT)) {
                 exact0();
-                candStore(c, l.nCand++, l.cTo, F0, R0, S0, LL0, -1);
+                candStore(c, l.nCand++, fwd ? l.cTo : l.cFrom, F0, R0, S0, LL0, -1);
             }
-            l.cTo = mTo; l.cF = P; l.cR = Q; l.cS = S; l.cLL = LL; l.cX = X;
+            if (fwd) l.cTo = mTo; else l.cFrom = mFrom;
+            l.cF = P; l.cR = Q; l.cS = S; l.cLL = LL; l.cX = X;
             if (!(S >= 1)) {
-                l.sTo = mTo;
-                if (l.nCand == 0) { l.sFrom--; driverAfter(c, l); }
-                else { l.state = ST_FB2; l.lo = 0; startIteration(l); }
-            }
-        } else if (st == ST_BF1) {
-            if (S != S0 && (int64_t)(l.L - l.cFrom) >= minLen && gateBack(c, l, LL0, l.cTo, llA, llC, llG, llT)) {
-                exact0();
-                candStore(c, l.nCand++, l.cFrom, F0, R0, S0, LL0, -1);
-            }
-            l.cFrom = mFrom; l.cF = P; l.cR = Q; l.cS = S; l.cLL = LL; l.cX = X;
-            if (!(S >= 1)) {
-                l.sFrom = mFrom;
-                if (l.nCand == 0) { l.sTo++; driverAfter(c, l); }
-                else { l.state = ST_BF2; l.lo = 0; startIteration(l); }
+                if (fwd) l.sTo = mTo; else l.sFrom = mFrom;
+                if (l.nCand == 0) {
+                    if (fwd) l.sFrom--; else l.sTo++;
+                    after = true;
+                } else {
+                    l.state = fwd ? ST_FB2 : ST_BF2;
+                    l.lo = 0;
+                    startIteration(l);
+                }
             }
         } else {
             const bool fb = (st == ST_FB2);
             const int len0 = fb ? (pos - l.sFrom) : (l.sTo - pos);
             if (S != S0 && S0 >= 1 && l.nValid == 0 && len0 >= minLen && LL0 <= a.T2) {
                 exact0();
-                if (fb) emit(c, l, l.sFrom, pos, F0, R0, S0, LL0);
-                else emit(c, l, pos, l.sTo, F0, R0, S0, LL0);
+                emit(c, l, fb ? l.sFrom : pos, fb ? pos : l.sTo, F0, R0, S0, LL0);
             }
             if (S >= 1 && S != l.s0) {
                 l.s0 = S;
@@ -441,9 +441,10 @@ __global__ void __launch_bounds__(256, 4) smem_kernel_t(DevIndex ix, SmemArgs a)
                 else { l.sTo++; more = l.sTo <= l.L; }
                 more = more && (l.lo <= l.nCand - 1);
                 if (more) startIteration(l);
-                else driverAfter(c, l);
+                else after = true;
             }
         }
+        if (after) driverAfter(c, l);
         if (l.state == ST_IDLE) a.cnt[l.rid] = l.nOut;
     }
     if (a.stats) {
@@ -591,7 +592,7 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
     l.nCand = 0; l.lo = 0; l.k = 0; l.w = 0; l.s0 = -1; l.nValid = 0;
     /* this lane's list entry in register mode (l.nValid == 1) or its phase-1 push */
     int rPos = 0;
-    int64_t rP = 0, rQ = 0, rS = 0, rX = -1;
+    int32_t rP = 0, rQ = 0, rS = 0, rX = -1;
     double rLL = 0;
     Ctx c{ix, a, ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G};
     const int64_t minLen = a.minLen;
@@ -605,7 +606,7 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
         const int st = l.state;
         if (st == ST_FB1 || st == ST_BF1) {
             /* ---- phase 1: one extension, alike in every lane of the group ---- */
-            int64_t P = l.cF, Q = l.cR, S = l.cS, X = l.cX;
+            int32_t P = l.cF, Q = l.cR, S = l.cS, X = l.cX;
             double LL = l.cLL;
             int mFrom = l.cFrom, mTo = l.cTo;
             const bool fwd = st == ST_FB1;
@@ -621,8 +622,8 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
                 }
             }
             const uint8_t b = fwd ? base(l, mTo) : (mFrom > 0 ? base(l, mFrom - 1) : (uint8_t)0);
-            int64_t F0 = P, R0 = Q;
-            const int64_t S0 = S, X0 = X;
+            int32_t F0 = P, R0 = Q;
+            const int32_t S0 = S, X0 = X;
             const double LL0 = LL;
             if (S == 1) {
                 if (X < 0) { X = ix.sa[fwd ? Q : P]; if (gl == 0) l.nLook++; }
@@ -641,7 +642,7 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
                                      gateBack(c, l, LL0, l.cTo, llA, llC, llG, llT));
             if (push) {
                 if (X0 >= 0) {
-                    const int64_t v = ix.isa[X0];
+                    const int32_t v = (int32_t)ix.isa[X0];
                     if (gl == 0) l.nLook++;
                     if (fwd) R0 = v; else F0 = v;
                 }
@@ -668,7 +669,7 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
                         /* registers: lane i takes push nCand-1-i (list order) */
                         const int src = gbase + max(0, l.nCand - 1 - gl);
                         rPos = __shfl(rPos, src, 64);
-                        rP = shfl64(rP, src); rQ = shfl64(rQ, src); rS = shfl64(rS, src); rX = shfl64(rX, src);
+                        rP = __shfl(rP, src, 64); rQ = __shfl(rQ, src, 64); rS = __shfl(rS, src, 64); rX = __shfl(rX, src, 64);
                         rLL = __longlong_as_double(shfl64(__double_as_longlong(rLL), src));
                         l.nValid = 1;
                     } else {
@@ -686,7 +687,7 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
             const int i = l.k + gl;
             const bool act = i < l.nCand;
             int pos = 0;
-            int64_t P = 0, Q = 0, S = 0, X = -1;
+            int32_t P = 0, Q = 0, S = 0, X = -1;
             double LL = 0;
             if (regs) {
                 if (act) { pos = rPos; P = rP; Q = rQ; S = rS; X = rX; LL = rLL; }
@@ -708,8 +709,8 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
             }
             int mFrom = fb ? l.sFrom : pos, mTo = fb ? pos : l.sTo;
             const uint8_t b = fwd ? base(l, mTo) : (mFrom > 0 ? base(l, mFrom - 1) : (uint8_t)0);
-            int64_t F0 = P, R0 = Q;
-            const int64_t S0 = S, X0 = X;
+            int32_t F0 = P, R0 = Q;
+            const int32_t S0 = S, X0 = X;
             const double LL0 = LL;
             if (act) {
                 if (S == 1) {
@@ -726,7 +727,7 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
                 else { if (mFrom > 0) LL += llOf(c, b, llA, llC, llG, llT); }
             }
             /* S' of the previous list entry */
-            const int64_t up = shfl64(S, gl == 0 ? lane : lane - 1);
+            const int32_t up = __shfl(S, gl == 0 ? lane : lane - 1, 64);
             const int64_t prev = gl == 0 ? (int64_t)l.s0 : up;
             const int len0 = fb ? (pos - l.sFrom) : (l.sTo - pos);
             const bool keep = act && S >= 1 && S != prev;
@@ -734,7 +735,7 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
             const uint64_t kMask = __ballot(keep) & gmask, eMask = __ballot(em) & gmask;
             if (em) {
                 if (X0 >= 0) {
-                    const int64_t v = ix.isa[X0];
+                    const int32_t v = (int32_t)ix.isa[X0];
                     l.nLook++;
                     if (fwd) R0 = v; else F0 = v;
                 }
@@ -751,7 +752,7 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
                 const int rank = keep ? __popcll(kMask & below) : nKeep + __popcll(~kMask & gmask & below);
                 const int dst = gbase + rank;
                 rPos = permute32(dst, pos);
-                rP = permute64(dst, P); rQ = permute64(dst, Q); rS = permute64(dst, S); rX = permute64(dst, X);
+                rP = permute32(dst, P); rQ = permute32(dst, Q); rS = permute32(dst, S); rX = permute32(dst, X);
                 rLL = permuteF64(dst, LL);
             } else if (keep) {
                 candStore(c, l.lo - (l.w + __popcll(kMask & below)), pos, P, Q, S, LL, X);
@@ -759,7 +760,7 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
             l.nOut += __popcll(eMask);
             l.w += nKeep;
             const int last = min(G, l.nCand - l.k) - 1;
-            l.s0 = (int)shfl64(S, gbase + last);
+            l.s0 = __shfl(S, gbase + last, 64);
             l.k += G;
             if (l.k >= l.nCand) {
                 l.nCand = l.w;
