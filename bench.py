@@ -53,6 +53,7 @@ def parse():
     ap.add_argument("--lanes-per-cu", type=int, default=0, help="fix the SMEM kernel's lanes per CU (no autotuning)")
     ap.add_argument("--group-lanes", type=int, default=-1, help="fix the form-2 lanes per read: 0, 8 or 16")
     ap.add_argument("--text-path", type=int, default=-1, help="fix the form-2 one-lane kernel's text path: 0 or 1")
+    ap.add_argument("--read-filter", type=int, default=-1, help="fix the form-2 read filter: 0 or 1")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 1 GiB device copy (counter calibration under rocprofv3 --pmc)")
     return ap.parse_args()
@@ -108,6 +109,8 @@ def main():
         m.set("group_lanes", a.group_lanes)
     if a.text_path >= 0:
         m.set("text_path", a.text_path)
+    if a.read_filter >= 0:
+        m.set("read_filter", a.read_filter)
     if not a.no_tune and not a.lanes_per_cu:
         lanes, tus = m.tune(0, p)
         tuned = {"lanes_per_cu": lanes, "group_lanes": m.get("group_lanes"), "text_path": m.get("text_path"),
@@ -183,6 +186,15 @@ def main():
         v0 = S0[:, 1].sum() / el0
         out["plain_fmd"] = {"value": round(v0, 1), "ms_per_step": round(1000 * el0 / a.steps, 3),
                             "smem_kernel_ms_per_step": S0[:, 5].sum() / a.steps / 1000,
+                            "stage_ms_per_step": {"seed_stage": S0[:, 8].sum() / a.steps / 1000,
+                                                  "dp_kernel": S0[:, 6].sum() / a.steps / 1000,
+                                                  "host_finish": S0[:, 9].sum() / a.steps / 1000},
+                            "host_side_ms_per_step": {
+                                "gpu_stages_wall": S0[:, 16].sum() / a.steps / 1000,
+                                "smem_wall": S0[:, 17].sum() / a.steps / 1000,
+                                "seeds_wall": S0[:, 18].sum() / a.steps / 1000,
+                                "chains_wall": S0[:, 19].sum() / a.steps / 1000,
+                                "dp_wall": S0[:, 20].sum() / a.steps / 1000},
                             "occ_extensions_per_read": S0[:, 2].sum() / S0[:, 0].sum(), "lanes_per_cu": lanes0}
         out["speedup_vs_plain_fmd"] = round(value / v0, 3)
         out["smem_stage_speedup_vs_plain_fmd"] = round((S0[:, 5].sum() + 0.0) / max(1.0, smem_us + S[:, 15].sum()), 3)
@@ -191,6 +203,10 @@ def main():
         if tuned:
             m.set("lanes_per_cu", tuned["lanes_per_cu"]).set("group_lanes", tuned["group_lanes"])
             m.set("text_path", tuned["text_path"]).set("read_filter", tuned["read_filter"])
+    tr, tr_src = measured_traffic(out["roofline"]["kernel"])
+    if tr is not None:
+        out["roofline"]["traffic"] = tr
+        out["roofline"]["traffic_source"] = tr_src
     if not a.no_gather:
         # random 64-B line gathers over buffers the size of the occ table and of occ + SA + ISA:
         # the random-access ceiling the SMEM kernel's lookups are held against
@@ -212,6 +228,20 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` as recorded in a committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE, scaled by the same counters on a copy of known size):
+    a line 'traffic_bytes_per_launch[<kernel>]: <bytes>' in profiles/*.md"""
+    import glob
+    import re
+    for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        for line in open(f):
+            mt = re.match(r"traffic_bytes_per_launch\[(.+?)\]:\s*([0-9.eE+]+)", line.strip())
+            if mt and mt.group(1) == kernel:
+                return float(mt.group(2)), os.path.relpath(f, ROOT)
+    return None, None
+
+
 def smem_kernel_name(m, form):
     g = m.get("group_lanes") if form == 2 else 0
     if g:
@@ -227,30 +257,36 @@ def run_steps(m, p, nb, steps, warmup, sync, log):
     batches the runs are pipelined: up to `depth` submitted batches are in flight, so that
     one batch's device stages overlap another's (two worker streams) and the host finishing
     of a third.  Returns (seconds, per-step counters)."""
-    for w in range(warmup):
-        m.submit(w % nb, p, True)
-        m.wait(w % nb)
-        if log:
-            log("warmup step %d done" % w)
     depth = 1 if (nb == 1 or run_steps.serial) else max(1, min(nb, run_steps.depth))
-    sync()
-    stats = []
-    t = time.time()
-    inflight = collections.deque()
-    tsub = twait = 0.0
-    for k in range(steps):
-        if len(inflight) >= depth:
+
+    def pipelined(n, timed):
+        out, inflight = [], collections.deque()
+        ts = tw = 0.0
+        for k in range(n):
+            if len(inflight) >= depth:
+                t1 = time.time()
+                out.append(m.wait(inflight.popleft()))
+                tw += time.time() - t1
+            t0 = time.time()
+            m.submit(k % nb, p, True)
+            ts += time.time() - t0
+            inflight.append(k % nb)
+            if not timed and log:
+                log("warmup step %d submitted" % k)
+        while inflight:
             t1 = time.time()
-            stats.append(m.wait(inflight.popleft()))
-            twait += time.time() - t1
-        t0 = time.time()
-        m.submit(k % nb, p, True)
-        tsub += time.time() - t0
-        inflight.append(k % nb)
-    while inflight:
-        t1 = time.time()
-        stats.append(m.wait(inflight.popleft()))
-        twait += time.time() - t1
+            out.append(m.wait(inflight.popleft()))
+            tw += time.time() - t1
+        return out, ts, tw
+
+    # warm-up at the timed depth over every resident batch, so that every buffer the
+    # pipeline cycles through (device pools of both worker streams, pinned result buffers,
+    # each slot's SAM text) exists before timing
+    if warmup > 0:
+        pipelined(max(warmup, depth + 1, min(nb, steps)), False)
+    sync()
+    t = time.time()
+    stats, tsub, twait = pipelined(steps, True)
     sync()
     run_steps.split = {"submit_ms_per_step": 1000 * tsub / steps, "wait_ms_per_step": 1000 * twait / steps}
     return time.time() - t, np.array(stats)

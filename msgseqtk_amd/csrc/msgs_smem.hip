@@ -55,6 +55,7 @@ struct Ctx {
     const DevIndex& ix;
     const SmemArgs& a;
     int64_t lane;
+    bool grouped;   /* lane groups keep each group's list contiguous (chunks of G entries) */
 };
 
 __device__ __forceinline__ void emit(const Ctx& c, Lane& l, int from, int to, int32_t F, int32_t R, int32_t S, double ll) {
@@ -65,9 +66,12 @@ __device__ __forceinline__ void emit(const Ctx& c, Lane& l, int from, int to, in
     l.nOut++;
 }
 
-/* candidate k of a lane (or group): interleaved over lanes, so that lanes at the same
- * list position touch neighbouring entries */
-__device__ __forceinline__ size_t cidx(const Ctx& c, int k) { return (size_t)k * c.a.nLanes + c.lane; }
+/* candidate k of a lane: interleaved over lanes, so that lanes at the same list position
+ * touch neighbouring entries; of a lane group: contiguous, so that a chunk of G entries is
+ * one coalesced access */
+__device__ __forceinline__ size_t cidx(const Ctx& c, int k) {
+    return c.grouped ? (size_t)c.lane * c.a.candCap + k : (size_t)k * c.a.nLanes + c.lane;
+}
 
 __device__ __forceinline__ void candStore(const Ctx& c, int k, int pos, int32_t F, int32_t R, int32_t S, double ll,
                                           int32_t X) {
@@ -330,7 +334,7 @@ __global__ void __launch_bounds__(256, 4) smem_kernel_t(DevIndex ix, SmemArgs a)
     Lane l;
     l.nExt = 0; l.nFast = 0; l.nLook = 0;
     l.state = ST_IDLE;
-    Ctx c{ix, a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x};
+    Ctx c{ix, a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, false};
     const int64_t minLen = a.minLen;
     const double llA = ix.ll[1], llC = ix.ll[2], llG = ix.ll[4], llT = ix.ll[8];
     uint32_t nIter = 0;
@@ -594,7 +598,7 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
     int rPos = 0;
     int32_t rP = 0, rQ = 0, rS = 0, rX = -1;
     double rLL = 0;
-    Ctx c{ix, a, ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G};
+    Ctx c{ix, a, ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G, true};
     const int64_t minLen = a.minLen;
     const double llA = ix.ll[1], llC = ix.ll[2], llG = ix.ll[4], llT = ix.ll[8];
     uint32_t nIter = 0;
