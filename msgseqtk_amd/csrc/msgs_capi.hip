@@ -277,7 +277,7 @@ int msgs_db_build(msgs_ctx* c, const uint8_t* seqs, const int64_t* lens, int32_t
     int rc = buildIndexDevice(db, st, err);
     if (rc != MSGS_OK) return fail(c, rc, err);
     /* targets and coordinates */
-    if (hipMalloc(&db.dTgt, db.tgtHost.size()) != hipSuccess ||
+    if (hipMalloc(&db.dTgt, db.tgtHost.size() + 64) != hipSuccess || hipMemset(db.dTgt, 0, db.tgtHost.size() + 64) != hipSuccess ||
         hipMalloc(&db.dTgtOff, nChrom * sizeof(int64_t)) != hipSuccess ||
         hipMalloc(&db.dChromLen, nChrom * sizeof(int64_t)) != hipSuccess ||
         hipMalloc(&db.dChromBD, (nChrom + 1) * sizeof(int64_t)) != hipSuccess)

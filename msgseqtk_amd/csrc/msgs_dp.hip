@@ -50,6 +50,17 @@ __device__ __forceinline__ int32_t addn(int32_t a, int32_t b) { return a == NEG 
 __device__ __forceinline__ int32_t subn(int32_t a, int32_t b) { return a == NEG ? NEG : a - b; }
 __device__ __forceinline__ int32_t max3(int32_t a, int32_t b, int32_t c) { return max(a, max(b, c)); }
 
+/* sequential byte reads through the aligned 8-byte word holding each byte */
+struct Bytes8 {
+    uint64_t w = 0;
+    uintptr_t wa = ~(uintptr_t)0;
+    __device__ __forceinline__ uint32_t get(const uint8_t* p) {
+        const uintptr_t a = (uintptr_t)p & ~(uintptr_t)7;
+        if (a != wa) { wa = a; w = *(const uint64_t*)a; }
+        return (uint32_t)(w >> (8 * ((uintptr_t)p & 7))) & 0xffu;
+    }
+};
+
 /* a rectangle with its tile offset and event index */
 struct Rc {
     int32_t r0, r1, c0, c1, ev;
@@ -642,12 +653,15 @@ __global__ void __launch_bounds__(256) dp_trace_kernel(DevIndex ix, DpArgs a) {
         double x = 0;
         for (int64_t c = 0; c < alnFrom; ++c) x += -a.clip;
         int64_t qi = alnFrom, tj = alnStart;
+        /* read, quality and target bytes come from 8-byte words (buffers padded) */
+        Bytes8 sQ, sV, sT;
         for (int64_t r = cgCap - nRun; r < cgCap; ++r) {
             const uint32_t rop = cg[r] & 15u, rlen = cg[r] >> 4;
             for (uint32_t u = 0; u < rlen; ++u) {
-                const uint8_t qb = (qi >= 0 && qi < L) ? q[qi] : (uint8_t)0;
-                const uint8_t tbv = tb(tj);
-                const int qv = (qi >= 0 && qi < L) ? qual[qi] : 0;
+                const bool qin = qi >= 0 && qi < L;
+                const uint8_t qb = qin ? (uint8_t)sQ.get(q + qi) : (uint8_t)0;
+                const uint8_t tbv = (tj >= 0 && tj <= chromLen) ? (uint8_t)sT.get(tg + tj) : (uint8_t)0;
+                const int qv = qin ? (int)sV.get(qual + qi) : 0;
                 uint32_t op = rop;
                 if (op == OP_M) {
                     x += (qb & tbv) ? a.qMatch[qv] : a.qMis[qv];

@@ -262,8 +262,8 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     DpArgs a;
     std::memset(&a, 0, sizeof(a));
     a.seqs = r.seq; a.quals = dQual; a.offs = r.offs;
-    uint8_t* rcS = c->x().bufs.get<uint8_t>("dp.rs", r.total + 1, e);
-    uint8_t* rcQ = c->x().bufs.get<uint8_t>("dp.rq", r.total + 1, e);
+    uint8_t* rcS = c->x().bufs.get<uint8_t>("dp.rs", r.total + 64, e);
+    uint8_t* rcQ = c->x().bufs.get<uint8_t>("dp.rq", r.total + 64, e);
     a.rcSeqs = rcS;
     a.rcQuals = rcQ;
     a.seedOff = sd.dSeedOff; a.seeds = sd.dSeeds; a.nJobs = nJobs;
@@ -666,7 +666,7 @@ struct BatchResult {
 
 static int uploadQual(msgs_ctx* c, const msgs_reads* r, const DevReads& d, const char* name, uint8_t** out) {
     hipError_t e = hipSuccess;
-    uint8_t* q = c->x().bufs.get<uint8_t>(name, d.total + 1, e);
+    uint8_t* q = c->x().bufs.get<uint8_t>(name, d.total + 64, e);   /* padded for word reads */
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "quality buffer");
     if (d.total) {
         if (r->qual) CK(hipMemcpyAsync(q, r->qual + r->offs[0], d.total, hipMemcpyHostToDevice, c->x().stream), "qual");
