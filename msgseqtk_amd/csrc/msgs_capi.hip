@@ -420,9 +420,13 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
         (void)hipEventRecord(c->x().ev2, c->x().stream);
         int rc = runLixFilter(c, r, (int32_t)p->minLen, T, a.gate1, &need, &exact, dStats);
         if (rc) return rc;
-        (void)hipEventRecord(c->x().ev3, c->x().stream);
         a.need = need;
         a.exact = exact;
+        int32_t* order = nullptr;
+        rc = runLixOrder(c, n, need, &order);
+        if (rc) return rc;
+        a.order = order;
+        (void)hipEventRecord(c->x().ev3, c->x().stream);
     }
     const int G = c->form == 2 ? c->groupLanes : 0;
     const int64_t lanesMax = (int64_t)std::max(1, c->nCU) * c->lanesPerCU;

@@ -168,5 +168,6 @@ int uploadReads(msgs_ctx* c, const msgs_reads* r, const char* pfx, DevReads& out
 int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out, unsigned long long* dStats);
 int runLixFilter(msgs_ctx* c, const DevReads& r, int32_t minLen, double T, int32_t gate1, uint8_t** needOut,
                  SmemRec** exactOut, unsigned long long* dStats);
+int runLixOrder(msgs_ctx* c, int64_t n, const uint8_t* need, int32_t** orderOut);
 
 #endif

@@ -39,9 +39,11 @@ struct SmemArgs {
     SmemCand* cand;            /* candidate scratch, [nLanes][candCap] */
     int64_t candCap;
     int64_t nLanes;
-    const uint8_t* need;       /* per read (may be NULL): 0 = the read cannot emit an SMEM, skip it;
-                                * 2 = its whole SMEM list is exact[r] alone; 1 or 3 = search */
+    const uint8_t* need;       /* per read (may be NULL), bits 0-1: 0 = the read cannot emit an SMEM,
+                                * skip it; 2 = its whole SMEM list is exact[r] alone; 1 or 3 =
+                                * search; bit 2: start it early */
     const SmemRec* exact;      /* per read, read where need[r] == 2 */
+    const int32_t* order;      /* the order in which the queue hands out reads (may be NULL) */
     const double* prefLL;      /* per read at offs[r] + r: L+1 left-to-right loglik prefix sums */
     unsigned long long* queue; /* next read */
     unsigned long long* stats; /* [0] occ-block extensions, [2] text steps, [3] SA/ISA lookups,

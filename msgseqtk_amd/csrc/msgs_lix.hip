@@ -415,9 +415,11 @@ __device__ __forceinline__ uint64_t readKey(const uint8_t* s) {
 
 /*
  * Form-2 front end: the learned index decides, before the SMEM search, what the search
- * would output for two kinds of reads.  need[r] codes: 0 = the read emits no SMEM (skip
- * it), 1 = search it, 2 = its SMEM list is exact[r] alone, 3 = undecided (searched unless
- * a later pass decides it).
+ * would output for two kinds of reads.  need[r] codes (bits 0-1): 0 = the read emits no
+ * SMEM (skip it), 1 = search it, 2 = its SMEM list is exact[r] alone, 3 = undecided
+ * (searched unless a later pass decides it); bit 2 marks searches to start first (the
+ * read's middle 16-mer occurs nowhere, the sign of a long search), and order[] lists the
+ * reads in that order (k_order_*).
  *
  * Exact reads (k_exact, two lanes per read).  Let h = L/2 and suppose every base is
  * A/C/G/T, the read occurs exactly once in the BD text, and so do read[0, h+1) and
@@ -476,10 +478,12 @@ __global__ void __launch_bounds__(256) k_exact(LixView m, DevIndex ix, const uin
         const int64_t xp = predict(m, xkey, xact, W1b, true, &xj);
         int32_t nPart = -1, nFull = 0;
         int64_t fRow = -1, fPos = -1;
+        bool emptyBlock = false;
         if (xact) {
             ++nq;
             int32_t pr = 0, fb = 0;
             int64_t r = lbBounded(m, xkey, xp, (int64_t)m.eSeg[xj], &pr, &fb);
+            emptyBlock = r >= m.n || m.ct[r] != xkey;
             const int64_t s0 = sub == 0 ? 0 : h, len = L - s0, part = sub == 0 ? h + 1 : L - h;
             nPart = 0;
             for (int k = 0; r < m.n && m.ct[r] == xkey; ++r, ++k) {
@@ -491,6 +495,9 @@ __global__ void __launch_bounds__(256) k_exact(LixView m, DevIndex ix, const uin
             }
         }
         const int32_t nSuffix = __shfl(nPart, lane + 1);
+        /* the read's middle 16-mer occurs nowhere: a read with errors there or from outside
+         * the database, likely among the longest searches */
+        const bool midEmpty = __shfl((int)emptyBlock, lane + 1) != 0;
         bool xok = xact && sub == 0 && nPart == 1 && nFull == 1 && nSuffix == 1;
         if (xok) {
             /* logP in the search's order: the base at h, the forward steps to L-1, then the
@@ -526,7 +533,7 @@ __global__ void __launch_bounds__(256) k_exact(LixView m, DevIndex ix, const uin
             }
         }
         if (mine && sub == 0) {
-            need[rid] = xok ? 2 : 3;
+            need[rid] = xok ? 2 : (midEmpty ? 7 : 3);
             nExact += xok ? 1 : 0;
         }
     }
@@ -594,7 +601,8 @@ __global__ void __launch_bounds__(256) k_window1(LixView m, DevIndex ix, const u
     unsigned long long nq = 0, nSkip = 0;
     for (int64_t r0 = wave * kGroups; r0 < nReads; r0 += nWaves * kGroups) {
         const int64_t rid = r0 + g;
-        const bool mine = rid < nReads && need[rid] == 3;
+        const uint8_t nd0 = rid < nReads ? need[rid] : 0;
+        const bool mine = (nd0 & 3) == 3;
         const int64_t off = mine ? offs[rid] : 0;
         const int64_t L = mine ? offs[rid + 1] - off : 0;
         const int64_t nw = L >= minLen ? L - minLen + 1 : 0;
@@ -615,7 +623,7 @@ __global__ void __launch_bounds__(256) k_window1(LixView m, DevIndex ix, const u
         const unsigned long long bal = __ballot(keep);
         const bool found = ((bal >> (g * kFilterLanes)) & ((1ull << kFilterLanes) - 1)) != 0;
         if (mine && sub == 0) {
-            if (found) need[rid] = 1;
+            if (found) need[rid] = (uint8_t)((nd0 & 4) | 1);
             else if (nw <= kFilterLanes) { need[rid] = 0; ++nSkip; }
             else open[atomicAdd(nOpen, 1ull)] = rid;
         }
@@ -664,7 +672,7 @@ __global__ void __launch_bounds__(256) k_window2(LixView m, DevIndex ix, const u
             found = __ballot(keep) != 0;
         }
         if (lane == 0) {
-            need[rid] = found ? 1 : 0;
+            need[rid] = found ? (uint8_t)((need[rid] & 4) | 1) : 0;
             nSkip += found ? 0 : 1;
         }
     }
@@ -672,6 +680,34 @@ __global__ void __launch_bounds__(256) k_window2(LixView m, DevIndex ix, const u
         atomicAdd(&stats[4], nq);
         if (nSkip) atomicAdd(&stats[9], nSkip);
     }
+}
+
+/* the search order: reads marked to start first, then the others (order inside each
+ * part is whatever the atomics give; results do not depend on it) */
+__global__ void __launch_bounds__(256) k_order_count(const uint8_t* need, int64_t n, unsigned long long* cnt) {
+    __shared__ unsigned int s;
+    if (threadIdx.x == 0) s = 0;
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && (need[i] & 4)) atomicAdd(&s, 1u);
+    __syncthreads();
+    if (threadIdx.x == 0 && s) atomicAdd(&cnt[0], (unsigned long long)s);
+}
+
+__global__ void __launch_bounds__(256) k_order_scatter(const uint8_t* need, int64_t n, unsigned long long* cnt,
+                                                       int32_t* order) {
+    __shared__ unsigned int s[2];
+    __shared__ unsigned long long base[2];
+    if (threadIdx.x < 2) s[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int part = -1;
+    unsigned int r = 0;
+    if (i < n) { part = (need[i] & 4) ? 0 : 1; r = atomicAdd(&s[part], 1u); }
+    __syncthreads();
+    if (threadIdx.x < 2 && s[threadIdx.x]) base[threadIdx.x] = atomicAdd(&cnt[1 + threadIdx.x], (unsigned long long)s[threadIdx.x]);
+    __syncthreads();
+    if (part >= 0) order[(part == 0 ? 0 : cnt[0]) + base[part] + r] = (int32_t)i;
 }
 
 /* raw model outputs (before rounding) for checks: mode 0 bf16 MFMA, mode 1 the same
@@ -915,6 +951,22 @@ int runLixFilter(msgs_ctx* c, const DevReads& r, int32_t minLen, double T, int32
     LCK(hipGetLastError(), "filter");
     *needOut = need;
     *exactOut = exact;
+    return MSGS_OK;
+}
+
+int runLixOrder(msgs_ctx* c, int64_t n, const uint8_t* need, int32_t** orderOut) {
+    hipError_t e = hipSuccess;
+    int32_t* order = c->x().bufs.get<int32_t>("lx.order", n + 1, e);
+    unsigned long long* cnt = c->x().bufs.get<unsigned long long>("lx.ocnt", 4, e);
+    if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "search order");
+    if (n) {
+        LCK(hipMemsetAsync(cnt, 0, 4 * 8, c->x().stream), "order counters");
+        const unsigned blocks = (unsigned)((n + 255) / 256);
+        k_order_count<<<blocks, 256, 0, c->x().stream>>>(need, n, cnt);
+        k_order_scatter<<<blocks, 256, 0, c->x().stream>>>(need, n, cnt, order);
+    }
+    LCK(hipGetLastError(), "order");
+    *orderOut = order;
     return MSGS_OK;
 }
 

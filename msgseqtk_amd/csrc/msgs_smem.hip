@@ -298,6 +298,7 @@ __device__ bool nextRead(const Ctx& c, Lane& l) {
     while (true) {
         int64_t rid = (int64_t)atomicAdd(c.a.queue, 1ull);
         if (rid >= c.a.nReads) return false;
+        if (c.a.order) rid = c.a.order[rid];
         l.rid = rid;
         const int64_t off = c.a.offs[rid];
         l.s = c.a.seqs + off;
@@ -313,7 +314,7 @@ __device__ bool nextRead(const Ctx& c, Lane& l) {
             driverAfter(c, l);
             if (l.state != ST_IDLE) return true;
         }
-        if (nd == 2) { c.a.out[rid * c.a.cap] = c.a.exact[rid]; l.nOut = 1; }
+        if ((nd & 3) == 2) { c.a.out[rid * c.a.cap] = c.a.exact[rid]; l.nOut = 1; }
         c.a.cnt[rid] = l.nOut;
     }
 }
@@ -530,6 +531,7 @@ __device__ bool nextReadGroup(const Ctx& c, Lane& l, int gbase, int gl) {
         if (gl == 0) rid = (int64_t)atomicAdd(c.a.queue, 1ull);
         rid = shfl64(rid, gbase);
         if (rid >= c.a.nReads) return false;
+        if (c.a.order) rid = c.a.order[rid];
         l.rid = rid;
         const int64_t off = c.a.offs[rid];
         l.s = c.a.seqs + off;
@@ -545,7 +547,7 @@ __device__ bool nextReadGroup(const Ctx& c, Lane& l, int gbase, int gl) {
             driverAfter(c, l);
             if (l.state != ST_IDLE) return true;
         }
-        if (nd == 2) {
+        if ((nd & 3) == 2) {
             if (gl == 0) c.a.out[rid * c.a.cap] = c.a.exact[rid];
             l.nOut = 1;
         }
