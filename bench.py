@@ -31,7 +31,7 @@ def log(*a):
 def parse():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--steps", type=int, default=24)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--genomes", type=int, default=64)
     ap.add_argument("--genome-len", type=int, default=4_000_000)
@@ -49,7 +49,8 @@ def parse():
     ap.add_argument("--lix-segments", type=int, default=1 << 20, help="learned index: root segments")
     ap.add_argument("--lix-steps", type=int, default=1000, help="learned index: training steps")
     ap.add_argument("--no-gather", action="store_true", help="skip the random-gather ceiling measurement")
-    ap.add_argument("--depth", type=int, default=3, help="batches in flight (device stages of two overlap)")
+    ap.add_argument("--depth", type=int, default=5, help="batches in flight")
+    ap.add_argument("--streams", type=int, default=3, help="batches whose device stages run at once")
     ap.add_argument("--lanes-per-cu", type=int, default=0, help="fix the SMEM kernel's lanes per CU (no autotuning)")
     ap.add_argument("--group-lanes", type=int, default=-1, help="fix the form-2 lanes per read: 0, 8 or 16")
     ap.add_argument("--text-path", type=int, default=-1, help="fix the form-2 one-lane kernel's text path: 0 or 1")
@@ -117,6 +118,7 @@ def main():
                  "read_filter": m.get("read_filter"), "smem_us": round(tus, 1)}
         log("autotuner: %d lanes per CU, %d lanes per read (%.0f us for the SMEM stage of batch 0)" % (
             lanes, tuned["group_lanes"], tus))
+    m.set("streams", a.streams)
     run_steps.serial = a.no_pipeline
     run_steps.depth = a.depth
     if a.host_threads:
@@ -255,8 +257,8 @@ def run_steps(m, p, nb, steps, warmup, sync, log):
     #  run_steps.depth: batches in flight at once)
     """Map `steps` batches (batch k = resident slot k % nb).  With two or more resident
     batches the runs are pipelined: up to `depth` submitted batches are in flight, so that
-    one batch's device stages overlap another's (two worker streams) and the host finishing
-    of a third.  Returns (seconds, per-step counters)."""
+    the device stages of several batches overlap (worker streams) and their host
+    finishing overlaps both.  Returns (seconds, per-step counters)."""
     depth = 1 if (nb == 1 or run_steps.serial) else max(1, min(nb, run_steps.depth))
 
     def pipelined(n, timed):

@@ -70,8 +70,9 @@ const char* msgs_last_error(msgs_ctx* ctx);
  * "text_path" form 2 with one lane per read: 1 follows unique intervals through the text
  *   (as form 1), 0 extends with occ blocks only (as form 0);
  * "lanes_per_cu" persistent lanes of the SMEM kernel per CU; "smem_cap" SMEM slots per
- * read before a regrow; "host_threads" threads of the host finishing step.  Results never
- * depend on these settings. */
+ * read before a regrow; "host_threads" threads of the host finishing step; "streams" how
+ * many submitted batches run their device stages at once (1..4, default 3).  Results
+ * never depend on these settings. */
 int msgs_ctx_set(msgs_ctx* ctx, const char* key, int64_t value);
 int msgs_ctx_get(msgs_ctx* ctx, const char* key, int64_t* value);
 
@@ -149,10 +150,11 @@ int msgs_align_batch(msgs_ctx* ctx, const msgs_reads* reads, const msgs_params* 
  * SAM text, PE pairing, MAPQ and alignment lists (n <= 32). */
 int msgs_batch_upload(msgs_ctx* ctx, int32_t slot, const msgs_reads* reads, const msgs_reads* mates);
 int msgs_batch_run(msgs_ctx* ctx, int32_t slot, const msgs_params* p, int32_t emitSam);
-/* msgs_batch_run in two halves: submit runs the GPU stages and leaves the host
- * finishing (records, MAPQ, SAM text) to a worker thread of the slot, so the caller can
- * submit the next batch while it runs; wait joins it and makes its counters and SAM the
- * context's last run (msgs_batch_stats / msgs_batch_sam). */
+/* msgs_batch_run in two halves: submit returns at once; a worker thread of the slot runs
+ * the device stages (on one of the context's worker streams, see "streams") and then the
+ * host finishing (records, MAPQ, SAM text), so the caller can submit further batches
+ * meanwhile; wait joins it and makes its counters and SAM the context's last run
+ * (msgs_batch_stats / msgs_batch_sam). */
 int msgs_batch_submit(msgs_ctx* ctx, int32_t slot, const msgs_params* p, int32_t emitSam);
 int msgs_batch_wait(msgs_ctx* ctx, int32_t slot);
 int msgs_batch_stats(msgs_ctx* ctx, double* stats, int32_t n);

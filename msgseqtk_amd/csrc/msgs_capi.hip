@@ -67,7 +67,7 @@ Exec* msgs_ctx::acquireWork() {
     std::unique_lock<std::mutex> lk(workMu);
     int k = -1;
     workCv.wait(lk, [&]() {
-        for (int i = 0; i < kWorkExecs; ++i)
+        for (int i = 0; i < std::min(workStreams, kWorkExecs); ++i)
             if (!workBusy[i]) { k = i; return true; }
         return false;
     });
@@ -629,6 +629,7 @@ extern "C" int msgs_ctx_set(msgs_ctx* c, const char* key, int64_t value) {
     else if (k == "host_threads") { if (value < 1 || value > 256) return fail(c, MSGS_E_ARG, "host_threads in [1, 256]"); c->hostThreads = (int)value; }
     else if (k == "read_filter") { if (value != 0 && value != 1) return fail(c, MSGS_E_ARG, "read_filter: 0 or 1"); c->readFilter = (int)value; }
     else if (k == "text_path") { if (value != 0 && value != 1) return fail(c, MSGS_E_ARG, "text_path: 0 or 1"); c->textPath = (int)value; }
+    else if (k == "streams") { if (value < 1 || value > kWorkExecs) return fail(c, MSGS_E_ARG, "streams in [1, 4]"); c->workStreams = (int)value; }
     else return fail(c, MSGS_E_ARG, "unknown option " + k);
     return MSGS_OK;
 }
@@ -643,6 +644,7 @@ extern "C" int msgs_ctx_get(msgs_ctx* c, const char* key, int64_t* value) {
     else if (k == "host_threads") *value = c->hostThreads;
     else if (k == "read_filter") *value = c->readFilter;
     else if (k == "text_path") *value = c->textPath;
+    else if (k == "streams") *value = c->workStreams;
     else return fail(c, MSGS_E_ARG, "unknown option " + k);
     return MSGS_OK;
 }

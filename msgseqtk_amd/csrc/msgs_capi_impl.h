@@ -49,6 +49,10 @@ struct DevReads {
     uint8_t* seq = nullptr;
     int64_t* offs = nullptr;
     std::vector<int64_t> offsHost;
+    /* resident batches: reverse complements of the reads and their qualities, made once at
+     * upload (NULL: the DP stage makes them per run) */
+    uint8_t* rcSeq = nullptr;
+    uint8_t* rcQual = nullptr;
 };
 
 struct SmemStage {
@@ -129,7 +133,7 @@ struct Exec {
     bool create(unsigned streamFlags);
     void destroy();
 };
-constexpr int kWorkExecs = 2;
+constexpr int kWorkExecs = 4;   /* worker executors created; "streams" of them are used */
 
 struct msgs_ctx {
     int device = 0;
@@ -142,6 +146,7 @@ struct msgs_ctx {
     int groupLanes = 8;   /* form 2: lanes per read of the SMEM kernel (0 = one lane per read) */
     Exec own;                      /* the caller's thread */
     Exec work[kWorkExecs];         /* submitted batches' device stages */
+    int workStreams = 3;           /* how many run at once */
     bool workBusy[kWorkExecs] = {};
     std::mutex workMu;
     std::condition_variable workCv;

@@ -577,11 +577,40 @@ __global__ void __launch_bounds__(256) dp_trace_kernel(DevIndex ix, DpArgs a) {
             const Rc R3 = rectR3(G);
             const int32_t cEnd = R3.ok ? R3.c0 - 1 : g.tLen;
             const int kL = rowSeg(G, L);
-            for (int32_t c = 1; c <= cEnd; ++c) {
-                const int32_t m = vMk(G, L, c, T, kL);
-                if (m > best) { best = m; alnEnd = c; s = OP_M; }
-                const int32_t x = vIk(G, L, c, T, kL);
-                if (x > best) { best = x; alnEnd = c; s = OP_I; }
+            /* Row L lies in the band of the last seed (every seed starts at or above it).
+             * Unless row L is the row just after a seed (whose I row is overwritten from
+             * the M row above), I is -inf on row L at every column >= 1, and M is -inf
+             * except on the last seed's diagonal and at step-c cells (vMk / vIk with
+             * i = L, e = T): -inf never passes the strict '>' from best = -inf, so only
+             * those columns are evaluated, in column order. */
+            bool general = kL < 0;
+            int32_t cand[3] = {-1, -1, -1};
+            if (!general) {
+                const int4 sk = G.sg[kL];
+                const bool inSeedRows = L < sk.x + sk.z;
+                const bool stepRow = L == sk.x + sk.z;
+                bool prevStepRow = false;
+                int4 pk = make_int4(0, 0, 0, 0);
+                if (kL >= 1) { pk = G.sg[kL - 1]; prevStepRow = L == pk.x + pk.z; }
+                if (stepRow || prevStepRow) general = true;
+                else if (inSeedRows) cand[0] = sk.y + (L - sk.x);
+            }
+            if (general) {
+                for (int32_t c = 1; c <= cEnd; ++c) {
+                    const int32_t m = vMk(G, L, c, T, kL);
+                    if (m > best) { best = m; alnEnd = c; s = OP_M; }
+                    const int32_t x = vIk(G, L, c, T, kL);
+                    if (x > best) { best = x; alnEnd = c; s = OP_I; }
+                }
+            } else {
+                for (int t = 0; t < 3; ++t) {
+                    const int32_t c = cand[t];
+                    if (c < 1 || c > cEnd) continue;
+                    const int32_t m = vMk(G, L, c, T, kL);
+                    if (m > best) { best = m; alnEnd = c; s = OP_M; }
+                    const int32_t x = vIk(G, L, c, T, kL);
+                    if (x > best) { best = x; alnEnd = c; s = OP_I; }
+                }
             }
             if (R3.ok && S.nwV > best) { best = S.nwV; alnEnd = S.nwC; s = S.nwS; }
         } else {

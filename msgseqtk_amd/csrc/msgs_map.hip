@@ -262,8 +262,9 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     DpArgs a;
     std::memset(&a, 0, sizeof(a));
     a.seqs = r.seq; a.quals = dQual; a.offs = r.offs;
-    uint8_t* rcS = c->x().bufs.get<uint8_t>("dp.rs", r.total + 64, e);
-    uint8_t* rcQ = c->x().bufs.get<uint8_t>("dp.rq", r.total + 64, e);
+    const bool haveRc = r.rcSeq && r.rcQual;
+    uint8_t* rcS = haveRc ? r.rcSeq : c->x().bufs.get<uint8_t>("dp.rs", r.total + 64, e);
+    uint8_t* rcQ = haveRc ? r.rcQual : c->x().bufs.get<uint8_t>("dp.rq", r.total + 64, e);
     a.rcSeqs = rcS;
     a.rcQuals = rcQ;
     a.seedOff = sd.dSeedOff; a.seeds = sd.dSeeds; a.nJobs = nJobs;
@@ -294,7 +295,8 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "dp scratch");
     CK(hipMemcpyAsync(dq, hq, sizeof(hq), hipMemcpyHostToDevice, st), "qtab");
     a.qMatch = dq; a.qMis = dq + 256; a.qGapOpen = dq + 512;
-    if (n) k_revcom<<<(unsigned)std::min<int64_t>((n + 3) / 4, 65536), 256, 0, st>>>(r.seq, dQual, r.offs, n, rcS, rcQ);
+    if (n && !haveRc)
+        k_revcom<<<(unsigned)std::min<int64_t>((n + 3) / 4, 65536), 256, 0, st>>>(r.seq, dQual, r.offs, n, rcS, rcQ);
     dp_setup_kernel<<<gridFor(nJobs), 256, 0, st>>>(c->db.view(), a);
     /* tile and path offsets (exclusive scans with the totals at nJobs) and the wavefront
      * job list, all on the device; the host reads back three totals */
@@ -970,6 +972,19 @@ extern "C" int msgs_batch_upload(msgs_ctx* c, int32_t slot, const msgs_reads* re
     std::snprintf(pfx, sizeof pfx, "slot%d.q", slot);
     rc = uploadQual(c, all, S.dev, pfx, &S.dQual);
     if (rc) return rc;
+    /* reverse complements for the DP, once per resident batch */
+    {
+        hipError_t e = hipSuccess;
+        std::snprintf(pfx, sizeof pfx, "slot%d.rs", slot);
+        S.dev.rcSeq = c->x().bufs.get<uint8_t>(pfx, S.dev.total + 64, e);
+        std::snprintf(pfx, sizeof pfx, "slot%d.rq", slot);
+        S.dev.rcQual = c->x().bufs.get<uint8_t>(pfx, S.dev.total + 64, e);
+        if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "reverse complements");
+        const int64_t n = S.dev.n;
+        if (n)
+            k_revcom<<<(unsigned)std::min<int64_t>((n + 3) / 4, 65536), 256, 0, c->x().stream>>>(
+                S.dev.seq, S.dQual, S.dev.offs, n, S.dev.rcSeq, S.dev.rcQual);
+    }
     if (hipStreamSynchronize(c->x().stream) != hipSuccess) return fail(c, MSGS_E_HIP, "upload");
     S.paired = mates != nullptr;
     S.used = true;
@@ -1067,7 +1082,7 @@ static int batchSubmit(msgs_ctx* c, int32_t slot, const msgs_params* p, int32_t 
     const auto t0 = std::chrono::steady_clock::now();
     S.worker = std::thread([c, &S, emit, t0]() {
         (void)hipSetDevice(c->device);
-        /* device stages on a free worker executor (two batches can be in them at once) */
+        /* device stages on a free worker executor (several batches can be in them at once) */
         Exec* X = c->acquireWork();
         int rc;
         try {

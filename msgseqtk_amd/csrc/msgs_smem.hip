@@ -482,7 +482,8 @@ __global__ void __launch_bounds__(256) smem_prefix_kernel(DevIndex ix, SmemArgs 
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int64_t r0 = ((int64_t)blockIdx.x * 4 + w) * 64; r0 < a.nReads; r0 += (int64_t)gridDim.x * 4 * 64) {
         const int64_t r = r0 + lane;
-        const bool mine = r < a.nReads;
+        /* reads the search skips (form 2: exact or filtered) need no sums */
+        const bool mine = r < a.nReads && (!a.need || (a.need[r] & 1));
         const int64_t o = mine ? a.offs[r] : 0;
         const int32_t L = mine ? (int32_t)(a.offs[r + 1] - o) : -1;
         sBase[w][lane] = o + r;
