@@ -50,6 +50,15 @@ __device__ __forceinline__ int32_t addn(int32_t a, int32_t b) { return a == NEG 
 __device__ __forceinline__ int32_t subn(int32_t a, int32_t b) { return a == NEG ? NEG : a - b; }
 __device__ __forceinline__ int32_t max3(int32_t a, int32_t b, int32_t c) { return max(a, max(b, c)); }
 
+/* Decision bytes of a rectangle are stored strip by strip (strips of 64 rows, as the
+ * fill sweeps them) and, inside a strip, by anti-diagonal step: the bytes one step of
+ * the fill writes are contiguous.  A strip of `rows` rows and C columns takes
+ * (C + rows - 1) * rows bytes. */
+__device__ __forceinline__ int64_t stripBytes(int64_t rows, int64_t cols) {
+    const int64_t full = rows >> 6, rem = rows & 63;
+    return full * 64 * (cols + 63) + (rem ? (cols + rem - 1) * rem : 0);
+}
+
 /* sequential byte reads through the aligned 8-byte word holding each byte */
 struct Bytes8 {
     uint64_t w = 0;
@@ -70,6 +79,12 @@ struct Rc {
     __device__ int64_t idx(int32_t i, int32_t j) const { return off + (int64_t)(j - c0) * (r1 - r0 + 1) + (i - r0); }
     __device__ bool hasCol(int32_t j) const { return j >= c0 && j <= c1; }
     __device__ int64_t area() const { return ok ? (int64_t)(r1 - r0 + 1) * (c1 - c0 + 1) : 0; }
+    /* decision byte of cell (i, j) (strip layout, see stripBytes) */
+    __device__ int64_t bitIdx(int32_t i, int32_t j) const {
+        const int32_t ri = i - r0, jj = j - c0, s0 = ri & ~63, l = ri & 63;
+        const int32_t rows = min(64, r1 - r0 + 1 - s0), C = c1 - c0 + 1;
+        return off + (int64_t)s0 * (C + 63) + (int64_t)(jj + l) * rows + l;
+    }
 };
 
 struct Geo {
@@ -316,7 +331,7 @@ __global__ void __launch_bounds__(256) dp_setup_kernel(DevIndex ix, DpArgs a) {
         /* R5 only when both sides are non-empty (src/Alignment.cpp:57-58) */
         g.r5r1 = (int32_t)f.from; g.r5c1 = (int32_t)(f.start - tStart);
         if (!(0 < f.from && tStart < f.start)) { g.r5r1 = 0; g.r5c1 = 0; }
-        int64_t area = (int64_t)g.r5r1 * g.r5c1;
+        int64_t area = (g.r5r1 >= 1 && g.r5c1 >= 1) ? stripBytes(g.r5r1, g.r5c1) : 0;
         int64_t bord = (g.r5r1 >= 1 && g.r5c1 >= 1) ? (int64_t)g.r5r1 + g.r5c1 : 0;
         int32_t maxCols = g.r5r1 >= 1 ? g.r5c1 : 0;
         for (int k = 0; k < ns; ++k) {
@@ -327,7 +342,11 @@ __global__ void __launch_bounds__(256) dp_setup_kernel(DevIndex ix, DpArgs a) {
                 const int64_t rows = t.from - s.to, cols = t.start - s.end;
                 a.auxGap[cs + k] = area;
                 a.auxGapB[cs + k] = bord;
-                if (rows > 0 && cols > 0) { area += rows * cols; bord += rows + cols; maxCols = max(maxCols, (int32_t)cols); }
+                if (rows > 0 && cols > 0) {
+                    area += stripBytes(rows, cols);
+                    bord += rows + cols;
+                    maxCols = max(maxCols, (int32_t)cols);
+                }
             }
         }
         g.r3off = area;
@@ -336,7 +355,7 @@ __global__ void __launch_bounds__(256) dp_setup_kernel(DevIndex ix, DpArgs a) {
         if (!(l.to < L && l.end < tEnd)) { g.r3r0 = g.L + 1; g.r3c0 = g.tLen + 1; }
         else {
             const int64_t rows = g.L - g.r3r0 + 1, cols = g.tLen - g.r3c0 + 1;
-            area += rows * cols;
+            area += stripBytes(rows, cols);
             bord += rows + cols;
             maxCols = max(maxCols, (int32_t)cols);
         }
@@ -427,7 +446,7 @@ __global__ void __launch_bounds__(64) dp_fill_kernel(DevIndex ix, DpArgs a) {
                                              ((subn(subn(cM, G.go), G.ge) > subn(cD, G.ge)) ? 0u : 8u) |
                                              (m > 0 ? 16u : 0u);
                         cM = m; cI = nI; cD = nD;
-                        G.bits[R.off + (int64_t)jj * Rn + (i - R.r0)] = (uint8_t)dec;
+                        G.bits[R.off + (int64_t)s0 * (C + 63) + (int64_t)t * rows + lane] = (uint8_t)dec;
                         if (i == R.r1) G.border[R.boff + jj] = make_int4(m, nI, nD, 0);
                         if (jj == C - 1) G.border[R.boff + C + (i - R.r0)] = make_int4(m, nI, nD, 0);
                         if (more && lane == rows - 1) { sTM[jj + 1] = m; sTI[jj + 1] = nI; sTD[jj + 1] = nD; }
@@ -636,7 +655,7 @@ __global__ void __launch_bounds__(256) dp_trace_kernel(DevIndex ix, DpArgs a) {
             /* inside a rectangle of a wavefront-filled job, decisions come from its bytes */
             Rc Rb;
             const bool inR = !G.serial && cellRect(G, i, j, Rb);
-            const uint32_t dec = inR ? G.bits[Rb.off + (int64_t)(j - Rb.c0) * (Rb.r1 - Rb.r0 + 1) + (i - Rb.r0)] : 0u;
+            const uint32_t dec = inR ? G.bits[Rb.bitIdx(i, j)] : 0u;
             const bool go_on = G.mode == 0 ? i > 0 : (s != OP_M || (inR ? (dec & 16u) != 0 : vM(G, i, j, T) > 0));
             if (!go_on || np >= pathCap) break;
             if (s == OP_M && G.match > 0 && i >= 0) {
