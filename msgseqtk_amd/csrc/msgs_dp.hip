@@ -51,13 +51,28 @@ __device__ __forceinline__ int32_t subn(int32_t a, int32_t b) { return a == NEG 
 __device__ __forceinline__ int32_t max3(int32_t a, int32_t b, int32_t c) { return max(a, max(b, c)); }
 
 /* Decision bytes of a rectangle are stored strip by strip (strips of 64 rows, as the
- * fill sweeps them) and, inside a strip, by anti-diagonal step: the bytes one step of
- * the fill writes are contiguous.  A strip of `rows` rows and C columns takes
- * (C + rows - 1) * rows bytes. */
+ * fill sweeps them) and, inside a strip, diagonal by diagonal: cell (l, jj) of a strip
+ * (row l, column jj) sits at (jj - l + rows - 1) * rows + l, so the cells a run of
+ * backtrace M steps visits, (l, jj), (l-1, jj-1), ..., are consecutive bytes.  A strip
+ * of `rows` rows and C columns takes (C + rows - 1) * rows bytes. */
 __device__ __forceinline__ int64_t stripBytes(int64_t rows, int64_t cols) {
     const int64_t full = rows >> 6, rem = rows & 63;
     return full * 64 * (cols + 63) + (rem ? (cols + rem - 1) * rem : 0);
 }
+
+/* byte reads through the aligned 16-byte chunk holding each byte (buffer padded to a
+ * multiple of 16 bytes) */
+struct Bytes16 {
+    uint4 w;
+    int64_t a = -1;
+    __device__ __forceinline__ uint32_t get(const uint8_t* base, int64_t idx) {
+        const int64_t c = idx & ~(int64_t)15;
+        if (c != a) { a = c; w = *(const uint4*)(base + c); }
+        const uint32_t k = (uint32_t)idx & 15u;
+        const uint32_t d = k < 8 ? (k < 4 ? w.x : w.y) : (k < 12 ? w.z : w.w);
+        return (d >> (8 * (k & 3))) & 0xffu;
+    }
+};
 
 /* sequential byte reads through the aligned 8-byte word holding each byte */
 struct Bytes8 {
@@ -83,7 +98,7 @@ struct Rc {
     __device__ int64_t bitIdx(int32_t i, int32_t j) const {
         const int32_t ri = i - r0, jj = j - c0, s0 = ri & ~63, l = ri & 63;
         const int32_t rows = min(64, r1 - r0 + 1 - s0), C = c1 - c0 + 1;
-        return off + (int64_t)s0 * (C + 63) + (int64_t)(jj + l) * rows + l;
+        return off + (int64_t)s0 * (C + 63) + (int64_t)(jj - l + rows - 1) * rows + l;
     }
 };
 
@@ -446,7 +461,8 @@ __global__ void __launch_bounds__(64) dp_fill_kernel(DevIndex ix, DpArgs a) {
                                              ((subn(subn(cM, G.go), G.ge) > subn(cD, G.ge)) ? 0u : 8u) |
                                              (m > 0 ? 16u : 0u);
                         cM = m; cI = nI; cD = nD;
-                        G.bits[R.off + (int64_t)s0 * (C + 63) + (int64_t)t * rows + lane] = (uint8_t)dec;
+                        G.bits[R.off + (int64_t)s0 * (C + 63) + (int64_t)(t - 2 * lane + rows - 1) * rows + lane] =
+                            (uint8_t)dec;
                         if (i == R.r1) G.border[R.boff + jj] = make_int4(m, nI, nD, 0);
                         if (jj == C - 1) G.border[R.boff + C + (i - R.r0)] = make_int4(m, nI, nD, 0);
                         if (more && lane == rows - 1) { sTM[jj + 1] = m; sTI[jj + 1] = nI; sTD[jj + 1] = nD; }
@@ -651,11 +667,17 @@ __global__ void __launch_bounds__(256) dp_trace_kernel(DevIndex ix, DpArgs a) {
             alnTo = bi; alnEnd = bj;
         }
         i = alnTo; j = alnEnd;
+        Rc Rb;                  /* the rectangle of the previous step (valid when rbOk) */
+        bool rbOk = false;
+        Bytes16 sDec;
         while (true) {
             /* inside a rectangle of a wavefront-filled job, decisions come from its bytes */
-            Rc Rb;
-            const bool inR = !G.serial && cellRect(G, i, j, Rb);
-            const uint32_t dec = inR ? G.bits[Rb.bitIdx(i, j)] : 0u;
+            bool inR = false;
+            if (!G.serial) {
+                if (rbOk && i >= Rb.r0 && i <= Rb.r1 && j >= Rb.c0 && j <= Rb.c1) inR = true;
+                else rbOk = inR = cellRect(G, i, j, Rb);
+            }
+            const uint32_t dec = inR ? sDec.get(G.bits, Rb.bitIdx(i, j)) : 0u;
             const bool go_on = G.mode == 0 ? i > 0 : (s != OP_M || (inR ? (dec & 16u) != 0 : vM(G, i, j, T) > 0));
             if (!go_on || np >= pathCap) break;
             if (s == OP_M && G.match > 0 && i >= 0) {

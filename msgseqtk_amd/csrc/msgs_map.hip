@@ -331,7 +331,7 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     const int64_t tiles = out.tot[0], paths = out.tot[1], nWave = out.tot[2], borders = out.tot[3],
                   serialCells = out.tot[4];
     if (phase) { phase[0] = (double)nJobs; phase[1] = (double)nWave; phase[2] = since(); }
-    a.bits = c->x().bufs.get<uint8_t>("dp.bits", tiles + 1, e);
+    a.bits = c->x().bufs.get<uint8_t>("dp.bits", tiles + 16, e);   /* read in 16-byte chunks */
     a.border = c->x().bufs.get<int4>("dp.border", borders + 1, e);
     a.tile = c->x().bufs.get<int4>("dp.tile", serialCells + 1, e);
     a.cigar = c->x().bufs.get<uint32_t>("dp.cig", paths + 2 * nJobs + 1, e);

@@ -30,6 +30,8 @@ struct Lane {
     int64_t rid;
     const uint8_t* s;
     const double* P;   /* loglik prefix sums of the read (NULL: sequential gates only) */
+    double gD;         /* P[i1] - P[i0] of the running phase-1 search's push gate */
+    double gPL;        /* |P[L]| */
     int L;
     /* findAllSMEMS variables */
     int dFrom, dTo, stage;
@@ -94,6 +96,8 @@ __device__ __forceinline__ bool startSearch(const Ctx& c, Lane& l, int f, bool f
     l.cLL = ll;
     l.nCand = 0;
     l.state = fb ? ST_FB1 : ST_BF1;
+    /* the push gate's read range stays fixed during phase 1: [0, from) forward, [to, L) back */
+    if (l.P) l.gD = fb ? l.P[f] - l.P[0] : l.P[l.L] - l.P[f + 1];
     return true;
 }
 
@@ -154,17 +158,18 @@ __device__ __forceinline__ bool gateSum(const Ctx& c, const Lane& l, double x, i
  * the band (and NaN estimates from -inf terms) take the sequential sum in the
  * reference's order. */
 __device__ __forceinline__ double gateSlack(const Lane& l, double ll0) {
-    return 1e-15 * (double)(l.L + 2) * (1.0 + fabs(ll0) + fabs(l.P[l.L]));
+    return 1e-15 * (double)(l.L + 2) * (1.0 + fabs(ll0) + l.gPL);
 }
 
 /* the phase-1 push gate: logP of the candidate plus the logliks of read[i0, i1) <= T1,
  * i.e. evalue(0, to) for a forward search ([0, from)) and evalue(from, L) for a backward
- * one ([to, L)) (src/SMEM.cpp:26-40, :81-82, :136-137) */
+ * one ([to, L)) (src/SMEM.cpp:26-40, :81-82, :136-137); [i0, i1) is the running search's
+ * fixed range, whose prefix-sum difference startSearch keeps in l.gD */
 __device__ __forceinline__ bool gatePush(const Ctx& c, const Lane& l, double ll0, int i0, int i1, double lA,
                                          double lC, double lG, double lT) {
     if (!c.a.gate1) return true;
     if (l.P) {
-        const double v = ll0 + (l.P[i1] - l.P[i0]), sl = gateSlack(l, ll0);
+        const double v = ll0 + l.gD, sl = gateSlack(l, ll0);
         if (v < c.a.T1 - sl) return true;
         if (v > c.a.T1 + sl) return false;
     }
@@ -307,6 +312,7 @@ __device__ bool nextRead(const Ctx& c, Lane& l) {
         l.nOut = 0;
         const uint8_t nd = c.a.need ? c.a.need[rid] : 1;
         if (l.L > 0 && (nd & 1)) {
+            if (l.P) l.gPL = fabs(l.P[l.L]);
             l.dFrom = l.L / 2;
             l.dTo = l.dFrom + 1;
             l.stage = STG_A;
@@ -339,14 +345,27 @@ __global__ void __launch_bounds__(256, 4) smem_kernel_t(DevIndex ix, SmemArgs a)
     const int64_t minLen = a.minLen;
     const double llA = ix.ll[1], llC = ix.ll[2], llG = ix.ll[4], llT = ix.ll[8];
     uint32_t nIter = 0;
+    /* the candidate of the next phase-2 step, loaded one step ahead (valid when nxOk) */
+    int32_t nxPos = 0, nxF = 0, nxR = 0, nxS = 0, nxX = -1;
+    double nxLL = 0;
+    bool nxOk = false;
     while (true) {
         if (l.state == ST_IDLE) {
             if (!nextRead(c, l)) break;
+            nxOk = false;
         }
         ++nIter;
         /* ---- operands of this step's extension ---- */
         const int st = l.state;
         const bool phase1 = (st == ST_FB1 || st == ST_BF1);
+        const bool fwd = (st == ST_FB1 || st == ST_BF2);
+        /* the base of this step comes from the lane's own state (phase 2: the pass's
+         * boundary), so its load need not wait for the candidate's */
+        uint8_t b = 0;
+        if (!FAST) {
+            const int bp = fwd ? (phase1 ? l.cTo : l.sTo) : (phase1 ? l.cFrom : l.sFrom) - 1;
+            b = bp >= 0 ? base(l, bp) : (uint8_t)0;
+        }
         int pos = 0;
         int32_t P, Q, S, X;
         double LL;
@@ -354,12 +373,24 @@ __global__ void __launch_bounds__(256, 4) smem_kernel_t(DevIndex ix, SmemArgs a)
         if (phase1) {
             P = l.cF; Q = l.cR; S = l.cS; X = l.cX; LL = l.cLL; mFrom = l.cFrom; mTo = l.cTo;
         } else {
-            const SmemCand cd = a.cand[cidx(c, l.k)];
-            pos = cd.pos; P = cd.F; Q = cd.R; S = cd.S; LL = cd.LL;
-            X = FAST ? cd.X : -1;
+            if (nxOk) {
+                pos = nxPos; P = nxF; Q = nxR; S = nxS; LL = nxLL; X = FAST ? nxX : -1;
+            } else {
+                const SmemCand cd = a.cand[cidx(c, l.k)];
+                pos = cd.pos; P = cd.F; Q = cd.R; S = cd.S; LL = cd.LL;
+                X = FAST ? cd.X : -1;
+            }
             if (st == ST_FB2) { mFrom = l.sFrom; mTo = pos; } else { mFrom = pos; mTo = l.sTo; }
+            /* load the next step's candidate now, beside this step's occ blocks: the next
+             * entry of this pass (this step stores at index w >= k, above it), or, at the
+             * pass's last entry, the first entry of the next pass when an earlier step of
+             * this pass stored it */
+            nxOk = l.k > l.lo || l.w < l.nCand - 1;
+            if (nxOk) {
+                const SmemCand cd = a.cand[cidx(c, l.k > l.lo ? l.k - 1 : l.nCand - 1)];
+                nxPos = cd.pos; nxF = cd.F; nxR = cd.R; nxS = cd.S; nxLL = cd.LL; nxX = cd.X;
+            }
         }
-        const bool fwd = (st == ST_FB1 || st == ST_BF2);
         /* unique interval with no other candidate in play: run over matching text bytes;
          * each of those steps keeps S = 1 and so pushes, emits and drops nothing */
         if (FAST && S == 1 && (phase1 || (l.k == l.lo && l.lo == l.nCand - 1))) {
@@ -375,7 +406,7 @@ __global__ void __launch_bounds__(256, 4) smem_kernel_t(DevIndex ix, SmemArgs a)
                 else l.sTo = mTo;
             }
         }
-        const uint8_t b = fwd ? base(l, mTo) : (mFrom > 0 ? base(l, mFrom - 1) : (uint8_t)0);
+        if (FAST) b = fwd ? base(l, mTo) : (mFrom > 0 ? base(l, mFrom - 1) : (uint8_t)0);
         int32_t F0 = P, R0 = Q;
         const int32_t S0 = S, X0 = X;
         const double LL0 = LL;
@@ -408,10 +439,15 @@ __global__ void __launch_bounds__(256, 4) smem_kernel_t(DevIndex ix, SmemArgs a)
         if (phase1) {
             /* push the pre-step SMEM when its size changes (src/SMEM.cpp:77-83, :130-138) */
             const int len = fwd ? l.cTo : (l.L - l.cFrom);
+            nxOk = false;
             if (S != S0 && len >= minLen &&
                 gatePush(c, l, LL0, fwd ? 0 : l.cTo, fwd ? l.cFrom : l.L, llA, llC, llG, llT)) {
                 exact0();
-                candStore(c, l.nCand++, fwd ? l.cTo : l.cFrom, F0, R0, S0, LL0, -1);
+                const int pp = fwd ? l.cTo : l.cFrom;
+                candStore(c, l.nCand++, pp, F0, R0, S0, LL0, -1);
+                /* the last push heads the phase-2 list if phase 1 ends here */
+                nxPos = pp; nxF = F0; nxR = R0; nxS = S0; nxLL = LL0; nxX = -1;
+                nxOk = true;
             }
             if (fwd) l.cTo = mTo; else l.cFrom = mFrom;
             l.cF = P; l.cR = Q; l.cS = S; l.cLL = LL; l.cX = X;
@@ -436,6 +472,12 @@ __global__ void __launch_bounds__(256, 4) smem_kernel_t(DevIndex ix, SmemArgs a)
             if (S >= 1 && S != l.s0) {
                 l.s0 = S;
                 l.nValid++;
+                /* a store at the top of the list heads the next pass (no earlier step of this
+                 * pass stored, so nothing was loaded ahead for it) */
+                if (l.w == l.nCand - 1 && l.k == l.lo) {
+                    nxPos = pos; nxF = P; nxR = Q; nxS = S; nxLL = LL; nxX = X;
+                    nxOk = true;
+                }
                 candStore(c, l.w--, pos, P, Q, S, LL, X);
             }
             l.k--;
@@ -449,7 +491,7 @@ __global__ void __launch_bounds__(256, 4) smem_kernel_t(DevIndex ix, SmemArgs a)
                 else after = true;
             }
         }
-        if (after) driverAfter(c, l);
+        if (after) { driverAfter(c, l); nxOk = false; }
         if (l.state == ST_IDLE) a.cnt[l.rid] = l.nOut;
     }
     if (a.stats) {
@@ -541,6 +583,7 @@ __device__ bool nextReadGroup(const Ctx& c, Lane& l, int gbase, int gl) {
         l.nOut = 0;
         const uint8_t nd = c.a.need ? c.a.need[rid] : 1;
         if (l.L > 0 && (nd & 1)) {
+            if (l.P) l.gPL = fabs(l.P[l.L]);
             l.dFrom = l.L / 2;
             l.dTo = l.dFrom + 1;
             l.stage = STG_A;
