@@ -5,9 +5,12 @@
  * sector each:
  *   cnt[4]   number of A, C, G, T in BWT[0, 128*k)            (16 B)
  *   spec[4]  bit j set when BWT[128*k + j] is '\0' or N         (16 B)
- *   code[8]  2-bit codes (A=0 C=1 G=2 T=3; '\0' and N stored as 0),
- *            position j in word j/16, bits 2*(j%16)..+1          (32 B)
- * A rank query at any i touches exactly one block.  '\0' counts come from
+ *   lo[4]    low bit of the 2-bit code of position j (A=0 C=1 G=2 T=3;
+ *            '\0' and N stored as 0), bit j%32 of word j/32      (16 B)
+ *   hi[4]    high bit of the same codes                          (16 B)
+ * The four bitmaps share one layout, so a rank query masks each word once and
+ * counts C+T, G+T, T and specials with four popcounts per word.  A rank query at
+ * any i touches exactly one block.  '\0' counts come from
  * positions minus the four base counts minus N positions; N positions (absent in
  * the synthetic workloads) are kept in a sorted side array.  Counts are 32-bit, so
  * the BD text must be shorter than 2^32.
@@ -25,8 +28,29 @@
 struct __attribute__((aligned(64))) OccBlock {
     uint32_t cnt[4];
     uint32_t spec[4];
-    uint32_t code[8];
+    uint32_t lo[4];
+    uint32_t hi[4];
 };
+
+/* over positions 0..j of a block: ones of the low plane (C, T), of the high plane (G, T),
+ * of both (T), and specials */
+__device__ __forceinline__ void blockCounts(const uint4& sp, const uint4& pl, const uint4& ph, uint32_t j,
+                                            uint32_t& nLo, uint32_t& nHi, uint32_t& nT, uint32_t& nSp) {
+    const uint32_t wj = j >> 5, part = (2u << (j & 31)) - 1u;   /* bits 0 .. j%32 */
+    const uint32_t S[4] = {sp.x, sp.y, sp.z, sp.w};
+    const uint32_t Lw[4] = {pl.x, pl.y, pl.z, pl.w};
+    const uint32_t Hw[4] = {ph.x, ph.y, ph.z, ph.w};
+    nLo = 0; nHi = 0; nT = 0; nSp = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+        const uint32_t m = k < wj ? 0xffffffffu : (k == wj ? part : 0u);
+        const uint32_t a = Lw[k] & m, b = Hw[k] & m;
+        nLo += __popc(a);
+        nHi += __popc(b);
+        nT += __popc(a & b);
+        nSp += __popc(S[k] & m);
+    }
+}
 
 /* Everything a kernel needs to read the index; passed by value. */
 struct DevIndex {
@@ -91,31 +115,11 @@ __device__ __forceinline__ int64_t countN(const DevIndex& ix, int64_t i) {
 }
 
 /* rank of '\0', A, C, G, T at i from an already-loaded block */
-__device__ __forceinline__ void occFromBlock(const DevIndex& ix, const uint4& h, const uint4& sp, const uint4& c0,
-                                             const uint4& c1, int64_t i, int64_t* r) {
+__device__ __forceinline__ void occFromBlock(const DevIndex& ix, const uint4& h, const uint4& sp, const uint4& pl,
+                                             const uint4& ph, int64_t i, int64_t* r) {
     const uint32_t j = (uint32_t)(i & 127);
-    const uint32_t wj = j >> 4, bj = j & 15;
-    const uint32_t part = (2u << (2 * bj + 1)) - 1u;   /* bits 0 .. 2*bj+1 */
-    const uint32_t w[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
-    uint32_t nT = 0, nLo = 0, nHi = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < 8; ++k) {
-        uint32_t mk = k < wj ? 0xffffffffu : (k == wj ? part : 0u);
-        uint32_t x = w[k] & mk;
-        uint32_t lo = x & 0x55555555u, hi = (x >> 1) & 0x55555555u;
-        nT += __popc(lo & hi);
-        nLo += __popc(lo);
-        nHi += __popc(hi);
-    }
-    const uint32_t sw = j >> 5, sb = j & 31;
-    const uint32_t smask = (2u << sb) - 1u;
-    const uint32_t s[4] = {sp.x, sp.y, sp.z, sp.w};
-    uint32_t nSp = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < 4; ++k) {
-        uint32_t mk = k < sw ? 0xffffffffu : (k == sw ? smask : 0u);
-        nSp += __popc(s[k] & mk);
-    }
+    uint32_t nLo, nHi, nT, nSp;
+    blockCounts(sp, pl, ph, j, nLo, nHi, nT, nSp);
     const uint32_t nC = nLo - nT, nG = nHi - nT;
     const uint32_t nA = (j + 1) - nLo - nHi + nT;
     r[1] = (int64_t)h.x + nA - nSp;
@@ -173,31 +177,11 @@ __device__ __forceinline__ int64_t dBackExt(const DevIndex& ix, int64_t& p, int6
 
 /* 32-bit forms of the ranks and of backExt for the search kernels (indexes hold fewer
  * than 2^31 rows, so every rank, row and size fits; same values as the 64-bit forms) */
-__device__ __forceinline__ void occFromBlock32(const DevIndex& ix, const uint4& h, const uint4& sp, const uint4& c0,
-                                               const uint4& c1, int32_t i, int32_t* r) {
+__device__ __forceinline__ void occFromBlock32(const DevIndex& ix, const uint4& h, const uint4& sp, const uint4& pl,
+                                               const uint4& ph, int32_t i, int32_t* r) {
     const uint32_t j = (uint32_t)i & 127u;
-    const uint32_t wj = j >> 4, bj = j & 15;
-    const uint32_t part = (2u << (2 * bj + 1)) - 1u;
-    const uint32_t w[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
-    uint32_t nT = 0, nLo = 0, nHi = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < 8; ++k) {
-        uint32_t mk = k < wj ? 0xffffffffu : (k == wj ? part : 0u);
-        uint32_t x = w[k] & mk;
-        uint32_t lo = x & 0x55555555u, hi = (x >> 1) & 0x55555555u;
-        nT += __popc(lo & hi);
-        nLo += __popc(lo);
-        nHi += __popc(hi);
-    }
-    const uint32_t sw = j >> 5, sb = j & 31;
-    const uint32_t smask = (2u << sb) - 1u;
-    const uint32_t s[4] = {sp.x, sp.y, sp.z, sp.w};
-    uint32_t nSp = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < 4; ++k) {
-        uint32_t mk = k < sw ? 0xffffffffu : (k == sw ? smask : 0u);
-        nSp += __popc(s[k] & mk);
-    }
+    uint32_t nLo, nHi, nT, nSp;
+    blockCounts(sp, pl, ph, j, nLo, nHi, nT, nSp);
     const uint32_t nC = nLo - nT, nG = nHi - nT;
     const uint32_t nA = (j + 1) - nLo - nHi + nT;
     r[1] = (int32_t)(h.x + nA - nSp);
@@ -223,15 +207,19 @@ __device__ __forceinline__ int32_t dBackExt32(const DevIndex& ix, int32_t& p, in
     int32_t lo[5], hi[5];
     if (va) occFromBlock32(ix, h0, s0, c00, c01, a, lo); else { lo[0] = lo[1] = lo[2] = lo[3] = lo[4] = -1; }
     if (ve) occFromBlock32(ix, h1, s1, c10, c11, e, hi); else { hi[0] = hi[1] = hi[2] = hi[3] = hi[4] = -1; }
-    const int32_t O = lo[c];
-    const int32_t sc = hi[c] - O;
+    /* rank c and the '\0' + above-c counts by selects (no indexed arrays), C[b] from the
+     * four uniform base entries */
+    const int32_t O = c == 1 ? lo[1] : (c == 2 ? lo[2] : (c == 3 ? lo[3] : lo[4]));
+    const int32_t H = c == 1 ? hi[1] : (c == 2 ? hi[2] : (c == 3 ? hi[3] : hi[4]));
+    const int32_t sc = H - O;
     if (sc <= 0) return (s = sc);
     if (sc != s) {
-        int32_t add = hi[0] - lo[0];
-        for (int j = c + 1; j <= 4; ++j) add += hi[j] - lo[j];
+        const int32_t add = (hi[0] - lo[0]) + (c < 2 ? hi[2] - lo[2] : 0) + (c < 3 ? hi[3] - lo[3] : 0) +
+                            (c < 4 ? hi[4] - lo[4] : 0);
         q = q + add;
     }
-    p = (int32_t)ix.C[b] + O;
+    const int32_t Cb = (int32_t)(c == 1 ? ix.C[1] : (c == 2 ? ix.C[2] : (c == 3 ? ix.C[4] : ix.C[8])));
+    p = Cb + O;
     s = sc;
     return s;
 }

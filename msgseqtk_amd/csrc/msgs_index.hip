@@ -83,14 +83,13 @@ __global__ void k_bwt(const uint8_t* bd, const uint32_t* sa, int64_t n, uint8_t*
     bwt[k] = s == 0 ? 0 : bd[s - 1];
 }
 
-/* one thread per 128-position block: codes, special mask, local base counts */
+/* one thread per 128-position block: code bit planes, special mask, local base counts */
 __global__ void k_occ(const uint8_t* bwt, int64_t n, int64_t nBlk, OccBlock* occ, uint32_t* local) {
     int64_t blk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (blk >= nBlk) return;
     OccBlock o;
     uint32_t cnt[4] = {0, 0, 0, 0};
-    for (int w = 0; w < 8; ++w) o.code[w] = 0;
-    for (int w = 0; w < 4; ++w) o.spec[w] = 0;
+    for (int w = 0; w < 4; ++w) { o.lo[w] = 0; o.hi[w] = 0; o.spec[w] = 0; }
     const int64_t base = blk * 128;
     for (int j = 0; j < 128; ++j) {
         int64_t i = base + j;
@@ -104,7 +103,11 @@ __global__ void k_occ(const uint8_t* bwt, int64_t n, int64_t nBlk, OccBlock* occ
             else if (b == 8) { c = 3; special = false; }
         }
         if (special) { if (i < n) o.spec[j >> 5] |= 1u << (j & 31); }
-        else { cnt[c]++; o.code[j >> 4] |= c << (2 * (j & 15)); }
+        else {
+            cnt[c]++;
+            o.lo[j >> 5] |= (c & 1u) << (j & 31);
+            o.hi[j >> 5] |= (c >> 1) << (j & 31);
+        }
     }
     for (int t = 0; t < 4; ++t) { o.cnt[t] = 0; local[blk * 4 + t] = cnt[t]; }
     occ[blk] = o;
