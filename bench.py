@@ -55,6 +55,8 @@ def parse():
     ap.add_argument("--group-lanes", type=int, default=-1, help="fix the form-2 lanes per read: 0, 8 or 16")
     ap.add_argument("--text-path", type=int, default=-1, help="fix the form-2 one-lane kernel's text path: 0 or 1")
     ap.add_argument("--read-filter", type=int, default=-1, help="fix the form-2 read filter: 0 or 1")
+    ap.add_argument("--split-lanes", type=int, default=-1,
+                    help="fix the form-2 split launch's lanes per read for likely-long reads: 0, 4, 8 or 16")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a 1 GiB device copy (counter calibration under rocprofv3 --pmc)")
     return ap.parse_args()
@@ -112,10 +114,13 @@ def main():
         m.set("text_path", a.text_path)
     if a.read_filter >= 0:
         m.set("read_filter", a.read_filter)
+    if a.split_lanes >= 0:
+        m.set("split_lanes", a.split_lanes)
     if not a.no_tune and not a.lanes_per_cu:
         lanes, tus = m.tune(0, p)
         tuned = {"lanes_per_cu": lanes, "group_lanes": m.get("group_lanes"), "text_path": m.get("text_path"),
-                 "read_filter": m.get("read_filter"), "smem_us": round(tus, 1)}
+                 "read_filter": m.get("read_filter"), "split_lanes": m.get("split_lanes"),
+                 "smem_us": round(tus, 1)}
         log("autotuner: %d lanes per CU, %d lanes per read (%.0f us for the SMEM stage of batch 0)" % (
             lanes, tuned["group_lanes"], tus))
     m.set("streams", a.streams)
@@ -205,6 +210,7 @@ def main():
         if tuned:
             m.set("lanes_per_cu", tuned["lanes_per_cu"]).set("group_lanes", tuned["group_lanes"])
             m.set("text_path", tuned["text_path"]).set("read_filter", tuned["read_filter"])
+            m.set("split_lanes", tuned["split_lanes"])
     tr, tr_src = measured_traffic(out["roofline"]["kernel"])
     if tr is not None:
         out["roofline"]["traffic"] = tr
@@ -249,7 +255,10 @@ def smem_kernel_name(m, form):
     if g:
         return "smem_group_kernel<%d>" % g
     fast = form == 1 or (form == 2 and m.get("text_path") == 1)
-    return "smem_kernel_t<%s>" % ("true" if fast else "false")
+    name = "smem_kernel_t<%s>" % ("true" if fast else "false")
+    s = m.get("split_lanes") if form == 2 else 0
+    # split: a lane-group launch for the likely-long reads, then the one-lane launch
+    return "smem_group_kernel<%d> + %s" % (s, name) if s else name
 
 
 def run_steps(m, p, nb, steps, warmup, sync, log):

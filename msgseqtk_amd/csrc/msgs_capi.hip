@@ -423,21 +423,26 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
         a.need = need;
         a.exact = exact;
         int32_t* order = nullptr;
-        rc = runLixOrder(c, n, need, &order);
+        rc = runLixOrder(c, n, need, &order, &a.qSplit);
         if (rc) return rc;
         a.order = order;
         (void)hipEventRecord(c->x().ev3, c->x().stream);
     }
     const int G = c->form == 2 ? c->groupLanes : 0;
+    /* split: the reads marked to start early (sorted first by the order) go to a launch of
+     * the lane-group kernel, the others to the one-lane kernel after it */
+    const int split = (G == 0 && a.order && a.qSplit) ? c->splitLanes : 0;
     const int64_t lanesMax = (int64_t)std::max(1, c->nCU) * c->lanesPerCU;
     const int64_t want = n * std::max(G, 1);
     const int64_t lanes = std::max<int64_t>(256, (std::min<int64_t>(((want + 255) / 256) * 256, lanesMax) / 256) * 256);
+    const int64_t lanesSplit = std::max<int64_t>(256, (lanesMax / 256) * 256);
     const int64_t candCap = r.maxL + 2;
     hipError_t e = hipSuccess;
-    a.cand = c->x().bufs.get<SmemCand>("sm.cand", candCap * lanes, e);
+    a.cand = c->x().bufs.get<SmemCand>("sm.cand", candCap * std::max(lanes, split ? lanesSplit / split : 0), e);
     a.candCap = candCap;
     a.nLanes = lanes;
-    a.queue = c->x().bufs.get<unsigned long long>("sm.q", 1, e);
+    a.queue = c->x().bufs.get<unsigned long long>("sm.q", 2, e);
+    if (!split) { a.qSplit = nullptr; a.qPart = 0; }
     a.cnt = c->x().bufs.get<int32_t>("sm.cnt", n + 1, e);
     a.stats = dStats;
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "smem scratch");
@@ -447,7 +452,7 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
         a.cap = cap;
         a.out = c->x().bufs.get<SmemRec>("sm.out", (size_t)cap * std::max<int64_t>(n, 1), e);
         if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "smem output");
-        (void)hipMemsetAsync(a.queue, 0, 8, st);
+        (void)hipMemsetAsync(a.queue, 0, 16, st);
         (void)hipEventRecord(c->x().ev0, st);
         if (n > 0 && a.gate1) {
             double* pref = c->x().bufs.get<double>("sm.pref", r.total + n + 1, e);
@@ -455,7 +460,18 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
             smem_prefix_kernel<<<(unsigned)std::min<int64_t>((n + 255) / 256, 4096), 256, 0, st>>>(c->db.view(), a, pref);
             a.prefLL = pref;
         }
-        if (n > 0) {
+        if (n > 0 && split) {
+            SmemArgs a1 = a, a2 = a;
+            a1.qPart = 1;
+            a2.qPart = 2;
+            a2.queue = a.queue + 1;
+            const unsigned g1 = (unsigned)(lanesSplit / 256), g2 = (unsigned)(lanes / 256);
+            if (split == 4) smem_group_kernel<4><<<g1, 256, 0, st>>>(c->db.view(), a1);
+            else if (split == 8) smem_group_kernel<8><<<g1, 256, 0, st>>>(c->db.view(), a1);
+            else smem_group_kernel<16><<<g1, 256, 0, st>>>(c->db.view(), a1);
+            if (c->textPath) smem_kernel_t<true><<<g2, 256, 0, st>>>(c->db.view(), a2);
+            else smem_kernel_t<false><<<g2, 256, 0, st>>>(c->db.view(), a2);
+        } else if (n > 0) {
             const unsigned grid = (unsigned)(lanes / 256);
             if (G == 8) smem_group_kernel<8><<<grid, 256, 0, st>>>(c->db.view(), a);
             else if (G == 16) smem_group_kernel<16><<<grid, 256, 0, st>>>(c->db.view(), a);
@@ -629,6 +645,10 @@ extern "C" int msgs_ctx_set(msgs_ctx* c, const char* key, int64_t value) {
     else if (k == "host_threads") { if (value < 1 || value > 256) return fail(c, MSGS_E_ARG, "host_threads in [1, 256]"); c->hostThreads = (int)value; }
     else if (k == "read_filter") { if (value != 0 && value != 1) return fail(c, MSGS_E_ARG, "read_filter: 0 or 1"); c->readFilter = (int)value; }
     else if (k == "text_path") { if (value != 0 && value != 1) return fail(c, MSGS_E_ARG, "text_path: 0 or 1"); c->textPath = (int)value; }
+    else if (k == "split_lanes") {
+        if (value != 0 && value != 4 && value != 8 && value != 16) return fail(c, MSGS_E_ARG, "split_lanes: 0, 4, 8 or 16");
+        c->splitLanes = (int)value;
+    }
     else if (k == "streams") { if (value < 1 || value > kWorkExecs) return fail(c, MSGS_E_ARG, "streams in [1, 4]"); c->workStreams = (int)value; }
     else return fail(c, MSGS_E_ARG, "unknown option " + k);
     return MSGS_OK;
@@ -644,6 +664,7 @@ extern "C" int msgs_ctx_get(msgs_ctx* c, const char* key, int64_t* value) {
     else if (k == "host_threads") *value = c->hostThreads;
     else if (k == "read_filter") *value = c->readFilter;
     else if (k == "text_path") *value = c->textPath;
+    else if (k == "split_lanes") *value = c->splitLanes;
     else if (k == "streams") *value = c->workStreams;
     else return fail(c, MSGS_E_ARG, "unknown option " + k);
     return MSGS_OK;

@@ -144,6 +144,8 @@ struct msgs_ctx {
     int textPath = 1;          /* form 2 with one lane per read: follow unique intervals through the text */
     int form = 1;
     int groupLanes = 8;   /* form 2: lanes per read of the SMEM kernel (0 = one lane per read) */
+    int splitLanes = 0;   /* form 2 with one lane per read: lanes per read for the reads marked
+                           * to start early, searched by a lane-group launch first (0 = off) */
     Exec own;                      /* the caller's thread */
     Exec work[kWorkExecs];         /* submitted batches' device stages */
     int workStreams = 3;           /* how many run at once */
@@ -173,6 +175,8 @@ int uploadReads(msgs_ctx* c, const msgs_reads* r, const char* pfx, DevReads& out
 int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out, unsigned long long* dStats);
 int runLixFilter(msgs_ctx* c, const DevReads& r, int32_t minLen, double T, int32_t gate1, uint8_t** needOut,
                  SmemRec** exactOut, unsigned long long* dStats);
-int runLixOrder(msgs_ctx* c, int64_t n, const uint8_t* need, int32_t** orderOut);
+/* the search order (reads marked to start early first); *splitOut: device count of those */
+int runLixOrder(msgs_ctx* c, int64_t n, const uint8_t* need, int32_t** orderOut,
+                const unsigned long long** splitOut);
 
 #endif

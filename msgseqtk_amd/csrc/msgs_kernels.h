@@ -45,7 +45,10 @@ struct SmemArgs {
     const SmemRec* exact;      /* per read, read where need[r] == 2 */
     const int32_t* order;      /* the order in which the queue hands out reads (may be NULL) */
     const double* prefLL;      /* per read at offs[r] + r: L+1 left-to-right loglik prefix sums */
-    unsigned long long* queue; /* next read */
+    unsigned long long* queue; /* next queue entry (counted from this launch's first entry) */
+    const unsigned long long* qSplit; /* queue entries before *qSplit are the reads marked to
+                                       * start early (NULL: no split) */
+    int32_t qPart;             /* 0 = the whole queue, 1 = entries before the split, 2 = from it */
     unsigned long long* stats; /* [0] occ-block extensions, [2] text steps, [3] SA/ISA lookups,
                                 * [5] lane iterations, [6] wavefront iterations, [7] max of [6] */
 };

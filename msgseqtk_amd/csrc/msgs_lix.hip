@@ -954,7 +954,8 @@ int runLixFilter(msgs_ctx* c, const DevReads& r, int32_t minLen, double T, int32
     return MSGS_OK;
 }
 
-int runLixOrder(msgs_ctx* c, int64_t n, const uint8_t* need, int32_t** orderOut) {
+int runLixOrder(msgs_ctx* c, int64_t n, const uint8_t* need, int32_t** orderOut,
+                const unsigned long long** splitOut) {
     hipError_t e = hipSuccess;
     int32_t* order = c->x().bufs.get<int32_t>("lx.order", n + 1, e);
     unsigned long long* cnt = c->x().bufs.get<unsigned long long>("lx.ocnt", 4, e);
@@ -967,6 +968,7 @@ int runLixOrder(msgs_ctx* c, int64_t n, const uint8_t* need, int32_t** orderOut)
     }
     LCK(hipGetLastError(), "order");
     *orderOut = order;
+    if (splitOut) *splitOut = cnt;
     return MSGS_OK;
 }
 

@@ -58,7 +58,15 @@ struct Ctx {
     const SmemArgs& a;
     int64_t lane;
     bool grouped;   /* lane groups keep each group's list contiguous (chunks of G entries) */
+    int64_t qBase, qEnd;   /* this launch's queue entries [qBase, qEnd) */
 };
+
+/* the queue entries of a launch: all of them, or one side of the split */
+__device__ __forceinline__ void queueRange(const SmemArgs& a, int64_t& base, int64_t& end) {
+    const int64_t split = a.qSplit ? (int64_t)*a.qSplit : 0;
+    base = a.qPart == 2 ? split : 0;
+    end = a.qPart == 1 ? split : a.nReads;
+}
 
 __device__ __forceinline__ void emit(const Ctx& c, Lane& l, int from, int to, int32_t F, int32_t R, int32_t S, double ll) {
     if (l.nOut < c.a.cap) {
@@ -301,8 +309,8 @@ __device__ __forceinline__ void extendOne(const DevIndex& ix, bool fwd, int32_t&
 /* fetch reads until one has a running search; false when the queue is empty */
 __device__ bool nextRead(const Ctx& c, Lane& l) {
     while (true) {
-        int64_t rid = (int64_t)atomicAdd(c.a.queue, 1ull);
-        if (rid >= c.a.nReads) return false;
+        int64_t rid = (int64_t)atomicAdd(c.a.queue, 1ull) + c.qBase;
+        if (rid >= c.qEnd) return false;
         if (c.a.order) rid = c.a.order[rid];
         l.rid = rid;
         const int64_t off = c.a.offs[rid];
@@ -341,7 +349,8 @@ __global__ void __launch_bounds__(256, 4) smem_kernel_t(DevIndex ix, SmemArgs a)
     Lane l;
     l.nExt = 0; l.nFast = 0; l.nLook = 0;
     l.state = ST_IDLE;
-    Ctx c{ix, a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, false};
+    Ctx c{ix, a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, false, 0, 0};
+    queueRange(a, c.qBase, c.qEnd);
     const int64_t minLen = a.minLen;
     const double llA = ix.ll[1], llC = ix.ll[2], llG = ix.ll[4], llT = ix.ll[8];
     uint32_t nIter = 0;
@@ -571,9 +580,9 @@ template <int G>
 __device__ bool nextReadGroup(const Ctx& c, Lane& l, int gbase, int gl) {
     while (true) {
         int64_t rid = 0;
-        if (gl == 0) rid = (int64_t)atomicAdd(c.a.queue, 1ull);
+        if (gl == 0) rid = (int64_t)atomicAdd(c.a.queue, 1ull) + c.qBase;
         rid = shfl64(rid, gbase);
-        if (rid >= c.a.nReads) return false;
+        if (rid >= c.qEnd) return false;
         if (c.a.order) rid = c.a.order[rid];
         l.rid = rid;
         const int64_t off = c.a.offs[rid];
@@ -644,7 +653,8 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
     int rPos = 0;
     int32_t rP = 0, rQ = 0, rS = 0, rX = -1;
     double rLL = 0;
-    Ctx c{ix, a, ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G, true};
+    Ctx c{ix, a, ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G, true, 0, 0};
+    queueRange(a, c.qBase, c.qEnd);
     const int64_t minLen = a.minLen;
     const double llA = ix.ll[1], llC = ix.ll[2], llG = ix.ll[4], llT = ix.ll[8];
     uint32_t nIter = 0;
@@ -849,5 +859,6 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
     }
 }
 
+template __global__ void smem_group_kernel<4>(DevIndex, SmemArgs);
 template __global__ void smem_group_kernel<8>(DevIndex, SmemArgs);
 template __global__ void smem_group_kernel<16>(DevIndex, SmemArgs);

@@ -70,12 +70,12 @@ def test_strains_learned_index_form():
     _say("learned index", info)
     seqs, quals, rn, _ = synth.simulate_reads(chroms, 20000, 150, sub=0.009, indel=0.001, seed=122)
     exp, _ = O.orc_map(odb, O.default_params(), seqs, quals, rn, nthreads=16)
-    for form, group in ((2, 8), (2, 0), (0, 0)):
-        m.set("form", form).set("group_lanes", group)
+    for form, group, split in ((2, 8, 0), (2, 0, 0), (2, 0, 8), (0, 0, 0)):
+        m.set("form", form).set("group_lanes", group).set("split_lanes", split)
         got = m.map((seqs, quals, rn), M.default_params())
         n, detail = _diff(exp, got)
-        assert n == 0, (form, group, detail)
-    m.set("form", 1).set("group_lanes", 8)
+        assert n == 0, (form, group, split, detail)
+    m.set("form", 1).set("group_lanes", 8).set("split_lanes", 0)
 
 
 def test_pairs_with_offtarget():
@@ -88,9 +88,9 @@ def test_pairs_with_offtarget():
                                                            indel=0.0005, seed=132, offtarget=0.05)
     exp, _ = O.orc_map(odb, O.pe_params(350, 50), s1, q1, n1, mates=(s2, q2, n2), nthreads=16)
     pm = M.default_params(insMean=350, insSD=50, insMin=350 - 500, insMax=350 + 500)
-    for form in (2, 1):
-        m.set("form", form)
+    for form, split in ((2, 0), (2, 4), (1, 0)):
+        m.set("form", form).set("group_lanes", 0 if split else 8).set("split_lanes", split)
         got = m.map((s1, q1, n1), pm, mates=(s2, q2, n2))
         n, detail = _diff(exp, got)
-        assert n == 0, (form, detail)
-    m.set("form", 1)
+        assert n == 0, (form, split, detail)
+    m.set("form", 1).set("group_lanes", 8).set("split_lanes", 0)

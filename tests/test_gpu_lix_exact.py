@@ -94,13 +94,27 @@ def _reads(chroms):
     return seqs
 
 
-@pytest.mark.parametrize("group,read_filter,text_path", [(8, 1, 1), (0, 1, 1), (0, 1, 0), (16, 1, 1), (8, 0, 1)])
-def test_exact_read_path_parity(env, group, read_filter, text_path):
+@pytest.mark.parametrize("group,read_filter,text_path,split",
+                         [(8, 1, 1, 0), (0, 1, 1, 0), (0, 1, 0, 0), (16, 1, 1, 0), (8, 0, 1, 0), (0, 1, 0, 8),
+                          (0, 1, 1, 4), (0, 0, 0, 16)])
+def test_exact_read_path_parity(env, group, read_filter, text_path, split):
     """every search kernel behind the exact-read path (one lane per read with and without
-    the text path, lane groups of 8 and 16), with and without the read filter"""
+    the text path, lane groups of 8 and 16), with and without the read filter, and the
+    split launch (reads with no 16-mer at their middle, here random reads and reads with
+    a substitution there, searched by lane groups of 4, 8 or 16 first)"""
     chroms, names, odb, m = env
     seqs = _reads(chroms)
+    if split:
+        rng = np.random.default_rng(9)
+        seqs += [synth.NT[rng.integers(0, 4, size=int(L))] for L in rng.integers(20, 300, 120)]
+        for k in range(60):
+            t = int(rng.integers(0, 4))
+            f = np.array(chroms[t][5000 + 113 * k:5000 + 113 * k + 150], np.uint8)
+            for j in (60 + k % 30, 90 - k % 20):
+                f[j] = {1: 2, 2: 4, 4: 8, 8: 1}[int(f[j])]
+            seqs.append(f)
     m.set("group_lanes", group).set("read_filter", read_filter).set("text_path", text_path)
+    m.set("split_lanes", split)
     try:
         for minLen, maxE in ((17, 0.01), (16, 1.0), (20, float("inf")), (40, 1e-30), (151, 0.01)):
             p = M.default_params(minLen=minLen, maxEvalue=maxE)
@@ -112,7 +126,7 @@ def test_exact_read_path_parity(env, group, read_filter, text_path):
             assert np.array_equal(r1, r2), (minLen, maxE)
             assert np.array_equal(l1.view(np.int64), l2.view(np.int64)), (minLen, maxE)
     finally:
-        m.set("form", 1).set("group_lanes", 8).set("read_filter", 1).set("text_path", 1)
+        m.set("form", 1).set("group_lanes", 8).set("read_filter", 1).set("text_path", 1).set("split_lanes", 0)
 
 
 def test_exact_read_path_sam_and_count(env):
