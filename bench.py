@@ -22,6 +22,8 @@ sys.path.insert(0, ROOT)
 
 METRIC = "mapped reads/s on 1 MI355X, learned-index vs plain FMD, bit-exact SAM; DP GCUPS"
 HBM_PEAK_GBS = 8000.0
+# integer VALU peak of one MI355X: 256 CUs x 64 lanes x 2.4 GHz (lane operations per second)
+VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
 
 
 def log(*a):
@@ -49,8 +51,8 @@ def parse():
     ap.add_argument("--lix-segments", type=int, default=1 << 20, help="learned index: root segments")
     ap.add_argument("--lix-steps", type=int, default=1000, help="learned index: training steps")
     ap.add_argument("--no-gather", action="store_true", help="skip the random-gather ceiling measurement")
-    ap.add_argument("--depth", type=int, default=5, help="batches in flight")
-    ap.add_argument("--streams", type=int, default=3, help="batches whose device stages run at once")
+    ap.add_argument("--depth", type=int, default=8, help="batches in flight")
+    ap.add_argument("--streams", type=int, default=4, help="batches whose device stages run at once")
     ap.add_argument("--lanes-per-cu", type=int, default=0, help="fix the SMEM kernel's lanes per CU (no autotuning)")
     ap.add_argument("--group-lanes", type=int, default=-1, help="fix the form-2 lanes per read: 0, 8 or 16")
     ap.add_argument("--text-path", type=int, default=-1, help="fix the form-2 one-lane kernel's text path: 0 or 1")
@@ -173,7 +175,10 @@ def main():
             "dp_wall": S[:, 20].sum() / a.steps / 1000,
             "dp_pools_ready": S[:, 23].sum() / a.steps / 1000, "dp_records_back": S[:, 24].sum() / a.steps / 1000,
             "dp_cigar_md_back": S[:, 25].sum() / a.steps / 1000}),
-        "dp_jobs_per_step": {"all": float(S[:, 21].mean()), "wavefront_fill": float(S[:, 22].mean())},
+        "dp_jobs_per_step": {"all": float(S[:, 21].mean()), "wavefront_fill": float(S[:, 22].mean()),
+                             "serial_fill": float(S[:, 32].mean()), "serial_fill_cells": float(S[:, 33].mean()),
+                             "largest_serial_fill_cells": float(S[:, 34].max()),
+                             "full_last_row_scans": float(S[:, 35].mean())},
         "host_finish_thread_ms_per_step": {k: S[:, 28 + i].sum() / a.steps / 1000 for i, k in
                                            enumerate(("sam_text", "pairing", "mapq", "alignment_lists"))},
         "index_build_s": round(build_s, 2),
@@ -215,6 +220,20 @@ def main():
     if tr is not None:
         out["roofline"]["traffic"] = tr
         out["roofline"]["traffic_source"] = tr_src
+    hr, hr_src = profile_value("l2_hit_rate", out["roofline"]["kernel"])
+    if hr is not None:
+        out["roofline"]["l2_hit_rate"] = hr
+        out["roofline"]["l2_hit_rate_source"] = hr_src
+    # DP: GCUPS against the integer VALU peak divided by the VALU lane operations our
+    # kernels issue per cell (SQ_INSTS_VALU x 64 over dp_fill_kernel + dp_trace_kernel,
+    # from a committed rocprofv3 --pmc summary)
+    ops, ops_src = profile_value("valu_lane_ops_per_cell", "dp")
+    if ops and out["dp_gcups"]:
+        ceil = VALU_PEAK_LANE_OPS / ops / 1e9
+        out["dp_roofline"] = {"bound": "valu", "achieved": out["dp_gcups"], "unit": "GCUPS",
+                              "peak": round(ceil, 1), "frac": round(out["dp_gcups"] / ceil, 4),
+                              "valu_peak_lane_ops_per_s": VALU_PEAK_LANE_OPS, "lane_ops_per_cell": ops,
+                              "source": ops_src}
     if not a.no_gather:
         # random 64-B line gathers over buffers the size of the occ table and of occ + SA + ISA:
         # the random-access ceiling the SMEM kernel's lookups are held against
@@ -240,11 +259,17 @@ def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` as recorded in a committed rocprofv3 --pmc summary
     (FETCH_SIZE and WRITE_SIZE, scaled by the same counters on a copy of known size):
     a line 'traffic_bytes_per_launch[<kernel>]: <bytes>' in profiles/*.md"""
+    return profile_value("traffic_bytes_per_launch", kernel)
+
+
+def profile_value(key, kernel):
+    """A figure recorded in a committed rocprofv3 summary as a line '<key>[<kernel>]: <value>'
+    in profiles/*.md; (value, file) or (None, None)"""
     import glob
     import re
     for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
         for line in open(f):
-            mt = re.match(r"traffic_bytes_per_launch\[(.+?)\]:\s*([0-9.eE+]+)", line.strip())
+            mt = re.match(re.escape(key) + r"\[(.+?)\]:\s*([0-9.eE+-]+)", line.strip())
             if mt and mt.group(1) == kernel:
                 return float(mt.group(2)), os.path.relpath(f, ROOT)
     return None, None

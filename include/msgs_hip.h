@@ -74,7 +74,7 @@ const char* msgs_last_error(msgs_ctx* ctx);
  *   searched by a lane-group launch before the one-lane launch takes the others;
  * "lanes_per_cu" persistent lanes of the SMEM kernel per CU; "smem_cap" SMEM slots per
  * read before a regrow; "host_threads" threads of the host finishing step; "streams" how
- * many submitted batches run their device stages at once (1..4, default 3).  Results
+ * many submitted batches run their device stages at once (1..4, default 4).  Results
  * never depend on these settings. */
 int msgs_ctx_set(msgs_ctx* ctx, const char* key, int64_t value);
 int msgs_ctx_get(msgs_ctx* ctx, const char* key, int64_t* value);
@@ -150,7 +150,9 @@ int msgs_align_batch(msgs_ctx* ctx, const msgs_reads* reads, const msgs_params* 
  * sizes, its records and its CIGAR/MD bytes were back on the host, [26] reads whose SMEM
  * list came from the learned index's exact-read path (form 2), [27] reads the form-2
  * filter skipped, [28]..[31] host finishing thread-microseconds (summed over threads) in
- * SAM text, PE pairing, MAPQ and alignment lists (n <= 32). */
+ * SAM text, PE pairing, MAPQ and alignment lists, [32] DP jobs filled serially by one
+ * lane, [33] their rectangle cells, [34] the largest one's cells, [35] DP jobs whose
+ * GLOBAL last-row scan visits every column (n <= 48). */
 int msgs_batch_upload(msgs_ctx* ctx, int32_t slot, const msgs_reads* reads, const msgs_reads* mates);
 int msgs_batch_run(msgs_ctx* ctx, int32_t slot, const msgs_params* p, int32_t emitSam);
 /* msgs_batch_run in two halves: submit returns at once; a worker thread of the slot runs

@@ -388,8 +388,8 @@ class Mapper:
         self._chk(lib().msgs_batch_run(self.h, slot, C.byref(params), int(emit_sam)), "msgs_batch_run")
 
     def stats(self):
-        s = np.zeros(32, np.float64)
-        self._chk(lib().msgs_batch_stats(self.h, _p(s, C.c_double), 32), "msgs_batch_stats")
+        s = np.zeros(48, np.float64)
+        self._chk(lib().msgs_batch_stats(self.h, _p(s, C.c_double), 48), "msgs_batch_stats")
         return s
 
     def sam(self):

@@ -109,7 +109,7 @@ struct Slot {
     msgs_params params{};
     std::vector<std::string> parts;
     std::vector<int64_t> lineOffs;
-    double stats[32] = {};
+    double stats[48] = {};
     int rc = 0;
     std::string err;
     void join() {
@@ -148,14 +148,14 @@ struct msgs_ctx {
                            * to start early, searched by a lane-group launch first (0 = off) */
     Exec own;                      /* the caller's thread */
     Exec work[kWorkExecs];         /* submitted batches' device stages */
-    int workStreams = 3;           /* how many run at once */
+    int workStreams = 4;           /* how many run at once */
     bool workBusy[kWorkExecs] = {};
     std::mutex workMu;
     std::condition_variable workCv;
     DbState db;
     LixState lix;
     Slot slots[16];
-    double stats[32] = {};
+    double stats[48] = {};
     int hostThreads = 16;
     /* the executor of the calling thread */
     Exec& x();

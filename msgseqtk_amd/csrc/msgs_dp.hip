@@ -375,6 +375,11 @@ __global__ void __launch_bounds__(256) dp_setup_kernel(DevIndex ix, DpArgs a) {
             maxCols = max(maxCols, (int32_t)cols);
         }
         g.cls = area == 0 ? CLS_NONE : ((ns <= FILL_NS && maxCols <= FILL_COLS) ? CLS_WAVE : CLS_SERIAL);
+        if (a.dstat && g.cls == CLS_SERIAL) {
+            atomicAdd(&a.dstat[0], 1ull);
+            atomicAdd(&a.dstat[1], (unsigned long long)area);
+            atomicMax(&a.dstat[2], (unsigned long long)area);
+        }
         a.geo[job] = g;
         a.jobCls[job] = g.cls;
         a.tileNeed[job] = area;
@@ -631,6 +636,7 @@ __global__ void __launch_bounds__(256) dp_trace_kernel(DevIndex ix, DpArgs a) {
                 else if (inSeedRows) cand[0] = sk.y + (L - sk.x);
             }
             if (general) {
+                if (a.dstat) atomicAdd(&a.dstat[3], 1ull);
                 for (int32_t c = 1; c <= cEnd; ++c) {
                     const int32_t m = vMk(G, L, c, T, kL);
                     if (m > best) { best = m; alnEnd = c; s = OP_M; }

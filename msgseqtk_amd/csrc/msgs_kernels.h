@@ -176,6 +176,8 @@ struct DpArgs {
     const double* qMis;        /* q / PHRED_SCALE by q */
     const double* qGapOpen;    /* q / PHRED_SCALE - gapO by q */
     double gapE_d;
+    unsigned long long* dstat; /* optional counters (NULL: none): serially filled jobs, their cells,
+                                * the largest one's cells, full last-row scans */
     int64_t* tileNeed;         /* rectangle cells */
     int64_t* pathNeed;
     int64_t* borderNeed;       /* border cells */

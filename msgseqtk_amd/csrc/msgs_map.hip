@@ -249,7 +249,8 @@ static int runChains(msgs_ctx* c, const DevReads& r, const SeedStage& sd, ChainS
 static bool integral(double x) { return std::isfinite(x) && x == std::floor(x) && std::fabs(x) <= 10000; }
 
 static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msgs_params* p, const SeedStage& sd,
-                 const ChainStage& ch, DpStage& out, double* dpMicros, double* phase = nullptr) {
+                 const ChainStage& ch, DpStage& out, double* dpMicros, double* phase = nullptr,
+                 unsigned long long* dstat = nullptr) {
     const int64_t n = r.n, nJobs = ch.nJobs;
     const auto t0 = std::chrono::steady_clock::now();
     auto since = [&t0]() { return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count(); };
@@ -279,6 +280,7 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     a.mode = p->mode;
     a.match = (int32_t)p->match; a.mismatch = (int32_t)p->mismatch; a.gapO = (int32_t)p->gapO; a.gapE = (int32_t)p->gapE;
     a.clip = p->clip; a.gapE_d = p->gapE;
+    a.dstat = dstat;
     /* quality terms of Alignment::evaluate with the host libm (src/Alignment.cpp:460-477,
      * src/QualStr.h:109-117) */
     double hq[3 * 256];
@@ -1032,7 +1034,7 @@ static int batchGpu(msgs_ctx* c, Slot& S, std::chrono::steady_clock::time_point 
         auto s3 = std::chrono::steady_clock::now();
         seedUs += us(s0, s3);
         double k = 0;
-        rc = runDp(c, S.dev, S.dQual, p, b->sd, b->ch, b->dp, &k, dpPhase);
+        rc = runDp(c, S.dev, S.dQual, p, b->sd, b->ch, b->dp, &k, dpPhase, dStats + 10);
         if (rc) return rc;
         auto s4 = std::chrono::steady_clock::now();
         dpUs += k;
@@ -1060,6 +1062,7 @@ static int batchGpu(msgs_ctx* c, Slot& S, std::chrono::steady_clock::time_point 
     for (int k = 0; k < 5; ++k) st[21 + k] = dpPhase[k];
     st[26] = (double)hs[8];
     st[27] = (double)hs[9];
+    for (int k = 0; k < 4; ++k) st[32 + k] = (double)hs[10 + k];
     return MSGS_OK;
 }
 
@@ -1139,7 +1142,7 @@ extern "C" int msgs_batch_run(msgs_ctx* c, int32_t slot, const msgs_params* p, i
 
 extern "C" int msgs_batch_stats(msgs_ctx* c, double* stats, int32_t n) {
     if (!c || !stats) return MSGS_E_ARG;
-    for (int32_t i = 0; i < n && i < 32; ++i) stats[i] = c->stats[i];
+    for (int32_t i = 0; i < n && i < 48; ++i) stats[i] = c->stats[i];
     return MSGS_OK;
 }
 
