@@ -288,17 +288,34 @@ __device__ __forceinline__ int32_t matchMax(int32_t match, int32_t ins, int32_t 
     return s;
 }
 
+/* nt16 code -> IUPAC symbol ("$ACMGRSVTWYHKDBN"), from two packed words */
 __device__ __forceinline__ char decodeBase(uint8_t b) {
-    const char* sym = "$ACMGRSVTWYHKDBN";
-    return sym[b & 15];
+    constexpr uint64_t kSym0 = (uint64_t)'$' | (uint64_t)'A' << 8 | (uint64_t)'C' << 16 | (uint64_t)'M' << 24 |
+                               (uint64_t)'G' << 32 | (uint64_t)'R' << 40 | (uint64_t)'S' << 48 | (uint64_t)'V' << 56;
+    constexpr uint64_t kSym1 = (uint64_t)'T' | (uint64_t)'W' << 8 | (uint64_t)'Y' << 16 | (uint64_t)'H' << 24 |
+                               (uint64_t)'K' << 32 | (uint64_t)'D' << 40 | (uint64_t)'B' << 48 | (uint64_t)'N' << 56;
+    const uint32_t c = b & 15u;
+    return (char)((c < 8 ? kSym0 : kSym1) >> (8 * (c & 7)));
 }
 
+/* decimal digits of v written at md[pos..], their count returned */
 __device__ __forceinline__ int putNum(char* md, int64_t pos, uint32_t v) {
-    char buf[12];
-    int n = 0;
-    do { buf[n++] = (char)('0' + v % 10); v /= 10; } while (v);
-    for (int k = 0; k < n; ++k) md[pos + k] = buf[n - 1 - k];
+    int n = 1;
+    for (uint32_t t = v; t >= 10; t /= 10) ++n;
+    for (int k = n - 1; k >= 0; --k) { md[pos + k] = (char)('0' + v % 10); v /= 10; }
     return n;
+}
+
+/* 8 bytes from p (any alignment) through the aligned words that hold them: the second
+ * word is read only when p is unaligned, and then it holds p[7], so no read goes past the
+ * aligned word of the last byte */
+__device__ __forceinline__ uint64_t load8u(const uint8_t* p) {
+    const uintptr_t a = (uintptr_t)p & ~(uintptr_t)7;
+    const uint32_t sh = 8u * (uint32_t)((uintptr_t)p & 7);
+    const uint64_t w0 = *(const uint64_t*)a;
+    if (!sh) return w0;
+    const uint64_t w1 = *(const uint64_t*)(a + 8);
+    return (w0 >> sh) | (w1 << (64 - sh));
 }
 
 __device__ __forceinline__ void makeGeo(Geo& G, const DpArgs& a, const JobGeo& g, const int4* sg, int64_t job) {
@@ -529,6 +546,14 @@ __global__ void __launch_bounds__(64) dp_fill_kernel(DevIndex ix, DpArgs a) {
 /* ------------------------------------------------------------------ trace */
 
 __global__ void __launch_bounds__(256) dp_trace_kernel(DevIndex ix, DpArgs a) {
+    /* the quality terms of evaluate, by quality value, in LDS (one read per base) */
+    __shared__ double sQM[256], sQX[256], sQG[256];
+    for (int t = threadIdx.x; t < 256; t += blockDim.x) {
+        sQM[t] = a.qMatch[t];
+        sQX[t] = a.qMis[t];
+        sQG[t] = a.qGapOpen[t];
+    }
+    __syncthreads();
     for (int64_t job = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; job < a.nJobs;
          job += (int64_t)gridDim.x * blockDim.x) {
         const JobGeo g = a.geo[job];
@@ -686,7 +711,9 @@ __global__ void __launch_bounds__(256) dp_trace_kernel(DevIndex ix, DpArgs a) {
             const uint32_t dec = inR ? sDec.get(G.bits, Rb.bitIdx(i, j)) : 0u;
             const bool go_on = G.mode == 0 ? i > 0 : (s != OP_M || (inR ? (dec & 16u) != 0 : vM(G, i, j, T) > 0));
             if (!go_on || np >= pathCap) break;
-            if (s == OP_M && G.match > 0 && i >= 0) {
+            if (!inR && s == OP_M && G.match > 0 && i >= 0) {
+                /* (a rectangle's rows are never a seed's rows, so only cells outside the
+                 * rectangles can lie on a seed diagonal) */
                 /* inside a seed diagonal the step is M with -inf I/D neighbours: cross to its second cell */
                 const int k = rowSeg(G, i);
                 if (k >= 0) {
@@ -733,17 +760,52 @@ __global__ void __launch_bounds__(256) dp_trace_kernel(DevIndex ix, DpArgs a) {
         Bytes8 sQ, sV, sT;
         for (int64_t r = cgCap - nRun; r < cgCap; ++r) {
             const uint32_t rop = cg[r] & 15u, rlen = cg[r] >> 4;
-            for (uint32_t u = 0; u < rlen; ++u) {
+            uint32_t u = 0;
+            if (rop == OP_M) {
+                /* M runs 8 bases at a time where all 8 read and target bytes are in range: the
+                 * words and table terms are loaded first, then summed in base order */
+                while (u + 8 <= rlen && qi >= 0 && qi + 8 <= L && tj >= 0 && tj + 8 <= chromLen + 1) {
+                    const uint64_t qw = load8u(q + qi), tw = load8u(tg + tj), vw = load8u(qual + qi);
+                    const uint64_t both = qw & tw;
+                    double tv[8];
+                    uint32_t eqm = 0;
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) {
+                        const uint32_t qv = (uint32_t)(vw >> (8 * k)) & 0xffu;
+                        const bool eq = ((both >> (8 * k)) & 0xffu) != 0;
+                        eqm |= (eq ? 1u : 0u) << k;
+                        tv[k] = eq ? sQM[qv] : sQX[qv];
+                    }
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) x += tv[k];
+                    if (eqm == 0xffu) {
+                        run += 8;
+                        prev = OP_EQ;
+                    } else {
+                        for (int k = 0; k < 8; ++k) {
+                            if ((eqm >> k) & 1u) { run++; prev = OP_EQ; }
+                            else {
+                                ml += putNum(md, ml, run);
+                                run = 0;
+                                md[ml++] = decodeBase((uint8_t)(tw >> (8 * k)));
+                                prev = OP_X;
+                            }
+                        }
+                    }
+                    qi += 8; tj += 8; u += 8;
+                }
+            }
+            for (; u < rlen; ++u) {
                 const bool qin = qi >= 0 && qi < L;
                 const uint8_t qb = qin ? (uint8_t)sQ.get(q + qi) : (uint8_t)0;
                 const uint8_t tbv = (tj >= 0 && tj <= chromLen) ? (uint8_t)sT.get(tg + tj) : (uint8_t)0;
                 const int qv = qin ? (int)sV.get(qual + qi) : 0;
                 uint32_t op = rop;
                 if (op == OP_M) {
-                    x += (qb & tbv) ? a.qMatch[qv] : a.qMis[qv];
+                    x += (qb & tbv) ? sQM[qv] : sQX[qv];
                     op = (qb & tbv) ? OP_EQ : OP_X;
                 } else {
-                    if (u == 0) x += a.qGapOpen[qv];   /* the first op of a run opens a gap */
+                    if (u == 0) x += sQG[qv];   /* the first op of a run opens a gap */
                     x += -a.gapE_d;
                 }
                 if (op == OP_EQ) { qi++; tj++; run++; }
