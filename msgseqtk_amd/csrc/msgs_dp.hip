@@ -42,8 +42,10 @@ namespace {
 constexpr int32_t NEG = -1073741824;
 enum : int32_t { OP_M = 0, OP_I = 1, OP_D = 2, OP_S = 4, OP_EQ = 7, OP_X = 8 };
 enum : int32_t { FWD = 1, REV = 2 };
-/* limits of the wavefront fill: seeds per job (LDS copy) and rectangle columns */
-constexpr int FILL_NS = 256;
+/* limits of the wavefront fill: seeds per job (LDS copy) and rectangle columns; with 64
+ * seeds a one-wavefront block holds 7.7 KB of LDS, so 20 blocks fit a CU (5 waves per SIMD,
+ * as many as the kernel's registers allow) */
+constexpr int FILL_NS = 64;
 constexpr int FILL_COLS = 512;
 
 __device__ __forceinline__ int32_t addn(int32_t a, int32_t b) { return a == NEG ? NEG : a + b; }
@@ -545,7 +547,9 @@ __global__ void __launch_bounds__(64) dp_fill_kernel(DevIndex ix, DpArgs a) {
 
 /* ------------------------------------------------------------------ trace */
 
-__global__ void __launch_bounds__(256) dp_trace_kernel(DevIndex ix, DpArgs a) {
+/* at least 5 waves per SIMD (at most 96 VGPRs): the kernel waits on memory, and 5 waves
+ * run it faster than 3 without spills or 6-8 with more of them */
+__global__ void __launch_bounds__(256, 5) dp_trace_kernel(DevIndex ix, DpArgs a) {
     /* the quality terms of evaluate, by quality value, in LDS (one read per base) */
     __shared__ double sQM[256], sQX[256], sQG[256];
     for (int t = threadIdx.x; t < 256; t += blockDim.x) {

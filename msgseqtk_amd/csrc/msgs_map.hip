@@ -350,7 +350,8 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     a.waveJobs = dWave; a.nWaveJobs = nWave;
     CK(hipEventRecord(c->x().ev2, st), "event");
     if (nWave) {
-        const int64_t blocks = std::min<int64_t>(nWave, (int64_t)std::max(1, c->nCU) * 16);
+        /* persistent: 20 one-wavefront blocks per CU, as many as are resident at once */
+        const int64_t blocks = std::min<int64_t>(nWave, (int64_t)std::max(1, c->nCU) * 20);
         dp_fill_kernel<<<(unsigned)blocks, 64, 0, st>>>(c->db.view(), a);
     }
     dp_trace_kernel<<<gridFor(nJobs), 256, 0, st>>>(c->db.view(), a);
