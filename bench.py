@@ -57,6 +57,7 @@ def parse():
     ap.add_argument("--group-lanes", type=int, default=-1, help="fix the form-2 lanes per read: 0, 8 or 16")
     ap.add_argument("--text-path", type=int, default=-1, help="fix the form-2 one-lane kernel's text path: 0 or 1")
     ap.add_argument("--read-filter", type=int, default=-1, help="fix the form-2 read filter: 0 or 1")
+    ap.add_argument("--pair-chains", type=int, default=-1, help="fix the one-lane kernels' paired chains: 0 or 1")
     ap.add_argument("--split-lanes", type=int, default=-1,
                     help="fix the form-2 split launch's lanes per read for likely-long reads: 0, 4, 8 or 16")
     ap.add_argument("--calib-copy", action="store_true",
@@ -118,11 +119,13 @@ def main():
         m.set("read_filter", a.read_filter)
     if a.split_lanes >= 0:
         m.set("split_lanes", a.split_lanes)
+    if a.pair_chains >= 0:
+        m.set("pair_chains", a.pair_chains)
     if not a.no_tune and not a.lanes_per_cu:
         lanes, tus = m.tune(0, p)
         tuned = {"lanes_per_cu": lanes, "group_lanes": m.get("group_lanes"), "text_path": m.get("text_path"),
                  "read_filter": m.get("read_filter"), "split_lanes": m.get("split_lanes"),
-                 "smem_us": round(tus, 1)}
+                 "pair_chains": m.get("pair_chains"), "smem_us": round(tus, 1)}
         log("autotuner: %d lanes per CU, %d lanes per read (%.0f us for the SMEM stage of batch 0)" % (
             lanes, tuned["group_lanes"], tus))
     m.set("streams", a.streams)
@@ -194,6 +197,7 @@ def main():
         lanes0 = None
         if not a.no_tune:
             lanes0, _ = m.tune(0, p)   # the plain form gets its own best setting
+        pair0 = m.get("pair_chains")
         el0, S0 = run_steps(m, p, n_batches, a.steps, a.warmup, sync, None)
         v0 = S0[:, 1].sum() / el0
         out["plain_fmd"] = {"value": round(v0, 1), "ms_per_step": round(1000 * el0 / a.steps, 3),
@@ -207,7 +211,8 @@ def main():
                                 "seeds_wall": S0[:, 18].sum() / a.steps / 1000,
                                 "chains_wall": S0[:, 19].sum() / a.steps / 1000,
                                 "dp_wall": S0[:, 20].sum() / a.steps / 1000},
-                            "occ_extensions_per_read": S0[:, 2].sum() / S0[:, 0].sum(), "lanes_per_cu": lanes0}
+                            "occ_extensions_per_read": S0[:, 2].sum() / S0[:, 0].sum(), "lanes_per_cu": lanes0,
+                            "pair_chains": pair0}
         out["speedup_vs_plain_fmd"] = round(value / v0, 3)
         out["smem_stage_speedup_vs_plain_fmd"] = round((S0[:, 5].sum() + 0.0) / max(1.0, smem_us + S[:, 15].sum()), 3)
         log("plain FMD form: %.0f mapped reads/s; learned-index form / plain = %.3f" % (v0, value / v0))
@@ -215,7 +220,7 @@ def main():
         if tuned:
             m.set("lanes_per_cu", tuned["lanes_per_cu"]).set("group_lanes", tuned["group_lanes"])
             m.set("text_path", tuned["text_path"]).set("read_filter", tuned["read_filter"])
-            m.set("split_lanes", tuned["split_lanes"])
+            m.set("split_lanes", tuned["split_lanes"]).set("pair_chains", tuned["pair_chains"])
     tr, tr_src = measured_traffic(out["roofline"]["kernel"])
     if tr is not None:
         out["roofline"]["traffic"] = tr
@@ -280,7 +285,7 @@ def smem_kernel_name(m, form):
     if g:
         return "smem_group_kernel<%d>" % g
     fast = form == 1 or (form == 2 and m.get("text_path") == 1)
-    name = "smem_kernel_t<%s>" % ("true" if fast else "false")
+    name = "smem_%s<%s>" % ("pair_kernel" if m.get("pair_chains") else "kernel_t", "true" if fast else "false")
     s = m.get("split_lanes") if form == 2 else 0
     # split: a lane-group launch for the likely-long reads, then the one-lane launch
     return "smem_group_kernel<%d> + %s" % (s, name) if s else name

@@ -72,6 +72,9 @@ const char* msgs_last_error(msgs_ctx* ctx);
  * "split_lanes" form 2 with one lane per read: 0 = off, or 4 / 8 / 16 lanes per read for the
  *   reads the learned index marks as likely long searches (no 16-mer at the read's middle),
  *   searched by a lane-group launch before the one-lane launch takes the others;
+ * "pair_chains" one lane per read (any form): 1 = two lanes per read, the first search and
+ *   then the forward searches of SMEM_LIST::findAllSMEMS on one, its backward searches on
+ *   the other, the two lists merged in the reference's order (default 0);
  * "lanes_per_cu" persistent lanes of the SMEM kernel per CU; "smem_cap" SMEM slots per
  * read before a regrow; "host_threads" threads of the host finishing step; "streams" how
  * many submitted batches run their device stages at once (1..4, default 4).  Results
@@ -168,8 +171,8 @@ int msgs_batch_sam(msgs_ctx* ctx, char* out, int64_t cap, int64_t* needed);
  * with each lanes-per-CU value in cand[0..nCand) (NULL: 128..1024), in form 2 for each
  * search kernel too ("group_lanes" 8 / 16, or one lane per read with "text_path" 1 / 0)
  * with and without the "read_filter", and one lane per read with "split_lanes" 4 / 8 / 16,
- * keeps the fastest in the context and reports the lanes per CU; results never depend on
- * the choice. */
+ * with "pair_chains" 0 / 1 in every form, keeps the fastest in the context and reports the
+ * lanes per CU; results never depend on the choice. */
 int msgs_tune(msgs_ctx* ctx, int32_t slot, const msgs_params* p, const int32_t* cand, int32_t nCand, int32_t* best,
               double* bestMicros);
 

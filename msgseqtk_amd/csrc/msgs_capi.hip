@@ -432,8 +432,10 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
     /* split: the reads marked to start early (sorted first by the order) go to a launch of
      * the lane-group kernel, the others to the one-lane kernel after it */
     const int split = (G == 0 && a.order && a.qSplit) ? c->splitLanes : 0;
+    /* paired chains: the one-lane kernel with two lanes per read */
+    const bool pair = G == 0 && !split && c->pairChains;
     const int64_t lanesMax = (int64_t)std::max(1, c->nCU) * c->lanesPerCU;
-    const int64_t want = n * std::max(G, 1);
+    const int64_t want = n * std::max(G, 1) * (pair ? 2 : 1);
     const int64_t lanes = std::max<int64_t>(256, (std::min<int64_t>(((want + 255) / 256) * 256, lanesMax) / 256) * 256);
     const int64_t lanesSplit = std::max<int64_t>(256, (lanesMax / 256) * 256);
     const int64_t candCap = r.maxL + 2;
@@ -471,6 +473,21 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
             else smem_group_kernel<16><<<g1, 256, 0, st>>>(c->db.view(), a1);
             if (c->textPath) smem_kernel_t<true><<<g2, 256, 0, st>>>(c->db.view(), a2);
             else smem_kernel_t<false><<<g2, 256, 0, st>>>(c->db.view(), a2);
+        } else if (n > 0 && pair) {
+            /* each lane of a pair writes its own list; the merge puts them in order */
+            SmemArgs k = a;
+            k.out = c->x().bufs.get<SmemRec>("sm.outA", (size_t)cap * n, e);
+            k.outB = c->x().bufs.get<SmemRec>("sm.outB", (size_t)cap * n, e);
+            k.tagA = c->x().bufs.get<int32_t>("sm.tagA", (size_t)cap * n, e);
+            k.tagB = c->x().bufs.get<int32_t>("sm.tagB", (size_t)cap * n, e);
+            k.cnt = c->x().bufs.get<int32_t>("sm.cntA", n + 1, e);
+            k.cntB = c->x().bufs.get<int32_t>("sm.cntB", n + 1, e);
+            if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "smem pair lists");
+            (void)hipMemsetAsync(k.cntB, 0, (n + 1) * sizeof(int32_t), st);
+            const unsigned grid = (unsigned)(lanes / 256);
+            if (c->form == 0 || (c->form == 2 && !c->textPath)) smem_pair_kernel<false><<<grid, 256, 0, st>>>(c->db.view(), k);
+            else smem_pair_kernel<true><<<grid, 256, 0, st>>>(c->db.view(), k);
+            smem_merge_kernel<<<(unsigned)std::min<int64_t>((n + 255) / 256, 65535), 256, 0, st>>>(k, a.out, a.cnt);
         } else if (n > 0) {
             const unsigned grid = (unsigned)(lanes / 256);
             if (G == 8) smem_group_kernel<8><<<grid, 256, 0, st>>>(c->db.view(), a);
@@ -649,6 +666,7 @@ extern "C" int msgs_ctx_set(msgs_ctx* c, const char* key, int64_t value) {
         if (value != 0 && value != 4 && value != 8 && value != 16) return fail(c, MSGS_E_ARG, "split_lanes: 0, 4, 8 or 16");
         c->splitLanes = (int)value;
     }
+    else if (k == "pair_chains") { if (value != 0 && value != 1) return fail(c, MSGS_E_ARG, "pair_chains: 0 or 1"); c->pairChains = (int)value; }
     else if (k == "streams") { if (value < 1 || value > kWorkExecs) return fail(c, MSGS_E_ARG, "streams in [1, 4]"); c->workStreams = (int)value; }
     else return fail(c, MSGS_E_ARG, "unknown option " + k);
     return MSGS_OK;
@@ -665,6 +683,7 @@ extern "C" int msgs_ctx_get(msgs_ctx* c, const char* key, int64_t* value) {
     else if (k == "read_filter") *value = c->readFilter;
     else if (k == "text_path") *value = c->textPath;
     else if (k == "split_lanes") *value = c->splitLanes;
+    else if (k == "pair_chains") *value = c->pairChains;
     else if (k == "streams") *value = c->workStreams;
     else return fail(c, MSGS_E_ARG, "unknown option " + k);
     return MSGS_OK;

@@ -146,6 +146,8 @@ struct msgs_ctx {
     int groupLanes = 8;   /* form 2: lanes per read of the SMEM kernel (0 = one lane per read) */
     int splitLanes = 0;   /* form 2 with one lane per read: lanes per read for the reads marked
                            * to start early, searched by a lane-group launch first (0 = off) */
+    int pairChains = 0;   /* one-lane search kernel: 1 = two lanes per read (forward and backward
+                           * searches of findAllSMEMS on separate lanes, smem_pair_kernel) */
     Exec own;                      /* the caller's thread */
     Exec work[kWorkExecs];         /* submitted batches' device stages */
     int workStreams = 4;           /* how many run at once */

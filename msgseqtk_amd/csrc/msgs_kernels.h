@@ -51,9 +51,21 @@ struct SmemArgs {
     int32_t qPart;             /* 0 = the whole queue, 1 = entries before the split, 2 = from it */
     unsigned long long* stats; /* [0] occ-block extensions, [2] text steps, [3] SA/ISA lookups,
                                 * [5] lane iterations, [6] wavefront iterations, [7] max of [6] */
+    /* paired chains (smem_pair_kernel): the even lane of a pair writes out / cnt / tagA, the odd
+     * lane outB / cntB / tagB; tags number the search of each SMEM (0 = the first search,
+     * k = the k-th forward or backward search); smem_merge_kernel interleaves them */
+    SmemRec* outB;
+    int32_t* cntB;
+    int32_t* tagA;
+    int32_t* tagB;
 };
 
 template <bool FAST> __global__ void smem_kernel_t(DevIndex ix, SmemArgs a);
+/* the same search with two lanes per read: after the first search from the midpoint, the
+ * forward searches run on one lane and the backward searches on the other */
+template <bool FAST> __global__ void smem_pair_kernel(DevIndex ix, SmemArgs a);
+/* the paired lists of each read in the reference's order, into out / cnt */
+__global__ void smem_merge_kernel(SmemArgs a, SmemRec* out, int32_t* cnt);
 /* the same search with one read per group of G lanes (same output) */
 template <int G> __global__ void smem_group_kernel(DevIndex ix, SmemArgs a);
 /* per-read loglik prefix sums (SmemArgs::prefLL) */

@@ -1175,22 +1175,27 @@ extern "C" int msgs_tune(msgs_ctx* c, int32_t slot, const msgs_params* p, const 
     for (int32_t i = 0; i < nCand; ++i)
         if (cand[i] < 64 || cand[i] > 4096 || cand[i] % 64) return fail(c, MSGS_E_ARG, "lanes per CU");
     const int keep = c->lanesPerCU, keepG = c->groupLanes, keepT = c->textPath, keepF = c->readFilter;
-    const int keepS = c->splitLanes;
+    const int keepS = c->splitLanes, keepP = c->pairChains;
     /* form 2 search front ends and kernels: (lanes per read, text path, read filter, lanes
-     * per read of the split launch for the reads marked to start early) */
-    static const int kKernels[][4] = {{8, 1, 0, 0}, {16, 1, 0, 0}, {0, 1, 0, 0}, {0, 0, 0, 0}, {8, 1, 1, 0},
-                                      {0, 0, 1, 0}, {0, 0, 0, 4}, {0, 0, 0, 8}, {0, 0, 0, 16}};
-    const int nK = c->form == 2 ? 9 : 1;
+     * per read of the split launch for the reads marked to start early, paired chains) */
+    static const int kKernels[][5] = {{8, 1, 0, 0, 0}, {16, 1, 0, 0, 0}, {0, 1, 0, 0, 0}, {0, 0, 0, 0, 0},
+                                      {8, 1, 1, 0, 0}, {0, 0, 1, 0, 0}, {0, 0, 0, 4, 0}, {0, 0, 0, 8, 0},
+                                      {0, 0, 0, 16, 0}, {0, 0, 0, 0, 1}, {0, 1, 0, 0, 1}};
+    /* the other forms: one lane per read, unpaired or paired */
+    const int nK = c->form == 2 ? 11 : 2;
     int32_t bestL = keep;
-    int bestG = keepG, bestT = keepT, bestF = keepF, bestS = keepS;
+    int bestG = keepG, bestT = keepT, bestF = keepF, bestS = keepS, bestP = keepP;
     double bestUs = 1e300;
     auto restore = [&]() {
         c->lanesPerCU = keep; c->groupLanes = keepG; c->textPath = keepT; c->readFilter = keepF; c->splitLanes = keepS;
+        c->pairChains = keepP;
     };
     for (int ki = 0; ki < nK; ++ki) {
         if (c->form == 2) {
             c->groupLanes = kKernels[ki][0]; c->textPath = kKernels[ki][1]; c->readFilter = kKernels[ki][2];
-            c->splitLanes = kKernels[ki][3];
+            c->splitLanes = kKernels[ki][3]; c->pairChains = kKernels[ki][4];
+        } else {
+            c->pairChains = ki;
         }
         for (int32_t i = 0; i < nCand; ++i) {
             c->lanesPerCU = cand[i];
@@ -1203,7 +1208,7 @@ extern "C" int msgs_tune(msgs_ctx* c, int32_t slot, const msgs_params* p, const 
             }
             if (t < bestUs) {
                 bestUs = t; bestL = cand[i]; bestG = c->groupLanes; bestT = c->textPath; bestF = c->readFilter;
-                bestS = c->splitLanes;
+                bestS = c->splitLanes; bestP = c->pairChains;
             }
         }
     }
@@ -1212,6 +1217,7 @@ extern "C" int msgs_tune(msgs_ctx* c, int32_t slot, const msgs_params* p, const 
     c->textPath = bestT;
     c->readFilter = bestF;
     c->splitLanes = bestS;
+    c->pairChains = bestP;
     if (best) *best = bestL;
     if (bestMicros) *bestMicros = bestUs;
     return MSGS_OK;

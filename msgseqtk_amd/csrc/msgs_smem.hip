@@ -22,7 +22,8 @@
 
 namespace {
 
-enum : int { ST_IDLE = 0, ST_FB1 = 1, ST_FB2 = 2, ST_BF1 = 3, ST_BF2 = 4 };
+enum : int { ST_IDLE = 0, ST_FB1 = 1, ST_FB2 = 2, ST_BF1 = 3, ST_BF2 = 4, ST_WAIT = 5 };
+constexpr int kNoHand = -2147483647 - 1;
 enum : int { STG_A = 0, STG_C = 1, STG_D = 2 };
 
 struct Lane {
@@ -47,6 +48,10 @@ struct Lane {
     int32_t s0;
     /* output */
     int nOut;
+    /* paired chains: role 1 = the pair's even lane (first search, then forward searches),
+     * 2 = its odd lane (backward searches), 0 = unpaired; tag = the running search's number;
+     * hand = the first search's final `from`, passed to the odd lane (kNoHand: none) */
+    int role, tag, hand;
     /* counters */
     int64_t nExt, nFast, nLook;
 };
@@ -70,8 +75,10 @@ __device__ __forceinline__ void queueRange(const SmemArgs& a, int64_t& base, int
 
 __device__ __forceinline__ void emit(const Ctx& c, Lane& l, int from, int to, int32_t F, int32_t R, int32_t S, double ll) {
     if (l.nOut < c.a.cap) {
-        SmemRec* o = c.a.out + (size_t)l.rid * c.a.cap + l.nOut;
+        const size_t k = (size_t)l.rid * c.a.cap + l.nOut;
+        SmemRec* o = (l.role == 2 ? c.a.outB : c.a.out) + k;
         o->from = from; o->to = to; o->fwd = F; o->rev = R; o->size = S; o->logP = ll;
+        if (l.role) (l.role == 2 ? c.a.tagB : c.a.tagA)[k] = l.tag;
     }
     l.nOut++;
 }
@@ -109,8 +116,41 @@ __device__ __forceinline__ bool startSearch(const Ctx& c, Lane& l, int f, bool f
     return true;
 }
 
-/* findAllSMEMS control flow after a search returned; ends with a running search or an idle lane */
+/* the backward searches of findAllSMEMS alone (a pair's odd lane): while from > 0, search
+ * back from --from; a search that ends at once leaves from unchanged */
+__device__ void pairBackChain(const Ctx& c, Lane& l) {
+    while (l.dFrom > 0) {
+        --l.dFrom;
+        l.tag++;
+        l.stage = STG_D;
+        if (startSearch(c, l, l.dFrom, false)) return;
+    }
+    l.state = ST_IDLE;
+}
+
+/* findAllSMEMS control flow after a search returned; ends with a running search or an idle lane.
+ * After the first search, the loop's forward searches depend only on `to` and its backward
+ * searches only on `from` (src/SMEM.cpp:196-203: from0 and to0 are copies), and the k-th of
+ * each runs in the loop's k-th iteration; a pair runs the two chains on its two lanes and tags
+ * each SMEM with k, so that the merge restores the loop's order. */
 __device__ void driverAfter(const Ctx& c, Lane& l) {
+    if (l.role == 1) {
+        if (l.stage == STG_A) { l.dFrom = l.sFrom; l.dTo = l.sTo; l.hand = l.dFrom; }
+        else l.dTo = l.sTo;
+        while (l.dTo < l.L) {
+            l.tag++;
+            l.stage = STG_C;
+            if (startSearch(c, l, l.dTo, true)) return;
+            l.dTo = l.sTo;   /* ended at once: to = from0 + 1 */
+        }
+        l.state = ST_IDLE;
+        return;
+    }
+    if (l.role == 2) {
+        l.dFrom = l.sFrom;
+        pairBackChain(c, l);
+        return;
+    }
     int pc;  /* 0 = loop check, 1 = back-search preparation */
     if (l.stage == STG_A) { l.dFrom = l.sFrom; l.dTo = l.sTo; pc = 0; }
     else if (l.stage == STG_C) { l.dTo = l.sTo; pc = 1; }
@@ -318,6 +358,8 @@ __device__ bool nextRead(const Ctx& c, Lane& l) {
         l.L = (int)(c.a.offs[rid + 1] - off);
         l.P = c.a.prefLL ? c.a.prefLL + off + rid : nullptr;
         l.nOut = 0;
+        l.tag = 0;
+        l.hand = kNoHand;
         const uint8_t nd = c.a.need ? c.a.need[rid] : 1;
         if (l.L > 0 && (nd & 1)) {
             if (l.P) l.gPL = fabs(l.P[l.L]);
@@ -326,11 +368,33 @@ __device__ bool nextRead(const Ctx& c, Lane& l) {
             l.stage = STG_A;
             if (startSearch(c, l, l.dFrom, true)) return true;
             driverAfter(c, l);
+            if (l.role == 1) {
+                /* the odd lane takes the read's backward searches whatever is left here */
+                if (l.state == ST_IDLE) c.a.cnt[rid] = l.nOut;
+                return true;
+            }
             if (l.state != ST_IDLE) return true;
         }
-        if ((nd & 3) == 2) { c.a.out[rid * c.a.cap] = c.a.exact[rid]; l.nOut = 1; }
+        if ((nd & 3) == 2) {
+            c.a.out[rid * c.a.cap] = c.a.exact[rid];
+            if (l.role) c.a.tagA[rid * c.a.cap] = 0;
+            l.nOut = 1;
+        }
         c.a.cnt[rid] = l.nOut;
     }
+}
+
+/* a pair's odd lane takes read rid (fetched by its even lane) and waits for the hand-over */
+__device__ void pairTake(const Ctx& c, Lane& l, int64_t rid) {
+    l.rid = rid;
+    const int64_t off = c.a.offs[rid];
+    l.s = c.a.seqs + off;
+    l.L = (int)(c.a.offs[rid + 1] - off);
+    l.P = c.a.prefLL ? c.a.prefLL + off + rid : nullptr;
+    if (l.P) l.gPL = fabs(l.P[l.L]);
+    l.nOut = 0;
+    l.tag = 0;
+    l.state = ST_WAIT;
 }
 
 }  // namespace
@@ -344,12 +408,15 @@ __device__ bool nextRead(const Ctx& c, Lane& l) {
  * back into a row (ISA[X]) only when the SMEM is pushed or emitted.  Values are
  * identical to the occ-block path; only the memory traffic differs.
  */
-template <bool FAST>
-__global__ void __launch_bounds__(256, 4) smem_kernel_t(DevIndex ix, SmemArgs a) {
+template <bool FAST, bool PAIR>
+__device__ __forceinline__ void smemBody(const DevIndex& ix, const SmemArgs& a) {
     Lane l;
     l.nExt = 0; l.nFast = 0; l.nLook = 0;
     l.state = ST_IDLE;
+    l.tag = 0;
+    l.hand = kNoHand;
     Ctx c{ix, a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, false, 0, 0};
+    l.role = PAIR ? ((c.lane & 1) ? 2 : 1) : 0;
     queueRange(a, c.qBase, c.qEnd);
     const int64_t minLen = a.minLen;
     const double llA = ix.ll[1], llC = ix.ll[2], llG = ix.ll[4], llT = ix.ll[8];
@@ -359,7 +426,33 @@ __global__ void __launch_bounds__(256, 4) smem_kernel_t(DevIndex ix, SmemArgs a)
     double nxLL = 0;
     bool nxOk = false;
     while (true) {
-        if (l.state == ST_IDLE) {
+        if (PAIR) {
+            /* both lanes of a pair run this exchange in every iteration: the even lane fetches
+             * when both are free, then passes the read and the first search's `from` across */
+            const int free = l.state == ST_IDLE ? 1 : 0;
+            const int pFree = __shfl_xor(free, 1, 64);
+            int got = -2;   /* -2: no fetch, -1: queue empty, else the fetched read */
+            if (l.role == 1 && free && pFree) {
+                got = nextRead(c, l) ? (int)l.rid : -1;
+                nxOk = false;
+            }
+            const int gotIn = __shfl_xor(got, 1, 64);
+            const int handIn = __shfl_xor(l.hand, 1, 64);
+            if (l.role == 1) {
+                if (got == -1) break;
+                l.hand = kNoHand;
+            } else {
+                if (gotIn == -1) break;
+                if (gotIn >= 0) pairTake(c, l, gotIn);
+                if (l.state == ST_WAIT && handIn != kNoHand) {
+                    l.dFrom = handIn;
+                    pairBackChain(c, l);
+                    nxOk = false;
+                    if (l.state == ST_IDLE) a.cntB[l.rid] = l.nOut;
+                }
+            }
+            if (l.state == ST_IDLE || l.state == ST_WAIT) continue;
+        } else if (l.state == ST_IDLE) {
             if (!nextRead(c, l)) break;
             nxOk = false;
         }
@@ -501,7 +594,7 @@ __global__ void __launch_bounds__(256, 4) smem_kernel_t(DevIndex ix, SmemArgs a)
             }
         }
         if (after) { driverAfter(c, l); nxOk = false; }
-        if (l.state == ST_IDLE) a.cnt[l.rid] = l.nOut;
+        if (l.state == ST_IDLE) (l.role == 2 ? a.cntB : a.cnt)[l.rid] = l.nOut;
     }
     if (a.stats) {
         atomicAdd(&a.stats[0], (unsigned long long)l.nExt);
@@ -519,8 +612,40 @@ __global__ void __launch_bounds__(256, 4) smem_kernel_t(DevIndex ix, SmemArgs a)
 }
 
 
+template <bool FAST>
+__global__ void __launch_bounds__(256, 4) smem_kernel_t(DevIndex ix, SmemArgs a) {
+    smemBody<FAST, false>(ix, a);
+}
+
+template <bool FAST>
+__global__ void __launch_bounds__(256, 4) smem_pair_kernel(DevIndex ix, SmemArgs a) {
+    smemBody<FAST, true>(ix, a);
+}
+
 template __global__ void smem_kernel_t<false>(DevIndex, SmemArgs);
 template __global__ void smem_kernel_t<true>(DevIndex, SmemArgs);
+template __global__ void smem_pair_kernel<false>(DevIndex, SmemArgs);
+template __global__ void smem_pair_kernel<true>(DevIndex, SmemArgs);
+
+/* one read per lane: the even lane's list (tags non-decreasing) and the odd lane's list (tags
+ * from 1, non-decreasing) merged by tag, the even lane's SMEMs first on equal tags: the first
+ * search, then the k-th forward and the k-th backward search of findAllSMEMS's loop
+ * (src/SMEM.cpp:190-203) */
+__global__ void __launch_bounds__(256) smem_merge_kernel(SmemArgs a, SmemRec* out, int32_t* cnt) {
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.nReads;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int32_t nA = a.cnt[r], nB = a.cntB[r];
+        const int32_t total = nA + nB;
+        cnt[r] = total;
+        if (total > a.cap) continue;   /* the host grows the slots and searches again */
+        const size_t base = (size_t)r * a.cap;
+        int32_t i = 0, j = 0, o = 0;
+        while (i < nA || j < nB) {
+            const bool takeA = j >= nB || (i < nA && a.tagA[base + i] <= a.tagB[base + j]);
+            out[base + o++] = takeA ? a.out[base + i++] : a.outB[base + j++];
+        }
+    }
+}
 
 /* per read, P[0] = 0 and P[i+1] = P[i] + loglik(base i) (the gates' estimates) */
 __global__ void __launch_bounds__(256) smem_prefix_kernel(DevIndex ix, SmemArgs a, double* pref) {
@@ -648,6 +773,7 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
     Lane l;
     l.nExt = 0; l.nFast = 0; l.nLook = 0;
     l.state = ST_IDLE;
+    l.role = 0; l.tag = 0; l.hand = kNoHand;
     l.nCand = 0; l.lo = 0; l.k = 0; l.w = 0; l.s0 = -1; l.nValid = 0;
     /* this lane's list entry in register mode (l.nValid == 1) or its phase-1 push */
     int rPos = 0;

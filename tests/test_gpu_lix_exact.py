@@ -94,14 +94,15 @@ def _reads(chroms):
     return seqs
 
 
-@pytest.mark.parametrize("group,read_filter,text_path,split",
-                         [(8, 1, 1, 0), (0, 1, 1, 0), (0, 1, 0, 0), (16, 1, 1, 0), (8, 0, 1, 0), (0, 1, 0, 8),
-                          (0, 1, 1, 4), (0, 0, 0, 16)])
-def test_exact_read_path_parity(env, group, read_filter, text_path, split):
+@pytest.mark.parametrize("group,read_filter,text_path,split,pair",
+                         [(8, 1, 1, 0, 0), (0, 1, 1, 0, 0), (0, 1, 0, 0, 0), (16, 1, 1, 0, 0), (8, 0, 1, 0, 0),
+                          (0, 1, 0, 8, 0), (0, 1, 1, 4, 0), (0, 0, 0, 16, 0), (0, 0, 0, 0, 1), (0, 1, 1, 0, 1)])
+def test_exact_read_path_parity(env, group, read_filter, text_path, split, pair):
     """every search kernel behind the exact-read path (one lane per read with and without
-    the text path, lane groups of 8 and 16), with and without the read filter, and the
-    split launch (reads with no 16-mer at their middle, here random reads and reads with
-    a substitution there, searched by lane groups of 4, 8 or 16 first)"""
+    the text path, lane groups of 8 and 16, two lanes per read with paired chains), with and
+    without the read filter, and the split launch (reads with no 16-mer at their middle,
+    here random reads and reads with a substitution there, searched by lane groups of 4, 8
+    or 16 first)"""
     chroms, names, odb, m = env
     seqs = _reads(chroms)
     if split:
@@ -114,7 +115,7 @@ def test_exact_read_path_parity(env, group, read_filter, text_path, split):
                 f[j] = {1: 2, 2: 4, 4: 8, 8: 1}[int(f[j])]
             seqs.append(f)
     m.set("group_lanes", group).set("read_filter", read_filter).set("text_path", text_path)
-    m.set("split_lanes", split)
+    m.set("split_lanes", split).set("pair_chains", pair)
     try:
         for minLen, maxE in ((17, 0.01), (16, 1.0), (20, float("inf")), (40, 1e-30), (151, 0.01)):
             p = M.default_params(minLen=minLen, maxEvalue=maxE)
@@ -127,6 +128,7 @@ def test_exact_read_path_parity(env, group, read_filter, text_path, split):
             assert np.array_equal(l1.view(np.int64), l2.view(np.int64)), (minLen, maxE)
     finally:
         m.set("form", 1).set("group_lanes", 8).set("read_filter", 1).set("text_path", 1).set("split_lanes", 0)
+        m.set("pair_chains", 0)
 
 
 def test_exact_read_path_sam_and_count(env):

@@ -68,19 +68,21 @@ def test_ext_parity(dbs):
 
 def _check_smems(odb, m, seqs, minLen, maxE):
     """every device form: plain occ-block extensions (0), the unique-interval text path
-    (1), and lane groups of 8 and 16 per read (2; no learned index trained here)"""
+    (1), lane groups of 8 and 16 per read (2; no learned index trained here), and the
+    paired chains (two lanes per read) with and without the text path"""
     p = M.default_params(minLen=minLen, maxEvalue=maxE)
     c1, r1, l1 = O.orc_smems(odb, seqs, minLen, maxE)
-    for form, group in ((0, 0), (1, 0), (2, 8), (2, 16)):
-        m.set("form", form)
-        if group:
-            m.set("group_lanes", group)
-        c2, r2, l2 = m.smems(seqs, p)
-        assert np.array_equal(c1, c2), (form, group)
-        assert np.array_equal(r1, r2), (form, group)
-        assert np.array_equal(l1.view(np.int64), l2.view(np.int64)), (form, group)
-    m.set("form", 1)
-    m.set("group_lanes", 8)
+    try:
+        for form, group, pair in ((0, 0, 0), (1, 0, 0), (2, 8, 0), (2, 16, 0), (0, 0, 1), (1, 0, 1), (2, 0, 1)):
+            m.set("form", form).set("pair_chains", pair)
+            if form == 2:
+                m.set("group_lanes", group)
+            c2, r2, l2 = m.smems(seqs, p)
+            assert np.array_equal(c1, c2), (form, group, pair)
+            assert np.array_equal(r1, r2), (form, group, pair)
+            assert np.array_equal(l1.view(np.int64), l2.view(np.int64)), (form, group, pair)
+    finally:
+        m.set("form", 1).set("group_lanes", 8).set("pair_chains", 0)
 
 
 @pytest.mark.parametrize("minLen,maxE", [(17, 0.01), (10, 1.0), (3, float("inf")), (30, 1e-9), (17, 0.0)])
@@ -89,6 +91,24 @@ def test_smem_parity(dbs, minLen, maxE):
     seqs, _, _, _ = synth.simulate_reads(chroms, 400, (30, 300), sub=0.02, indel=0.003, seed=minLen,
                                          offtarget=0.05)
     _check_smems(odb, m, seqs, minLen, maxE)
+
+
+def test_smem_pair_slot_growth(dbs):
+    """paired chains with one SMEM slot per read: the merged counts exceed the slots, the
+    host grows them and searches again, and the lists still equal the oracle's"""
+    chroms, names, odb, m = dbs
+    seqs, _, _, _ = synth.simulate_reads(chroms, 300, (30, 300), sub=0.03, indel=0.003, seed=77,
+                                         offtarget=0.1)
+    p = M.default_params()
+    c1, r1, l1 = O.orc_smems(odb, seqs, 17, 0.01)
+    try:
+        m.set("form", 0).set("pair_chains", 1).set("smem_cap", 1)
+        c2, r2, l2 = m.smems(seqs, p)
+    finally:
+        m.set("form", 1).set("pair_chains", 0).set("smem_cap", 32)
+    assert c1.max() > 1
+    assert np.array_equal(c1, c2) and np.array_equal(r1, r2)
+    assert np.array_equal(l1.view(np.int64), l2.view(np.int64))
 
 
 def test_smem_edge_reads(dbs):
