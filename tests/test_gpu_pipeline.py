@@ -100,6 +100,24 @@ def test_sam_parity_pe(env):
     _cmp_sam(sam1, sam2)
 
 
+@pytest.mark.parametrize("form", [0, 1])
+def test_sam_parity_pe_paired_chains(env, form):
+    """the whole path with two lanes per read in the SMEM search (the autotuner's usual
+    choice on the benchmark workload), both one-lane forms"""
+    chroms, names, odb, m = env
+    (a, qa, na), (b, qb, nb) = synth.simulate_pairs(chroms, 400, 150, insert=(350, 50), seed=96 + form,
+                                                    offtarget=0.05)
+    po = O.pe_params(350, 50)
+    pm = M.default_params(insMean=350, insSD=50, insMin=350 - 500, insMax=350 + 500)
+    sam1, _ = O.orc_map(odb, po, a, qa, na, mates=(b, qb, nb))
+    try:
+        m.set("form", form).set("pair_chains", 1)
+        sam2 = m.map((a, qa, na), pm, mates=(b, qb, nb))
+    finally:
+        m.set("form", 1).set("pair_chains", 0)
+    _cmp_sam(sam1, sam2)
+
+
 def test_resident_batch(env):
     chroms, names, odb, m = env
     seqs, quals, rn, _ = synth.simulate_reads(chroms, 500, 150, sub=0.01, indel=0.001, seed=99)
