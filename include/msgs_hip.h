@@ -117,6 +117,18 @@ int msgs_ext_batch(msgs_ctx* ctx, int64_t* pqs, const uint8_t* bases, int32_t fo
  * Returns MSGS_E_CAP with *needed set when cap rows are not enough. */
 int msgs_smem_batch(msgs_ctx* ctx, const msgs_reads* reads, const msgs_params* p, int64_t* counts,
                     int64_t* rows5, double* logP, int64_t cap, int64_t* needed);
+/* SMEM_LIST::findMEMS (src/SMEM.cpp:174-184, SMEM::findMEM :42-62) with p->minLen and
+ * p->maxEvalue, one read per lane; rows5 / logP / counts / cap / needed as msgs_smem_batch,
+ * readLoglik[i] = SMEM_LIST::loglik (src/SMEM.cpp:307-312) of read i's list (may be NULL). */
+int msgs_mems_batch(msgs_ctx* ctx, const msgs_reads* reads, const msgs_params* p, int64_t* counts,
+                    int64_t* rows5, double* logP, double* readLoglik, int64_t cap, int64_t* needed);
+/* msgseqtk-clean's per-read decision (src/msgseqtk-clean.cpp:413-488): the loglik of the
+ * read's MEM list in the reference database `ref` and in the background database `bg`
+ * (SMEM_LIST::findMEMS / findMEMS_PE and SMEM_LIST::loglik, src/SMEM.h:373-394; with mates,
+ * the first read's list plus the mate's), and keep[i] = (-refLoglik[i] + bgLoglik[i] >
+ * minLod).  mates may be NULL (single-end); any output may be NULL. */
+int msgs_clean_batch(msgs_ctx* ref, msgs_ctx* bg, const msgs_reads* reads, const msgs_reads* mates,
+                     const msgs_params* p, double minLod, double* refLoglik, double* bgLoglik, uint8_t* keep);
 /* SMEM_LIST::findSeeds (src/SMEM.cpp:271-305) incl. SeedPair::removeRedundant
  * (src/SeedPair.cpp:71-88).  rows6 = (from, to, start, end, tid, strand). */
 int msgs_seed_batch(msgs_ctx* ctx, const msgs_reads* reads, const msgs_params* p, int64_t* counts,

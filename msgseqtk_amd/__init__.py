@@ -23,7 +23,7 @@ MSGS_E_CAP = -6
 EXPORTS = [
     "msgs_params_default", "msgs_ctx_create", "msgs_ctx_destroy", "msgs_last_error", "msgs_ctx_set",
     "msgs_db_build", "msgs_db_load", "msgs_db_nchrom", "msgs_db_chrom", "msgs_db_length", "msgs_db_counts", "msgs_db_copy", "msgs_db_loglik",
-    "msgs_rank_batch", "msgs_ext_batch", "msgs_smem_batch", "msgs_seed_batch",
+    "msgs_rank_batch", "msgs_ext_batch", "msgs_smem_batch", "msgs_mems_batch", "msgs_clean_batch", "msgs_seed_batch",
     "msgs_map_batch", "msgs_align_batch", "msgs_batch_upload", "msgs_batch_run",
     "msgs_batch_stats", "msgs_batch_sam", "msgs_util_sort_by_key",
     "msgs_batch_submit", "msgs_batch_wait", "msgs_tune", "msgs_util_calib_copy", "msgs_util_random_gather", "msgs_util_format_g6", "msgs_ctx_get", "msgs_lix_train", "msgs_lix_query", "msgs_lix_predict", "msgs_lix_grad", "msgs_lix_weights", "msgs_lix_knots",
@@ -82,6 +82,9 @@ def lib():
     L.msgs_ext_batch.argtypes = [vp, P64, PU8, C.c_int32, P64, C.c_int64]
     L.msgs_smem_batch.argtypes = [vp, C.POINTER(Reads), C.POINTER(Params), P64, P64, PD, C.c_int64, P64]
     L.msgs_seed_batch.argtypes = [vp, C.POINTER(Reads), C.POINTER(Params), P64, P64, PD, C.c_int64, P64]
+    L.msgs_mems_batch.argtypes = [vp, C.POINTER(Reads), C.POINTER(Params), P64, P64, PD, PD, C.c_int64, P64]
+    L.msgs_clean_batch.argtypes = [vp, vp, C.POINTER(Reads), C.POINTER(Reads), C.POINTER(Params), C.c_double,
+                                   PD, PD, C.POINTER(C.c_uint8)]
     L.msgs_map_batch.argtypes = [vp, C.POINTER(Reads), C.POINTER(Reads), C.POINTER(Params), C.c_char_p,
                                  C.c_int64, P64, P64]
     L.msgs_align_batch.argtypes = [vp, C.POINTER(Reads), C.POINTER(Params), P64, P64, PD, C.c_char_p,
@@ -243,6 +246,40 @@ class Mapper:
             self._chk(rc, "msgs_smem_batch")
             k = need.value
             return counts[:rb.n].copy(), rows[:k].copy(), lp[:k].copy()
+
+    def mems(self, seqs, params):
+        """SMEM_LIST::findMEMS of each read: (counts, rows5, logP, per-read list loglik)"""
+        rb = ReadBatch(seqs)
+        counts = np.zeros(max(rb.n, 1), np.int64)
+        rll = np.zeros(max(rb.n, 1), np.float64)
+        need = C.c_int64(0)
+        cap = max(16, 8 * rb.n)
+        while True:
+            rows = np.zeros((cap, 5), np.int64)
+            lp = np.zeros(cap, np.float64)
+            rc = lib().msgs_mems_batch(self.h, C.byref(rb.st), C.byref(params), _p(counts, C.c_int64),
+                                       _p(rows, C.c_int64), _p(lp, C.c_double), _p(rll, C.c_double), cap,
+                                       C.byref(need))
+            if rc == MSGS_E_CAP:
+                cap = need.value
+                continue
+            self._chk(rc, "msgs_mems_batch")
+            k = need.value
+            return counts[:rb.n].copy(), rows[:k].copy(), lp[:k].copy(), rll[:rb.n].copy()
+
+    def clean(self, background, seqs, params, min_lod, mates=None):
+        """msgseqtk-clean's decision with this mapper's database as the reference and
+        `background`'s as the background: (ref loglik, bg loglik, keep) per read or pair"""
+        rb = ReadBatch(seqs)
+        mb = ReadBatch(mates) if mates is not None else None
+        n = max(rb.n, 1)
+        rl = np.zeros(n, np.float64)
+        bl = np.zeros(n, np.float64)
+        keep = np.zeros(n, np.uint8)
+        self._chk(lib().msgs_clean_batch(self.h, background.h, C.byref(rb.st), C.byref(mb.st) if mb else None,
+                                         C.byref(params), float(min_lod), _p(rl, C.c_double), _p(bl, C.c_double),
+                                         keep.ctypes.data_as(C.POINTER(C.c_uint8))), "msgs_clean_batch")
+        return rl[:rb.n].copy(), bl[:rb.n].copy(), keep[:rb.n].astype(bool)
 
     def seeds(self, seqs, params):
         rb = ReadBatch(seqs)

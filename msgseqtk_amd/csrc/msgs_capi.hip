@@ -523,6 +523,134 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
     return fail(c, MSGS_E_HIP, "SMEM slot growth did not settle");
 }
 
+/* SMEM_LIST::findMEMS of every read (mem_kernel) and the loglik of each read's list; the
+ * records stay on the device at *recs ([n][cap]), counts and logliks come to the host */
+static int runMems(msgs_ctx* c, const DevReads& r, const msgs_params* p, int& capOut, SmemRec** recs,
+                   std::vector<int32_t>& cnt, std::vector<double>& ll) {
+    const int64_t n = r.n;
+    double T;
+    if (!evalueThreshold((double)c->db.n, p->maxEvalue, &T))
+        return fail(c, MSGS_E_UNSUPPORTED, "e-value predicate is not monotone near its threshold");
+    SmemArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.seqs = r.seq; a.offs = r.offs; a.nReads = n;
+    a.minLen = p->minLen;
+    a.T1 = T; a.T2 = T;
+    hipError_t e = hipSuccess;
+    a.cnt = c->x().bufs.get<int32_t>("mm.cnt", n + 1, e);
+    double* dLL = c->x().bufs.get<double>("mm.ll", n + 1, e);
+    if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "mem scratch");
+    hipStream_t st = c->x().stream;
+    cnt.assign(n, 0);
+    ll.assign(n, 0.0);
+    int cap = c->smemCap;
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        a.cap = cap;
+        a.out = c->x().bufs.get<SmemRec>("mm.out", (size_t)cap * std::max<int64_t>(n, 1), e);
+        if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "mem output");
+        if (n > 0) {
+            mem_kernel<<<(unsigned)std::min<int64_t>((n + 255) / 256, 65535), 256, 0, st>>>(c->db.view(), a, dLL);
+            if (hipGetLastError() != hipSuccess) return fail(c, MSGS_E_HIP, "mem launch");
+            (void)hipMemcpyAsync(cnt.data(), a.cnt, n * 4, hipMemcpyDeviceToHost, st);
+            (void)hipMemcpyAsync(ll.data(), dLL, n * 8, hipMemcpyDeviceToHost, st);
+        }
+        if (hipStreamSynchronize(st) != hipSuccess) return fail(c, MSGS_E_HIP, "mem kernel");
+        int mx = 0;
+        for (int64_t i = 0; i < n; ++i) mx = std::max(mx, cnt[i]);
+        if (mx <= cap) {
+            capOut = cap;
+            *recs = a.out;
+            return MSGS_OK;
+        }
+        cap = mx;
+    }
+    return fail(c, MSGS_E_HIP, "MEM slot growth did not settle");
+}
+
+extern "C" int msgs_mems_batch(msgs_ctx* c, const msgs_reads* reads, const msgs_params* p, int64_t* counts,
+                               int64_t* rows5, double* logP, double* readLoglik, int64_t cap, int64_t* needed) {
+    if (!c || !c->db.n) return fail(c, MSGS_E_STATE, "no database");
+    if (!reads || !p) return fail(c, MSGS_E_ARG, "bad arguments");
+    DevReads r;
+    int rc = uploadReads(c, reads, "rd", r);
+    if (rc) return rc;
+    int slots = 0;
+    SmemRec* dRecs = nullptr;
+    std::vector<int32_t> cnt;
+    std::vector<double> ll;
+    rc = runMems(c, r, p, slots, &dRecs, cnt, ll);
+    if (rc) return rc;
+    const int64_t n = r.n;
+    std::vector<SmemRec> recs((size_t)slots * n);
+    if (n) (void)hipMemcpyAsync(recs.data(), dRecs, recs.size() * sizeof(SmemRec), hipMemcpyDeviceToHost, c->x().stream);
+    if (hipStreamSynchronize(c->x().stream) != hipSuccess) return fail(c, MSGS_E_HIP, "copy");
+    int64_t total = 0;
+    for (int64_t i = 0; i < n; ++i) total += cnt[i];
+    if (needed) *needed = total;
+    if (readLoglik)
+        for (int64_t i = 0; i < n; ++i) readLoglik[i] = ll[i];
+    if (total > cap) return fail(c, MSGS_E_CAP, "row capacity");
+    int64_t k = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (counts) counts[i] = cnt[i];
+        for (int j = 0; j < cnt[i]; ++j, ++k) {
+            const SmemRec& m = recs[(size_t)i * slots + j];
+            if (rows5) {
+                int64_t* o = rows5 + 5 * k;
+                o[0] = m.from; o[1] = m.to; o[2] = m.fwd; o[3] = m.rev; o[4] = m.size;
+            }
+            if (logP) logP[k] = m.logP;
+        }
+    }
+    return MSGS_OK;
+}
+
+/* the loglik of each read's MEM list (of each pair's two lists, the first read's plus the
+ * mate's) in one database */
+static int memLoglik(msgs_ctx* c, const msgs_reads* reads, const msgs_reads* mates, const msgs_params* p,
+                     std::vector<double>& out) {
+    if (!c || !c->db.n) return fail(c, MSGS_E_STATE, "no database");
+    DevReads r;
+    int rc = uploadReads(c, reads, "rd", r);
+    if (rc) return rc;
+    int slots = 0;
+    SmemRec* dRecs = nullptr;
+    std::vector<int32_t> cnt;
+    rc = runMems(c, r, p, slots, &dRecs, cnt, out);
+    if (rc || !mates) return rc;
+    DevReads r2;
+    rc = uploadReads(c, mates, "rd2", r2);
+    if (rc) return rc;
+    std::vector<double> ll2;
+    rc = runMems(c, r2, p, slots, &dRecs, cnt, ll2);
+    if (rc) return rc;
+    for (size_t i = 0; i < out.size(); ++i) out[i] = out[i] + ll2[i];   /* SMEM_LIST::loglik(MEM_LIST_PE) */
+    return MSGS_OK;
+}
+
+extern "C" int msgs_clean_batch(msgs_ctx* ref, msgs_ctx* bg, const msgs_reads* reads, const msgs_reads* mates,
+                                const msgs_params* p, double minLod, double* refLoglik, double* bgLoglik,
+                                uint8_t* keep) {
+    if (!ref || !bg) return MSGS_E_ARG;
+    if (!reads || !p || (mates && mates->n != reads->n)) return fail(ref, MSGS_E_ARG, "bad arguments");
+    try {
+        std::vector<double> rl, bl;
+        int rc = memLoglik(ref, reads, mates, p, rl);
+        if (rc) return rc;
+        rc = memLoglik(bg, reads, mates, p, bl);
+        if (rc) return rc;
+        for (int64_t i = 0; i < reads->n; ++i) {
+            const double lod = -rl[i] + bl[i];   /* msgseqtk-clean main_SE / main_PE */
+            if (refLoglik) refLoglik[i] = rl[i];
+            if (bgLoglik) bgLoglik[i] = bl[i];
+            if (keep) keep[i] = lod > minLod ? 1 : 0;
+        }
+        return MSGS_OK;
+    } catch (const std::bad_alloc&) {
+        return fail(ref, MSGS_E_NOMEM, "host memory");
+    }
+}
+
 extern "C" int msgs_smem_batch(msgs_ctx* c, const msgs_reads* reads, const msgs_params* p, int64_t* counts,
                                int64_t* rows5, double* logP, int64_t cap, int64_t* needed) {
     if (!c || !c->db.n) return fail(c, MSGS_E_STATE, "no database");

@@ -66,6 +66,8 @@ template <bool FAST> __global__ void smem_kernel_t(DevIndex ix, SmemArgs a);
 template <bool FAST> __global__ void smem_pair_kernel(DevIndex ix, SmemArgs a);
 /* the paired lists of each read in the reference's order, into out / cnt */
 __global__ void smem_merge_kernel(SmemArgs a, SmemRec* out, int32_t* cnt);
+/* SMEM_LIST::findMEMS per read (out / cnt, a.minLen, a.T2) and the list's loglik */
+__global__ void mem_kernel(DevIndex ix, SmemArgs a, double* listLL);
 /* the same search with one read per group of G lanes (same output) */
 template <int G> __global__ void smem_group_kernel(DevIndex ix, SmemArgs a);
 /* per-read loglik prefix sums (SmemArgs::prefLL) */

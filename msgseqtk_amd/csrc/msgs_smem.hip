@@ -988,3 +988,52 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
 template __global__ void smem_group_kernel<4>(DevIndex, SmemArgs);
 template __global__ void smem_group_kernel<8>(DevIndex, SmemArgs);
 template __global__ void smem_group_kernel<16>(DevIndex, SmemArgs);
+
+/*
+ * SMEM_LIST::findMEMS (src/SMEM.cpp:174-184) with SMEM::findMEM (src/SMEM.cpp:42-62), one
+ * lane per read, and the list's SMEM_LIST::loglik (src/SMEM.cpp:307-312).
+ *
+ * findMEM extends [from, from+1) forward until the size drops to 0 and keeps the last
+ * extension that succeeded only for the next start (to = mem0.to); its backward loop then
+ * starts from the failed MEM itself (mem0 = mem), whose size is 0, so it ends after one
+ * step with mem0 unchanged: the MEM returned is the failed forward extension (size 0, end
+ * one past the last success, logP including the failing base when it lies inside the
+ * read), and from is unchanged.  An extension by any base keeps (p, q) when the new size
+ * is 0 (FMDIndex::backExt, src/FMDIndex.cpp:577-610).
+ */
+__global__ void __launch_bounds__(256) mem_kernel(DevIndex ix, SmemArgs a, double* listLL) {
+    for (int64_t rid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; rid < a.nReads;
+         rid += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t off = a.offs[rid];
+        const uint8_t* s = a.seqs + off;
+        const int L = (int)(a.offs[rid + 1] - off);
+        int nOut = 0;
+        double sum = 0;
+        for (int from = 0, to = 1; from < L; from = to + 1) {
+            const uint8_t b = s[from];
+            int32_t P = (int32_t)ix.C[b], Q = (int32_t)ix.C[comp16(b)], S = (int32_t)(ix.C[b + 1] - ix.C[b]);
+            double LL = 0;
+            LL += ix.ll[b];
+            int mTo = from + 1, to0 = mTo;
+            while (true) {
+                const uint8_t nb = mTo < L ? s[mTo] : (uint8_t)0;   /* seq[L] is the terminator */
+                extendOne(ix, true, P, Q, S, nb);
+                if (mTo < L) LL += ix.ll[nb];
+                mTo++;
+                if (S > 0) to0 = mTo;
+                else break;
+            }
+            to = to0;
+            if (mTo - from >= a.minLen && LL <= a.T2) {
+                if (nOut < a.cap) {
+                    SmemRec* o = a.out + (size_t)rid * a.cap + nOut;
+                    o->from = from; o->to = mTo; o->fwd = P; o->rev = Q; o->size = S; o->logP = LL;
+                }
+                nOut++;
+                sum += LL;
+            }
+        }
+        a.cnt[rid] = nOut;
+        listLL[rid] = sum;
+    }
+}
