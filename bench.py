@@ -41,6 +41,7 @@ def parse():
     ap.add_argument("--batch-reads", type=int, default=262144)
     ap.add_argument("--cpu-pairs", type=int, default=40000, help="pairs in the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-clean", action="store_true", help="skip the msgseqtk-clean seeding measurement")
     ap.add_argument("--no-compare", action="store_true", help="skip the plain-FMD comparison run")
     ap.add_argument("--no-tune", action="store_true", help="keep the default lanes per CU")
     ap.add_argument("--no-torch", action="store_true", help="no torch device synchronisation (profiler runs)")
@@ -253,6 +254,9 @@ def main():
         log("random 64-B gather ceiling: %s" % g)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(m, chroms, names, batches[0], p, a.cpu_pairs)
+    if not a.no_clean:
+        out["clean_seeding"] = clean_seeding(m, batches[0])
+        log("clean seeding: %s" % out["clean_seeding"])
     if a.calib_copy:
         # a plain device copy of known size (k_calib_copy) to scale FETCH_SIZE / WRITE_SIZE
         out["calib_copy_bytes"] = 1 << 30
@@ -331,6 +335,24 @@ def run_steps(m, p, nb, steps, warmup, sync, log):
     sync()
     run_steps.split = {"submit_ms_per_step": 1000 * tsub / steps, "wait_ms_per_step": 1000 * twait / steps}
     return time.time() - t, np.array(stats)
+
+
+def clean_seeding(m, batch):
+    """msgseqtk-clean's seeding (SMEM_LIST::findMEMS per read and the list's loglik,
+    msgs_mems_batch) over the first reads of the pairs of batch 0 against the same database:
+    reads/s on the host clock with the packing and upload of the reads and the copy back of
+    the MEM records included (best of 3)"""
+    import msgseqtk_amd as M
+    seqs = batch[0][0]
+    p = M.default_params()
+    best, cnt = 1e30, None
+    for _ in range(3):
+        t = time.time()
+        cnt, _, _, _ = m.mems(seqs, p)
+        best = min(best, time.time() - t)
+    return {"value": round(len(seqs) / best, 1), "unit": "reads/s", "reads": len(seqs),
+            "seconds": round(best, 4), "mems_per_read": round(float(cnt.mean()), 3),
+            "timing": "host clock, reads packed and uploaded and MEM records copied back inside"}
 
 
 def cpu_baseline(m, chroms, names, batch, p, npairs):
