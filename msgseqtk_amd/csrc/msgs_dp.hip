@@ -474,15 +474,22 @@ __global__ void __launch_bounds__(64) dp_fill_kernel(DevIndex ix, DpArgs a) {
                         const int32_t j = R.c0 + jj;
                         const uint8_t tb = sTg[jj];
                         const int32_t sc = ((qb & tb) && qb != 15 && tb != 15) ? G.match : -G.mis;
-                        int32_t m = max3(addn(dM, sc), addn(dD, sc), addn(dI, sc));
+                        /* max3(addn(dM, sc), addn(dD, sc), addn(dI, sc)): NEG is below every
+                         * value, so the largest predecessor decides and -inf stays -inf */
+                        const int32_t mx = max3(dM, dD, dI);
+                        int32_t m = mx == NEG ? NEG : mx + sc;
                         if (G.mode == 1) m = max(m, 0);
-                        const int32_t nI = max(addn(uM, G.o), addn(uI, -G.ge));
-                        const int32_t nD = max(addn(cM, G.o), addn(cD, -G.ge));
+                        /* the gap penalties are non-negative (the host refuses others), so a
+                         * step from -inf lands at or below NEG and a clamp at NEG is addn / subn:
+                         * nI = max(addn(uM, o), addn(uI, -ge)), nD likewise */
+                        const int32_t nI = max3(uM + G.o, uI - G.ge, NEG);
+                        const int32_t nD = max3(cM + G.o, cD - G.ge, NEG);
                         /* the backtrace's decisions at (i, j), from the values the recurrences
-                         * read (src/Alignment.cpp:188-205, src/Alignment.h:577-597) */
+                         * read (src/Alignment.cpp:188-205, src/Alignment.h:577-597):
+                         * subn(subn(uM, go), ge) > subn(uI, ge), and the same for D */
                         const uint32_t dec = (uint32_t)matchMax(dM, dI, dD) |
-                                             ((subn(subn(uM, G.go), G.ge) > subn(uI, G.ge)) ? 0u : 4u) |
-                                             ((subn(subn(cM, G.go), G.ge) > subn(cD, G.ge)) ? 0u : 8u) |
+                                             ((max(uM + G.o, NEG) > max(uI - G.ge, NEG)) ? 0u : 4u) |
+                                             ((max(cM + G.o, NEG) > max(cD - G.ge, NEG)) ? 0u : 8u) |
                                              (m > 0 ? 16u : 0u);
                         cM = m; cI = nI; cD = nD;
                         G.bits[R.off + (int64_t)s0 * (C + 63) + (int64_t)(t - 2 * lane + rows - 1) * rows + lane] =

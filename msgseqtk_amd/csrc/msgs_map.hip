@@ -258,6 +258,8 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     hipError_t e = hipSuccess;
     if (!integral(p->match) || !integral(p->mismatch) || !integral(p->gapO) || !integral(p->gapE))
         return fail(c, MSGS_E_UNSUPPORTED, "the device DP needs integer-valued match/mismatch/gap scores");
+    if (p->gapO < 0 || p->gapE < 0)
+        return fail(c, MSGS_E_UNSUPPORTED, "the device DP needs non-negative gap penalties");
     out.recs.clear();
     if (nJobs == 0) return MSGS_OK;
     DpArgs a;
