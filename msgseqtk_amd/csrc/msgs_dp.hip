@@ -51,6 +51,9 @@ constexpr int FILL_COLS = 512;
 __device__ __forceinline__ int32_t addn(int32_t a, int32_t b) { return a == NEG ? NEG : a + b; }
 __device__ __forceinline__ int32_t subn(int32_t a, int32_t b) { return a == NEG ? NEG : a - b; }
 __device__ __forceinline__ int32_t max3(int32_t a, int32_t b, int32_t c) { return max(a, max(b, c)); }
+/* the value of lane - 1 (lane 0 keeps its own), by a DPP whole-wavefront shift right by one
+ * lane (wave_shr:1) instead of an LDS permute; every lane of the wavefront runs it */
+__device__ __forceinline__ int32_t laneUp1(int32_t v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
 
 /* Decision bytes of a rectangle are stored strip by strip (strips of 64 rows, as the
  * fill sweeps them) and, inside a strip, diagonal by diagonal: cell (l, jj) of a strip
@@ -461,7 +464,7 @@ __global__ void __launch_bounds__(64) dp_fill_kernel(DevIndex ix, DpArgs a) {
                 const int32_t steps = C + rows - 1;
                 for (int32_t t = 0; t < steps; ++t) {
                     /* up = (i-1, j): row above, one step ago; diagonal = (i-1, j-1): two steps ago */
-                    int32_t uM = __shfl_up(cM, 1, 64), uI = __shfl_up(cI, 1, 64), uD = __shfl_up(cD, 1, 64);
+                    int32_t uM = laneUp1(cM), uI = laneUp1(cI), uD = laneUp1(cD);
                     int32_t dM = pM, dI = pI, dD = pD;
                     const int32_t jj = t - lane;
                     if (lane == 0) {
