@@ -51,13 +51,14 @@ Exec& msgs_ctx::x() { return tlsExec ? *tlsExec : own; }
 bool Exec::create(unsigned streamFlags) {
     return hipStreamCreateWithFlags(&stream, streamFlags) == hipSuccess && hipEventCreate(&ev0) == hipSuccess &&
            hipEventCreate(&ev1) == hipSuccess && hipEventCreate(&ev2) == hipSuccess &&
-           hipEventCreate(&ev3) == hipSuccess;
+           hipEventCreate(&ev3) == hipSuccess && hipEventCreate(&ev4) == hipSuccess &&
+           hipEventCreate(&ev5) == hipSuccess;
 }
 
 void Exec::destroy() {
     if (stream) (void)hipStreamSynchronize(stream);
     bufs.releaseAll();
-    for (hipEvent_t* e : {&ev0, &ev1, &ev2, &ev3})
+    for (hipEvent_t* e : {&ev0, &ev1, &ev2, &ev3, &ev4, &ev5})
         if (*e) { (void)hipEventDestroy(*e); *e = nullptr; }
     if (stream) { (void)hipStreamDestroy(stream); stream = nullptr; }
 }
@@ -462,6 +463,8 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
             smem_prefix_kernel<<<(unsigned)std::min<int64_t>((n + 255) / 256, 4096), 256, 0, st>>>(c->db.view(), a, pref);
             a.prefLL = pref;
         }
+        /* ev4 .. ev5: the search kernel(s) alone */
+        (void)hipEventRecord(c->x().ev4, st);
         if (n > 0 && split) {
             SmemArgs a1 = a, a2 = a;
             a1.qPart = 1;
@@ -487,6 +490,7 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
             const unsigned grid = (unsigned)(lanes / 256);
             if (c->form == 0 || (c->form == 2 && !c->textPath)) smem_pair_kernel<false><<<grid, 256, 0, st>>>(c->db.view(), k);
             else smem_pair_kernel<true><<<grid, 256, 0, st>>>(c->db.view(), k);
+            (void)hipEventRecord(c->x().ev5, st);
             smem_merge_kernel<<<(unsigned)std::min<int64_t>((n + 255) / 256, 65535), 256, 0, st>>>(k, a.out, a.cnt);
         } else if (n > 0) {
             const unsigned grid = (unsigned)(lanes / 256);
@@ -495,6 +499,7 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
             else if (c->form == 0 || (c->form == 2 && !c->textPath)) smem_kernel_t<false><<<grid, 256, 0, st>>>(c->db.view(), a);
             else smem_kernel_t<true><<<grid, 256, 0, st>>>(c->db.view(), a);
         }
+        if (!(n > 0 && pair)) (void)hipEventRecord(c->x().ev5, st);
         (void)hipEventRecord(c->x().ev1, st);
         if (hipGetLastError() != hipSuccess) return fail(c, MSGS_E_HIP, "smem launch");
         /* largest count */
@@ -504,9 +509,11 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
         int mx = 0;
         for (int64_t i = 0; i < n; ++i) mx = std::max(mx, cnt[i]);
         if (mx <= cap) {
-            float ms = 0;
+            float ms = 0, ks = 0;
             (void)hipEventElapsedTime(&ms, c->x().ev0, c->x().ev1);
-            c->x().lastSmemMicros = ms * 1000.0;
+            (void)hipEventElapsedTime(&ks, c->x().ev4, c->x().ev5);
+            c->x().lastStageMicros = ms * 1000.0;
+            c->x().lastSmemMicros = ks * 1000.0;
             if (a.need) {
                 float fm = 0;
                 (void)hipEventElapsedTime(&fm, c->x().ev2, c->x().ev3);

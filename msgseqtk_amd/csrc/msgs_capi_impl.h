@@ -125,9 +125,10 @@ struct Slot {
  * batches' kernels can overlap on the GPU. */
 struct Exec {
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr, ev4 = nullptr, ev5 = nullptr;
     BufPool bufs;
-    double lastSmemMicros = 0;
+    double lastSmemMicros = 0;    /* the SMEM search kernel(s) of the last SMEM stage */
+    double lastStageMicros = 0;   /* the whole SMEM stage (prefix sums, search, merge) */
     double lastFilterMicros = 0;
     std::string err;
     bool create(unsigned streamFlags);

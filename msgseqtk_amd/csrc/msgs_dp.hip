@@ -130,7 +130,7 @@ __device__ __forceinline__ int32_t initI(const Geo& G, int32_t i, int32_t j) {
 }
 
 /* largest k with bi_k <= i (-1 if none) */
-__device__ int rowSeg(const Geo& G, int32_t i) {
+__device__ __forceinline__ int rowSeg(const Geo& G, int32_t i) {
     int lo = -1, hi = G.ns - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -139,7 +139,7 @@ __device__ int rowSeg(const Geo& G, int32_t i) {
     return lo;
 }
 /* largest k with bj_k <= j (-1 if none) */
-__device__ int colSeg(const Geo& G, int32_t j) {
+__device__ __forceinline__ int colSeg(const Geo& G, int32_t j) {
     int lo = -1, hi = G.ns - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -187,7 +187,7 @@ __device__ __forceinline__ int4 rectVal(const Geo& G, const Rc& R, int32_t i, in
 }
 
 /* M at (i, j) after events 1..e; (i, j) inside the matrix, k = rowSeg(i) */
-__device__ int32_t vMk(const Geo& G, int32_t i, int32_t j, int e, int k) {
+__device__ __forceinline__ int32_t vMk(const Geo& G, int32_t i, int32_t j, int e, int k) {
     if (k < 0) {
         const Rc R = rectR5(G);
         if (i >= 1 && R.ok && e >= 1 && R.hasCol(j)) return rectVal(G, R, i, j).x;
@@ -208,7 +208,7 @@ __device__ int32_t vMk(const Geo& G, int32_t i, int32_t j, int e, int k) {
     return initM(G, i, j);
 }
 
-__device__ int32_t vIk(const Geo& G, int32_t i, int32_t j, int e, int k) {
+__device__ __forceinline__ int32_t vIk(const Geo& G, int32_t i, int32_t j, int e, int k) {
     if (k >= 0) {
         const int4 s = G.sg[k];
         if (i >= s.x + s.z) {
@@ -227,7 +227,7 @@ __device__ int32_t vIk(const Geo& G, int32_t i, int32_t j, int e, int k) {
     return initI(G, i, j);
 }
 
-__device__ int32_t vDk(const Geo& G, int32_t i, int32_t j, int e, int k) {
+__device__ __forceinline__ int32_t vDk(const Geo& G, int32_t i, int32_t j, int e, int k) {
     if (k < 0) {
         const Rc R = rectR5(G);
         if (i >= 1 && R.ok && e >= 1 && R.hasCol(j)) return rectVal(G, R, i, j).z;
@@ -255,12 +255,12 @@ __device__ __forceinline__ bool inside(Geo& G, int32_t i, int32_t j) {
     if (i < 0 || j < 0 || i > G.L || j > G.tLen) { G.hazard = 1; return false; }
     return true;
 }
-__device__ int32_t vM(Geo& G, int32_t i, int32_t j, int e) { return inside(G, i, j) ? vMk(G, i, j, e, rowSeg(G, i)) : NEG; }
-__device__ int32_t vI(Geo& G, int32_t i, int32_t j, int e) { return inside(G, i, j) ? vIk(G, i, j, e, rowSeg(G, i)) : NEG; }
-__device__ int32_t vD(Geo& G, int32_t i, int32_t j, int e) { return inside(G, i, j) ? vDk(G, i, j, e, rowSeg(G, i)) : NEG; }
+__device__ __forceinline__ int32_t vM(Geo& G, int32_t i, int32_t j, int e) { return inside(G, i, j) ? vMk(G, i, j, e, rowSeg(G, i)) : NEG; }
+__device__ __forceinline__ int32_t vI(Geo& G, int32_t i, int32_t j, int e) { return inside(G, i, j) ? vIk(G, i, j, e, rowSeg(G, i)) : NEG; }
+__device__ __forceinline__ int32_t vD(Geo& G, int32_t i, int32_t j, int e) { return inside(G, i, j) ? vDk(G, i, j, e, rowSeg(G, i)) : NEG; }
 
 /* the rectangle holding cell (i, j), if any (the band rules of vMk) */
-__device__ bool cellRect(const Geo& G, int32_t i, int32_t j, Rc& R) {
+__device__ __forceinline__ bool cellRect(const Geo& G, int32_t i, int32_t j, Rc& R) {
     if (i < 1 || j < 1 || i > G.L || j > G.tLen) return false;
     const int k = rowSeg(G, i);
     if (k < 0) R = rectR5(G);
@@ -273,7 +273,7 @@ __device__ bool cellRect(const Geo& G, int32_t i, int32_t j, Rc& R) {
 }
 
 /* first diagonal value of seed k (src/Alignment.cpp:146-150), from its corner at event 1+2k */
-__device__ int32_t seedD0(Geo& G, int k) {
+__device__ __forceinline__ int32_t seedD0(Geo& G, int k) {
     const int4 s = G.sg[k];
     const int e = 1 + 2 * k;
     const int32_t sm = G.match;
@@ -557,9 +557,11 @@ __global__ void __launch_bounds__(64) dp_fill_kernel(DevIndex ix, DpArgs a) {
 
 /* ------------------------------------------------------------------ trace */
 
-/* at least 5 waves per SIMD (at most 96 VGPRs): the kernel waits on memory, and 5 waves
- * run it faster than 3 without spills or 6-8 with more of them */
-__global__ void __launch_bounds__(256, 5) dp_trace_kernel(DevIndex ix, DpArgs a) {
+/* at least 4 waves per SIMD (at most 128 VGPRs).  The band lookups (vM / vI / vD, cellRect,
+ * rowSeg, ...) are inlined, so the job geometry G stays in registers instead of private
+ * memory that every backtrace step would wait on; 4 waves per SIMD leave room for it
+ * (5 spill, 3 hide less latency) */
+__global__ void __launch_bounds__(256, 4) dp_trace_kernel(DevIndex ix, DpArgs a) {
     /* the quality terms of evaluate, by quality value, in LDS (one read per base) */
     __shared__ double sQM[256], sQX[256], sQG[256];
     for (int t = threadIdx.x; t < 256; t += blockDim.x) {

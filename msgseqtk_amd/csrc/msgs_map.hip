@@ -1206,7 +1206,7 @@ extern "C" int msgs_tune(msgs_ctx* c, int32_t slot, const msgs_params* p, const 
                 SmemStage sm;
                 rc = runSmem(c, S.dev, p, sm, nullptr);
                 if (rc) { restore(); return rc; }
-                t = std::min(t, c->x().lastSmemMicros + c->x().lastFilterMicros);
+                t = std::min(t, c->x().lastStageMicros + c->x().lastFilterMicros);
             }
             if (t < bestUs) {
                 bestUs = t; bestL = cand[i]; bestG = c->groupLanes; bestT = c->textPath; bestF = c->readFilter;
