@@ -126,7 +126,8 @@ int msgs_mems_batch(msgs_ctx* ctx, const msgs_reads* reads, const msgs_params* p
  * read's MEM list in the reference database `ref` and in the background database `bg`
  * (SMEM_LIST::findMEMS / findMEMS_PE and SMEM_LIST::loglik, src/SMEM.h:373-394; with mates,
  * the first read's list plus the mate's), and keep[i] = (-refLoglik[i] + bgLoglik[i] >
- * minLod).  mates may be NULL (single-end); any output may be NULL. */
+ * minLod).  mates may be NULL (single-end); any output may be NULL.  A failure inside one
+ * database's search is reported by msgs_last_error of that database's context. */
 int msgs_clean_batch(msgs_ctx* ref, msgs_ctx* bg, const msgs_reads* reads, const msgs_reads* mates,
                      const msgs_params* p, double minLod, double* refLoglik, double* bgLoglik, uint8_t* keep);
 /* SMEM_LIST::findSeeds (src/SMEM.cpp:271-305) incl. SeedPair::removeRedundant

@@ -276,9 +276,13 @@ class Mapper:
         rl = np.zeros(n, np.float64)
         bl = np.zeros(n, np.float64)
         keep = np.zeros(n, np.uint8)
-        self._chk(lib().msgs_clean_batch(self.h, background.h, C.byref(rb.st), C.byref(mb.st) if mb else None,
-                                         C.byref(params), float(min_lod), _p(rl, C.c_double), _p(bl, C.c_double),
-                                         keep.ctypes.data_as(C.POINTER(C.c_uint8))), "msgs_clean_batch")
+        rc = lib().msgs_clean_batch(self.h, background.h, C.byref(rb.st), C.byref(mb.st) if mb else None,
+                                    C.byref(params), float(min_lod), _p(rl, C.c_double), _p(bl, C.c_double),
+                                    keep.ctypes.data_as(C.POINTER(C.c_uint8)))
+        if rc:
+            # the failing database's context holds the message
+            msg = lib().msgs_last_error(self.h).decode() or lib().msgs_last_error(background.h).decode()
+            raise RuntimeError("msgs_clean_batch failed (%d): %s" % (rc, msg))
         return rl[:rb.n].copy(), bl[:rb.n].copy(), keep[:rb.n].astype(bool)
 
     def seeds(self, seqs, params):
