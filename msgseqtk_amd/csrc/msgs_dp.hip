@@ -43,8 +43,8 @@ constexpr int32_t NEG = -1073741824;
 enum : int32_t { OP_M = 0, OP_I = 1, OP_D = 2, OP_S = 4, OP_EQ = 7, OP_X = 8 };
 enum : int32_t { FWD = 1, REV = 2 };
 /* limits of the wavefront fill: seeds per job (LDS copy) and rectangle columns; with 64
- * seeds a one-wavefront block holds 7.7 KB of LDS, so 20 blocks fit a CU (5 waves per SIMD,
- * as many as the kernel's registers allow) */
+ * seeds a one-wavefront block holds 7.7 KB of LDS, so LDS allows 20 blocks per CU and the
+ * kernel's registers decide (the host sizes the persistent grid by the occupancy query) */
 constexpr int FILL_NS = 64;
 constexpr int FILL_COLS = 512;
 

@@ -352,8 +352,14 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     a.waveJobs = dWave; a.nWaveJobs = nWave;
     CK(hipEventRecord(c->x().ev2, st), "event");
     if (nWave) {
-        /* persistent: 20 one-wavefront blocks per CU, as many as are resident at once */
-        const int64_t blocks = std::min<int64_t>(nWave, (int64_t)std::max(1, c->nCU) * 20);
+        /* persistent: as many one-wavefront blocks per CU as are resident at once (blocks
+         * started late would take large jobs of the first stride last) */
+        static const int fillPerCU = []() {
+            int nb = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dp_fill_kernel, 64, 0) != hipSuccess || nb < 1) nb = 16;
+            return nb;
+        }();
+        const int64_t blocks = std::min<int64_t>(nWave, (int64_t)std::max(1, c->nCU) * fillPerCU);
         dp_fill_kernel<<<(unsigned)blocks, 64, 0, st>>>(c->db.view(), a);
     }
     dp_trace_kernel<<<gridFor(nJobs), 256, 0, st>>>(c->db.view(), a);
