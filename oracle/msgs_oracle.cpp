@@ -48,6 +48,8 @@ nt16_t toBasic(nt16_t b) { return b < 16 ? kBasic[b] : 0; }
 int toInt(nt16_t b) { return b == A ? 1 : b == C ? 2 : b == G ? 3 : b == T ? 4 : 0; }
 bool isAmbiguous(nt16_t b) { return !(b == A || b == C || b == G || b == T); }
 nt16_t encodeInt(int i) { return i == 1 ? A : i == 2 ? C : i == 3 ? G : i == 4 ? T : 0; }
+/* quality::encode (src/QualStr.h:40-42): the difference is taken in uint8_t, then clamped at 2 */
+uint8_t qualEncode(char c) { return std::max<uint8_t>((uint8_t)(c - 33), 2); }
 
 /* ------------------------------------------------------------------ A3 */
 /* BD text: seq + 0 + revcom(seq) + 0 after toBasic (src/MetaGenome.cpp:104-113);
@@ -371,7 +373,7 @@ std::vector<Smem> findMEMS(const Db& db, const Read& r, int64_t minLen, double m
 
 /* ------------------------------------------------------------------ A14/A15 */
 /* SMEM::getSeeds (src/SMEM.cpp:217-269) */
-static void getSeeds(const Db& db, const Read& r, const Smem& m, std::vector<Seed>& out) {
+void getSeeds(const Db& db, const Read& r, const Smem& m, std::vector<Seed>& out) {
     const int64_t len = m.to - m.from;
     for (int64_t i = 0; i < m.size; ++i) {
         {

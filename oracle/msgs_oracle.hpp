@@ -30,6 +30,7 @@ nt16_t toBasic(nt16_t b);          /* base2basic */
 int toInt(nt16_t b);               /* base2int */
 bool isAmbiguous(nt16_t b);        /* src/DNAalphabet.h:135-142 */
 nt16_t encodeInt(int i);           /* DNAalphabet::encode(int) */
+uint8_t qualEncode(char c);        /* quality::encode with the default shift 33 */
 
 /* GLoc::STRAND (src/GLoc.h:20) */
 enum Strand { FWD = 1, REV = 2, UNK = 3 };
@@ -100,6 +101,8 @@ std::vector<Smem> findMEMS(const Db& db, const Read& r, int64_t minLen, double m
 /* ---- A16 ---- */
 struct Seed { int64_t from, to, start, end, tid; int32_t strand; double logP; };
 std::vector<Seed> findSeeds(const Db& db, const Read& r, int64_t minLen, double maxE, int64_t maxNSeed);
+/* SMEM::getSeeds of one SMEM, appended to out (src/SMEM.cpp:217-269) */
+void getSeeds(const Db& db, const Read& r, const Smem& m, std::vector<Seed>& out);
 void removeRedundant(std::vector<Seed>& seeds);                                  /* A17 */
 std::vector<std::vector<Seed>> getChains(const std::vector<Seed>& seeds, int64_t maxMM, int64_t maxIndel); /* A18 */
 

@@ -259,4 +259,89 @@ int64_t orc_map(void* h, const orc_params* p, const uint8_t* seq, const uint8_t*
 
 int32_t orc_max_threads() { return omp_get_max_threads(); }
 
+/* The oracle's alphabet tables in the layout of ref_alphabet (oracle/ref_harness.cpp). */
+void orc_alphabet(uint8_t* enc256, char* dec16, uint8_t* comp16, uint8_t* basic16, int32_t* int16, uint8_t* amb16,
+                  uint8_t* qenc256) {
+    for (int c = 0; c < 256; ++c) { enc256[c] = encodeSym((char)c); qenc256[c] = qualEncode((char)c); }
+    for (int b = 0; b < 16; ++b) {
+        dec16[b] = decodeBase((nt16_t)b);
+        comp16[b] = complement((nt16_t)b);
+        basic16[b] = toBasic((nt16_t)b);
+        int16[b] = toInt((nt16_t)b);
+        amb16[b] = isAmbiguous((nt16_t)b) ? 1 : 0;
+    }
+}
+
+/* revcomRead of one read: codes and scores */
+void orc_revcom_read(const uint8_t* seq, const uint8_t* qual, int64_t L, uint8_t* outSeq, uint8_t* outQual) {
+    Read r = makeRead("r", seq, qual, L);
+    Read rc = revcomRead(r);
+    std::memcpy(outSeq, rc.seq.data(), L);
+    std::memcpy(outQual, rc.qual.data(), L);
+}
+
+/* Seeds of every SMEM of each read, before the loglik sort, the maxNSeed cut and
+ * removeRedundant: per SMEM its size and the number of seeds getSeeds made (the reference
+ * asserts they are equal, src/SMEM.cpp:240,267), and the seed rows (from, to, start, end,
+ * tid, strand).  Returns the number of seed rows (rows beyond cap are counted, not written). */
+int64_t orc_raw_seeds(void* h, const uint8_t* seq, const int64_t* offs, int64_t n, int64_t minLen, double maxE,
+                      int64_t* smemCount, int64_t* sizeAndSeeds, int64_t smemCap, int64_t* out6, int64_t cap) {
+    Db* db = (Db*)h;
+    std::vector<Read> reads = makeReads(seq, nullptr, offs, nullptr, n);
+    int64_t k = 0, m = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        std::vector<Smem> v = findAllSMEMS(*db, reads[i], minLen, maxE);
+        smemCount[i] = (int64_t)v.size();
+        for (const Smem& s : v) {
+            std::vector<Seed> seeds;
+            getSeeds(*db, reads[i], s, seeds);
+            if (m < smemCap) { sizeAndSeeds[2 * m] = s.size; sizeAndSeeds[2 * m + 1] = (int64_t)seeds.size(); }
+            ++m;
+            for (const Seed& x : seeds) {
+                if (k < cap) {
+                    int64_t* o = out6 + 6 * k;
+                    o[0] = x.from; o[1] = x.to; o[2] = x.start; o[3] = x.end; o[4] = x.tid; o[5] = x.strand;
+                }
+                ++k;
+            }
+        }
+    }
+    return k;
+}
+
+/* One chain of caller-given seeds (rows from, to, start, end, tid, strand) aligned against
+ * chromosome tid of the database: res10 as in orc_alignments; the score, Alignment::evaluate's
+ * log10P, the CIGAR and the MD tag. */
+int64_t orc_align_chain(void* h, const orc_params* p, const uint8_t* seq, const uint8_t* qual, int64_t L,
+                        const int64_t* rows6, int64_t nSeeds, int64_t* res10, double* scoreOut, double* log10POut,
+                        char* cigar, char* md, int64_t strCap) {
+    Db* db = (Db*)h;
+    Params P = toParams(p);
+    std::vector<uint8_t> q30(L, 30);
+    Read r = makeRead("q", seq, qual ? qual : q30.data(), L);
+    Read rc = revcomRead(r);
+    std::vector<Seed> chain(nSeeds);
+    for (int64_t i = 0; i < nSeeds; ++i) {
+        const int64_t* x = rows6 + 6 * i;
+        chain[i] = Seed{x[0], x[1], x[2], x[3], x[4], (int32_t)x[5], 0};
+    }
+    Aln a = alignChain(*db, P, r, rc, chain);
+    evaluate(P, a);
+    res10[0] = a.tid; res10[1] = a.qStrand; res10[2] = a.tStart; res10[3] = a.tEnd; res10[4] = a.alnFrom;
+    res10[5] = a.alnTo; res10[6] = a.alnStart; res10[7] = a.alnEnd; res10[8] = a.hazard; res10[9] = a.dpCells;
+    *scoreOut = a.alnScore;
+    *log10POut = a.log10P;
+    std::snprintf(cigar, strCap, "%s", cigarString(a).c_str());
+    std::snprintf(md, strCap, "%s", mdTag(a).c_str());
+    return 0;
+}
+
+/* Alignment::calcMapQ over alignments with the given log10P values */
+void orc_calc_mapq(const double* log10P, int64_t n, double* postP, uint8_t* mapQ) {
+    std::vector<Aln> v(n);
+    for (int64_t i = 0; i < n; ++i) v[i].log10P = log10P[i];
+    calcMapQ(v);
+    for (int64_t i = 0; i < n; ++i) { postP[i] = v[i].postP; mapQ[i] = v[i].mapQ; }
+}
+
 }  // extern "C"

@@ -13,7 +13,11 @@
 #include <sstream>
 #include <string>
 
+#include "DNAalphabet.h"
+#include "DNAseq.h"
 #include "FMDIndex.h"
+#include "PrimarySeq.h"
+#include "QualStr.h"
 #include "SMEM.h"
 #include "SeedChain.h"
 #include "SeedPair.h"
@@ -21,6 +25,43 @@
 using namespace EGriceLab::MSGseqTK;
 
 extern "C" {
+
+/* The reference's alphabet tables (src/DNAalphabet.h:45-190): DNAalphabet::encode of every
+ * char, and per nt16 code decode / complement / toBasic / toInt / isAmbiguous; plus
+ * quality::encode of every char (src/QualStr.h:40-42). */
+void ref_alphabet(uint8_t* enc256, char* dec16, uint8_t* comp16, uint8_t* basic16, int32_t* int16, uint8_t* amb16,
+                  uint8_t* qenc256) {
+    for (int c = 0; c < 256; ++c) {
+        enc256[c] = DNAalphabet::encode((char)c);
+        qenc256[c] = quality::encode((char)c);
+    }
+    for (int b = 0; b < 16; ++b) {
+        dec16[b] = DNAalphabet::decode((nt16_t)b);
+        comp16[b] = DNAalphabet::complement((nt16_t)b);
+        basic16[b] = DNAalphabet::toBasic((nt16_t)b);
+        int16[b] = DNAalphabet::toInt((nt16_t)b);
+        amb16[b] = DNAalphabet::isAmbiguous((nt16_t)b) ? 1 : 0;
+    }
+}
+/* dna::encode of a string (src/DNAseq.cpp) */
+int64_t ref_encode(const char* s, uint8_t* out) {
+    DNAseq d = dna::encode(std::string(s));
+    std::copy(d.begin(), d.end(), out);
+    return (int64_t)d.size();
+}
+/* dna::revcom (src/DNAseq.h:134-152) */
+void ref_revcom(const uint8_t* s, int64_t n, uint8_t* out) {
+    DNAseq d = dna::revcom(DNAseq(s, s + n));
+    std::copy(d.begin(), d.end(), out);
+}
+/* PrimarySeq(seqStr, name, "", qStr).revcom() (src/PrimarySeq.h:54-57,144-152): codes and scores */
+int64_t ref_primary_revcom(const char* seq, const char* qual, uint8_t* outSeq, uint8_t* outQual) {
+    PrimarySeq p(std::string(seq), "r", "", std::string(qual));
+    PrimarySeq r = p.revcom();
+    std::copy(r.getSeq().begin(), r.getSeq().end(), outSeq);
+    std::copy(r.getQual().begin(), r.getQual().end(), outQual);
+    return (int64_t)r.length();
+}
 
 void* ref_index_build(const uint8_t* bd, int64_t n) {
     try {

@@ -96,6 +96,15 @@ def orc():
                                 C.c_char_p, C.c_int64, C.c_int32, C.c_char_p, C.c_int64, P64, PD]
         lib.orc_map.restype = C.c_int64
         lib.orc_max_threads.restype = C.c_int32
+        lib.orc_alphabet.argtypes = [PU8, C.c_char_p, PU8, PU8, C.POINTER(C.c_int32), PU8, PU8]
+        lib.orc_revcom_read.argtypes = [PU8, PU8, C.c_int64, PU8, PU8]
+        lib.orc_raw_seeds.argtypes = [C.c_void_p, PU8, P64, C.c_int64, C.c_int64, C.c_double, P64, P64,
+                                      C.c_int64, P64, C.c_int64]
+        lib.orc_raw_seeds.restype = C.c_int64
+        lib.orc_align_chain.argtypes = [C.c_void_p, C.POINTER(Params), PU8, PU8, C.c_int64, P64, C.c_int64, P64,
+                                        PD, PD, C.c_char_p, C.c_char_p, C.c_int64]
+        lib.orc_align_chain.restype = C.c_int64
+        lib.orc_calc_mapq.argtypes = [PD, C.c_int64, PD, PU8]
         _orc = lib
     return _orc
 
@@ -133,6 +142,12 @@ def ref():
         lib.ref_remove_redundant.restype = C.c_int64
         lib.ref_get_chains.argtypes = [P64, C.c_int64, C.c_int64, C.c_int64, P64, C.c_int64]
         lib.ref_get_chains.restype = C.c_int64
+        lib.ref_alphabet.argtypes = [PU8, C.c_char_p, PU8, PU8, C.POINTER(C.c_int32), PU8, PU8]
+        lib.ref_encode.argtypes = [C.c_char_p, PU8]
+        lib.ref_encode.restype = C.c_int64
+        lib.ref_revcom.argtypes = [PU8, C.c_int64, PU8]
+        lib.ref_primary_revcom.argtypes = [C.c_char_p, C.c_char_p, PU8, PU8]
+        lib.ref_primary_revcom.restype = C.c_int64
         _ref = lib
     return _ref
 
@@ -271,3 +286,58 @@ class RefIndex:
                                  ptr(out, C.c_int64), ptr(lp, C.c_double), cap)
         assert 0 <= k <= cap
         return out[:k].copy(), lp[:k].copy()
+
+
+def alphabet(lib, prefix):
+    """(encode[256], decode[16], complement[16], toBasic[16], toInt[16], isAmbiguous[16],
+    quality encode[256]) from orc_alphabet or ref_alphabet"""
+    enc = np.zeros(256, np.uint8)
+    dec = C.create_string_buffer(16)
+    comp = np.zeros(16, np.uint8)
+    basic = np.zeros(16, np.uint8)
+    toint = np.zeros(16, np.int32)
+    amb = np.zeros(16, np.uint8)
+    qenc = np.zeros(256, np.uint8)
+    getattr(lib, prefix + "_alphabet")(ptr(enc, C.c_uint8), dec, ptr(comp, C.c_uint8), ptr(basic, C.c_uint8),
+                                       ptr(toint, C.c_int32), ptr(amb, C.c_uint8), ptr(qenc, C.c_uint8))
+    return enc, dec.raw[:16], comp, basic, toint, amb, qenc
+
+
+def orc_raw_seeds(db, reads, minLen=17, maxE=0.01):
+    """per read its SMEM count; per SMEM (size, seeds made); all seed rows"""
+    seq, offs = pack_reads(reads)
+    n = len(reads)
+    counts = np.zeros(max(n, 1), np.int64)
+    scap, cap = 1 << 16, 1 << 18
+    while True:
+        sz = np.zeros((scap, 2), np.int64)
+        out = np.zeros((cap, 6), np.int64)
+        k = orc().orc_raw_seeds(db.h, ptr(seq, C.c_uint8), ptr(offs, C.c_int64), n, minLen, maxE,
+                                ptr(counts, C.c_int64), ptr(sz, C.c_int64), scap, ptr(out, C.c_int64), cap)
+        m = int(counts[:n].sum())
+        if k <= cap and m <= scap:
+            return counts[:n].copy(), sz[:m].copy(), out[:k].copy()
+        cap, scap = max(cap, k), max(scap, m)
+
+
+def orc_align_chain(db, params, read, seeds, qual=None):
+    """one chain of seed rows (from, to, start, end, tid, strand): (res10, score, log10P, CIGAR, MD)"""
+    r = np.ascontiguousarray(read, np.uint8)
+    q = np.ascontiguousarray(qual if qual is not None else np.full(len(r), 30), np.uint8)
+    rows = np.ascontiguousarray(seeds, np.int64).reshape(-1, 6)
+    res = np.zeros(10, np.int64)
+    sc = np.zeros(1)
+    lp = np.zeros(1)
+    cig, md = C.create_string_buffer(4096), C.create_string_buffer(4096)
+    orc().orc_align_chain(db.h, C.byref(params), ptr(r, C.c_uint8), ptr(q, C.c_uint8), len(r),
+                          ptr(rows, C.c_int64), len(rows), ptr(res, C.c_int64), ptr(sc, C.c_double),
+                          ptr(lp, C.c_double), cig, md, 4096)
+    return res, float(sc[0]), float(lp[0]), cig.value.decode(), md.value.decode()
+
+
+def orc_calc_mapq(log10p):
+    v = np.ascontiguousarray(log10p, np.float64)
+    pp = np.zeros(len(v))
+    mq = np.zeros(len(v), np.uint8)
+    orc().orc_calc_mapq(ptr(v, C.c_double), len(v), ptr(pp, C.c_double), ptr(mq, C.c_uint8))
+    return pp, mq
