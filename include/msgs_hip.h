@@ -135,6 +135,30 @@ int msgs_clean_batch(msgs_ctx* ref, msgs_ctx* bg, const msgs_reads* reads, const
 int msgs_seed_batch(msgs_ctx* ctx, const msgs_reads* reads, const msgs_params* p, int64_t* counts,
                     int64_t* rows6, double* logP, int64_t cap, int64_t* needed);
 
+/* One SMEM in device memory (the layout msgs_smem_batch_noalloc writes): read interval
+ * [from, to), bi-interval (fwdStart, revStart, size), loglik. */
+typedef struct {
+    int32_t from, to;
+    int64_t fwdStart, revStart, size;
+    double logP;
+} msgs_smem_rec;
+
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the SMEM
+ * search of msgs_smem_batch (SMEM_LIST::findAllSMEMS per read) over n reads already in device
+ * memory (dSeq nt16 codes, dOffs[n+1] offsets into dSeq), written to caller device memory:
+ * dCounts[i] SMEMs of read i (it may exceed cap: only the first cap are stored) at
+ * dRecs[i * cap ...].  It allocates nothing: its scratch (work queue, loglik prefix sums,
+ * candidate lists) is a fixed 128 MiB device array per GPU, held until the process ends;
+ * inputs whose prefix sums (8 bytes per base + 8 per read) or candidate lists (32 bytes per
+ * base of the longest read + 64, times 256 lanes at least) do not fit return MSGS_E_CAP.
+ * *maxCount (may be NULL) is the largest count; MSGS_E_CAP when it exceeds cap.  Calls that
+ * overlap on one GPU share the scratch and race.  Runs on the context's own stream and waits
+ * for it. */
+int msgs_smem_batch_noalloc(msgs_ctx* ctx, const uint8_t* dSeq, const int64_t* dOffs, int64_t n,
+                            const msgs_params* p, int32_t cap, int32_t* dCounts, msgs_smem_rec* dRecs,
+                            int32_t* maxCount);
+
 /* ---- whole path ----
  * msgseqtk-align main_SE / main_PE (src/msgseqtk-align.cpp:595-758): seeds, chains
  * (SeedChain::getChains, src/SeedChain.cpp:40-73), Alignment::buildAlignments

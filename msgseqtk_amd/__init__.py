@@ -27,6 +27,7 @@ EXPORTS = [
     "msgs_map_batch", "msgs_align_batch", "msgs_batch_upload", "msgs_batch_run",
     "msgs_batch_stats", "msgs_batch_sam", "msgs_util_sort_by_key",
     "msgs_batch_submit", "msgs_batch_wait", "msgs_tune", "msgs_util_calib_copy", "msgs_util_random_gather", "msgs_util_format_g6", "msgs_ctx_get", "msgs_lix_train", "msgs_lix_query", "msgs_lix_predict", "msgs_lix_grad", "msgs_lix_weights", "msgs_lix_knots",
+    "msgs_smem_batch_noalloc",
 ]
 N_LIX_PARAMS = 16 * 64 + 64 + 64 + 1
 
@@ -111,6 +112,8 @@ def lib():
     L.msgs_lix_grad.argtypes = [vp, PF, PF, PF, C.c_int32, PF, PF]
     L.msgs_lix_weights.argtypes = [vp, PF, PF]
     L.msgs_lix_knots.argtypes = [vp, PU64, P64, C.POINTER(C.c_int32)]
+    L.msgs_smem_batch_noalloc.argtypes = [vp, vp, vp, C.c_int64, C.POINTER(Params), C.c_int32, vp, vp,
+                                          C.POINTER(C.c_int32)]
     _lib = L
     return L
 

@@ -172,6 +172,7 @@ int msgs_ctx_create(int device, msgs_ctx** out) {
     /* the worker streams do not synchronize with the null stream */
     bool ok = c->own.create(hipStreamDefault);
     for (Exec& e : c->work) ok = ok && e.create(hipStreamNonBlocking);
+    ok = ok && noallocInit(device) == MSGS_OK;
     if (!ok) {
         c->own.destroy();
         for (Exec& e : c->work) e.destroy();

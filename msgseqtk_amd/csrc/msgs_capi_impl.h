@@ -173,6 +173,8 @@ struct msgs_ctx {
 };
 
 int fail(msgs_ctx* c, int code, const std::string& msg);
+/* looks up the allocation-free scratch array's address on `device` (the current device) */
+int noallocInit(int device);
 bool evalueThreshold(double n, double maxE, double* T);
 int uploadReads(msgs_ctx* c, const msgs_reads* r, const char* pfx, DevReads& out);
 int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out, unsigned long long* dStats);
