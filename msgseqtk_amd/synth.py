@@ -14,8 +14,10 @@ SYM = np.frombuffer(b"=ACMGRSVTWYHKDBN", dtype=np.uint8)
 
 
 def random_genome(rng, length, gc=0.5):
-    p = np.array([(1 - gc) / 2, gc / 2, gc / 2, (1 - gc) / 2])
-    return NT[rng.choice(4, size=length, p=p)]
+    """i.i.d. bases with P(A) = P(T) = (1 - gc) / 2 and P(C) = P(G) = gc / 2"""
+    u = rng.random(length)
+    c1, c2, c3 = (1 - gc) / 2, 0.5, (1 + gc) / 2
+    return NT[(u >= c1).astype(np.uint8) + (u >= c2) + (u >= c3)]
 
 
 def mutate(rng, seq, sub, indel, maxlen=3):
@@ -194,3 +196,57 @@ def simulate_pairs_fast(chroms, n, length=150, insert=(350, 50), sub=0.005, inde
         out.append((seqs, quals))
     names = ["%s%d" % (prefix, i) for i in range(n)]
     return (out[0][0], out[0][1], names), (out[1][0], out[1][1], names)
+
+
+def simulate_reads_fast(chroms, n, length=150, sub=0.01, indel=0.001, qual=(20, 40), seed=1, offtarget=0.0,
+                        prefix="r"):
+    """Vectorized single-end simulation for large batches (same model as simulate_reads with a
+    fixed length: uniform genome / position / strand, substitutions and 1-3 bp indels, Phred
+    scores uniform in qual; a fraction `offtarget` of reads is uniform random sequence).
+    Returns (seqs, quals, names)."""
+    rng = np.random.default_rng(seed)
+    lens = np.array([len(c) for c in chroms], np.int64)
+    starts = np.concatenate([[0], np.cumsum(lens)[:-1]])
+    allseq = np.concatenate(chroms)
+    span = length + 8
+    g = rng.integers(0, len(chroms), n)
+    pos = (rng.random(n) * np.maximum(1, lens[g] - span)).astype(np.int64)
+    a = np.clip(starts[g] + pos, 0, len(allseq) - span)
+    w = allseq[a[:, None] + np.arange(span)[None, :]]
+    strand = rng.integers(0, 2, n).astype(bool)
+    w[strand] = COMP[w[strand][:, ::-1]]
+    off = rng.random(n) < offtarget
+    if off.any():
+        w[off] = NT[rng.integers(0, 4, (int(off.sum()), span))]
+    subm = rng.random(w.shape) < sub
+    cur = np.searchsorted(NT, w[subm]) % 4
+    w[subm] = NT[(cur + rng.integers(1, 4, int(subm.sum()))) % 4]
+    seqs = [row[:length] for row in w]
+    k = rng.binomial(span, indel, n) if indel > 0 else np.zeros(n, np.int64)
+    for i in np.nonzero(k)[0]:
+        s = mutate(rng, w[i], 0.0, indel)[:length]
+        if len(s) < length:
+            s = np.concatenate([s, w[i][len(s):length]])
+        seqs[i] = s
+    seqs = [np.ascontiguousarray(s, np.uint8) for s in seqs]
+    quals = list(rng.integers(qual[0], qual[1] + 1, (n, length)).astype(np.uint8))
+    names = ["%s%d" % (prefix, i) for i in range(n)]
+    return seqs, quals, names
+
+
+# The five benchmark configurations' databases (genome seeds 0xC0FFEE + config number):
+#   1: 8 x 250 kb, GC 0.5;  2 and 3: 64 x 4 Mb, GC U(0.3, 0.7);
+#   4: 256 x 4 Mb, GC U(0.3, 0.7), 20% strains at U(1%, 5%) divergence from a parent;
+#   5: 1000 genomes, lengths log-uniform in 1-8 Mb, 30% strains at U(0.5%, 5%).
+def config_genomes(k):
+    seed = 0xC0FFEE + k
+    if k == 1:
+        return metagenome(8, 250_000, gc=None, seed=seed)
+    if k in (2, 3):
+        return metagenome(64, 4_000_000, gc=(0.3, 0.7), seed=0xC0FFEE + 2)
+    if k == 4:
+        return metagenome(256, 4_000_000, gc=(0.3, 0.7), seed=seed, strain_frac=0.2, div=(0.01, 0.05))
+    if k == 5:
+        return metagenome(1000, (1_000_000, 8_000_000), gc=(0.3, 0.7), seed=seed, strain_frac=0.3,
+                          div=(0.005, 0.05))
+    raise ValueError("configurations are 1..5")
