@@ -87,12 +87,12 @@ int64_t getTid(const Db& db, int64_t pos) {
     return -1;
 }
 /* getStrand (src/MetaGenome.h:161-167) */
-static int getStrand(const Db& db, int64_t tid, int64_t pos) {
+int getStrand(const Db& db, int64_t tid, int64_t pos) {
     int64_t s = db.bdStart[tid], len = db.bdEnd[tid] - db.bdStart[tid];
     return (s <= pos && pos < s + len / 2) ? FWD : REV;
 }
 /* getLoc (src/MetaGenome.h:147-153) */
-static int64_t getLoc(const Db& db, int64_t tid, int64_t pos) {
+int64_t getLoc(const Db& db, int64_t tid, int64_t pos) {
     if (getStrand(db, tid, pos) == FWD) return pos - db.bdStart[tid];
     return db.bdEnd[tid] - pos - 1;
 }
@@ -168,8 +168,11 @@ int64_t rank(const Db& db, nt16_t s, int64_t i) {
 
 /* ------------------------------------------------------------------ A6 */
 /* FMDIndex::backExt (src/FMDIndex.cpp:577-610) */
+bool g_countExt = false;
+int64_t g_nExt = 0;
 int64_t backExt(const Db& db, int64_t& p, int64_t& q, int64_t& s, nt16_t b) {
     if (isAmbiguous(b)) return (s = 0);
+    if (g_countExt) ++g_nExt;   /* single-threaded counting runs only (orc_count_ext) */
     int64_t sB[5], qB[5];
     int c = toInt(b);
     int64_t O = rank(db, b, p - 1);
@@ -956,12 +959,17 @@ static std::vector<Aln> buildAlignments(const Db& db, const Params& P, const Rea
 /* ------------------------------------------------------------------ A32 */
 /* main_SE per-read task (src/msgseqtk-align.cpp:607-649) */
 std::string mapSE(const Db& db, const Params& P, const Read& read) {
+    return mapSE(db, P, read,
+                 [&](const Read& r) { return findSeeds(db, r, P.minLen, P.maxEvalue, P.maxNSeed); }, getChains);
+}
+
+std::string mapSE(const Db& db, const Params& P, const Read& read, const SeedFn& seedFn, const ChainFn& chainFn) {
     Read rc = revcomRead(read);
-    std::vector<Seed> seeds = findSeeds(db, read, P.minLen, P.maxEvalue, P.maxNSeed);
+    std::vector<Seed> seeds = seedFn(read);
     if (seeds.empty()) return outputUnmapped(db, read, 4);
     const int64_t L = read.L;
     const int64_t maxMM = (int64_t)std::ceil(L * 0.2), maxIndel = (int64_t)std::ceil(L * 0.1);
-    auto chains = getChains(seeds, maxMM, maxIndel);
+    auto chains = chainFn(seeds, maxMM, maxIndel);
     std::vector<Aln> alns = buildAlignments(db, P, read, rc, chains);
     /* the alignments point into read/rc, which live in this frame */
     filterAlns(alns, P.minScoreRate);
@@ -992,13 +1000,19 @@ static void calcMapQPE(const Params& P, std::vector<Pair>& v) {
 
 /* main_PE per-pair task (src/msgseqtk-align.cpp:679-754) */
 std::string mapPE(const Db& db, const Params& P, const Read& r1, const Read& r2) {
+    return mapPE(db, P, r1, r2,
+                 [&](const Read& r) { return findSeeds(db, r, P.minLen, P.maxEvalue, P.maxNSeed); }, getChains);
+}
+
+std::string mapPE(const Db& db, const Params& P, const Read& r1, const Read& r2, const SeedFn& seedFn,
+                  const ChainFn& chainFn) {
     Read rc1 = revcomRead(r1), rc2 = revcomRead(r2);
-    std::vector<Seed> s1 = findSeeds(db, r1, P.minLen, P.maxEvalue, P.maxNSeed);
-    std::vector<Seed> s2 = findSeeds(db, r2, P.minLen, P.maxEvalue, P.maxNSeed);
+    std::vector<Seed> s1 = seedFn(r1);
+    std::vector<Seed> s2 = seedFn(r2);
     if (s1.empty() && s2.empty())
         return outputUnmapped(db, r1, 4 | 8 | 1 | 64) + outputUnmapped(db, r2, 4 | 8 | 1 | 128);
-    auto ch1 = getChains(s1, (int64_t)std::ceil(r1.L * 0.2), (int64_t)std::ceil(r1.L * 0.1));
-    auto ch2 = getChains(s2, (int64_t)std::ceil(r2.L * 0.2), (int64_t)std::ceil(r2.L * 0.1));
+    auto ch1 = chainFn(s1, (int64_t)std::ceil(r1.L * 0.2), (int64_t)std::ceil(r1.L * 0.1));
+    auto ch2 = chainFn(s2, (int64_t)std::ceil(r2.L * 0.2), (int64_t)std::ceil(r2.L * 0.1));
     std::vector<Aln> a1 = buildAlignments(db, P, r1, rc1, ch1);
     std::vector<Aln> a2 = buildAlignments(db, P, r2, rc2, ch2);
     filterAlns(a1, P.minScoreRate);

@@ -14,6 +14,7 @@
 #define MSGS_ORACLE_HPP
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -78,10 +79,15 @@ void buildIndex(Db& db, const int64_t* saIn);
 
 int64_t rank(const Db& db, nt16_t s, int64_t i);                                 /* A5 */
 int64_t backExt(const Db& db, int64_t& p, int64_t& q, int64_t& s, nt16_t b);      /* A6 */
+/* counting of backExt/fwdExt calls with an unambiguous base (single-threaded runs only) */
+extern bool g_countExt;
+extern int64_t g_nExt;
 int64_t fwdExt(const Db& db, int64_t& p, int64_t& q, int64_t& s, nt16_t b);       /* A7 */
 int64_t accessSA(const Db& db, int64_t i);                                       /* A8 */
 double loglik(const Db& db, nt16_t b);                                           /* A9 */
 int64_t getTid(const Db& db, int64_t pos);                                       /* A3 */
+int getStrand(const Db& db, int64_t tid, int64_t pos);                            /* A3 */
+int64_t getLoc(const Db& db, int64_t tid, int64_t pos);                         /* A3 */
 
 /* ---- reads (A2) ---- */
 struct Read {
@@ -131,6 +137,19 @@ void calcMapQ(std::vector<Aln>& alns);
 /* ---- A31/A32 ---- */
 std::string mapSE(const Db& db, const Params& P, const Read& r);
 std::string mapPE(const Db& db, const Params& P, const Read& r1, const Read& r2);
+/* the same tasks with the seeding (SMEM_LIST::findSeeds) and chaining (SeedChain::getChains)
+ * supplied by the caller */
+using SeedFn = std::function<std::vector<Seed>(const Read&)>;
+using ChainFn = std::function<std::vector<std::vector<Seed>>(const std::vector<Seed>&, int64_t, int64_t)>;
+std::string mapSE(const Db& db, const Params& P, const Read& r, const SeedFn& seeds, const ChainFn& chains);
+std::string mapPE(const Db& db, const Params& P, const Read& r1, const Read& r2, const SeedFn& seeds,
+                  const ChainFn& chains);
+/* a batch of reads (SE: seq2 == nullptr) mapped per read / pair, OpenMP over tasks; returns the
+ * bytes of SAM text (out may be null), -1 if cap is too small; see orc_map (orc_capi.cpp) */
+int64_t mapBatch(const Db& db, const Params& P, const uint8_t* seq, const uint8_t* qual, const int64_t* offs,
+                 const char* names, const uint8_t* seq2, const uint8_t* qual2, const int64_t* offs2,
+                 const char* names2, int64_t n, int32_t nThreads, char* out, int64_t cap, int64_t* lineOffs,
+                 double* seconds, const SeedFn* seeds, const ChainFn* chains);
 
 } // namespace orc
 #endif

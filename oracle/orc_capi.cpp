@@ -231,16 +231,33 @@ int64_t orc_map(void* h, const orc_params* p, const uint8_t* seq, const uint8_t*
                 const char* names, const uint8_t* seq2, const uint8_t* qual2, const int64_t* offs2,
                 const char* names2, int64_t n, int32_t nThreads, char* out, int64_t cap, int64_t* lineOffs,
                 double* seconds) {
-    Db* db = (Db*)h;
-    Params P = toParams(p);
+    return mapBatch(*(Db*)h, toParams(p), seq, qual, offs, names, seq2, qual2, offs2, names2, n, nThreads, out, cap,
+                    lineOffs, seconds, nullptr, nullptr);
+}
+
+/* the caller's parameter block as oracle parameters (same layout as msgs_params) */
+void orc_params_convert(const orc_params* p, void* out) { *(Params*)out = toParams(p); }
+
+}  // extern "C"
+
+namespace orc {
+int64_t mapBatch(const Db& dbr, const Params& P, const uint8_t* seq, const uint8_t* qual, const int64_t* offs,
+                 const char* names, const uint8_t* seq2, const uint8_t* qual2, const int64_t* offs2,
+                 const char* names2, int64_t n, int32_t nThreads, char* out, int64_t cap, int64_t* lineOffs,
+                 double* seconds, const SeedFn* seedFn, const ChainFn* chainFn) {
+    const Db* db = &dbr;
     std::vector<Read> r1 = makeReads(seq, qual, offs, names, n);
     std::vector<Read> r2;
     if (offs2) r2 = makeReads(seq2, qual2, offs2, names2, n);
     std::vector<std::string> res(n);
     auto t0 = std::chrono::steady_clock::now();
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nThreads > 0 ? nThreads : 1)
-    for (int64_t i = 0; i < n; ++i)
-        res[i] = offs2 ? mapPE(*db, P, r1[i], r2[i]) : mapSE(*db, P, r1[i]);
+    for (int64_t i = 0; i < n; ++i) {
+        if (seedFn)
+            res[i] = offs2 ? mapPE(*db, P, r1[i], r2[i], *seedFn, *chainFn) : mapSE(*db, P, r1[i], *seedFn, *chainFn);
+        else
+            res[i] = offs2 ? mapPE(*db, P, r1[i], r2[i]) : mapSE(*db, P, r1[i]);
+    }
     auto t1 = std::chrono::steady_clock::now();
     if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
     int64_t total = 0;
@@ -256,6 +273,9 @@ int64_t orc_map(void* h, const orc_params* p, const uint8_t* seq, const uint8_t*
     if (lineOffs) lineOffs[n] = k;
     return total;
 }
+}  // namespace orc
+
+extern "C" {
 
 int32_t orc_max_threads() { return omp_get_max_threads(); }
 
@@ -334,6 +354,18 @@ int64_t orc_align_chain(void* h, const orc_params* p, const uint8_t* seq, const 
     std::snprintf(cigar, strCap, "%s", cigarString(a).c_str());
     std::snprintf(md, strCap, "%s", mdTag(a).c_str());
     return 0;
+}
+
+/* backExt / fwdExt calls with an unambiguous base (the FMD extensions, two occ-block reads
+ * each) made by findAllSMEMS over the reads, single-threaded */
+int64_t orc_count_ext(void* h, const uint8_t* seq, const int64_t* offs, int64_t n, int64_t minLen, double maxE) {
+    Db* db = (Db*)h;
+    std::vector<Read> reads = makeReads(seq, nullptr, offs, nullptr, n);
+    g_nExt = 0;
+    g_countExt = true;
+    for (int64_t i = 0; i < n; ++i) (void)findAllSMEMS(*db, reads[i], minLen, maxE);
+    g_countExt = false;
+    return g_nExt;
 }
 
 /* Alignment::calcMapQ over alignments with the given log10P values */

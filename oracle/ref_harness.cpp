@@ -154,3 +154,151 @@ int64_t ref_get_chains(const int64_t* rows6, int64_t n, int64_t maxMM, int64_t m
 }
 
 }  // extern "C"
+
+/* ------------------------------------------------------------------ CPU baseline */
+
+#include <chrono>
+#include <cmath>
+#include <sstream>
+#include <unordered_map>
+
+#include "StringUtils.h"
+#include "msgs_oracle.hpp"
+
+extern "C" {
+
+/* The reference's FMDIndex over the BD text bd[0..n) with the given suffix array, made of
+ * the reference's own components the way its constructor makes them (src/FMDIndex.cpp:96-116:
+ * counts, BWT, gapSA = SA[0 .. numGaps), SA sampled at i % 32 == 0 or BWT[i] == 0 into an RRR
+ * bit set, WaveletTreeRRR over the BWT) and read back through FMDIndex::load from their own
+ * save() output in FMDIndex::save's order (src/FMDIndex.cpp:143-161).  Replaces the suffix
+ * sort of the constructor with the caller's (equal) suffix array. */
+void* ref_index_from_sa(const uint8_t* bd, const int64_t* sa, int64_t n) {
+    try {
+        std::array<int64_t, 256> B = {}, C = {};
+        for (int64_t i = 0; i < n; ++i) B[bd[i]]++;
+        int64_t S = 0;
+        for (int i = 0; i <= DNAalphabet::SIZE; ++i) { C[i] = S; S += B[i]; }
+        DNAseq bwt(n, 0);
+        for (int64_t i = 0; i < n; ++i) bwt[i] = sa[i] == 0 ? 0 : bd[sa[i] - 1];
+        /* sampled SA first: the wavelet tree's constructor consumes the BWT (as in the
+         * reference's constructor, which samples before it builds the tree) */
+        BitStr32 bstr(n);
+        FMDIndex::SAarr_t sampled;
+        sampled.reserve(n / FMDIndex::SA_SAMPLE_RATE + B[0] + 1);
+        for (int64_t i = 0; i < n; ++i)
+            if (i % FMDIndex::SA_SAMPLE_RATE == 0 || bwt[i] == 0) { bstr.set(i); sampled.push_back(sa[i]); }
+        std::stringstream ss;
+        ss.write((const char*)B.data(), sizeof(B));
+        ss.write((const char*)C.data(), sizeof(C));
+        {
+            WaveletTreeRRR wt(bwt, 0, DNAalphabet::NT16_MAX, FMDIndex::RRR_SAMPLE_RATE);
+            wt.save(ss);
+        }
+        FMDIndex::GAParr_t gap(sa, sa + B[0]);
+        EGriceLab::StringUtils::saveString(gap, ss);
+        BitSeqRRR(bstr, FMDIndex::RRR_SAMPLE_RATE).save(ss);
+        EGriceLab::StringUtils::saveString(sampled, ss);
+        FMDIndex* f = new FMDIndex();
+        f->load(ss);
+        if (!ss || f->length() != n) { delete f; return nullptr; }
+        return f;
+    } catch (...) {
+        return nullptr;
+    }
+}
+
+}  // extern "C"
+
+namespace {
+
+/* SMEM_LIST::findSeeds (src/SMEM.cpp:271-305) with the reference's own SMEM search, FMDIndex
+ * (RRR wavelet-tree ranks, LF-walk accessSA with the per-read SA cache) and
+ * SeedPair::removeRedundant; only the BD-coordinate lookups (MetaGenome::getTid / getLoc /
+ * getStrand, whose source needs Boost) come from the oracle's restatement.  The SMEM's
+ * SA start rows are read from its formatted output, the reference's public face for them. */
+std::vector<orc::Seed> refSeeds(const FMDIndex* f, const orc::Db& db, const orc::Params& P, const orc::Read& r) {
+    PrimarySeq seq(DNAseq(r.seq.data(), r.seq.data() + r.L), r.name);
+    SMEM_LIST smems = SMEM_LIST::findAllSMEMS(&seq, nullptr, f, P.minLen, P.maxEvalue);
+    std::sort(smems.begin(), smems.end(), [](const SMEM& l, const SMEM& rr) { return l.loglik() < rr.loglik(); });
+    SeedList all;
+    SMEM::SAmap_t cache;
+    const bool cached = smems.size() > 1;
+    char buf[256];
+    for (const SMEM& m : smems) {
+        std::ostringstream os;
+        os << std::setprecision(17) << m;
+        long long from, to, size, fs, rs;
+        std::snprintf(buf, sizeof buf, "%s", os.str().c_str());
+        if (std::sscanf(buf, "%lld-%lld:%lld:%lld:%lld:", &from, &to, &size, &fs, &rs) != 5) continue;
+        const int64_t len = to - from;
+        for (int64_t i = 0; i < size; ++i) {
+            for (int side = 0; side < 2; ++side) {
+                const int64_t row = (side == 0 ? fs : rs) + i;
+                int64_t bd;
+                if (cached) {
+                    auto it = cache.find(row);
+                    bd = it != cache.end() ? it->second : (cache[row] = f->accessSA(row));
+                } else {
+                    bd = f->accessSA(row);
+                }
+                const int64_t tid = orc::getTid(db, bd);
+                if (orc::getStrand(db, tid, bd) != orc::FWD) continue;
+                const int64_t start = orc::getLoc(db, tid, bd);
+                all.push_back(SeedPair(side == 0 ? from : r.L - to, start, len, tid,
+                                       side == 0 ? GLoc::FWD : GLoc::REV, m.loglik()));
+            }
+        }
+        if ((int64_t)all.size() >= P.maxNSeed) break;
+    }
+    SeedPair::removeRedundant(all);
+    std::vector<orc::Seed> out;
+    out.reserve(all.size());
+    for (const SeedPair& s : all)
+        out.push_back(orc::Seed{s.getFrom(), s.getTo(), s.getStart(), s.getEnd(), s.getTid(), (int32_t)s.getStrand(),
+                                s.loglik()});
+    return out;
+}
+
+/* SeedChain::getChains (src/SeedChain.cpp:40-73), the reference's own */
+std::vector<std::vector<orc::Seed>> refChains(const std::vector<orc::Seed>& seeds, int64_t maxMM, int64_t maxIndel) {
+    SeedList sl;
+    sl.reserve(seeds.size());
+    for (const orc::Seed& s : seeds)
+        sl.push_back(SeedPair(s.from, s.start, s.to - s.from, s.tid, (GLoc::STRAND)s.strand, s.logP));
+    ChainList chains = SeedChain::getChains(sl, maxMM, maxIndel);
+    std::vector<std::vector<orc::Seed>> out;
+    out.reserve(chains.size());
+    for (const SeedChain& ch : chains) {
+        std::vector<orc::Seed> v;
+        for (const SeedPair& s : ch)
+            v.push_back(orc::Seed{s.getFrom(), s.getTo(), s.getStart(), s.getEnd(), s.getTid(), (int32_t)s.getStrand(),
+                                  s.loglik()});
+        out.push_back(std::move(v));
+    }
+    return out;
+}
+
+}  // namespace
+
+extern "C" {
+
+/* The CPU baseline: msgseqtk-align's per-read (per-pair) tasks with the reference's own
+ * seeding and chaining over `fmd` (ref_index_from_sa) and the oracle's DP, evaluation, MAPQ,
+ * pairing and records over the oracle database `orcDb` (orc_db_create of liborc.so); OpenMP
+ * over tasks.  Arguments and result as orc_map (oracle/orc_capi.cpp); `params` is the
+ * msgs_params block converted by orc_params_convert. */
+int64_t ref_baseline_map(void* fmd, void* orcDb, const void* params, const uint8_t* seq, const uint8_t* qual,
+                         const int64_t* offs, const char* names, const uint8_t* seq2, const uint8_t* qual2,
+                         const int64_t* offs2, const char* names2, int64_t n, int32_t nThreads, char* out,
+                         int64_t cap, int64_t* lineOffs, double* seconds) {
+    const FMDIndex* f = (const FMDIndex*)fmd;
+    const orc::Db& db = *(const orc::Db*)orcDb;
+    const orc::Params& P = *(const orc::Params*)params;
+    orc::SeedFn sf = [f, &db, &P](const orc::Read& r) { return refSeeds(f, db, P, r); };
+    orc::ChainFn cf = refChains;
+    return orc::mapBatch(db, P, seq, qual, offs, names, seq2, qual2, offs2, names2, n, nThreads, out, cap, lineOffs,
+                         seconds, &sf, &cf);
+}
+
+}  // extern "C"

@@ -105,6 +105,9 @@ def orc():
                                         PD, PD, C.c_char_p, C.c_char_p, C.c_int64]
         lib.orc_align_chain.restype = C.c_int64
         lib.orc_calc_mapq.argtypes = [PD, C.c_int64, PD, PU8]
+        lib.orc_params_convert.argtypes = [C.POINTER(Params), C.c_void_p]
+        lib.orc_count_ext.argtypes = [C.c_void_p, PU8, P64, C.c_int64, C.c_int64, C.c_double]
+        lib.orc_count_ext.restype = C.c_int64
         _orc = lib
     return _orc
 
@@ -148,6 +151,11 @@ def ref():
         lib.ref_revcom.argtypes = [PU8, C.c_int64, PU8]
         lib.ref_primary_revcom.argtypes = [C.c_char_p, C.c_char_p, PU8, PU8]
         lib.ref_primary_revcom.restype = C.c_int64
+        lib.ref_index_from_sa.restype = C.c_void_p
+        lib.ref_index_from_sa.argtypes = [PU8, P64, C.c_int64]
+        lib.ref_baseline_map.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, PU8, PU8, P64, C.c_char_p, PU8, PU8, P64,
+                                         C.c_char_p, C.c_int64, C.c_int32, C.c_char_p, C.c_int64, P64, PD]
+        lib.ref_baseline_map.restype = C.c_int64
         _ref = lib
     return _ref
 
@@ -257,11 +265,16 @@ def orc_map(db, params, reads, quals, names, mates=None, nthreads=1):
 
 
 class RefIndex:
-    """The reference's own FMDIndex built by its own divsufsort from a BD text."""
+    """The reference's own FMDIndex built by its own divsufsort from a BD text, or (sa given)
+    from its own components over that suffix array (ref_index_from_sa)."""
 
-    def __init__(self, bd):
+    def __init__(self, bd, sa=None):
         bd = np.ascontiguousarray(bd, dtype=np.uint8)
-        self.h = ref().ref_index_build(ptr(bd, C.c_uint8), len(bd))
+        if sa is None:
+            self.h = ref().ref_index_build(ptr(bd, C.c_uint8), len(bd))
+        else:
+            sa = np.ascontiguousarray(sa, np.int64)
+            self.h = ref().ref_index_from_sa(ptr(bd, C.c_uint8), ptr(sa, C.c_int64), len(bd))
         if not self.h:
             raise ValueError("reference FMDIndex refused the text")
         self.n = len(bd)
@@ -341,3 +354,35 @@ def orc_calc_mapq(log10p):
     mq = np.zeros(len(v), np.uint8)
     orc().orc_calc_mapq(ptr(v, C.c_double), len(v), ptr(pp, C.c_double), ptr(mq, C.c_uint8))
     return pp, mq
+
+
+def ref_baseline_map(rix, db, params, reads, quals, names, mates=None, nthreads=1):
+    """msgseqtk-align's tasks with the reference's own seeding and chaining (RefIndex rix) and the
+    oracle's DP and records (OracleDb db): SAM text per read (pair) and the mapping seconds."""
+    seq, offs = pack_reads(reads)
+    qual, _ = pack_reads(quals)
+    n = len(reads)
+    nm = "\n".join(names).encode()
+    if mates is not None:
+        seq2, offs2 = pack_reads(mates[0])
+        qual2, _ = pack_reads(mates[1])
+        nm2 = "\n".join(mates[2]).encode()
+        a2 = (ptr(seq2, C.c_uint8), ptr(qual2, C.c_uint8), ptr(offs2, C.c_int64), nm2)
+    else:
+        a2 = (None, None, None, None)
+    pbuf = C.create_string_buffer(1024)
+    orc().orc_params_convert(C.byref(params), pbuf)
+    secs = np.zeros(1, np.float64)
+    args = (rix.h, db.h, pbuf, ptr(seq, C.c_uint8), ptr(qual, C.c_uint8), ptr(offs, C.c_int64), nm) + a2
+    need = ref().ref_baseline_map(*args, n, nthreads, None, 0, None, ptr(secs, C.c_double))
+    buf = C.create_string_buffer(int(need) + 1)
+    lo = np.zeros(n + 1, np.int64)
+    ref().ref_baseline_map(*args, n, nthreads, buf, need + 1, ptr(lo, C.c_int64), ptr(secs, C.c_double))
+    text = buf.raw[:need].decode()
+    return [text[lo[i]:lo[i + 1]] for i in range(n)], float(secs[0])
+
+
+def orc_count_ext(db, reads, minLen=17, maxE=0.01):
+    """FMD extensions (backExt/fwdExt with an unambiguous base) of findAllSMEMS over the reads"""
+    seq, offs = pack_reads(reads)
+    return int(orc().orc_count_ext(db.h, ptr(seq, C.c_uint8), ptr(offs, C.c_int64), len(reads), minLen, maxE))

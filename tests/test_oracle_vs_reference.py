@@ -131,3 +131,30 @@ def test_chains_match_reference():
         k1 = O.orc().orc_get_chains(O.ptr(rows, O.C.c_int64), len(rows), mm, ind, O.ptr(o1, O.C.c_int64), 4096)
         k2 = O.ref().ref_get_chains(O.ptr(rows, O.C.c_int64), len(rows), mm, ind, O.ptr(o2, O.C.c_int64), 4096)
         assert k1 == k2 and np.array_equal(o1[:k1], o2[:k2])
+
+
+@pytest.mark.skipif(not O.ref_available(), reason="oracle/_ref not built (reference sources absent)")
+def test_reference_index_from_suffix_array_and_baseline_sam():
+    """The reference's FMDIndex assembled from its own components over a given suffix array
+    (ref_index_from_sa, the CPU baseline's index) answers like the one its constructor builds
+    with divsufsort; the CPU baseline (the reference's own seeding and chaining + the oracle's
+    DP and records) writes the oracle's SAM, single-end and paired."""
+    chroms, names = synth.metagenome(6, 40000, gc=(0.3, 0.7), seed=77)
+    odb = O.OracleDb(chroms, names)
+    bd, bwt, sa = odb.arrays()
+    r1 = O.RefIndex(bd, sa)
+    r2 = O.RefIndex(bd)
+    seqs, quals, rn, _ = synth.simulate_reads(chroms, 600, 120, sub=0.015, indel=0.002, seed=78)
+    for r in seqs[:150]:
+        a, b = r1.smems(r), r2.smems(r)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1].view(np.int64), b[1].view(np.int64))
+    for i in range(0, len(bd), 997):
+        assert O.ref().ref_access_sa(r1.h, i) == sa[i]
+    exp, _ = O.orc_map(odb, O.default_params(), seqs, quals, rn, nthreads=4)
+    got, _ = O.ref_baseline_map(r1, odb, O.default_params(), seqs, quals, rn, nthreads=4)
+    assert exp == got
+    (a1, q1, n1), (a2, q2, n2) = synth.simulate_pairs_fast(chroms, 300, 120, insert=(300, 40), seed=79, offtarget=0.05)
+    pp = O.pe_params(300, 40)
+    exp, _ = O.orc_map(odb, pp, a1, q1, n1, mates=(a2, q2, n2), nthreads=4)
+    got, _ = O.ref_baseline_map(r1, odb, pp, a1, q1, n1, mates=(a2, q2, n2), nthreads=4)
+    assert exp == got
