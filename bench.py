@@ -1,12 +1,21 @@
 #!/usr/bin/env python3
-"""Mapped reads/s of the MI355X mapping path on the paired-end workload of 64 genomes of
-4 Mb (256 Mbp, GC ~ U(0.3, 0.7)), 2x150 bp pairs with insert N(350, 50), 0.5%
-substitutions, 0.05% indels and 5% off-target pairs, in batches of 256k reads
-(128k pairs) resident in HBM.  A step maps one batch end to end (SMEMs, seeds,
-chains, DP + backtrace on the GPU, pairing/MAPQ/SAM records on the host).
+"""Mapped reads/s of the MI355X mapping path, learned-index form against the plain FMD form,
+with bit-exact SAM.
 
-Prints one JSON line (see --help).  The CPU baseline is the oracle restatement of the
-reference's path timed on the host cores over a bounded sample of the same pairs.
+Default workload (the learned-index configuration): 256 genomes of 4 Mb (1 Gbp, GC ~
+U(0.3, 0.7)), 20% of them near-duplicate strains at 1-5% divergence from a parent; 4M
+single-end 150 bp reads at 1% error (0.9% substitutions, 0.1% 1-3 bp indels), Phred U(20, 40),
+in 15 resident batches of 256k reads (3.9M reads).  `--workload pairs` runs the paired
+configuration instead (64 genomes of 4 Mb, 1M pairs 2x150, insert N(350, 50), 5% off-target
+pairs).
+
+A step maps one batch end to end (SMEMs, seeds, chains, DP + backtrace on the GPU; MAPQ,
+pairing and SAM records on the host) with the batch's reads already in HBM; batches are
+pipelined (several in flight on the worker streams).  Prints one JSON line.  The CPU baseline
+is the reference's own seeding and chaining code (RRR wavelet-tree ranks, LF-walk suffix-array
+lookups; compiled from its sources into oracle/_ref) with the oracle's DP and records, timed
+on the host cores over a bounded sample of the same reads (median of 3 runs); the oracle
+restatement with a plain occ table is reported beside it.
 """
 import argparse
 import collections
@@ -24,6 +33,19 @@ METRIC = "mapped reads/s on 1 MI355X, learned-index vs plain FMD, bit-exact SAM;
 HBM_PEAK_GBS = 8000.0
 # integer VALU peak of one MI355X: 256 CUs x 64 lanes x 2.4 GHz (lane operations per second)
 VALU_PEAK_LANE_OPS = 256 * 64 * 2.4e9
+# integer operations one DP cell needs as written (src/Alignment.cpp:81-100): the substitution
+# score (compare + select, 2), M = max of three diagonal predecessors + score (3), I and D = max
+# of a gap open and a gap extension (2 subtractions + 1 max each, 6), the -inf clamps of those
+# four steps (4), the backtrace decisions (matchMax 4 compares/selects, insMax 1, delMax 1, the
+# M > 0 test 1, packing 4: 11), the row/column indexing of the anti-diagonal step (2)
+DP_OPS_PER_CELL = 28
+
+WORKLOADS = {
+    "strains": "SE 150 bp at 1% error: 256 genomes x 4 Mb (1 Gbp, GC U(0.3,0.7)), 20% near-duplicate strains "
+               "at 1-5% divergence; 15 batches of 256k reads resident in HBM",
+    "pairs": "PE 2x150: 64 genomes x 4 Mb (256 Mbp, GC U(0.3,0.7)), insert N(350,50), 0.5% sub, 0.05% indel, "
+             "5% off-target pairs; 1M pairs in 256k-read batches resident in HBM",
+}
 
 
 def log(*a):
@@ -35,20 +57,21 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=24)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--genomes", type=int, default=64)
-    ap.add_argument("--genome-len", type=int, default=4_000_000)
-    ap.add_argument("--pairs", type=int, default=1_000_000)
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="strains")
+    ap.add_argument("--reads", type=int, default=0, help="reads of the workload (0: 4M single-end / 2M paired)")
     ap.add_argument("--batch-reads", type=int, default=262144)
-    ap.add_argument("--cpu-pairs", type=int, default=40000, help="pairs in the CPU-baseline sample")
+    ap.add_argument("--cpu-reads", type=int, default=20000, help="reads in the CPU-baseline sample")
+    ap.add_argument("--cpu-runs", type=int, default=3, help="CPU-baseline runs (the median is reported)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-clean", action="store_true", help="skip the msgseqtk-clean seeding measurement")
     ap.add_argument("--no-compare", action="store_true", help="skip the plain-FMD comparison run")
     ap.add_argument("--no-tune", action="store_true", help="keep the default lanes per CU")
     ap.add_argument("--no-torch", action="store_true", help="no torch device synchronisation (profiler runs)")
     ap.add_argument("--no-pipeline", action="store_true", help="finish each batch before submitting the next")
+    ap.add_argument("--no-ingest", action="store_true", help="skip the run with the per-batch host-to-device copy")
     ap.add_argument("--host-threads", type=int, default=0, help="host finishing threads (0: the library default)")
     ap.add_argument("--form", type=int, default=2,
-                    help="0 plain FMD extensions, 1 unique-interval text path, 2 form 1 + learned-index read filter")
+                    help="0 plain FMD extensions, 1 unique-interval text path, 2 form 1 + learned index")
     ap.add_argument("--lix-segments", type=int, default=1 << 20, help="learned index: root segments")
     ap.add_argument("--lix-steps", type=int, default=1000, help="learned index: training steps")
     ap.add_argument("--no-gather", action="store_true", help="skip the random-gather ceiling measurement")
@@ -66,6 +89,26 @@ def parse():
     return ap.parse_args()
 
 
+def make_data(a, synth):
+    """(chroms, names, batches, paired); a batch is (reads[, mates]) of (seqs, quals, names)"""
+    if a.workload == "strains":
+        chroms, names = synth.config_genomes(4)
+        nreads = a.reads or 4_000_000
+        per = a.batch_reads
+        nb = max(1, min((nreads + per - 1) // per, 15))   # slot 15 is the CPU-baseline check's
+        batches = [(synth.simulate_reads_fast(chroms, per, 150, sub=0.009, indel=0.001, seed=0xC0FFEE + 1004 + b,
+                                              prefix="b%dr" % b),) for b in range(nb)]
+        return chroms, names, batches, False
+    chroms, names = synth.config_genomes(2)
+    nreads = a.reads or 2_000_000
+    per = a.batch_reads // 2
+    nb = max(1, min((nreads // 2 + per - 1) // per, 15))
+    batches = [synth.simulate_pairs_fast(chroms, per, 150, insert=(350, 50), sub=0.005, indel=0.0005,
+                                         seed=0xC0FFEE + 1002 + b, offtarget=0.05, prefix="b%dp" % b)
+               for b in range(nb)]
+    return chroms, names, batches, True
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -76,19 +119,14 @@ def main():
     from msgseqtk_amd import synth
 
     t0 = time.time()
-    chroms, names = synth.metagenome(a.genomes, a.genome_len, gc=(0.3, 0.7), seed=0xC0FFEE + 2)
-    pairs_per_batch = a.batch_reads // 2
-    n_batches = max(1, min((a.pairs + pairs_per_batch - 1) // pairs_per_batch, 16))
-    batches = []
-    for b in range(n_batches):
-        batches.append(synth.simulate_pairs_fast(chroms, pairs_per_batch, 150, insert=(350, 50), sub=0.005,
-                                                 indel=0.0005, seed=0xC0FFEE + 1002 + b, offtarget=0.05,
-                                                 prefix="b%dp" % b))
-    log("data: %d genomes, %d batches of %d pairs in %.1fs" % (len(chroms), n_batches, pairs_per_batch,
+    chroms, names, batches, paired = make_data(a, synth)
+    n_batches = len(batches)
+    log("data: %d genomes, %d batches of %d reads in %.1fs" % (len(chroms), n_batches, a.batch_reads,
                                                               time.time() - t0))
     t1 = time.time()
     m = M.Mapper(0).build(chroms, names)
     build_s = time.time() - t1
+    log("index built on the GPU in %.1fs (BD length %d)" % (build_s, m.n))
     lix = None
     if a.form == 2:
         t3 = time.time()
@@ -96,11 +134,13 @@ def main():
         lix["train_s"] = round(time.time() - t3, 2)
         log("learned index trained in %.1fs: %s" % (lix["train_s"], lix))
     m.set("form", a.form)
-    log("index built on the GPU in %.1fs (BD length %d)" % (build_s, m.n))
-    for b, (r1, r2) in enumerate(batches):
-        m.upload(b, r1, r2)
-    p = M.default_params(insMean=350, insSD=50, insMin=350 - 10 * 50, insMax=350 + 10 * 50)
-    sync = lambda: None   # msgs_batch_run returns after its stream has drained
+    for b, bt in enumerate(batches):
+        m.upload(b, *bt)
+    if paired:
+        p = M.default_params(insMean=350, insSD=50, insMin=350 - 10 * 50, insMax=350 + 10 * 50)
+    else:
+        p = M.default_params()
+    sync = lambda: None   # msgs_batch_wait returns after the batch's stream has drained
     if not a.no_torch:
         try:
             import torch
@@ -110,62 +150,54 @@ def main():
             pass
 
     tuned = None
-    if a.lanes_per_cu:
-        m.set("lanes_per_cu", a.lanes_per_cu)
-    if a.group_lanes >= 0:
-        m.set("group_lanes", a.group_lanes)
-    if a.text_path >= 0:
-        m.set("text_path", a.text_path)
-    if a.read_filter >= 0:
-        m.set("read_filter", a.read_filter)
-    if a.split_lanes >= 0:
-        m.set("split_lanes", a.split_lanes)
-    if a.pair_chains >= 0:
-        m.set("pair_chains", a.pair_chains)
+    for key, v in (("lanes_per_cu", a.lanes_per_cu), ("group_lanes", a.group_lanes), ("text_path", a.text_path),
+                   ("read_filter", a.read_filter), ("split_lanes", a.split_lanes), ("pair_chains", a.pair_chains)):
+        if (key == "lanes_per_cu" and v) or (key != "lanes_per_cu" and v >= 0):
+            m.set(key, v)
     if not a.no_tune and not a.lanes_per_cu:
         lanes, tus = m.tune(0, p)
-        tuned = {"lanes_per_cu": lanes, "group_lanes": m.get("group_lanes"), "text_path": m.get("text_path"),
-                 "read_filter": m.get("read_filter"), "split_lanes": m.get("split_lanes"),
-                 "pair_chains": m.get("pair_chains"), "smem_us": round(tus, 1)}
-        log("autotuner: %d lanes per CU, %d lanes per read (%.0f us for the SMEM stage of batch 0)" % (
-            lanes, tuned["group_lanes"], tus))
+        tuned = {k: m.get(k) for k in ("lanes_per_cu", "group_lanes", "text_path", "read_filter", "split_lanes",
+                                       "pair_chains")}
+        tuned["smem_us"] = round(tus, 1)
+        log("autotuner: %s" % tuned)
     m.set("streams", a.streams)
     run_steps.serial = a.no_pipeline
     run_steps.depth = a.depth
     if a.host_threads:
         m.set("host_threads", a.host_threads)
+    m.set("ingest", 0)
     elapsed, S = run_steps(m, p, n_batches, a.steps, a.warmup, sync, log)
     reads, mapped = S[:, 0].sum(), S[:, 1].sum()
     value = mapped / elapsed
     ext_calls, sa_lookups, cells = S[:, 2].sum(), S[:, 3].sum(), S[:, 4].sum()
     smem_us, dp_us = S[:, 5].sum(), S[:, 6].sum()
-    # dominant kernel: SMEM search; algorithmic bytes = one 64-B occ block per rank side
-    # (2 per extension with an unambiguous base)
-    launches = a.steps  # one SMEM launch per step: the mates follow the reads in one batch
+    # dominant kernel: the SMEM search; algorithmic bytes = one 64-B occ block per rank side
+    # (2 per extension with an unambiguous base) + one 64-B line per SA/ISA lookup + 1 B per
+    # text step of the unique-interval path
+    launches = a.steps
     fast_steps, lookups = S[:, 10].sum(), S[:, 11].sum()
     bytes_per_launch = (64.0 * 2.0 * ext_calls + 64.0 * lookups + fast_steps) / launches
     avg_us = smem_us / launches
     achieved = bytes_per_launch / (avg_us * 1e-6) / 1e9
+    kname = smem_kernel_name(m, a.form)
     log("timed %d steps in %.2fs: %.0f mapped reads/s (%.1f%% mapped)" % (a.steps, elapsed, value,
                                                                           100 * mapped / reads))
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "mapped reads/s", "n_gpus": 1,
         "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(1000 * elapsed / a.steps, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32+fp64",
         "data": "synthetic",
-        "config": {"workload": "PE 2x150: 64 genomes x 4 Mb (256 Mbp, GC U(0.3,0.7)), insert N(350,50), "
-                               "0.5% sub, 0.05% indel, 5% off-target pairs; 256k-read batches in HBM",
-                   "batch_reads": 2 * pairs_per_batch, "bd_length": int(m.n),
+        "config": {"workload": WORKLOADS[a.workload], "batch_reads": a.batch_reads, "resident_batches": n_batches,
+                   "bd_length": int(m.n),
                    "form": ["plain FMD", "unique-interval text path",
                             "learned-index exact-read path and read filter + unique-interval text path"][a.form]},
-        "roofline": {"bound": "hbm", "kernel": smem_kernel_name(m, a.form), "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "bytes_per_launch": bytes_per_launch, "avg_launch_us": round(avg_us, 1)},
         "dp_gcups": round(cells / (dp_us * 1e-6) / 1e9, 3) if dp_us > 0 else None,
         "smem_iterations": {"lane_per_read": S[:, 12].sum() / reads,
                             "simd_utilisation": S[:, 12].sum() / max(1.0, 64.0 * S[:, 13].sum()),
-                            "max_wavefront": int(S[:, 14].max()),
-                            "us_per_iteration_of_slowest_wavefront": smem_us / a.steps / max(1.0, S[:, 14].mean())},
+                            "max_wavefront": int(S[:, 14].max())},
         "per_read": {"occ_extensions": ext_calls / reads, "text_steps": fast_steps / reads,
                      "smem_sa_isa_lookups": lookups / reads, "seed_sa_lookups": sa_lookups / reads,
                      "dp_cells": cells / reads},
@@ -173,27 +205,29 @@ def main():
                               "smem_kernel": smem_us / a.steps / 1000, "seed_stage": S[:, 8].sum() / a.steps / 1000,
                               "dp_kernel": dp_us / a.steps / 1000, "host_finish": S[:, 9].sum() / a.steps / 1000,
                               "whole": S[:, 7].sum() / a.steps / 1000},
-        "host_side_ms_per_step": dict(run_steps.split, **{
-            "gpu_stages_wall": S[:, 16].sum() / a.steps / 1000, "smem_wall": S[:, 17].sum() / a.steps / 1000,
-            "seeds_wall": S[:, 18].sum() / a.steps / 1000, "chains_wall": S[:, 19].sum() / a.steps / 1000,
-            "dp_wall": S[:, 20].sum() / a.steps / 1000,
-            "dp_pools_ready": S[:, 23].sum() / a.steps / 1000, "dp_records_back": S[:, 24].sum() / a.steps / 1000,
-            "dp_cigar_md_back": S[:, 25].sum() / a.steps / 1000}),
         "dp_jobs_per_step": {"all": float(S[:, 21].mean()), "wavefront_fill": float(S[:, 22].mean()),
-                             "serial_fill": float(S[:, 32].mean()), "serial_fill_cells": float(S[:, 33].mean()),
-                             "largest_serial_fill_cells": float(S[:, 34].max()),
-                             "full_last_row_scans": float(S[:, 35].mean())},
-        "host_finish_thread_ms_per_step": {k: S[:, 28 + i].sum() / a.steps / 1000 for i, k in
-                                           enumerate(("sam_text", "pairing", "mapq", "alignment_lists"))},
+                             "serial_fill": float(S[:, 32].mean())},
         "index_build_s": round(build_s, 2),
-        "learned_index": dict(lix or {}, **({"exact_path_reads_frac": float(S[:, 26].sum() / max(1.0, S[:, 0].sum())),
-                                              "filtered_reads_frac": float(S[:, 27].sum() / max(1.0, S[:, 0].sum()))}
+        "learned_index": dict(lix or {}, **({"exact_path_reads_frac": float(S[:, 26].sum() / max(1.0, reads)),
+                                              "filtered_reads_frac": float(S[:, 27].sum() / max(1.0, reads))}
                                              if lix else {})),
         "autotune": tuned,
         "mapped_frac": round(mapped / reads, 4),
     }
+    if not a.no_ingest:
+        # the PCIe-inclusive rate: every run first copies its batch's bases and qualities to HBM
+        # from pinned host memory (overlapped with other batches' stages)
+        m.set("ingest", 1)
+        el1, S1 = run_steps(m, p, n_batches, a.steps, a.warmup, sync, None)
+        m.set("ingest", 0)
+        out["with_ingest"] = {"value": round(S1[:, 1].sum() / el1, 1), "ms_per_step": round(1000 * el1 / a.steps, 3),
+                              "ingest_bytes_per_step": float(S1[:, 36].mean()),
+                              "ingest_ms_per_step": float(S1[:, 37].mean() / 1000)}
+        log("with per-batch ingest: %s" % out["with_ingest"])
     if a.form != 0 and not a.no_compare:
         # the plain-FMD form on the same batches, same process, same clock
+        saved = {k: m.get(k) for k in ("lanes_per_cu", "group_lanes", "text_path", "read_filter", "split_lanes",
+                                       "pair_chains")}
         m.set("form", 0)
         lanes0 = None
         if not a.no_tune:
@@ -206,56 +240,54 @@ def main():
                             "stage_ms_per_step": {"seed_stage": S0[:, 8].sum() / a.steps / 1000,
                                                   "dp_kernel": S0[:, 6].sum() / a.steps / 1000,
                                                   "host_finish": S0[:, 9].sum() / a.steps / 1000},
-                            "host_side_ms_per_step": {
-                                "gpu_stages_wall": S0[:, 16].sum() / a.steps / 1000,
-                                "smem_wall": S0[:, 17].sum() / a.steps / 1000,
-                                "seeds_wall": S0[:, 18].sum() / a.steps / 1000,
-                                "chains_wall": S0[:, 19].sum() / a.steps / 1000,
-                                "dp_wall": S0[:, 20].sum() / a.steps / 1000},
                             "occ_extensions_per_read": S0[:, 2].sum() / S0[:, 0].sum(), "lanes_per_cu": lanes0,
                             "pair_chains": pair0}
         out["speedup_vs_plain_fmd"] = round(value / v0, 3)
-        out["smem_stage_speedup_vs_plain_fmd"] = round((S0[:, 5].sum() + 0.0) / max(1.0, smem_us + S[:, 15].sum()), 3)
+        out["smem_stage_speedup_vs_plain_fmd"] = round(S0[:, 5].sum() / max(1.0, smem_us + S[:, 15].sum()), 3)
         log("plain FMD form: %.0f mapped reads/s; learned-index form / plain = %.3f" % (v0, value / v0))
         m.set("form", a.form)
-        if tuned:
-            m.set("lanes_per_cu", tuned["lanes_per_cu"]).set("group_lanes", tuned["group_lanes"])
-            m.set("text_path", tuned["text_path"]).set("read_filter", tuned["read_filter"])
-            m.set("split_lanes", tuned["split_lanes"]).set("pair_chains", tuned["pair_chains"])
-    tr, tr_src = measured_traffic(out["roofline"]["kernel"])
+        for k, v in saved.items():
+            m.set(k, v)
+    # HBM traffic and L2 hit rate of the same kernel on the same workload, from the committed
+    # rocprofv3 --pmc summaries (profiles/*.md, lines '<key>[<kernel>@<workload>]: <value>')
+    tag = "%s@%s" % (kname, a.workload)
+    tr, tr_src = profile_value("traffic_bytes_per_launch", tag)
     if tr is not None:
         out["roofline"]["traffic"] = tr
         out["roofline"]["traffic_source"] = tr_src
-    hr, hr_src = profile_value("l2_hit_rate", out["roofline"]["kernel"])
+        out["roofline"]["traffic_per_algorithmic_byte"] = round(tr / bytes_per_launch, 3)
+    hr, hr_src = profile_value("l2_hit_rate", tag)
     if hr is not None:
         out["roofline"]["l2_hit_rate"] = hr
         out["roofline"]["l2_hit_rate_source"] = hr_src
-    # DP: GCUPS against the integer VALU peak divided by the VALU lane operations our
-    # kernels issue per cell (SQ_INSTS_VALU x 64 over dp_fill_kernel + dp_trace_kernel,
-    # from a committed rocprofv3 --pmc summary)
-    ops, ops_src = profile_value("valu_lane_ops_per_cell", "dp")
-    if ops and out["dp_gcups"]:
-        ceil = VALU_PEAK_LANE_OPS / ops / 1e9
+    if out["dp_gcups"]:
+        ceil = VALU_PEAK_LANE_OPS / DP_OPS_PER_CELL / 1e9
         out["dp_roofline"] = {"bound": "valu", "achieved": out["dp_gcups"], "unit": "GCUPS",
                               "peak": round(ceil, 1), "frac": round(out["dp_gcups"] / ceil, 4),
-                              "valu_peak_lane_ops_per_s": VALU_PEAK_LANE_OPS, "lane_ops_per_cell": ops,
-                              "source": ops_src}
+                              "valu_peak_lane_ops_per_s": VALU_PEAK_LANE_OPS,
+                              "algorithmic_int_ops_per_cell": DP_OPS_PER_CELL}
+        ops, ops_src = profile_value("valu_lane_ops_per_cell", "dp@" + a.workload)
+        if ops:
+            out["dp_roofline"]["issued_lane_ops_per_cell"] = ops
+            out["dp_roofline"]["issued_source"] = ops_src
     if not a.no_gather:
-        # random 64-B line gathers over buffers the size of the occ table and of occ + SA + ISA:
-        # the random-access ceiling the SMEM kernel's lookups are held against
+        # random 64-B line gathers over a buffer of the size the chosen kernel reads: the occ
+        # table alone (occ-block extensions only) or occ + SA + ISA + BD text (unique-interval
+        # text path, SA/ISA lookups)
         occ_bytes = 64 * (int(m.n) // 128 + 1)
-        g = {}
-        for name, nb in (("occ_table", occ_bytes), ("occ_sa_isa", occ_bytes + 8 * int(m.n))):
-            nb = (nb + 63) // 64 * 64
-            us = m.random_gather(nb, 1 << 27)
-            g[name] = {"buffer_bytes": nb, "gbs": round(64.0 * (1 << 27) / us / 1e3, 1)}
+        text_path = lookups > 0 or fast_steps > 0
+        nb = occ_bytes + (9 * int(m.n) if text_path else 0)
+        nb = (nb + 63) // 64 * 64
+        us = m.random_gather(nb, 1 << 27)
+        g = {"buffer": "occ + SA + ISA + BD" if text_path else "occ table", "buffer_bytes": nb,
+             "gbs": round(64.0 * (1 << 27) / us / 1e3, 1)}
         out["roofline"]["random_gather_ceiling"] = g
-        out["roofline"]["frac_of_gather_ceiling"] = round(achieved / g["occ_sa_isa"]["gbs"], 4)
+        out["roofline"]["frac_of_gather_ceiling"] = round(achieved / g["gbs"], 4)
         log("random 64-B gather ceiling: %s" % g)
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(m, chroms, names, batches[0], p, a.cpu_pairs)
+        out["cpu_baseline"] = cpu_baseline(m, chroms, names, batches[0], p, paired, a.cpu_reads, a.cpu_runs)
     if not a.no_clean:
-        out["clean_seeding"] = clean_seeding(m, batches[0])
+        out["clean_seeding"] = clean_seeding(m, batches[0][0][0])
         log("clean seeding: %s" % out["clean_seeding"])
     if a.calib_copy:
         # a plain device copy of known size (k_calib_copy) to scale FETCH_SIZE / WRITE_SIZE
@@ -264,22 +296,15 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` as recorded in a committed rocprofv3 --pmc summary
-    (FETCH_SIZE and WRITE_SIZE, scaled by the same counters on a copy of known size):
-    a line 'traffic_bytes_per_launch[<kernel>]: <bytes>' in profiles/*.md"""
-    return profile_value("traffic_bytes_per_launch", kernel)
-
-
-def profile_value(key, kernel):
-    """A figure recorded in a committed rocprofv3 summary as a line '<key>[<kernel>]: <value>'
+def profile_value(key, tag):
+    """A figure recorded in a committed rocprofv3 summary as a line '<key>[<tag>]: <value>'
     in profiles/*.md; (value, file) or (None, None)"""
     import glob
     import re
     for f in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
         for line in open(f):
             mt = re.match(re.escape(key) + r"\[(.+?)\]:\s*([0-9.eE+-]+)", line.strip())
-            if mt and mt.group(1) == kernel:
+            if mt and mt.group(1) == tag:
                 return float(mt.group(2)), os.path.relpath(f, ROOT)
     return None, None
 
@@ -306,44 +331,35 @@ def run_steps(m, p, nb, steps, warmup, sync, log):
 
     def pipelined(n, timed):
         out, inflight = [], collections.deque()
-        ts = tw = 0.0
         for k in range(n):
             if len(inflight) >= depth:
-                t1 = time.time()
                 out.append(m.wait(inflight.popleft()))
-                tw += time.time() - t1
-            t0 = time.time()
             m.submit(k % nb, p, True)
-            ts += time.time() - t0
             inflight.append(k % nb)
             if not timed and log:
                 log("warmup step %d submitted" % k)
         while inflight:
-            t1 = time.time()
             out.append(m.wait(inflight.popleft()))
-            tw += time.time() - t1
-        return out, ts, tw
+        return out
 
     # warm-up at the timed depth over every resident batch, so that every buffer the
-    # pipeline cycles through (device pools of both worker streams, pinned result buffers,
+    # pipeline cycles through (device pools of the worker streams, pinned result buffers,
     # each slot's SAM text) exists before timing
     if warmup > 0:
         pipelined(max(warmup, depth + 1, min(nb, steps)), False)
     sync()
     t = time.time()
-    stats, tsub, twait = pipelined(steps, True)
+    stats = pipelined(steps, True)
     sync()
-    run_steps.split = {"submit_ms_per_step": 1000 * tsub / steps, "wait_ms_per_step": 1000 * twait / steps}
     return time.time() - t, np.array(stats)
 
 
-def clean_seeding(m, batch):
+def clean_seeding(m, seqs):
     """msgseqtk-clean's seeding (SMEM_LIST::findMEMS per read and the list's loglik,
-    msgs_mems_batch) over the first reads of the pairs of batch 0 against the same database:
-    reads/s on the host clock with the packing and upload of the reads and the copy back of
-    the MEM records included (best of 3)"""
+    msgs_mems_batch) over the reads of batch 0 against the same database: reads/s on the host
+    clock with the packing and upload of the reads and the copy back of the MEM records
+    included (best of 3)"""
     import msgseqtk_amd as M
-    seqs = batch[0][0]
     p = M.default_params()
     best, cnt = 1e30, None
     for _ in range(3):
@@ -355,25 +371,69 @@ def clean_seeding(m, batch):
             "timing": "host clock, reads packed and uploaded and MEM records copied back inside"}
 
 
-def cpu_baseline(m, chroms, names, batch, p, npairs):
-    """The oracle (CPU restatement of the reference path) over the first npairs pairs of
-    batch 0, OpenMP over all host cores; its SAM text is also compared with the GPU's."""
+def cpu_baseline(m, chroms, names, batch, p, paired, nreads, runs):
+    """The CPU baseline over the first reads (pairs) of batch 0, OpenMP over all host cores
+    (at most 16), median of `runs`: the reference's own seeding and chaining (oracle/_ref) with
+    the oracle's DP and records ("reference"), and the oracle restatement with a plain occ table
+    ("port").  Both SAM texts are compared with the GPU's; the oracle's count of FMD extensions
+    on the sample is compared with the plain-form kernel's counter."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
+    import msgseqtk_amd as M
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    _, _, sa = m.arrays()
+    cores = max(1, min(16, cores))
+    bd, _, sa = m.arrays()
+    t = time.time()
     odb = O.OracleDb(chroms, names, sa=sa)
-    del sa
-    (s1, q1, n1), (s2, q2, n2) = batch
-    k = min(npairs, len(s1))
-    po = O.pe_params(350, 50)
-    sam_cpu, secs = O.orc_map(odb, po, s1[:k], q1[:k], n1[:k], mates=(s2[:k], q2[:k], n2[:k]), nthreads=cores)
-    sam_gpu = m.map((s1[:k], q1[:k], n1[:k]), p, mates=(s2[:k], q2[:k], n2[:k]))
-    mapped = sum(1 for rec in sam_cpu for line in rec.splitlines() if not (int(line.split("\t")[1]) & 4))
-    log("cpu baseline: %d pairs in %.2fs on %d threads" % (k, secs, cores))
-    return {"value": round(mapped / secs, 1), "unit": "mapped reads/s", "cores": cores, "kind": "port",
-            "sample": "first %d pairs (%d reads) of batch 0, oracle restatement with full SA, OpenMP per pair" % (k, 2 * k),
-            "seconds": round(secs, 2), "sam_identical_to_gpu": sam_cpu == sam_gpu}
+    log("oracle index from the device suffix array in %.1fs" % (time.time() - t))
+    rix = None
+    if O.ref_available():
+        t = time.time()
+        rix = O.RefIndex(bd, sa)
+        log("reference FMDIndex (RRR wavelet tree, sampled SA) in %.1fs" % (time.time() - t))
+    del sa, bd
+    reads = batch[0]
+    mates = batch[1] if paired else None
+    k = min(nreads // (2 if paired else 1), len(reads[0]))
+    r1 = tuple(x[:k] for x in reads)
+    r2 = tuple(x[:k] for x in mates) if mates else None
+    po = O.pe_params(350, 50) if paired else O.default_params()
+    sam_gpu = m.map(r1, p, mates=r2)
+    nr = k * (2 if paired else 1)
+    res = {}
+    for kind in (("reference", "port") if rix is not None else ("port",)):
+        secs = []
+        for _ in range(runs):
+            if kind == "reference":
+                sam, s = O.ref_baseline_map(rix, odb, po, *r1, mates=r2, nthreads=cores)
+            else:
+                sam, s = O.orc_map(odb, po, *r1, mates=r2, nthreads=cores)
+            secs.append(s)
+        med = float(np.median(secs))
+        mapped = sum(1 for rec in sam for line in rec.splitlines() if not (int(line.split("\t")[1]) & 4))
+        res[kind] = {"value": round(mapped / med, 1), "seconds": [round(x, 3) for x in secs],
+                     "sam_identical_to_gpu": sam == sam_gpu}
+        log("cpu baseline (%s): %d reads, median %.2fs on %d threads" % (kind, nr, med, cores))
+    # FMD extensions: oracle count on the sample (first reads) against the plain kernel's counter
+    seqs = r1[0][:2000]
+    n_orc = O.orc_count_ext(odb, seqs, int(p.minLen), float(p.maxEvalue))
+    saved = {kk: m.get(kk) for kk in ("form", "group_lanes", "pair_chains")}
+    m.set("form", 0).set("group_lanes", 0).set("pair_chains", 0)
+    m.upload(15, (seqs, [np.full(len(x), 30, np.uint8) for x in seqs], ["x%d" % i for i in range(len(seqs))]))
+    m.run(15, M.default_params(minLen=p.minLen, maxEvalue=p.maxEvalue), False)
+    n_gpu = int(m.stats()[2])
+    for kk, v in saved.items():
+        m.set(kk, v)
+    main_kind = "reference" if "reference" in res else "port"
+    out = {"value": res[main_kind]["value"], "unit": "mapped reads/s", "cores": cores, "kind": main_kind,
+           "sample": "first %d %s of batch 0 (%d reads), median of %d runs, OpenMP per %s; %s" % (
+               k, "pairs" if paired else "reads", nr, runs, "pair" if paired else "read",
+               "the reference's own SMEM search, FMDIndex (RRR wavelet tree, LF-walk SA), SeedPair and SeedChain "
+               "compiled from its sources, with the oracle's dense double DP and records (Alignment.cpp needs "
+               "Eigen/htslib, absent)" if main_kind == "reference" else "oracle restatement, plain occ table, full SA"),
+           "sam_identical_to_gpu": res[main_kind]["sam_identical_to_gpu"], "runs": res,
+           "fmd_extensions_check": {"reads": len(seqs), "oracle": n_orc, "plain_kernel_counter": n_gpu}}
+    return out
 
 
 if __name__ == "__main__":

@@ -77,8 +77,10 @@ const char* msgs_last_error(msgs_ctx* ctx);
  *   the other, the two lists merged in the reference's order (default 0);
  * "lanes_per_cu" persistent lanes of the SMEM kernel per CU; "smem_cap" SMEM slots per
  * read before a regrow; "host_threads" threads of the host finishing step; "streams" how
- * many submitted batches run their device stages at once (1..4, default 4).  Results
- * never depend on these settings. */
+ * many submitted batches run their device stages at once (1..4, default 4); "ingest" 1 = a
+ * submitted run of a resident batch first copies the batch's bases and qualities to the device
+ * again from pinned host memory (streaming input; default 0).  Results never depend on these
+ * settings. */
 int msgs_ctx_set(msgs_ctx* ctx, const char* key, int64_t value);
 int msgs_ctx_get(msgs_ctx* ctx, const char* key, int64_t* value);
 
@@ -192,7 +194,9 @@ int msgs_align_batch(msgs_ctx* ctx, const msgs_reads* reads, const msgs_params* 
  * filter skipped, [28]..[31] host finishing thread-microseconds (summed over threads) in
  * SAM text, PE pairing, MAPQ and alignment lists, [32] DP jobs filled serially by one
  * lane, [33] their rectangle cells, [34] the largest one's cells, [35] DP jobs whose
- * GLOBAL last-row scan visits every column (n <= 48). */
+ * GLOBAL last-row scan visits every column (n <= 48), [36] bytes copied host-to-device by the
+ * run's ingest (context option "ingest" 1: the batch's bases and qualities from pinned host
+ * memory at the start of every run, then its reverse complements), [37] their microseconds. */
 int msgs_batch_upload(msgs_ctx* ctx, int32_t slot, const msgs_reads* reads, const msgs_reads* mates);
 int msgs_batch_run(msgs_ctx* ctx, int32_t slot, const msgs_params* p, int32_t emitSam);
 /* msgs_batch_run in two halves: submit returns at once; a worker thread of the slot runs

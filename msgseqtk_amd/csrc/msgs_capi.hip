@@ -52,13 +52,14 @@ bool Exec::create(unsigned streamFlags) {
     return hipStreamCreateWithFlags(&stream, streamFlags) == hipSuccess && hipEventCreate(&ev0) == hipSuccess &&
            hipEventCreate(&ev1) == hipSuccess && hipEventCreate(&ev2) == hipSuccess &&
            hipEventCreate(&ev3) == hipSuccess && hipEventCreate(&ev4) == hipSuccess &&
-           hipEventCreate(&ev5) == hipSuccess;
+           hipEventCreate(&ev5) == hipSuccess && hipEventCreate(&ev6) == hipSuccess &&
+           hipEventCreate(&ev7) == hipSuccess;
 }
 
 void Exec::destroy() {
     if (stream) (void)hipStreamSynchronize(stream);
     bufs.releaseAll();
-    for (hipEvent_t* e : {&ev0, &ev1, &ev2, &ev3, &ev4, &ev5})
+    for (hipEvent_t* e : {&ev0, &ev1, &ev2, &ev3, &ev4, &ev5, &ev6, &ev7})
         if (*e) { (void)hipEventDestroy(*e); *e = nullptr; }
     if (stream) { (void)hipStreamDestroy(stream); stream = nullptr; }
 }
@@ -804,6 +805,7 @@ extern "C" int msgs_ctx_set(msgs_ctx* c, const char* key, int64_t value) {
     }
     else if (k == "pair_chains") { if (value != 0 && value != 1) return fail(c, MSGS_E_ARG, "pair_chains: 0 or 1"); c->pairChains = (int)value; }
     else if (k == "streams") { if (value < 1 || value > kWorkExecs) return fail(c, MSGS_E_ARG, "streams in [1, 4]"); c->workStreams = (int)value; }
+    else if (k == "ingest") { if (value != 0 && value != 1) return fail(c, MSGS_E_ARG, "ingest: 0 or 1"); c->ingest = (int)value; }
     else return fail(c, MSGS_E_ARG, "unknown option " + k);
     return MSGS_OK;
 }
@@ -821,6 +823,7 @@ extern "C" int msgs_ctx_get(msgs_ctx* c, const char* key, int64_t* value) {
     else if (k == "split_lanes") *value = c->splitLanes;
     else if (k == "pair_chains") *value = c->pairChains;
     else if (k == "streams") *value = c->workStreams;
+    else if (k == "ingest") *value = c->ingest;
     else return fail(c, MSGS_E_ARG, "unknown option " + k);
     return MSGS_OK;
 }

@@ -103,6 +103,9 @@ struct Slot {
     HostBatch host[2];
     DevReads dev;
     uint8_t* dQual = nullptr;
+    /* the device batch's bases and qualities in pinned host memory (reads then mates), copied
+     * to the device again at the start of every run when the context option "ingest" is 1 */
+    PVec<uint8_t> inSeq, inQual;
     std::thread worker;
     bool pending = false;
     std::shared_ptr<BatchResult> res;
@@ -126,6 +129,7 @@ struct Slot {
 struct Exec {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr, ev4 = nullptr, ev5 = nullptr;
+    hipEvent_t ev6 = nullptr, ev7 = nullptr;   /* per-run ingest of a resident batch */
     BufPool bufs;
     double lastSmemMicros = 0;    /* the SMEM search kernel(s) of the last SMEM stage */
     double lastStageMicros = 0;   /* the whole SMEM stage (prefix sums, search, merge) */
@@ -149,6 +153,8 @@ struct msgs_ctx {
                            * to start early, searched by a lane-group launch first (0 = off) */
     int pairChains = 0;   /* one-lane search kernel: 1 = two lanes per read (forward and backward
                            * searches of findAllSMEMS on separate lanes, smem_pair_kernel) */
+    int ingest = 0;       /* resident batches: 1 = every run first copies the batch's reads and
+                           * qualities from pinned host memory and makes the reverse complements */
     Exec own;                      /* the caller's thread */
     Exec work[kWorkExecs];         /* submitted batches' device stages */
     int workStreams = 4;           /* how many run at once */

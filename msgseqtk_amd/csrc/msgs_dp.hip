@@ -55,29 +55,25 @@ __device__ __forceinline__ int32_t max3(int32_t a, int32_t b, int32_t c) { retur
  * lane (wave_shr:1) instead of an LDS permute; every lane of the wavefront runs it */
 __device__ __forceinline__ int32_t laneUp1(int32_t v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
 
-/* Decision bytes of a rectangle are stored strip by strip (strips of 64 rows, as the
- * fill sweeps them) and, inside a strip, diagonal by diagonal: cell (l, jj) of a strip
- * (row l, column jj) sits at (jj - l + rows - 1) * rows + l, so the cells a run of
- * backtrace M steps visits, (l, jj), (l-1, jj-1), ..., are consecutive bytes.  A strip
- * of `rows` rows and C columns takes (C + rows - 1) * rows bytes. */
-__device__ __forceinline__ int64_t stripBytes(int64_t rows, int64_t cols) {
+/* Backtrace decisions of a rectangle are packed per row and per sweep step: the fill sweeps
+ * a strip of `rows` (<= 64) rows and C columns in C + rows - 1 anti-diagonal steps, and the
+ * lane of row l decides cell (l, jj) at step t = jj + l.  Each lane packs the decisions of
+ * `spw` consecutive steps into one 64-bit word (GLOBAL: 16 steps x 4 bits; LOCAL: 8 steps x
+ * 8 bits, bit 4 being M > 0), and the lanes store their words side by side, so that every
+ * store of the strip is one contiguous run of rows x 8 bytes: cell (l, jj) is in word
+ * ((jj + l) / spw) * rows + l at bit ((jj + l) % spw) * bits.  Strips of 64 rows follow each
+ * other; a strip of `rows` rows takes ceil((C + rows - 1) / spw) * rows words. */
+__device__ __forceinline__ int decShift(int mode) { return mode == 0 ? 4 : 3; }   /* log2 of steps per word */
+__device__ __forceinline__ int decBits(int mode) { return mode == 0 ? 4 : 8; }
+__device__ __forceinline__ int64_t stripWords(int64_t rows, int64_t cols, int mode) {
+    const int sh = decShift(mode);
+    return ((cols + rows - 1 + (1 << sh) - 1) >> sh) * rows;
+}
+__device__ __forceinline__ int64_t stripBytes(int64_t rows, int64_t cols, int mode) {
     const int64_t full = rows >> 6, rem = rows & 63;
-    return full * 64 * (cols + 63) + (rem ? (cols + rem - 1) * rem : 0);
+    return 8 * (full * stripWords(64, cols, mode) + (rem ? stripWords(rem, cols, mode) : 0));
 }
 
-/* byte reads through the aligned 16-byte chunk holding each byte (buffer padded to a
- * multiple of 16 bytes) */
-struct Bytes16 {
-    uint4 w;
-    int64_t a = -1;
-    __device__ __forceinline__ uint32_t get(const uint8_t* base, int64_t idx) {
-        const int64_t c = idx & ~(int64_t)15;
-        if (c != a) { a = c; w = *(const uint4*)(base + c); }
-        const uint32_t k = (uint32_t)idx & 15u;
-        const uint32_t d = k < 8 ? (k < 4 ? w.x : w.y) : (k < 12 ? w.z : w.w);
-        return (d >> (8 * (k & 3))) & 0xffu;
-    }
-};
 
 /* sequential byte reads through the aligned 8-byte word holding each byte */
 struct Bytes8 {
@@ -99,11 +95,14 @@ struct Rc {
     __device__ int64_t idx(int32_t i, int32_t j) const { return off + (int64_t)(j - c0) * (r1 - r0 + 1) + (i - r0); }
     __device__ bool hasCol(int32_t j) const { return j >= c0 && j <= c1; }
     __device__ int64_t area() const { return ok ? (int64_t)(r1 - r0 + 1) * (c1 - c0 + 1) : 0; }
-    /* decision byte of cell (i, j) (strip layout, see stripBytes) */
-    __device__ int64_t bitIdx(int32_t i, int32_t j) const {
+    /* the word holding the decisions of cell (i, j) and their bit position (see stripBytes) */
+    __device__ int64_t decWord(int32_t i, int32_t j, int mode, int& shift) const {
         const int32_t ri = i - r0, jj = j - c0, s0 = ri & ~63, l = ri & 63;
         const int32_t rows = min(64, r1 - r0 + 1 - s0), C = c1 - c0 + 1;
-        return off + (int64_t)s0 * (C + 63) + (int64_t)(jj - l + rows - 1) * rows + l;
+        const int sh = decShift(mode);
+        const int32_t t = jj + l;
+        shift = (t & ((1 << sh) - 1)) * decBits(mode);
+        return (off >> 3) + (int64_t)(s0 >> 6) * stripWords(64, C, mode) + (int64_t)(t >> sh) * rows + l;
     }
 };
 
@@ -368,7 +367,7 @@ __global__ void __launch_bounds__(256) dp_setup_kernel(DevIndex ix, DpArgs a) {
         /* R5 only when both sides are non-empty (src/Alignment.cpp:57-58) */
         g.r5r1 = (int32_t)f.from; g.r5c1 = (int32_t)(f.start - tStart);
         if (!(0 < f.from && tStart < f.start)) { g.r5r1 = 0; g.r5c1 = 0; }
-        int64_t area = (g.r5r1 >= 1 && g.r5c1 >= 1) ? stripBytes(g.r5r1, g.r5c1) : 0;
+        int64_t area = (g.r5r1 >= 1 && g.r5c1 >= 1) ? stripBytes(g.r5r1, g.r5c1, a.mode) : 0;
         int64_t bord = (g.r5r1 >= 1 && g.r5c1 >= 1) ? (int64_t)g.r5r1 + g.r5c1 : 0;
         int32_t maxCols = g.r5r1 >= 1 ? g.r5c1 : 0;
         for (int k = 0; k < ns; ++k) {
@@ -380,7 +379,7 @@ __global__ void __launch_bounds__(256) dp_setup_kernel(DevIndex ix, DpArgs a) {
                 a.auxGap[cs + k] = area;
                 a.auxGapB[cs + k] = bord;
                 if (rows > 0 && cols > 0) {
-                    area += stripBytes(rows, cols);
+                    area += stripBytes(rows, cols, a.mode);
                     bord += rows + cols;
                     maxCols = max(maxCols, (int32_t)cols);
                 }
@@ -392,7 +391,7 @@ __global__ void __launch_bounds__(256) dp_setup_kernel(DevIndex ix, DpArgs a) {
         if (!(l.to < L && l.end < tEnd)) { g.r3r0 = g.L + 1; g.r3c0 = g.tLen + 1; }
         else {
             const int64_t rows = g.L - g.r3r0 + 1, cols = g.tLen - g.r3c0 + 1;
-            area += stripBytes(rows, cols);
+            area += stripBytes(rows, cols, a.mode);
             bord += rows + cols;
             maxCols = max(maxCols, (int32_t)cols);
         }
@@ -462,6 +461,11 @@ __global__ void __launch_bounds__(64) dp_fill_kernel(DevIndex ix, DpArgs a) {
                 const uint8_t qb = act ? q[i - 1] : (uint8_t)0;
                 int32_t pM = NEG, pI = NEG, pD = NEG;
                 const int32_t steps = C + rows - 1;
+                /* this strip's decision words, packed per lane over spw steps */
+                uint64_t* const dw = (uint64_t*)G.bits + (R.off >> 3) + (int64_t)(s0 >> 6) * stripWords(64, C, G.mode);
+                const int dsh = decShift(G.mode), dbits = decBits(G.mode);
+                const int32_t dmask = (1 << dsh) - 1;
+                uint64_t acc = 0;
                 for (int32_t t = 0; t < steps; ++t) {
                     /* up = (i-1, j): row above, one step ago; diagonal = (i-1, j-1): two steps ago */
                     int32_t uM = laneUp1(cM), uI = laneUp1(cI), uD = laneUp1(cD);
@@ -493,10 +497,9 @@ __global__ void __launch_bounds__(64) dp_fill_kernel(DevIndex ix, DpArgs a) {
                         const uint32_t dec = (uint32_t)matchMax(dM, dI, dD) |
                                              ((max(uM + G.o, NEG) > max(uI - G.ge, NEG)) ? 0u : 4u) |
                                              ((max(cM + G.o, NEG) > max(cD - G.ge, NEG)) ? 0u : 8u) |
-                                             (m > 0 ? 16u : 0u);
+                                             ((G.mode == 1 && m > 0) ? 16u : 0u);
                         cM = m; cI = nI; cD = nD;
-                        G.bits[R.off + (int64_t)s0 * (C + 63) + (int64_t)(t - 2 * lane + rows - 1) * rows + lane] =
-                            (uint8_t)dec;
+                        acc |= (uint64_t)dec << ((t & dmask) * dbits);
                         if (i == R.r1) G.border[R.boff + jj] = make_int4(m, nI, nD, 0);
                         if (jj == C - 1) G.border[R.boff + C + (i - R.r0)] = make_int4(m, nI, nD, 0);
                         if (more && lane == rows - 1) { sTM[jj + 1] = m; sTI[jj + 1] = nI; sTD[jj + 1] = nD; }
@@ -506,6 +509,12 @@ __global__ void __launch_bounds__(64) dp_fill_kernel(DevIndex ix, DpArgs a) {
                             if (m > nv) { nv = m; nc = j; nst = OP_M; }
                             if (nI > nv) { nv = nI; nc = j; nst = OP_I; }
                         }
+                    }
+                    /* every spw steps (and at the strip's end) the lanes store their words side
+                     * by side: one contiguous run of rows x 8 bytes */
+                    if ((t & dmask) == dmask || t == steps - 1) {
+                        if (act) dw[(int64_t)(t >> dsh) * rows + lane] = acc;
+                        acc = 0;
                     }
                 }
                 __syncthreads();
@@ -716,7 +725,8 @@ __global__ void __launch_bounds__(256, 4) dp_trace_kernel(DevIndex ix, DpArgs a)
         i = alnTo; j = alnEnd;
         Rc Rb;                  /* the rectangle of the previous step (valid when rbOk) */
         bool rbOk = false;
-        Bytes16 sDec;
+        uint64_t decW = 0;
+        int64_t decWi = -1;
         while (true) {
             /* inside a rectangle of a wavefront-filled job, decisions come from its bytes */
             bool inR = false;
@@ -724,7 +734,13 @@ __global__ void __launch_bounds__(256, 4) dp_trace_kernel(DevIndex ix, DpArgs a)
                 if (rbOk && i >= Rb.r0 && i <= Rb.r1 && j >= Rb.c0 && j <= Rb.c1) inR = true;
                 else rbOk = inR = cellRect(G, i, j, Rb);
             }
-            const uint32_t dec = inR ? sDec.get(G.bits, Rb.bitIdx(i, j)) : 0u;
+            uint32_t dec = 0u;
+            if (inR) {
+                int sft;
+                const int64_t wi = Rb.decWord(i, j, G.mode, sft);
+                if (wi != decWi) { decWi = wi; decW = ((const uint64_t*)G.bits)[wi]; }
+                dec = (uint32_t)(decW >> sft) & (G.mode == 0 ? 15u : 255u);
+            }
             const bool go_on = G.mode == 0 ? i > 0 : (s != OP_M || (inR ? (dec & 16u) != 0 : vM(G, i, j, T) > 0));
             if (!go_on || np >= pathCap) break;
             if (!inR && s == OP_M && G.match > 0 && i >= 0) {

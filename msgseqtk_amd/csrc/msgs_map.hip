@@ -997,6 +997,14 @@ extern "C" int msgs_batch_upload(msgs_ctx* c, int32_t slot, const msgs_reads* re
                 S.dev.seq, S.dQual, S.dev.offs, n, S.dev.rcSeq, S.dev.rcQual);
     }
     if (hipStreamSynchronize(c->x().stream) != hipSuccess) return fail(c, MSGS_E_HIP, "upload");
+    /* pinned copies for the per-run ingest */
+    S.inSeq.resize(S.dev.total + 1);
+    S.inQual.resize(S.dev.total + 1);
+    if (S.dev.total) {
+        std::memcpy(S.inSeq.data(), all->seq + all->offs[0], S.dev.total);
+        if (all->qual) std::memcpy(S.inQual.data(), all->qual + all->offs[0], S.dev.total);
+        else std::memset(S.inQual.data(), 30, S.dev.total);
+    }
     S.paired = mates != nullptr;
     S.used = true;
     return MSGS_OK;
@@ -1021,7 +1029,19 @@ static int batchGpu(msgs_ctx* c, Slot& S, std::chrono::steady_clock::time_point 
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "stats");
     CK(hipMemsetAsync(dStats, 0, 16 * 8, c->x().stream), "stats");
     std::shared_ptr<BatchResult> b = S.res;
-    double seedUs = 0, smemUs = 0, dpUs = 0, filterUs = 0;
+    double seedUs = 0, smemUs = 0, dpUs = 0, filterUs = 0, ingestUs = 0;
+    if (c->ingest && S.dev.total) {
+        /* the batch's bases and qualities from pinned host memory, then the reverse
+         * complements, on this worker's stream (other batches' stages run meanwhile) */
+        hipStream_t st = c->x().stream;
+        const int64_t n = S.dev.n;
+        CK(hipEventRecord(c->x().ev6, st), "event");
+        CK(hipMemcpyAsync(S.dev.seq, S.inSeq.data(), S.dev.total, hipMemcpyHostToDevice, st), "ingest bases");
+        CK(hipMemcpyAsync(S.dQual, S.inQual.data(), S.dev.total, hipMemcpyHostToDevice, st), "ingest qualities");
+        k_revcom<<<(unsigned)std::min<int64_t>((n + 3) / 4, 65536), 256, 0, st>>>(S.dev.seq, S.dQual, S.dev.offs, n,
+                                                                                  S.dev.rcSeq, S.dev.rcQual);
+        CK(hipEventRecord(c->x().ev7, st), "event");
+    }
     double wall[4] = {0, 0, 0, 0};   /* host wall time of the SMEM, seed, chain and DP stages */
     double dpPhase[5] = {0, 0, 0, 0, 0};
     int rc;
@@ -1052,6 +1072,11 @@ static int batchGpu(msgs_ctx* c, Slot& S, std::chrono::steady_clock::time_point 
     unsigned long long hs[16];
     CK(hipMemcpyAsync(hs, dStats, sizeof(hs), hipMemcpyDeviceToHost, c->x().stream), "stats");
     CK(hipStreamSynchronize(c->x().stream), "stats");
+    if (c->ingest && S.dev.total) {
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, c->x().ev6, c->x().ev7);
+        ingestUs = ms * 1000.0;
+    }
     double* st = S.stats;
     std::memset(st, 0, sizeof(S.stats));
     st[0] = (double)(S.host[0].view.n * (S.paired ? 2 : 1));
@@ -1072,6 +1097,8 @@ static int batchGpu(msgs_ctx* c, Slot& S, std::chrono::steady_clock::time_point 
     st[26] = (double)hs[8];
     st[27] = (double)hs[9];
     for (int k = 0; k < 4; ++k) st[32 + k] = (double)hs[10 + k];
+    st[36] = c->ingest ? 2.0 * (double)S.dev.total : 0.0;
+    st[37] = ingestUs;
     return MSGS_OK;
 }
 
