@@ -412,6 +412,7 @@ __global__ void __launch_bounds__(256) dp_setup_kernel(DevIndex ix, DpArgs a) {
 
 /* ------------------------------------------------------------------ wavefront fill */
 
+template <int MODE>
 __global__ void __launch_bounds__(64) dp_fill_kernel(DevIndex ix, DpArgs a) {
     __shared__ int4 sSG[FILL_NS];
     __shared__ int32_t sTM[FILL_COLS + 1], sTI[FILL_COLS + 1], sTD[FILL_COLS + 1];
@@ -462,8 +463,8 @@ __global__ void __launch_bounds__(64) dp_fill_kernel(DevIndex ix, DpArgs a) {
                 int32_t pM = NEG, pI = NEG, pD = NEG;
                 const int32_t steps = C + rows - 1;
                 /* this strip's decision words, packed per lane over spw steps */
-                uint64_t* const dw = (uint64_t*)G.bits + (R.off >> 3) + (int64_t)(s0 >> 6) * stripWords(64, C, G.mode);
-                const int dsh = decShift(G.mode), dbits = decBits(G.mode);
+                uint64_t* const dw = (uint64_t*)G.bits + (R.off >> 3) + (int64_t)(s0 >> 6) * stripWords(64, C, MODE);
+                constexpr int dsh = MODE == 0 ? 4 : 3, dbits = MODE == 0 ? 4 : 8;
                 const int32_t dmask = (1 << dsh) - 1;
                 uint64_t acc = 0;
                 for (int32_t t = 0; t < steps; ++t) {
@@ -485,7 +486,7 @@ __global__ void __launch_bounds__(64) dp_fill_kernel(DevIndex ix, DpArgs a) {
                          * value, so the largest predecessor decides and -inf stays -inf */
                         const int32_t mx = max3(dM, dD, dI);
                         int32_t m = mx == NEG ? NEG : mx + sc;
-                        if (G.mode == 1) m = max(m, 0);
+                        if (MODE == 1) m = max(m, 0);
                         /* the gap penalties are non-negative (the host refuses others), so a
                          * step from -inf lands at or below NEG and a clamp at NEG is addn / subn:
                          * nI = max(addn(uM, o), addn(uI, -ge)), nD likewise */
@@ -497,13 +498,13 @@ __global__ void __launch_bounds__(64) dp_fill_kernel(DevIndex ix, DpArgs a) {
                         const uint32_t dec = (uint32_t)matchMax(dM, dI, dD) |
                                              ((max(uM + G.o, NEG) > max(uI - G.ge, NEG)) ? 0u : 4u) |
                                              ((max(cM + G.o, NEG) > max(cD - G.ge, NEG)) ? 0u : 8u) |
-                                             ((G.mode == 1 && m > 0) ? 16u : 0u);
+                                             ((MODE == 1 && m > 0) ? 16u : 0u);
                         cM = m; cI = nI; cD = nD;
                         acc |= (uint64_t)dec << ((t & dmask) * dbits);
                         if (i == R.r1) G.border[R.boff + jj] = make_int4(m, nI, nD, 0);
                         if (jj == C - 1) G.border[R.boff + C + (i - R.r0)] = make_int4(m, nI, nD, 0);
                         if (more && lane == rows - 1) { sTM[jj + 1] = m; sTI[jj + 1] = nI; sTD[jj + 1] = nD; }
-                        if (G.mode == 1) {
+                        if (MODE == 1) {
                             if (swBetter(m, i, j, bv, bi, bj)) { bv = m; bi = i; bj = j; }
                         } else if (i == G.L) {
                             if (m > nv) { nv = m; nc = j; nst = OP_M; }
@@ -563,6 +564,9 @@ __global__ void __launch_bounds__(64) dp_fill_kernel(DevIndex ix, DpArgs a) {
         __syncthreads();
     }
 }
+
+template __global__ void dp_fill_kernel<0>(DevIndex, DpArgs);
+template __global__ void dp_fill_kernel<1>(DevIndex, DpArgs);
 
 /* ------------------------------------------------------------------ trace */
 

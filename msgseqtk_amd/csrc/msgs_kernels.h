@@ -199,8 +199,8 @@ struct DpArgs {
 };
 /* seeds, rectangles and sizes of each job */
 __global__ void dp_setup_kernel(DevIndex ix, DpArgs a);
-/* rectangle fills, one wavefront per listed job */
-__global__ void dp_fill_kernel(DevIndex ix, DpArgs a);
+/* rectangle fills, one wavefront per listed job (MODE 0 GLOBAL, 1 LOCAL) */
+template <int MODE> __global__ void dp_fill_kernel(DevIndex ix, DpArgs a);
 /* remaining fills, backtrace, CIGAR, MD and evaluate, one lane per job */
 __global__ void dp_trace_kernel(DevIndex ix, DpArgs a);
 

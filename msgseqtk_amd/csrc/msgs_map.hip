@@ -354,13 +354,18 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     if (nWave) {
         /* persistent: as many one-wavefront blocks per CU as are resident at once (blocks
          * started late would take large jobs of the first stride last) */
-        static const int fillPerCU = []() {
+        static const int fillPerCU[2] = {[]() {
             int nb = 0;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dp_fill_kernel, 64, 0) != hipSuccess || nb < 1) nb = 16;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dp_fill_kernel<0>, 64, 0) != hipSuccess || nb < 1) nb = 16;
             return nb;
-        }();
-        const int64_t blocks = std::min<int64_t>(nWave, (int64_t)std::max(1, c->nCU) * fillPerCU);
-        dp_fill_kernel<<<(unsigned)blocks, 64, 0, st>>>(c->db.view(), a);
+        }(), []() {
+            int nb = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dp_fill_kernel<1>, 64, 0) != hipSuccess || nb < 1) nb = 16;
+            return nb;
+        }()};
+        const int64_t blocks = std::min<int64_t>(nWave, (int64_t)std::max(1, c->nCU) * fillPerCU[p->mode ? 1 : 0]);
+        if (p->mode == 0) dp_fill_kernel<0><<<(unsigned)blocks, 64, 0, st>>>(c->db.view(), a);
+        else dp_fill_kernel<1><<<(unsigned)blocks, 64, 0, st>>>(c->db.view(), a);
     }
     dp_trace_kernel<<<gridFor(nJobs), 256, 0, st>>>(c->db.view(), a);
     CK(hipEventRecord(c->x().ev3, st), "event");

@@ -649,12 +649,16 @@ __global__ void __launch_bounds__(256) smem_merge_kernel(SmemArgs a, SmemRec* ou
 
 /* per read, P[0] = 0 and P[i+1] = P[i] + loglik(base i) (the gates' estimates) */
 __global__ void __launch_bounds__(256) smem_prefix_kernel(DevIndex ix, SmemArgs a, double* pref) {
-    /* one read per lane, summed in read order; each chunk of 16 sums per read is staged in
-     * LDS and stored by 16 lanes at a time, so that stores fill whole 128-byte lines */
+    /* one read per lane, summed in read order over chunks of 16 bases loaded as two 8-byte
+     * words (the read buffer is padded); each chunk's 16 sums per read are staged in LDS and
+     * stored by 16 lanes at a time, so that stores fill whole 128-byte lines */
     constexpr int kChunk = 16;
     __shared__ double sv[4][64][kChunk + 1];
     __shared__ int64_t sBase[4][64];
     __shared__ int32_t sLen[4][64];
+    __shared__ double sll[16];
+    if (threadIdx.x < 16) sll[threadIdx.x] = ix.ll[threadIdx.x];
+    __syncthreads();
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int64_t r0 = ((int64_t)blockIdx.x * 4 + w) * 64; r0 < a.nReads; r0 += (int64_t)gridDim.x * 4 * 64) {
         const int64_t r = r0 + lane;
@@ -664,21 +668,26 @@ __global__ void __launch_bounds__(256) smem_prefix_kernel(DevIndex ix, SmemArgs 
         const int32_t L = mine ? (int32_t)(a.offs[r + 1] - o) : -1;
         sBase[w][lane] = o + r;
         sLen[w][lane] = L;
+        if (mine) pref[o + r] = 0.0;   /* P[0] */
         int32_t Lmax = L;
         for (int d = 32; d >= 1; d >>= 1) Lmax = max(Lmax, __shfl_xor(Lmax, d, 64));
         double x = 0;
-        for (int32_t k0 = 0; k0 <= Lmax; k0 += kChunk) {
+        for (int32_t k0 = 0; k0 < Lmax; k0 += kChunk) {
+            /* bases k0 .. k0 + 15 of the read: sums P[k0 + 1] .. P[k0 + 16] */
+            uint64_t q[2] = {0, 0};
+            if (k0 < L) load16(a.seqs, o + k0, q[0], q[1]);
+#pragma unroll
             for (int t = 0; t < kChunk; ++t) {
-                const int32_t k = k0 + t;
-                if (k >= 1 && k <= L) x += ix.ll[a.seqs[o + k - 1] & 15];
-                sv[w][lane][t] = x;   /* P[k] */
+                const uint32_t b = (uint32_t)(q[t >> 3] >> (8 * (t & 7))) & 15u;
+                if (k0 + t < L) x += sll[b];
+                sv[w][lane][t] = x;
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             for (int j = lane; j < 64 * kChunk; j += 64) {
                 const int rr = j / kChunk, t = j % kChunk;
-                const int32_t k = k0 + t;
+                const int32_t k = k0 + t + 1;
                 if (k <= sLen[w][rr]) pref[sBase[w][rr] + k] = sv[w][rr][t];
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
