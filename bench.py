@@ -72,7 +72,7 @@ def parse():
     ap.add_argument("--host-threads", type=int, default=0, help="host finishing threads (0: the library default)")
     ap.add_argument("--form", type=int, default=2,
                     help="0 plain FMD extensions, 1 unique-interval text path, 2 form 1 + learned index")
-    ap.add_argument("--lix-segments", type=int, default=1 << 20, help="learned index: root segments")
+    ap.add_argument("--lix-segments", type=int, default=1 << 22, help="learned index: root segments")
     ap.add_argument("--lix-steps", type=int, default=1000, help="learned index: training steps")
     ap.add_argument("--no-gather", action="store_true", help="skip the random-gather ceiling measurement")
     ap.add_argument("--depth", type=int, default=8, help="batches in flight")

@@ -95,6 +95,20 @@ int msgs_db_build(msgs_ctx* ctx, const uint8_t* chromSeqs, const int64_t* chromL
  * src/MetaGenome.cpp:42-73, src/Genome.cpp:29-97, header src/util/ProgEnv.cpp:24-64);
  * the FMD index is rebuilt on the device from the sequences. */
 int msgs_db_load(msgs_ctx* ctx, const char* prefix);
+/* FMDIndex::load of a DB.fmd file (src/FMDIndex.cpp:153-161; with the program-info header of
+ * src/util/ProgEnv.cpp:24-64 when header is 1): the BWT symbol count *n and, into the buffers
+ * given (any may be NULL), the BWT decoded from its RRR wavelet tree, the counts B[256] / C[256],
+ * gapSA (*nGap values) and the sampled SA (*nSampled values).  Host only (no context); MSGS_E_DB
+ * for a malformed file.  msgs_db_load also reads DB.fmd when it exists and checks it against the
+ * index it builds from the sequences. */
+int msgs_fmd_read(const char* path, int32_t header, int64_t* n, uint8_t* bwt, int64_t* B256, int64_t* C256,
+                  int64_t* gapSA, int64_t* nGap, int64_t* saSampled, int64_t* nSampled);
+/* FMDIndex(bdSeq, true, 32) + save (src/FMDIndex.cpp:29-116,143-151) for the resident
+ * database: the reference's DB.fmd format (RRR wavelet tree of the BWT, gapSA, SA sampled at
+ * every 32nd row and every '\0' row), with the program-info header when header is 1. */
+int msgs_fmd_write(msgs_ctx* ctx, const char* path, int32_t header);
+/* the same file from host arrays: the BWT (n nt16 codes) and suffix array of a BD text */
+int msgs_fmd_write_arrays(const char* path, int32_t header, const uint8_t* bwt, const int64_t* sa, int64_t n);
 /* MetaGenome::numChroms / getChromName / getChrom(i).size() (src/MetaGenome.h:64-112) */
 int32_t msgs_db_nchrom(msgs_ctx* ctx);
 int msgs_db_chrom(msgs_ctx* ctx, int32_t tid, char* name, int64_t cap, int64_t* len);
@@ -112,6 +126,19 @@ int msgs_rank_batch(msgs_ctx* ctx, const uint8_t* sym, const int64_t* pos, int64
 /* FMDIndex::backExt / fwdExt (src/FMDIndex.cpp:577-610, src/FMDIndex.h:226-228):
  * pqs is n rows of (p, q, s), updated in place; ret[i] = new s. */
 int msgs_ext_batch(msgs_ctx* ctx, int64_t* pqs, const uint8_t* bases, int32_t forward, int64_t* ret, int64_t n);
+
+/* Pattern searches, one lane per pattern (patterns as reads: nt16 codes + offsets[n+1]);
+ * pqs[3i..3i+2] = the bi-interval (p, q, s) after the search.  mode 0 = FMDIndex::count's
+ * backward search from the last base (src/FMDIndex.cpp:163-176; count = max(s, 0)), mode 1 =
+ * FMDIndex::locateAllFwd's forward search from the first base (src/FMDIndex.cpp:529-551; the
+ * locations are SA[p .. p+s)), mode 2 = locateAllRev's backward search (src/FMDIndex.cpp:553-575;
+ * SA[q .. q+s)); modes 1 and 2 give s = 0 for an empty pattern or an ambiguous start base. */
+int msgs_search_batch(msgs_ctx* ctx, const uint8_t* seqs, const int64_t* offs, int64_t n, int32_t mode, int64_t* pqs);
+/* FMDIndex::accessSA (src/FMDIndex.cpp:520-527) of n suffix-array rows (0 <= row < length). */
+int msgs_sa_batch(msgs_ctx* ctx, const int64_t* rows, int64_t n, int64_t* out);
+/* MetaGenome::getSeq(tid) (src/MetaGenome.h:319-321): the chromosome's nt16 codes (*len bases,
+ * without the terminator); out may be NULL to ask the length. */
+int msgs_db_chrom_seq(msgs_ctx* ctx, int32_t tid, uint8_t* out, int64_t cap, int64_t* len);
 
 /* ---- seeding ----
  * SMEM_LIST::findAllSMEMS (src/SMEM.cpp:186-215), one read per lane.  Per read i,
@@ -176,6 +203,24 @@ int msgs_map_batch(msgs_ctx* ctx, const msgs_reads* reads, const msgs_reads* mat
 int msgs_align_batch(msgs_ctx* ctx, const msgs_reads* reads, const msgs_params* p, int64_t* counts,
                      int64_t* rows10, double* scores, char* cigars, int64_t cigCap, int64_t cap,
                      int64_t* needed);
+
+/* SeedChain::getChains (src/SeedChain.cpp:40-73) over caller seed lists: set i is
+ * rows6[seedOffs[i] .. seedOffs[i+1]) of (from, to, start, end, tid, strand), in SeedPair order
+ * (as SMEM_LIST::findSeeds returns them), chained with maxMismatch[i] / maxIndel[i].  Output:
+ * chainCounts[i] chains of set i; out holds, set after set and chain after chain, the chain's
+ * seed count followed by its seeds' indexes into the set; *needed = entries of out. */
+int msgs_chain_batch(msgs_ctx* ctx, const int64_t* rows6, const int64_t* seedOffs, int64_t nSets,
+                     const int64_t* maxMismatch, const int64_t* maxIndel, int64_t* chainCounts, int64_t* out,
+                     int64_t cap, int64_t* needed);
+/* Alignment::buildAlignments + evaluate (src/Alignment.cpp:420-427,446-489) over caller chains:
+ * read i's chains are chainOffs[i] .. chainOffs[i+1]; chain k's seeds are
+ * rows6[chainSeedOffs[k] .. chainSeedOffs[k+1]) (from, to, start, end, tid, strand; compatible and
+ * in order, inside the read and the chromosome; others return MSGS_E_ARG).  Per chain: rows10 as
+ * msgs_align_batch, the score, log10P (Alignment::evaluate), and the CIGAR and MD strings each
+ * ended by '\n' in cigars / mds (any output may be NULL); *needed = chains. */
+int msgs_dp_batch(msgs_ctx* ctx, const msgs_reads* reads, const int64_t* chainOffs, const int64_t* chainSeedOffs,
+                  const int64_t* rows6, const msgs_params* p, int64_t* rows10, double* scores, double* log10P,
+                  char* cigars, int64_t cigCap, char* mds, int64_t mdCap, int64_t* needed);
 
 /* ---- resident batches (bench): reads copied to HBM once, mapped many times ----
  * slot 0..15.  msgs_batch_run maps the resident batch and keeps the records on the

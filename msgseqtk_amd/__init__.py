@@ -27,7 +27,8 @@ EXPORTS = [
     "msgs_map_batch", "msgs_align_batch", "msgs_batch_upload", "msgs_batch_run",
     "msgs_batch_stats", "msgs_batch_sam", "msgs_util_sort_by_key",
     "msgs_batch_submit", "msgs_batch_wait", "msgs_tune", "msgs_util_calib_copy", "msgs_util_random_gather", "msgs_util_format_g6", "msgs_ctx_get", "msgs_lix_train", "msgs_lix_query", "msgs_lix_predict", "msgs_lix_grad", "msgs_lix_weights", "msgs_lix_knots",
-    "msgs_smem_batch_noalloc",
+    "msgs_smem_batch_noalloc", "msgs_search_batch", "msgs_sa_batch", "msgs_db_chrom_seq", "msgs_chain_batch",
+    "msgs_dp_batch", "msgs_fmd_read", "msgs_fmd_write", "msgs_fmd_write_arrays",
 ]
 N_LIX_PARAMS = 16 * 64 + 64 + 64 + 1
 
@@ -114,6 +115,15 @@ def lib():
     L.msgs_lix_knots.argtypes = [vp, PU64, P64, C.POINTER(C.c_int32)]
     L.msgs_smem_batch_noalloc.argtypes = [vp, vp, vp, C.c_int64, C.POINTER(Params), C.c_int32, vp, vp,
                                           C.POINTER(C.c_int32)]
+    L.msgs_search_batch.argtypes = [vp, PU8, P64, C.c_int64, C.c_int32, P64]
+    L.msgs_sa_batch.argtypes = [vp, P64, C.c_int64, P64]
+    L.msgs_db_chrom_seq.argtypes = [vp, C.c_int32, PU8, C.c_int64, P64]
+    L.msgs_chain_batch.argtypes = [vp, P64, P64, C.c_int64, P64, P64, P64, P64, C.c_int64, P64]
+    L.msgs_dp_batch.argtypes = [vp, C.POINTER(Reads), P64, P64, P64, C.POINTER(Params), P64, PD, PD, C.c_char_p,
+                                C.c_int64, C.c_char_p, C.c_int64, P64]
+    L.msgs_fmd_read.argtypes = [C.c_char_p, C.c_int32, P64, PU8, P64, P64, P64, P64, P64, P64]
+    L.msgs_fmd_write.argtypes = [vp, C.c_char_p, C.c_int32]
+    L.msgs_fmd_write_arrays.argtypes = [C.c_char_p, C.c_int32, PU8, P64, C.c_int64]
     _lib = L
     return L
 
@@ -249,6 +259,79 @@ class Mapper:
             self._chk(rc, "msgs_smem_batch")
             k = need.value
             return counts[:rb.n].copy(), rows[:k].copy(), lp[:k].copy()
+
+    def search(self, patterns, mode):
+        """bi-intervals (p, q, s) after msgs_search_batch's search (0 count, 1 fwd, 2 rev)"""
+        rb = ReadBatch(patterns)
+        out = np.zeros((max(rb.n, 1), 3), np.int64)
+        self._chk(lib().msgs_search_batch(self.h, _p(rb.seq, C.c_uint8), _p(rb.offs, C.c_int64), rb.n, mode,
+                                          _p(out, C.c_int64)), "msgs_search_batch")
+        return out[:rb.n]
+
+    def access_sa(self, rows):
+        rows = np.ascontiguousarray(rows, np.int64)
+        out = np.zeros(max(len(rows), 1), np.int64)
+        self._chk(lib().msgs_sa_batch(self.h, _p(rows, C.c_int64), len(rows), _p(out, C.c_int64)), "msgs_sa_batch")
+        return out[:len(rows)]
+
+    def chains(self, seed_sets, max_mm, max_indel):
+        """SeedChain::getChains of each list of seed rows: per set a list of index lists"""
+        rows = np.concatenate([np.asarray(s, np.int64).reshape(-1, 6) for s in seed_sets]) if seed_sets else \
+            np.zeros((0, 6), np.int64)
+        rows = np.ascontiguousarray(rows if len(rows) else np.zeros((1, 6), np.int64))
+        offs = np.zeros(len(seed_sets) + 1, np.int64)
+        offs[1:] = np.cumsum([len(s) for s in seed_sets])
+        mm = np.ascontiguousarray(max_mm, np.int64)
+        ind = np.ascontiguousarray(max_indel, np.int64)
+        counts = np.zeros(max(1, len(seed_sets)), np.int64)
+        need = C.c_int64(0)
+        cap = 4 * int(offs[-1]) + 16
+        while True:
+            out = np.zeros(cap, np.int64)
+            rc = lib().msgs_chain_batch(self.h, _p(rows, C.c_int64), _p(offs, C.c_int64), len(seed_sets),
+                                        _p(mm, C.c_int64), _p(ind, C.c_int64), _p(counts, C.c_int64),
+                                        _p(out, C.c_int64), cap, C.byref(need))
+            if rc == MSGS_E_CAP:
+                cap = need.value + 1
+                continue
+            self._chk(rc, "msgs_chain_batch")
+            break
+        res, k = [], 0
+        for i in range(len(seed_sets)):
+            cs = []
+            for _ in range(int(counts[i])):
+                n = int(out[k])
+                cs.append([int(x) for x in out[k + 1:k + 1 + n]])
+                k += 1 + n
+            res.append(cs)
+        return res
+
+    def dp(self, reads, quals, chains_per_read, params):
+        """msgs_dp_batch: chains_per_read[i] = list of chains, a chain = seed rows (from, to, start,
+        end, tid, strand); returns per chain (rows10, score, log10P, CIGAR, MD)"""
+        rb = ReadBatch(reads, quals)
+        chain_offs = np.zeros(len(reads) + 1, np.int64)
+        chain_offs[1:] = np.cumsum([len(c) for c in chains_per_read])
+        flat = [np.asarray(ch, np.int64).reshape(-1, 6) for cs in chains_per_read for ch in cs]
+        seed_offs = np.zeros(len(flat) + 1, np.int64)
+        seed_offs[1:] = np.cumsum([len(x) for x in flat])
+        rows = np.ascontiguousarray(np.concatenate(flat) if flat else np.zeros((1, 6), np.int64))
+        nj = len(flat)
+        r10 = np.zeros((max(nj, 1), 10), np.int64)
+        sc = np.zeros(max(nj, 1))
+        lp = np.zeros(max(nj, 1))
+        cap = 64 * 1024 + 1200 * max(nj, 1)
+        cig, md = C.create_string_buffer(cap), C.create_string_buffer(cap)
+        need = C.c_int64(0)
+        self._chk(lib().msgs_dp_batch(self.h, C.byref(rb.st), _p(chain_offs, C.c_int64), _p(seed_offs, C.c_int64),
+                                      _p(rows, C.c_int64), C.byref(params), _p(r10, C.c_int64), _p(sc, C.c_double),
+                                      _p(lp, C.c_double), cig, cap, md, cap, C.byref(need)), "msgs_dp_batch")
+        cl = cig.value.decode().split("\n")[:nj]
+        ml = md.value.decode().split("\n")[:nj]
+        return [(r10[k], sc[k], lp[k], cl[k], ml[k]) for k in range(nj)]
+
+    def fmd_write(self, path, header=True):
+        self._chk(lib().msgs_fmd_write(self.h, path.encode(), int(header)), "msgs_fmd_write")
 
     def mems(self, seqs, params):
         """SMEM_LIST::findMEMS of each read: (counts, rows5, logP, per-read list loglik)"""
@@ -451,3 +534,20 @@ def sort_by_key(keys):
     perm = np.arange(len(keys), dtype=np.int32)
     lib().msgs_util_sort_by_key(_p(keys, C.c_double), _p(perm, C.c_int32), len(keys))
     return perm
+
+
+def fmd_read(path, header=True):
+    """DB.fmd decoded: (bwt, B, C, gapSA, SAsampled) (msgs_fmd_read, host only)"""
+    n, ng, ns = C.c_int64(0), C.c_int64(0), C.c_int64(0)
+    rc = lib().msgs_fmd_read(path.encode(), int(header), C.byref(n), None, None, None, None, C.byref(ng), None,
+                             C.byref(ns))
+    if rc != 0:
+        raise RuntimeError("msgs_fmd_read failed (%d)" % rc)
+    bwt = np.zeros(max(n.value, 1), np.uint8)
+    B, Cc = np.zeros(256, np.int64), np.zeros(256, np.int64)
+    gap, smp = np.zeros(max(ng.value, 1), np.int64), np.zeros(max(ns.value, 1), np.int64)
+    rc = lib().msgs_fmd_read(path.encode(), int(header), C.byref(n), _p(bwt, C.c_uint8), _p(B, C.c_int64),
+                             _p(Cc, C.c_int64), _p(gap, C.c_int64), C.byref(ng), _p(smp, C.c_int64), C.byref(ns))
+    if rc != 0:
+        raise RuntimeError("msgs_fmd_read failed (%d)" % rc)
+    return bwt[:n.value], B, Cc, gap[:ng.value], smp[:ns.value]

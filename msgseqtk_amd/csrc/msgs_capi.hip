@@ -143,6 +143,37 @@ __global__ void k_ext(DevIndex ix, int64_t* pqs, const uint8_t* bases, int32_t f
     pqs[3 * i] = p; pqs[3 * i + 1] = q; pqs[3 * i + 2] = s;
 }
 
+/* the bi-interval after a pattern search: mode 0 = FMDIndex::count's backward search from the
+ * last base (src/FMDIndex.cpp:163-176), 1 = locateAllFwd's forward search from the first base
+ * (:529-551), 2 = locateAllRev's backward search (:553-575); modes 1 and 2 give s = 0 for an
+ * empty pattern or an ambiguous start base, as the reference returns no locations then */
+__global__ void k_search(DevIndex ix, const uint8_t* seqs, const int64_t* offs, int64_t n, int32_t mode,
+                         int64_t* pqs) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint8_t* pat = seqs + offs[i];
+    const int64_t L = offs[i + 1] - offs[i];
+    int64_t p = 0, q = 0, s = 0;
+    if (L > 0) {
+        const uint8_t b = mode == 1 ? pat[0] : pat[L - 1];
+        if (mode == 0 || !amb(b)) {
+            p = ix.C[b]; q = ix.C[comp16(b)]; s = ix.C[b + 1] - ix.C[b];
+            if (mode == 1) {
+                for (int64_t k = 1; k < L && dFwdExt(ix, p, q, s, pat[k]) > 0; ++k) continue;
+            } else {
+                for (int64_t k = L - 1; k > 0 && dBackExt(ix, p, q, s, pat[k - 1]) > 0; --k) continue;
+            }
+        }
+    }
+    pqs[3 * i] = p; pqs[3 * i + 1] = q; pqs[3 * i + 2] = s;
+}
+
+/* SA values of the given rows (FMDIndex::accessSA, src/FMDIndex.cpp:520-527) */
+__global__ void k_sa(DevIndex ix, const int64_t* rows, int64_t* out, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (int64_t)ix.sa[rows[i]];
+}
+
 static inline unsigned gridFor(int64_t n, int t = 256) { return (unsigned)std::max<int64_t>(1, (n + t - 1) / t); }
 
 /* ------------------------------------------------------------------ context */
@@ -358,6 +389,75 @@ int msgs_ext_batch(msgs_ctx* c, int64_t* pqs, const uint8_t* bases, int32_t fwd,
     (void)hipMemcpyAsync(pqs, dQ, 3 * n * 8, hipMemcpyDeviceToHost, st);
     if (ret) (void)hipMemcpyAsync(ret, dR, n * 8, hipMemcpyDeviceToHost, st);
     if (hipStreamSynchronize(st) != hipSuccess) return fail(c, MSGS_E_HIP, "ext kernel");
+    return MSGS_OK;
+}
+
+int msgs_search_batch(msgs_ctx* c, const uint8_t* seqs, const int64_t* offs, int64_t n, int32_t mode, int64_t* pqs) {
+    if (!c || !c->db.n) return fail(c, MSGS_E_STATE, "no database");
+    if (mode < 0 || mode > 2 || n < 0 || (n > 0 && (!seqs || !offs || !pqs))) return fail(c, MSGS_E_ARG, "bad arguments");
+    if (n == 0) return MSGS_OK;
+    const int64_t total = offs[n] - offs[0];
+    for (int64_t i = 0; i < n; ++i)
+        if (offs[i + 1] < offs[i]) return fail(c, MSGS_E_ARG, "pattern offsets must not decrease");
+    for (int64_t k = 0; k < total; ++k)
+        if (seqs[offs[0] + k] > 15) return fail(c, MSGS_E_ARG, "pattern bases must be nt16 codes");
+    hipError_t e = hipSuccess;
+    uint8_t* dS = c->x().bufs.get<uint8_t>("se.s", total + 1, e);
+    int64_t* dO = c->x().bufs.get<int64_t>("se.o", n + 1, e);
+    int64_t* dQ = c->x().bufs.get<int64_t>("se.q", 3 * n, e);
+    if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "search buffers");
+    std::vector<int64_t> o(n + 1);
+    for (int64_t i = 0; i <= n; ++i) o[i] = offs[i] - offs[0];
+    hipStream_t st = c->x().stream;
+    if (total) (void)hipMemcpyAsync(dS, seqs + offs[0], total, hipMemcpyHostToDevice, st);
+    (void)hipMemcpyAsync(dO, o.data(), (n + 1) * 8, hipMemcpyHostToDevice, st);
+    k_search<<<gridFor(n), 256, 0, st>>>(c->db.view(), dS, dO, n, mode, dQ);
+    (void)hipMemcpyAsync(pqs, dQ, 3 * n * 8, hipMemcpyDeviceToHost, st);
+    if (hipStreamSynchronize(st) != hipSuccess) return fail(c, MSGS_E_HIP, "search kernel");
+    return MSGS_OK;
+}
+
+int msgs_sa_batch(msgs_ctx* c, const int64_t* rows, int64_t n, int64_t* out) {
+    if (!c || !c->db.n) return fail(c, MSGS_E_STATE, "no database");
+    if (n < 0 || (n > 0 && (!rows || !out))) return fail(c, MSGS_E_ARG, "bad arguments");
+    for (int64_t i = 0; i < n; ++i)
+        if (rows[i] < 0 || rows[i] >= c->db.n) return fail(c, MSGS_E_ARG, "suffix-array row out of range");
+    if (n == 0) return MSGS_OK;
+    hipError_t e = hipSuccess;
+    int64_t* dR = c->x().bufs.get<int64_t>("sa.r", n, e);
+    int64_t* dO = c->x().bufs.get<int64_t>("sa.o", n, e);
+    if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "SA buffers");
+    hipStream_t st = c->x().stream;
+    (void)hipMemcpyAsync(dR, rows, n * 8, hipMemcpyHostToDevice, st);
+    k_sa<<<gridFor(n), 256, 0, st>>>(c->db.view(), dR, dO, n);
+    (void)hipMemcpyAsync(out, dO, n * 8, hipMemcpyDeviceToHost, st);
+    if (hipStreamSynchronize(st) != hipSuccess) return fail(c, MSGS_E_HIP, "SA kernel");
+    return MSGS_OK;
+}
+
+int msgs_fmd_write(msgs_ctx* c, const char* path, int32_t header) {
+    if (!c || !c->db.n) return fail(c, MSGS_E_STATE, "no database");
+    if (!path) return fail(c, MSGS_E_ARG, "no path");
+    try {
+        std::vector<uint8_t> bwt(c->db.n);
+        std::vector<int64_t> sa(c->db.n);
+        int rc = msgs_db_copy(c, nullptr, bwt.data(), sa.data());
+        if (rc) return rc;
+        rc = fmdWrite(path, header != 0, bwt.data(), sa.data(), c->db.n);
+        return rc ? fail(c, rc, std::string("cannot write ") + path) : MSGS_OK;
+    } catch (const std::bad_alloc&) {
+        return fail(c, MSGS_E_NOMEM, "host memory");
+    }
+}
+
+int msgs_db_chrom_seq(msgs_ctx* c, int32_t tid, uint8_t* out, int64_t cap, int64_t* len) {
+    if (!c || !c->db.n) return fail(c, MSGS_E_STATE, "no database");
+    if (tid < 0 || tid >= c->db.nChrom) return fail(c, MSGS_E_ARG, "bad chrom id");
+    const int64_t L = c->db.chromLen[tid];
+    if (len) *len = L;
+    if (!out) return MSGS_OK;
+    if (cap < L) return fail(c, MSGS_E_CAP, "chrom sequence capacity");
+    std::memcpy(out, c->db.tgtHost.data() + c->db.tgtOff[tid], L);
     return MSGS_OK;
 }
 
@@ -769,7 +869,23 @@ extern "C" int msgs_db_load(msgs_ctx* c, const char* prefix) {
     }
     std::string joined;
     for (size_t t = 0; t < names.size(); ++t) { joined += names[t]; joined += '\n'; }
-    return msgs_db_build(c, seqs.data(), lens.data(), (int32_t)lens.size(), joined.c_str());
+    int rc = msgs_db_build(c, seqs.data(), lens.data(), (int32_t)lens.size(), joined.c_str());
+    if (rc) return rc;
+    /* DB.fmd (FMDIndex::load, src/FMDIndex.cpp:153-161), when present: its counts and BWT must
+     * be those of the index just built from the sequences */
+    const std::string fmd = pre + ".fmd";
+    if (std::ifstream(fmd, std::ios::binary).good()) {
+        int64_t n = 0, B[256], C[256];
+        rc = msgs_fmd_read(fmd.c_str(), 1, &n, nullptr, B, C, nullptr, nullptr, nullptr, nullptr);
+        if (rc) return fail(c, MSGS_E_DB, "unreadable FMD-index file " + fmd);
+        if (n != c->db.n || std::memcmp(B, c->db.B, sizeof B) != 0 || std::memcmp(C, c->db.C, sizeof C) != 0)
+            return fail(c, MSGS_E_DB, "FMD-index file " + fmd + " does not match the database sequences");
+        std::vector<uint8_t> fb(n), db(n);
+        rc = msgs_fmd_read(fmd.c_str(), 1, &n, fb.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr);
+        if (rc == MSGS_OK) rc = msgs_db_copy(c, nullptr, db.data(), nullptr);
+        if (rc || fb != db) return fail(c, MSGS_E_DB, "FMD-index file " + fmd + " holds another BWT");
+    }
+    return MSGS_OK;
 }
 
 /* chrom names and lengths of the resident database (for record headers) */

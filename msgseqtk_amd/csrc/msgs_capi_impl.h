@@ -179,6 +179,8 @@ struct msgs_ctx {
 };
 
 int fail(msgs_ctx* c, int code, const std::string& msg);
+/* writes DB.fmd of a BD text with this BWT and suffix array (msgs_fmd.cpp) */
+int fmdWrite(const std::string& path, bool header, const uint8_t* bwt, const int64_t* sa, int64_t n);
 /* looks up the allocation-free scratch array's address on `device` (the current device) */
 int noallocInit(int device);
 bool evalueThreshold(double n, double maxE, double* T);

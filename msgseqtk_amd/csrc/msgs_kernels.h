@@ -117,6 +117,8 @@ struct ChainArgs {
     int32_t* chainLen;
     int32_t* chainSeeds;       /* seed index (relative to the read's seeds) */
     int32_t pass;
+    const int64_t* maxMM;      /* per read maxMismatch / maxIndel (NULL: ceil(L * .2) / ceil(L * .1)) */
+    const int64_t* maxIndel;
 };
 __global__ void chain_kernel(ChainArgs a);
 

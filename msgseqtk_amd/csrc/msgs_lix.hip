@@ -492,6 +492,7 @@ __global__ void __launch_bounds__(256) k_exact(LixView m, DevIndex ix, const uin
                 const int64_t run = fwdMatch(ix.bd, pos, seqs, off + s0, len);
                 if (run >= part) nPart++;
                 if (run == len) { nFull++; fRow = r; fPos = pos; }
+                if (nPart > 1) break;   /* not unique: the read goes to the search either way */
             }
         }
         const int32_t nSuffix = __shfl(nPart, lane + 1);

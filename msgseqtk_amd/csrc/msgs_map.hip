@@ -198,7 +198,8 @@ static int runSeeds(msgs_ctx* c, const DevReads& r, const msgs_params* p, const 
     return MSGS_OK;
 }
 
-static int runChains(msgs_ctx* c, const DevReads& r, const SeedStage& sd, ChainStage& out) {
+static int runChains(msgs_ctx* c, const DevReads& r, const SeedStage& sd, ChainStage& out,
+                     const int64_t* dMaxMM = nullptr, const int64_t* dMaxIndel = nullptr) {
     const int64_t n = r.n;
     hipStream_t st = c->x().stream;
     hipError_t e = hipSuccess;
@@ -206,6 +207,7 @@ static int runChains(msgs_ctx* c, const DevReads& r, const SeedStage& sd, ChainS
     ChainArgs a;
     std::memset(&a, 0, sizeof(a));
     a.offs = r.offs; a.nReads = n; a.seedOff = sd.dSeedOff; a.seedCnt = sd.dSeedCnt; a.seeds = sd.dSeeds;
+    a.maxMM = dMaxMM; a.maxIndel = dMaxIndel;
     a.stackNode = c->x().bufs.get<int32_t>("ch.node", seedTotal + 1, e);
     a.stackNext = c->x().bufs.get<int32_t>("ch.next", seedTotal + 1, e);
     a.seen = c->x().bufs.get<uint8_t>("ch.seen", seedTotal + 1, e);
@@ -800,6 +802,210 @@ extern "C" int msgs_align_batch(msgs_ctx* c, const msgs_reads* reads, const msgs
                                 int64_t* needed) {
     try {
         return impl_align_batch(c, reads, p, counts, rows10, scores, cigars, cigCap, cap, needed);
+    } catch (const std::bad_alloc&) {
+        return fail(c, MSGS_E_NOMEM, "host memory");
+    }
+}
+
+/* ------------------------------------------------------------------ caller seeds and chains */
+
+/* a seed row (from, to, start, end, tid, strand) as a SeedRec */
+static SeedRec seedOf(const int64_t* r) {
+    SeedRec s;
+    s.from = r[0]; s.to = r[1]; s.start = r[2]; s.end = r[3]; s.tid = (int32_t)r[4]; s.strand = (int32_t)r[5]; s.logP = 0;
+    return s;
+}
+
+/* device copies of per-read seed lists */
+static int uploadSeeds(msgs_ctx* c, const std::vector<SeedRec>& seeds, const std::vector<int64_t>& off, SeedStage& sd) {
+    const int64_t n = (int64_t)off.size() - 1;
+    hipError_t e = hipSuccess;
+    hipStream_t st = c->x().stream;
+    sd.seedOff = off;
+    sd.seedCnt.resize(n);
+    for (int64_t i = 0; i < n; ++i) sd.seedCnt[i] = (int32_t)(off[i + 1] - off[i]);
+    sd.dSeedOff = c->x().bufs.get<int64_t>("cs.off", n + 1, e);
+    sd.dSeedCnt = c->x().bufs.get<int32_t>("cs.cnt", n + 1, e);
+    sd.dSeeds = c->x().bufs.get<SeedRec>("cs.seeds", seeds.size() + 1, e);
+    if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "seed buffers");
+    CK(hipMemcpyAsync(sd.dSeedOff, off.data(), (n + 1) * 8, hipMemcpyHostToDevice, st), "seed offsets");
+    if (n) CK(hipMemcpyAsync(sd.dSeedCnt, sd.seedCnt.data(), n * 4, hipMemcpyHostToDevice, st), "seed counts");
+    if (!seeds.empty())
+        CK(hipMemcpyAsync(sd.dSeeds, seeds.data(), seeds.size() * sizeof(SeedRec), hipMemcpyHostToDevice, st), "seeds");
+    return MSGS_OK;
+}
+
+static int impl_chain_batch(msgs_ctx* c, const int64_t* rows6, const int64_t* seedOffs, int64_t nSets,
+                            const int64_t* maxMM, const int64_t* maxIndel, int64_t* chainCounts, int64_t* out,
+                            int64_t cap, int64_t* needed) {
+    if (!c) return MSGS_E_ARG;
+    if (nSets < 0 || (nSets > 0 && (!seedOffs || !maxMM || !maxIndel))) return fail(c, MSGS_E_ARG, "bad arguments");
+    if (nSets == 0) { if (needed) *needed = 0; return MSGS_OK; }
+    std::vector<int64_t> off(nSets + 1);
+    for (int64_t i = 0; i <= nSets; ++i) {
+        off[i] = seedOffs[i] - seedOffs[0];
+        if (i && (off[i] < off[i - 1] || off[i] - off[i - 1] > (1 << 24))) return fail(c, MSGS_E_ARG, "bad seed offsets");
+    }
+    if (off[nSets] > 0 && !rows6) return fail(c, MSGS_E_ARG, "no seed rows");
+    std::vector<SeedRec> seeds(off[nSets]);
+    for (int64_t k = 0; k < off[nSets]; ++k) seeds[k] = seedOf(rows6 + 6 * (seedOffs[0] + k));
+    SeedStage sd;
+    int rc = uploadSeeds(c, seeds, off, sd);
+    if (rc) return rc;
+    hipError_t e = hipSuccess;
+    hipStream_t st = c->x().stream;
+    DevReads r;
+    r.n = nSets;
+    r.offs = c->x().bufs.get<int64_t>("cs.roffs", nSets + 1, e);   /* lengths unused: limits given */
+    int64_t* dMM = c->x().bufs.get<int64_t>("cs.mm", nSets, e);
+    int64_t* dID = c->x().bufs.get<int64_t>("cs.id", nSets, e);
+    if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "chain limits");
+    CK(hipMemsetAsync(r.offs, 0, (nSets + 1) * 8, st), "offsets");
+    CK(hipMemcpyAsync(dMM, maxMM, nSets * 8, hipMemcpyHostToDevice, st), "maxMismatch");
+    CK(hipMemcpyAsync(dID, maxIndel, nSets * 8, hipMemcpyHostToDevice, st), "maxIndel");
+    ChainStage ch;
+    rc = runChains(c, r, sd, ch, dMM, dID);
+    if (rc) return rc;
+    std::vector<int32_t> cStart(ch.nJobs + 1), cLen(ch.nJobs + 1), cSeeds(ch.nChainSeeds + 1);
+    if (ch.nJobs) {
+        CK(hipMemcpyAsync(cStart.data(), ch.dChainStart, ch.nJobs * 4, hipMemcpyDeviceToHost, st), "chains");
+        CK(hipMemcpyAsync(cLen.data(), ch.dChainLen, ch.nJobs * 4, hipMemcpyDeviceToHost, st), "chains");
+    }
+    if (ch.nChainSeeds) CK(hipMemcpyAsync(cSeeds.data(), ch.dChainSeeds, ch.nChainSeeds * 4, hipMemcpyDeviceToHost, st), "chains");
+    CK(hipStreamSynchronize(st), "chain copy");
+    const int64_t total = ch.nJobs + ch.nChainSeeds;
+    if (needed) *needed = total;
+    if (total > cap || !out) return fail(c, MSGS_E_CAP, "chain output capacity");
+    int64_t k = 0;
+    for (int64_t i = 0; i < nSets; ++i) {
+        if (chainCounts) chainCounts[i] = ch.chainOff[i + 1] - ch.chainOff[i];
+        for (int64_t j = ch.chainOff[i]; j < ch.chainOff[i + 1]; ++j) {
+            out[k++] = cLen[j];
+            for (int32_t t = 0; t < cLen[j]; ++t) out[k++] = cSeeds[cStart[j] + t];
+        }
+    }
+    return MSGS_OK;
+}
+
+extern "C" int msgs_chain_batch(msgs_ctx* c, const int64_t* rows6, const int64_t* seedOffs, int64_t nSets,
+                                const int64_t* maxMismatch, const int64_t* maxIndel, int64_t* chainCounts,
+                                int64_t* out, int64_t cap, int64_t* needed) {
+    try {
+        return impl_chain_batch(c, rows6, seedOffs, nSets, maxMismatch, maxIndel, chainCounts, out, cap, needed);
+    } catch (const std::bad_alloc&) {
+        return fail(c, MSGS_E_NOMEM, "host memory");
+    }
+}
+
+static int impl_dp_batch(msgs_ctx* c, const msgs_reads* reads, const int64_t* chainOffs, const int64_t* chainSeedOffs,
+                         const int64_t* rows6, const msgs_params* p, int64_t* rows10, double* scores, double* log10P,
+                         char* cigars, int64_t cigCap, char* mds, int64_t mdCap, int64_t* needed) {
+    if (!c || !c->db.n) return fail(c, MSGS_E_STATE, "no database");
+    int rc = checkParams(c, p);
+    if (rc) return rc;
+    if (!reads || !chainOffs || !chainSeedOffs) return fail(c, MSGS_E_ARG, "bad arguments");
+    const int64_t n = reads->n;
+    const int64_t nJobs = chainOffs[n] - chainOffs[0];
+    if (needed) *needed = nJobs;
+    /* every chain is non-empty, its seeds are exact-length pairs inside the read and the
+     * chromosome, on one tid and strand, in order on both sequences (SeedChain's invariant,
+     * src/SeedPair.h:156-158); the DP's geometry relies on it */
+    std::vector<SeedRec> seeds;
+    std::vector<int64_t> seedOff(n + 1, 0), chainOff(n + 1, 0);
+    std::vector<int32_t> jobRead(nJobs), cStart(nJobs), cLen(nJobs), cSeeds;
+    int64_t j = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        const int64_t L = reads->offs[i + 1] - reads->offs[i];
+        seedOff[i] = (int64_t)seeds.size();
+        chainOff[i] = j;
+        if (chainOffs[i + 1] < chainOffs[i]) return fail(c, MSGS_E_ARG, "chain offsets must not decrease");
+        for (int64_t q = chainOffs[i]; q < chainOffs[i + 1]; ++q, ++j) {
+            const int64_t s0 = chainSeedOffs[q], s1 = chainSeedOffs[q + 1];
+            if (s1 <= s0) return fail(c, MSGS_E_ARG, "empty chain");
+            jobRead[j] = (int32_t)i;
+            cStart[j] = (int32_t)cSeeds.size();
+            cLen[j] = (int32_t)(s1 - s0);
+            for (int64_t k = s0; k < s1; ++k) {
+                const SeedRec sr = seedOf(rows6 + 6 * k);
+                if (sr.tid < 0 || sr.tid >= c->db.nChrom || (sr.strand != 1 && sr.strand != 2) || sr.from < 0 ||
+                    sr.to > L || sr.to <= sr.from || sr.end - sr.start != sr.to - sr.from || sr.start < 0 ||
+                    sr.end > c->db.chromLen[sr.tid])
+                    return fail(c, MSGS_E_ARG, "seed outside the read or the chromosome");
+                if (k > s0) {
+                    const SeedRec pr = seedOf(rows6 + 6 * (k - 1));
+                    if (pr.tid != sr.tid || pr.strand != sr.strand || pr.to > sr.from || pr.end > sr.start)
+                        return fail(c, MSGS_E_ARG, "chain seeds not compatible and ordered");
+                }
+                cSeeds.push_back((int32_t)(seeds.size() - seedOff[i]));
+                seeds.push_back(sr);
+            }
+        }
+    }
+    seedOff[n] = (int64_t)seeds.size();
+    chainOff[n] = j;
+    DevReads r;
+    rc = uploadReads(c, reads, "dq", r);
+    if (rc) return rc;
+    uint8_t* dQual = nullptr;
+    rc = uploadQual(c, reads, r, "dq.qual", &dQual);
+    if (rc) return rc;
+    SeedStage sd;
+    rc = uploadSeeds(c, seeds, seedOff, sd);
+    if (rc) return rc;
+    hipError_t e = hipSuccess;
+    hipStream_t st = c->x().stream;
+    ChainStage ch;
+    ch.nJobs = nJobs;
+    ch.chainOff = chainOff;
+    ch.nChainSeeds = (int64_t)cSeeds.size();
+    ch.dJobRead = c->x().bufs.get<int32_t>("dq.read", nJobs + 1, e);
+    ch.dChainStart = c->x().bufs.get<int32_t>("dq.start", nJobs + 1, e);
+    ch.dChainLen = c->x().bufs.get<int32_t>("dq.len", nJobs + 1, e);
+    ch.dChainSeeds = c->x().bufs.get<int32_t>("dq.seeds", cSeeds.size() + 1, e);
+    if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "chain buffers");
+    if (nJobs) {
+        CK(hipMemcpyAsync(ch.dJobRead, jobRead.data(), nJobs * 4, hipMemcpyHostToDevice, st), "chains");
+        CK(hipMemcpyAsync(ch.dChainStart, cStart.data(), nJobs * 4, hipMemcpyHostToDevice, st), "chains");
+        CK(hipMemcpyAsync(ch.dChainLen, cLen.data(), nJobs * 4, hipMemcpyHostToDevice, st), "chains");
+        CK(hipMemcpyAsync(ch.dChainSeeds, cSeeds.data(), cSeeds.size() * 4, hipMemcpyHostToDevice, st), "chains");
+    }
+    DpStage dp;
+    rc = runDp(c, r, dQual, p, sd, ch, dp, nullptr);
+    if (rc) return rc;
+    Finisher F{c->db, *p, dp};
+    std::string cig, md;
+    for (int64_t k = 0; k < nJobs; ++k) {
+        const AlnRec& a = dp.recs[k];
+        if (rows10) {
+            int64_t* o = rows10 + 10 * k;
+            o[0] = a.tid; o[1] = a.strand; o[2] = a.tStart; o[3] = a.tEnd; o[4] = a.alnFrom; o[5] = a.alnTo;
+            o[6] = a.alnStart; o[7] = a.alnEnd; o[8] = a.hazard; o[9] = a.cells;
+        }
+        if (scores) scores[k] = a.score == NEG ? -std::numeric_limits<double>::infinity() : (double)a.score;
+        if (log10P) log10P[k] = a.log10P;
+        cig += F.cigarOf(k);
+        cig += '\n';
+        md.append(dp.md.data() + dp.mdOff[k], (size_t)a.mdLen);
+        md += '\n';
+    }
+    if (cigars) {
+        if ((int64_t)cig.size() + 1 > cigCap) return fail(c, MSGS_E_CAP, "cigar capacity");
+        std::memcpy(cigars, cig.c_str(), cig.size() + 1);
+    }
+    if (mds) {
+        if ((int64_t)md.size() + 1 > mdCap) return fail(c, MSGS_E_CAP, "MD capacity");
+        std::memcpy(mds, md.c_str(), md.size() + 1);
+    }
+    return MSGS_OK;
+}
+
+extern "C" int msgs_dp_batch(msgs_ctx* c, const msgs_reads* reads, const int64_t* chainOffs,
+                             const int64_t* chainSeedOffs, const int64_t* rows6, const msgs_params* p, int64_t* rows10,
+                             double* scores, double* log10P, char* cigars, int64_t cigCap, char* mds, int64_t mdCap,
+                             int64_t* needed) {
+    try {
+        return impl_dp_batch(c, reads, chainOffs, chainSeedOffs, rows6, p, rows10, scores, log10P, cigars, cigCap, mds,
+                             mdCap, needed);
     } catch (const std::bad_alloc&) {
         return fail(c, MSGS_E_NOMEM, "host memory");
     }

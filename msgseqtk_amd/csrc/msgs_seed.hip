@@ -143,8 +143,8 @@ __global__ void __launch_bounds__(256) chain_kernel(ChainArgs a) {
          rid += (int64_t)gridDim.x * blockDim.x) {
         const int64_t L = a.offs[rid + 1] - a.offs[rid];
         /* maxMismatch = ceil(L * .2), maxIndel = ceil(L * .1) (src/msgseqtk-align.cpp:619-620) */
-        const int64_t maxMM = (int64_t)ceil((double)L * 0.2);
-        const int64_t maxIndel = (int64_t)ceil((double)L * 0.1);
+        const int64_t maxMM = a.maxMM ? a.maxMM[rid] : (int64_t)ceil((double)L * 0.2);
+        const int64_t maxIndel = a.maxIndel ? a.maxIndel[rid] : (int64_t)ceil((double)L * 0.1);
         const int32_t n = a.seedCnt[rid];
         const int64_t so = a.seedOff[rid];
         const SeedRec* s = a.seeds + so;

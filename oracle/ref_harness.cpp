@@ -155,6 +155,34 @@ int64_t ref_get_chains(const int64_t* rows6, int64_t n, int64_t maxMM, int64_t m
 
 }  // extern "C"
 
+/* ------------------------------------------------------------------ DB.fmd files */
+
+#include <fstream>
+
+extern "C" {
+
+/* FMDIndex::save of a reference index to `path` (the body of DB.fmd, no program header) */
+int32_t ref_index_save(void* h, const char* path) {
+    std::ofstream out(path, std::ios::binary);
+    ((FMDIndex*)h)->save(out);
+    return out ? 0 : -1;
+}
+/* FMDIndex::load of the body of a DB.fmd file written elsewhere */
+void* ref_index_load(const char* path, int64_t skip) {
+    try {
+        std::ifstream in(path, std::ios::binary);
+        in.seekg(skip);
+        FMDIndex* f = new FMDIndex();
+        f->load(in);
+        if (!in) { delete f; return nullptr; }
+        return f;
+    } catch (...) {
+        return nullptr;
+    }
+}
+
+}  // extern "C"
+
 /* ------------------------------------------------------------------ CPU baseline */
 
 #include <chrono>

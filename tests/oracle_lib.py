@@ -156,6 +156,10 @@ def ref():
         lib.ref_baseline_map.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, PU8, PU8, P64, C.c_char_p, PU8, PU8, P64,
                                          C.c_char_p, C.c_int64, C.c_int32, C.c_char_p, C.c_int64, P64, PD]
         lib.ref_baseline_map.restype = C.c_int64
+        lib.ref_index_save.argtypes = [C.c_void_p, C.c_char_p]
+        lib.ref_index_save.restype = C.c_int32
+        lib.ref_index_load.argtypes = [C.c_char_p, C.c_int64]
+        lib.ref_index_load.restype = C.c_void_p
         _ref = lib
     return _ref
 
