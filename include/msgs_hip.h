@@ -306,6 +306,9 @@ void msgs_util_format_g6(const float* v, int64_t n, char* out);
  * src/Alignment.h:629-633) as the library restates them for the device; runs on
  * the host.  perm must hold 0..n-1 on entry. */
 void msgs_util_sort_by_key(const double* keys, int32_t* perm, int64_t n);
+/* Alignment::calcMapQ (src/Alignment.cpp:491-512) and, pe = 1, AlignmentPE::calcMapQ (:514-539)
+ * over n log-likelihoods (natural log): posterior probabilities and MAPQ; runs on the host. */
+void msgs_util_mapq(const double* loglik, int64_t n, int32_t pe, double* postP, uint8_t* mapQ);
 
 #ifdef __cplusplus
 }

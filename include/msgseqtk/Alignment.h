@@ -1,0 +1,2 @@
+/* Alignment.h -- the Alignment class of the drop-in face (msgseqtk_api.h) */
+#include "msgseqtk_api.h"

@@ -1,0 +1,2 @@
+/* DNAseq.h -- the DNAseq class of the drop-in face (msgseqtk_api.h) */
+#include "msgseqtk_api.h"

@@ -1,0 +1,2 @@
+/* FMDIndex.h -- the FMDIndex class of the drop-in face (msgseqtk_api.h) */
+#include "msgseqtk_api.h"

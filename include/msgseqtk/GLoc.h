@@ -1,0 +1,2 @@
+/* GLoc.h -- the GLoc class of the drop-in face (msgseqtk_api.h) */
+#include "msgseqtk_api.h"

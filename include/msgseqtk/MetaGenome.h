@@ -1,0 +1,2 @@
+/* MetaGenome.h -- the MetaGenome class of the drop-in face (msgseqtk_api.h) */
+#include "msgseqtk_api.h"

@@ -1,0 +1,2 @@
+/* PrimarySeq.h -- the PrimarySeq class of the drop-in face (msgseqtk_api.h) */
+#include "msgseqtk_api.h"

@@ -1,0 +1,2 @@
+/* SeedChain.h -- the SeedChain class of the drop-in face (msgseqtk_api.h) */
+#include "msgseqtk_api.h"

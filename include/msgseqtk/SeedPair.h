@@ -1,0 +1,2 @@
+/* SeedPair.h -- the SeedPair class of the drop-in face (msgseqtk_api.h) */
+#include "msgseqtk_api.h"

@@ -28,7 +28,7 @@ EXPORTS = [
     "msgs_batch_stats", "msgs_batch_sam", "msgs_util_sort_by_key",
     "msgs_batch_submit", "msgs_batch_wait", "msgs_tune", "msgs_util_calib_copy", "msgs_util_random_gather", "msgs_util_format_g6", "msgs_ctx_get", "msgs_lix_train", "msgs_lix_query", "msgs_lix_predict", "msgs_lix_grad", "msgs_lix_weights", "msgs_lix_knots",
     "msgs_smem_batch_noalloc", "msgs_search_batch", "msgs_sa_batch", "msgs_db_chrom_seq", "msgs_chain_batch",
-    "msgs_dp_batch", "msgs_fmd_read", "msgs_fmd_write", "msgs_fmd_write_arrays",
+    "msgs_dp_batch", "msgs_fmd_read", "msgs_fmd_write", "msgs_fmd_write_arrays", "msgs_util_mapq",
 ]
 N_LIX_PARAMS = 16 * 64 + 64 + 64 + 1
 
@@ -124,6 +124,8 @@ def lib():
     L.msgs_fmd_read.argtypes = [C.c_char_p, C.c_int32, P64, PU8, P64, P64, P64, P64, P64, P64]
     L.msgs_fmd_write.argtypes = [vp, C.c_char_p, C.c_int32]
     L.msgs_fmd_write_arrays.argtypes = [C.c_char_p, C.c_int32, PU8, P64, C.c_int64]
+    L.msgs_util_mapq.argtypes = [PD, C.c_int64, C.c_int32, PD, PU8]
+    L.msgs_util_mapq.restype = None
     _lib = L
     return L
 
@@ -551,3 +553,13 @@ def fmd_read(path, header=True):
     if rc != 0:
         raise RuntimeError("msgs_fmd_read failed (%d)" % rc)
     return bwt[:n.value], B, Cc, gap[:ng.value], smp[:ns.value]
+
+
+def calc_mapq(loglik, pe=False):
+    """Alignment::calcMapQ / AlignmentPE::calcMapQ (src/Alignment.cpp:491-540) over natural-log
+    likelihoods: the posterior probabilities and MAPQ values (run on the host)."""
+    ll = np.ascontiguousarray(loglik, np.float64)
+    pp = np.zeros(len(ll))
+    mq = np.zeros(len(ll), np.uint8)
+    lib().msgs_util_mapq(_p(ll, C.c_double), len(ll), 1 if pe else 0, _p(pp, C.c_double), _p(mq, C.c_uint8))
+    return pp, mq

@@ -807,6 +807,18 @@ extern "C" int msgs_align_batch(msgs_ctx* c, const msgs_reads* reads, const msgs
     }
 }
 
+/* Alignment::calcMapQ / AlignmentPE::calcMapQ on log-likelihoods (natural log) */
+extern "C" void msgs_util_mapq(const double* loglik, int64_t n, int32_t pe, double* postP, uint8_t* mapQ) {
+    if (n <= 0) return;
+    std::vector<double> ll(loglik, loglik + n), pr, pp;
+    std::vector<uint8_t> mq;
+    mapq(ll, pr, pp, mq, pe != 0);
+    for (int64_t i = 0; i < n; ++i) {
+        if (postP) postP[i] = pp[i];
+        if (mapQ) mapQ[i] = mq[i];
+    }
+}
+
 /* ------------------------------------------------------------------ caller seeds and chains */
 
 /* a seed row (from, to, start, end, tid, strand) as a SeedRec */
