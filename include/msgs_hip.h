@@ -79,8 +79,10 @@ const char* msgs_last_error(msgs_ctx* ctx);
  * read before a regrow; "host_threads" threads of the host finishing step; "streams" how
  * many submitted batches run their device stages at once (1..4, default 4); "ingest" 1 = a
  * submitted run of a resident batch first copies the batch's bases and qualities to the device
- * again from pinned host memory (streaming input; default 0).  Results never depend on these
- * settings. */
+ * again from pinned host memory (streaming input; default 0); "dp_serial" 1 = every DP job is
+ * filled by the serial path (all of M, I, D kept per cell) that otherwise takes only jobs with
+ * more than 64 seeds or a rectangle wider than 512 columns (default 0).  Results never depend
+ * on these settings. */
 int msgs_ctx_set(msgs_ctx* ctx, const char* key, int64_t value);
 int msgs_ctx_get(msgs_ctx* ctx, const char* key, int64_t* value);
 

@@ -142,6 +142,9 @@ def _p(a, t):
     return a.ctypes.data_as(C.POINTER(t))
 
 
+_RETRIES = 6   # capacity regrows of one call (each sizes from the reported need)
+
+
 class ReadBatch:
     """Reads packed for the C-ABI; keeps the numpy buffers alive."""
 
@@ -250,7 +253,7 @@ class Mapper:
         counts = np.zeros(max(rb.n, 1), np.int64)
         need = C.c_int64(0)
         cap = max(16, 8 * rb.n)
-        while True:
+        for _ in range(_RETRIES):
             rows = np.zeros((cap, 5), np.int64)
             lp = np.zeros(cap, np.float64)
             rc = lib().msgs_smem_batch(self.h, C.byref(rb.st), C.byref(params), _p(counts, C.c_int64),
@@ -261,6 +264,7 @@ class Mapper:
             self._chk(rc, "msgs_smem_batch")
             k = need.value
             return counts[:rb.n].copy(), rows[:k].copy(), lp[:k].copy()
+        raise RuntimeError("output capacity not reached after %d tries" % _RETRIES)
 
     def search(self, patterns, mode):
         """bi-intervals (p, q, s) after msgs_search_batch's search (0 count, 1 fwd, 2 rev)"""
@@ -288,7 +292,7 @@ class Mapper:
         counts = np.zeros(max(1, len(seed_sets)), np.int64)
         need = C.c_int64(0)
         cap = 4 * int(offs[-1]) + 16
-        while True:
+        for _ in range(_RETRIES):
             out = np.zeros(cap, np.int64)
             rc = lib().msgs_chain_batch(self.h, _p(rows, C.c_int64), _p(offs, C.c_int64), len(seed_sets),
                                         _p(mm, C.c_int64), _p(ind, C.c_int64), _p(counts, C.c_int64),
@@ -298,6 +302,7 @@ class Mapper:
                 continue
             self._chk(rc, "msgs_chain_batch")
             break
+        raise RuntimeError("output capacity not reached after %d tries" % _RETRIES)
         res, k = [], 0
         for i in range(len(seed_sets)):
             cs = []
@@ -342,7 +347,7 @@ class Mapper:
         rll = np.zeros(max(rb.n, 1), np.float64)
         need = C.c_int64(0)
         cap = max(16, 8 * rb.n)
-        while True:
+        for _ in range(_RETRIES):
             rows = np.zeros((cap, 5), np.int64)
             lp = np.zeros(cap, np.float64)
             rc = lib().msgs_mems_batch(self.h, C.byref(rb.st), C.byref(params), _p(counts, C.c_int64),
@@ -354,6 +359,7 @@ class Mapper:
             self._chk(rc, "msgs_mems_batch")
             k = need.value
             return counts[:rb.n].copy(), rows[:k].copy(), lp[:k].copy(), rll[:rb.n].copy()
+        raise RuntimeError("output capacity not reached after %d tries" % _RETRIES)
 
     def clean(self, background, seqs, params, min_lod, mates=None):
         """msgseqtk-clean's decision with this mapper's database as the reference and
@@ -378,7 +384,7 @@ class Mapper:
         counts = np.zeros(max(rb.n, 1), np.int64)
         need = C.c_int64(0)
         cap = max(16, 8 * rb.n)
-        while True:
+        for _ in range(_RETRIES):
             rows = np.zeros((cap, 6), np.int64)
             lp = np.zeros(cap, np.float64)
             rc = lib().msgs_seed_batch(self.h, C.byref(rb.st), C.byref(params), _p(counts, C.c_int64),
@@ -389,26 +395,33 @@ class Mapper:
             self._chk(rc, "msgs_seed_batch")
             k = need.value
             return counts[:rb.n].copy(), rows[:k].copy(), lp[:k].copy()
+        raise RuntimeError("output capacity not reached after %d tries" % _RETRIES)
 
     def alignments(self, seqs, quals, params):
         rb = ReadBatch(seqs, quals)
         counts = np.zeros(max(rb.n, 1), np.int64)
         need = C.c_int64(0)
         cap = max(16, 4 * rb.n)
-        while True:
+        ccap = 64 * cap + 64
+        for _ in range(_RETRIES):
             rows = np.zeros((cap, 10), np.int64)
             sc = np.zeros(cap, np.float64)
-            cig = C.create_string_buffer(64 * cap + 64)
+            cig = C.create_string_buffer(ccap)
             rc = lib().msgs_align_batch(self.h, C.byref(rb.st), C.byref(params), _p(counts, C.c_int64),
                                         _p(rows, C.c_int64), _p(sc, C.c_double), cig, len(cig), cap,
                                         C.byref(need))
             if rc == MSGS_E_CAP:
-                cap = need.value
+                if need.value > cap:      # rows
+                    cap = need.value
+                    ccap = max(ccap, 64 * cap + 64)
+                else:                     # CIGAR text
+                    ccap *= 4
                 continue
             self._chk(rc, "msgs_align_batch")
             k = need.value
             cigars = cig.value.decode().split("\n")[:k]
             return counts[:rb.n].copy(), rows[:k].copy(), sc[:k].copy(), cigars
+        raise RuntimeError("msgs_align_batch: output capacity not reached after %d tries" % _RETRIES)
 
     def map(self, reads, params, mates=None):
         """reads/mates: (seqs, quals, names).  Returns the SAM text per read (pair)."""
@@ -417,7 +430,7 @@ class Mapper:
         need = C.c_int64(0)
         lo = np.zeros(rb.n + 1, np.int64)
         cap = 1024 * max(rb.n, 1)
-        while True:
+        for _ in range(_RETRIES):
             buf = C.create_string_buffer(cap)
             rc = lib().msgs_map_batch(self.h, C.byref(rb.st), C.byref(mb.st) if mb else None, C.byref(params),
                                       buf, cap, _p(lo, C.c_int64), C.byref(need))
@@ -427,6 +440,7 @@ class Mapper:
             self._chk(rc, "msgs_map_batch")
             text = buf.raw[:need.value].decode()
             return [text[lo[i]:lo[i + 1]] for i in range(rb.n)]
+        raise RuntimeError("output capacity not reached after %d tries" % _RETRIES)
 
     def submit(self, slot, params, emit_sam=True):
         self._chk(lib().msgs_batch_submit(self.h, slot, C.byref(params), 1 if emit_sam else 0), "msgs_batch_submit")

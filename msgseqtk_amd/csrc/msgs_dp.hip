@@ -395,18 +395,34 @@ __global__ void __launch_bounds__(256) dp_setup_kernel(DevIndex ix, DpArgs a) {
             bord += rows + cols;
             maxCols = max(maxCols, (int32_t)cols);
         }
-        g.cls = area == 0 ? CLS_NONE : ((ns <= FILL_NS && maxCols <= FILL_COLS) ? CLS_WAVE : CLS_SERIAL);
+        g.cls = area == 0 ? CLS_NONE
+                          : ((!a.forceSerial && ns <= FILL_NS && maxCols <= FILL_COLS) ? CLS_WAVE : CLS_SERIAL);
+        int64_t cellsAll = 0;
+        if (g.cls == CLS_SERIAL) {
+            /* a serially filled job keeps M, I, D per cell in the tile pool instead of packed
+             * decisions: its rectangle offsets count cells */
+            cellsAll = (g.r5r1 >= 1 && g.r5c1 >= 1) ? (int64_t)g.r5r1 * g.r5c1 : 0;
+            for (int k = 0; k + 1 < ns; ++k) {
+                const SeedRec& s = seeds[idx[k]];
+                const SeedRec& t = seeds[idx[k + 1]];
+                const int64_t rows = t.from - s.to, cols = t.start - s.end;
+                a.auxGap[cs + k] = cellsAll;
+                if (rows > 0 && cols > 0) cellsAll += rows * cols;
+            }
+            g.r3off = cellsAll;
+            if (g.r3r0 <= g.L) cellsAll += (int64_t)(g.L - g.r3r0 + 1) * (g.tLen - g.r3c0 + 1);
+        }
         if (a.dstat && g.cls == CLS_SERIAL) {
             atomicAdd(&a.dstat[0], 1ull);
-            atomicAdd(&a.dstat[1], (unsigned long long)area);
-            atomicMax(&a.dstat[2], (unsigned long long)area);
+            atomicAdd(&a.dstat[1], (unsigned long long)cellsAll);
+            atomicMax(&a.dstat[2], (unsigned long long)cellsAll);
         }
         a.geo[job] = g;
         a.jobCls[job] = g.cls;
-        a.tileNeed[job] = area;
+        a.tileNeed[job] = g.cls == CLS_WAVE ? area : 0;
         a.pathNeed[job] = L + g.tLen + 3;
-        a.borderNeed[job] = bord;
-        a.serialNeed[job] = g.cls == CLS_SERIAL ? area : 0;
+        a.borderNeed[job] = g.cls == CLS_WAVE ? bord : 0;
+        a.serialNeed[job] = cellsAll;
     }
 }
 

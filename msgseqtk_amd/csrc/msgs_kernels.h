@@ -172,10 +172,12 @@ struct DpArgs {
     int32_t* jobCls;
     const int64_t* waveJobs;   /* jobs filled by the wavefront kernel, largest first */
     int64_t nWaveJobs;
-    const int64_t* tileOff;    /* per job offset into the decision pool (one byte per rectangle cell) */
-    uint8_t* bits;             /* per rectangle cell of a wavefront-filled job, the backtrace's next
-                                * state from M (bits 0-1), I (bit 2: 1 = stay), D (bit 3: 1 = stay)
-                                * and M > 0 (bit 4) */
+    const int64_t* tileOff;    /* per job byte offset into the decision pool */
+    uint8_t* bits;             /* decisions of wavefront-filled jobs, per 64-row strip of a rectangle:
+                                * one 64-bit word per row and 16 sweep steps (GLOBAL, 4 bits per cell)
+                                * or 8 (LOCAL, 8 bits per cell); per cell the backtrace's next state
+                                * from M (bits 0-1), I (bit 2: 1 = stay), D (bit 3: 1 = stay) and, in
+                                * LOCAL mode, M > 0 (bit 4) */
     const int64_t* borderOff;  /* per job offset into the border pool */
     int4* border;              /* M, I, D of each rectangle's last row, then its last column */
     const int64_t* serialOff;  /* per job offset into the tile pool (serially filled jobs only) */
@@ -192,6 +194,7 @@ struct DpArgs {
     const double* qMis;        /* q / PHRED_SCALE by q */
     const double* qGapOpen;    /* q / PHRED_SCALE - gapO by q */
     double gapE_d;
+    int32_t forceSerial;       /* every job filled serially (context option dp_serial) */
     unsigned long long* dstat; /* optional counters (NULL: none): serially filled jobs, their cells,
                                 * the largest one's cells, full last-row scans */
     int64_t* tileNeed;         /* rectangle cells */

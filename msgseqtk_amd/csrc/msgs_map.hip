@@ -285,6 +285,7 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     a.match = (int32_t)p->match; a.mismatch = (int32_t)p->mismatch; a.gapO = (int32_t)p->gapO; a.gapE = (int32_t)p->gapE;
     a.clip = p->clip; a.gapE_d = p->gapE;
     a.dstat = dstat;
+    a.forceSerial = c->dpSerial;
     /* quality terms of Alignment::evaluate with the host libm (src/Alignment.cpp:460-477,
      * src/QualStr.h:109-117) */
     double hq[3 * 256];
