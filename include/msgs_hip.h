@@ -82,7 +82,10 @@ const char* msgs_last_error(msgs_ctx* ctx);
  * again from pinned host memory (streaming input; default 0); "dp_serial" 1 = every DP job is
  * filled by the serial path (all of M, I, D kept per cell) that otherwise takes only jobs with
  * more than 64 seeds or a rectangle wider than 512 columns (default 0).  Results never depend
- * on these settings. */
+ * on these settings, except "record_format": the records msgs_map_batch and msgs_batch_sam
+ * return are SAM text lines (0, default) or the same records in the BAM layout (1): for each,
+ * what htslib's bam_write1 puts in the BGZF stream (int32 block_size, the 32-byte core, read name,
+ * CIGAR, 4-bit SEQ, QUAL and tags as the reference's BAM::setAux stores them), uncompressed. */
 int msgs_ctx_set(msgs_ctx* ctx, const char* key, int64_t value);
 int msgs_ctx_get(msgs_ctx* ctx, const char* key, int64_t* value);
 

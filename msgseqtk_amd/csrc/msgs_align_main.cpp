@@ -2,7 +2,7 @@
  * msgs_align_main.cpp -- msgseqtk-align-amd, a drop-in for msgseqtk-align
  * (src/msgseqtk-align.cpp) that maps through libmsgs_hip.so.
  *
- *   msgseqtk-align-amd <DB> <READ-FILE> [MATE-FILE] -o <OUT.sam> [options]
+ *   msgseqtk-align-amd <DB> <READ-FILE> [MATE-FILE] -o <OUT.sam|OUT.bam> [options]
  *
  * DB is the reference's database prefix (DB.mtg + DB.mgs) or a FASTA file of
  * chromosomes.  Reads are FASTA or FASTQ (by extension, SeqIO::guessFormat,
@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "../../include/msgs_hip.h"
+#include "msgs_bgzf.h"
 
 namespace {
 
@@ -105,18 +106,92 @@ void usage(const char* prog) {
 
 std::vector<std::string> g_names;
 
-int writeHeader(msgs_ctx* ctx, std::ostream& out, const std::string& cmd) {
-    out << "@HD\tVN:1.6\tSO:unsorted\n";
-    int32_t n = msgs_db_nchrom(ctx);
+/* The header msgseqtk-align builds (src/msgseqtk-align.cpp:509-520): the BAMheader of the
+ * chromosomes, the @HD tag set by setHDTag("SQ", "unsorted") (the reference's key, "SQ"), and the
+ * @PG tag "ID:MSGseqTK VN:v1.2.3 PN:<argv[0]> CL:<argv[0]> <options>" with its fields separated by
+ * spaces (ProgEnv's progName / progVer, CommandOptions::getProg / getCmdStr,
+ * src/util/CommandOptions.cpp:31-55).  The text layout is htslib 1.9's (the reference leaves
+ * the htslib version open): sam_hdr_change_HD puts "@HD\tVN:1.6\t<key>:<val>" in front of the
+ * text and BAMheader::addTag appends "@PG\t<value>"; a SAM file gets that text followed by one
+ * @SQ line per target (sam_hdr_write adds them when the text has none), a BAM file the text and
+ * the binary target list (bam_hdr_write). */
+struct Header {
+    std::string text;
+    std::vector<std::string> names;
+    std::vector<int64_t> lens;
+};
+
+bool makeHeader(msgs_ctx* ctx, const std::string& prog, const std::string& cmd, Header& h) {
+    h.text = "@HD\tVN:1.6\tSQ:unsorted\n@PG\tID:MSGseqTK VN:v1.2.3 PN:" + prog + " CL:" + cmd + "\n";
+    const int32_t n = msgs_db_nchrom(ctx);
     char buf[4096];
     for (int32_t t = 0; t < n; ++t) {
         int64_t len = 0;
-        if (msgs_db_chrom(ctx, t, buf, sizeof buf, &len) != 0) return 1;
-        out << "@SQ\tSN:" << buf << "\tLN:" << len << "\n";
+        if (msgs_db_chrom(ctx, t, buf, sizeof buf, &len) != 0) return false;
+        h.names.push_back(buf);
+        h.lens.push_back(len);
     }
-    out << "@PG\tID:msgseqtk-align\tVN:v1.2.3\tPN:msgseqtk-align-amd\tCL:" << cmd << "\n";
-    return 0;
+    return true;
 }
+
+/* records out: SAM text, or BAM through BGZF */
+struct Sink {
+    bool bam = false;
+    std::ofstream sam;
+    FILE* f = nullptr;
+    msgs_bgzf::Writer* z = nullptr;
+
+    bool open(const std::string& fn, bool asBam) {
+        bam = asBam;
+        if (!bam) { sam.open(fn); return (bool)sam; }
+        f = std::fopen(fn.c_str(), "wb");
+        if (!f) return false;
+        z = new msgs_bgzf::Writer(f);
+        return true;
+    }
+    bool header(const Header& h) {
+        if (!bam) {
+            sam << h.text;
+            for (size_t t = 0; t < h.names.size(); ++t) sam << "@SQ\tSN:" << h.names[t] << "\tLN:" << h.lens[t] << "\n";
+            return (bool)sam;
+        }
+        auto i32 = [this](int64_t v) { const int32_t x = (int32_t)v; z->write(&x, 4); };
+        z->write("BAM\1", 4);
+        i32((int64_t)h.text.size());
+        z->write(h.text.data(), h.text.size());
+        i32((int64_t)h.names.size());
+        for (size_t t = 0; t < h.names.size(); ++t) {
+            i32((int64_t)h.names[t].size() + 1);
+            z->write(h.names[t].c_str(), h.names[t].size() + 1);
+            i32(h.lens[t]);
+        }
+        z->flush();
+        return z->ok();
+    }
+    /* SAM text, or BAM records (each int32 block_size + block_size bytes) */
+    bool records(const char* p, int64_t n) {
+        if (!bam) { sam.write(p, n); return (bool)sam; }
+        int64_t k = 0;
+        while (k < n) {
+            if (n - k < 4) return false;
+            uint32_t bs;
+            std::memcpy(&bs, p + k, 4);
+            if ((int64_t)bs > n - k - 4) return false;
+            z->reserveRecord(4 + (size_t)bs);
+            z->write(p + k, 4 + (size_t)bs);
+            k += 4 + bs;
+        }
+        return z->ok();
+    }
+    bool close() {
+        if (!bam) { sam.close(); return !sam.fail(); }
+        z->close();
+        const bool ok = z->ok();
+        delete z;
+        z = nullptr;
+        return std::fclose(f) == 0 && ok;
+    }
+};
 
 bool loadFastaDb(msgs_ctx* ctx, const std::string& fn) {
     std::ifstream in(fn);
@@ -208,7 +283,9 @@ int main(int argc, char** argv) {
     const bool paired = pos.size() == 3;
     Fmt fmt = guessFormat(pos[1]);
     if (fmt == UNK) { std::cerr << "Unrecognized sequence format for file '" << pos[1] << "'\n"; return EXIT_FAILURE; }
-    if (!endsWith(outFn, ".sam")) { std::cerr << "Unable to determine SAM output mode (only .sam is written)\n"; return EXIT_FAILURE; }
+    /* output mode by suffix (src/msgseqtk-align.cpp:472-481, src/MSGseqTKConst.h:31-32) */
+    const bool asBam = endsWith(outFn, ".bam");
+    if (!asBam && !endsWith(outFn, ".sam")) { std::cerr << "Unable to determine SAM/BAM output mode\n"; return EXIT_FAILURE; }
     std::ifstream in1(pos[1]), in2;
     if (!in1) { std::cerr << "Unable to open '" << pos[1] << "' " << std::strerror(errno) << "\n"; return EXIT_FAILURE; }
     if (paired) {
@@ -219,8 +296,14 @@ int main(int argc, char** argv) {
     if (msgs_ctx_create(0, &ctx) != 0) { std::cerr << "no usable GPU\n"; return EXIT_FAILURE; }
     bool ok = guessFormat(pos[0]) == FASTA ? loadFastaDb(ctx, pos[0]) : (msgs_db_load(ctx, pos[0].c_str()) == 0);
     if (!ok) { std::cerr << "Unable to load database: " << msgs_last_error(ctx) << "\n"; msgs_ctx_destroy(ctx); return EXIT_FAILURE; }
-    std::ofstream out(outFn);
-    if (!out || writeHeader(ctx, out, cmd)) { std::cerr << "Unable to write '" << outFn << "'\n"; return EXIT_FAILURE; }
+    Header hdr;
+    Sink out;
+    if (asBam && msgs_ctx_set(ctx, "record_format", 1) != 0) { std::cerr << msgs_last_error(ctx) << "\n"; return EXIT_FAILURE; }
+    if (!makeHeader(ctx, argv[0], cmd, hdr) || !out.open(outFn, asBam) || !out.header(hdr)) {
+        std::cerr << "Unable to write '" << outFn << "'\n";
+        msgs_ctx_destroy(ctx);
+        return EXIT_FAILURE;
+    }
 
     Reader r1{in1, fmt}, r2{in2, fmt};
     uint64_t total = 0;
@@ -275,13 +358,14 @@ int main(int argc, char** argv) {
                 rc = msgs_map_batch(ctx, &m1, paired ? &m2 : nullptr, &p, &text[0], (int64_t)text.size(), lo.data(), &need);
             }
             if (rc) { std::cerr << "mapping failed: " << msgs_last_error(ctx) << "\n"; break; }
-            out.write(text.data(), need);
+            if (!out.records(text.data(), need)) { std::cerr << "Unable to write '" << outFn << "'\n"; rc = 1; break; }
             total += a.size();
         }
     } catch (const std::exception& e) {
         std::cerr << e.what() << "\n";
         rc = 1;
     }
+    if (!out.close() && !rc) { std::cerr << "Unable to write '" << outFn << "'\n"; rc = 1; }
     std::cerr << "Read alignment finished. Total processed reads: " << total << "\n";
     msgs_ctx_destroy(ctx);
     return rc ? EXIT_FAILURE : EXIT_SUCCESS;

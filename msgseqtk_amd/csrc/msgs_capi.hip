@@ -922,6 +922,7 @@ extern "C" int msgs_ctx_set(msgs_ctx* c, const char* key, int64_t value) {
     else if (k == "pair_chains") { if (value != 0 && value != 1) return fail(c, MSGS_E_ARG, "pair_chains: 0 or 1"); c->pairChains = (int)value; }
     else if (k == "streams") { if (value < 1 || value > kWorkExecs) return fail(c, MSGS_E_ARG, "streams in [1, 4]"); c->workStreams = (int)value; }
     else if (k == "ingest") { if (value != 0 && value != 1) return fail(c, MSGS_E_ARG, "ingest: 0 or 1"); c->ingest = (int)value; }
+    else if (k == "record_format") { if (value != 0 && value != 1) return fail(c, MSGS_E_ARG, "record_format: 0 or 1"); c->recordFormat = (int)value; }
     else if (k == "dp_serial") { if (value != 0 && value != 1) return fail(c, MSGS_E_ARG, "dp_serial: 0 or 1"); c->dpSerial = (int)value; }
     else return fail(c, MSGS_E_ARG, "unknown option " + k);
     return MSGS_OK;
@@ -942,6 +943,7 @@ extern "C" int msgs_ctx_get(msgs_ctx* c, const char* key, int64_t* value) {
     else if (k == "streams") *value = c->workStreams;
     else if (k == "ingest") *value = c->ingest;
     else if (k == "dp_serial") *value = c->dpSerial;
+    else if (k == "record_format") *value = c->recordFormat;
     else return fail(c, MSGS_E_ARG, "unknown option " + k);
     return MSGS_OK;
 }

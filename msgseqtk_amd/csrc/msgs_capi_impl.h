@@ -155,6 +155,7 @@ struct msgs_ctx {
                            * searches of findAllSMEMS on separate lanes, smem_pair_kernel) */
     int ingest = 0;       /* resident batches: 1 = every run first copies the batch's reads and
                            * qualities from pinned host memory and makes the reverse complements */
+    int recordFormat = 0; /* records of msgs_map_batch / msgs_batch_sam: 0 SAM text, 1 BAM */
     int dpSerial = 0;     /* 1 = every DP job takes the serial fill (M, I, D kept per cell) instead
                            * of the wavefront fill */
     Exec own;                      /* the caller's thread */

@@ -465,6 +465,7 @@ struct Finisher {
     const msgs_params& P;
     const DpStage& dp;
     bool emit = true;
+    bool bam = false;    /* records in the BAM layout instead of SAM text */
     int64_t mapped = 0;
     const double log10e = ::log10(M_E);
     /* time-stamp-counter ticks spent in: [0] SAM text, [1] pairing, [2] MAPQ, [3] alignment lists */
@@ -486,6 +487,15 @@ struct Finisher {
         static const char kOps[] = "MIDNSHP=XB";
         if (!emit) return;
         const uint64_t t0 = __rdtsc();
+        if (bam) {
+            msgs_sam::BamFields f{rd.name, (int)rd.nameLen, flag, tid, pos, mapq, cg, nc, isize, rd.seq, rd.qual,
+                                  rd.L, rc, tags != nullptr, tags ? tags[0] : 0, md, mdLen, xh,
+                                  tags ? tags[1] : 0, tags ? tags[2] : 0, xp};
+            char* p = out.reserve(msgs_sam::bamBound(f));
+            out.commit(msgs_sam::putBam(p, f));
+            tk[0] += __rdtsc() - t0;
+            return;
+        }
         const std::string* cn = tid >= 0 ? &db.names[tid] : nullptr;
         const size_t bound = (size_t)rd.nameLen + (cn ? cn->size() : 1) + 2 * (size_t)rd.L + 16 * (size_t)nc +
                              (size_t)mdLen + 256;
@@ -499,7 +509,7 @@ struct Finisher {
         *p++ = '\t';
         if (nc == 0) *p++ = '*';
         for (int k = 0; k < nc; ++k) { p = msgs_sam::putInt(p, cg[k] >> 4); *p++ = kOps[cg[k] & 15]; }
-        std::memcpy(p, "\t*\t0\t", 5); p += 5;
+        std::memcpy(p, "\t*\t1\t", 5); p += 5;   /* no mate: mtid -1, mpos 0 (printed + 1) */
         p = msgs_sam::putInt(p, isize);
         *p++ = '\t';
         if (rd.L == 0) *p++ = '*';
@@ -1050,6 +1060,7 @@ static int64_t samText(msgs_ctx* c, const msgs_params* p, const msgs_reads* read
         const int64_t i0 = n * t / nt, i1 = n * (t + 1) / nt;
         Finisher F{c->db, *p, b.dp};
         F.emit = emit;
+        F.bam = c->recordFormat == 1;
         std::vector<uint8_t> q30;
         auto host = [&](const msgs_reads* r, const std::vector<int64_t>& nm, int64_t i) {
             HostRead h;

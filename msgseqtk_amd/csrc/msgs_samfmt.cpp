@@ -144,6 +144,87 @@ __attribute__((target("ssse3"))) char* putQual(char* p, const uint8_t* q, int64_
     return p + L;
 }
 
+/* ------------------------------------------------------------------ BAM records */
+
+int32_t reg2bin(int64_t beg, int64_t end) {
+    /* the smallest bin of the UCSC binning scheme (levels of 2^14, 2^17, ... 2^29 bp) holding
+     * [beg, end): level l's bins start at (8^l - 1) / 7 */
+    --end;
+    int s = 14, t = ((1 << 15) - 1) / 7;
+    for (int l = 5; l > 0; --l, s += 3, t -= 1 << (3 * l))
+        if ((beg >> s) == (end >> s)) return (int32_t)(t + (beg >> s));
+    return 0;
+}
+
+size_t bamBound(const BamFields& f) {
+    return 36 + (size_t)f.nameLen + 1 + 4 * (size_t)f.nCigar + (size_t)(f.L + 1) / 2 + (size_t)f.L +
+           (f.tags ? 7 * 4 + (size_t)f.mdLen + 4 + 14 : 0);
+}
+
+static inline char* put32(char* p, uint32_t v) { std::memcpy(p, &v, 4); return p + 4; }
+
+/* bam_aux_update_int's choice for a new tag: 'i' below INT16_MIN, 's' below INT8_MIN, 'c'
+ * below 0, 'C' below UINT8_MAX, 'S' below UINT16_MAX, else 'I' (strict bounds as htslib has
+ * them: 255 is stored as 'S') */
+static char* putAuxInt(char* p, const char* tag, int64_t v) {
+    *p++ = tag[0]; *p++ = tag[1];
+    if (v < -32768) { *p++ = 'i'; return put32(p, (uint32_t)(int32_t)v); }
+    if (v < -128) { *p++ = 's'; const int16_t x = (int16_t)v; std::memcpy(p, &x, 2); return p + 2; }
+    if (v < 0) { *p++ = 'c'; *p++ = (char)(int8_t)v; return p; }
+    if (v < 255) { *p++ = 'C'; *p++ = (char)(uint8_t)v; return p; }
+    if (v < 65535) { *p++ = 'S'; const uint16_t x = (uint16_t)v; std::memcpy(p, &x, 2); return p + 2; }
+    *p++ = 'I';
+    return put32(p, (uint32_t)v);
+}
+
+static char* putAuxFloat(char* p, const char* tag, float v) {
+    *p++ = tag[0]; *p++ = tag[1]; *p++ = 'f';
+    std::memcpy(p, &v, 4);
+    return p + 4;
+}
+
+char* putBam(char* p, const BamFields& f) {
+    static const uint8_t kComp[16] = {0, 8, 4, 12, 2, 10, 6, 14, 1, 9, 5, 13, 3, 11, 7, 15};
+    char* start = p;
+    p += 4;   /* block_size, below */
+    int64_t rlen = 0;
+    for (int k = 0; k < f.nCigar; ++k) {
+        const uint32_t op = f.cigar[k] & 15u;
+        if (op == 0 || op == 2 || op == 3 || op == 7 || op == 8) rlen += f.cigar[k] >> 4;   /* M D N = X */
+    }
+    p = put32(p, (uint32_t)f.tid);
+    p = put32(p, (uint32_t)(int32_t)f.pos);
+    p = put32(p, (uint32_t)reg2bin(f.pos, f.pos + rlen) << 16 | (uint32_t)(f.mapq & 255) << 8 |
+                     (uint32_t)(f.nameLen + 1));
+    p = put32(p, (uint32_t)f.flag << 16 | (uint32_t)f.nCigar);
+    p = put32(p, (uint32_t)f.L);
+    p = put32(p, 0xffffffffu);   /* next refID -1 */
+    p = put32(p, 0u);            /* next pos 0 */
+    p = put32(p, (uint32_t)f.isize);
+    std::memcpy(p, f.name, f.nameLen); p += f.nameLen;
+    *p++ = 0;
+    if (f.nCigar) { std::memcpy(p, f.cigar, 4 * (size_t)f.nCigar); p += 4 * (size_t)f.nCigar; }
+    /* SEQ: two codes per byte, the first in the high nibble, 0 after an odd last base */
+    for (int64_t i = 0; i < f.L; i += 2) {
+        const uint8_t a = f.rc ? kComp[f.seq[f.L - 1 - i] & 15] : (uint8_t)(f.seq[i] & 15);
+        const uint8_t b = i + 1 < f.L ? (f.rc ? kComp[f.seq[f.L - 2 - i] & 15] : (uint8_t)(f.seq[i + 1] & 15)) : 0;
+        *p++ = (char)(a << 4 | b);
+    }
+    for (int64_t i = 0; i < f.L; ++i) *p++ = (char)(f.rc ? f.qual[f.L - 1 - i] : f.qual[i]);
+    if (f.tags) {
+        p = putAuxInt(p, "AS", f.as);
+        *p++ = 'M'; *p++ = 'D'; *p++ = 'Z';
+        std::memcpy(p, f.md, f.mdLen); p += f.mdLen;
+        *p++ = 0;
+        p = putAuxFloat(p, "XH", f.xh);
+        p = putAuxInt(p, "NH", f.nh);
+        p = putAuxInt(p, "XN", f.xn);
+        p = putAuxFloat(p, "XP", f.xp);
+    }
+    put32(start, (uint32_t)(p - start - 4));
+    return p;
+}
+
 }  // namespace msgs_sam
 
 /* test hook (no reference counterpart): putG6 of n floats, each result NUL-terminated in

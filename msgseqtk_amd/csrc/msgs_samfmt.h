@@ -21,6 +21,27 @@ char* putG6(char* p, float f);
 char* putSeq(char* p, const uint8_t* s, int64_t L, bool rc);
 char* putQual(char* p, const uint8_t* q, int64_t L, bool rc);
 
+/* One alignment record's fields, as the reference's BAM constructor and output() set them
+ * (src/SAMtools/BAM.cpp:20-56, src/Alignment.cpp:429-444, src/msgseqtk-align.cpp:538-593):
+ * no mate (mtid -1, mpos 0), SEQ / QUAL of the read or of its reverse complement. */
+struct BamFields {
+    const char* name; int nameLen;
+    int flag; int32_t tid; int64_t pos; int mapq;
+    const uint32_t* cigar; int nCigar;      /* BAM CIGAR words (length << 4 | op) */
+    int32_t isize;
+    const uint8_t* seq; const uint8_t* qual; int64_t L; bool rc;
+    bool tags;                              /* AS, MD, XH, NH, XN, XP (mapped records) */
+    int64_t as; const char* md; int mdLen; float xh; int64_t nh, xn; float xp;
+};
+/* bytes putBam writes at most */
+size_t bamBound(const BamFields& f);
+/* the record as bam_write1 puts it in the BGZF stream: block_size, the 32-byte core, the read
+ * name with one NUL, CIGAR, 4-bit SEQ, QUAL and the tags (integer tags in the smallest type
+ * bam_aux_update_int picks, float tags as 'f', MD as 'Z') */
+char* putBam(char* p, const BamFields& f);
+/* hts_reg2bin(beg, end, 14, 5) */
+int32_t reg2bin(int64_t beg, int64_t end);
+
 /* an output text with a write cursor: grows its string in large steps; finish() trims it */
 struct Out {
     std::string* buf = nullptr;

@@ -882,7 +882,10 @@ static std::string samLine(const Db& db, const Rec& r) {
     s += '\t'; s += std::to_string(r.pos + 1);
     s += '\t'; s += std::to_string(r.mapq);
     s += '\t'; s += r.cigar.empty() ? std::string("*") : r.cigar;
-    s += "\t*\t0\t"; s += std::to_string(r.isize);
+    /* RNEXT / PNEXT: every record is built with the BAM constructor's mate defaults, mtid -1
+     * and mpos 0 (src/SAMtools/BAM.h:61-63,73-75; AlignmentPE::exportFwdBAM / exportRevBAM set
+     * only MAPQ, ISIZE and flags, src/Alignment.h:512-529), and sam_format1 prints mpos + 1 */
+    s += "\t*\t1\t"; s += std::to_string(r.isize);
     s += '\t';
     static const char kNt16[17] = "=ACMGRSVTWYHKDBN";
     if (r.q->L == 0) s += '*';

@@ -355,3 +355,26 @@ def test_calc_mapq_closed_forms():
     assert pp[0] == 0.5 and list(mq) == [3, 3]
     pp, mq = O.orc_calc_mapq([-500.0, -501.0])
     assert abs(pp[0] - 10 / 11) < 1e-12 and list(mq) == [10, 0]
+
+
+def test_record_mate_fields_and_unmapped_layout():
+    """Records are built by the BAM constructor with mtid -1, mpos 0 and, for unmapped reads,
+    tid -1, pos 0, MAPQ 0 and no CIGAR (src/SAMtools/BAM.h:61-75; AlignmentPE::exportFwdBAM /
+    exportRevBAM change only MAPQ, ISIZE and flags, src/Alignment.h:512-529); sam_format1 prints
+    RNEXT '*' for mtid < 0, PNEXT as mpos + 1 and POS as pos + 1.  So every record reads
+    '*', '1' in columns 7-8, and an unmapped one '*', '1', '0', '*' in columns 3-6; TLEN is 0
+    for single-end records and +-ISIZE for pairs."""
+    chroms, names = synth.metagenome(3, 20000, gc=(0.4, 0.6), seed=5)
+    db = O.OracleDb(chroms, names)
+    seqs, quals, rn, _ = synth.simulate_reads(chroms, 60, 100, sub=0.01, indel=0.0, seed=6, offtarget=0.2)
+    sam, _ = O.orc_map(db, O.default_params(), seqs, quals, rn)
+    lines = "".join(sam).splitlines()
+    unmapped = [l.split("\t") for l in lines if int(l.split("\t")[1]) & 4]
+    assert unmapped and all(f[2:6] == ["*", "1", "0", "*"] for f in unmapped)
+    assert all(l.split("\t")[6:9] == ["*", "1", "0"] for l in lines)
+    (a, qa, na), (b, qb, nb) = synth.simulate_pairs(chroms, 40, 100, insert=(250, 30), seed=7)
+    pe, _ = O.orc_map(db, O.pe_params(250, 30), a, qa, na, mates=(b, qb, nb))
+    pl = [l.split("\t") for l in "".join(pe).splitlines()]
+    assert all(f[6:8] == ["*", "1"] for f in pl)
+    proper = [f for f in pl if int(f[1]) & 2]
+    assert proper and all(abs(int(f[8])) >= 100 for f in proper)
