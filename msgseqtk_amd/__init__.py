@@ -264,7 +264,8 @@ class Mapper:
             self._chk(rc, "msgs_smem_batch")
             k = need.value
             return counts[:rb.n].copy(), rows[:k].copy(), lp[:k].copy()
-        raise RuntimeError("output capacity not reached after %d tries" % _RETRIES)
+        else:
+            raise RuntimeError("output capacity not reached after %d tries" % _RETRIES)
 
     def search(self, patterns, mode):
         """bi-intervals (p, q, s) after msgs_search_batch's search (0 count, 1 fwd, 2 rev)"""
@@ -302,7 +303,8 @@ class Mapper:
                 continue
             self._chk(rc, "msgs_chain_batch")
             break
-        raise RuntimeError("output capacity not reached after %d tries" % _RETRIES)
+        else:
+            raise RuntimeError("output capacity not reached after %d tries" % _RETRIES)
         res, k = [], 0
         for i in range(len(seed_sets)):
             cs = []
@@ -359,7 +361,8 @@ class Mapper:
             self._chk(rc, "msgs_mems_batch")
             k = need.value
             return counts[:rb.n].copy(), rows[:k].copy(), lp[:k].copy(), rll[:rb.n].copy()
-        raise RuntimeError("output capacity not reached after %d tries" % _RETRIES)
+        else:
+            raise RuntimeError("output capacity not reached after %d tries" % _RETRIES)
 
     def clean(self, background, seqs, params, min_lod, mates=None):
         """msgseqtk-clean's decision with this mapper's database as the reference and
@@ -395,7 +398,8 @@ class Mapper:
             self._chk(rc, "msgs_seed_batch")
             k = need.value
             return counts[:rb.n].copy(), rows[:k].copy(), lp[:k].copy()
-        raise RuntimeError("output capacity not reached after %d tries" % _RETRIES)
+        else:
+            raise RuntimeError("output capacity not reached after %d tries" % _RETRIES)
 
     def alignments(self, seqs, quals, params):
         rb = ReadBatch(seqs, quals)
@@ -421,7 +425,8 @@ class Mapper:
             k = need.value
             cigars = cig.value.decode().split("\n")[:k]
             return counts[:rb.n].copy(), rows[:k].copy(), sc[:k].copy(), cigars
-        raise RuntimeError("msgs_align_batch: output capacity not reached after %d tries" % _RETRIES)
+        else:
+            raise RuntimeError("msgs_align_batch: output capacity not reached after %d tries" % _RETRIES)
 
     def map(self, reads, params, mates=None):
         """reads/mates: (seqs, quals, names).  Returns the SAM text per read (pair)."""
@@ -440,7 +445,8 @@ class Mapper:
             self._chk(rc, "msgs_map_batch")
             text = buf.raw[:need.value].decode()
             return [text[lo[i]:lo[i + 1]] for i in range(rb.n)]
-        raise RuntimeError("output capacity not reached after %d tries" % _RETRIES)
+        else:
+            raise RuntimeError("output capacity not reached after %d tries" % _RETRIES)
 
     def submit(self, slot, params, emit_sam=True):
         self._chk(lib().msgs_batch_submit(self.h, slot, C.byref(params), 1 if emit_sam else 0), "msgs_batch_submit")

@@ -47,6 +47,11 @@ enum : int32_t { FWD = 1, REV = 2 };
  * kernel's registers decide (the host sizes the persistent grid by the occupancy query) */
 constexpr int FILL_NS = 64;
 constexpr int FILL_COLS = 512;
+/* limits of the 16-lane fill (four jobs per wavefront): seeds per job and rectangle columns;
+ * its LDS holds, per job, the seeds (512 B) and the row above the current strip (M, I, D and
+ * the target bases: 3.4 KB), 15.4 KB per four-job block */
+constexpr int NS16 = 32;
+constexpr int COLS16 = 256;
 
 __device__ __forceinline__ int32_t addn(int32_t a, int32_t b) { return a == NEG ? NEG : a + b; }
 __device__ __forceinline__ int32_t subn(int32_t a, int32_t b) { return a == NEG ? NEG : a - b; }
@@ -54,6 +59,13 @@ __device__ __forceinline__ int32_t max3(int32_t a, int32_t b, int32_t c) { retur
 /* the value of lane - 1 (lane 0 keeps its own), by a DPP whole-wavefront shift right by one
  * lane (wave_shr:1) instead of an LDS permute; every lane of the wavefront runs it */
 __device__ __forceinline__ int32_t laneUp1(int32_t v) { return __builtin_amdgcn_update_dpp(v, v, 0x138, 0xf, 0xf, false); }
+/* within each 16-lane DPP row: the value of lane - 1, lane 0 of the row taking its own `first`
+ * (row_shr:1 with the out-of-row source keeping the old value) */
+__device__ __forceinline__ int32_t rowUp1(int32_t first, int32_t v) {
+    return __builtin_amdgcn_update_dpp(first, v, 0x111, 0xf, 0xf, false);
+}
+/* within each 16-lane DPP row: the value of lane + 1 (row_ror:15, lane 15 taking lane 0's) */
+__device__ __forceinline__ int32_t rowNext(int32_t v) { return __builtin_amdgcn_update_dpp(v, v, 0x12f, 0xf, 0xf, false); }
 
 /* Backtrace decisions of a rectangle are packed per row and per sweep step: the fill sweeps
  * a strip of `rows` (<= 64) rows and C columns in C + rows - 1 anti-diagonal steps, and the
@@ -61,17 +73,19 @@ __device__ __forceinline__ int32_t laneUp1(int32_t v) { return __builtin_amdgcn_
  * `spw` consecutive steps into one 64-bit word (GLOBAL: 16 steps x 4 bits; LOCAL: 8 steps x
  * 8 bits, bit 4 being M > 0), and the lanes store their words side by side, so that every
  * store of the strip is one contiguous run of rows x 8 bytes: cell (l, jj) is in word
- * ((jj + l) / spw) * rows + l at bit ((jj + l) % spw) * bits.  Strips of 64 rows follow each
- * other; a strip of `rows` rows takes ceil((C + rows - 1) / spw) * rows words. */
+ * ((jj + l) / spw) * rows + l at bit ((jj + l) % spw) * bits.  Strips of H rows follow each
+ * other (H = 64 for jobs filled one per wavefront, 16 for jobs filled 16 lanes each); a strip
+ * of `rows` rows takes ceil((C + rows - 1) / spw) * rows words. */
 __device__ __forceinline__ int decShift(int mode) { return mode == 0 ? 4 : 3; }   /* log2 of steps per word */
 __device__ __forceinline__ int decBits(int mode) { return mode == 0 ? 4 : 8; }
 __device__ __forceinline__ int64_t stripWords(int64_t rows, int64_t cols, int mode) {
     const int sh = decShift(mode);
     return ((cols + rows - 1 + (1 << sh) - 1) >> sh) * rows;
 }
-__device__ __forceinline__ int64_t stripBytes(int64_t rows, int64_t cols, int mode) {
-    const int64_t full = rows >> 6, rem = rows & 63;
-    return 8 * (full * stripWords(64, cols, mode) + (rem ? stripWords(rem, cols, mode) : 0));
+/* hs = log2 of the strip height H */
+__device__ __forceinline__ int64_t stripBytes(int64_t rows, int64_t cols, int mode, int hs) {
+    const int64_t full = rows >> hs, rem = rows & ((1 << hs) - 1);
+    return 8 * (full * stripWords(1 << hs, cols, mode) + (rem ? stripWords(rem, cols, mode) : 0));
 }
 
 
@@ -96,13 +110,14 @@ struct Rc {
     __device__ bool hasCol(int32_t j) const { return j >= c0 && j <= c1; }
     __device__ int64_t area() const { return ok ? (int64_t)(r1 - r0 + 1) * (c1 - c0 + 1) : 0; }
     /* the word holding the decisions of cell (i, j) and their bit position (see stripBytes) */
-    __device__ int64_t decWord(int32_t i, int32_t j, int mode, int& shift) const {
-        const int32_t ri = i - r0, jj = j - c0, s0 = ri & ~63, l = ri & 63;
-        const int32_t rows = min(64, r1 - r0 + 1 - s0), C = c1 - c0 + 1;
+    __device__ int64_t decWord(int32_t i, int32_t j, int mode, int hs, int& shift) const {
+        const int32_t H = 1 << hs;
+        const int32_t ri = i - r0, jj = j - c0, s0 = ri & ~(H - 1), l = ri & (H - 1);
+        const int32_t rows = min(H, r1 - r0 + 1 - s0), C = c1 - c0 + 1;
         const int sh = decShift(mode);
         const int32_t t = jj + l;
         shift = (t & ((1 << sh) - 1)) * decBits(mode);
-        return (off >> 3) + (int64_t)(s0 >> 6) * stripWords(64, C, mode) + (int64_t)(t >> sh) * rows + l;
+        return (off >> 3) + (int64_t)(s0 >> hs) * stripWords(H, C, mode) + (int64_t)(t >> sh) * rows + l;
     }
 };
 
@@ -118,6 +133,7 @@ struct Geo {
     int64_t r3off, r3boff;
     int32_t hazard;
     bool serial;             /* rectangle values kept in full (tile) */
+    int32_t hs;              /* log2 of the decision layout's strip height */
     int32_t interior;        /* set if an interior value of a border-only rectangle was asked for */
 };
 
@@ -327,6 +343,7 @@ __device__ __forceinline__ void makeGeo(Geo& G, const DpArgs& a, const JobGeo& g
     G.gapOff = a.auxGap + g.cs;
     G.gapBOff = a.auxGapB + g.cs;
     G.serial = g.cls == CLS_SERIAL;
+    G.hs = g.cls == CLS_WAVE16 ? 4 : 6;
     G.tile = a.tile + a.serialOff[job];
     G.bits = a.bits + a.tileOff[job];
     G.border = a.border + a.borderOff[job];
@@ -367,9 +384,26 @@ __global__ void __launch_bounds__(256) dp_setup_kernel(DevIndex ix, DpArgs a) {
         /* R5 only when both sides are non-empty (src/Alignment.cpp:57-58) */
         g.r5r1 = (int32_t)f.from; g.r5c1 = (int32_t)(f.start - tStart);
         if (!(0 < f.from && tStart < f.start)) { g.r5r1 = 0; g.r5c1 = 0; }
-        int64_t area = (g.r5r1 >= 1 && g.r5c1 >= 1) ? stripBytes(g.r5r1, g.r5c1, a.mode) : 0;
+        g.r3r0 = (int32_t)(l.to + 1); g.r3c0 = (int32_t)(l.end - tStart + 1);
+        if (!(l.to < L && l.end < tEnd)) { g.r3r0 = g.L + 1; g.r3c0 = g.tLen + 1; }
+        /* the fill class from the rectangles' widths and the seed count; it sets the strip
+         * height of the decision layout */
+        bool anyRect = g.r5r1 >= 1 && g.r5c1 >= 1;
+        int32_t maxCols = anyRect ? g.r5c1 : 0;
+        for (int k = 0; k + 1 < ns; ++k) {
+            const SeedRec& s = seeds[idx[k]];
+            const SeedRec& t = seeds[idx[k + 1]];
+            const int64_t rows = t.from - s.to, cols = t.start - s.end;
+            if (rows > 0 && cols > 0) { anyRect = true; maxCols = max(maxCols, (int32_t)cols); }
+        }
+        if (g.r3r0 <= g.L) { anyRect = true; maxCols = max(maxCols, g.tLen - g.r3c0 + 1); }
+        g.cls = !anyRect ? CLS_NONE
+                : a.forceSerial ? CLS_SERIAL
+                : (ns <= NS16 && maxCols <= COLS16) ? CLS_WAVE16
+                : (ns <= FILL_NS && maxCols <= FILL_COLS) ? CLS_WAVE : CLS_SERIAL;
+        const int hs = g.cls == CLS_WAVE16 ? 4 : 6;
+        int64_t area = (g.r5r1 >= 1 && g.r5c1 >= 1) ? stripBytes(g.r5r1, g.r5c1, a.mode, hs) : 0;
         int64_t bord = (g.r5r1 >= 1 && g.r5c1 >= 1) ? (int64_t)g.r5r1 + g.r5c1 : 0;
-        int32_t maxCols = g.r5r1 >= 1 ? g.r5c1 : 0;
         for (int k = 0; k < ns; ++k) {
             const SeedRec& s = seeds[idx[k]];
             a.auxSG[cs + k] = make_int4((int32_t)(s.from + 1), (int32_t)(s.start - tStart + 1), (int32_t)(s.to - s.from), 0);
@@ -379,24 +413,18 @@ __global__ void __launch_bounds__(256) dp_setup_kernel(DevIndex ix, DpArgs a) {
                 a.auxGap[cs + k] = area;
                 a.auxGapB[cs + k] = bord;
                 if (rows > 0 && cols > 0) {
-                    area += stripBytes(rows, cols, a.mode);
+                    area += stripBytes(rows, cols, a.mode, hs);
                     bord += rows + cols;
-                    maxCols = max(maxCols, (int32_t)cols);
                 }
             }
         }
         g.r3off = area;
         g.r3boff = bord;
-        g.r3r0 = (int32_t)(l.to + 1); g.r3c0 = (int32_t)(l.end - tStart + 1);
-        if (!(l.to < L && l.end < tEnd)) { g.r3r0 = g.L + 1; g.r3c0 = g.tLen + 1; }
-        else {
+        if (g.r3r0 <= g.L) {
             const int64_t rows = g.L - g.r3r0 + 1, cols = g.tLen - g.r3c0 + 1;
-            area += stripBytes(rows, cols, a.mode);
+            area += stripBytes(rows, cols, a.mode, hs);
             bord += rows + cols;
-            maxCols = max(maxCols, (int32_t)cols);
         }
-        g.cls = area == 0 ? CLS_NONE
-                          : ((!a.forceSerial && ns <= FILL_NS && maxCols <= FILL_COLS) ? CLS_WAVE : CLS_SERIAL);
         int64_t cellsAll = 0;
         if (g.cls == CLS_SERIAL) {
             /* a serially filled job keeps M, I, D per cell in the tile pool instead of packed
@@ -419,9 +447,10 @@ __global__ void __launch_bounds__(256) dp_setup_kernel(DevIndex ix, DpArgs a) {
         }
         a.geo[job] = g;
         a.jobCls[job] = g.cls;
-        a.tileNeed[job] = g.cls == CLS_WAVE ? area : 0;
+        const bool wave = g.cls == CLS_WAVE || g.cls == CLS_WAVE16;
+        a.tileNeed[job] = wave ? area : 0;
         a.pathNeed[job] = L + g.tLen + 3;
-        a.borderNeed[job] = g.cls == CLS_WAVE ? bord : 0;
+        a.borderNeed[job] = wave ? bord : 0;
         a.serialNeed[job] = cellsAll;
     }
 }
@@ -584,6 +613,236 @@ __global__ void __launch_bounds__(64) dp_fill_kernel(DevIndex ix, DpArgs a) {
 template __global__ void dp_fill_kernel<0>(DevIndex, DpArgs);
 template __global__ void dp_fill_kernel<1>(DevIndex, DpArgs);
 
+/* ------------------------------------------------------------------ 16-lane wavefront fill */
+
+/* Jobs whose rectangles have at most COLS16 columns and whose chains have at most NS16 seeds
+ * are filled 16 lanes per job: each 16-lane DPP row of the wavefront runs its own job, taking
+ * the next one from a work counter when it is done, so four jobs of different shapes proceed
+ * side by side.  Strips are 16 rows (lane = row), swept along anti-diagonals as in
+ * dp_fill_kernel; a 16-row strip leaves fewer lanes idle on its ramp diagonals and on partial
+ * strips than a 64-row one.  Per 16-lane job, in LDS: the seeds and the row above the current
+ * strip (M, I, D and the target bases).  The row lane 0 reads, one entry per step, is kept in a
+ * register chunk of 16 entries that rotates by one lane each step (reloaded every 16 steps),
+ * and lane 0 takes it as the out-of-row value of the row_shr shift that brings the other lanes
+ * their upper neighbours; the target base reaches lane l one step after lane l - 1 the same
+ * way.  Each group runs a small state machine (next job, next rectangle, next strip, step) in
+ * one loop, so the groups need not be in the same phase.  The values, decisions and summaries
+ * are those of dp_fill_kernel. */
+template <int MODE>
+__global__ void __launch_bounds__(64) dp_fill16_kernel(DevIndex ix, DpArgs a) {
+    constexpr int W = COLS16 + 32;   /* chunk reads reach 31 entries past a row of COLS16 */
+    __shared__ int4 sSG[4][NS16];
+    __shared__ int32_t sTM[4][W], sTI[4][W], sTD[4][W];
+    __shared__ uint8_t sTg[4][W];
+    const int lane = threadIdx.x, grp = lane >> 4, l = lane & 15;
+    int4* const SG = sSG[grp];
+    int32_t* const TM = sTM[grp];
+    int32_t* const TI = sTI[grp];
+    int32_t* const TD = sTD[grp];
+    uint8_t* const TG = sTg[grp];
+    constexpr int dsh = MODE == 0 ? 4 : 3, dbits = MODE == 0 ? 4 : 8;
+    constexpr int32_t dmask = (1 << dsh) - 1;
+    const int32_t match = a.match, mis = a.mismatch, o = -(a.gapO + a.gapE), ge = a.gapE;
+
+    enum { FETCH, RECT, STRIP, STEP, DONE };
+    int phase = FETCH;
+    int64_t job = 0;
+    JobGeo g{};
+    Geo G{};
+    const uint8_t* q = nullptr;
+    const uint8_t* tg = nullptr;
+    int64_t chromLen = 0;
+    int ri = 0;
+    bool prevOk = false, act = false;
+    Rc R{};
+    int32_t Rn = 0, C = 0, s0 = 0, rows = 0, t = 0, steps = 0, i = 0;
+    int32_t cM = NEG, cI = NEG, cD = NEG, pM = NEG, pI = NEG, pD = NEG;
+    int32_t chM = NEG, chI = NEG, chD = NEG, chT = 0, tbv = 0, qm = 0;
+    int32_t bM0 = NEG, bI0 = NEG, bD0 = NEG, cm = NEG, ci = NEG, cd = NEG;
+    uint64_t acc = 0;
+    uint64_t* dw = nullptr;
+    int32_t bv = NEG, bi = 0, bj = 0, sbv = NEG, sbj = 0;   /* LOCAL: best cell (lane's rows) */
+    int32_t nv = NEG, nc = 0, nst = OP_M;                   /* GLOBAL: scan of R3's last row */
+
+    while (__any(phase != DONE)) {
+        if (phase == FETCH) {
+            unsigned int k = 0;
+            if (l == 0) k = atomicAdd(a.fill16Next, 1u);
+            k = __shfl(k, grp * 16, 64);
+            if ((int64_t)k >= a.nWave16Jobs) {
+                phase = DONE;
+            } else {
+                job = a.wave16Jobs[k];
+                g = a.geo[job];
+                for (int x = l; x < g.ns; x += 16) SG[x] = a.auxSG[g.cs + x];
+                makeGeo(G, a, g, SG, job);
+                q = ((g.strand == FWD) ? a.seqs : a.rcSeqs) + a.offs[g.rid];
+                tg = ix.tgt + ix.tgtOff[g.tid];
+                chromLen = ix.chromLen[g.tid];
+                ri = 0; prevOk = false;
+                bv = NEG; bi = 0; bj = 0;
+                nv = NEG; nc = 0; nst = OP_M;
+                phase = RECT;
+            }
+        }
+        if (phase == RECT) {
+            /* the next rectangle in event order, after the first diagonal value of each seed
+             * passed (the corner of the rectangle just filled, or the write history) */
+            bool found = false;
+            while (ri <= g.ns) {
+                if (ri >= 1) {
+                    const int k = ri - 1;
+                    const int32_t d0 = prevOk ? max(max3(addn(cm, match), addn(cd, match), addn(ci, match)), 0)
+                                              : seedD0(G, k);
+                    if (l == 0) SG[k].w = d0;
+                }
+                R = ri == 0 ? rectR5(G) : (ri < g.ns ? rectGap(G, ri - 1, SG[ri - 1], SG[ri]) : rectR3(G));
+                prevOk = R.ok;
+                if (R.ok) { found = true; break; }
+                ++ri;
+            }
+            if (!found) {
+                /* job end: the seeds' first diagonal values and the summaries */
+                for (int x = l; x < g.ns; x += 16) a.auxSG[g.cs + x].w = SG[x].w;
+                if (MODE == 1) {
+                    for (int off = 8; off >= 1; off >>= 1) {
+                        const int32_t ov = __shfl_xor(bv, off, 16), oi = __shfl_xor(bi, off, 16),
+                                      oj = __shfl_xor(bj, off, 16);
+                        if (swBetter(ov, oi, oj, bv, bi, bj)) { bv = ov; bi = oi; bj = oj; }
+                    }
+                }
+                if (l == 0) {
+                    JobSum S;
+                    S.swV = bv; S.swI = bi; S.swJ = bj; S.nwV = nv; S.nwC = nc; S.nwS = nst; S.pad0 = 0; S.pad1 = 0;
+                    a.sum[job] = S;
+                }
+                phase = FETCH;
+            } else {
+                /* the row above the rectangle (read at event R.ev - 1) and its target bases
+                 * (N read as 0, so that one AND decides a match) */
+                C = R.c1 - R.c0 + 1;
+                Rn = R.r1 - R.r0 + 1;
+                const int e = R.ev;
+                for (int x = l; x <= C; x += 16) {
+                    const int32_t j = R.c0 - 1 + x;
+                    TM[x] = vM(G, R.r0 - 1, j, e - 1);
+                    TI[x] = vI(G, R.r0 - 1, j, e - 1);
+                    TD[x] = vD(G, R.r0 - 1, j, e - 1);
+                    if (x < C) {
+                        const int64_t tp = g.tStart + (R.c0 + x) - 1;
+                        const uint8_t b = (tp >= 0 && tp <= chromLen) ? tg[tp] : (uint8_t)0;
+                        TG[x] = b == 15 ? (uint8_t)0 : b;
+                    }
+                }
+                s0 = 0;
+                phase = STRIP;
+            }
+        }
+        if (phase == STRIP) {
+            rows = min(16, Rn - s0);
+            i = R.r0 + s0 + l;
+            act = l < rows;
+            const int e = R.ev;
+            cM = NEG; cI = NEG; cD = NEG;
+            if (act) {
+                cM = vM(G, i, R.c0 - 1, e - 1);
+                cI = vI(G, i, R.c0 - 1, e - 1);
+                cD = vD(G, i, R.c0 - 1, e - 1);
+            }
+            bM0 = cM; bI0 = cI; bD0 = cD;
+            const uint8_t qb = act ? q[i - 1] : (uint8_t)0;
+            qm = qb == 15 ? 0 : qb;
+            pM = TM[0]; pI = TI[0]; pD = TD[0];
+            chM = TM[1 + l]; chI = TI[1 + l]; chD = TD[1 + l]; chT = TG[l];
+            tbv = 0;
+            t = 0;
+            steps = C + rows - 1;
+            dw = (uint64_t*)G.bits + (R.off >> 3) + (int64_t)(s0 >> 4) * stripWords(16, C, MODE);
+            acc = 0;
+            sbv = NEG; sbj = 0;
+            phase = STEP;
+        }
+        if (phase == STEP) {
+            if ((t & 15) == 0 && t > 0) {
+                /* lane k of the chunk: row entry t + 1 + k (M, I, D), target entry t + k */
+                chM = TM[t + 1 + l]; chI = TI[t + 1 + l]; chD = TD[t + 1 + l]; chT = TG[t + l];
+            }
+            /* up = (i - 1, j): row above, one step ago (lane 0: the row above the strip);
+             * diagonal = (i - 1, j - 1): the up value of the step before */
+            const int32_t uM = rowUp1(chM, cM), uI = rowUp1(chI, cI), uD = rowUp1(chD, cD);
+            tbv = rowUp1(chT, tbv);
+            const int32_t dM = pM, dI = pI, dD = pD;
+            pM = uM; pI = uI; pD = uD;
+            const int32_t jj = t - l;
+            const bool valid = act && (uint32_t)jj < (uint32_t)C;
+            const int32_t sc = (qm & tbv) ? match : -mis;
+            /* the recurrences of dp_fill_kernel: -inf (NEG) absorbs, gaps are non-negative,
+             * and every finite value lies far above NEG, so the comparisons of the backtrace's
+             * decisions need no clamping */
+            const int32_t mx = max3(dM, dD, dI);
+            int32_t m = mx == NEG ? NEG : mx + sc;
+            if (MODE == 1) m = max(m, 0);
+            const int32_t a1 = uM + o, b1 = uI - ge, a2 = cM + o, b2 = cD - ge;
+            const int32_t nI = max3(a1, b1, NEG), nD = max3(a2, b2, NEG);
+            /* matchMax with its quirk: I wins over M, otherwise D if above M */
+            uint32_t dec = dI > dM ? 1u : (dD > dM ? 2u : 0u);
+            dec |= (a1 > b1 ? 0u : 4u) | (a2 > b2 ? 0u : 8u);
+            if (MODE == 1) dec |= m > 0 ? 16u : 0u;
+            acc |= (uint64_t)dec << ((t & dmask) * dbits);
+            cM = valid ? m : cM;
+            cI = valid ? nI : cI;
+            cD = valid ? nD : cD;
+            if (MODE == 1 && valid && m > sbv) { sbv = m; sbj = R.c0 + jj; }
+            if (valid && l == rows - 1) { TM[jj + 1] = m; TI[jj + 1] = nI; TD[jj + 1] = nD; }
+            chM = rowNext(chM); chI = rowNext(chI); chD = rowNext(chD); chT = rowNext(chT);
+            if ((t & dmask) == dmask || t == steps - 1) {
+                if (act) dw[(int64_t)(t >> dsh) * rows + l] = acc;
+                acc = 0;
+            }
+            ++t;
+            if (t == steps) {
+                /* strip end: the lanes' last-column cells, the corner of the next strip's top
+                 * row, the LOCAL best of each lane */
+                if (act) G.border[R.boff + C + (i - R.r0)] = make_int4(cM, cI, cD, 0);
+                if (MODE == 1 && act && sbv != NEG && swBetter(sbv, i, sbj, bv, bi, bj)) { bv = sbv; bi = i; bj = sbj; }
+                if (l == rows - 1) { TM[0] = bM0; TI[0] = bI0; TD[0] = bD0; }
+                if (s0 + rows < Rn) {
+                    s0 += 16;
+                    phase = STRIP;
+                } else {
+                    /* rectangle end: its corner, its last row to the border, and in GLOBAL mode
+                     * the scan of R3's last row (row L): first maximum in column order, M before
+                     * I, strict '>' from -inf */
+                    cm = __shfl(cM, grp * 16 + rows - 1, 64);
+                    ci = __shfl(cI, grp * 16 + rows - 1, 64);
+                    cd = __shfl(cD, grp * 16 + rows - 1, 64);
+                    for (int x = l; x < C; x += 16) G.border[R.boff + x] = make_int4(TM[x + 1], TI[x + 1], TD[x + 1], 0);
+                    if (MODE == 0 && ri == g.ns) {
+                        int32_t v = NEG, vx = 0x7fffffff, vs = OP_M;
+                        for (int x = l; x < C; x += 16) {
+                            const int32_t mv = TM[x + 1], iv = TI[x + 1];
+                            if (mv > v) { v = mv; vx = x; vs = OP_M; }
+                            if (iv > v) { v = iv; vx = x; vs = OP_I; }
+                        }
+                        for (int off = 8; off >= 1; off >>= 1) {
+                            const int32_t ov = __shfl_xor(v, off, 16), ox = __shfl_xor(vx, off, 16),
+                                          os = __shfl_xor(vs, off, 16);
+                            if (ov > v || (ov == v && (ox < vx || (ox == vx && os < vs)))) { v = ov; vx = ox; vs = os; }
+                        }
+                        if (v != NEG) { nv = v; nc = R.c0 + vx; nst = vs; }
+                    }
+                    prevOk = true;
+                    ++ri;
+                    phase = RECT;
+                }
+            }
+        }
+    }
+}
+
+template __global__ void dp_fill16_kernel<0>(DevIndex, DpArgs);
+template __global__ void dp_fill16_kernel<1>(DevIndex, DpArgs);
+
 /* ------------------------------------------------------------------ trace */
 
 /* at least 4 waves per SIMD (at most 128 VGPRs).  The band lookups (vM / vI / vD, cellRect,
@@ -614,7 +873,7 @@ __global__ void __launch_bounds__(256, 4) dp_trace_kernel(DevIndex ix, DpArgs a)
         const int32_t L = g.L, ns = g.ns;
         JobSum S;
         S.swV = NEG; S.swI = 0; S.swJ = 0; S.nwV = NEG; S.nwC = 0; S.nwS = OP_M;
-        if (g.cls == CLS_WAVE) {
+        if (g.cls == CLS_WAVE || g.cls == CLS_WAVE16) {
             S = a.sum[job];
         } else {
             /* calculateScores(chain) (src/Alignment.cpp:54-70) in event order, one lane */
@@ -757,7 +1016,7 @@ __global__ void __launch_bounds__(256, 4) dp_trace_kernel(DevIndex ix, DpArgs a)
             uint32_t dec = 0u;
             if (inR) {
                 int sft;
-                const int64_t wi = Rb.decWord(i, j, G.mode, sft);
+                const int64_t wi = Rb.decWord(i, j, G.mode, G.hs, sft);
                 if (wi != decWi) { decWi = wi; decW = ((const uint64_t*)G.bits)[wi]; }
                 dec = (uint32_t)(decW >> sft) & (G.mode == 0 ? 15u : 255u);
             }

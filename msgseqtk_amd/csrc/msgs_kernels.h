@@ -134,7 +134,7 @@ struct AlnRec {
 };
 
 /* DP job classes: no rectangle, wavefront fill (one wave per job), serial fill */
-enum : int32_t { CLS_NONE = 0, CLS_WAVE = 1, CLS_SERIAL = 2 };
+enum : int32_t { CLS_NONE = 0, CLS_WAVE = 1, CLS_SERIAL = 2, CLS_WAVE16 = 3 };
 
 /* per-job geometry in matrix coordinates (row i = query base i-1 + 1, column j =
  * target base tStart + j - 1); R5 = rows 1..r5r1 x cols 1..r5c1, R3 = rows r3r0..L x
@@ -142,7 +142,9 @@ enum : int32_t { CLS_NONE = 0, CLS_WAVE = 1, CLS_SERIAL = 2 };
 struct JobGeo {
     int64_t tStart, tEnd, r3off, r3boff;   /* R3's cell and border offsets (job-relative) */
     int32_t L, tLen, ns, cs;        /* cs: first chain-seed slot of the job */
-    int32_t tid, strand, rid, cls;  /* cls: 0 no rectangle, 1 wavefront fill, 2 serial fill */
+    int32_t tid, strand, rid, cls;  /* cls: 0 no rectangle, 1 wavefront fill (one job per wavefront,
+                                     * 64-row strips), 2 serial fill, 3 wavefront fill of 16 lanes
+                                     * (four jobs per wavefront, 16-row strips) */
     int32_t r5r1, r5c1, r3r0, r3c0;
 };
 /* rectangle summaries left by the fill: the column-major first maximum of M over all
@@ -172,6 +174,9 @@ struct DpArgs {
     int32_t* jobCls;
     const int64_t* waveJobs;   /* jobs filled by the wavefront kernel, largest first */
     int64_t nWaveJobs;
+    const int64_t* wave16Jobs; /* jobs filled 16 lanes each, largest first */
+    int64_t nWave16Jobs;
+    unsigned int* fill16Next;  /* work counter of the 16-lane fill (zero before its launch) */
     const int64_t* tileOff;    /* per job byte offset into the decision pool */
     uint8_t* bits;             /* decisions of wavefront-filled jobs, per 64-row strip of a rectangle:
                                 * one 64-bit word per row and 16 sweep steps (GLOBAL, 4 bits per cell)
@@ -206,6 +211,8 @@ struct DpArgs {
 __global__ void dp_setup_kernel(DevIndex ix, DpArgs a);
 /* rectangle fills, one wavefront per listed job (MODE 0 GLOBAL, 1 LOCAL) */
 template <int MODE> __global__ void dp_fill_kernel(DevIndex ix, DpArgs a);
+/* rectangle fills of small jobs, 16 lanes per job, four jobs per wavefront */
+template <int MODE> __global__ void dp_fill16_kernel(DevIndex ix, DpArgs a);
 /* remaining fills, backtrace, CIGAR, MD and evaluate, one lane per job */
 __global__ void dp_trace_kernel(DevIndex ix, DpArgs a);
 

@@ -76,12 +76,12 @@ __global__ void k_rec_counts(const AlnRec* recs, int64_t nJobs, int64_t* nc, int
 __device__ __forceinline__ int areaBucket(int64_t area) { return area > 0 ? 64 - __clzll((unsigned long long)area) : 0; }
 
 __global__ void __launch_bounds__(256) k_wave_hist(const int32_t* cls, const int64_t* area, int64_t nJobs,
-                                                    unsigned long long* hist) {
+                                                    unsigned long long* hist, int32_t want) {
     __shared__ unsigned int h[64];
     if (threadIdx.x < 64) h[threadIdx.x] = 0;
     __syncthreads();
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < nJobs && cls[j] == CLS_WAVE) atomicAdd(&h[areaBucket(area[j]) & 63], 1u);
+    if (j < nJobs && cls[j] == want) atomicAdd(&h[areaBucket(area[j]) & 63], 1u);
     __syncthreads();
     if (threadIdx.x < 64 && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], (unsigned long long)h[threadIdx.x]);
 }
@@ -95,7 +95,7 @@ __global__ void k_wave_base(const unsigned long long* hist, unsigned long long* 
 }
 
 __global__ void __launch_bounds__(256) k_wave_scatter(const int32_t* cls, const int64_t* area, int64_t nJobs,
-                                                       unsigned long long* cur, int64_t* wave) {
+                                                       unsigned long long* cur, int64_t* wave, int32_t want) {
     __shared__ unsigned int h[64];
     __shared__ unsigned long long base[64];
     if (threadIdx.x < 64) h[threadIdx.x] = 0;
@@ -103,7 +103,7 @@ __global__ void __launch_bounds__(256) k_wave_scatter(const int32_t* cls, const 
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int b = -1;
     unsigned int r = 0;
-    if (j < nJobs && cls[j] == CLS_WAVE) { b = areaBucket(area[j]) & 63; r = atomicAdd(&h[b], 1u); }
+    if (j < nJobs && cls[j] == want) { b = areaBucket(area[j]) & 63; r = atomicAdd(&h[b], 1u); }
     __syncthreads();
     if (threadIdx.x < 64 && h[threadIdx.x]) base[threadIdx.x] = atomicAdd(&cur[threadIdx.x], (unsigned long long)h[threadIdx.x]);
     __syncthreads();
@@ -111,6 +111,7 @@ __global__ void __launch_bounds__(256) k_wave_scatter(const int32_t* cls, const 
 }
 
 __global__ void k_copy_i64(const int64_t* src, int64_t* dst) { if (threadIdx.x == 0) *dst = *src; }
+__global__ void k_zero_u32(unsigned int* p) { if (threadIdx.x == 0) *p = 0; }
 
 __global__ void k_fill_u8(uint8_t* p, int64_t n, uint8_t v) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
@@ -164,6 +165,10 @@ struct DpStage {
     PVec<uint32_t> cigar;
     PVec<char> md;
     PVec<int64_t> tot;   /* device totals read back: tiles, paths, wave jobs, CIGAR ops, MD bytes */
+    double scoreScale = 1;   /* the DP's integer scores are the reference's alnScore x scoreScale */
+    double score(int32_t v) const {
+        return v == NEG ? -std::numeric_limits<double>::infinity() : (double)v / scoreScale;
+    }
 };
 
 static int runSeeds(msgs_ctx* c, const DevReads& r, const msgs_params* p, const SmemStage& sm, SeedStage& out,
@@ -248,7 +253,21 @@ static int runChains(msgs_ctx* c, const DevReads& r, const SeedStage& sd, ChainS
     return MSGS_OK;
 }
 
+/* The reference's DP is double arithmetic on the scores (src/Alignment.cpp:72-133).  When every
+ * score is a multiple of 2^-k, every value the recurrences form is a multiple of 2^-k far below
+ * 2^(53-k) in size, so each double add, subtract, max and compare is exact and equals the same
+ * operation on the integers value x 2^k.  The device DP computes those integers (int32, |score
+ * x 2^k| <= 10000 as for integer scores); k is the smallest in [0, 8] that makes all four
+ * scores integral, and other score sets are refused. */
 static bool integral(double x) { return std::isfinite(x) && x == std::floor(x) && std::fabs(x) <= 10000; }
+static int dyadicShift(const msgs_params* p) {
+    for (int k = 0; k <= 8; ++k) {
+        const double f = std::ldexp(1.0, k);
+        if (integral(p->match * f) && integral(p->mismatch * f) && integral(p->gapO * f) && integral(p->gapE * f))
+            return k;
+    }
+    return -1;
+}
 
 static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msgs_params* p, const SeedStage& sd,
                  const ChainStage& ch, DpStage& out, double* dpMicros, double* phase = nullptr,
@@ -258,8 +277,11 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     auto since = [&t0]() { return std::chrono::duration<double, std::micro>(std::chrono::steady_clock::now() - t0).count(); };
     hipStream_t st = c->x().stream;
     hipError_t e = hipSuccess;
-    if (!integral(p->match) || !integral(p->mismatch) || !integral(p->gapO) || !integral(p->gapE))
-        return fail(c, MSGS_E_UNSUPPORTED, "the device DP needs integer-valued match/mismatch/gap scores");
+    const int shift = dyadicShift(p);
+    if (shift < 0)
+        return fail(c, MSGS_E_UNSUPPORTED,
+                    "the device DP needs match/mismatch/gap scores that are multiples of 1/256 (at most 10000 x 256 "
+                    "in units of 1/256)");
     if (p->gapO < 0 || p->gapE < 0)
         return fail(c, MSGS_E_UNSUPPORTED, "the device DP needs non-negative gap penalties");
     out.recs.clear();
@@ -282,7 +304,10 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     a.sum = c->x().bufs.get<JobSum>("dp.sum", nJobs + 1, e);
     a.jobCls = c->x().bufs.get<int32_t>("dp.cls", nJobs + 1, e);
     a.mode = p->mode;
-    a.match = (int32_t)p->match; a.mismatch = (int32_t)p->mismatch; a.gapO = (int32_t)p->gapO; a.gapE = (int32_t)p->gapE;
+    const double sf = std::ldexp(1.0, shift);
+    out.scoreScale = sf;
+    a.match = (int32_t)(p->match * sf); a.mismatch = (int32_t)(p->mismatch * sf);
+    a.gapO = (int32_t)(p->gapO * sf); a.gapE = (int32_t)(p->gapE * sf);
     a.clip = p->clip; a.gapE_d = p->gapE;
     a.dstat = dstat;
     a.forceSerial = c->dpSerial;
@@ -317,27 +342,32 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     int64_t* dBorderOff = c->x().bufs.get<int64_t>("dp.boff", nJobs + 1, e);
     int64_t* dSerialOff = c->x().bufs.get<int64_t>("dp.soff", nJobs + 1, e);
     int64_t* dWave = c->x().bufs.get<int64_t>("dp.wave", nJobs + 1, e);
-    unsigned long long* dHist = c->x().bufs.get<unsigned long long>("dp.whist", 128, e);
+    int64_t* dWave16 = c->x().bufs.get<int64_t>("dp.wave16", nJobs + 1, e);
+    unsigned long long* dHist = c->x().bufs.get<unsigned long long>("dp.whist", 256, e);
     int64_t* dTot = c->x().bufs.get<int64_t>("dp.tot", 8, e);
+    unsigned int* dNext = c->x().bufs.get<unsigned int>("dp.next16", 1, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "dp scans");
-    CK(hipMemsetAsync(dHist, 0, 128 * 8, st), "hist");
+    CK(hipMemsetAsync(dHist, 0, 256 * 8, st), "hist");
     CK(hipcub::DeviceScan::ExclusiveSum(scanTmp, scanBytes, a.tileNeed, dTileOff, (int)(nJobs + 1), st), "tile scan");
     CK(hipcub::DeviceScan::ExclusiveSum(scanTmp, scanBytes, a.pathNeed, dPathOff, (int)(nJobs + 1), st), "path scan");
     CK(hipcub::DeviceScan::ExclusiveSum(scanTmp, scanBytes, a.borderNeed, dBorderOff, (int)(nJobs + 1), st), "border scan");
     CK(hipcub::DeviceScan::ExclusiveSum(scanTmp, scanBytes, a.serialNeed, dSerialOff, (int)(nJobs + 1), st), "serial scan");
-    k_wave_hist<<<nb, 256, 0, st>>>(a.jobCls, a.tileNeed, nJobs, dHist);
+    k_wave_hist<<<nb, 256, 0, st>>>(a.jobCls, a.tileNeed, nJobs, dHist, CLS_WAVE);
     k_wave_base<<<1, 64, 0, st>>>(dHist, dHist + 64, dTot + 2);
-    k_wave_scatter<<<nb, 256, 0, st>>>(a.jobCls, a.tileNeed, nJobs, dHist + 64, dWave);
+    k_wave_scatter<<<nb, 256, 0, st>>>(a.jobCls, a.tileNeed, nJobs, dHist + 64, dWave, CLS_WAVE);
+    k_wave_hist<<<nb, 256, 0, st>>>(a.jobCls, a.tileNeed, nJobs, dHist + 128, CLS_WAVE16);
+    k_wave_base<<<1, 64, 0, st>>>(dHist + 128, dHist + 192, dTot + 5);
+    k_wave_scatter<<<nb, 256, 0, st>>>(a.jobCls, a.tileNeed, nJobs, dHist + 192, dWave16, CLS_WAVE16);
     k_copy_i64<<<1, 64, 0, st>>>(dTileOff + nJobs, dTot + 0);
     k_copy_i64<<<1, 64, 0, st>>>(dPathOff + nJobs, dTot + 1);
     k_copy_i64<<<1, 64, 0, st>>>(dBorderOff + nJobs, dTot + 3);
     k_copy_i64<<<1, 64, 0, st>>>(dSerialOff + nJobs, dTot + 4);
     out.tot.resize(8);
-    CK(hipMemcpyAsync(out.tot.data(), dTot, 5 * 8, hipMemcpyDeviceToHost, st), "dp totals");
+    CK(hipMemcpyAsync(out.tot.data(), dTot, 6 * 8, hipMemcpyDeviceToHost, st), "dp totals");
     CK(hipStreamSynchronize(st), "dp setup");
     const int64_t tiles = out.tot[0], paths = out.tot[1], nWave = out.tot[2], borders = out.tot[3],
-                  serialCells = out.tot[4];
-    if (phase) { phase[0] = (double)nJobs; phase[1] = (double)nWave; phase[2] = since(); }
+                  serialCells = out.tot[4], nWave16 = out.tot[5];
+    if (phase) { phase[0] = (double)nJobs; phase[1] = (double)(nWave + nWave16); phase[2] = since(); }
     a.bits = c->x().bufs.get<uint8_t>("dp.bits", tiles + 16, e);   /* read in 16-byte chunks */
     a.border = c->x().bufs.get<int4>("dp.border", borders + 1, e);
     a.tile = c->x().bufs.get<int4>("dp.tile", serialCells + 1, e);
@@ -353,6 +383,7 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "dp pools");
     a.tileOff = dTileOff; a.pathOff = dPathOff; a.borderOff = dBorderOff; a.serialOff = dSerialOff;
     a.waveJobs = dWave; a.nWaveJobs = nWave;
+    a.wave16Jobs = dWave16; a.nWave16Jobs = nWave16; a.fill16Next = dNext;
     CK(hipEventRecord(c->x().ev2, st), "event");
     if (nWave) {
         /* persistent: as many one-wavefront blocks per CU as are resident at once (blocks
@@ -369,6 +400,23 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
         const int64_t blocks = std::min<int64_t>(nWave, (int64_t)std::max(1, c->nCU) * fillPerCU[p->mode ? 1 : 0]);
         if (p->mode == 0) dp_fill_kernel<0><<<(unsigned)blocks, 64, 0, st>>>(c->db.view(), a);
         else dp_fill_kernel<1><<<(unsigned)blocks, 64, 0, st>>>(c->db.view(), a);
+    }
+    if (nWave16) {
+        /* persistent, four jobs per block; groups take jobs from a counter until none is left */
+        static const int fill16PerCU[2] = {[]() {
+            int nb = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dp_fill16_kernel<0>, 64, 0) != hipSuccess || nb < 1) nb = 8;
+            return nb;
+        }(), []() {
+            int nb = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, dp_fill16_kernel<1>, 64, 0) != hipSuccess || nb < 1) nb = 8;
+            return nb;
+        }()};
+        const int64_t blocks = std::min<int64_t>((nWave16 + 3) / 4,
+                                                 (int64_t)std::max(1, c->nCU) * fill16PerCU[p->mode ? 1 : 0]);
+        k_zero_u32<<<1, 64, 0, st>>>(dNext);
+        if (p->mode == 0) dp_fill16_kernel<0><<<(unsigned)blocks, 64, 0, st>>>(c->db.view(), a);
+        else dp_fill16_kernel<1><<<(unsigned)blocks, 64, 0, st>>>(c->db.view(), a);
     }
     dp_trace_kernel<<<gridFor(nJobs), 256, 0, st>>>(c->db.view(), a);
     CK(hipEventRecord(c->x().ev3, st), "event");
@@ -542,7 +590,7 @@ struct Finisher {
         const int64_t job = &r - dp.recs.data();
         const bool rev = r.strand == REV;
         int flag = (rev ? 16 : 0) | flagOr;
-        int64_t tags[3] = {(int64_t)(int)std::round((double)r.score), nh, xn};
+        int64_t tags[3] = {(int64_t)(int)std::round(dp.score(r.score)), nh, xn};
         line(s, rd, rev, flag, r.tid, r.alnStart, mapq, dp.cigar.data() + dp.cigOff[job], r.nCigar, isize, tags,
              dp.md.data() + dp.mdOff[job], r.mdLen, (float)a.log10P, (float)postP);
     }
@@ -561,7 +609,7 @@ struct Finisher {
         const double thr = (double)rd.L * P.minScoreRate;
         for (int64_t j = j0; j < j1; ++j) {
             const AlnRec& r = dp.recs[j];
-            double sc = r.score == NEG ? -std::numeric_limits<double>::infinity() : (double)r.score;
+            const double sc = dp.score(r.score);
             if (!(sc < thr)) v.push_back(HAln{&r, r.log10P});
         }
         tk[3] += __rdtsc() - t0;
@@ -797,7 +845,7 @@ static int impl_align_batch(msgs_ctx* c, const msgs_reads* reads, const msgs_par
         int64_t* o = rows10 + 10 * j;
         o[0] = a.tid; o[1] = a.strand; o[2] = a.tStart; o[3] = a.tEnd; o[4] = a.alnFrom; o[5] = a.alnTo;
         o[6] = a.alnStart; o[7] = a.alnEnd; o[8] = a.hazard; o[9] = a.cells;
-        if (scores) scores[j] = a.score == NEG ? -std::numeric_limits<double>::infinity() : (double)a.score;
+        if (scores) scores[j] = b.dp.score(a.score);
         cig += F.cigarOf(j);
         cig += '\n';
     }
@@ -1004,7 +1052,7 @@ static int impl_dp_batch(msgs_ctx* c, const msgs_reads* reads, const int64_t* ch
             o[0] = a.tid; o[1] = a.strand; o[2] = a.tStart; o[3] = a.tEnd; o[4] = a.alnFrom; o[5] = a.alnTo;
             o[6] = a.alnStart; o[7] = a.alnEnd; o[8] = a.hazard; o[9] = a.cells;
         }
-        if (scores) scores[k] = a.score == NEG ? -std::numeric_limits<double>::infinity() : (double)a.score;
+        if (scores) scores[k] = dp.score(a.score);
         if (log10P) log10P[k] = a.log10P;
         cig += F.cigarOf(k);
         cig += '\n';

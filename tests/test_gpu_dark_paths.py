@@ -7,7 +7,8 @@
     --no-contain, --no-overlap, --max-npair; src/Alignment.cpp:556-571) on pairs built to need
     them: short inserts (overlapping, contained and tail-over mates), mates from other pairs
     (discordant) and random mates (mixed);
-  * non-default integer score sets in both alignment modes;
+  * non-default score sets in both alignment modes: integers, and multiples of 2^-k (whose double
+    arithmetic is exact, computed as integers scaled by 2^k); other scores are refused;
   * the quality clamp max(q - 33, 2) of the FASTQ reader (src/QualStr.h:40-42) through the
     driver."""
 import os
@@ -162,10 +163,12 @@ SCORE_SETS = [
     dict(match=1, mismatch=1, gapO=0, gapE=1),
     dict(match=3, mismatch=2, gapO=10, gapE=3),
     dict(match=1, mismatch=5, gapO=2, gapE=0),
+    dict(match=1, mismatch=2.5, gapO=4.5, gapE=1.5),        # multiples of 2^-k: exact in doubles
+    dict(match=0.75, mismatch=3.25, gapO=5.125, gapE=2.0),
 ]
 
 
-@pytest.mark.parametrize("scores", SCORE_SETS, ids=["m%(match)d-x%(mismatch)d-o%(gapO)d-e%(gapE)d" % s for s in SCORE_SETS])
+@pytest.mark.parametrize("scores", SCORE_SETS, ids=["m%(match)g-x%(mismatch)g-o%(gapO)g-e%(gapE)g" % s for s in SCORE_SETS])
 def test_integer_score_sets(env, scores):
     chroms, names, odb, m = env
     seqs, quals, rn, _ = synth.simulate_reads(chroms, 300, (40, 300), err_range=(0.0, 0.06), seed=190)
@@ -176,6 +179,15 @@ def test_integer_score_sets(env, scores):
         _same_alignments(odb, m, seqs, quals, po, pm)
         sam1, _ = O.orc_map(odb, po, seqs, quals, rn)
         _same_sam(sam1, m.map((seqs, quals, rn), pm))
+
+
+def test_non_dyadic_scores_refused(env):
+    """scores that are not multiples of 1/256 would need the reference's rounding of every double
+    operation: refused with an error, not approximated"""
+    chroms, names, odb, m = env
+    seqs, quals, rn, _ = synth.simulate_reads(chroms, 20, 100, seed=195)
+    with pytest.raises(RuntimeError, match="multiples of 1/256"):
+        m.alignments(seqs, quals, M.default_params(mismatch=2.1))
 
 
 def test_driver_quality_clamp(tmp_path):
