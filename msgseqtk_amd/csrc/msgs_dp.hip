@@ -49,9 +49,9 @@ constexpr int FILL_NS = 64;
 constexpr int FILL_COLS = 512;
 /* limits of the 16-lane fill (four jobs per wavefront): seeds per job and rectangle columns;
  * its LDS holds, per job, the seeds (512 B) and the row above the current strip (M, I, D and
- * the target bases: 3.4 KB), 15.4 KB per four-job block */
+ * the target bases: 2.8 KB), 13.2 KB per four-job block */
 constexpr int NS16 = 32;
-constexpr int COLS16 = 256;
+constexpr int COLS16 = 192;
 
 __device__ __forceinline__ int32_t addn(int32_t a, int32_t b) { return a == NEG ? NEG : a + b; }
 __device__ __forceinline__ int32_t subn(int32_t a, int32_t b) { return a == NEG ? NEG : a - b; }
@@ -242,7 +242,8 @@ __device__ __forceinline__ int32_t vIk(const Geo& G, int32_t i, int32_t j, int e
     return initI(G, i, j);
 }
 
-__device__ __forceinline__ int32_t vDk(const Geo& G, int32_t i, int32_t j, int e, int k) {
+/* D at (i, j) after events 1..e; k = rowSeg(i), m = colSeg(j) */
+__device__ __forceinline__ int32_t vDkm(const Geo& G, int32_t i, int32_t j, int e, int k, int m) {
     if (k < 0) {
         const Rc R = rectR5(G);
         if (i >= 1 && R.ok && e >= 1 && R.hasCol(j)) return rectVal(G, R, i, j).z;
@@ -254,7 +255,6 @@ __device__ __forceinline__ int32_t vDk(const Geo& G, int32_t i, int32_t j, int e
         }
     }
     /* D col end_m + 1 := M col end_m + o at seed m's event; m is colSeg(j) or the one before */
-    const int m = colSeg(G, j);
     if (m >= 0) {
         const int4 s = G.sg[m];
         if (j == s.y + s.z && e >= 2 + 2 * m) return addn(vMk(G, i, j - 1, 2 + 2 * m, k), G.o);
@@ -264,6 +264,9 @@ __device__ __forceinline__ int32_t vDk(const Geo& G, int32_t i, int32_t j, int e
         }
     }
     return NEG;
+}
+__device__ __forceinline__ int32_t vDk(const Geo& G, int32_t i, int32_t j, int e, int k) {
+    return vDkm(G, i, j, e, k, colSeg(G, j));
 }
 
 __device__ __forceinline__ bool inside(Geo& G, int32_t i, int32_t j) {
@@ -629,7 +632,7 @@ template __global__ void dp_fill_kernel<1>(DevIndex, DpArgs);
  * one loop, so the groups need not be in the same phase.  The values, decisions and summaries
  * are those of dp_fill_kernel. */
 template <int MODE>
-__global__ void __launch_bounds__(64) dp_fill16_kernel(DevIndex ix, DpArgs a) {
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) dp_fill16_kernel(DevIndex ix, DpArgs a) {
     constexpr int W = COLS16 + 32;   /* chunk reads reach 31 entries past a row of COLS16 */
     __shared__ int4 sSG[4][NS16];
     __shared__ int32_t sTM[4][W], sTI[4][W], sTD[4][W];
@@ -665,141 +668,151 @@ __global__ void __launch_bounds__(64) dp_fill16_kernel(DevIndex ix, DpArgs a) {
     int32_t nv = NEG, nc = 0, nst = OP_M;                   /* GLOBAL: scan of R3's last row */
 
     while (__any(phase != DONE)) {
-        if (phase == FETCH) {
-            unsigned int k = 0;
-            if (l == 0) k = atomicAdd(a.fill16Next, 1u);
-            k = __shfl(k, grp * 16, 64);
-            if ((int64_t)k >= a.nWave16Jobs) {
-                phase = DONE;
-            } else {
-                job = a.wave16Jobs[k];
-                g = a.geo[job];
-                for (int x = l; x < g.ns; x += 16) SG[x] = a.auxSG[g.cs + x];
-                makeGeo(G, a, g, SG, job);
-                q = ((g.strand == FWD) ? a.seqs : a.rcSeqs) + a.offs[g.rid];
-                tg = ix.tgt + ix.tgtOff[g.tid];
-                chromLen = ix.chromLen[g.tid];
-                ri = 0; prevOk = false;
-                bv = NEG; bi = 0; bj = 0;
-                nv = NEG; nc = 0; nst = OP_M;
-                phase = RECT;
-            }
-        }
-        if (phase == RECT) {
-            /* the next rectangle in event order, after the first diagonal value of each seed
-             * passed (the corner of the rectangle just filled, or the write history) */
-            bool found = false;
-            while (ri <= g.ns) {
-                if (ri >= 1) {
-                    const int k = ri - 1;
-                    const int32_t d0 = prevOk ? max(max3(addn(cm, match), addn(cd, match), addn(ci, match)), 0)
-                                              : seedD0(G, k);
-                    if (l == 0) SG[k].w = d0;
+        /* ---- control, per group: until it is stepping a strip or out of jobs ---- */
+        while (phase == FETCH || phase == RECT || phase == STRIP) {
+            if (phase == FETCH) {
+                unsigned int k = 0;
+                if (l == 0) k = atomicAdd(a.fill16Next, 1u);
+                k = __shfl(k, grp * 16, 64);
+                if ((int64_t)k >= a.nWave16Jobs) {
+                    phase = DONE;
+                } else {
+                    job = a.wave16Jobs[k];
+                    g = a.geo[job];
+                    for (int x = l; x < g.ns; x += 16) SG[x] = a.auxSG[g.cs + x];
+                    makeGeo(G, a, g, SG, job);
+                    q = ((g.strand == FWD) ? a.seqs : a.rcSeqs) + a.offs[g.rid];
+                    tg = ix.tgt + ix.tgtOff[g.tid];
+                    chromLen = ix.chromLen[g.tid];
+                    ri = 0; prevOk = false;
+                    bv = NEG; bi = 0; bj = 0;
+                    nv = NEG; nc = 0; nst = OP_M;
+                    phase = RECT;
                 }
-                R = ri == 0 ? rectR5(G) : (ri < g.ns ? rectGap(G, ri - 1, SG[ri - 1], SG[ri]) : rectR3(G));
-                prevOk = R.ok;
-                if (R.ok) { found = true; break; }
-                ++ri;
-            }
-            if (!found) {
-                /* job end: the seeds' first diagonal values and the summaries */
-                for (int x = l; x < g.ns; x += 16) a.auxSG[g.cs + x].w = SG[x].w;
-                if (MODE == 1) {
-                    for (int off = 8; off >= 1; off >>= 1) {
-                        const int32_t ov = __shfl_xor(bv, off, 16), oi = __shfl_xor(bi, off, 16),
-                                      oj = __shfl_xor(bj, off, 16);
-                        if (swBetter(ov, oi, oj, bv, bi, bj)) { bv = ov; bi = oi; bj = oj; }
+            } else if (phase == RECT) {
+                /* the next rectangle in event order, after the first diagonal value of each
+                 * seed passed (the corner of the rectangle just filled, or the write history) */
+                bool found = false;
+                while (ri <= g.ns) {
+                    if (ri >= 1) {
+                        const int k = ri - 1;
+                        const int32_t d0 = prevOk ? max(max3(addn(cm, match), addn(cd, match), addn(ci, match)), 0)
+                                                  : seedD0(G, k);
+                        if (l == 0) SG[k].w = d0;
                     }
+                    R = ri == 0 ? rectR5(G) : (ri < g.ns ? rectGap(G, ri - 1, SG[ri - 1], SG[ri]) : rectR3(G));
+                    prevOk = R.ok;
+                    if (R.ok) { found = true; break; }
+                    ++ri;
                 }
-                if (l == 0) {
-                    JobSum S;
-                    S.swV = bv; S.swI = bi; S.swJ = bj; S.nwV = nv; S.nwC = nc; S.nwS = nst; S.pad0 = 0; S.pad1 = 0;
-                    a.sum[job] = S;
-                }
-                phase = FETCH;
-            } else {
-                /* the row above the rectangle (read at event R.ev - 1) and its target bases
-                 * (N read as 0, so that one AND decides a match) */
-                C = R.c1 - R.c0 + 1;
-                Rn = R.r1 - R.r0 + 1;
-                const int e = R.ev;
-                for (int x = l; x <= C; x += 16) {
-                    const int32_t j = R.c0 - 1 + x;
-                    TM[x] = vM(G, R.r0 - 1, j, e - 1);
-                    TI[x] = vI(G, R.r0 - 1, j, e - 1);
-                    TD[x] = vD(G, R.r0 - 1, j, e - 1);
-                    if (x < C) {
-                        const int64_t tp = g.tStart + (R.c0 + x) - 1;
-                        const uint8_t b = (tp >= 0 && tp <= chromLen) ? tg[tp] : (uint8_t)0;
-                        TG[x] = b == 15 ? (uint8_t)0 : b;
+                if (!found) {
+                    /* job end: the seeds' first diagonal values and the summaries */
+                    for (int x = l; x < g.ns; x += 16) a.auxSG[g.cs + x].w = SG[x].w;
+                    if (MODE == 1) {
+                        for (int off = 8; off >= 1; off >>= 1) {
+                            const int32_t ov = __shfl_xor(bv, off, 16), oi = __shfl_xor(bi, off, 16),
+                                          oj = __shfl_xor(bj, off, 16);
+                            if (swBetter(ov, oi, oj, bv, bi, bj)) { bv = ov; bi = oi; bj = oj; }
+                        }
                     }
+                    if (l == 0) {
+                        JobSum S;
+                        S.swV = bv; S.swI = bi; S.swJ = bj; S.nwV = nv; S.nwC = nc; S.nwS = nst; S.pad0 = 0; S.pad1 = 0;
+                        a.sum[job] = S;
+                    }
+                    phase = FETCH;
+                } else {
+                    /* the row above the rectangle (at event R.ev - 1) and its target bases (N
+                     * read as 0, so that one AND decides a match).  The row and the columns
+                     * R touches lie in the band of the seed before it (ri - 1, -1 for R5) in
+                     * both directions, so no band search is needed */
+                    C = R.c1 - R.c0 + 1;
+                    Rn = R.r1 - R.r0 + 1;
+                    const int e = R.ev, kb = ri - 1;
+                    for (int x = l; x <= C; x += 16) {
+                        const int32_t j = R.c0 - 1 + x;
+                        TM[x] = vMk(G, R.r0 - 1, j, e - 1, kb);
+                        TI[x] = vIk(G, R.r0 - 1, j, e - 1, kb);
+                        TD[x] = vDkm(G, R.r0 - 1, j, e - 1, kb, kb);
+                        if (x < C) {
+                            const int64_t tp = g.tStart + (R.c0 + x) - 1;
+                            const uint8_t b = (tp >= 0 && tp <= chromLen) ? tg[tp] : (uint8_t)0;
+                            TG[x] = b == 15 ? (uint8_t)0 : b;
+                        }
+                    }
+                    s0 = 0;
+                    phase = STRIP;
                 }
-                s0 = 0;
-                phase = STRIP;
-            }
-        }
-        if (phase == STRIP) {
-            rows = min(16, Rn - s0);
-            i = R.r0 + s0 + l;
-            act = l < rows;
-            const int e = R.ev;
-            cM = NEG; cI = NEG; cD = NEG;
-            if (act) {
-                cM = vM(G, i, R.c0 - 1, e - 1);
-                cI = vI(G, i, R.c0 - 1, e - 1);
-                cD = vD(G, i, R.c0 - 1, e - 1);
-            }
-            bM0 = cM; bI0 = cI; bD0 = cD;
-            const uint8_t qb = act ? q[i - 1] : (uint8_t)0;
-            qm = qb == 15 ? 0 : qb;
-            pM = TM[0]; pI = TI[0]; pD = TD[0];
-            chM = TM[1 + l]; chI = TI[1 + l]; chD = TD[1 + l]; chT = TG[l];
-            tbv = 0;
-            t = 0;
-            steps = C + rows - 1;
-            dw = (uint64_t*)G.bits + (R.off >> 3) + (int64_t)(s0 >> 4) * stripWords(16, C, MODE);
-            acc = 0;
-            sbv = NEG; sbj = 0;
-            phase = STEP;
-        }
-        if (phase == STEP) {
-            if ((t & 15) == 0 && t > 0) {
-                /* lane k of the chunk: row entry t + 1 + k (M, I, D), target entry t + k */
-                chM = TM[t + 1 + l]; chI = TI[t + 1 + l]; chD = TD[t + 1 + l]; chT = TG[t + l];
-            }
-            /* up = (i - 1, j): row above, one step ago (lane 0: the row above the strip);
-             * diagonal = (i - 1, j - 1): the up value of the step before */
-            const int32_t uM = rowUp1(chM, cM), uI = rowUp1(chI, cI), uD = rowUp1(chD, cD);
-            tbv = rowUp1(chT, tbv);
-            const int32_t dM = pM, dI = pI, dD = pD;
-            pM = uM; pI = uI; pD = uD;
-            const int32_t jj = t - l;
-            const bool valid = act && (uint32_t)jj < (uint32_t)C;
-            const int32_t sc = (qm & tbv) ? match : -mis;
-            /* the recurrences of dp_fill_kernel: -inf (NEG) absorbs, gaps are non-negative,
-             * and every finite value lies far above NEG, so the comparisons of the backtrace's
-             * decisions need no clamping */
-            const int32_t mx = max3(dM, dD, dI);
-            int32_t m = mx == NEG ? NEG : mx + sc;
-            if (MODE == 1) m = max(m, 0);
-            const int32_t a1 = uM + o, b1 = uI - ge, a2 = cM + o, b2 = cD - ge;
-            const int32_t nI = max3(a1, b1, NEG), nD = max3(a2, b2, NEG);
-            /* matchMax with its quirk: I wins over M, otherwise D if above M */
-            uint32_t dec = dI > dM ? 1u : (dD > dM ? 2u : 0u);
-            dec |= (a1 > b1 ? 0u : 4u) | (a2 > b2 ? 0u : 8u);
-            if (MODE == 1) dec |= m > 0 ? 16u : 0u;
-            acc |= (uint64_t)dec << ((t & dmask) * dbits);
-            cM = valid ? m : cM;
-            cI = valid ? nI : cI;
-            cD = valid ? nD : cD;
-            if (MODE == 1 && valid && m > sbv) { sbv = m; sbj = R.c0 + jj; }
-            if (valid && l == rows - 1) { TM[jj + 1] = m; TI[jj + 1] = nI; TD[jj + 1] = nD; }
-            chM = rowNext(chM); chI = rowNext(chI); chD = rowNext(chD); chT = rowNext(chT);
-            if ((t & dmask) == dmask || t == steps - 1) {
-                if (act) dw[(int64_t)(t >> dsh) * rows + l] = acc;
+            } else {
+                rows = min(16, Rn - s0);
+                i = R.r0 + s0 + l;
+                act = l < rows;
+                const int e = R.ev, kb = ri - 1;
+                cM = NEG; cI = NEG; cD = NEG;
+                if (act) {
+                    cM = vMk(G, i, R.c0 - 1, e - 1, kb);
+                    cI = vIk(G, i, R.c0 - 1, e - 1, kb);
+                    cD = vDkm(G, i, R.c0 - 1, e - 1, kb, kb);
+                }
+                bM0 = cM; bI0 = cI; bD0 = cD;
+                const uint8_t qb = act ? q[i - 1] : (uint8_t)0;
+                qm = qb == 15 ? 0 : qb;
+                pM = TM[0]; pI = TI[0]; pD = TD[0];
+                chM = TM[1 + l]; chI = TI[1 + l]; chD = TD[1 + l]; chT = TG[l];
+                tbv = 0;
+                t = 0;
+                steps = C + rows - 1;
+                dw = (uint64_t*)G.bits + (R.off >> 3) + (int64_t)(s0 >> 4) * stripWords(16, C, MODE);
                 acc = 0;
+                sbv = NEG; sbj = 0;
+                phase = STEP;
             }
-            ++t;
+        }
+        /* ---- steps: as many as the stepping group nearest to its strip end has left, so
+         * that no group changes phase inside the loop ---- */
+        int32_t rem = phase == STEP ? steps - t : 0x7fffffff;
+        rem = min(min(__shfl(rem, 0, 64), __shfl(rem, 16, 64)), min(__shfl(rem, 32, 64), __shfl(rem, 48, 64)));
+        if (rem == 0x7fffffff) continue;   /* no group stepping: all are out of jobs */
+        if (phase == STEP) {
+            for (int32_t r = 0; r < rem; ++r) {
+                if ((t & 15) == 0 && t > 0) {
+                    /* lane k of the chunk: row entry t + 1 + k (M, I, D), target entry t + k */
+                    chM = TM[t + 1 + l]; chI = TI[t + 1 + l]; chD = TD[t + 1 + l]; chT = TG[t + l];
+                }
+                /* up = (i - 1, j): row above, one step ago (lane 0: the row above the strip);
+                 * diagonal = (i - 1, j - 1): the up value of the step before */
+                const int32_t uM = rowUp1(chM, cM), uI = rowUp1(chI, cI), uD = rowUp1(chD, cD);
+                tbv = rowUp1(chT, tbv);
+                const int32_t dM = pM, dI = pI, dD = pD;
+                pM = uM; pI = uI; pD = uD;
+                const int32_t jj = t - l;
+                const bool valid = act && (uint32_t)jj < (uint32_t)C;
+                const int32_t sc = (qm & tbv) ? match : -mis;
+                /* the recurrences of dp_fill_kernel: -inf (NEG) absorbs, gaps are non-negative,
+                 * and every finite value lies far above NEG, so the comparisons of the
+                 * backtrace's decisions need no clamping */
+                const int32_t mx = max3(dM, dD, dI);
+                int32_t m = mx == NEG ? NEG : mx + sc;
+                if (MODE == 1) m = max(m, 0);
+                const int32_t a1 = uM + o, b1 = uI - ge, a2 = cM + o, b2 = cD - ge;
+                const int32_t nI = max3(a1, b1, NEG), nD = max3(a2, b2, NEG);
+                /* matchMax with its quirk: I wins over M, otherwise D if above M */
+                uint32_t dec = dI > dM ? 1u : (dD > dM ? 2u : 0u);
+                dec |= (a1 > b1 ? 0u : 4u) | (a2 > b2 ? 0u : 8u);
+                if (MODE == 1) dec |= m > 0 ? 16u : 0u;
+                acc |= (uint64_t)dec << ((t & dmask) * dbits);
+                cM = valid ? m : cM;
+                cI = valid ? nI : cI;
+                cD = valid ? nD : cD;
+                if (MODE == 1 && valid && m > sbv) { sbv = m; sbj = R.c0 + jj; }
+                if (valid && l == rows - 1) { TM[jj + 1] = m; TI[jj + 1] = nI; TD[jj + 1] = nD; }
+                chM = rowNext(chM); chI = rowNext(chI); chD = rowNext(chD); chT = rowNext(chT);
+                if ((t & dmask) == dmask || t == steps - 1) {
+                    if (act) dw[(int64_t)(t >> dsh) * rows + l] = acc;
+                    acc = 0;
+                }
+                ++t;
+            }
             if (t == steps) {
                 /* strip end: the lanes' last-column cells, the corner of the next strip's top
                  * row, the LOCAL best of each lane */
