@@ -800,17 +800,18 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) dp
                 uint32_t dec = dI > dM ? 1u : (dD > dM ? 2u : 0u);
                 dec |= (a1 > b1 ? 0u : 4u) | (a2 > b2 ? 0u : 8u);
                 if (MODE == 1) dec |= m > 0 ? 16u : 0u;
-                acc |= (uint64_t)dec << ((t & dmask) * dbits);
+                /* decisions enter the word from the top; a full word has step t & ~dmask in
+                 * its low bits, and older steps have been shifted out */
+                acc = (acc >> dbits) | ((uint64_t)dec << (64 - dbits));
                 cM = valid ? m : cM;
                 cI = valid ? nI : cI;
                 cD = valid ? nD : cD;
                 if (MODE == 1 && valid && m > sbv) { sbv = m; sbj = R.c0 + jj; }
                 if (valid && l == rows - 1) { TM[jj + 1] = m; TI[jj + 1] = nI; TD[jj + 1] = nD; }
                 chM = rowNext(chM); chI = rowNext(chI); chD = rowNext(chD); chT = rowNext(chT);
-                if ((t & dmask) == dmask || t == steps - 1) {
-                    if (act) dw[(int64_t)(t >> dsh) * rows + l] = acc;
-                    acc = 0;
-                }
+                const int32_t pos = t & dmask;
+                if (act && (pos == dmask || t == steps - 1))
+                    dw[(int64_t)(t >> dsh) * rows + l] = acc >> ((dmask - pos) * dbits);
                 ++t;
             }
             if (t == steps) {
