@@ -256,6 +256,12 @@ def main():
         out["roofline"]["traffic"] = tr
         out["roofline"]["traffic_source"] = tr_src
         out["roofline"]["traffic_per_algorithmic_byte"] = round(tr / bytes_per_launch, 3)
+    tg, _ = profile_value("traffic_bytes_per_launch_gather_scaled", tag)
+    if tg is not None:
+        # FETCH_SIZE scaled as for random 64-B gathers (factor 1.0) instead of the copy's 128-B
+        # requests (factor 2.0): the lower end of the kernel's traffic
+        out["roofline"]["traffic_gather_scaled"] = tg
+        out["roofline"]["traffic_gather_scaled_per_algorithmic_byte"] = round(tg / bytes_per_launch, 3)
     hr, hr_src = profile_value("l2_hit_rate", tag)
     if hr is not None:
         out["roofline"]["l2_hit_rate"] = hr
