@@ -226,6 +226,20 @@ int msgs_chain_batch(msgs_ctx* ctx, const int64_t* rows6, const int64_t* seedOff
 int msgs_dp_batch(msgs_ctx* ctx, const msgs_reads* reads, const int64_t* chainOffs, const int64_t* chainSeedOffs,
                   const int64_t* rows6, const msgs_params* p, int64_t* rows10, double* scores, double* log10P,
                   char* cigars, int64_t cigCap, char* mds, int64_t mdCap, int64_t* needed);
+/* AlignmentPE::getPairs + sort + filter (src/Alignment.cpp:556-581, src/Alignment.h:635-639) on
+ * the device over caller alignment lists, as the PE driver calls them (src/msgseqtk-align.cpp:
+ * 712-717): for read pair i (i < n) the read's alignments are rows alnOffs[i] .. alnOffs[i+1] and
+ * its mate's rows alnOffs[n+i] .. alnOffs[n+i+1] of rows4 (tid, strand 1 = FWD / 2 = REV,
+ * alnStart, alnEnd) and log10P (Alignment::log10lik), the lists as Alignment::filter left them
+ * (alnOffs[0] = 0, 2n+1 entries).  Uses p->insMean, insSD, insMin, insMax (PairingScheme's
+ * fields after updateParam / updateRange), noDiscordant, noTailOver, noContain, noOverlap,
+ * maxNPair.  Output: pairCounts[i] pairs of read pair i, and read pair after read pair, each
+ * pair's rows3 (row of the read's alignment, row of the mate's, insert size) and lik2
+ * (AlignmentPE::log10lik, loglik) in the reference's order; either may be NULL.  *needed =
+ * pairs; MSGS_E_CAP when more than cap (pairCounts is filled). */
+int msgs_pair_batch(msgs_ctx* ctx, int64_t n, const int64_t* alnOffs, const int64_t* rows4, const double* log10P,
+                    const msgs_params* p, int64_t* pairCounts, int64_t* pairRows3, double* pairLik2, int64_t cap,
+                    int64_t* needed);
 
 /* ---- resident batches (bench): reads copied to HBM once, mapped many times ----
  * slot 0..15.  msgs_batch_run maps the resident batch and keeps the records on the
@@ -314,6 +328,10 @@ void msgs_util_sort_by_key(const double* keys, int32_t* perm, int64_t n);
 /* Alignment::calcMapQ (src/Alignment.cpp:491-512) and, pe = 1, AlignmentPE::calcMapQ (:514-539)
  * over n log-likelihoods (natural log): posterior probabilities and MAPQ; runs on the host. */
 void msgs_util_mapq(const double* loglik, int64_t n, int32_t pe, double* postP, uint8_t* mapQ);
+/* PairingScheme::pr (src/PairingScheme.cpp:18-25) of the insert model p->insMean, insSD, insMin,
+ * insMax at x: boost's normal pdf inside [insMin, insMax], 0 outside, 1 when insMean is 0 -- the
+ * values the device pairing's likelihood terms are made from; runs on the host. */
+double msgs_util_insert_pr(const msgs_params* p, double x);
 
 #ifdef __cplusplus
 }

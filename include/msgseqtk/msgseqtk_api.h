@@ -18,6 +18,7 @@
 #ifndef MSGSEQTK_API_H
 #define MSGSEQTK_API_H
 
+#include <cmath>
 #include <cstdint>
 #include <istream>
 #include <memory>
@@ -307,6 +308,7 @@ public:
     int32_t getAlnTLen() const { return (int32_t)(alnEnd - alnStart); }
     double getAlnScore() const { return alnScore; }
     double getLog10P() const { return log10P; }
+    double log10lik() const { return log10P; }
     double loglik() const;
     double getPostP() const { return postP; }
     uint8_t getMapQ() const { return mapQ; }
@@ -325,6 +327,7 @@ public:
     static ALIGN_LIST& calcMapQ(ALIGN_LIST& alnList);
     static ALIGN_LIST& sort(ALIGN_LIST& alnList);
 private:
+    friend class AlignmentPE;
     const PrimarySeq* read = nullptr;
     int32_t tid = -1;
     GLoc::STRAND qStrand = GLoc::UNK;
@@ -332,6 +335,82 @@ private:
     double alnScore = 0, log10P = 0, postP = 0;
     uint8_t mapQ = 0xff;
     string cigar, md;
+};
+
+/* the insert model (src/PairingScheme.h:26-122, src/PairingScheme.cpp:14-25): a normal density
+ * truncated to [min, max], or 1 everywhere when the mean is 0 */
+class PairingScheme {
+public:
+    PairingScheme() : m(DEFAULT_MEAN_INSERT) { updateParam(); updateRange(); }
+    double getMean() const { return m; }
+    double getSD() const { return s; }
+    double getMin() const { return min; }
+    double getMax() const { return max; }
+    void setMean(double v) { m = v; }
+    void setSD(double v) { s = v; }
+    void setMin(double v) { min = v; }
+    void setMax(double v) { max = v; }
+    void setRange(double lo, double hi) { min = lo; max = hi; }
+    void updateParam() {
+        if (m == 0) s = 1;
+        if (s == 0) s = m * DEFAULT_CV_INSERT;
+    }
+    void updateRange() {
+        if (m > 0 && s > 0) {
+            min = m - s * DEFAULT_OUTLIER_DIST;
+            max = m + s * DEFAULT_OUTLIER_DIST;
+        }
+    }
+    double pr(double x) const;
+    double loglik(double x) const { return std::log(pr(x)); }
+    double log10lik(double x) const { return std::log10(pr(x)); }
+    static const double DEFAULT_MEAN_INSERT, DEFAULT_CV_INSERT, DEFAULT_OUTLIER_DIST;
+private:
+    double m = 0, s = 0, min = 0, max = 0;
+};
+
+class AlignmentPE;
+typedef vector<AlignmentPE> PAIR_LIST;
+
+/* a read's alignment with its mate's (src/Alignment.h:446-575) */
+class AlignmentPE {
+public:
+    AlignmentPE() = default;
+    AlignmentPE(const Alignment* fwd, const Alignment* rev) : fwdAln(fwd), revAln(rev) {
+        setInsertSize(getInsertSize(*fwd, *rev));
+    }
+    bool isConcordant() const { return fwdAln->tid == revAln->tid && (fwdAln->qStrand & revAln->qStrand) == 0; }
+    bool isTailOver() const {
+        return (fwdAln->alnStart <= revAln->alnStart && fwdAln->alnEnd > revAln->alnStart) ||
+               (revAln->alnStart <= fwdAln->alnStart && revAln->alnEnd > fwdAln->alnStart);
+    }
+    bool isContained() const {
+        return (fwdAln->alnStart <= revAln->alnStart && fwdAln->alnEnd >= revAln->alnEnd) ||
+               (revAln->alnStart <= fwdAln->alnStart && revAln->alnEnd >= fwdAln->alnEnd);
+    }
+    bool isOverlap() const { return fwdAln->alnStart < revAln->alnEnd && fwdAln->alnEnd > revAln->alnStart; }
+    double getAlnScore() const { return fwdAln->alnScore + revAln->alnScore; }
+    double log10lik() const { return fwdAln->log10lik() + revAln->log10lik() + ps.log10lik(getInsertSize()); }
+    double loglik() const { return fwdAln->loglik() + revAln->loglik() + ps.loglik(getInsertSize()); }
+    int32_t getInsertSize() const { return isize; }
+    void setInsertSize(int32_t v) { isize = v; }
+    uint8_t getMapQ() const { return mapQ; }
+    double getPostP() const { return postP; }
+
+    const Alignment* fwdAln = nullptr;
+    const Alignment* revAln = nullptr;
+    int32_t isize = -1;
+    uint8_t mapQ = 0xff;
+    double postP = 0;
+
+    static PAIR_LIST getPairs(const ALIGN_LIST& fwdAlnList, const ALIGN_LIST& revAlnList);
+    static PAIR_LIST& filter(PAIR_LIST& pairList, bool noDiscordant, bool noTailOver, bool noContain, bool noOverlap,
+                             int64_t maxNPair = MAX_NPAIR);
+    static PAIR_LIST& calcMapQ(PAIR_LIST& pairList);
+    static PAIR_LIST& sort(PAIR_LIST& pairList);
+    static int32_t getInsertSize(const Alignment& lhs, const Alignment& rhs);
+    static PairingScheme ps;
+    static const int64_t MAX_NPAIR = 0;
 };
 
 }  // namespace MSGseqTK

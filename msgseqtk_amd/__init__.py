@@ -29,6 +29,7 @@ EXPORTS = [
     "msgs_batch_submit", "msgs_batch_wait", "msgs_tune", "msgs_util_calib_copy", "msgs_util_random_gather", "msgs_util_format_g6", "msgs_ctx_get", "msgs_lix_train", "msgs_lix_query", "msgs_lix_predict", "msgs_lix_grad", "msgs_lix_weights", "msgs_lix_knots",
     "msgs_smem_batch_noalloc", "msgs_search_batch", "msgs_sa_batch", "msgs_db_chrom_seq", "msgs_chain_batch",
     "msgs_dp_batch", "msgs_fmd_read", "msgs_fmd_write", "msgs_fmd_write_arrays", "msgs_util_mapq",
+    "msgs_pair_batch", "msgs_util_insert_pr",
 ]
 N_LIX_PARAMS = 16 * 64 + 64 + 64 + 1
 
@@ -124,8 +125,11 @@ def lib():
     L.msgs_fmd_read.argtypes = [C.c_char_p, C.c_int32, P64, PU8, P64, P64, P64, P64, P64, P64]
     L.msgs_fmd_write.argtypes = [vp, C.c_char_p, C.c_int32]
     L.msgs_fmd_write_arrays.argtypes = [C.c_char_p, C.c_int32, PU8, P64, C.c_int64]
+    L.msgs_pair_batch.argtypes = [vp, C.c_int64, P64, P64, PD, C.POINTER(Params), P64, P64, PD, C.c_int64, P64]
     L.msgs_util_mapq.argtypes = [PD, C.c_int64, C.c_int32, PD, PU8]
     L.msgs_util_mapq.restype = None
+    L.msgs_util_insert_pr.argtypes = [C.POINTER(Params), C.c_double]
+    L.msgs_util_insert_pr.restype = C.c_double
     _lib = L
     return L
 
@@ -314,6 +318,34 @@ class Mapper:
                 k += 1 + n
             res.append(cs)
         return res
+
+    def pairs(self, aln_offs, rows4, log10p, params):
+        """msgs_pair_batch: AlignmentPE::getPairs + sort + filter on the device over alignment
+        lists (read pair i: rows aln_offs[i]..aln_offs[i+1] and aln_offs[n+i]..aln_offs[n+i+1] of
+        rows4 = tid, strand, alnStart, alnEnd); returns (counts, rows3, lik2)"""
+        offs = np.ascontiguousarray(aln_offs, np.int64)
+        n = (len(offs) - 1) // 2
+        r4 = np.ascontiguousarray(np.asarray(rows4, np.int64).reshape(-1, 4) if len(rows4) else
+                                  np.zeros((1, 4), np.int64))
+        lp = np.ascontiguousarray(log10p if len(log10p) else np.zeros(1), np.float64)
+        counts = np.zeros(max(n, 1), np.int64)
+        need = C.c_int64(0)
+        cap = 16
+        for _ in range(_RETRIES):
+            rows3 = np.zeros((cap, 3), np.int64)
+            lik2 = np.zeros((cap, 2))
+            rc = lib().msgs_pair_batch(self.h, n, _p(offs, C.c_int64), _p(r4, C.c_int64), _p(lp, C.c_double),
+                                       C.byref(params), _p(counts, C.c_int64), _p(rows3, C.c_int64),
+                                       _p(lik2, C.c_double), cap, C.byref(need))
+            if rc == MSGS_E_CAP:
+                cap = need.value + 1
+                continue
+            self._chk(rc, "msgs_pair_batch")
+            break
+        else:
+            raise RuntimeError("output capacity not reached after %d tries" % _RETRIES)
+        k = int(counts[:n].sum())
+        return counts[:n], rows3[:k], lik2[:k]
 
     def dp(self, reads, quals, chains_per_read, params):
         """msgs_dp_batch: chains_per_read[i] = list of chains, a chain = seed rows (from, to, start,

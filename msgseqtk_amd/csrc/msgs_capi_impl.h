@@ -94,6 +94,22 @@ template <class T> using PVec = std::vector<T, PinnedAlloc<T>>;
 
 struct BatchResult;
 
+/* candidate mate pairs of a paired batch after AlignmentPE's sort and filter: read pair i's
+ * are pairs[off[i] .. off[i] + kept[i]) in the reference's order */
+struct PairStage {
+    std::vector<int64_t> off;
+    PVec<int32_t> kept;
+    PVec<PairRec> pairs;
+};
+
+/* getPairs + sort + filter on the device for n read pairs whose alignment records are
+ * dRecs[jobOff[i] .. jobOff[i+1]) (read i) and dRecs[jobOff[n+i] .. jobOff[n+i+1]) (its
+ * mate).  scoreFilter: first drop records with alnScore < L * minScoreRate (Alignment::filter;
+ * read lengths from dReadOffs[2n+1], scores are integers x scoreScale); otherwise the lists
+ * are taken as given. */
+int runPairs(msgs_ctx* c, int64_t n, const std::vector<int64_t>& jobOff, const AlnRec* dRecs,
+             const int64_t* dReadOffs, bool scoreFilter, double scoreScale, const msgs_params* p, PairStage& out);
+
 /* a resident batch (reads kept in HBM between runs); paired batches hold the reads
  * followed by their mates on the device.  A submitted run finishes its records on a
  * host worker thread while the caller goes on with other batches. */
@@ -134,6 +150,9 @@ struct Exec {
     double lastSmemMicros = 0;    /* the SMEM search kernel(s) of the last SMEM stage */
     double lastStageMicros = 0;   /* the whole SMEM stage (prefix sums, search, merge) */
     double lastFilterMicros = 0;
+    /* the insert model's table in bufs "pr.tab": its parameters and range when filled */
+    double pairTabKey[6] = {};
+    bool pairTabValid = false;
     std::string err;
     bool create(unsigned streamFlags);
     void destroy();

@@ -9,7 +9,9 @@
  * another first), used by the reference's FMDIndex constructor
  * (src/FMDIndex.cpp:29-68).  Then BWT (src/FMDIndex.cpp:385-393) and occ blocks.
  */
-#include <hipcub/hipcub.hpp>
+#include <cstring>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <cstdio>
@@ -61,13 +63,21 @@ __global__ void k_round_keys(const uint32_t* val, const uint32_t* rank, int64_t 
     key[k] = ((uint64_t)rank[i] << 32) | r2;
 }
 
-__global__ void k_count_distinct(const uint32_t* gstart, int64_t n, unsigned long long* cnt) {
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+/* number of groups (k with gstart[k] == k): a grid-stride count per thread, a block sum in
+ * LDS and one atomic per block (one atomic per wavefront serialises ~10^7 atomics on one
+ * address for a 10^9-symbol text) */
+__global__ void __launch_bounds__(256) k_count_distinct(const uint32_t* gstart, int64_t n, unsigned long long* cnt) {
+    __shared__ unsigned long long part[4];
     unsigned long long c = 0;
-    if (k < n) c = (k == 0 || gstart[k] == (uint32_t)k) ? 1 : 0;
-    /* wave-level reduction then one atomic per wave */
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+        c += gstart[k] == (uint32_t)k ? 1 : 0;
     for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
-    if ((threadIdx.x & 63) == 0 && c) atomicAdd(cnt, c);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long t = part[0] + part[1] + part[2] + part[3];
+        if (t) atomicAdd(cnt, t);
+    }
 }
 
 __global__ void k_isa(const uint32_t* sa, int64_t n, uint32_t* isa) {
@@ -162,8 +172,8 @@ int buildIndexDevice(DbState& db, hipStream_t st, std::string& err) {
     unsigned long long* dcnt = tmp.alloc<unsigned long long>(1, e); HCHK(e);
 
     size_t sortBytes = 0, scanBytes = 0;
-    HCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, sortBytes, k0, k1, v0, v1, (int)n, 0, 64, st));
-    HCHK(hipcub::DeviceScan::InclusiveScan(nullptr, scanBytes, head, head, hipcub::Max(), (int)n, st));
+    HCHK(rocprim::radix_sort_pairs(nullptr, sortBytes, k0, k1, v0, v1, (size_t)n, 0u, 64u, st));
+    HCHK(rocprim::inclusive_scan(nullptr, scanBytes, head, head, (size_t)n, rocprim::maximum<uint32_t>(), st));
     void* tmpStore = tmp.alloc<uint8_t>(std::max(sortBytes, scanBytes), e); HCHK(e);
     const size_t tmpBytes = std::max(sortBytes, scanBytes);
 
@@ -176,15 +186,15 @@ int buildIndexDevice(DbState& db, hipStream_t st, std::string& err) {
     int endBit = 63;
     for (int round = 0;; ++round) {
         size_t tb = tmpBytes;
-        HCHK(hipcub::DeviceRadixSort::SortPairs(tmpStore, tb, k0, k1, v0, v1, (int)n, 0, endBit, st));
+        HCHK(rocprim::radix_sort_pairs(tmpStore, tb, k0, k1, v0, v1, (size_t)n, 0u, (unsigned)endBit, st));
         std::swap(k0, k1);
         std::swap(v0, v1);
         k_heads<<<grid(n), 256, 0, st>>>(k0, n, head);
         tb = tmpBytes;
-        HCHK(hipcub::DeviceScan::InclusiveScan(tmpStore, tb, head, head, hipcub::Max(), (int)n, st));
+        HCHK(rocprim::inclusive_scan(tmpStore, tb, head, head, (size_t)n, rocprim::maximum<uint32_t>(), st));
         k_scatter_rank<<<grid(n), 256, 0, st>>>(v0, head, n, rank);
         HCHK(hipMemsetAsync(dcnt, 0, sizeof(unsigned long long), st));
-        k_count_distinct<<<grid(n), 256, 0, st>>>(head, n, dcnt);
+        k_count_distinct<<<std::min<unsigned>(grid(n), 8192u), 256, 0, st>>>(head, n, dcnt);
         unsigned long long distinct = 0;
         HCHK(hipMemcpyAsync(&distinct, dcnt, sizeof(distinct), hipMemcpyDeviceToHost, st));
         HCHK(hipStreamSynchronize(st));

@@ -133,6 +133,14 @@ struct AlnRec {
     int64_t cells;
 };
 
+/* one candidate mate pair (AlignmentPE, src/Alignment.h:446-575): the read's and the mate's
+ * alignments (job or row indexes), the insert size, and the pair's log10 and natural-log
+ * likelihoods (AlignmentPE::log10lik / loglik) */
+struct PairRec {
+    int32_t f, r, isize, pad;
+    double l10, ll;
+};
+
 /* DP job classes: no rectangle, wavefront fill (one wave per job), serial fill */
 enum : int32_t { CLS_NONE = 0, CLS_WAVE = 1, CLS_SERIAL = 2, CLS_WAVE16 = 3 };
 

@@ -1,14 +1,15 @@
 /*
  * msgs_map.hip -- the whole mapping path over one batch: SMEMs -> seeds -> chains ->
- * DP/backtrace on the device, then the host finishing step of msgseqtk-align
- * (src/msgseqtk-align.cpp:538-758): Alignment::filter, calcMapQ, sort, PE pairing
- * and record output.
+ * DP/backtrace on the device (and, for pairs, the candidate mate pairs: msgs_pair.hip),
+ * then the host finishing step of msgseqtk-align (src/msgseqtk-align.cpp:538-758):
+ * Alignment::filter, calcMapQ, sort and record output.
  *
  * Host finishing keeps the libm-dependent decisions (exp, log10, round) on the host,
  * where the reference computes them, and reorders with the restated libstdc++ sort.
  */
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <cstring>
+#include <rocprim/device/device_scan.hpp>
 #include <x86intrin.h>
 
 #include <algorithm>
@@ -121,6 +122,11 @@ inline unsigned gridFor(int64_t n, int t = 256) {
     return (unsigned)std::max<int64_t>(1, std::min<int64_t>((n + t - 1) / t, 1 << 16));
 }
 
+/* device exclusive prefix sum of n int64 values (rocPRIM; a null temp asks for its size) */
+inline hipError_t exclusiveSum(void* tmp, size_t& bytes, const int64_t* in, int64_t* out, int64_t n, hipStream_t st) {
+    return rocprim::exclusive_scan(tmp, bytes, in, out, (int64_t)0, (size_t)n, rocprim::plus<int64_t>(), st);
+}
+
 template <class T> int64_t exclusiveScan(const std::vector<T>& v, std::vector<int64_t>& off) {
     off.resize(v.size() + 1);
     int64_t s = 0;
@@ -166,6 +172,7 @@ struct DpStage {
     PVec<char> md;
     PVec<int64_t> tot;   /* device totals read back: tiles, paths, wave jobs, CIGAR ops, MD bytes */
     double scoreScale = 1;   /* the DP's integer scores are the reference's alnScore x scoreScale */
+    const AlnRec* dRecs = nullptr;   /* the records on the device (the executor's DP pool) */
     double score(int32_t v) const {
         return v == NEG ? -std::numeric_limits<double>::infinity() : (double)v / scoreScale;
     }
@@ -334,7 +341,7 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
      * job list, all on the device; the host reads back three totals */
     const int nb = (int)((nJobs + 255) / 256);
     size_t scanBytes = 0;
-    if (hipcub::DeviceScan::ExclusiveSum(nullptr, scanBytes, a.tileNeed, a.tileNeed, (int)(nJobs + 1), st) != hipSuccess)
+    if (exclusiveSum(nullptr, scanBytes, a.tileNeed, a.tileNeed, nJobs + 1, st) != hipSuccess)
         return fail(c, MSGS_E_HIP, "scan size");
     void* scanTmp = c->x().bufs.get<uint8_t>("dp.scantmp", scanBytes + 1, e);
     int64_t* dTileOff = c->x().bufs.get<int64_t>("dp.toff", nJobs + 1, e);
@@ -348,10 +355,10 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     unsigned int* dNext = c->x().bufs.get<unsigned int>("dp.next16", 1, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "dp scans");
     CK(hipMemsetAsync(dHist, 0, 256 * 8, st), "hist");
-    CK(hipcub::DeviceScan::ExclusiveSum(scanTmp, scanBytes, a.tileNeed, dTileOff, (int)(nJobs + 1), st), "tile scan");
-    CK(hipcub::DeviceScan::ExclusiveSum(scanTmp, scanBytes, a.pathNeed, dPathOff, (int)(nJobs + 1), st), "path scan");
-    CK(hipcub::DeviceScan::ExclusiveSum(scanTmp, scanBytes, a.borderNeed, dBorderOff, (int)(nJobs + 1), st), "border scan");
-    CK(hipcub::DeviceScan::ExclusiveSum(scanTmp, scanBytes, a.serialNeed, dSerialOff, (int)(nJobs + 1), st), "serial scan");
+    CK(exclusiveSum(scanTmp, scanBytes, a.tileNeed, dTileOff, nJobs + 1, st), "tile scan");
+    CK(exclusiveSum(scanTmp, scanBytes, a.pathNeed, dPathOff, nJobs + 1, st), "path scan");
+    CK(exclusiveSum(scanTmp, scanBytes, a.borderNeed, dBorderOff, nJobs + 1, st), "border scan");
+    CK(exclusiveSum(scanTmp, scanBytes, a.serialNeed, dSerialOff, nJobs + 1, st), "serial scan");
     k_wave_hist<<<nb, 256, 0, st>>>(a.jobCls, a.tileNeed, nJobs, dHist, CLS_WAVE);
     k_wave_base<<<1, 64, 0, st>>>(dHist, dHist + 64, dTot + 2);
     k_wave_scatter<<<nb, 256, 0, st>>>(a.jobCls, a.tileNeed, nJobs, dHist + 64, dWave, CLS_WAVE);
@@ -422,13 +429,14 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     CK(hipEventRecord(c->x().ev3, st), "event");
     /* compaction of CIGAR ops / MD bytes, offsets scanned on the device */
     k_rec_counts<<<gridFor(nJobs + 1), 256, 0, st>>>(a.out, nJobs, dNc, dNm);
-    CK(hipcub::DeviceScan::ExclusiveSum(scanTmp, scanBytes, dNc, dCOff, (int)(nJobs + 1), st), "cigar scan");
-    CK(hipcub::DeviceScan::ExclusiveSum(scanTmp, scanBytes, dNm, dMOff, (int)(nJobs + 1), st), "md scan");
+    CK(exclusiveSum(scanTmp, scanBytes, dNc, dCOff, nJobs + 1, st), "cigar scan");
+    CK(exclusiveSum(scanTmp, scanBytes, dNm, dMOff, nJobs + 1, st), "md scan");
     k_compact<<<gridFor(nJobs), 256, 0, st>>>(a.out, nJobs, dPathOff, a.cigar, a.md, dCOff, dMOff, dC, dM);
     out.recs.resize(nJobs);
     out.cigOff.resize(nJobs + 1);
     out.mdOff.resize(nJobs + 1);
     CK(hipMemcpyAsync(out.recs.data(), a.out, nJobs * sizeof(AlnRec), hipMemcpyDeviceToHost, st), "recs");
+    out.dRecs = a.out;
     CK(hipMemcpyAsync(out.cigOff.data(), dCOff, (nJobs + 1) * 8, hipMemcpyDeviceToHost, st), "coff");
     CK(hipMemcpyAsync(out.mdOff.data(), dMOff, (nJobs + 1) * 8, hipMemcpyDeviceToHost, st), "moff");
     CK(hipStreamSynchronize(st), "dp kernel");
@@ -516,7 +524,8 @@ struct Finisher {
     bool bam = false;    /* records in the BAM layout instead of SAM text */
     int64_t mapped = 0;
     const double log10e = ::log10(M_E);
-    /* time-stamp-counter ticks spent in: [0] SAM text, [1] pairing, [2] MAPQ, [3] alignment lists */
+    /* time-stamp-counter ticks spent in: [0] SAM text, [1] pairing (0: the candidate pairs come from the
+     * device), [2] MAPQ, [3] alignment lists */
     mutable uint64_t tk[4] = {0, 0, 0, 0};
 
     std::string cigarOf(int64_t job) const {
@@ -595,8 +604,6 @@ struct Finisher {
              dp.md.data() + dp.mdOff[job], r.mdLen, (float)a.log10P, (float)postP);
     }
 
-    struct Pr { const HAln* f; const HAln* r; int32_t isize; double l10; double ll; double postP; uint8_t mq; };
-    std::vector<Pr> prBuf;
     /* per-thread scratch, reused across reads */
     std::vector<HAln> fBuf, rBuf;
     std::vector<double> ll, pr, pp;
@@ -646,69 +653,31 @@ struct Finisher {
         outputAlns(s, rd, v, 0);
     }
 
-    /* PairingScheme::pr with boost::math::pdf(normal) evaluated as boost does it */
-    double pairPr(double x) const {
-        if (P.insMean == 0) return 1;
-        if (P.insMin <= x && x <= P.insMax) {
-            double ex = x - P.insMean;
-            ex *= -ex;
-            ex /= 2 * P.insSD * P.insSD;
-            double r = std::exp(ex);
-            r /= P.insSD * 2.506628274631000502415765284811045253e+00;
-            return r;
-        }
-        return 0;
-    }
-
-    /* main_PE task (src/msgseqtk-align.cpp:679-754) */
+    /* main_PE task (src/msgseqtk-align.cpp:679-754); the candidate pairs q[0, nq) come from the
+     * device (getPairs, sort and filter, msgs_pair.hip) */
     void pe(msgs_sam::Out& s, const HostRead& r1, const HostRead& r2, int32_t ns1, int32_t ns2, int64_t a0, int64_t a1,
-            int64_t b0, int64_t b1) {
+            int64_t b0, int64_t b1, const PairRec* q, int32_t nq) {
         if (ns1 == 0 && ns2 == 0) { unmapped(s, r1, 4 | 8 | 1 | 64); unmapped(s, r2, 4 | 8 | 1 | 128); return; }
         std::vector<HAln>& f = fBuf;
         std::vector<HAln>& r = rBuf;
         alnList(r1, a0, a1, f);
         alnList(r2, b0, b1, r);
         if (f.empty() && r.empty()) { unmapped(s, r1, 4 | 8 | 1 | 64); unmapped(s, r2, 4 | 8 | 1 | 128); return; }
-        const uint64_t tp0 = __rdtsc();
-        std::vector<Pr>& pairs = prBuf;
-        pairs.clear();
-        for (const HAln& x : f)
-            for (const HAln& y : r)
-                if ((x.r->strand & y.r->strand) == 0) {
-                    int32_t isize = x.r->tid != y.r->tid ? -1 :
-                        (int32_t)(std::max(x.r->alnEnd, y.r->alnEnd) - std::min(x.r->alnStart, y.r->alnStart));
-                    double pr = pairPr(isize);
-                    pairs.push_back(Pr{&x, &y, isize, x.log10P + y.log10P + std::log10(pr),
-                                       x.log10P / log10e + y.log10P / log10e + std::log(pr), 0, 0xff});
-                }
-        msgs_sort::sort(pairs.data(), pairs.data() + pairs.size(), [](const Pr& l, const Pr& r) { return l.l10 > r.l10; });
-        auto rm = [&](auto g) { pairs.erase(std::remove_if(pairs.begin(), pairs.end(), g), pairs.end()); };
-        rm([&](const Pr& q) { return !(P.insMin <= q.isize && q.isize <= P.insMax); });
-        if (P.noDiscordant) rm([](const Pr& q) { return !(q.f->r->tid == q.r->r->tid && (q.f->r->strand & q.r->r->strand) == 0); });
-        if (P.noTailOver) rm([](const Pr& q) {
-            const AlnRec &a = *q.f->r, &b = *q.r->r;
-            return (a.alnStart <= b.alnStart && a.alnEnd > b.alnStart) || (b.alnStart <= a.alnStart && b.alnEnd > a.alnStart); });
-        if (P.noContain) rm([](const Pr& q) {
-            const AlnRec &a = *q.f->r, &b = *q.r->r;
-            return (a.alnStart <= b.alnStart && a.alnEnd >= b.alnEnd) || (b.alnStart <= a.alnStart && b.alnEnd >= a.alnEnd); });
-        if (P.noOverlap) rm([](const Pr& q) { return q.f->r->alnStart < q.r->r->alnEnd && q.f->r->alnEnd > q.r->r->alnStart; });
-        if (P.maxNPair > 0 && (int64_t)pairs.size() > P.maxNPair) pairs.erase(pairs.begin() + P.maxNPair, pairs.end());
         const uint64_t tp1 = __rdtsc();
-        tk[1] += tp1 - tp0;
-        if (!pairs.empty()) {
+        if (nq > 0) {
             mapped += 2;
-            ll.resize(pairs.size());
-            for (size_t i = 0; i < pairs.size(); ++i) ll[i] = pairs[i].ll;
+            ll.resize(nq);
+            for (int32_t i = 0; i < nq; ++i) ll[i] = q[i].ll;
             mapq(ll, pr, pp, mq, true);
             tk[2] += __rdtsc() - tp1;
-            size_t num = P.maxReport == 0 ? pairs.size() : std::min((size_t)(uint32_t)P.maxReport, pairs.size());
+            size_t num = P.maxReport == 0 ? (size_t)nq : std::min((size_t)(uint32_t)P.maxReport, (size_t)nq);
             for (size_t i = 0; i < num; ++i) {
-                const Pr& q = pairs[i];
-                const AlnRec &a = *q.f->r, &b = *q.r->r;
-                int32_t fi = a.alnStart < b.alnStart ? q.isize : -q.isize;
+                const AlnRec &a = dp.recs[q[i].f], &b = dp.recs[q[i].r];
+                const HAln hf{&a, a.log10P}, hr{&b, b.log10P};
+                int32_t fi = a.alnStart < b.alnStart ? q[i].isize : -q[i].isize;
                 int sec = i > 0 ? 256 : 0;
-                alnRecord(s, r1, *q.f, 1 | 2 | 64 | sec, mq[i], fi, (int64_t)num, (int64_t)pairs.size(), pp[i]);
-                alnRecord(s, r2, *q.r, 1 | 2 | 128 | sec, mq[i], -fi, (int64_t)num, (int64_t)pairs.size(), pp[i]);
+                alnRecord(s, r1, hf, 1 | 2 | 64 | sec, mq[i], fi, (int64_t)num, (int64_t)nq, pp[i]);
+                alnRecord(s, r2, hr, 1 | 2 | 128 | sec, mq[i], -fi, (int64_t)num, (int64_t)nq, pp[i]);
             }
         } else if (!P.noMixed) {
             mapped += (f.empty() ? 0 : 1) + (r.empty() ? 0 : 1);
@@ -741,6 +710,7 @@ struct BatchResult {
     SeedStage sd;
     ChainStage ch;
     DpStage dp;
+    PairStage pr;   /* paired batches: the candidate mate pairs */
 };
 
 static int uploadQual(msgs_ctx* c, const msgs_reads* r, const DevReads& d, const char* name, uint8_t** out) {
@@ -1130,7 +1100,7 @@ static int64_t samText(msgs_ctx* c, const msgs_params* p, const msgs_reads* read
             } else {
                 HostRead h2 = host(mates, nm2, i);
                 F.pe(o, h1, h2, b.sd.seedCnt[i], b.sd.seedCnt[n + i], b.ch.chainOff[i], b.ch.chainOff[i + 1],
-                     b.ch.chainOff[n + i], b.ch.chainOff[n + i + 1]);
+                     b.ch.chainOff[n + i], b.ch.chainOff[n + i + 1], b.pr.pairs.data() + b.pr.off[i], b.pr.kept[i]);
             }
         }
         o.finish();
@@ -1207,6 +1177,10 @@ static int impl_map_batch(msgs_ctx* c, const msgs_reads* reads, const msgs_reads
         if (rc) return rc;
         rc = runAll(c, r, dQual, p, b, nullptr, nullptr);
         if (rc) return rc;
+        if (mates) {
+            rc = runPairs(c, reads->n, b.ch.chainOff, b.dp.dRecs, r.offs, true, b.dp.scoreScale, p, b.pr);
+            if (rc) return rc;
+        }
     }
     std::vector<std::string> parts;
     std::vector<int64_t> lo;
@@ -1348,6 +1322,11 @@ static int batchGpu(msgs_ctx* c, Slot& S, std::chrono::steady_clock::time_point 
         double k = 0;
         rc = runDp(c, S.dev, S.dQual, p, b->sd, b->ch, b->dp, &k, dpPhase, dStats + 10);
         if (rc) return rc;
+        if (S.paired) {
+            rc = runPairs(c, S.host[0].view.n, b->ch.chainOff, b->dp.dRecs, S.dev.offs, true, b->dp.scoreScale, p,
+                          b->pr);
+            if (rc) return rc;
+        }
         auto s4 = std::chrono::steady_clock::now();
         dpUs += k;
         wall[0] = us(s0, s1); wall[1] = us(s1, s2); wall[2] = us(s2, s3); wall[3] = us(s3, s4);

@@ -628,5 +628,71 @@ ALIGN_LIST& Alignment::sort(ALIGN_LIST& alnList) {
     return alnList;
 }
 
+/* ---- PairingScheme / AlignmentPE (src/PairingScheme.cpp:14-25, src/Alignment.cpp:514-581,
+ * src/Alignment.h:635-639) ---- */
+const double PairingScheme::DEFAULT_MEAN_INSERT = 200;
+const double PairingScheme::DEFAULT_CV_INSERT = 0.1;
+const double PairingScheme::DEFAULT_OUTLIER_DIST = 10;
+PairingScheme AlignmentPE::ps;
+
+double PairingScheme::pr(double x) const {
+    msgs_params p;
+    msgs_params_default(&p);
+    p.insMean = m;
+    p.insSD = s;
+    p.insMin = min;
+    p.insMax = max;
+    return msgs_util_insert_pr(&p, x);
+}
+
+int32_t AlignmentPE::getInsertSize(const Alignment& lhs, const Alignment& rhs) {
+    if (lhs.tid != rhs.tid) return -1;
+    return (int32_t)(std::max(lhs.alnEnd, rhs.alnEnd) - std::min(lhs.alnStart, rhs.alnStart));
+}
+
+PAIR_LIST AlignmentPE::getPairs(const ALIGN_LIST& fwdAlnList, const ALIGN_LIST& revAlnList) {
+    PAIR_LIST out;
+    out.reserve(fwdAlnList.size() * revAlnList.size());
+    for (const Alignment& f : fwdAlnList)
+        for (const Alignment& r : revAlnList)
+            if ((f.qStrand & r.qStrand) == 0) out.push_back(AlignmentPE(&f, &r));
+    return out;
+}
+
+PAIR_LIST& AlignmentPE::filter(PAIR_LIST& v, bool noDiscordant, bool noTailOver, bool noContain, bool noOverlap,
+                               int64_t maxNPair) {
+    auto rm = [&v](auto pred) { v.erase(std::remove_if(v.begin(), v.end(), pred), v.end()); };
+    rm([](const AlignmentPE& q) { return !(ps.getMin() <= q.getInsertSize() && q.getInsertSize() <= ps.getMax()); });
+    if (noDiscordant) rm([](const AlignmentPE& q) { return !q.isConcordant(); });
+    if (noTailOver) rm([](const AlignmentPE& q) { return q.isTailOver(); });
+    if (noContain) rm([](const AlignmentPE& q) { return q.isContained(); });
+    if (noOverlap) rm([](const AlignmentPE& q) { return q.isOverlap(); });
+    if (maxNPair > 0 && (int64_t)v.size() > maxNPair) v.erase(v.begin() + maxNPair, v.end());
+    return v;
+}
+
+PAIR_LIST& AlignmentPE::calcMapQ(PAIR_LIST& v) {
+    if (v.empty()) return v;
+    vector<double> ll(v.size()), pp(v.size());
+    vector<uint8_t> mq(v.size());
+    for (size_t i = 0; i < v.size(); ++i) ll[i] = v[i].loglik();
+    msgs_util_mapq(ll.data(), (int64_t)ll.size(), 1, pp.data(), mq.data());
+    for (size_t i = 0; i < v.size(); ++i) { v[i].postP = pp[i]; v[i].mapQ = mq[i]; }
+    return v;
+}
+
+PAIR_LIST& AlignmentPE::sort(PAIR_LIST& v) {
+    /* std::sort by log10lik decreasingly, as Alignment::sort */
+    vector<double> key(v.size());
+    vector<int32_t> perm(v.size());
+    for (size_t i = 0; i < v.size(); ++i) { key[i] = -v[i].log10lik(); perm[i] = (int32_t)i; }
+    msgs_util_sort_by_key(key.data(), perm.data(), (int64_t)perm.size());
+    PAIR_LIST sorted;
+    sorted.reserve(v.size());
+    for (int32_t i : perm) sorted.push_back(v[i]);
+    v.swap(sorted);
+    return v;
+}
+
 }  // namespace MSGseqTK
 }  // namespace EGriceLab

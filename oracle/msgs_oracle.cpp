@@ -860,6 +860,43 @@ static double pairLoglik(const Params& P, const Pair& p) {
     return alnLoglik(*p.f) + alnLoglik(*p.r) + std::log(pairPr(P, p.isize));
 }
 
+/* AlignmentPE::getPairs (src/Alignment.cpp:573-581), AlignmentPE::sort (src/Alignment.h:635-639)
+ * and AlignmentPE::filter (src/Alignment.cpp:556-571) over two filtered, evaluated lists */
+static std::vector<Pair> candidatePairs(const Params& P, const std::vector<Aln>& a1, const std::vector<Aln>& a2) {
+    std::vector<Pair> pairs;
+    for (const Aln& f : a1)
+        for (const Aln& r : a2)
+            if ((f.qStrand & r.qStrand) == 0) {
+                int32_t isize = f.tid != r.tid ? -1 :
+                    (int32_t)(std::max(f.alnEnd, r.alnEnd) - std::min(f.alnStart, r.alnStart));
+                pairs.push_back(Pair{&f, &r, isize, 0, 0xff});
+            }
+    std::sort(pairs.begin(), pairs.end(),
+              [&](const Pair& l, const Pair& r) { return pairLog10lik(P, l) > pairLog10lik(P, r); });
+    auto rm = [&](std::function<bool(const Pair&)> f) {
+        pairs.erase(std::remove_if(pairs.begin(), pairs.end(), f), pairs.end());
+    };
+    rm([&](const Pair& p) { return !(P.insMin <= p.isize && p.isize <= P.insMax); });
+    if (P.noDiscordant) rm([](const Pair& p) { return !(p.f->tid == p.r->tid && (p.f->qStrand & p.r->qStrand) == 0); });
+    if (P.noTailOver) rm([](const Pair& p) {
+        return (p.f->alnStart <= p.r->alnStart && p.f->alnEnd > p.r->alnStart) ||
+               (p.r->alnStart <= p.f->alnStart && p.r->alnEnd > p.f->alnStart); });
+    if (P.noContain) rm([](const Pair& p) {
+        return (p.f->alnStart <= p.r->alnStart && p.f->alnEnd >= p.r->alnEnd) ||
+               (p.r->alnStart <= p.f->alnStart && p.r->alnEnd >= p.f->alnEnd); });
+    if (P.noOverlap) rm([](const Pair& p) { return p.f->alnStart < p.r->alnEnd && p.f->alnEnd > p.r->alnStart; });
+    if (P.maxNPair > 0 && (int64_t)pairs.size() > P.maxNPair) pairs.erase(pairs.begin() + P.maxNPair, pairs.end());
+    return pairs;
+}
+
+void pairAlignments(const Params& P, const std::vector<Aln>& a1, const std::vector<Aln>& a2,
+                    std::vector<PairOut>& out) {
+    out.clear();
+    for (const Pair& p : candidatePairs(P, a1, a2))
+        out.push_back(PairOut{(int64_t)(p.f - a1.data()), (int64_t)(p.r - a2.data()), p.isize, pairLog10lik(P, p),
+                              pairLoglik(P, p)});
+}
+
 /* ------------------------------------------------------------------ A31 */
 struct Rec {
     std::string qname; int flag; int32_t tid; int64_t pos; int mapq; std::string cigar;
@@ -1024,32 +1061,8 @@ std::string mapPE(const Db& db, const Params& P, const Read& r1, const Read& r2,
         return outputUnmapped(db, r1, 4 | 8 | 1 | 64) + outputUnmapped(db, r2, 4 | 8 | 1 | 128);
     for (Aln& a : a1) evaluate(P, a);
     for (Aln& a : a2) evaluate(P, a);
-    /* getPairs (src/Alignment.cpp:573-581) */
-    std::vector<Pair> pairs;
-    for (const Aln& f : a1)
-        for (const Aln& r : a2)
-            if ((f.qStrand & r.qStrand) == 0) {
-                int32_t isize = f.tid != r.tid ? -1 :
-                    (int32_t)(std::max(f.alnEnd, r.alnEnd) - std::min(f.alnStart, r.alnStart));
-                pairs.push_back(Pair{&f, &r, isize, 0, 0xff});
-            }
-    /* AlignmentPE::sort (src/Alignment.h:635-639) */
-    std::sort(pairs.begin(), pairs.end(),
-              [&](const Pair& l, const Pair& r) { return pairLog10lik(P, l) > pairLog10lik(P, r); });
-    /* AlignmentPE::filter (src/Alignment.cpp:557-571) */
-    auto rm = [&](std::function<bool(const Pair&)> f) {
-        pairs.erase(std::remove_if(pairs.begin(), pairs.end(), f), pairs.end());
-    };
-    rm([&](const Pair& p) { return !(P.insMin <= p.isize && p.isize <= P.insMax); });
-    if (P.noDiscordant) rm([](const Pair& p) { return !(p.f->tid == p.r->tid && (p.f->qStrand & p.r->qStrand) == 0); });
-    if (P.noTailOver) rm([](const Pair& p) {
-        return (p.f->alnStart <= p.r->alnStart && p.f->alnEnd > p.r->alnStart) ||
-               (p.r->alnStart <= p.f->alnStart && p.r->alnEnd > p.f->alnStart); });
-    if (P.noContain) rm([](const Pair& p) {
-        return (p.f->alnStart <= p.r->alnStart && p.f->alnEnd >= p.r->alnEnd) ||
-               (p.r->alnStart <= p.f->alnStart && p.r->alnEnd >= p.f->alnEnd); });
-    if (P.noOverlap) rm([](const Pair& p) { return p.f->alnStart < p.r->alnEnd && p.f->alnEnd > p.r->alnStart; });
-    if (P.maxNPair > 0 && (int64_t)pairs.size() > P.maxNPair) pairs.erase(pairs.begin() + P.maxNPair, pairs.end());
+    /* getPairs, AlignmentPE::sort, AlignmentPE::filter */
+    std::vector<Pair> pairs = candidatePairs(P, a1, a2);
     std::string out;
     if (!pairs.empty()) {
         calcMapQPE(P, pairs);

@@ -135,6 +135,11 @@ void evaluate(const Params& P, Aln& a);
 void calcMapQ(std::vector<Aln>& alns);
 
 /* ---- A31/A32 ---- */
+/* A30: the candidate pairs of two filtered, evaluated alignment lists after AlignmentPE's
+ * getPairs, sort and filter: indexes into a1 / a2, insert size, log10lik, loglik */
+struct PairOut { int64_t f, r; int32_t isize; double l10, ll; };
+void pairAlignments(const Params& P, const std::vector<Aln>& a1, const std::vector<Aln>& a2,
+                    std::vector<PairOut>& out);
 std::string mapSE(const Db& db, const Params& P, const Read& r);
 std::string mapPE(const Db& db, const Params& P, const Read& r1, const Read& r2);
 /* the same tasks with the seeding (SMEM_LIST::findSeeds) and chaining (SeedChain::getChains)

@@ -368,6 +368,46 @@ int64_t orc_count_ext(void* h, const uint8_t* seq, const int64_t* offs, int64_t 
     return g_nExt;
 }
 
+/* AlignmentPE::getPairs + sort + filter over caller alignment lists, laid out as msgs_pair_batch
+ * takes them (read pair i: rows alnOffs[i] .. alnOffs[i+1] and alnOffs[n+i] .. alnOffs[n+i+1] of
+ * rows4 = tid, strand, alnStart, alnEnd, with log10P); returns the number of pairs written (rows3:
+ * absolute rows of both alignments and the insert size, lik2: log10lik and loglik), or -1 when
+ * more than cap */
+int64_t orc_pairs(const orc_params* p, int64_t n, const int64_t* alnOffs, const int64_t* rows4, const double* log10P,
+                  int64_t* pairCounts, int64_t* rows3, double* lik2, int64_t cap) {
+    const Params P = toParams(p);
+    int64_t k = 0;
+    std::vector<PairOut> out;
+    for (int64_t i = 0; i < n; ++i) {
+        std::vector<Aln> a1, a2;
+        auto take = [&](int64_t j0, int64_t j1, std::vector<Aln>& v) {
+            for (int64_t j = j0; j < j1; ++j) {
+                Aln a;
+                a.tid = (int32_t)rows4[4 * j];
+                a.qStrand = (int32_t)rows4[4 * j + 1];
+                a.alnStart = rows4[4 * j + 2];
+                a.alnEnd = rows4[4 * j + 3];
+                a.log10P = log10P[j];
+                v.push_back(a);
+            }
+        };
+        take(alnOffs[i], alnOffs[i + 1], a1);
+        take(alnOffs[n + i], alnOffs[n + i + 1], a2);
+        pairAlignments(P, a1, a2, out);
+        pairCounts[i] = (int64_t)out.size();
+        for (const PairOut& q : out) {
+            if (k >= cap) return -1;
+            rows3[3 * k] = alnOffs[i] + q.f;
+            rows3[3 * k + 1] = alnOffs[n + i] + q.r;
+            rows3[3 * k + 2] = q.isize;
+            lik2[2 * k] = q.l10;
+            lik2[2 * k + 1] = q.ll;
+            ++k;
+        }
+    }
+    return k;
+}
+
 /* Alignment::calcMapQ over alignments with the given log10P values */
 void orc_calc_mapq(const double* log10P, int64_t n, double* postP, uint8_t* mapQ) {
     std::vector<Aln> v(n);

@@ -11,6 +11,10 @@
  *     the middle, mismatch, insertion, deletion; LOCAL for the first three), aligned along the
  *     chains of their seeds by Alignment::buildAlignments; the CIGAR covers the whole read and
  *     mdTag2alnQLen(MD) equals the aligned query length.
+ *   the PE driver's pairing (src/msgseqtk-align.cpp:700-717): mates whose target segments occur
+ *     twice give two in-range candidate pairs with equal log10lik and two out of range; getPairs,
+ *     sort, filter and calcMapQ through AlignmentPE, and the same lists through the device pairing
+ *     entry msgs_pair_batch, which must give the same pairs, order, insert sizes and log10lik.
  * Plus known answers that pin the device results: FMD counts of the BD text, count / locateAll
  * against a naive scan, accessSA against a suffix sort, the alignments' scores and strings of the
  * Alignment_test cases, calcMapQ / sort on the reference's formulas.  Prints PASS and exits 0.
@@ -30,6 +34,7 @@
 #include "msgseqtk/MetaGenome.h"
 #include "msgseqtk/SMEM.h"
 #include "msgseqtk/SeedChain.h"
+#include "msgs_hip.h"
 
 using namespace std;
 using namespace EGriceLab::MSGseqTK;
@@ -225,11 +230,81 @@ static void alignmentTest() {
     CHECK(Alignment::mdTag2alnQLen("") == 0);
 }
 
+static void pairTest() {
+    /* a 2 kb chromosome (fixed LCG) whose [100, 160) reappears at [1000, 1060) and whose
+     * [300, 360) reappears at [1200, 1260); read 1 = [100, 160) forward, read 2 = the reverse
+     * complement of [300, 360) */
+    uint64_t x = 12345;
+    string chrom(2000, 'A');
+    for (char& c : chrom) { x = x * 6364136223846793005ULL + 1442695040888963407ULL; c = "ACGT"[(x >> 33) & 3]; }
+    chrom.replace(1000, 60, chrom.substr(100, 60));
+    chrom.replace(1200, 60, chrom.substr(300, 60));
+    MetaGenome mtg;
+    mtg.addChrom("pe", dna::encode(chrom));
+    FMDIndex fmdidx(mtg.getBDSeq());
+    const PrimarySeq r1(chrom.substr(100, 60), "p1");
+    const PrimarySeq r2 = PrimarySeq(chrom.substr(300, 60), "p2").revcom();
+    const PrimarySeq rc1 = r1.revcom(), rc2 = r2.revcom();
+    auto align = [&](const PrimarySeq& r, const PrimarySeq& rc) {
+        const SeedList seeds = SMEM_LIST::findSeeds(&r, &mtg, &fmdidx, 17, 0.01);
+        const ChainList chains = SeedChain::getChains(seeds, r.length() * 0.2, r.length() * 0.1);
+        ALIGN_LIST alns = Alignment::buildAlignments(&r, &rc, mtg, chains, Alignment::GLOBAL);
+        Alignment::filter(alns, 0.65);
+        return alns;
+    };
+    const ALIGN_LIST f = align(r1, rc1), g = align(r2, rc2);
+    CHECK(f.size() == 2 && g.size() == 2);
+    AlignmentPE::ps.setMean(250);
+    AlignmentPE::ps.setSD(30);
+    AlignmentPE::ps.updateParam();
+    AlignmentPE::ps.updateRange();
+    CHECK(AlignmentPE::ps.getMin() == -50 && AlignmentPE::ps.getMax() == 550);
+    PAIR_LIST pairs = AlignmentPE::getPairs(f, g);
+    CHECK(pairs.size() == 4);
+    AlignmentPE::sort(pairs);
+    for (size_t i = 1; i < pairs.size(); ++i) CHECK(pairs[i - 1].log10lik() >= pairs[i].log10lik());
+    PAIR_LIST kept = pairs;
+    AlignmentPE::filter(kept, false, false, false, false);
+    CHECK(kept.size() == 2);
+    for (const AlignmentPE& q : kept) {
+        CHECK(q.getInsertSize() == 260 && q.isConcordant() && !q.isOverlap());
+        CHECK(q.log10lik() == kept.front().log10lik());
+    }
+    AlignmentPE::calcMapQ(kept);
+    CHECK(kept[0].getPostP() == 0.5 && kept[1].getPostP() == 0.5 && kept[0].getMapQ() == 3);
+    /* the same lists through the device pairing */
+    vector<int64_t> offs = {0, (int64_t)f.size(), (int64_t)(f.size() + g.size())}, rows;
+    vector<double> lp;
+    for (const ALIGN_LIST* l : {&f, &g})
+        for (const Alignment& a : *l) {
+            rows.insert(rows.end(), {a.getTid(), (int64_t)a.getStrand(), a.getAlnStart(), a.getAlnEnd()});
+            lp.push_back(a.log10lik());
+        }
+    msgs_params p;
+    msgs_params_default(&p);
+    p.insMean = 250; p.insSD = 30; p.insMin = -50; p.insMax = 550;
+    msgs_ctx* ctx = nullptr;
+    CHECK(msgs_ctx_create(0, &ctx) == MSGS_OK);
+    int64_t cnt = 0, need = 0, r3[3 * 4];
+    double lik[2 * 4];
+    CHECK(msgs_pair_batch(ctx, 1, offs.data(), rows.data(), lp.data(), &p, &cnt, r3, lik, 4, &need) == MSGS_OK);
+    CHECK(cnt == (int64_t)kept.size());
+    for (int64_t k = 0; k < cnt && k < (int64_t)kept.size(); ++k) {
+        CHECK(&f[r3[3 * k]] == kept[k].fwdAln && &g[r3[3 * k + 1] - (int64_t)f.size()] == kept[k].revAln);
+        CHECK(r3[3 * k + 2] == kept[k].getInsertSize());
+        CHECK(lik[2 * k] == kept[k].log10lik() && lik[2 * k + 1] == kept[k].loglik());
+    }
+    msgs_ctx_destroy(ctx);
+    cout << "pairing: " << pairs.size() << " candidates, " << kept.size() << " in range, insert size "
+         << kept.front().getInsertSize() << ", mapQ " << (int)kept.front().getMapQ() << endl;
+}
+
 int main() {
     try {
         smemTest();
         seedChainTest();
         alignmentTest();
+        pairTest();
     } catch (const std::exception& e) {
         cerr << "exception: " << e.what() << endl;
         return 2;

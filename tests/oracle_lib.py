@@ -105,6 +105,8 @@ def orc():
                                         PD, PD, C.c_char_p, C.c_char_p, C.c_int64]
         lib.orc_align_chain.restype = C.c_int64
         lib.orc_calc_mapq.argtypes = [PD, C.c_int64, PD, PU8]
+        lib.orc_pairs.argtypes = [C.POINTER(Params), C.c_int64, P64, P64, PD, P64, P64, PD, C.c_int64]
+        lib.orc_pairs.restype = C.c_int64
         lib.orc_params_convert.argtypes = [C.POINTER(Params), C.c_void_p]
         lib.orc_count_ext.argtypes = [C.c_void_p, PU8, P64, C.c_int64, C.c_int64, C.c_double]
         lib.orc_count_ext.restype = C.c_int64
@@ -350,6 +352,25 @@ def orc_align_chain(db, params, read, seeds, qual=None):
                           ptr(rows, C.c_int64), len(rows), ptr(res, C.c_int64), ptr(sc, C.c_double),
                           ptr(lp, C.c_double), cig, md, 4096)
     return res, float(sc[0]), float(lp[0]), cig.value.decode(), md.value.decode()
+
+
+def orc_pairs(params, aln_offs, rows4, log10p):
+    """AlignmentPE::getPairs + sort + filter of the oracle over alignment lists laid out as
+    msgs_pair_batch takes them: (counts, rows3, lik2)"""
+    offs = np.ascontiguousarray(aln_offs, np.int64)
+    n = (len(offs) - 1) // 2
+    r4 = np.ascontiguousarray(np.asarray(rows4, np.int64).reshape(-1, 4) if len(rows4) else np.zeros((1, 4), np.int64))
+    lp = np.ascontiguousarray(log10p if len(log10p) else np.zeros(1), np.float64)
+    cap = 1
+    for i in range(n):
+        cap += int((offs[i + 1] - offs[i]) * (offs[n + i + 1] - offs[n + i]))
+    counts = np.zeros(max(n, 1), np.int64)
+    rows3 = np.zeros((cap, 3), np.int64)
+    lik2 = np.zeros((cap, 2))
+    k = orc().orc_pairs(C.byref(params), n, ptr(offs, C.c_int64), ptr(r4, C.c_int64), ptr(lp, C.c_double),
+                        ptr(counts, C.c_int64), ptr(rows3, C.c_int64), ptr(lik2, C.c_double), cap)
+    assert k >= 0
+    return counts[:n], rows3[:k], lik2[:k]
 
 
 def orc_calc_mapq(log10p):

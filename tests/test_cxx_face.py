@@ -1,6 +1,7 @@
 """The C++ drop-in face (include/msgseqtk/, libmsgs_cxx.so) and its test program
 tests/cxx/msgseqtk_api_test.cpp: the reference's SMEM_test / SeedChain_test / Alignment_test
-cases and invariants through EGriceLab::MSGseqTK classes backed by the device."""
+cases and invariants, and the PE driver's pairing (AlignmentPE, PairingScheme) checked against
+the device pairing entry, through EGriceLab::MSGseqTK classes backed by the device."""
 import os
 import re
 import subprocess
@@ -30,14 +31,17 @@ def test_cxx_library_exports_the_face():
                  "EGriceLab::MSGseqTK::SeedChain::getChains(", "EGriceLab::MSGseqTK::Alignment::buildAlignments(",
                  "EGriceLab::MSGseqTK::Alignment::calcMapQ(", "EGriceLab::MSGseqTK::Alignment::sort(",
                  "EGriceLab::MSGseqTK::Alignment::filter(", "EGriceLab::MSGseqTK::Alignment::mdTag2alnQLen(",
-                 "EGriceLab::MSGseqTK::dna::revcom(", "EGriceLab::MSGseqTK::PrimarySeq::revcom("]:
+                 "EGriceLab::MSGseqTK::dna::revcom(", "EGriceLab::MSGseqTK::PrimarySeq::revcom(",
+                 "EGriceLab::MSGseqTK::AlignmentPE::getPairs(", "EGriceLab::MSGseqTK::AlignmentPE::filter(",
+                 "EGriceLab::MSGseqTK::AlignmentPE::calcMapQ(", "EGriceLab::MSGseqTK::AlignmentPE::sort(",
+                 "EGriceLab::MSGseqTK::AlignmentPE::getInsertSize(", "EGriceLab::MSGseqTK::PairingScheme::pr("]:
         assert re.search(re.escape(name[:-1]) + r"(\[abi:cxx11\])?\(", syms), name
 
 
 def test_cxx_header_namespace():
     hdr = open(os.path.join(ROOT, "include", "msgseqtk", "msgseqtk_api.h")).read()
     assert "namespace EGriceLab" in hdr and "namespace MSGseqTK" in hdr
-    for h in ["FMDIndex.h", "SMEM.h", "SeedChain.h", "Alignment.h", "MetaGenome.h", "PrimarySeq.h"]:
+    for h in ["FMDIndex.h", "SMEM.h", "SeedChain.h", "Alignment.h", "MetaGenome.h", "PrimarySeq.h", "PairingScheme.h"]:
         assert os.path.exists(os.path.join(ROOT, "include", "msgseqtk", h))
 
 
@@ -57,3 +61,4 @@ def test_cxx_program_on_gpu():
     print(r.stderr)
     assert r.returncode == 0 and r.stdout.rstrip().endswith("PASS")
     assert re.search(r"Alignment_test 5: \d+ alignments, best 11M1D10M MD:Z:21", r.stdout)
+    assert re.search(r"pairing: 4 candidates, 2 in range, insert size 260, mapQ 3", r.stdout)

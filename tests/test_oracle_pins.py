@@ -1,7 +1,7 @@
 """Pins of the oracle functions that the reference's seeding stack alone does not fix:
 the nt16 alphabet and read types (A1/A2), the BD text (A3), getSeeds / findSeeds
 (A14/A15), chaining invariants (A18), the chain DP with its quirks (A20-A25),
-Alignment::evaluate (A28) and calcMapQ (A29).
+Alignment::evaluate (A28), calcMapQ (A29) and AlignmentPE's pairs (A30).
 
 Each check is one of: the reference's own known-answer tests (test/DNAseq_test.cpp,
 test/PrimarySeq_test.cpp), the reference's own invariants (test/SMEM_test.cpp
@@ -355,6 +355,61 @@ def test_calc_mapq_closed_forms():
     assert pp[0] == 0.5 and list(mq) == [3, 3]
     pp, mq = O.orc_calc_mapq([-500.0, -501.0])
     assert abs(pp[0] - 10 / 11) < 1e-12 and list(mq) == [10, 0]
+
+
+def test_pairing_known_answers():
+    """AlignmentPE (src/Alignment.cpp:540-581, src/Alignment.h:446-575,635-639) and
+    PairingScheme::pr (src/PairingScheme.cpp:18-25) on hand-built lists: only opposite strands
+    pair; the insert size is max(end) - min(start) on one chromosome and -1 across two; pr is the
+    normal pdf inside [insMin, insMax] (so log10lik adds log10(1 / (sd sqrt(2 pi))) at the mean),
+    0 outside (log10lik -inf, then removed by the range filter) and 1 when insMean is 0; pairs sort
+    by log10lik, largest first; each filter removes exactly the pairs its predicate names."""
+    P = O.default_params()                          # N(200, 20) on [0, 400]
+    c0 = math.log10(1 / (20 * math.sqrt(2 * math.pi)))
+    # read: FWD at [1000, 1150) (log10P -2), REV at [1100, 1250) (-1), FWD on chrom 1 (-3)
+    # mate: REV at [1050, 1200) (-1), FWD at [900, 1000) (-2)
+    rows = [(0, 1, 1000, 1150), (0, 2, 1100, 1250), (1, 1, 1000, 1150),
+            (0, 2, 1050, 1200), (0, 1, 900, 1000)]
+    lp = [-2.0, -1.0, -3.0, -1.0, -2.0]
+    offs = [0, 3, 5]
+    cnt, r3, l2 = O.orc_pairs(P, offs, rows, lp)
+    # candidates: (0,3) isize 200, (1,4) isize 350, (2,3) isize -1 (removed: -1 < 0)
+    assert list(cnt) == [2]
+    by = {(int(a), int(b)): (int(s), x, y) for (a, b, s), (x, y) in zip(r3, l2)}
+    assert set(by) == {(0, 3), (1, 4)}
+    assert by[(0, 3)][0] == 200 and by[(1, 4)][0] == 350
+    assert math.isclose(by[(0, 3)][1], -3.0 + c0, rel_tol=1e-14)
+    z = (350 - 200) / 20
+    assert math.isclose(by[(1, 4)][1], -3.0 + c0 - z * z / 2 * math.log10(math.e), rel_tol=1e-13)
+    assert math.isclose(by[(0, 3)][2], (-3.0 + c0) * math.log(10), rel_tol=1e-13)
+    assert [tuple(x[:2]) for x in r3] == [(0, 3), (1, 4)]        # larger log10lik first
+    # insert size 400 is in range, 401 is not
+    c, r, _ = O.orc_pairs(P, [0, 1, 2], [(0, 1, 0, 100), (0, 2, 300, 400)], [-1.0, -1.0])
+    assert list(c) == [1] and r[0][2] == 400
+    c, r, _ = O.orc_pairs(P, [0, 1, 2], [(0, 1, 0, 100), (0, 2, 300, 401)], [-1.0, -1.0])
+    assert list(c) == [0]
+    # same strand: no candidate at all
+    c, _, _ = O.orc_pairs(P, [0, 1, 2], [(0, 1, 0, 100), (0, 1, 100, 200)], [-1.0, -1.0])
+    assert list(c) == [0]
+    # uniform model: log10lik is the sum of the mates' values; -1 is kept when the range holds it
+    U = O.default_params(insMean=0, insSD=1, insMin=-1, insMax=400)
+    c, r, l = O.orc_pairs(U, offs, rows, lp)
+    assert list(c) == [3] and [int(x[2]) for x in r] == [200, 350, -1]
+    assert [x[0] for x in l] == [-3.0, -3.0, -4.0]
+    # filters: mates at [0, 100) and [50, 150) overlap (tail-over); [0, 200) contains [50, 150)
+    # (which is also tail-over: one starts first and ends past the other's start); [0, 100) and
+    # [100, 200) only touch
+    tail = [(0, 1, 0, 100), (0, 2, 50, 150)]
+    cont = [(0, 1, 0, 200), (0, 2, 50, 150)]
+    far = [(0, 1, 0, 100), (0, 2, 100, 200)]
+    for flags, kept in [(dict(noTailOver=1), (0, 0, 1)), (dict(noContain=1), (1, 0, 1)),
+                        (dict(noOverlap=1), (0, 0, 1)), (dict(noDiscordant=1), (1, 1, 1))]:
+        got = tuple(int(O.orc_pairs(O.default_params(insMin=0, insMax=1000, **flags), [0, 1, 2], x,
+                                    [-1.0, -1.0])[0][0]) for x in (tail, cont, far))
+        assert got == kept, flags
+    # maxNPair keeps the first pairs of the sorted list
+    c, r, _ = O.orc_pairs(O.default_params(insMax=1000, maxNPair=1), offs, rows, lp)
+    assert list(c) == [1] and tuple(r[0][:2]) == (0, 3)
 
 
 def test_record_mate_fields_and_unmapped_layout():
