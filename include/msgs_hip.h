@@ -81,7 +81,12 @@ const char* msgs_last_error(msgs_ctx* ctx);
  * submitted run of a resident batch first copies the batch's bases and qualities to the device
  * again from pinned host memory (streaming input; default 0); "dp_serial" 1 = every DP job is
  * filled by the serial path (all of M, I, D kept per cell) that otherwise takes only jobs with
- * more than 64 seeds or a rectangle wider than 512 columns (default 0).  Results never depend
+ * more than 64 seeds or a rectangle wider than 512 columns (default 0); "wide_index" (read when a
+ * database is built or loaded) 0 = the wide index layout (64-bit SA / ISA rows and
+ * bi-intervals, occ-block counts relative to superblocks) only for BD texts of 2^31 symbols or
+ * more (default), 1 = for every database, with superblocks of 2^14 positions and suffix-sorting
+ * ranges of 2^16 suffixes, so that small texts run the wide code paths; msgs_ctx_get
+ * "index_wide" tells which layout the resident database has.  Results never depend
  * on these settings, except "record_format": the records msgs_map_batch and msgs_batch_sam
  * return are SAM text lines (0, default) or the same records in the BAM layout (1): for each,
  * what htslib's bam_write1 puts in the BGZF stream (int32 block_size, the 32-byte core, read name,
@@ -93,7 +98,9 @@ int msgs_ctx_get(msgs_ctx* ctx, const char* key, int64_t* value);
  * Replaces MetaGenome::load/loadSeq + FMDIndex::load (src/MetaGenome.cpp:42-73,
  * src/FMDIndex.cpp:153-161) and, for building, FMDIndex(const DNAseq&, ...)
  * (src/FMDIndex.cpp:29-116) over MetaGenome::getBDSeq (src/MetaGenome.cpp:104-113).
- * The suffix array is built on the device.  names: '\n'-separated chrom names. */
+ * The suffix array is built on the device.  names: '\n'-separated chrom names.  BD texts of
+ * up to 2^34 - 3 symbols (chrom lengths summing to about 8.5 Gbp); longer ones return
+ * MSGS_E_UNSUPPORTED. */
 int msgs_db_build(msgs_ctx* ctx, const uint8_t* chromSeqs, const int64_t* chromLens, int32_t nChrom,
                   const char* names);
 /* The reference's database files DB.mtg + DB.mgs (MetaGenome::load / loadSeq,

@@ -19,9 +19,10 @@
 /* ------------------------------------------------------------------ state */
 
 void DbState::release() {
-    void* ptrs[] = {dBd, dSa, dIsa, dBwt, dOcc, dNPos, dChromBD, dTgtOff, dChromLen, dTgt};
+    void* ptrs[] = {dBd, dSa, dIsa, dBwt, dOcc, dNPos, dChromBD, dTgtOff, dChromLen, dTgt, dSup};
     for (void* p : ptrs) if (p) (void)hipFree(p);
-    dBd = nullptr; dSa = nullptr; dIsa = nullptr; dBwt = nullptr; dOcc = nullptr; dNPos = nullptr;
+    dBd = nullptr; dSa = nullptr; dIsa = nullptr; dBwt = nullptr; dOcc = nullptr; dNPos = nullptr; dSup = nullptr;
+    wide = false;
     dChromBD = nullptr; dTgtOff = nullptr; dChromLen = nullptr; dTgt = nullptr;
     n = 0;
 }
@@ -29,6 +30,7 @@ void DbState::release() {
 DevIndex DbState::view() const {
     DevIndex v;
     v.occ = dOcc; v.sa = dSa; v.isa = dIsa; v.bd = dBd; v.nPos = dNPos; v.nN = nN; v.n = n;
+    v.wide = wide ? 1 : 0; v.supShift = supShift; v.sup = dSup;
     for (int i = 0; i < 17; ++i) v.C[i] = C[i];
     for (int i = 0; i < 16; ++i) v.ll[i] = ll[i];
     v.chromBD = dChromBD; v.nChrom = nChrom; v.tgt = dTgt; v.tgtOff = dTgtOff; v.chromLen = dChromLen;
@@ -132,7 +134,10 @@ __global__ void k_rank(DevIndex ix, const uint8_t* sym, const int64_t* pos, int6
     int64_t r[5];
     occ5(ix, pos[i], r);
     uint8_t s = sym[i];
-    out[i] = s == 0 ? r[0] : s == 1 ? r[1] : s == 2 ? r[2] : s == 4 ? r[3] : s == 8 ? r[4] : INT64_MIN;
+    /* N from the sorted side array; the other IUPAC codes never occur in a BD text */
+    const bool in = pos[i] >= 0 && pos[i] < ix.n;
+    out[i] = s == 0 ? r[0] : s == 1 ? r[1] : s == 2 ? r[2] : s == 4 ? r[3] : s == 8 ? r[4]
+           : !in ? -1 : s == 15 ? countN(ix, pos[i]) : 0;
 }
 
 __global__ void k_ext(DevIndex ix, int64_t* pqs, const uint8_t* bases, int32_t fwd, int64_t* ret, int64_t n) {
@@ -171,10 +176,39 @@ __global__ void k_search(DevIndex ix, const uint8_t* seqs, const int64_t* offs, 
 /* SA values of the given rows (FMDIndex::accessSA, src/FMDIndex.cpp:520-527) */
 __global__ void k_sa(DevIndex ix, const int64_t* rows, int64_t* out, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = (int64_t)ix.sa[rows[i]];
+    if (i < n) out[i] = saAt(ix, rows[i]);
 }
 
 static inline unsigned gridFor(int64_t n, int t = 256) { return (unsigned)std::max<int64_t>(1, (n + t - 1) / t); }
+
+/* the search kernels compiled for the database's index kind: 32-bit rows for narrow indexes,
+ * 64-bit rows for wide ones */
+template <class Row>
+static void launchGroupSearchT(int G, unsigned grid, const DevIndex& ix, const SmemArgs& a, hipStream_t st) {
+    if (G == 4) smem_group_kernel<4, Row><<<grid, 256, 0, st>>>(ix, a);
+    else if (G == 8) smem_group_kernel<8, Row><<<grid, 256, 0, st>>>(ix, a);
+    else smem_group_kernel<16, Row><<<grid, 256, 0, st>>>(ix, a);
+}
+static void launchGroupSearch(bool wide, int G, unsigned grid, const DevIndex& ix, const SmemArgs& a, hipStream_t st) {
+    if (wide) launchGroupSearchT<int64_t>(G, grid, ix, a, st);
+    else launchGroupSearchT<int32_t>(G, grid, ix, a, st);
+}
+template <class Row>
+static void launchLaneSearchT(bool fast, bool pair, unsigned grid, const DevIndex& ix, const SmemArgs& a,
+                              hipStream_t st) {
+    if (pair) {
+        if (fast) smem_pair_kernel<true, Row><<<grid, 256, 0, st>>>(ix, a);
+        else smem_pair_kernel<false, Row><<<grid, 256, 0, st>>>(ix, a);
+    } else {
+        if (fast) smem_kernel_t<true, Row><<<grid, 256, 0, st>>>(ix, a);
+        else smem_kernel_t<false, Row><<<grid, 256, 0, st>>>(ix, a);
+    }
+}
+static void launchLaneSearch(bool wide, bool fast, bool pair, unsigned grid, const DevIndex& ix, const SmemArgs& a,
+                             hipStream_t st) {
+    if (wide) launchLaneSearchT<int64_t>(fast, pair, grid, ix, a, st);
+    else launchLaneSearchT<int32_t>(fast, pair, grid, ix, a, st);
+}
 
 /* ------------------------------------------------------------------ context */
 
@@ -251,8 +285,9 @@ int msgs_db_build(msgs_ctx* c, const uint8_t* seqs, const int64_t* lens, int32_t
         total += 2 * (lens[t] + 1);
         seqTotal += lens[t];
     }
-    /* 32-bit occ counts, SA/ISA entries and bi-interval fields: one-block indexes only */
-    if (total >= (int64_t)1 << 31) return fail(c, MSGS_E_UNSUPPORTED, "BD text of 2^31 or more bases");
+    /* narrow indexes (32-bit rows) below 2^31 symbols, wide ones (64-bit rows, superblock
+     * counts) up to 2^34 */
+    if (total >= ((int64_t)1 << 34) - 2) return fail(c, MSGS_E_UNSUPPORTED, "BD text of 2^34 or more bases");
     /* names */
     const char* p = names ? names : "";
     for (int32_t t = 0; t < nChrom; ++t) {
@@ -308,7 +343,11 @@ int msgs_db_build(msgs_ctx* c, const uint8_t* seqs, const int64_t* lens, int32_t
     if (hipMemcpyAsync(db.dBd, bd.data(), total, hipMemcpyHostToDevice, st) != hipSuccess)
         return fail(c, MSGS_E_HIP, "BD upload");
     std::string err;
-    int rc = buildIndexDevice(db, st, err);
+    db.wide = c->wideIndex != 0 || total > (int64_t)INT32_MAX;
+    /* wide_index = 1 puts superblock and suffix-sorting range boundaries into small texts */
+    db.supShift = c->wideIndex ? 14 : 31;
+    const int64_t rangeMax = c->wideIndex ? (int64_t)1 << 16 : (int64_t)1 << 29;
+    int rc = buildIndexDevice(db, st, err, rangeMax);
     if (rc != MSGS_OK) return fail(c, rc, err);
     /* targets and coordinates */
     if (hipMalloc(&db.dTgt, db.tgtHost.size() + 64) != hipSuccess || hipMemset(db.dTgt, 0, db.tgtHost.size() + 64) != hipSuccess ||
@@ -345,7 +384,9 @@ int msgs_db_copy(msgs_ctx* c, uint8_t* bd, uint8_t* bwt, int64_t* sa) {
     hipStream_t st = c->x().stream;
     if (bd && hipMemcpyAsync(bd, db.dBd, db.n, hipMemcpyDeviceToHost, st) != hipSuccess) return fail(c, MSGS_E_HIP, "copy");
     if (bwt && hipMemcpyAsync(bwt, db.dBwt, db.n, hipMemcpyDeviceToHost, st) != hipSuccess) return fail(c, MSGS_E_HIP, "copy");
-    if (sa) {
+    if (sa && db.wide) {
+        if (hipMemcpyAsync(sa, db.dSa, db.n * 8, hipMemcpyDeviceToHost, st) != hipSuccess) return fail(c, MSGS_E_HIP, "copy");
+    } else if (sa) {
         std::vector<uint32_t> s(db.n);
         if (hipMemcpyAsync(s.data(), db.dSa, db.n * 4, hipMemcpyDeviceToHost, st) != hipSuccess) return fail(c, MSGS_E_HIP, "copy");
         if (hipStreamSynchronize(st) != hipSuccess) return fail(c, MSGS_E_HIP, "sync");
@@ -543,7 +584,10 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
     const int64_t lanesSplit = std::max<int64_t>(256, (lanesMax / 256) * 256);
     const int64_t candCap = r.maxL + 2;
     hipError_t e = hipSuccess;
-    a.cand = c->x().bufs.get<SmemCand>("sm.cand", candCap * std::max(lanes, split ? lanesSplit / split : 0), e);
+    const int64_t nCand = candCap * std::max(lanes, split ? lanesSplit / split : 0);
+    const bool wide = c->db.wide;
+    a.cand = wide ? (void*)c->x().bufs.get<SmemCandT<int64_t>>("sm.cand", nCand, e)
+                  : (void*)c->x().bufs.get<SmemCandT<int32_t>>("sm.cand", nCand, e);
     a.candCap = candCap;
     a.nLanes = lanes;
     a.queue = c->x().bufs.get<unsigned long long>("sm.q", 2, e);
@@ -573,11 +617,8 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
             a2.qPart = 2;
             a2.queue = a.queue + 1;
             const unsigned g1 = (unsigned)(lanesSplit / 256), g2 = (unsigned)(lanes / 256);
-            if (split == 4) smem_group_kernel<4><<<g1, 256, 0, st>>>(c->db.view(), a1);
-            else if (split == 8) smem_group_kernel<8><<<g1, 256, 0, st>>>(c->db.view(), a1);
-            else smem_group_kernel<16><<<g1, 256, 0, st>>>(c->db.view(), a1);
-            if (c->textPath) smem_kernel_t<true><<<g2, 256, 0, st>>>(c->db.view(), a2);
-            else smem_kernel_t<false><<<g2, 256, 0, st>>>(c->db.view(), a2);
+            launchGroupSearch(wide, split, g1, c->db.view(), a1, st);
+            launchLaneSearch(wide, c->textPath != 0, false, g2, c->db.view(), a2, st);
         } else if (n > 0 && pair) {
             /* each lane of a pair writes its own list; the merge puts them in order */
             SmemArgs k = a;
@@ -590,16 +631,14 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
             if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "smem pair lists");
             (void)hipMemsetAsync(k.cntB, 0, (n + 1) * sizeof(int32_t), st);
             const unsigned grid = (unsigned)(lanes / 256);
-            if (c->form == 0 || (c->form == 2 && !c->textPath)) smem_pair_kernel<false><<<grid, 256, 0, st>>>(c->db.view(), k);
-            else smem_pair_kernel<true><<<grid, 256, 0, st>>>(c->db.view(), k);
+            const bool fast = !(c->form == 0 || (c->form == 2 && !c->textPath));
+            launchLaneSearch(wide, fast, true, grid, c->db.view(), k, st);
             (void)hipEventRecord(c->x().ev5, st);
             smem_merge_kernel<<<(unsigned)std::min<int64_t>((n + 255) / 256, 65535), 256, 0, st>>>(k, a.out, a.cnt);
         } else if (n > 0) {
             const unsigned grid = (unsigned)(lanes / 256);
-            if (G == 8) smem_group_kernel<8><<<grid, 256, 0, st>>>(c->db.view(), a);
-            else if (G == 16) smem_group_kernel<16><<<grid, 256, 0, st>>>(c->db.view(), a);
-            else if (c->form == 0 || (c->form == 2 && !c->textPath)) smem_kernel_t<false><<<grid, 256, 0, st>>>(c->db.view(), a);
-            else smem_kernel_t<true><<<grid, 256, 0, st>>>(c->db.view(), a);
+            if (G == 8 || G == 16) launchGroupSearch(wide, G, grid, c->db.view(), a, st);
+            else launchLaneSearch(wide, !(c->form == 0 || (c->form == 2 && !c->textPath)), false, grid, c->db.view(), a, st);
         }
         if (!(n > 0 && pair)) (void)hipEventRecord(c->x().ev5, st);
         (void)hipEventRecord(c->x().ev1, st);
@@ -658,7 +697,9 @@ static int runMems(msgs_ctx* c, const DevReads& r, const msgs_params* p, int& ca
         a.out = c->x().bufs.get<SmemRec>("mm.out", (size_t)cap * std::max<int64_t>(n, 1), e);
         if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "mem output");
         if (n > 0) {
-            mem_kernel<<<(unsigned)std::min<int64_t>((n + 255) / 256, 65535), 256, 0, st>>>(c->db.view(), a, dLL);
+            const unsigned grid = (unsigned)std::min<int64_t>((n + 255) / 256, 65535);
+            if (c->db.wide) mem_kernel<int64_t><<<grid, 256, 0, st>>>(c->db.view(), a, dLL);
+            else mem_kernel<int32_t><<<grid, 256, 0, st>>>(c->db.view(), a, dLL);
             if (hipGetLastError() != hipSuccess) return fail(c, MSGS_E_HIP, "mem launch");
             (void)hipMemcpyAsync(cnt.data(), a.cnt, n * 4, hipMemcpyDeviceToHost, st);
             (void)hipMemcpyAsync(ll.data(), dLL, n * 8, hipMemcpyDeviceToHost, st);
@@ -924,6 +965,7 @@ extern "C" int msgs_ctx_set(msgs_ctx* c, const char* key, int64_t value) {
     else if (k == "ingest") { if (value != 0 && value != 1) return fail(c, MSGS_E_ARG, "ingest: 0 or 1"); c->ingest = (int)value; }
     else if (k == "record_format") { if (value != 0 && value != 1) return fail(c, MSGS_E_ARG, "record_format: 0 or 1"); c->recordFormat = (int)value; }
     else if (k == "dp_serial") { if (value != 0 && value != 1) return fail(c, MSGS_E_ARG, "dp_serial: 0 or 1"); c->dpSerial = (int)value; }
+    else if (k == "wide_index") { if (value != 0 && value != 1) return fail(c, MSGS_E_ARG, "wide_index: 0 or 1"); c->wideIndex = (int)value; }
     else return fail(c, MSGS_E_ARG, "unknown option " + k);
     return MSGS_OK;
 }
@@ -944,6 +986,8 @@ extern "C" int msgs_ctx_get(msgs_ctx* c, const char* key, int64_t* value) {
     else if (k == "ingest") *value = c->ingest;
     else if (k == "dp_serial") *value = c->dpSerial;
     else if (k == "record_format") *value = c->recordFormat;
+    else if (k == "wide_index") *value = c->wideIndex;
+    else if (k == "index_wide") *value = c->db.wide ? 1 : 0;
     else return fail(c, MSGS_E_ARG, "unknown option " + k);
     return MSGS_OK;
 }

@@ -12,8 +12,13 @@
  * counts C+T, G+T, T and specials with four popcounts per word.  A rank query at
  * any i touches exactly one block.  '\0' counts come from
  * positions minus the four base counts minus N positions; N positions (absent in
- * the synthetic workloads) are kept in a sorted side array.  Counts are 32-bit, so
- * the BD text must be shorter than 2^32.
+ * the synthetic workloads) are kept in a sorted side array.
+ *
+ * Narrow indexes (BD text shorter than 2^31) hold absolute counts in cnt[] and 32-bit
+ * SA / ISA rows.  Wide indexes (up to 2^34) hold 64-bit SA / ISA rows, and cnt[] counts
+ * from the start of the block's superblock (2^supShift positions, at most 2^31); the
+ * superblock's own counts sit in a small side table (sup, four per superblock), so a
+ * block is still one 64-byte sector.
  *
  * The rank semantics are those of WaveletTreeRRR::rank
  * (src/libsds/src/WaveletTreeRRR.cpp:101-125): count in BWT[0..i] inclusive, and
@@ -55,8 +60,11 @@ __device__ __forceinline__ void blockCounts(const uint4& sp, const uint4& pl, co
 /* Everything a kernel needs to read the index; passed by value. */
 struct DevIndex {
     const OccBlock* occ;
-    const uint32_t* sa;        /* full suffix array */
-    const uint32_t* isa;       /* inverse suffix array (ISA[SA[i]] = i) */
+    const void* sa;            /* full suffix array: uint32_t rows (narrow) or int64_t (wide) */
+    const void* isa;           /* inverse suffix array (ISA[SA[i]] = i), same width */
+    int32_t wide;              /* 1: 64-bit rows and superblock counts */
+    int32_t supShift;          /* wide: log2 of the superblock length in positions */
+    const int64_t* sup;        /* wide: A, C, G, T counts before each superblock */
     const uint8_t* bd;         /* BD text */
     const int64_t* nPos;       /* sorted BWT positions holding N */
     int64_t nN;
@@ -69,6 +77,24 @@ struct DevIndex {
     const int64_t* tgtOff;     /* start of chrom t in tgt */
     const int64_t* chromLen;   /* length of chrom t (without the terminator) */
 };
+
+/* SA / ISA row k of either width (a uniform branch on the index kind) */
+__device__ __forceinline__ int64_t saAt(const DevIndex& ix, int64_t k) {
+    return ix.wide ? ((const int64_t*)ix.sa)[k] : (int64_t)((const uint32_t*)ix.sa)[k];
+}
+__device__ __forceinline__ int64_t isaAt(const DevIndex& ix, int64_t k) {
+    return ix.wide ? ((const int64_t*)ix.isa)[k] : (int64_t)((const uint32_t*)ix.isa)[k];
+}
+/* the same rows for a search kernel compiled for one index kind: R = int32_t reads a narrow
+ * index, R = int64_t a wide one */
+template <class R> __device__ __forceinline__ R saRow(const DevIndex& ix, int64_t k) {
+    if constexpr (sizeof(R) == 8) return ((const int64_t*)ix.sa)[k];
+    else return (R)((const uint32_t*)ix.sa)[k];
+}
+template <class R> __device__ __forceinline__ R isaRow(const DevIndex& ix, int64_t k) {
+    if constexpr (sizeof(R) == 8) return ((const int64_t*)ix.isa)[k];
+    else return (R)((const uint32_t*)ix.isa)[k];
+}
 
 __device__ __forceinline__ bool amb(uint8_t b) { return !(b == 1 || b == 2 || b == 4 || b == 8); }
 __device__ __forceinline__ int toInt(uint8_t b) { return b == 1 ? 1 : b == 2 ? 2 : b == 4 ? 3 : b == 8 ? 4 : 0; }
@@ -122,11 +148,16 @@ __device__ __forceinline__ void occFromBlock(const DevIndex& ix, const uint4& h,
     blockCounts(sp, pl, ph, j, nLo, nHi, nT, nSp);
     const uint32_t nC = nLo - nT, nG = nHi - nT;
     const uint32_t nA = (j + 1) - nLo - nHi + nT;
-    r[1] = (int64_t)h.x + nA - nSp;
-    r[2] = (int64_t)h.y + nC;
-    r[3] = (int64_t)h.z + nG;
-    r[4] = (int64_t)h.w + nT;
-    int64_t special = (i & ~(int64_t)127) - ((int64_t)h.x + h.y + h.z + h.w) + nSp;
+    int64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    if (ix.wide) {
+        const int64_t* sb = ix.sup + 4 * (i >> ix.supShift);
+        s0 = sb[0]; s1 = sb[1]; s2 = sb[2]; s3 = sb[3];
+    }
+    r[1] = s0 + h.x + nA - nSp;
+    r[2] = s1 + h.y + nC;
+    r[3] = s2 + h.z + nG;
+    r[4] = s3 + h.w + nT;
+    int64_t special = (i & ~(int64_t)127) - (s0 + s1 + s2 + s3 + (int64_t)h.x + h.y + h.z + h.w) + nSp;
     r[0] = ix.nN ? special - countN(ix, i) : special;
 }
 
@@ -175,50 +206,65 @@ __device__ __forceinline__ int64_t dBackExt(const DevIndex& ix, int64_t& p, int6
     return s;
 }
 
-/* 32-bit forms of the ranks and of backExt for the search kernels (indexes hold fewer
- * than 2^31 rows, so every rank, row and size fits; same values as the 64-bit forms) */
-__device__ __forceinline__ void occFromBlock32(const DevIndex& ix, const uint4& h, const uint4& sp, const uint4& pl,
-                                               const uint4& ph, int32_t i, int32_t* r) {
+/* Rank rows and backExt for the search kernels, over the row type of one index kind:
+ * R = int32_t for narrow indexes (fewer than 2^31 rows, so every rank, row and size fits;
+ * the block counts are absolute), R = int64_t for wide ones (block counts plus the
+ * superblock's, loaded beside the block).  Same values as the generic 64-bit forms. */
+template <class R>
+__device__ __forceinline__ void occFromBlockT(const DevIndex& ix, const uint4& h, const uint4& sp, const uint4& pl,
+                                              const uint4& ph, R i, R* r) {
     const uint32_t j = (uint32_t)i & 127u;
     uint32_t nLo, nHi, nT, nSp;
     blockCounts(sp, pl, ph, j, nLo, nHi, nT, nSp);
     const uint32_t nC = nLo - nT, nG = nHi - nT;
     const uint32_t nA = (j + 1) - nLo - nHi + nT;
-    r[1] = (int32_t)(h.x + nA - nSp);
-    r[2] = (int32_t)(h.y + nC);
-    r[3] = (int32_t)(h.z + nG);
-    r[4] = (int32_t)(h.w + nT);
-    const int32_t special = (int32_t)(((uint32_t)i & ~127u) - (h.x + h.y + h.z + h.w) + nSp);
-    r[0] = ix.nN ? special - (int32_t)countN(ix, i) : special;
+    if constexpr (sizeof(R) == 8) {
+        const int64_t* sb = ix.sup + 4 * (i >> ix.supShift);
+        const int64_t s0 = sb[0], s1 = sb[1], s2 = sb[2], s3 = sb[3];
+        r[1] = s0 + (int64_t)(h.x + nA - nSp);
+        r[2] = s1 + (int64_t)(h.y + nC);
+        r[3] = s2 + (int64_t)(h.z + nG);
+        r[4] = s3 + (int64_t)(h.w + nT);
+        const int64_t special = (i & ~(int64_t)127) - (s0 + s1 + s2 + s3) - ((int64_t)h.x + h.y + h.z + h.w) + nSp;
+        r[0] = ix.nN ? special - countN(ix, i) : special;
+    } else {
+        r[1] = (int32_t)(h.x + nA - nSp);
+        r[2] = (int32_t)(h.y + nC);
+        r[3] = (int32_t)(h.z + nG);
+        r[4] = (int32_t)(h.w + nT);
+        const int32_t special = (int32_t)(((uint32_t)i & ~127u) - (h.x + h.y + h.z + h.w) + nSp);
+        r[0] = ix.nN ? special - (int32_t)countN(ix, i) : special;
+    }
 }
 
-__device__ __forceinline__ int32_t dBackExt32(const DevIndex& ix, int32_t& p, int32_t& q, int32_t& s, uint8_t b) {
+template <class R>
+__device__ __forceinline__ R dBackExtT(const DevIndex& ix, R& p, R& q, R& s, uint8_t b) {
     if (amb(b)) return (s = 0);
     const int c = toInt(b);
-    const int32_t a = p - 1, e = p + s - 1;
+    const R a = p - 1, e = p + s - 1;
     const bool va = a >= 0 && (int64_t)a < ix.n, ve = e >= 0 && (int64_t)e < ix.n;
-    const int32_t ba = va ? (a >> 7) : 0, be = ve ? (e >> 7) : 0;
+    const R ba = va ? (a >> 7) : 0, be = ve ? (e >> 7) : 0;
     const uint4* pa = reinterpret_cast<const uint4*>(ix.occ + ba);
     const uint4* pe = reinterpret_cast<const uint4*>(ix.occ + be);
     const uint4 h0 = pa[0], s0 = pa[1], c00 = pa[2], c01 = pa[3];
     uint4 h1, s1, c10, c11;
     if (be != ba) { h1 = pe[0]; s1 = pe[1]; c10 = pe[2]; c11 = pe[3]; }
     else { h1 = h0; s1 = s0; c10 = c00; c11 = c01; }
-    int32_t lo[5], hi[5];
-    if (va) occFromBlock32(ix, h0, s0, c00, c01, a, lo); else { lo[0] = lo[1] = lo[2] = lo[3] = lo[4] = -1; }
-    if (ve) occFromBlock32(ix, h1, s1, c10, c11, e, hi); else { hi[0] = hi[1] = hi[2] = hi[3] = hi[4] = -1; }
+    R lo[5], hi[5];
+    if (va) occFromBlockT<R>(ix, h0, s0, c00, c01, a, lo); else { lo[0] = lo[1] = lo[2] = lo[3] = lo[4] = -1; }
+    if (ve) occFromBlockT<R>(ix, h1, s1, c10, c11, e, hi); else { hi[0] = hi[1] = hi[2] = hi[3] = hi[4] = -1; }
     /* rank c and the '\0' + above-c counts by selects (no indexed arrays), C[b] from the
      * four uniform base entries */
-    const int32_t O = c == 1 ? lo[1] : (c == 2 ? lo[2] : (c == 3 ? lo[3] : lo[4]));
-    const int32_t H = c == 1 ? hi[1] : (c == 2 ? hi[2] : (c == 3 ? hi[3] : hi[4]));
-    const int32_t sc = H - O;
+    const R O = c == 1 ? lo[1] : (c == 2 ? lo[2] : (c == 3 ? lo[3] : lo[4]));
+    const R H = c == 1 ? hi[1] : (c == 2 ? hi[2] : (c == 3 ? hi[3] : hi[4]));
+    const R sc = H - O;
     if (sc <= 0) return (s = sc);
     if (sc != s) {
-        const int32_t add = (hi[0] - lo[0]) + (c < 2 ? hi[2] - lo[2] : 0) + (c < 3 ? hi[3] - lo[3] : 0) +
-                            (c < 4 ? hi[4] - lo[4] : 0);
+        const R add = (hi[0] - lo[0]) + (c < 2 ? hi[2] - lo[2] : 0) + (c < 3 ? hi[3] - lo[3] : 0) +
+                      (c < 4 ? hi[4] - lo[4] : 0);
         q = q + add;
     }
-    const int32_t Cb = (int32_t)(c == 1 ? ix.C[1] : (c == 2 ? ix.C[2] : (c == 3 ? ix.C[4] : ix.C[8])));
+    const R Cb = (R)(c == 1 ? ix.C[1] : (c == 2 ? ix.C[2] : (c == 3 ? ix.C[4] : ix.C[8])));
     p = Cb + O;
     s = sc;
     return s;

@@ -23,8 +23,14 @@ struct DbState {
     int32_t nChrom = 0;
     /* device */
     uint8_t* dBd = nullptr;
-    uint32_t* dSa = nullptr;
-    uint32_t* dIsa = nullptr;
+    /* narrow index (n < 2^31): uint32_t rows and absolute occ-block counts; wide index
+     * (n < 2^34, or every index with the context option wide_index): int64_t rows and
+     * occ-block counts from the start of a superblock of 2^supShift positions */
+    bool wide = false;
+    int32_t supShift = 31;
+    void* dSa = nullptr;
+    void* dIsa = nullptr;
+    int64_t* dSup = nullptr;   /* wide: A, C, G, T counts before each superblock */
     uint8_t* dBwt = nullptr;
     OccBlock* dOcc = nullptr;
     int64_t* dNPos = nullptr;
@@ -38,6 +44,9 @@ struct DbState {
     DevIndex view() const;
 };
 
-int buildIndexDevice(DbState& db, hipStream_t st, std::string& err);
+/* suffix array, ISA, BWT and occ blocks of db.dBd; wide: build the wide layout (set by the
+ * caller; required from 2^31 symbols on) with superblocks of 2^supShift positions and
+ * suffix-sorting ranges of at most rangeMax suffixes (the test shape uses small ones) */
+int buildIndexDevice(DbState& db, hipStream_t st, std::string& err, int64_t rangeMax);
 
 #endif

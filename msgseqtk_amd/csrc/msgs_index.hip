@@ -8,10 +8,25 @@
  * This is the order libdivsufsort gives (byte order, a suffix that is a prefix of
  * another first), used by the reference's FMDIndex constructor
  * (src/FMDIndex.cpp:29-68).  Then BWT (src/FMDIndex.cpp:385-393) and occ blocks.
+ *
+ * Narrow texts (n < 2^31) sort all suffixes at once with 32-bit rows.  Wide texts (up to
+ * 2^34) cannot: 64-bit rows make a whole-text key/value sort 32 bytes per suffix twice
+ * over.  They are cut into ranges of the final suffix array instead: the 4096 buckets of
+ * the first four symbols are counted, consecutive buckets are packed into ranges of at
+ * most rangeMax suffixes, and each range is sorted on its own.  Round 0 gathers a range's
+ * suffixes by their bucket and sorts them by the 21-symbol key; later rounds sort only the
+ * suffixes still in groups of two or more (a bitmap marks the settled rows) by
+ * ((rank[i] - range start) << 34 | rank[i+h] + 1).  Ranks are the SA row of the first
+ * member of a suffix's group and are refined in place, range after range: a rank read in
+ * a round may already be finer than that round's, which orders ties by a longer prefix
+ * and so stays consistent with the suffix order (the argument of Larsson and Sadakane's
+ * qsufsort).  The ranks become the ISA.
  */
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_select.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
 
 #include <algorithm>
 #include <cstdio>
@@ -80,16 +95,137 @@ __global__ void __launch_bounds__(256) k_count_distinct(const uint32_t* gstart, 
     }
 }
 
+/* ---------------------------------------------------------------- wide texts */
+
+constexpr int kBinSyms = 4;                    /* bucket = the first four 3-bit symbols */
+constexpr int kBins = 1 << (3 * kBinSyms);
+
+__device__ __forceinline__ uint32_t binOf(const uint8_t* bd, int64_t n, int64_t i) {
+    uint32_t b = 0;
+    for (int t = 0; t < kBinSyms; ++t) b = (b << 3) | (uint32_t)(i + t < n ? sym3(bd[i + t]) : 0);
+    return b;
+}
+
+__device__ __forceinline__ uint64_t key21(const uint8_t* bd, int64_t n, int64_t i) {
+    uint64_t k = 0;
+    for (int t = 0; t < 21; ++t) k = (k << 3) | (i + t < n ? sym3(bd[i + t]) : 0);
+    return k;
+}
+
+/* suffixes per bucket: LDS counts per block, added to the global table once per block */
+__global__ void __launch_bounds__(256) k_bin_hist(const uint8_t* bd, int64_t n, unsigned long long* hist) {
+    __shared__ uint32_t h[kBins];
+    for (int b = threadIdx.x; b < kBins; b += blockDim.x) h[b] = 0;
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&h[binOf(bd, n, i)], 1u);
+    __syncthreads();
+    for (int b = threadIdx.x; b < kBins; b += blockDim.x)
+        if (h[b]) atomicAdd(&hist[b], (unsigned long long)h[b]);
+}
+
+/* the positions whose bucket lies in [b0, b1), in any order: each block takes 4096
+ * positions per step and reserves its slots with one global atomic */
+__global__ void __launch_bounds__(256) k_bin_select(const uint8_t* bd, int64_t n, uint32_t b0, uint32_t b1,
+                                                    int64_t* out, unsigned long long* counter) {
+    __shared__ uint32_t sCnt;
+    __shared__ unsigned long long sBase;
+    constexpr int kPer = 16;
+    for (int64_t base = (int64_t)blockIdx.x * 256 * kPer; base < n; base += (int64_t)gridDim.x * 256 * kPer) {
+        uint32_t mask = 0;
+        for (int r = 0; r < kPer; ++r) {
+            const int64_t i = base + r * 256 + threadIdx.x;
+            if (i < n) {
+                const uint32_t b = binOf(bd, n, i);
+                if (b >= b0 && b < b1) mask |= 1u << r;
+            }
+        }
+        if (threadIdx.x == 0) sCnt = 0;
+        __syncthreads();
+        const uint32_t c = __popc(mask);
+        const uint32_t off = c ? atomicAdd(&sCnt, c) : 0u;
+        __syncthreads();
+        if (threadIdx.x == 0) sBase = atomicAdd(counter, (unsigned long long)sCnt);
+        __syncthreads();
+        int64_t o = (int64_t)(sBase + off);
+        for (int r = 0; r < kPer; ++r)
+            if (mask >> r & 1u) out[o++] = base + r * 256 + threadIdx.x;
+        __syncthreads();
+    }
+}
+
+__global__ void k_keys21(const uint8_t* bd, int64_t n, const int64_t* v, int64_t m, uint64_t* key) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < m) key[j] = key21(bd, n, v[j]);
+}
+
+/* the rows of a range's unsettled suffixes: flag[j] = 1 when row rs + j is not settled */
+__global__ void k_unsettled(const uint32_t* done, int64_t rs, int64_t m, uint8_t* flag) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m) return;
+    const int64_t p = rs + j;
+    flag[j] = (done[p >> 5] >> (p & 31) & 1u) ? 0 : 1;
+}
+
+/* round keys of the unsettled rows P[0..m): (rank - rs) << 34 | rank[v + h] + 1 */
+__global__ void k_round_keys_w(const int64_t* P, int64_t m, const int64_t* sa, const int64_t* rank, int64_t n,
+                               int64_t h, int64_t rs, int64_t* v, uint64_t* key) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m) return;
+    const int64_t s = sa[P[j]];
+    v[j] = s;
+    const uint64_t r2 = s + h < n ? (uint64_t)rank[s + h] + 1 : 0;
+    key[j] = ((uint64_t)(rank[s] - rs) << 34) | r2;
+}
+
+/* sorted suffixes into their rows: P[j], or rs + j when P is NULL */
+__global__ void k_place(const int64_t* P, const int64_t* v, int64_t m, int64_t rs, int64_t* sa) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < m) sa[P ? P[j] : rs + j] = v[j];
+}
+
+__global__ void k_heads_w(const uint64_t* key, int64_t m, uint32_t* head) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < m) head[j] = (j == 0 || key[j] != key[j - 1]) ? (uint32_t)j : 0u;
+}
+
+/* rank = the row of the group's first member; a suffix alone in its group is settled */
+__global__ void __launch_bounds__(256) k_rank_w(const uint64_t* key, const int64_t* v, const uint32_t* gstart,
+                                                const int64_t* P, int64_t m, int64_t rs, int64_t* rank,
+                                                uint32_t* done, unsigned long long* settled) {
+    __shared__ unsigned long long part[4];
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long one = 0;
+    if (j < m) {
+        const int64_t g = gstart[j];
+        rank[v[j]] = P ? P[g] : rs + g;
+        const bool alone = (j == 0 || key[j] != key[j - 1]) && (j == m - 1 || key[j] != key[j + 1]);
+        if (alone) {
+            const int64_t p = P ? P[j] : rs + j;
+            atomicOr(&done[p >> 5], 1u << (p & 31));
+            one = 1;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) one += __shfl_down(one, off, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = one;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned long long t = part[0] + part[1] + part[2] + part[3];
+        if (t) atomicAdd(settled, t);
+    }
+}
+
 __global__ void k_isa(const uint32_t* sa, int64_t n, uint32_t* isa) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     isa[sa[k]] = (uint32_t)k;
 }
 
-__global__ void k_bwt(const uint8_t* bd, const uint32_t* sa, int64_t n, uint8_t* bwt) {
+template <class T>
+__global__ void k_bwt(const uint8_t* bd, const T* sa, int64_t n, uint8_t* bwt) {
     int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
-    uint32_t s = sa[k];
+    const int64_t s = (int64_t)sa[k];
     bwt[k] = s == 0 ? 0 : bd[s - 1];
 }
 
@@ -154,11 +290,13 @@ struct Freer {
         }                                                                       \
     } while (0)
 
-/* Suffix array, BWT and occ blocks of db.dBd (n bytes) into db's device buffers. */
-int buildIndexDevice(DbState& db, hipStream_t st, std::string& err) {
+namespace {
+
+/* Suffix array of a narrow text (n < 2^31) into db.dSa / db.dIsa (uint32_t rows). */
+int sortNarrow(DbState& db, hipStream_t st, std::string& err) {
     const int64_t n = db.n;
-    if (n <= 0 || n >= (int64_t)0xffffffffLL || n > (int64_t)INT32_MAX) {
-        err = "BD text length must be in [1, 2^31)";
+    if (n <= 0 || n > (int64_t)INT32_MAX) {
+        err = "a narrow index needs a BD text length in [1, 2^31)";
         return MSGS_E_UNSUPPORTED;
     }
     Freer tmp;
@@ -209,10 +347,151 @@ int buildIndexDevice(DbState& db, hipStream_t st, std::string& err) {
     HCHK(hipMalloc(&db.dSa, n * sizeof(uint32_t)));
     HCHK(hipMemcpyAsync(db.dSa, v0, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st));
     HCHK(hipMalloc(&db.dIsa, n * sizeof(uint32_t)));
-    k_isa<<<grid(n), 256, 0, st>>>(db.dSa, n, db.dIsa);
+    k_isa<<<grid(n), 256, 0, st>>>((const uint32_t*)db.dSa, n, (uint32_t*)db.dIsa);
     HCHK(hipGetLastError());
+    HCHK(hipStreamSynchronize(st));
+    return MSGS_OK;
+}
+
+/* Suffix array and ISA of a wide text into db.dSa / db.dIsa (int64_t rows), range by range
+ * (the file header describes the rounds). */
+int sortWide(DbState& db, hipStream_t st, std::string& err, int64_t rangeMax) {
+    const int64_t n = db.n;
+    if (n <= 0 || n >= ((int64_t)1 << 34) - 2) {
+        err = "BD text length must be in [1, 2^34 - 2)";
+        return MSGS_E_UNSUPPORTED;
+    }
+    Freer tmp;
+    hipError_t e;
+    /* bucket sizes and the ranges of the suffix array they pack into */
+    unsigned long long* dHist = tmp.alloc<unsigned long long>(kBins + 1, e); HCHK(e);
+    HCHK(hipMemsetAsync(dHist, 0, (kBins + 1) * sizeof(unsigned long long), st));
+    k_bin_hist<<<2048, 256, 0, st>>>(db.dBd, n, dHist);
+    HCHK(hipGetLastError());
+    std::vector<unsigned long long> hist(kBins);
+    HCHK(hipMemcpyAsync(hist.data(), dHist, kBins * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+    HCHK(hipStreamSynchronize(st));
+    int64_t biggest = 0;
+    for (unsigned long long c : hist) biggest = std::max<int64_t>(biggest, (int64_t)c);
+    const int64_t cap = std::max(rangeMax, biggest);
+    if (cap > ((int64_t)1 << 30)) {
+        err = "a 4-symbol bucket holds more than 2^30 suffixes";
+        return MSGS_E_UNSUPPORTED;
+    }
+    struct Range { int64_t rs, m; uint32_t b0, b1; int64_t open; };
+    std::vector<Range> ranges;
+    {
+        int64_t rs = 0, m = 0;
+        uint32_t b0 = 0;
+        for (uint32_t b = 0; b < (uint32_t)kBins; ++b) {
+            if (m + (int64_t)hist[b] > cap) {
+                ranges.push_back({rs, m, b0, b, m});
+                rs += m; m = 0; b0 = b;
+            }
+            m += (int64_t)hist[b];
+        }
+        if (m) ranges.push_back({rs, m, b0, (uint32_t)kBins, m});
+    }
+    int64_t mMax = 0;
+    for (const Range& r : ranges) mMax = std::max(mMax, r.m);
+    /* whole-text arrays: rows, ranks (the ISA at the end), settled-row bitmap */
+    HCHK(hipMalloc(&db.dSa, n * sizeof(int64_t)));
+    HCHK(hipMalloc(&db.dIsa, n * sizeof(int64_t)));
+    int64_t* sa = (int64_t*)db.dSa;
+    int64_t* rank = (int64_t*)db.dIsa;
+    uint32_t* done = tmp.alloc<uint32_t>(n / 32 + 1, e); HCHK(e);
+    HCHK(hipMemsetAsync(done, 0, (n / 32 + 1) * sizeof(uint32_t), st));
+    /* per-range work buffers */
+    uint64_t* k0 = tmp.alloc<uint64_t>(mMax, e); HCHK(e);
+    uint64_t* k1 = tmp.alloc<uint64_t>(mMax, e); HCHK(e);
+    int64_t* v0 = tmp.alloc<int64_t>(mMax, e); HCHK(e);
+    int64_t* v1 = tmp.alloc<int64_t>(mMax, e); HCHK(e);
+    int64_t* P = tmp.alloc<int64_t>(mMax, e); HCHK(e);
+    uint32_t* head = tmp.alloc<uint32_t>(mMax, e); HCHK(e);
+    uint8_t* flag = tmp.alloc<uint8_t>(mMax, e); HCHK(e);
+    unsigned long long* dCnt = tmp.alloc<unsigned long long>(2, e); HCHK(e);
+    size_t sortBytes = 0, scanBytes = 0, selBytes = 0;
+    HCHK(rocprim::radix_sort_pairs(nullptr, sortBytes, k0, k1, v0, v1, (size_t)mMax, 0u, 64u, st));
+    HCHK(rocprim::inclusive_scan(nullptr, scanBytes, head, head, (size_t)mMax, rocprim::maximum<uint32_t>(), st));
+    HCHK(rocprim::select(nullptr, selBytes, rocprim::counting_iterator<int64_t>(0), flag, P, dCnt, (size_t)mMax, st));
+    const size_t tmpBytes = std::max(sortBytes, std::max(scanBytes, selBytes));
+    void* tmpStore = tmp.alloc<uint8_t>(tmpBytes, e); HCHK(e);
+    int mBits = 1;
+    while (mBits < 31 && ((int64_t)1 << mBits) <= mMax) ++mBits;
+
+    /* groups of a sorted range: ranks, settled rows; returns the rows settled */
+    auto settle = [&](const uint64_t* key, const int64_t* v, const int64_t* rows, int64_t m, int64_t rs,
+                      unsigned long long& settled) -> int {
+        k_heads_w<<<grid(m), 256, 0, st>>>(key, m, head);
+        size_t tb = tmpBytes;
+        HCHK(rocprim::inclusive_scan(tmpStore, tb, head, head, (size_t)m, rocprim::maximum<uint32_t>(), st));
+        HCHK(hipMemsetAsync(dCnt + 1, 0, sizeof(unsigned long long), st));
+        k_rank_w<<<grid(m), 256, 0, st>>>(key, v, head, rows, m, rs, rank, done, dCnt + 1);
+        HCHK(hipGetLastError());
+        HCHK(hipMemcpyAsync(&settled, dCnt + 1, sizeof(settled), hipMemcpyDeviceToHost, st));
+        HCHK(hipStreamSynchronize(st));
+        return MSGS_OK;
+    };
+
+    /* round 0: each range's suffixes by their first 21 symbols */
+    int64_t open = 0;
+    for (Range& r : ranges) {
+        HCHK(hipMemsetAsync(dCnt, 0, sizeof(unsigned long long), st));
+        k_bin_select<<<2048, 256, 0, st>>>(db.dBd, n, r.b0, r.b1, v0, dCnt);
+        k_keys21<<<grid(r.m), 256, 0, st>>>(db.dBd, n, v0, r.m, k0);
+        size_t tb = tmpBytes;
+        HCHK(rocprim::radix_sort_pairs(tmpStore, tb, k0, k1, v0, v1, (size_t)r.m, 0u, 63u, st));
+        k_place<<<grid(r.m), 256, 0, st>>>(nullptr, v1, r.m, r.rs, sa);
+        unsigned long long got = 0;
+        HCHK(hipMemcpyAsync(&got, dCnt, sizeof(got), hipMemcpyDeviceToHost, st));
+        unsigned long long settled = 0;
+        int rc = settle(k1, v1, nullptr, r.m, r.rs, settled);
+        if (rc) return rc;
+        if ((int64_t)got != r.m) { err = "bucket gather miscounted"; return MSGS_E_HIP; }
+        r.open = r.m - (int64_t)settled;
+        open += r.open;
+    }
+    /* later rounds: the unsettled rows of each range by (rank, rank[i + h]) */
+    int64_t h = 21;
+    for (int round = 1; open > 0; ++round, h *= 2) {
+        if (round > 40) { err = "suffix sorting did not converge"; return MSGS_E_HIP; }
+        open = 0;
+        for (Range& r : ranges) {
+            if (r.open == 0) continue;
+            k_unsettled<<<grid(r.m), 256, 0, st>>>(done, r.rs, r.m, flag);
+            size_t tb = tmpBytes;
+            HCHK(rocprim::select(tmpStore, tb, rocprim::counting_iterator<int64_t>(r.rs), flag, P, dCnt, (size_t)r.m, st));
+            unsigned long long mu = 0;
+            HCHK(hipMemcpyAsync(&mu, dCnt, sizeof(mu), hipMemcpyDeviceToHost, st));
+            HCHK(hipStreamSynchronize(st));
+            const int64_t m = (int64_t)mu;
+            if (m != r.open) { err = "unsettled rows miscounted"; return MSGS_E_HIP; }
+            k_round_keys_w<<<grid(m), 256, 0, st>>>(P, m, sa, rank, n, h, r.rs, v0, k0);
+            tb = tmpBytes;
+            HCHK(rocprim::radix_sort_pairs(tmpStore, tb, k0, k1, v0, v1, (size_t)m, 0u, (unsigned)(34 + mBits), st));
+            k_place<<<grid(m), 256, 0, st>>>(P, v1, m, 0, sa);
+            unsigned long long settled = 0;
+            int rc = settle(k1, v1, P, m, r.rs, settled);
+            if (rc) return rc;
+            r.open = m - (int64_t)settled;
+            open += r.open;
+        }
+    }
+    return MSGS_OK;
+}
+
+}  // namespace
+
+/* Suffix array, ISA, BWT and occ blocks of db.dBd (n bytes) into db's device buffers. */
+int buildIndexDevice(DbState& db, hipStream_t st, std::string& err, int64_t rangeMax) {
+    const int64_t n = db.n;
+    int rc = db.wide ? sortWide(db, st, err, rangeMax) : sortNarrow(db, st, err);
+    if (rc != MSGS_OK) return rc;
+    Freer tmp;
+    hipError_t e;
     HCHK(hipMalloc(&db.dBwt, n));
-    k_bwt<<<grid(n), 256, 0, st>>>(db.dBd, db.dSa, n, db.dBwt);
+    if (db.wide) k_bwt<int64_t><<<grid(n), 256, 0, st>>>(db.dBd, (const int64_t*)db.dSa, n, db.dBwt);
+    else k_bwt<uint32_t><<<grid(n), 256, 0, st>>>(db.dBd, (const uint32_t*)db.dSa, n, db.dBwt);
     HCHK(hipGetLastError());
 
     const int64_t nBlk = n / 128 + 1;
@@ -223,14 +502,26 @@ int buildIndexDevice(DbState& db, hipStream_t st, std::string& err) {
     std::vector<uint32_t> h_local(nBlk * 4);
     HCHK(hipMemcpyAsync(h_local.data(), local, nBlk * 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
     HCHK(hipStreamSynchronize(st));
-    uint64_t run[4] = {0, 0, 0, 0};
-    for (int64_t b = 0; b < nBlk; ++b)
+    /* block counts: absolute (narrow), or from the start of the block's superblock (wide),
+     * whose own counts go to db.dSup */
+    const int64_t blkPerSup = db.wide ? (int64_t)1 << (db.supShift - 7) : nBlk;
+    const int64_t nSup = (nBlk + blkPerSup - 1) / blkPerSup;
+    std::vector<int64_t> sup(4 * nSup, 0);
+    int64_t run[4] = {0, 0, 0, 0}, base[4] = {0, 0, 0, 0};
+    for (int64_t b = 0; b < nBlk; ++b) {
+        if (db.wide && b % blkPerSup == 0)
+            for (int t = 0; t < 4; ++t) { base[t] = run[t]; sup[4 * (b / blkPerSup) + t] = run[t]; }
         for (int t = 0; t < 4; ++t) {
-            uint64_t c = h_local[b * 4 + t];
-            h_local[b * 4 + t] = (uint32_t)run[t];
+            const int64_t c = h_local[b * 4 + t];
+            h_local[b * 4 + t] = (uint32_t)(run[t] - base[t]);
             run[t] += c;
         }
+    }
     HCHK(hipMemcpyAsync(local, h_local.data(), nBlk * 4 * sizeof(uint32_t), hipMemcpyHostToDevice, st));
+    if (db.wide) {
+        HCHK(hipMalloc(&db.dSup, sup.size() * sizeof(int64_t)));
+        HCHK(hipMemcpyAsync(db.dSup, sup.data(), sup.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    }
     k_fill_cnt<<<grid(nBlk), 256, 0, st>>>(db.dOcc, local, nBlk);
     HCHK(hipGetLastError());
 

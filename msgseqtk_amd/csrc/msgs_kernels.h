@@ -19,9 +19,10 @@ struct SmemRec {
 
 /* a candidate of a search's lock-step list: bi-interval, loglik, its moving side as a
  * text position (X, -1 when the row values are exact) and its fixed read end (pos);
- * 32 bytes (indexes hold fewer than 2^31 rows) */
-struct SmemCand {
-    int32_t F, R, S, X;
+ * rows of the index's width: 32 bytes for narrow indexes (Row = int32_t), 48 for wide */
+template <class Row>
+struct SmemCandT {
+    Row F, R, S, X;
     double LL;
     int32_t pos, pad;
 };
@@ -36,7 +37,7 @@ struct SmemArgs {
     int32_t cap;               /* SMEM slots per read */
     int32_t* cnt;              /* SMEMs emitted per read (may exceed cap) */
     SmemRec* out;              /* [nReads][cap] */
-    SmemCand* cand;            /* candidate scratch, [nLanes][candCap] */
+    void* cand;                /* candidate scratch, [nLanes][candCap] SmemCandT of the index's width */
     int64_t candCap;
     int64_t nLanes;
     const uint8_t* need;       /* per read (may be NULL), bits 0-1: 0 = the read cannot emit an SMEM,
@@ -60,16 +61,17 @@ struct SmemArgs {
     int32_t* tagB;
 };
 
-template <bool FAST> __global__ void smem_kernel_t(DevIndex ix, SmemArgs a);
+/* Row: int32_t for narrow indexes, int64_t for wide ones (DevIndex::wide) */
+template <bool FAST, class Row> __global__ void smem_kernel_t(DevIndex ix, SmemArgs a);
 /* the same search with two lanes per read: after the first search from the midpoint, the
  * forward searches run on one lane and the backward searches on the other */
-template <bool FAST> __global__ void smem_pair_kernel(DevIndex ix, SmemArgs a);
+template <bool FAST, class Row> __global__ void smem_pair_kernel(DevIndex ix, SmemArgs a);
 /* the paired lists of each read in the reference's order, into out / cnt */
 __global__ void smem_merge_kernel(SmemArgs a, SmemRec* out, int32_t* cnt);
 /* SMEM_LIST::findMEMS per read (out / cnt, a.minLen, a.T2) and the list's loglik */
-__global__ void mem_kernel(DevIndex ix, SmemArgs a, double* listLL);
+template <class Row> __global__ void mem_kernel(DevIndex ix, SmemArgs a, double* listLL);
 /* the same search with one read per group of G lanes (same output) */
-template <int G> __global__ void smem_group_kernel(DevIndex ix, SmemArgs a);
+template <int G, class Row> __global__ void smem_group_kernel(DevIndex ix, SmemArgs a);
 /* per-read loglik prefix sums (SmemArgs::prefLL) */
 __global__ void smem_prefix_kernel(DevIndex ix, SmemArgs a, double* pref);
 

@@ -51,9 +51,11 @@ __device__ __forceinline__ short toBf16(float f) {
 }
 __device__ __forceinline__ float fromBf16(short h) { return __uint_as_float(((uint32_t)(uint16_t)h) << 16); }
 
-__global__ void k_ct(const uint8_t* bd, const uint32_t* sa, int64_t n, uint64_t* ct) {
+__global__ void k_ct(DevIndex ix, uint64_t* ct) {
+    const uint8_t* bd = ix.bd;
+    const int64_t n = ix.n;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t p = sa[i];
+        const int64_t p = saAt(ix, i);
         uint64_t k = 0;
         for (int t = 0; t < 16; ++t) k = (k << 3) | (p + t < n ? sym3(bd[p + t]) : 0);
         ct[i] = k;
@@ -488,7 +490,7 @@ __global__ void __launch_bounds__(256) k_exact(LixView m, DevIndex ix, const uin
             nPart = 0;
             for (int k = 0; r < m.n && m.ct[r] == xkey; ++r, ++k) {
                 if (k == kBlock) { nPart = -1; break; }
-                const int64_t pos = ix.sa[r];
+                const int64_t pos = saAt(ix, r);
                 const int64_t run = fwdMatch(ix.bd, pos, seqs, off + s0, len);
                 if (run >= part) nPart++;
                 if (run == len) { nFull++; fRow = r; fPos = pos; }
@@ -529,7 +531,7 @@ __global__ void __launch_bounds__(256) k_exact(LixView m, DevIndex ix, const uin
                 const int64_t mir = 2 * cs + cl - 1 - fPos - L;
                 SmemRec x;
                 x.from = 0; x.to = (int32_t)L;
-                x.fwd = fRow; x.rev = (int64_t)ix.isa[mir]; x.size = 1; x.logP = lb;
+                x.fwd = fRow; x.rev = isaAt(ix, mir); x.size = 1; x.logP = lb;
                 exact[rid] = x;
             }
         }
@@ -573,7 +575,7 @@ __device__ __forceinline__ bool windowOccurs(const LixView& m, const DevIndex& i
     const int64_t wl = c - a;
     for (int k = 0; r < m.n && m.ct[r] == key; ++r, ++k) {
         if (k == kBlock) return true;
-        const int64_t pos = ix.sa[r];
+        const int64_t pos = saAt(ix, r);
         if (pos + wl > ix.n) continue;
         if (fwdMatch(ix.bd, pos + 16, seqs, off + a + 16, wl - 16) == wl - 16) return true;
     }
@@ -814,7 +816,7 @@ extern "C" int msgs_lix_train(msgs_ctx* c, int32_t segments, int64_t samples, in
     LCK(hipMalloc(&L.dESegRoot, segments * sizeof(uint32_t)), "bounds");
     hipEvent_t e0 = c->x().ev0, e1 = c->x().ev1;
     LCK(hipEventRecord(e0, st), "event");
-    k_ct<<<gridFor(n), 256, 0, st>>>(c->db.dBd, c->db.dSa, n, L.dCt);
+    k_ct<<<gridFor(n), 256, 0, st>>>(c->db.view(), L.dCt);
     k_knots<<<gridFor(segments + 1), 256, 0, st>>>(L.dCt, n, segments, L.dKey, L.dRank);
     k_buckets<<<gridFor(((int64_t)1 << kBucketBits) + 1), 256, 0, st>>>(L.dKey, segments, L.dBucket);
     std::vector<float> W;

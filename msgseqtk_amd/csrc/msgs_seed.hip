@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(256) seed_kernel(DevIndex ix, SeedArgs a) {
             const int64_t len = m.to - m.from;
             for (int64_t i = 0; i < m.size; ++i) {
                 for (int side = 0; side < 2; ++side) {
-                    const int64_t bd = ix.sa[side == 0 ? m.fwd + i : m.rev + i];
+                    const int64_t bd = saAt(ix, side == 0 ? m.fwd + i : m.rev + i);
                     nSA++;
                     const int32_t tid = findTid(ix, bd);
                     const int64_t cs = ix.chromBD[tid], cl = ix.chromBD[tid + 1] - cs;

@@ -26,6 +26,7 @@ enum : int { ST_IDLE = 0, ST_FB1 = 1, ST_FB2 = 2, ST_BF1 = 3, ST_BF2 = 4, ST_WAI
 constexpr int kNoHand = -2147483647 - 1;
 enum : int { STG_A = 0, STG_C = 1, STG_D = 2 };
 
+template <class Row>
 struct Lane {
     /* read */
     int64_t rid;
@@ -41,11 +42,11 @@ struct Lane {
     int state;
     /* phase-1 SMEM */
     int cFrom, cTo;
-    int32_t cF, cR, cS, cX;   /* rows and sizes (indexes hold fewer than 2^31 rows) */
+    Row cF, cR, cS, cX;       /* rows and sizes (int32_t for narrow indexes, int64_t for wide) */
     double cLL;
     /* candidate list */
     int nCand, lo, k, w, nValid;
-    int32_t s0;
+    Row s0;
     /* output */
     int nOut;
     /* paired chains: role 1 = the pair's even lane (first search, then forward searches),
@@ -56,8 +57,10 @@ struct Lane {
     int64_t nExt, nFast, nLook;
 };
 
-__device__ __forceinline__ uint8_t base(const Lane& l, int i) { return i < l.L ? l.s[i] : (uint8_t)0; }
+template <class Row>
+__device__ __forceinline__ uint8_t base(const Lane<Row>& l, int i) { return i < l.L ? l.s[i] : (uint8_t)0; }
 
+template <class Row>
 struct Ctx {
     const DevIndex& ix;
     const SmemArgs& a;
@@ -73,7 +76,8 @@ __device__ __forceinline__ void queueRange(const SmemArgs& a, int64_t& base, int
     end = a.qPart == 1 ? split : a.nReads;
 }
 
-__device__ __forceinline__ void emit(const Ctx& c, Lane& l, int from, int to, int32_t F, int32_t R, int32_t S, double ll) {
+template <class Row>
+__device__ __forceinline__ void emit(const Ctx<Row>& c, Lane<Row>& l, int from, int to, Row F, Row R, Row S, double ll) {
     if (l.nOut < c.a.cap) {
         const size_t k = (size_t)l.rid * c.a.cap + l.nOut;
         SmemRec* o = (l.role == 2 ? c.a.outB : c.a.out) + k;
@@ -86,19 +90,26 @@ __device__ __forceinline__ void emit(const Ctx& c, Lane& l, int from, int to, in
 /* candidate k of a lane: interleaved over lanes, so that lanes at the same list position
  * touch neighbouring entries; of a lane group: contiguous, so that a chunk of G entries is
  * one coalesced access */
-__device__ __forceinline__ size_t cidx(const Ctx& c, int k) {
+template <class Row>
+__device__ __forceinline__ size_t cidx(const Ctx<Row>& c, int k) {
     return c.grouped ? (size_t)c.lane * c.a.candCap + k : (size_t)k * c.a.nLanes + c.lane;
 }
 
-__device__ __forceinline__ void candStore(const Ctx& c, int k, int pos, int32_t F, int32_t R, int32_t S, double ll,
-                                          int32_t X) {
-    SmemCand v;
+template <class Row>
+__device__ __forceinline__ SmemCandT<Row>& candAt(const Ctx<Row>& c, int k) {
+    return static_cast<SmemCandT<Row>*>(c.a.cand)[cidx(c, k)];
+}
+
+template <class Row>
+__device__ __forceinline__ void candStore(const Ctx<Row>& c, int k, int pos, Row F, Row R, Row S, double ll, Row X) {
+    SmemCandT<Row> v;
     v.F = F; v.R = R; v.S = S; v.X = X; v.LL = ll; v.pos = pos; v.pad = 0;
-    c.a.cand[cidx(c, k)] = v;
+    candAt(c, k) = v;
 }
 
 /* start findFwdBackSMEMS / findBackFwdSMEMS at f; false if it ends at once (ambiguous base) */
-__device__ __forceinline__ bool startSearch(const Ctx& c, Lane& l, int f, bool fb) {
+template <class Row>
+__device__ __forceinline__ bool startSearch(const Ctx<Row>& c, Lane<Row>& l, int f, bool fb) {
     l.sFrom = f;
     l.sTo = f + 1;
     uint8_t b = base(l, f);
@@ -118,7 +129,8 @@ __device__ __forceinline__ bool startSearch(const Ctx& c, Lane& l, int f, bool f
 
 /* the backward searches of findAllSMEMS alone (a pair's odd lane): while from > 0, search
  * back from --from; a search that ends at once leaves from unchanged */
-__device__ void pairBackChain(const Ctx& c, Lane& l) {
+template <class Row>
+__device__ void pairBackChain(const Ctx<Row>& c, Lane<Row>& l) {
     while (l.dFrom > 0) {
         --l.dFrom;
         l.tag++;
@@ -133,7 +145,8 @@ __device__ void pairBackChain(const Ctx& c, Lane& l) {
  * searches only on `from` (src/SMEM.cpp:196-203: from0 and to0 are copies), and the k-th of
  * each runs in the loop's k-th iteration; a pair runs the two chains on its two lanes and tags
  * each SMEM with k, so that the merge restores the loop's order. */
-__device__ void driverAfter(const Ctx& c, Lane& l) {
+template <class Row>
+__device__ void driverAfter(const Ctx<Row>& c, Lane<Row>& l) {
     if (l.role == 1) {
         if (l.stage == STG_A) { l.dFrom = l.sFrom; l.dTo = l.sTo; l.hand = l.dFrom; }
         else l.dTo = l.sTo;
@@ -178,14 +191,16 @@ __device__ void driverAfter(const Ctx& c, Lane& l) {
 }
 
 /* loglik of base code v: A/C/G/T from registers, other codes from the table */
-__device__ __forceinline__ double llOf(const Ctx& c, uint32_t v, double lA, double lC, double lG, double lT) {
+template <class Row>
+__device__ __forceinline__ double llOf(const Ctx<Row>& c, uint32_t v, double lA, double lC, double lG, double lT) {
     return v == 1 ? lA : (v == 2 ? lC : (v == 4 ? lG : (v == 8 ? lT : c.ix.ll[v])));
 }
 
 /* x0 + ll(s[i0]) + ll(s[i0+1]) + ... + ll(s[i1-1]) <= T, summed in that order; stops once
  * the running sum is <= T (every term is <= 0).  Bases are read 8 at a time from
  * aligned words (the read buffer is padded). */
-__device__ __forceinline__ bool gateSum(const Ctx& c, const Lane& l, double x, int i0, int i1, double T, double lA,
+template <class Row>
+__device__ __forceinline__ bool gateSum(const Ctx<Row>& c, const Lane<Row>& l, double x, int i0, int i1, double T, double lA,
                                         double lC, double lG, double lT) {
     const int64_t off = l.s - c.a.seqs;
     int64_t wb = -1;
@@ -205,7 +220,8 @@ __device__ __forceinline__ bool gateSum(const Ctx& c, const Lane& l, double x, i
  * T +- slack, slack = 1e-15 (L+2) (1 + |ll0| + |P[L]|), decides exactly; estimates inside
  * the band (and NaN estimates from -inf terms) take the sequential sum in the
  * reference's order. */
-__device__ __forceinline__ double gateSlack(const Lane& l, double ll0) {
+template <class Row>
+__device__ __forceinline__ double gateSlack(const Lane<Row>& l, double ll0) {
     return 1e-15 * (double)(l.L + 2) * (1.0 + fabs(ll0) + l.gPL);
 }
 
@@ -213,7 +229,8 @@ __device__ __forceinline__ double gateSlack(const Lane& l, double ll0) {
  * i.e. evalue(0, to) for a forward search ([0, from)) and evalue(from, L) for a backward
  * one ([to, L)) (src/SMEM.cpp:26-40, :81-82, :136-137); [i0, i1) is the running search's
  * fixed range, whose prefix-sum difference startSearch keeps in l.gD */
-__device__ __forceinline__ bool gatePush(const Ctx& c, const Lane& l, double ll0, int i0, int i1, double lA,
+template <class Row>
+__device__ __forceinline__ bool gatePush(const Ctx<Row>& c, const Lane<Row>& l, double ll0, int i0, int i1, double lA,
                                          double lC, double lG, double lT) {
     if (!c.a.gate1) return true;
     if (l.P) {
@@ -225,11 +242,13 @@ __device__ __forceinline__ bool gatePush(const Ctx& c, const Lane& l, double ll0
     x += ll0;
     return gateSum(c, l, x, i0, i1, c.a.T1, lA, lC, lG, lT);
 }
-__device__ __forceinline__ bool gateFwd(const Ctx& c, const Lane& l, double ll0, int from, double lA, double lC,
+template <class Row>
+__device__ __forceinline__ bool gateFwd(const Ctx<Row>& c, const Lane<Row>& l, double ll0, int from, double lA, double lC,
                                         double lG, double lT) {
     return gatePush(c, l, ll0, 0, from, lA, lC, lG, lT);
 }
-__device__ __forceinline__ bool gateBack(const Ctx& c, const Lane& l, double ll0, int to, double lA, double lC,
+template <class Row>
+__device__ __forceinline__ bool gateBack(const Ctx<Row>& c, const Lane<Row>& l, double ll0, int to, double lA, double lC,
                                          double lG, double lT) {
     return gatePush(c, l, ll0, to, l.L, lA, lC, lG, lT);
 }
@@ -248,7 +267,8 @@ constexpr int kRun = 16;
  * read and X - t > 0.  Text and read bytes come from three aligned 8-byte words each
  * (both buffers are padded past their ends).
  */
-__device__ __forceinline__ int matchRun(const DevIndex& ix, const SmemArgs& a, const Lane& l, int64_t X, int r0,
+template <class Row>
+__device__ __forceinline__ int matchRun(const DevIndex& ix, const SmemArgs& a, const Lane<Row>& l, int64_t X, int r0,
                                         bool fwd, double& LL, double lA, double lC, double lG, double lT) {
     if (X <= 0) return 0;
     const int64_t tb = X > kRun ? X - kRun : 0;
@@ -292,7 +312,8 @@ __device__ __forceinline__ uint64_t comp8(uint64_t v) {
 /* leading run (<= kRun) of matching steps, 8 bases per word: text byte X-1-t against
  * read base r0+t (forward, complemented) or r0-1-t (backward); same steps and LL sums as
  * matchRun.  Falls back to matchRun near the start of the text or of the read buffer. */
-__device__ __forceinline__ int matchRunW(const DevIndex& ix, const SmemArgs& a, const Lane& l, int64_t X, int r0,
+template <class Row>
+__device__ __forceinline__ int matchRunW(const DevIndex& ix, const SmemArgs& a, const Lane<Row>& l, int64_t X, int r0,
                                          bool fwd, double& LL, double lA, double lC, double lG, double lT) {
     const int64_t off = l.s - a.seqs;
     if (X < kRun || (!fwd && off + r0 < kRun)) return matchRun(ix, a, l, X, r0, fwd, LL, lA, lC, lG, lT);
@@ -328,7 +349,8 @@ __device__ __forceinline__ int matchRunW(const DevIndex& ix, const SmemArgs& a, 
     return m;
 }
 
-__device__ __forceinline__ void startIteration(Lane& l) {
+template <class Row>
+__device__ __forceinline__ void startIteration(Lane<Row>& l) {
     l.k = l.nCand - 1;
     l.w = l.nCand - 1;
     l.s0 = -1;
@@ -339,15 +361,17 @@ __device__ __forceinline__ void startIteration(Lane& l) {
  * reverse complement with the roles of p and q swapped (src/FMDIndex.h:226-228).  One
  * inlined extension serves both directions, so a wavefront whose lanes go different
  * ways runs the rank code once. */
-__device__ __forceinline__ void extendOne(const DevIndex& ix, bool fwd, int32_t& P, int32_t& Q, int32_t& S, uint8_t b) {
-    int32_t x = fwd ? Q : P, y = fwd ? P : Q;
-    dBackExt32(ix, x, y, S, fwd ? comp16(b) : b);
+template <class Row>
+__device__ __forceinline__ void extendOne(const DevIndex& ix, bool fwd, Row& P, Row& Q, Row& S, uint8_t b) {
+    Row x = fwd ? Q : P, y = fwd ? P : Q;
+    dBackExtT<Row>(ix, x, y, S, fwd ? comp16(b) : b);
     P = fwd ? y : x;
     Q = fwd ? x : y;
 }
 
 /* fetch reads until one has a running search; false when the queue is empty */
-__device__ bool nextRead(const Ctx& c, Lane& l) {
+template <class Row>
+__device__ bool nextRead(const Ctx<Row>& c, Lane<Row>& l) {
     while (true) {
         int64_t rid = (int64_t)atomicAdd(c.a.queue, 1ull) + c.qBase;
         if (rid >= c.qEnd) return false;
@@ -385,7 +409,8 @@ __device__ bool nextRead(const Ctx& c, Lane& l) {
 }
 
 /* a pair's odd lane takes read rid (fetched by its even lane) and waits for the hand-over */
-__device__ void pairTake(const Ctx& c, Lane& l, int64_t rid) {
+template <class Row>
+__device__ void pairTake(const Ctx<Row>& c, Lane<Row>& l, int64_t rid) {
     l.rid = rid;
     const int64_t off = c.a.offs[rid];
     l.s = c.a.seqs + off;
@@ -408,21 +433,22 @@ __device__ void pairTake(const Ctx& c, Lane& l, int64_t rid) {
  * back into a row (ISA[X]) only when the SMEM is pushed or emitted.  Values are
  * identical to the occ-block path; only the memory traffic differs.
  */
-template <bool FAST, bool PAIR>
+template <bool FAST, bool PAIR, class Row>
 __device__ __forceinline__ void smemBody(const DevIndex& ix, const SmemArgs& a) {
-    Lane l;
+    Lane<Row> l;
     l.nExt = 0; l.nFast = 0; l.nLook = 0;
     l.state = ST_IDLE;
     l.tag = 0;
     l.hand = kNoHand;
-    Ctx c{ix, a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, false, 0, 0};
+    Ctx<Row> c{ix, a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x, false, 0, 0};
     l.role = PAIR ? ((c.lane & 1) ? 2 : 1) : 0;
     queueRange(a, c.qBase, c.qEnd);
     const int64_t minLen = a.minLen;
     const double llA = ix.ll[1], llC = ix.ll[2], llG = ix.ll[4], llT = ix.ll[8];
     uint32_t nIter = 0;
     /* the candidate of the next phase-2 step, loaded one step ahead (valid when nxOk) */
-    int32_t nxPos = 0, nxF = 0, nxR = 0, nxS = 0, nxX = -1;
+    int nxPos = 0;
+    Row nxF = 0, nxR = 0, nxS = 0, nxX = -1;
     double nxLL = 0;
     bool nxOk = false;
     while (true) {
@@ -469,7 +495,7 @@ __device__ __forceinline__ void smemBody(const DevIndex& ix, const SmemArgs& a) 
             b = bp >= 0 ? base(l, bp) : (uint8_t)0;
         }
         int pos = 0;
-        int32_t P, Q, S, X;
+        Row P, Q, S, X;
         double LL;
         int mFrom, mTo;
         if (phase1) {
@@ -478,7 +504,7 @@ __device__ __forceinline__ void smemBody(const DevIndex& ix, const SmemArgs& a) 
             if (nxOk) {
                 pos = nxPos; P = nxF; Q = nxR; S = nxS; LL = nxLL; X = FAST ? nxX : -1;
             } else {
-                const SmemCand cd = a.cand[cidx(c, l.k)];
+                const SmemCandT<Row> cd = candAt(c, l.k);
                 pos = cd.pos; P = cd.F; Q = cd.R; S = cd.S; LL = cd.LL;
                 X = FAST ? cd.X : -1;
             }
@@ -489,14 +515,14 @@ __device__ __forceinline__ void smemBody(const DevIndex& ix, const SmemArgs& a) 
              * this pass stored it */
             nxOk = l.k > l.lo || l.w < l.nCand - 1;
             if (nxOk) {
-                const SmemCand cd = a.cand[cidx(c, l.k > l.lo ? l.k - 1 : l.nCand - 1)];
+                const SmemCandT<Row> cd = candAt(c, l.k > l.lo ? l.k - 1 : l.nCand - 1);
                 nxPos = cd.pos; nxF = cd.F; nxR = cd.R; nxS = cd.S; nxLL = cd.LL; nxX = cd.X;
             }
         }
         /* unique interval with no other candidate in play: run over matching text bytes;
          * each of those steps keeps S = 1 and so pushes, emits and drops nothing */
         if (FAST && S == 1 && (phase1 || (l.k == l.lo && l.lo == l.nCand - 1))) {
-            if (X < 0) { X = ix.sa[fwd ? Q : P]; l.nLook++; }
+            if (X < 0) { X = saRow<Row>(ix, fwd ? Q : P); l.nLook++; }
             const int m = matchRunW(ix, a, l, X, fwd ? mTo : mFrom, fwd, LL, llA, llC, llG, llT);
             if (m > 0) {
                 X -= m;
@@ -509,12 +535,12 @@ __device__ __forceinline__ void smemBody(const DevIndex& ix, const SmemArgs& a) 
             }
         }
         if (FAST) b = fwd ? base(l, mTo) : (mFrom > 0 ? base(l, mFrom - 1) : (uint8_t)0);
-        int32_t F0 = P, R0 = Q;
-        const int32_t S0 = S, X0 = X;
+        Row F0 = P, R0 = Q;
+        const Row S0 = S, X0 = X;
         const double LL0 = LL;
         /* ---- the extension (SMEM::fwdExt / SMEM::backExt, src/SMEM.h:140-177) ---- */
         if (FAST && S == 1) {
-            if (X < 0) { X = ix.sa[fwd ? Q : P]; l.nLook++; }
+            if (X < 0) { X = saRow<Row>(ix, fwd ? Q : P); l.nLook++; }
             const uint8_t bb = fwd ? comp16(b) : b;
             if (!amb(b) && X > 0 && ix.bd[X - 1] == bb) { X--; l.nFast++; }
             else S = 0;
@@ -528,7 +554,7 @@ __device__ __forceinline__ void smemBody(const DevIndex& ix, const SmemArgs& a) 
         /* the pre-step SMEM with its moving side turned back into a row */
         auto exact0 = [&]() {
             if (FAST && X0 >= 0) {
-                const int32_t v = (int32_t)ix.isa[X0];
+                const Row v = isaRow<Row>(ix, X0);
                 l.nLook++;
                 if (fwd) R0 = v; else F0 = v;
             }
@@ -546,7 +572,7 @@ __device__ __forceinline__ void smemBody(const DevIndex& ix, const SmemArgs& a) 
                 gatePush(c, l, LL0, fwd ? 0 : l.cTo, fwd ? l.cFrom : l.L, llA, llC, llG, llT)) {
                 exact0();
                 const int pp = fwd ? l.cTo : l.cFrom;
-                candStore(c, l.nCand++, pp, F0, R0, S0, LL0, -1);
+                candStore(c, l.nCand++, pp, F0, R0, S0, LL0, (Row)-1);
                 /* the last push heads the phase-2 list if phase 1 ends here */
                 nxPos = pp; nxF = F0; nxR = R0; nxS = S0; nxLL = LL0; nxX = -1;
                 nxOk = true;
@@ -612,20 +638,24 @@ __device__ __forceinline__ void smemBody(const DevIndex& ix, const SmemArgs& a) 
 }
 
 
-template <bool FAST>
+template <bool FAST, class Row>
 __global__ void __launch_bounds__(256, 4) smem_kernel_t(DevIndex ix, SmemArgs a) {
-    smemBody<FAST, false>(ix, a);
+    smemBody<FAST, false, Row>(ix, a);
 }
 
-template <bool FAST>
+template <bool FAST, class Row>
 __global__ void __launch_bounds__(256, 4) smem_pair_kernel(DevIndex ix, SmemArgs a) {
-    smemBody<FAST, true>(ix, a);
+    smemBody<FAST, true, Row>(ix, a);
 }
 
-template __global__ void smem_kernel_t<false>(DevIndex, SmemArgs);
-template __global__ void smem_kernel_t<true>(DevIndex, SmemArgs);
-template __global__ void smem_pair_kernel<false>(DevIndex, SmemArgs);
-template __global__ void smem_pair_kernel<true>(DevIndex, SmemArgs);
+template __global__ void smem_kernel_t<false, int32_t>(DevIndex, SmemArgs);
+template __global__ void smem_kernel_t<true, int32_t>(DevIndex, SmemArgs);
+template __global__ void smem_pair_kernel<false, int32_t>(DevIndex, SmemArgs);
+template __global__ void smem_pair_kernel<true, int32_t>(DevIndex, SmemArgs);
+template __global__ void smem_kernel_t<false, int64_t>(DevIndex, SmemArgs);
+template __global__ void smem_kernel_t<true, int64_t>(DevIndex, SmemArgs);
+template __global__ void smem_pair_kernel<false, int64_t>(DevIndex, SmemArgs);
+template __global__ void smem_pair_kernel<true, int64_t>(DevIndex, SmemArgs);
 
 /* one read per lane: the even lane's list (tags non-decreasing) and the odd lane's list (tags
  * from 1, non-decreasing) merged by tag, the even lane's SMEMs first on equal tags: the first
@@ -710,8 +740,8 @@ __device__ __forceinline__ int64_t shfl64(int64_t v, int src) {
 
 /* the group leader takes reads from the queue until one has a running search; the group
  * follows with the same values (false when the queue is empty) */
-template <int G>
-__device__ bool nextReadGroup(const Ctx& c, Lane& l, int gbase, int gl) {
+template <int G, class Row>
+__device__ bool nextReadGroup(const Ctx<Row>& c, Lane<Row>& l, int gbase, int gl) {
     while (true) {
         int64_t rid = 0;
         if (gl == 0) rid = (int64_t)atomicAdd(c.a.queue, 1ull) + c.qBase;
@@ -754,6 +784,15 @@ __device__ __forceinline__ int64_t permute64(int dst, int64_t v) {
 __device__ __forceinline__ double permuteF64(int dst, double v) {
     return __longlong_as_double(permute64(dst, __double_as_longlong(v)));
 }
+/* rows and sizes of either width */
+template <class Row> __device__ __forceinline__ Row permuteRow(int dst, Row v) {
+    if constexpr (sizeof(Row) == 8) return permute64(dst, v);
+    else return permute32(dst, v);
+}
+template <class Row> __device__ __forceinline__ Row shflRow(Row v, int src) {
+    if constexpr (sizeof(Row) == 8) return shfl64(v, src);
+    else return __shfl(v, src, 64);
+}
 
 /*
  * The SMEM search with one read per group of G lanes (64 / G reads per wavefront).  The
@@ -772,41 +811,41 @@ __device__ __forceinline__ double permuteF64(int dst, double v) {
  * per step, each chunk continuing from the last S' of the previous one.  A lone unique
  * candidate walks the text as in the per-lane kernel.  Output equals smem_kernel_t's.
  */
-template <int G>
+template <int G, class Row>
 __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a) {
     static_assert(G == 4 || G == 8 || G == 16 || G == 32, "group size");
     const int lane = threadIdx.x & 63;
     const int gl = lane % G, gbase = lane - gl;
     const uint64_t gmask = ((1ull << G) - 1) << gbase;
     const uint64_t below = (1ull << lane) - 1;
-    Lane l;
+    Lane<Row> l;
     l.nExt = 0; l.nFast = 0; l.nLook = 0;
     l.state = ST_IDLE;
     l.role = 0; l.tag = 0; l.hand = kNoHand;
     l.nCand = 0; l.lo = 0; l.k = 0; l.w = 0; l.s0 = -1; l.nValid = 0;
     /* this lane's list entry in register mode (l.nValid == 1) or its phase-1 push */
     int rPos = 0;
-    int32_t rP = 0, rQ = 0, rS = 0, rX = -1;
+    Row rP = 0, rQ = 0, rS = 0, rX = -1;
     double rLL = 0;
-    Ctx c{ix, a, ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G, true, 0, 0};
+    Ctx<Row> c{ix, a, ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G, true, 0, 0};
     queueRange(a, c.qBase, c.qEnd);
     const int64_t minLen = a.minLen;
     const double llA = ix.ll[1], llC = ix.ll[2], llG = ix.ll[4], llT = ix.ll[8];
     uint32_t nIter = 0;
     while (true) {
         if (l.state == ST_IDLE) {
-            if (!nextReadGroup<G>(c, l, gbase, gl)) break;
+            if (!nextReadGroup<G, Row>(c, l, gbase, gl)) break;
         }
         ++nIter;
         const int st = l.state;
         if (st == ST_FB1 || st == ST_BF1) {
             /* ---- phase 1: one extension, alike in every lane of the group ---- */
-            int32_t P = l.cF, Q = l.cR, S = l.cS, X = l.cX;
+            Row P = l.cF, Q = l.cR, S = l.cS, X = l.cX;
             double LL = l.cLL;
             int mFrom = l.cFrom, mTo = l.cTo;
             const bool fwd = st == ST_FB1;
             if (S == 1) {
-                if (X < 0) { X = ix.sa[fwd ? Q : P]; if (gl == 0) l.nLook++; }
+                if (X < 0) { X = saRow<Row>(ix, fwd ? Q : P); if (gl == 0) l.nLook++; }
                 const int m = matchRunW(ix, a, l, X, fwd ? mTo : mFrom, fwd, LL, llA, llC, llG, llT);
                 if (m > 0) {
                     X -= m;
@@ -817,11 +856,11 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
                 }
             }
             const uint8_t b = fwd ? base(l, mTo) : (mFrom > 0 ? base(l, mFrom - 1) : (uint8_t)0);
-            int32_t F0 = P, R0 = Q;
-            const int32_t S0 = S, X0 = X;
+            Row F0 = P, R0 = Q;
+            const Row S0 = S, X0 = X;
             const double LL0 = LL;
             if (S == 1) {
-                if (X < 0) { X = ix.sa[fwd ? Q : P]; if (gl == 0) l.nLook++; }
+                if (X < 0) { X = saRow<Row>(ix, fwd ? Q : P); if (gl == 0) l.nLook++; }
                 const uint8_t bb = fwd ? comp16(b) : b;
                 if (!amb(b) && X > 0 && ix.bd[X - 1] == bb) { X--; if (gl == 0) l.nFast++; }
                 else S = 0;
@@ -837,7 +876,7 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
                                      gateBack(c, l, LL0, l.cTo, llA, llC, llG, llT));
             if (push) {
                 if (X0 >= 0) {
-                    const int32_t v = (int32_t)ix.isa[X0];
+                    const Row v = isaRow<Row>(ix, X0);
                     if (gl == 0) l.nLook++;
                     if (fwd) R0 = v; else F0 = v;
                 }
@@ -845,7 +884,7 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
                 if (l.nCand < G) {
                     if (gl == l.nCand) { rPos = pp; rP = F0; rQ = R0; rS = S0; rX = -1; rLL = LL0; }
                 } else if (gl == 0) {
-                    candStore(c, l.nCand, pp, F0, R0, S0, LL0, -1);
+                    candStore(c, l.nCand, pp, F0, R0, S0, LL0, (Row)-1);
                 }
                 l.nCand++;
             }
@@ -864,7 +903,7 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
                         /* registers: lane i takes push nCand-1-i (list order) */
                         const int src = gbase + max(0, l.nCand - 1 - gl);
                         rPos = __shfl(rPos, src, 64);
-                        rP = __shfl(rP, src, 64); rQ = __shfl(rQ, src, 64); rS = __shfl(rS, src, 64); rX = __shfl(rX, src, 64);
+                        rP = shflRow(rP, src); rQ = shflRow(rQ, src); rS = shflRow(rS, src); rX = shflRow(rX, src);
                         rLL = __longlong_as_double(shfl64(__double_as_longlong(rLL), src));
                         l.nValid = 1;
                     } else {
@@ -882,19 +921,19 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
             const int i = l.k + gl;
             const bool act = i < l.nCand;
             int pos = 0;
-            int32_t P = 0, Q = 0, S = 0, X = -1;
+            Row P = 0, Q = 0, S = 0, X = -1;
             double LL = 0;
             if (regs) {
                 if (act) { pos = rPos; P = rP; Q = rQ; S = rS; X = rX; LL = rLL; }
             } else if (act) {
-                const SmemCand cd = a.cand[cidx(c, l.lo - i)];
+                const SmemCandT<Row> cd = candAt(c, l.lo - i);
                 pos = cd.pos; P = cd.F; Q = cd.R; S = cd.S; X = cd.X; LL = cd.LL;
             }
             /* a lone unique candidate walks the text; the whole group follows its boundary */
             if (l.nCand == 1 && l.k == 0) {
                 int m = 0;
                 if (gl == 0 && S == 1) {
-                    if (X < 0) { X = ix.sa[fwd ? Q : P]; l.nLook++; }
+                    if (X < 0) { X = saRow<Row>(ix, fwd ? Q : P); l.nLook++; }
                     m = matchRunW(ix, a, l, X, fwd ? l.sTo : l.sFrom, fwd, LL, llA, llC, llG, llT);
                     X -= m;
                     l.nFast += m;
@@ -904,12 +943,12 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
             }
             int mFrom = fb ? l.sFrom : pos, mTo = fb ? pos : l.sTo;
             const uint8_t b = fwd ? base(l, mTo) : (mFrom > 0 ? base(l, mFrom - 1) : (uint8_t)0);
-            int32_t F0 = P, R0 = Q;
-            const int32_t S0 = S, X0 = X;
+            Row F0 = P, R0 = Q;
+            const Row S0 = S, X0 = X;
             const double LL0 = LL;
             if (act) {
                 if (S == 1) {
-                    if (X < 0) { X = ix.sa[fwd ? Q : P]; l.nLook++; }
+                    if (X < 0) { X = saRow<Row>(ix, fwd ? Q : P); l.nLook++; }
                     const uint8_t bb = fwd ? comp16(b) : b;
                     if (!amb(b) && X > 0 && ix.bd[X - 1] == bb) { X--; l.nFast++; }
                     else S = 0;
@@ -922,7 +961,7 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
                 else { if (mFrom > 0) LL += llOf(c, b, llA, llC, llG, llT); }
             }
             /* S' of the previous list entry */
-            const int32_t up = __shfl(S, gl == 0 ? lane : lane - 1, 64);
+            const Row up = shflRow(S, gl == 0 ? lane : lane - 1);
             const int64_t prev = gl == 0 ? (int64_t)l.s0 : up;
             const int len0 = fb ? (pos - l.sFrom) : (l.sTo - pos);
             const bool keep = act && S >= 1 && S != prev;
@@ -930,7 +969,7 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
             const uint64_t kMask = __ballot(keep) & gmask, eMask = __ballot(em) & gmask;
             if (em) {
                 if (X0 >= 0) {
-                    const int32_t v = (int32_t)ix.isa[X0];
+                    const Row v = isaRow<Row>(ix, X0);
                     l.nLook++;
                     if (fwd) R0 = v; else F0 = v;
                 }
@@ -947,7 +986,7 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
                 const int rank = keep ? __popcll(kMask & below) : nKeep + __popcll(~kMask & gmask & below);
                 const int dst = gbase + rank;
                 rPos = permute32(dst, pos);
-                rP = permute32(dst, P); rQ = permute32(dst, Q); rS = permute32(dst, S); rX = permute32(dst, X);
+                rP = permuteRow(dst, P); rQ = permuteRow(dst, Q); rS = permuteRow(dst, S); rX = permuteRow(dst, X);
                 rLL = permuteF64(dst, LL);
             } else if (keep) {
                 candStore(c, l.lo - (l.w + __popcll(kMask & below)), pos, P, Q, S, LL, X);
@@ -955,7 +994,7 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
             l.nOut += __popcll(eMask);
             l.w += nKeep;
             const int last = min(G, l.nCand - l.k) - 1;
-            l.s0 = __shfl(S, gbase + last, 64);
+            l.s0 = shflRow(S, gbase + last);
             l.k += G;
             if (l.k >= l.nCand) {
                 l.nCand = l.w;
@@ -968,7 +1007,7 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
                     if (!regs && l.nCand <= G) {
                         /* the list now fits the group: load it once */
                         if (gl < l.nCand) {
-                            const SmemCand cd = a.cand[cidx(c, l.lo - gl)];
+                            const SmemCandT<Row> cd = candAt(c, l.lo - gl);
                             rPos = cd.pos; rP = cd.F; rQ = cd.R; rS = cd.S; rX = cd.X; rLL = cd.LL;
                         }
                         l.nValid = 1;
@@ -994,9 +1033,12 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
     }
 }
 
-template __global__ void smem_group_kernel<4>(DevIndex, SmemArgs);
-template __global__ void smem_group_kernel<8>(DevIndex, SmemArgs);
-template __global__ void smem_group_kernel<16>(DevIndex, SmemArgs);
+template __global__ void smem_group_kernel<4, int32_t>(DevIndex, SmemArgs);
+template __global__ void smem_group_kernel<8, int32_t>(DevIndex, SmemArgs);
+template __global__ void smem_group_kernel<16, int32_t>(DevIndex, SmemArgs);
+template __global__ void smem_group_kernel<4, int64_t>(DevIndex, SmemArgs);
+template __global__ void smem_group_kernel<8, int64_t>(DevIndex, SmemArgs);
+template __global__ void smem_group_kernel<16, int64_t>(DevIndex, SmemArgs);
 
 /*
  * SMEM_LIST::findMEMS (src/SMEM.cpp:174-184) with SMEM::findMEM (src/SMEM.cpp:42-62), one
@@ -1010,6 +1052,7 @@ template __global__ void smem_group_kernel<16>(DevIndex, SmemArgs);
  * read), and from is unchanged.  An extension by any base keeps (p, q) when the new size
  * is 0 (FMDIndex::backExt, src/FMDIndex.cpp:577-610).
  */
+template <class Row>
 __global__ void __launch_bounds__(256) mem_kernel(DevIndex ix, SmemArgs a, double* listLL) {
     for (int64_t rid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; rid < a.nReads;
          rid += (int64_t)gridDim.x * blockDim.x) {
@@ -1020,7 +1063,7 @@ __global__ void __launch_bounds__(256) mem_kernel(DevIndex ix, SmemArgs a, doubl
         double sum = 0;
         for (int from = 0, to = 1; from < L; from = to + 1) {
             const uint8_t b = s[from];
-            int32_t P = (int32_t)ix.C[b], Q = (int32_t)ix.C[comp16(b)], S = (int32_t)(ix.C[b + 1] - ix.C[b]);
+            Row P = (Row)ix.C[b], Q = (Row)ix.C[comp16(b)], S = (Row)(ix.C[b + 1] - ix.C[b]);
             double LL = 0;
             LL += ix.ll[b];
             int mTo = from + 1, to0 = mTo;
@@ -1046,3 +1089,6 @@ __global__ void __launch_bounds__(256) mem_kernel(DevIndex ix, SmemArgs a, doubl
         listLL[rid] = sum;
     }
 }
+
+template __global__ void mem_kernel<int32_t>(DevIndex, SmemArgs, double*);
+template __global__ void mem_kernel<int64_t>(DevIndex, SmemArgs, double*);

@@ -131,6 +131,12 @@ def test_smem_db_with_n():
     bd1, bwt1, sa1 = odb.arrays()
     bd2, bwt2, sa2 = m.arrays()
     assert np.array_equal(sa1, sa2) and np.array_equal(bwt1, bwt2)
+    # rank of every code, N and never-occurring IUPAC codes included
+    pos = np.arange(-1, m.n + 1, dtype=np.int64)
+    for b in range(16):
+        got = m.rank(np.full(len(pos), b, np.uint8), pos)
+        exp = np.array([O.orc().orc_rank(odb.h, b, int(i)) for i in pos])
+        assert np.array_equal(got, exp), b
     seqs = [synth.encode("ACGTAGTA"), synth.encode("ACGTCGTAGTACTACGNACG"), synth.encode("TACGNAC")]
     _check_smems(odb, m, seqs, 1, float("inf"))
     _check_smems(odb, m, seqs, 4, 10.0)

@@ -63,3 +63,14 @@ def test_noalloc_matches_smem_batch_twice():
     # a second, smaller call (shorter reads, no e-value limit) on the same scratch
     small = [r[:90] for r in reads[:500]]
     check(m, small, M.default_params(maxEvalue=float("inf"), minLen=12))
+
+
+def test_noalloc_wide_index_twice():
+    """the same on a database with the wide index layout (64-bit rows: larger candidate
+    records in the scratch)"""
+    chroms, names = synth.metagenome(4, 50000, gc=(0.3, 0.7), seed=43, strain_frac=0.25)
+    reads = synth.simulate_reads(chroms, 2000, (40, 250), sub=0.01, indel=0.001, seed=44)[0]
+    m = M.Mapper(0).set("wide_index", 1).build(chroms, names)
+    assert m.get("index_wide") == 1
+    check(m, reads, M.default_params())
+    check(m, [r[:70] for r in reads[:300]], M.default_params(minLen=12))
