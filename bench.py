@@ -45,7 +45,12 @@ WORKLOADS = {
                "at 1-5% divergence; 15 batches of 256k reads resident in HBM",
     "pairs": "PE 2x150: 64 genomes x 4 Mb (256 Mbp, GC U(0.3,0.7)), insert N(350,50), 0.5% sub, 0.05% indel, "
              "5% off-target pairs; 1M pairs in 256k-read batches resident in HBM",
+    "large": "PE 2x150 at 1% error: 1000 genomes log-uniform 1-8 Mb (3.40 Gbp, BD text 6.80e9 symbols, wide "
+             "index layout), GC U(0.3,0.7), 30% near-duplicate strains at 0.5-5% divergence, insert N(400,80), "
+             "10% off-target pairs; 15 batches of 128k pairs (256k reads) resident in HBM",
 }
+# insert-size model of each paired workload (mean, sd): msgseqtk-align -m/-d
+INSERT = {"pairs": (350, 50), "large": (400, 80)}
 
 
 def log(*a):
@@ -63,6 +68,9 @@ def parse():
     ap.add_argument("--cpu-reads", type=int, default=20000, help="reads in the CPU-baseline sample")
     ap.add_argument("--cpu-runs", type=int, default=3, help="CPU-baseline runs (the median is reported)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-kind", choices=("both", "port"), default="",
+                    help="CPU baselines: the reference's seeding + the oracle ('both'), or the oracle alone "
+                         "(default: both, except on the large workload)")
     ap.add_argument("--no-clean", action="store_true", help="skip the msgseqtk-clean seeding measurement")
     ap.add_argument("--no-compare", action="store_true", help="skip the plain-FMD comparison run")
     ap.add_argument("--no-tune", action="store_true", help="keep the default lanes per CU")
@@ -99,18 +107,26 @@ def make_data(a, synth):
         batches = [(synth.simulate_reads_fast(chroms, per, 150, sub=0.009, indel=0.001, seed=0xC0FFEE + 1004 + b,
                                               prefix="b%dr" % b),) for b in range(nb)]
         return chroms, names, batches, False
-    chroms, names = synth.config_genomes(2)
-    nreads = a.reads or 2_000_000
+    if a.workload == "large":
+        chroms, names = synth.config_genomes(5)
+        nreads = a.reads or 20_000_000
+        err, off, seed = (0.009, 0.001), 0.1, 0xC0FFEE + 1005
+    else:
+        chroms, names = synth.config_genomes(2)
+        nreads = a.reads or 2_000_000
+        err, off, seed = (0.005, 0.0005), 0.05, 0xC0FFEE + 1002
     per = a.batch_reads // 2
     nb = max(1, min((nreads // 2 + per - 1) // per, 15))
-    batches = [synth.simulate_pairs_fast(chroms, per, 150, insert=(350, 50), sub=0.005, indel=0.0005,
-                                         seed=0xC0FFEE + 1002 + b, offtarget=0.05, prefix="b%dp" % b)
+    batches = [synth.simulate_pairs_fast(chroms, per, 150, insert=INSERT[a.workload], sub=err[0], indel=err[1],
+                                         seed=seed + b, offtarget=off, prefix="b%dp" % b)
                for b in range(nb)]
     return chroms, names, batches, True
 
 
 def main():
     a = parse()
+    if not a.cpu_kind:
+        a.cpu_kind = "port" if a.workload == "large" else "both"
     if a.gpus != 1:
         print("bench.py runs on one GPU only: the mapping path shards by read batch with a replicated "
               "index, and this build measures the single-GPU kernel; use --gpus 1", file=sys.stderr)
@@ -137,7 +153,8 @@ def main():
     for b, bt in enumerate(batches):
         m.upload(b, *bt)
     if paired:
-        p = M.default_params(insMean=350, insSD=50, insMin=350 - 10 * 50, insMax=350 + 10 * 50)
+        mu, sd = INSERT[a.workload]
+        p = M.default_params(insMean=mu, insSD=sd, insMin=mu - 10 * sd, insMax=mu + 10 * sd)
     else:
         p = M.default_params()
     sync = lambda: None   # msgs_batch_wait returns after the batch's stream has drained
@@ -291,7 +308,8 @@ def main():
         out["roofline"]["frac_of_gather_ceiling"] = round(achieved / g["gbs"], 4)
         log("random 64-B gather ceiling: %s" % g)
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(m, chroms, names, batches[0], p, paired, a.cpu_reads, a.cpu_runs)
+        out["cpu_baseline"] = cpu_baseline(m, chroms, names, batches[0], p, INSERT.get(a.workload), a.cpu_reads,
+                                           a.cpu_runs, a.cpu_kind)
     if not a.no_clean:
         out["clean_seeding"] = clean_seeding(m, batches[0][0][0])
         log("clean seeding: %s" % out["clean_seeding"])
@@ -377,12 +395,14 @@ def clean_seeding(m, seqs):
             "timing": "host clock, reads packed and uploaded and MEM records copied back inside"}
 
 
-def cpu_baseline(m, chroms, names, batch, p, paired, nreads, runs):
+def cpu_baseline(m, chroms, names, batch, p, insert, nreads, runs, kinds):
     """The CPU baseline over the first reads (pairs) of batch 0, OpenMP over all host cores
     (at most 16), median of `runs`: the reference's own seeding and chaining (oracle/_ref) with
     the oracle's DP and records ("reference"), and the oracle restatement with a plain occ table
     ("port").  Both SAM texts are compared with the GPU's; the oracle's count of FMD extensions
-    on the sample is compared with the plain-form kernel's counter."""
+    on the sample is compared with the plain-form kernel's counter.  insert: the paired
+    workload's (mean, sd), None for single-end reads; kinds: "both" or "port" (the oracle
+    alone: the reference's RRR wavelet tree over a 6.8e9-symbol text takes minutes to build)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     import msgseqtk_amd as M
@@ -392,8 +412,9 @@ def cpu_baseline(m, chroms, names, batch, p, paired, nreads, runs):
     t = time.time()
     odb = O.OracleDb(chroms, names, sa=sa)
     log("oracle index from the device suffix array in %.1fs" % (time.time() - t))
+    paired = insert is not None
     rix = None
-    if O.ref_available():
+    if kinds == "both" and O.ref_available():
         t = time.time()
         rix = O.RefIndex(bd, sa)
         log("reference FMDIndex (RRR wavelet tree, sampled SA) in %.1fs" % (time.time() - t))
@@ -403,7 +424,7 @@ def cpu_baseline(m, chroms, names, batch, p, paired, nreads, runs):
     k = min(nreads // (2 if paired else 1), len(reads[0]))
     r1 = tuple(x[:k] for x in reads)
     r2 = tuple(x[:k] for x in mates) if mates else None
-    po = O.pe_params(350, 50) if paired else O.default_params()
+    po = O.pe_params(*insert) if paired else O.default_params()
     sam_gpu = m.map(r1, p, mates=r2)
     nr = k * (2 if paired else 1)
     res = {}

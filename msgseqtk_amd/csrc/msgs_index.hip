@@ -221,12 +221,13 @@ __global__ void k_isa(const uint32_t* sa, int64_t n, uint32_t* isa) {
     isa[sa[k]] = (uint32_t)k;
 }
 
+/* grid-stride: a launch covers at most 2^32 work-items, fewer than a wide text's rows */
 template <class T>
 __global__ void k_bwt(const uint8_t* bd, const T* sa, int64_t n, uint8_t* bwt) {
-    int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const int64_t s = (int64_t)sa[k];
-    bwt[k] = s == 0 ? 0 : bd[s - 1];
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = (int64_t)sa[k];
+        bwt[k] = s == 0 ? 0 : bd[s - 1];
+    }
 }
 
 /* one thread per 128-position block: code bit planes, special mask, local base counts */
@@ -490,8 +491,9 @@ int buildIndexDevice(DbState& db, hipStream_t st, std::string& err, int64_t rang
     Freer tmp;
     hipError_t e;
     HCHK(hipMalloc(&db.dBwt, n));
-    if (db.wide) k_bwt<int64_t><<<grid(n), 256, 0, st>>>(db.dBd, (const int64_t*)db.dSa, n, db.dBwt);
-    else k_bwt<uint32_t><<<grid(n), 256, 0, st>>>(db.dBd, (const uint32_t*)db.dSa, n, db.dBwt);
+    const unsigned gb = std::min(grid(n), 1u << 18);
+    if (db.wide) k_bwt<int64_t><<<gb, 256, 0, st>>>(db.dBd, (const int64_t*)db.dSa, n, db.dBwt);
+    else k_bwt<uint32_t><<<gb, 256, 0, st>>>(db.dBd, (const uint32_t*)db.dSa, n, db.dBwt);
     HCHK(hipGetLastError());
 
     const int64_t nBlk = n / 128 + 1;
