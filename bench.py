@@ -2,20 +2,22 @@
 """Mapped reads/s of the MI355X mapping path, learned-index form against the plain FMD form,
 with bit-exact SAM.
 
-Default workload (the learned-index configuration): 256 genomes of 4 Mb (1 Gbp, GC ~
-U(0.3, 0.7)), 20% of them near-duplicate strains at 1-5% divergence from a parent; 4M
-single-end 150 bp reads at 1% error (0.9% substitutions, 0.1% 1-3 bp indels), Phred U(20, 40),
-in 15 resident batches of 256k reads (3.9M reads).  `--workload pairs` runs the paired
-configuration instead (64 genomes of 4 Mb, 1M pairs 2x150, insert N(350, 50), 5% off-target
-pairs).
+Default workload (the largest configuration, combined form): 1000 genomes with sizes
+log-uniform in 1-8 Mb (3.40 Gbp; BD text 6.80e9 symbols, so the index takes the wide layout),
+GC ~ U(0.3, 0.7), 30% near-duplicate strains at 0.5-5% divergence; read pairs 2x150 at 1% error
+(0.9% substitutions, 0.1% 1-3 bp indels), insert N(400, 80), 10% off-target pairs, Phred
+U(20, 40), in 15 resident batches of 128k pairs (3.9M reads).  `--workload strains` runs the
+learned-index configuration (256 genomes of 4 Mb with 20% strains, single-end 150 bp reads),
+`--workload pairs` the 64 x 4 Mb paired configuration.
 
-A step maps one batch end to end (SMEMs, seeds, chains, DP + backtrace on the GPU; MAPQ,
-pairing and SAM records on the host) with the batch's reads already in HBM; batches are
-pipelined (several in flight on the worker streams).  Prints one JSON line.  The CPU baseline
-is the reference's own seeding and chaining code (RRR wavelet-tree ranks, LF-walk suffix-array
-lookups; compiled from its sources into oracle/_ref) with the oracle's DP and records, timed
-on the host cores over a bounded sample of the same reads (median of 3 runs); the oracle
-restatement with a plain occ table is reported beside it.
+A step maps one batch end to end (SMEMs, seeds, chains, DP + backtrace and pairing on the GPU;
+MAPQ and SAM records on the host) with the batch's reads already in HBM; batches are pipelined
+(several in flight on the worker streams).  Prints one JSON line.  The CPU baseline is timed on
+the host cores over a bounded sample of the same reads (median of 3 runs): the reference's own
+seeding and chaining code (RRR wavelet-tree ranks, LF-walk suffix-array lookups; compiled from
+its sources into oracle/_ref) with the oracle's DP and records, and the oracle restatement with
+a plain occ table beside it; on the large workload the oracle restatement alone (building the
+reference's RRR wavelet tree over 6.8e9 symbols takes minutes).
 """
 import argparse
 import collections
@@ -62,8 +64,9 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=24)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="strains")
-    ap.add_argument("--reads", type=int, default=0, help="reads of the workload (0: 4M single-end / 2M paired)")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="large")
+    ap.add_argument("--reads", type=int, default=0,
+                    help="reads of the workload (0: 4M single-end / 2M paired / 20M paired for large)")
     ap.add_argument("--batch-reads", type=int, default=262144)
     ap.add_argument("--cpu-reads", type=int, default=20000, help="reads in the CPU-baseline sample")
     ap.add_argument("--cpu-runs", type=int, default=3, help="CPU-baseline runs (the median is reported)")

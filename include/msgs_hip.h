@@ -86,7 +86,14 @@ const char* msgs_last_error(msgs_ctx* ctx);
  * bi-intervals, occ-block counts relative to superblocks) only for BD texts of 2^31 symbols or
  * more (default), 1 = for every database, with superblocks of 2^14 positions and suffix-sorting
  * ranges of 2^16 suffixes, so that small texts run the wide code paths; msgs_ctx_get
- * "index_wide" tells which layout the resident database has.  Results never depend
+ * "index_wide" tells which layout the resident database has; "build_block_mb" (read when a
+ * database is built or loaded) msgseqtk-build's -b: a database whose BD text has this many
+ * MB (2^20 symbols) or more gets the suffix-array order of msgseqtk-build's block-by-block
+ * build, whole chromosomes in blocks below that size merged with FMDIndex::prepend
+ * (src/msgseqtk-build.cpp:440-467, src/FMDIndex.cpp:267-308, 612-630), so that row numbers
+ * and DB.fmd match a database the reference built with the same -b (default 2048, the
+ * reference's; 0 = one block whatever the size); msgs_ctx_get "index_blocks" tells how many
+ * blocks the resident database was ordered by.  Results never depend
  * on these settings, except "record_format": the records msgs_map_batch and msgs_batch_sam
  * return are SAM text lines (0, default) or the same records in the BAM layout (1): for each,
  * what htslib's bam_write1 puts in the BGZF stream (int32 block_size, the 32-byte core, read name,

@@ -343,7 +343,31 @@ int msgs_db_build(msgs_ctx* c, const uint8_t* seqs, const int64_t* lens, int32_t
     if (hipMemcpyAsync(db.dBd, bd.data(), total, hipMemcpyHostToDevice, st) != hipSuccess)
         return fail(c, MSGS_E_HIP, "BD upload");
     std::string err;
-    db.wide = c->wideIndex != 0 || total > (int64_t)INT32_MAX;
+    /* msgseqtk-build's block-by-block build (src/msgseqtk-build.cpp:403-415, 440-467): a BD
+     * text of blockSize symbols or more is indexed in runs of whole chromosomes taken from the
+     * last one, each run closed when one more chromosome would reach blockSize, and the run
+     * indexes merged; the suffix order follows that build (msgs_index.hip) */
+    db.blockEnds.clear();
+    const int64_t blockSize = c->buildBlockMb * ((int64_t)1 << 20);
+    if (blockSize > 0 && total >= blockSize) {
+        std::vector<int64_t> bdLen(nChrom);
+        for (int32_t t = 0; t < nChrom; ++t) bdLen[t] = 2 * (lens[t] + 1);
+        auto span = [&](int64_t a, int64_t b) { return db.chromBD[b] - db.chromBD[a]; };   /* chroms [a, b) */
+        std::vector<int64_t> starts;   /* first chrom of each block, last block first */
+        int64_t blockEnd = nChrom;
+        for (int64_t i = nChrom; i > 0; --i) {
+            const int64_t blockStart = i - 1;
+            if (blockStart == 0 || (span(blockStart, blockEnd) < blockSize && span(blockStart - 1, blockEnd) >= blockSize)) {
+                starts.push_back(blockStart);
+                blockEnd = blockStart;
+            }
+        }
+        std::reverse(starts.begin(), starts.end());
+        for (size_t b = 0; b < starts.size(); ++b)
+            db.blockEnds.push_back(b + 1 < starts.size() ? db.chromBD[starts[b + 1]] : total);
+        if (db.blockEnds.size() == 1) db.blockEnds.clear();
+    }
+    db.wide = c->wideIndex != 0 || total > (int64_t)INT32_MAX || !db.blockEnds.empty();
     /* wide_index = 1 puts superblock and suffix-sorting range boundaries into small texts */
     db.supShift = c->wideIndex ? 14 : 31;
     const int64_t rangeMax = c->wideIndex ? (int64_t)1 << 16 : (int64_t)1 << 29;
@@ -966,6 +990,7 @@ extern "C" int msgs_ctx_set(msgs_ctx* c, const char* key, int64_t value) {
     else if (k == "record_format") { if (value != 0 && value != 1) return fail(c, MSGS_E_ARG, "record_format: 0 or 1"); c->recordFormat = (int)value; }
     else if (k == "dp_serial") { if (value != 0 && value != 1) return fail(c, MSGS_E_ARG, "dp_serial: 0 or 1"); c->dpSerial = (int)value; }
     else if (k == "wide_index") { if (value != 0 && value != 1) return fail(c, MSGS_E_ARG, "wide_index: 0 or 1"); c->wideIndex = (int)value; }
+    else if (k == "build_block_mb") { if (value < 0 || value > (1 << 20)) return fail(c, MSGS_E_ARG, "build_block_mb in [0, 2^20]"); c->buildBlockMb = value; }
     else return fail(c, MSGS_E_ARG, "unknown option " + k);
     return MSGS_OK;
 }
@@ -987,6 +1012,8 @@ extern "C" int msgs_ctx_get(msgs_ctx* c, const char* key, int64_t* value) {
     else if (k == "dp_serial") *value = c->dpSerial;
     else if (k == "record_format") *value = c->recordFormat;
     else if (k == "wide_index") *value = c->wideIndex;
+    else if (k == "build_block_mb") *value = c->buildBlockMb;
+    else if (k == "index_blocks") *value = c->db.blockEnds.empty() ? 1 : (int64_t)c->db.blockEnds.size();
     else if (k == "index_wide") *value = c->db.wide ? 1 : 0;
     else return fail(c, MSGS_E_ARG, "unknown option " + k);
     return MSGS_OK;

@@ -177,6 +177,8 @@ struct msgs_ctx {
     int recordFormat = 0; /* records of msgs_map_batch / msgs_batch_sam: 0 SAM text, 1 BAM */
     int dpSerial = 0;     /* 1 = every DP job takes the serial fill (M, I, D kept per cell) instead
                            * of the wavefront fill */
+    int64_t buildBlockMb = 2048; /* msgseqtk-build's -b (MB): databases of this many MB or more take
+                                  * the suffix order of its block-by-block build (0: one block) */
     int wideIndex = 0;    /* databases built next: 0 = the wide index layout (64-bit rows) only from
                            * 2^31 symbols on; 1 = always, with 2^14-position superblocks and
                            * 2^16-suffix sorting ranges (the wide code paths on small texts) */

@@ -31,6 +31,9 @@ struct DbState {
     void* dSa = nullptr;
     void* dIsa = nullptr;
     int64_t* dSup = nullptr;   /* wide: A, C, G, T counts before each superblock */
+    /* the reference's block-by-block build (msgseqtk-build with a block size below the BD
+     * length): BD end of each block, in text order; empty for one block */
+    std::vector<int64_t> blockEnds;
     uint8_t* dBwt = nullptr;
     OccBlock* dOcc = nullptr;
     int64_t* dNPos = nullptr;

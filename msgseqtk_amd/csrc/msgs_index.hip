@@ -21,6 +21,16 @@
  * a round may already be finer than that round's, which orders ties by a longer prefix
  * and so stays consistent with the suffix order (the argument of Larsson and Sadakane's
  * qsufsort).  The ranks become the ISA.
+ *
+ * Block order.  msgseqtk-build indexes a database of blockSize symbols or more block by
+ * block (src/msgseqtk-build.cpp:440-467: runs of whole chromosomes taken from the end) and
+ * merges the block indexes with FMDIndex::prepend (src/FMDIndex.cpp:267-308, 612-630):
+ * within a block the order is the block text's own (a suffix ends at its block's end), and
+ * the merge places a suffix of an earlier block after every suffix of a later block that
+ * equals it up to and including a '\0' (the interleaving walk starts each earlier-block
+ * '\0' row after all later-block '\0' rows).  The wide sort gives that order by coding
+ * '\0' per block (later blocks lower, all below A) and cutting every suffix at its block's
+ * end; one block is the plain order.
  */
 #include <cstring>
 #include <rocprim/device/device_radix_sort.hpp>
@@ -97,28 +107,58 @@ __global__ void __launch_bounds__(256) k_count_distinct(const uint32_t* gstart, 
 
 /* ---------------------------------------------------------------- wide texts */
 
-constexpr int kBinSyms = 4;                    /* bucket = the first four 3-bit symbols */
-constexpr int kBins = 1 << (3 * kBinSyms);
+constexpr int kBins = 1 << 12;                 /* buckets: the first 12 bits of the key */
+constexpr int kMaxBlocks = 9;                  /* '\0' codes of every block fit 4-bit codes */
 
-__device__ __forceinline__ uint32_t binOf(const uint8_t* bd, int64_t n, int64_t i) {
-    uint32_t b = 0;
-    for (int t = 0; t < kBinSyms; ++t) b = (b << 3) | (uint32_t)(i + t < n ? sym3(bd[i + t]) : 0);
+/* the blocks of the build order: one block, or the reference's block-by-block build */
+struct Blocks {
+    int32_t k;                  /* blocks (1: the whole text) */
+    int32_t bits;               /* bits per symbol code: 3 for one block, else 4 */
+    int32_t syms;               /* symbols per round-0 key: 21 or 15 */
+    int32_t binSyms;            /* symbols per bucket: 4 or 3 (12 bits) */
+    int64_t end[kMaxBlocks];    /* end of each block in the BD text */
+};
+
+__device__ __forceinline__ int blockOf(const Blocks& B, int64_t i) {
+    int b = 0;
+    while (b < B.k - 1 && i >= B.end[b]) ++b;
     return b;
 }
 
-__device__ __forceinline__ uint64_t key21(const uint8_t* bd, int64_t n, int64_t i) {
+/* code of BD byte c in block blk: past the block's end 0, '\0' 1 + (k - 1 - blk) (later
+ * blocks lower), A, C, G, T, N from k + 1; with one block the codes of sym3 */
+__device__ __forceinline__ uint64_t codeOf(const Blocks& B, uint8_t c, int blk) {
+    const uint64_t k = (uint64_t)B.k;
+    switch (c) {
+    case 0: return 1 + (k - 1 - (uint64_t)blk); case 1: return k + 1; case 2: return k + 2; case 4: return k + 3;
+    case 8: return k + 4; case 15: return k + 5;
+    default: return k + 6;
+    }
+}
+
+__device__ __forceinline__ uint32_t binOf(const uint8_t* bd, const Blocks& B, int64_t i) {
+    const int blk = blockOf(B, i);
+    const int64_t e = B.end[blk];
+    uint32_t b = 0;
+    for (int t = 0; t < B.binSyms; ++t) b = (b << B.bits) | (uint32_t)(i + t < e ? codeOf(B, bd[i + t], blk) : 0);
+    return b;
+}
+
+__device__ __forceinline__ uint64_t keyOf(const uint8_t* bd, const Blocks& B, int64_t i) {
+    const int blk = blockOf(B, i);
+    const int64_t e = B.end[blk];
     uint64_t k = 0;
-    for (int t = 0; t < 21; ++t) k = (k << 3) | (i + t < n ? sym3(bd[i + t]) : 0);
+    for (int t = 0; t < B.syms; ++t) k = (k << B.bits) | (i + t < e ? codeOf(B, bd[i + t], blk) : 0);
     return k;
 }
 
 /* suffixes per bucket: LDS counts per block, added to the global table once per block */
-__global__ void __launch_bounds__(256) k_bin_hist(const uint8_t* bd, int64_t n, unsigned long long* hist) {
+__global__ void __launch_bounds__(256) k_bin_hist(const uint8_t* bd, int64_t n, Blocks B, unsigned long long* hist) {
     __shared__ uint32_t h[kBins];
     for (int b = threadIdx.x; b < kBins; b += blockDim.x) h[b] = 0;
     __syncthreads();
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        atomicAdd(&h[binOf(bd, n, i)], 1u);
+        atomicAdd(&h[binOf(bd, B, i)], 1u);
     __syncthreads();
     for (int b = threadIdx.x; b < kBins; b += blockDim.x)
         if (h[b]) atomicAdd(&hist[b], (unsigned long long)h[b]);
@@ -126,7 +166,7 @@ __global__ void __launch_bounds__(256) k_bin_hist(const uint8_t* bd, int64_t n, 
 
 /* the positions whose bucket lies in [b0, b1), in any order: each block takes 4096
  * positions per step and reserves its slots with one global atomic */
-__global__ void __launch_bounds__(256) k_bin_select(const uint8_t* bd, int64_t n, uint32_t b0, uint32_t b1,
+__global__ void __launch_bounds__(256) k_bin_select(const uint8_t* bd, int64_t n, Blocks B, uint32_t b0, uint32_t b1,
                                                     int64_t* out, unsigned long long* counter) {
     __shared__ uint32_t sCnt;
     __shared__ unsigned long long sBase;
@@ -136,7 +176,7 @@ __global__ void __launch_bounds__(256) k_bin_select(const uint8_t* bd, int64_t n
         for (int r = 0; r < kPer; ++r) {
             const int64_t i = base + r * 256 + threadIdx.x;
             if (i < n) {
-                const uint32_t b = binOf(bd, n, i);
+                const uint32_t b = binOf(bd, B, i);
                 if (b >= b0 && b < b1) mask |= 1u << r;
             }
         }
@@ -154,9 +194,9 @@ __global__ void __launch_bounds__(256) k_bin_select(const uint8_t* bd, int64_t n
     }
 }
 
-__global__ void k_keys21(const uint8_t* bd, int64_t n, const int64_t* v, int64_t m, uint64_t* key) {
+__global__ void k_keys0(const uint8_t* bd, Blocks B, const int64_t* v, int64_t m, uint64_t* key) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < m) key[j] = key21(bd, n, v[j]);
+    if (j < m) key[j] = keyOf(bd, B, v[j]);
 }
 
 /* the rows of a range's unsettled suffixes: flag[j] = 1 when row rs + j is not settled */
@@ -167,14 +207,15 @@ __global__ void k_unsettled(const uint32_t* done, int64_t rs, int64_t m, uint8_t
     flag[j] = (done[p >> 5] >> (p & 31) & 1u) ? 0 : 1;
 }
 
-/* round keys of the unsettled rows P[0..m): (rank - rs) << 34 | rank[v + h] + 1 */
-__global__ void k_round_keys_w(const int64_t* P, int64_t m, const int64_t* sa, const int64_t* rank, int64_t n,
+/* round keys of the unsettled rows P[0..m): (rank - rs) << 34 | rank[v + h] + 1 (0 past the
+ * end of v's block) */
+__global__ void k_round_keys_w(const int64_t* P, int64_t m, const int64_t* sa, const int64_t* rank, Blocks B,
                                int64_t h, int64_t rs, int64_t* v, uint64_t* key) {
     const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= m) return;
     const int64_t s = sa[P[j]];
     v[j] = s;
-    const uint64_t r2 = s + h < n ? (uint64_t)rank[s + h] + 1 : 0;
+    const uint64_t r2 = s + h < B.end[blockOf(B, s)] ? (uint64_t)rank[s + h] + 1 : 0;
     key[j] = ((uint64_t)(rank[s] - rs) << 34) | r2;
 }
 
@@ -362,12 +403,23 @@ int sortWide(DbState& db, hipStream_t st, std::string& err, int64_t rangeMax) {
         err = "BD text length must be in [1, 2^34 - 2)";
         return MSGS_E_UNSUPPORTED;
     }
+    Blocks B;
+    std::memset(&B, 0, sizeof(B));
+    B.k = db.blockEnds.empty() ? 1 : (int32_t)db.blockEnds.size();
+    if (B.k > kMaxBlocks) {
+        err = "more than 9 build blocks";
+        return MSGS_E_UNSUPPORTED;
+    }
+    for (int b = 0; b < B.k; ++b) B.end[b] = db.blockEnds.empty() ? n : db.blockEnds[b];
+    B.bits = B.k == 1 ? 3 : 4;
+    B.syms = 63 / B.bits;
+    B.binSyms = 12 / B.bits;
     Freer tmp;
     hipError_t e;
     /* bucket sizes and the ranges of the suffix array they pack into */
     unsigned long long* dHist = tmp.alloc<unsigned long long>(kBins + 1, e); HCHK(e);
     HCHK(hipMemsetAsync(dHist, 0, (kBins + 1) * sizeof(unsigned long long), st));
-    k_bin_hist<<<2048, 256, 0, st>>>(db.dBd, n, dHist);
+    k_bin_hist<<<2048, 256, 0, st>>>(db.dBd, n, B, dHist);
     HCHK(hipGetLastError());
     std::vector<unsigned long long> hist(kBins);
     HCHK(hipMemcpyAsync(hist.data(), dHist, kBins * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
@@ -434,14 +486,14 @@ int sortWide(DbState& db, hipStream_t st, std::string& err, int64_t rangeMax) {
         return MSGS_OK;
     };
 
-    /* round 0: each range's suffixes by their first 21 symbols */
+    /* round 0: each range's suffixes by their first 21 (15) symbols */
     int64_t open = 0;
     for (Range& r : ranges) {
         HCHK(hipMemsetAsync(dCnt, 0, sizeof(unsigned long long), st));
-        k_bin_select<<<2048, 256, 0, st>>>(db.dBd, n, r.b0, r.b1, v0, dCnt);
-        k_keys21<<<grid(r.m), 256, 0, st>>>(db.dBd, n, v0, r.m, k0);
+        k_bin_select<<<2048, 256, 0, st>>>(db.dBd, n, B, r.b0, r.b1, v0, dCnt);
+        k_keys0<<<grid(r.m), 256, 0, st>>>(db.dBd, B, v0, r.m, k0);
         size_t tb = tmpBytes;
-        HCHK(rocprim::radix_sort_pairs(tmpStore, tb, k0, k1, v0, v1, (size_t)r.m, 0u, 63u, st));
+        HCHK(rocprim::radix_sort_pairs(tmpStore, tb, k0, k1, v0, v1, (size_t)r.m, 0u, (unsigned)(B.syms * B.bits), st));
         k_place<<<grid(r.m), 256, 0, st>>>(nullptr, v1, r.m, r.rs, sa);
         unsigned long long got = 0;
         HCHK(hipMemcpyAsync(&got, dCnt, sizeof(got), hipMemcpyDeviceToHost, st));
@@ -453,7 +505,7 @@ int sortWide(DbState& db, hipStream_t st, std::string& err, int64_t rangeMax) {
         open += r.open;
     }
     /* later rounds: the unsettled rows of each range by (rank, rank[i + h]) */
-    int64_t h = 21;
+    int64_t h = B.syms;
     for (int round = 1; open > 0; ++round, h *= 2) {
         if (round > 40) { err = "suffix sorting did not converge"; return MSGS_E_HIP; }
         open = 0;
@@ -467,7 +519,7 @@ int sortWide(DbState& db, hipStream_t st, std::string& err, int64_t rangeMax) {
             HCHK(hipStreamSynchronize(st));
             const int64_t m = (int64_t)mu;
             if (m != r.open) { err = "unsettled rows miscounted"; return MSGS_E_HIP; }
-            k_round_keys_w<<<grid(m), 256, 0, st>>>(P, m, sa, rank, n, h, r.rs, v0, k0);
+            k_round_keys_w<<<grid(m), 256, 0, st>>>(P, m, sa, rank, B, h, r.rs, v0, k0);
             tb = tmpBytes;
             HCHK(rocprim::radix_sort_pairs(tmpStore, tb, k0, k1, v0, v1, (size_t)m, 0u, (unsigned)(34 + mBits), st));
             k_place<<<grid(m), 256, 0, st>>>(P, v1, m, 0, sa);

@@ -101,12 +101,28 @@ int64_t getLoc(const Db& db, int64_t tid, int64_t pos) {
 /* Suffix array of t[0..n) in byte order where a suffix that is a proper
  * prefix of another sorts first -- the order divsufsort produces
  * (src/FMDIndex.cpp:29-68).  Prefix doubling (Manber & Myers). */
-std::vector<int64_t> suffixArray(const nt16_t* t, int64_t n) {
-    std::vector<int64_t> sa(n), rk(n), tmp(n);
-    for (int64_t i = 0; i < n; ++i) { sa[i] = i; rk[i] = t[i]; }
+std::vector<int64_t> suffixArray(const nt16_t* t, int64_t n) { return suffixArrayBlocks(t, n, {}); }
+
+/* The order of msgseqtk-build's block-by-block build (src/msgseqtk-build.cpp:440-467): each
+ * block indexed by divsufsort over its own text (src/FMDIndex.cpp:29-68), the block indexes
+ * merged by FMDIndex::prepend (src/FMDIndex.cpp:267-308) with buildInterleavingBS
+ * (:612-630), whose walk starts the rows of an earlier block's '\0' suffixes after all rows
+ * of the later block's '\0' suffixes.  So: a suffix ends at its block's end (ends[] in text
+ * order; empty = one block), and at a '\0' shared by suffixes of two blocks the later
+ * block's suffix sorts first.  Ranks of round 0: '\0' of block b = k - 1 - b, base codes
+ * above every '\0' rank; past a block's end -1. */
+std::vector<int64_t> suffixArrayBlocks(const nt16_t* t, int64_t n, const std::vector<int64_t>& ends) {
+    const int64_t k = ends.empty() ? 1 : (int64_t)ends.size();
+    std::vector<int64_t> sa(n), rk(n), tmp(n), bend(n);
+    for (int64_t i = 0, b = 0; i < n; ++i) {
+        while (b + 1 < k && i >= ends[b]) ++b;
+        bend[i] = ends.empty() ? n : ends[b];
+        sa[i] = i;
+        rk[i] = t[i] == 0 ? k - 1 - b : k + t[i];
+    }
     if (n <= 1) return sa;
     for (int64_t h = 1;; h <<= 1) {
-        auto key2 = [&](int64_t i) { return i + h < n ? rk[i + h] : (int64_t)-1; };
+        auto key2 = [&](int64_t i) { return i + h < bend[i] ? rk[i + h] : (int64_t)-1; };
         std::sort(sa.begin(), sa.end(), [&](int64_t a, int64_t b) {
             if (rk[a] != rk[b]) return rk[a] < rk[b];
             return key2(a) < key2(b);
@@ -120,6 +136,27 @@ std::vector<int64_t> suffixArray(const nt16_t* t, int64_t n) {
         if (rk[sa[n - 1]] == n - 1) break;
     }
     return sa;
+}
+
+std::vector<int64_t> buildBlockEnds(const std::vector<int64_t>& chromBDLen, int64_t blockSize) {
+    const int64_t NC = (int64_t)chromBDLen.size();
+    std::vector<int64_t> start(NC + 1, 0);
+    for (int64_t i = 0; i < NC; ++i) start[i + 1] = start[i] + chromBDLen[i];
+    std::vector<int64_t> ends;
+    if (blockSize <= 0 || start[NC] < blockSize) return ends;   /* one step */
+    auto len = [&](int64_t a, int64_t b) { return start[b] - start[a]; };
+    int64_t blockEnd = NC;
+    std::vector<int64_t> firsts;
+    for (int64_t i = NC; i > 0; --i) {
+        const int64_t blockStart = i - 1;
+        if (blockStart == 0 || (len(blockStart, blockEnd) < blockSize && len(blockStart - 1, blockEnd) >= blockSize)) {
+            firsts.push_back(blockStart);
+            blockEnd = blockStart;
+        }
+    }
+    for (size_t b = firsts.size(); b-- > 0;) ends.push_back(b == 0 ? start[NC] : start[firsts[b - 1]]);
+    if (ends.size() == 1) ends.clear();
+    return ends;
 }
 
 /* buildCounts (src/FMDIndex.cpp:118-134), buildBWT (:385-393), buildGap (:406-410). */

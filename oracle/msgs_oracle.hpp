@@ -74,6 +74,12 @@ struct Db {
 void buildMeta(Db& db, const nt16_t* seqs, const int64_t* lens, int nChrom, const std::vector<std::string>& names);
 /* Suffix array by prefix doubling; byte order, shorter suffix first. */
 std::vector<int64_t> suffixArray(const nt16_t* t, int64_t n);
+/* The suffix order of msgseqtk-build's block-by-block build, blocks ending at ends[] (text
+ * order; empty = one block) */
+std::vector<int64_t> suffixArrayBlocks(const nt16_t* t, int64_t n, const std::vector<int64_t>& ends);
+/* BD ends of the blocks msgseqtk-build forms from chromosome BD lengths (src/msgseqtk-build.cpp:
+ * 403-415, 440-467); empty when the text is built in one step */
+std::vector<int64_t> buildBlockEnds(const std::vector<int64_t>& chromBDLen, int64_t blockSize);
 /* counts / BWT / gapSA / occ (src/FMDIndex.cpp:29-141,385-415); sa may be given. */
 void buildIndex(Db& db, const int64_t* saIn);
 

@@ -63,6 +63,23 @@ void* orc_db_create(const uint8_t* seqs, const int64_t* lens, int32_t nChrom, co
     }
 }
 void orc_db_free(void* h) { delete (Db*)h; }
+
+/* msgseqtk-build's blocks for chromosomes of these lengths: BD ends into ends (at most cap),
+ * returns their number (0: one step) */
+int64_t orc_block_ends(const int64_t* lens, int32_t nChrom, int64_t blockSize, int64_t* ends, int64_t cap) {
+    std::vector<int64_t> bdLen(nChrom);
+    for (int32_t t = 0; t < nChrom; ++t) bdLen[t] = 2 * (lens[t] + 1);
+    std::vector<int64_t> e = buildBlockEnds(bdLen, blockSize);
+    for (size_t i = 0; i < e.size() && (int64_t)i < cap; ++i) ends[i] = e[i];
+    return (int64_t)e.size();
+}
+
+/* suffix array of a BD text in the block-by-block build order (k = 0: one block) */
+void orc_suffix_array(const uint8_t* bd, int64_t n, const int64_t* ends, int64_t k, int64_t* sa) {
+    std::vector<int64_t> e(ends, ends + k);
+    std::vector<int64_t> s = suffixArrayBlocks(bd, n, e);
+    std::memcpy(sa, s.data(), n * sizeof(int64_t));
+}
 int64_t orc_db_len(void* h) { return ((Db*)h)->n; }
 void orc_db_counts(void* h, int64_t* B, int64_t* C) {
     Db* db = (Db*)h;

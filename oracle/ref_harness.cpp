@@ -71,6 +71,33 @@ void* ref_index_build(const uint8_t* bd, int64_t n) {
     }
 }
 void ref_index_free(void* h) { delete (FMDIndex*)h; }
+
+/* msgseqtk-build's block-by-block build (src/msgseqtk-build.cpp:403-415, 440-467, restated:
+ * the build program is not part of this library): chromosomes whose BD texts start at
+ * chromBD[0..nChrom] (chromBD[nChrom] = n) are taken from the last one into blocks, a block
+ * closed when one more chromosome would reach blockSize; each block is indexed by the
+ * reference's FMDIndex constructor and prepended with the reference's FMDIndex::prepend;
+ * then the reference's buildSA.  A text shorter than blockSize is built in one step. */
+void* ref_index_build_blocks(const uint8_t* bd, const int64_t* chromBD, int32_t nChrom, int64_t blockSize) {
+    try {
+        const int64_t n = chromBD[nChrom];
+        if (n < blockSize) return new FMDIndex(DNAseq(bd, bd + n), true, FMDIndex::SA_SAMPLE_RATE);
+        auto len = [&](int64_t a, int64_t b) { return chromBD[b] - chromBD[a]; };
+        FMDIndex* f = new FMDIndex();
+        int64_t blockEnd = nChrom;
+        for (int64_t i = nChrom; i > 0; --i) {
+            const int64_t blockStart = i - 1;
+            if (blockStart == 0 || (len(blockStart, blockEnd) < blockSize && len(blockStart - 1, blockEnd) >= blockSize)) {
+                f->prepend(FMDIndex(DNAseq(bd + chromBD[blockStart], bd + chromBD[blockEnd])));
+                blockEnd = blockStart;
+            }
+        }
+        f->buildSA(FMDIndex::SA_SAMPLE_RATE);
+        return f;
+    } catch (...) {
+        return nullptr;
+    }
+}
 int64_t ref_length(void* h) { return ((FMDIndex*)h)->length(); }
 void ref_counts(void* h, int64_t* B, int64_t* C) {
     const FMDIndex* f = (const FMDIndex*)h;

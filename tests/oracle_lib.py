@@ -62,6 +62,9 @@ def orc():
         lib.orc_db_create.restype = C.c_void_p
         lib.orc_db_create.argtypes = [PU8, P64, C.c_int32, C.c_char_p, P64]
         lib.orc_db_free.argtypes = [C.c_void_p]
+        lib.orc_block_ends.argtypes = [P64, C.c_int32, C.c_int64, P64, C.c_int64]
+        lib.orc_block_ends.restype = C.c_int64
+        lib.orc_suffix_array.argtypes = [PU8, C.c_int64, P64, C.c_int64, P64]
         lib.orc_db_len.argtypes = [C.c_void_p]
         lib.orc_db_len.restype = C.c_int64
         lib.orc_db_counts.argtypes = [C.c_void_p, P64, P64]
@@ -153,6 +156,8 @@ def ref():
         lib.ref_revcom.argtypes = [PU8, C.c_int64, PU8]
         lib.ref_primary_revcom.argtypes = [C.c_char_p, C.c_char_p, PU8, PU8]
         lib.ref_primary_revcom.restype = C.c_int64
+        lib.ref_index_build_blocks.restype = C.c_void_p
+        lib.ref_index_build_blocks.argtypes = [PU8, P64, C.c_int32, C.c_int64]
         lib.ref_index_from_sa.restype = C.c_void_p
         lib.ref_index_from_sa.argtypes = [PU8, P64, C.c_int64]
         lib.ref_baseline_map.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, PU8, PU8, P64, C.c_char_p, PU8, PU8, P64,
@@ -164,6 +169,23 @@ def ref():
         lib.ref_index_load.restype = C.c_void_p
         _ref = lib
     return _ref
+
+
+def block_ends(lens, block_size):
+    """msgseqtk-build's blocks for chromosomes of these lengths (BD ends; empty: one step)"""
+    lens = np.ascontiguousarray(lens, np.int64)
+    out = np.zeros(len(lens) + 1, np.int64)
+    k = orc().orc_block_ends(ptr(lens, C.c_int64), len(lens), int(block_size), ptr(out, C.c_int64), len(out))
+    return out[:k].copy()
+
+
+def block_suffix_array(bd, ends):
+    """the oracle's suffix array of a BD text in the block-by-block build order"""
+    bd = np.ascontiguousarray(bd, np.uint8)
+    ends = np.ascontiguousarray(ends, np.int64)
+    sa = np.zeros(len(bd), np.int64)
+    orc().orc_suffix_array(ptr(bd, C.c_uint8), len(bd), ptr(ends, C.c_int64), len(ends), ptr(sa, C.c_int64))
+    return sa
 
 
 class OracleDb:

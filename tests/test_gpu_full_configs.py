@@ -31,23 +31,33 @@ def _diff(a, b):
     return len(bad), (bad[:3], [(a[i], b[i]) for i in bad[:2]])
 
 
-def check_suffix_order(bd, bwt, sa, samples=20000, seed=5):
+def check_suffix_order(bd, bwt, sa, samples=20000, seed=5, ends=()):
     """adjacent rows of the suffix array in byte order (0 lowest, a suffix that is a prefix of
-    another first) on sampled rows whose first 64 symbols differ; BWT[i] = BD[SA[i]-1]"""
+    another first) on sampled rows whose first 64 symbols differ; BWT[i] = BD[SA[i]-1].  With
+    msgseqtk-build's blocks (BD ends in `ends`): a suffix ends at its block's end, and two
+    suffixes of different blocks that agree up to a shared '\0' are ordered later block first."""
     n = len(sa)
     rng = np.random.default_rng(seed)
     rows = rng.integers(0, n - 1, samples)
     pad = np.concatenate([bd, np.zeros(64, np.uint8)])
     a, b = sa[rows], sa[rows + 1]
-    wa = pad[a[:, None] + np.arange(64)[None, :]].astype(np.int16)
-    wb = pad[b[:, None] + np.arange(64)[None, :]].astype(np.int16)
-    # past the text end a suffix is exhausted: mark it below every symbol
-    wa[a[:, None] + np.arange(64)[None, :] >= n] = -1
-    wb[b[:, None] + np.arange(64)[None, :] >= n] = -1
+    ends = np.asarray(ends if len(ends) else [n], np.int64)
+    blk_a, blk_b = np.searchsorted(ends, a, side="right"), np.searchsorted(ends, b, side="right")
+    end_a, end_b = ends[blk_a], ends[blk_b]
+    off = np.arange(64)[None, :]
+    wa = pad[a[:, None] + off].astype(np.int16)
+    wb = pad[b[:, None] + off].astype(np.int16)
+    # past its block's end a suffix is exhausted: mark it below every symbol
+    wa[a[:, None] + off >= end_a[:, None]] = -1
+    wb[b[:, None] + off >= end_b[:, None]] = -1
     diff = wa != wb
     has = diff.any(axis=1)
     k = diff.argmax(axis=1)
-    ok = wa[np.arange(samples), k] < wb[np.arange(samples), k]
+    # a '\0' both share before the first difference: the block rule decides
+    zero_before = ((wa == 0) & (off < k[:, None])).any(axis=1)
+    cross = zero_before & (blk_a != blk_b)
+    plain = wa[np.arange(samples), k] < wb[np.arange(samples), k]
+    ok = np.where(cross, blk_a > blk_b, plain)
     assert ok[has].all(), "suffix array rows out of order"
     r = rng.integers(0, n, samples)
     want = np.where(sa[r] == 0, 0, bd[np.maximum(sa[r] - 1, 0)])

@@ -5,7 +5,8 @@ wide layout (64-bit rows, superblock counts, range-by-range suffix sorting); the
 index is trained on the GPU over it.  A sample of 20k read pairs 2x150 (insert N(400, 80),
 1% error, 10% off-target pairs) is mapped in the learned-index form and the plain forms and
 compared record by record with the oracle, whose index is built from the device's suffix
-array after sampled order checks (adjacent suffixes in order, BWT = text before each suffix);
+array after sampled order checks (adjacent suffixes in the order of msgseqtk-build's default
+block-by-block build, BWT = text before each suffix);
 the device suffix sort equals the oracle's own at the sizes where that runs
 (test_gpu_wide.py, test_gpu_seeding.py)."""
 import numpy as np
@@ -36,8 +37,12 @@ def test_large_4gbp_pairs():
     _say("device index: BD length %d, wide layout %d" % (m.n, m.get("index_wide")))
     assert m.n == sum(2 * (len(c) + 1) for c in chroms) and m.n > 2 ** 32
     assert m.get("index_wide") == 1
+    # msgseqtk-build's default 2048 MB blocks: the suffix order of its block-by-block build
+    ends = O.block_ends([len(c) for c in chroms], 2048 << 20)
+    _say("build blocks ending at %s" % ends)
+    assert len(ends) >= 3 and m.get("index_blocks") == len(ends)
     bd, bwt, sa = m.arrays()
-    check_suffix_order(bd, bwt, sa, samples=40000)
+    check_suffix_order(bd, bwt, sa, samples=40000, ends=ends)
     del bd, bwt
     odb = O.OracleDb(chroms, names, sa=sa)
     del sa

@@ -191,3 +191,65 @@ def test_wide_db_with_n():
     _check_smems(odb, m, seqs, 1, float("inf"))
     _check_smems(odb, m, seqs, 17, 0.01)
     m.close()
+
+
+def _shared_ends(seed, n, length):
+    """genomes whose ends and whole sequences recur (strains at low divergence, a shared tail,
+    exact copies), so that many suffixes of different blocks agree up to a '\\0'"""
+    rng = np.random.default_rng(seed)
+    chroms, names = synth.metagenome(n, length, gc=(0.3, 0.7), seed=seed, strain_frac=0.4, div=(0.002, 0.01))
+    tail = synth.random_genome(rng, 60)
+    chroms = [np.concatenate([c[:len(c) - 60], tail]) if i % 3 == 0 else c for i, c in enumerate(chroms)]
+    chroms += [chroms[0].copy(), chroms[5].copy(), synth.revcom(chroms[2])]
+    return chroms, ["c%03d" % i for i in range(len(chroms))]
+
+
+@pytest.fixture(scope="module")
+def blockdb():
+    chroms, names = _shared_ends(71, 24, 45000)
+    lens = np.array([len(c) for c in chroms], np.int64)
+    ends = O.block_ends(lens, 1 << 20)
+    odb = O.OracleDb(chroms, names)
+    bd = odb.arrays()[0]
+    _say("block test database: BD length %d, blocks ending at %s" % (len(bd), ends))
+    return chroms, names, ends, bd, O.block_suffix_array(bd, ends), O.block_suffix_array(bd, [])
+
+
+@pytest.mark.parametrize("wide", [0, 1])
+def test_block_build_order(blockdb, wide):
+    """msgseqtk-build's block-by-block suffix order (1 MB blocks over a 2.4 MB text): the
+    device suffix array equals the oracle's block-order suffix array (itself equal to the
+    reference's merged index, test_oracle_vs_reference.py), and SMEM lists and SAM equal the
+    oracle's over that index"""
+    chroms, names, ends, bd0, want, one_step = blockdb
+    assert len(ends) >= 3
+    m = M.Mapper(0).set("wide_index", wide).set("build_block_mb", 1).build(chroms, names)
+    assert m.get("index_blocks") == len(ends) and m.get("index_wide") == 1
+    bd, bwt, sa = m.arrays()
+    assert np.array_equal(bd, bd0)
+    assert np.array_equal(sa, want)
+    assert not np.array_equal(sa, one_step)
+    assert np.array_equal(bwt, np.where(sa == 0, 0, bd[np.maximum(sa - 1, 0)]))
+    odb = O.OracleDb(chroms, names, sa=sa)
+    # reads at chromosome ends (where suffixes of different blocks tie through '\0') and inside
+    rng = np.random.default_rng(5)
+    seqs = []
+    for t in range(len(chroms)):
+        for L in (40, 150):
+            seqs.append(np.ascontiguousarray(chroms[t][len(chroms[t]) - L:]))
+            seqs.append(synth.revcom(chroms[t][len(chroms[t]) - L:]))
+    more = synth.simulate_reads(chroms, 3000, (50, 250), sub=0.01, indel=0.001, seed=72)
+    seqs += more[0]
+    _check_smems(odb, m, seqs, 17, 0.01)
+    quals = [np.full(len(x), 30, np.uint8) for x in seqs]
+    rn = ["r%d" % i for i in range(len(seqs))]
+    exp, _ = O.orc_map(odb, O.default_params(), seqs, quals, rn, nthreads=16)
+    got = m.map((seqs, quals, rn), M.default_params())
+    n, detail = _diff(exp, got)
+    assert n == 0, detail
+    # the default block size builds this database in one step
+    one = M.Mapper(0).build(chroms, names)
+    assert one.get("index_blocks") == 1
+    assert np.array_equal(one.arrays()[2], one_step)
+    one.close()
+    m.close()

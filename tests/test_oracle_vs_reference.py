@@ -37,6 +37,59 @@ def test_index_matches_reference(seed):
             np.isinf(L.ref_loglik(ri.h, b)) and np.isinf(O.orc().orc_loglik(db.h, b)))
 
 
+def _shared_ends_db(seed):
+    """chromosomes whose ends (and whole sequences) recur, so that many suffixes of different
+    chromosomes and blocks agree up to a '\0'"""
+    rng = np.random.default_rng(seed)
+    chroms, names = synth.metagenome(10, 3000, gc=(0.3, 0.7), seed=seed, strain_frac=0.4, div=(0.002, 0.01))
+    tail = synth.random_genome(rng, 40)
+    chroms = [np.concatenate([c[:len(c) - 40], tail]) if i % 3 == 0 else c for i, c in enumerate(chroms)]
+    chroms += [chroms[0].copy(), chroms[4].copy(), synth.revcom(chroms[2])]
+    return chroms
+
+
+@pytest.mark.parametrize("block_size", [0, 7000, 15000, 26000, 40000, 78032, 78033])
+def test_block_build_order_matches_reference(block_size):
+    """the oracle's suffix array in msgseqtk-build's block-by-block order (blocks indexed
+    separately and merged by FMDIndex::prepend) equals the reference's merged index, row by
+    row (accessBWT everywhere, accessSA everywhere but the rows below numGaps that are
+    multiples of 32), for block sizes giving 1 to 13 blocks.  The exception is the
+    reference's: FMDIndex::buildSA (src/FMDIndex.cpp:496-508) stores a sample for every '\0'
+    row i at SAsampled[rank1(i) - 1], but only rows with i % 32 == 0 or BWT[i] == 0 are
+    sampled, so those writes land on the samples of rows 0, 32, ... (a merged index only;
+    the one-step constructor samples with sampleSA).  Those rows hold suffixes that start
+    with '\0', which no read search reaches."""
+    chroms = _shared_ends_db(11)
+    lens = np.array([len(c) for c in chroms], np.int64)
+    db = O.OracleDb(chroms)
+    bd, _, _ = db.arrays()
+    n = len(bd)
+    starts = np.concatenate([[0], np.cumsum(2 * (lens + 1))]).astype(np.int64)
+    ends = O.block_ends(lens, block_size)
+    sa = O.block_suffix_array(bd, ends)
+    if 0 < block_size <= n:
+        assert len(ends) >= 2 and ends[-1] == n
+    ri = O.ref().ref_index_build_blocks(O.ptr(bd, O.C.c_uint8), O.ptr(starts, O.C.c_int64), len(chroms),
+                                         block_size if block_size else n + 1)
+    assert ri
+    try:
+        L = O.ref()
+        gaps = int((bd == 0).sum())
+        clobbered = 0
+        for i in range(n):
+            assert L.ref_access_bwt(ri, i) == (0 if sa[i] == 0 else bd[sa[i] - 1]), (i, len(ends))
+            if len(ends) >= 2 and i < gaps and i % 32 == 0:
+                clobbered += L.ref_access_sa(ri, i) != sa[i]
+                continue
+            assert L.ref_access_sa(ri, i) == sa[i], (i, len(ends))
+        if len(ends) < 2:
+            assert clobbered == 0
+    finally:
+        O.ref().ref_index_free(ri)
+    if len(ends) >= 2:
+        assert not np.array_equal(sa, O.block_suffix_array(bd, []))   # the order differs from one block
+
+
 def test_backext_matches_reference():
     chroms, names, db = _db(3)
     bd, _, _ = db.arrays()
