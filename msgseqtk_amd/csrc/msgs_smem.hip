@@ -7,10 +7,10 @@
  * step machine: every loop iteration of a lane performs exactly one FMD extension, so
  * all lanes of a wavefront issue their occ-block loads at the same instruction no
  * matter which phase of which search they are in.  The lock-step candidate list of a
- * search (the reference's `prev`) lives in a per-lane scratch area (32-byte records,
- * contiguous per lane so that consecutive candidates share cache lines); it is
- * traversed in reverse of its push order and compacted in place exactly like the
- * reference's reverse + erase(remove_if).
+ * search (the reference's `prev`) keeps its first entries in LDS and the rest in a
+ * per-lane global scratch area (records interleaved over lanes); it is traversed in
+ * reverse of its push order and compacted in place exactly like the reference's
+ * reverse + erase(remove_if).
  *
  * Floating point: logP is accumulated in fp64 in the reference's order.  The e-value
  * gates N*exp(logP) <= maxE are evaluated as logP <= T, with T found on the host from
@@ -105,6 +105,45 @@ __device__ __forceinline__ void candStore(const Ctx<Row>& c, int k, int pos, Row
     SmemCandT<Row> v;
     v.F = F; v.R = R; v.S = S; v.X = X; v.LL = ll; v.pos = pos; v.pad = 0;
     candAt(c, k) = v;
+}
+
+/* The per-lane kernels keep the first W entries of each lane's list in LDS (one slot column
+ * per thread of the 256-thread block, structure of arrays) and only later entries in the
+ * global scratch: a list holds one entry per distinct occurrence count along a search, so
+ * it is short for reads from one or a few genomes.  W is what fits four blocks per CU
+ * (at most 38 KB per block); X is kept only by the unique-interval path. */
+template <bool FAST, class Row>
+struct CandSlots {
+    static constexpr int W = FAST ? (sizeof(Row) == 8 ? 3 : 5) : (sizeof(Row) == 8 ? 4 : 6);
+    Row F[W][256], R[W][256], S[W][256];
+    Row X[FAST ? W : 1][256];
+    double LL[W][256];
+    int32_t pos[W][256];
+};
+
+template <bool FAST, class Row>
+__device__ __forceinline__ SmemCandT<Row> laneLoad(const Ctx<Row>& c, const CandSlots<FAST, Row>& s, int k) {
+    if (k < CandSlots<FAST, Row>::W) {
+        const int t = threadIdx.x;
+        SmemCandT<Row> v;
+        v.F = s.F[k][t]; v.R = s.R[k][t]; v.S = s.S[k][t]; v.X = FAST ? s.X[k][t] : (Row)-1;
+        v.LL = s.LL[k][t]; v.pos = s.pos[k][t]; v.pad = 0;
+        return v;
+    }
+    return candAt(c, k);
+}
+
+template <bool FAST, class Row>
+__device__ __forceinline__ void laneStore(const Ctx<Row>& c, CandSlots<FAST, Row>& s, int k, int pos, Row F, Row R,
+                                          Row S, double ll, Row X) {
+    if (k < CandSlots<FAST, Row>::W) {
+        const int t = threadIdx.x;
+        s.F[k][t] = F; s.R[k][t] = R; s.S[k][t] = S;
+        if (FAST) s.X[k][t] = X;
+        s.LL[k][t] = ll; s.pos[k][t] = pos;
+        return;
+    }
+    candStore(c, k, pos, F, R, S, ll, X);
 }
 
 /* start findFwdBackSMEMS / findBackFwdSMEMS at f; false if it ends at once (ambiguous base) */
@@ -435,6 +474,7 @@ __device__ void pairTake(const Ctx<Row>& c, Lane<Row>& l, int64_t rid) {
  */
 template <bool FAST, bool PAIR, class Row>
 __device__ __forceinline__ void smemBody(const DevIndex& ix, const SmemArgs& a) {
+    __shared__ CandSlots<FAST, Row> slots;
     Lane<Row> l;
     l.nExt = 0; l.nFast = 0; l.nLook = 0;
     l.state = ST_IDLE;
@@ -504,7 +544,7 @@ __device__ __forceinline__ void smemBody(const DevIndex& ix, const SmemArgs& a) 
             if (nxOk) {
                 pos = nxPos; P = nxF; Q = nxR; S = nxS; LL = nxLL; X = FAST ? nxX : -1;
             } else {
-                const SmemCandT<Row> cd = candAt(c, l.k);
+                const SmemCandT<Row> cd = laneLoad(c, slots, l.k);
                 pos = cd.pos; P = cd.F; Q = cd.R; S = cd.S; LL = cd.LL;
                 X = FAST ? cd.X : -1;
             }
@@ -515,7 +555,7 @@ __device__ __forceinline__ void smemBody(const DevIndex& ix, const SmemArgs& a) 
              * this pass stored it */
             nxOk = l.k > l.lo || l.w < l.nCand - 1;
             if (nxOk) {
-                const SmemCandT<Row> cd = candAt(c, l.k > l.lo ? l.k - 1 : l.nCand - 1);
+                const SmemCandT<Row> cd = laneLoad(c, slots, l.k > l.lo ? l.k - 1 : l.nCand - 1);
                 nxPos = cd.pos; nxF = cd.F; nxR = cd.R; nxS = cd.S; nxLL = cd.LL; nxX = cd.X;
             }
         }
@@ -572,7 +612,7 @@ __device__ __forceinline__ void smemBody(const DevIndex& ix, const SmemArgs& a) 
                 gatePush(c, l, LL0, fwd ? 0 : l.cTo, fwd ? l.cFrom : l.L, llA, llC, llG, llT)) {
                 exact0();
                 const int pp = fwd ? l.cTo : l.cFrom;
-                candStore(c, l.nCand++, pp, F0, R0, S0, LL0, (Row)-1);
+                laneStore(c, slots, l.nCand++, pp, F0, R0, S0, LL0, (Row)-1);
                 /* the last push heads the phase-2 list if phase 1 ends here */
                 nxPos = pp; nxF = F0; nxR = R0; nxS = S0; nxLL = LL0; nxX = -1;
                 nxOk = true;
@@ -606,7 +646,7 @@ __device__ __forceinline__ void smemBody(const DevIndex& ix, const SmemArgs& a) 
                     nxPos = pos; nxF = P; nxR = Q; nxS = S; nxLL = LL; nxX = X;
                     nxOk = true;
                 }
-                candStore(c, l.w--, pos, P, Q, S, LL, X);
+                laneStore(c, slots, l.w--, pos, P, Q, S, LL, X);
             }
             l.k--;
             if (l.k < l.lo) {
@@ -638,13 +678,18 @@ __device__ __forceinline__ void smemBody(const DevIndex& ix, const SmemArgs& a) 
 }
 
 
+/* 4 waves per SIMD (128 VGPRs) for 32-bit rows; the 64-bit rows of wide indexes need more
+ * registers than that (they spilled 52-124 bytes per lane to scratch, which the counters
+ * showed as ~13 GB of writes per launch on a 6.8e9-symbol index), so 3 waves per SIMD */
+template <class Row> constexpr int smemWaves() { return sizeof(Row) == 8 ? 3 : 4; }
+
 template <bool FAST, class Row>
-__global__ void __launch_bounds__(256, 4) smem_kernel_t(DevIndex ix, SmemArgs a) {
+__global__ void __launch_bounds__(256, smemWaves<Row>()) smem_kernel_t(DevIndex ix, SmemArgs a) {
     smemBody<FAST, false, Row>(ix, a);
 }
 
 template <bool FAST, class Row>
-__global__ void __launch_bounds__(256, 4) smem_pair_kernel(DevIndex ix, SmemArgs a) {
+__global__ void __launch_bounds__(256, smemWaves<Row>()) smem_pair_kernel(DevIndex ix, SmemArgs a) {
     smemBody<FAST, true, Row>(ix, a);
 }
 
