@@ -51,13 +51,23 @@ __device__ __forceinline__ short toBf16(float f) {
 }
 __device__ __forceinline__ float fromBf16(short h) { return __uint_as_float(((uint32_t)(uint16_t)h) << 16); }
 
+/* key of every SA row: the 16 text bytes at SA[i] from three aligned words (the BD text is
+ * padded by 64 bytes), not 16 byte loads at a random address */
 __global__ void k_ct(DevIndex ix, uint64_t* ct) {
     const uint8_t* bd = ix.bd;
     const int64_t n = ix.n;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t p = saAt(ix, i);
+        const uint64_t* w = (const uint64_t*)(bd + (p & ~(int64_t)7));
+        const uint64_t w0 = w[0], w1 = w[1], w2 = w[2];
+        const uint32_t sh = 8u * (uint32_t)(p & 7);
+        const uint64_t lo = sh ? (w0 >> sh) | (w1 << (64 - sh)) : w0;
+        const uint64_t hi = sh ? (w1 >> sh) | (w2 << (64 - sh)) : w1;
         uint64_t k = 0;
-        for (int t = 0; t < 16; ++t) k = (k << 3) | (p + t < n ? sym3(bd[p + t]) : 0);
+        for (int t = 0; t < 16; ++t) {
+            const uint8_t b = (uint8_t)((t < 8 ? lo : hi) >> (8 * (t & 7)));
+            k = (k << 3) | (p + t < n ? sym3(b) : 0);
+        }
         ct[i] = k;
     }
 }
