@@ -93,8 +93,9 @@ const char* msgs_last_error(msgs_ctx* ctx);
  * (src/msgseqtk-build.cpp:440-467, src/FMDIndex.cpp:267-308, 612-630), so that row numbers
  * and DB.fmd match a database the reference built with the same -b (default 2048, the
  * reference's; 0 = one block whatever the size); msgs_ctx_get "index_blocks" tells how many
- * blocks the resident database was ordered by.  Results never depend
- * on these settings, except "record_format": the records msgs_map_batch and msgs_batch_sam
+ * blocks the resident database was ordered by (row numbers and bi-intervals follow it; the
+ * mapping output matches the reference's over a database it built with the same -b).  Results
+ * never depend on the other settings, except "record_format": the records msgs_map_batch and msgs_batch_sam
  * return are SAM text lines (0, default) or the same records in the BAM layout (1): for each,
  * what htslib's bam_write1 puts in the BGZF stream (int32 block_size, the 32-byte core, read name,
  * CIGAR, 4-bit SEQ, QUAL and tags as the reference's BAM::setAux stores them), uncompressed. */
