@@ -107,14 +107,17 @@ __device__ __forceinline__ void candStore(const Ctx<Row>& c, int k, int pos, Row
     candAt(c, k) = v;
 }
 
-/* The per-lane kernels keep the first W entries of each lane's list in LDS (one slot column
- * per thread of the 256-thread block, structure of arrays) and only later entries in the
- * global scratch: a list holds one entry per distinct occurrence count along a search, so
- * it is short for reads from one or a few genomes.  W is what fits four blocks per CU
- * (at most 38 KB per block); X is kept only by the unique-interval path. */
+/* The per-lane kernels keep the top W entries of each lane's list (indices nCand - W ..
+ * nCand - 1) in LDS, one slot column per thread of the 256-thread block (structure of
+ * arrays, slot = index % W), and the entries below them in the global scratch.  Phase 1
+ * pushes at the top, evicting the entry that leaves the window to the global scratch;
+ * phase 2 reads the list from the top down and stores the kept entries from the top down,
+ * so its passes after the first stay in LDS while at most W entries survive.  W is what
+ * fits the blocks per CU of the row width (four for 32-bit rows: at most 38 KB per block;
+ * three for 64-bit rows: at most 48 KB); X is kept only by the unique-interval path. */
 template <bool FAST, class Row>
 struct CandSlots {
-    static constexpr int W = FAST ? (sizeof(Row) == 8 ? 3 : 5) : (sizeof(Row) == 8 ? 4 : 6);
+    static constexpr int W = FAST ? (sizeof(Row) == 8 ? 4 : 5) : (sizeof(Row) == 8 ? 5 : 6);
     Row F[W][256], R[W][256], S[W][256];
     Row X[FAST ? W : 1][256];
     double LL[W][256];
@@ -122,28 +125,46 @@ struct CandSlots {
 };
 
 template <bool FAST, class Row>
-__device__ __forceinline__ SmemCandT<Row> laneLoad(const Ctx<Row>& c, const CandSlots<FAST, Row>& s, int k) {
-    if (k < CandSlots<FAST, Row>::W) {
-        const int t = threadIdx.x;
-        SmemCandT<Row> v;
-        v.F = s.F[k][t]; v.R = s.R[k][t]; v.S = s.S[k][t]; v.X = FAST ? s.X[k][t] : (Row)-1;
-        v.LL = s.LL[k][t]; v.pos = s.pos[k][t]; v.pad = 0;
-        return v;
-    }
+__device__ __forceinline__ SmemCandT<Row> slotLoad(const CandSlots<FAST, Row>& s, int q) {
+    const int t = threadIdx.x;
+    SmemCandT<Row> v;
+    v.F = s.F[q][t]; v.R = s.R[q][t]; v.S = s.S[q][t]; v.X = FAST ? s.X[q][t] : (Row)-1;
+    v.LL = s.LL[q][t]; v.pos = s.pos[q][t]; v.pad = 0;
+    return v;
+}
+
+/* entry k of a list of nCand entries */
+template <bool FAST, class Row>
+__device__ __forceinline__ SmemCandT<Row> laneLoad(const Ctx<Row>& c, const CandSlots<FAST, Row>& s, int k, int nCand) {
+    constexpr int W = CandSlots<FAST, Row>::W;
+    if (k >= nCand - W) return slotLoad(s, k % W);
     return candAt(c, k);
 }
 
 template <bool FAST, class Row>
-__device__ __forceinline__ void laneStore(const Ctx<Row>& c, CandSlots<FAST, Row>& s, int k, int pos, Row F, Row R,
-                                          Row S, double ll, Row X) {
-    if (k < CandSlots<FAST, Row>::W) {
-        const int t = threadIdx.x;
-        s.F[k][t] = F; s.R[k][t] = R; s.S[k][t] = S;
-        if (FAST) s.X[k][t] = X;
-        s.LL[k][t] = ll; s.pos[k][t] = pos;
+__device__ __forceinline__ void laneStore(const Ctx<Row>& c, CandSlots<FAST, Row>& s, int k, int nCand, int pos, Row F,
+                                          Row R, Row S, double ll, Row X) {
+    constexpr int W = CandSlots<FAST, Row>::W;
+    if (k >= nCand - W) {
+        const int t = threadIdx.x, q = k % W;
+        s.F[q][t] = F; s.R[q][t] = R; s.S[q][t] = S;
+        if (FAST) s.X[q][t] = X;
+        s.LL[q][t] = ll; s.pos[q][t] = pos;
         return;
     }
     candStore(c, k, pos, F, R, S, ll, X);
+}
+
+/* push entry k = nCand at the top: the entry k - W leaves the window for the global scratch */
+template <bool FAST, class Row>
+__device__ __forceinline__ void lanePush(const Ctx<Row>& c, CandSlots<FAST, Row>& s, int k, int pos, Row F, Row R,
+                                         Row S, double ll, Row X) {
+    constexpr int W = CandSlots<FAST, Row>::W;
+    if (k >= W) {
+        const SmemCandT<Row> o = slotLoad(s, k % W);
+        candStore(c, k - W, o.pos, o.F, o.R, o.S, o.LL, o.X);
+    }
+    laneStore(c, s, k, k + 1, pos, F, R, S, ll, X);
 }
 
 /* start findFwdBackSMEMS / findBackFwdSMEMS at f; false if it ends at once (ambiguous base) */
@@ -544,7 +565,7 @@ __device__ __forceinline__ void smemBody(const DevIndex& ix, const SmemArgs& a) 
             if (nxOk) {
                 pos = nxPos; P = nxF; Q = nxR; S = nxS; LL = nxLL; X = FAST ? nxX : -1;
             } else {
-                const SmemCandT<Row> cd = laneLoad(c, slots, l.k);
+                const SmemCandT<Row> cd = laneLoad(c, slots, l.k, l.nCand);
                 pos = cd.pos; P = cd.F; Q = cd.R; S = cd.S; LL = cd.LL;
                 X = FAST ? cd.X : -1;
             }
@@ -555,7 +576,7 @@ __device__ __forceinline__ void smemBody(const DevIndex& ix, const SmemArgs& a) 
              * this pass stored it */
             nxOk = l.k > l.lo || l.w < l.nCand - 1;
             if (nxOk) {
-                const SmemCandT<Row> cd = laneLoad(c, slots, l.k > l.lo ? l.k - 1 : l.nCand - 1);
+                const SmemCandT<Row> cd = laneLoad(c, slots, l.k > l.lo ? l.k - 1 : l.nCand - 1, l.nCand);
                 nxPos = cd.pos; nxF = cd.F; nxR = cd.R; nxS = cd.S; nxLL = cd.LL; nxX = cd.X;
             }
         }
@@ -612,7 +633,7 @@ __device__ __forceinline__ void smemBody(const DevIndex& ix, const SmemArgs& a) 
                 gatePush(c, l, LL0, fwd ? 0 : l.cTo, fwd ? l.cFrom : l.L, llA, llC, llG, llT)) {
                 exact0();
                 const int pp = fwd ? l.cTo : l.cFrom;
-                laneStore(c, slots, l.nCand++, pp, F0, R0, S0, LL0, (Row)-1);
+                lanePush(c, slots, l.nCand++, pp, F0, R0, S0, LL0, (Row)-1);
                 /* the last push heads the phase-2 list if phase 1 ends here */
                 nxPos = pp; nxF = F0; nxR = R0; nxS = S0; nxLL = LL0; nxX = -1;
                 nxOk = true;
@@ -646,7 +667,7 @@ __device__ __forceinline__ void smemBody(const DevIndex& ix, const SmemArgs& a) 
                     nxPos = pos; nxF = P; nxR = Q; nxS = S; nxLL = LL; nxX = X;
                     nxOk = true;
                 }
-                laneStore(c, slots, l.w--, pos, P, Q, S, LL, X);
+                laneStore(c, slots, l.w--, l.nCand, pos, P, Q, S, LL, X);
             }
             l.k--;
             if (l.k < l.lo) {
