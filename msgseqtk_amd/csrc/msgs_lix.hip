@@ -466,7 +466,7 @@ __device__ __forceinline__ uint64_t readKey(const uint8_t* s) {
 constexpr int kBlock = 16;
 constexpr int kFilterLanes = 8;   /* lanes (windows in flight) per read in k_window1 */
 
-__global__ void __launch_bounds__(256) k_exact(LixView m, DevIndex ix, const uint8_t* seqs, const int64_t* offs,
+__global__ void __launch_bounds__(256, 4) k_exact(LixView m, DevIndex ix, const uint8_t* seqs, const int64_t* offs,
                                                int64_t nReads, int32_t minLen, double T, int32_t gate1,
                                                uint8_t* need, SmemRec* exact, unsigned long long* stats) {
     __shared__ short W1b[16 * 64];
@@ -598,7 +598,7 @@ __device__ __forceinline__ int64_t windowStart(int64_t t, int64_t nw) {
     return (t & 1) ? mid - (t + 1) / 2 : mid + t / 2;
 }
 
-__global__ void __launch_bounds__(256) k_window1(LixView m, DevIndex ix, const uint8_t* seqs, const int64_t* offs,
+__global__ void __launch_bounds__(256, 3) k_window1(LixView m, DevIndex ix, const uint8_t* seqs, const int64_t* offs,
                                                  int64_t nReads, int32_t minLen, double T, uint8_t* need,
                                                  int64_t* open, unsigned long long* nOpen,
                                                  unsigned long long* stats) {
@@ -647,7 +647,7 @@ __global__ void __launch_bounds__(256) k_window1(LixView m, DevIndex ix, const u
     }
 }
 
-__global__ void __launch_bounds__(256) k_window2(LixView m, DevIndex ix, const uint8_t* seqs, const int64_t* offs,
+__global__ void __launch_bounds__(256, 3) k_window2(LixView m, DevIndex ix, const uint8_t* seqs, const int64_t* offs,
                                                  int32_t minLen, double T, uint8_t* need, const int64_t* open,
                                                  const unsigned long long* nOpen, unsigned long long* stats) {
     __shared__ short W1b[16 * 64];
