@@ -227,6 +227,12 @@ def main():
                               "whole": S[:, 7].sum() / a.steps / 1000},
         "dp_jobs_per_step": {"all": float(S[:, 21].mean()), "wavefront_fill": float(S[:, 22].mean()),
                              "serial_fill": float(S[:, 32].mean())},
+        # rectangles of the jobs filled 16 lanes each: the step loop's lane use is their cells over
+        # 16 x (C + rows - 1) lane-steps per strip of at most 16 rows
+        "dp_fill16_rects": {"per_step": float(S[:, 38].mean()),
+                            "cells_per_rect": float(S[:, 39].sum() / max(1.0, S[:, 38].sum())),
+                            "strips_per_rect": float(S[:, 41].sum() / max(1.0, S[:, 38].sum())),
+                            "step_lane_use": float(S[:, 39].sum() / max(1.0, S[:, 40].sum()))},
         "index_build_s": round(build_s, 2),
         "learned_index": dict(lix or {}, **({"exact_path_reads_frac": float(S[:, 26].sum() / max(1.0, reads)),
                                               "filtered_reads_frac": float(S[:, 27].sum() / max(1.0, reads))}

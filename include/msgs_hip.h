@@ -275,7 +275,9 @@ int msgs_pair_batch(msgs_ctx* ctx, int64_t n, const int64_t* alnOffs, const int6
  * lane, [33] their rectangle cells, [34] the largest one's cells, [35] DP jobs whose
  * GLOBAL last-row scan visits every column (n <= 48), [36] bytes copied host-to-device by the
  * run's ingest (context option "ingest" 1: the batch's bases and qualities from pinned host
- * memory at the start of every run, then its reverse complements), [37] their microseconds. */
+ * memory at the start of every run, then its reverse complements), [37] their microseconds,
+ * [38]..[41] of the DP jobs filled 16 lanes each: rectangles, their cells, the lane-steps of
+ * their strips (16 x (columns + rows - 1) per strip of at most 16 rows) and the strips. */
 int msgs_batch_upload(msgs_ctx* ctx, int32_t slot, const msgs_reads* reads, const msgs_reads* mates);
 int msgs_batch_run(msgs_ctx* ctx, int32_t slot, const msgs_params* p, int32_t emitSam);
 /* msgs_batch_run in two halves: submit returns at once; a worker thread of the slot runs

@@ -37,6 +37,8 @@
  */
 #include "msgs_kernels.h"
 
+#include <type_traits>
+
 namespace {
 
 constexpr int32_t NEG = -1073741824;
@@ -64,28 +66,47 @@ __device__ __forceinline__ int32_t laneUp1(int32_t v) { return __builtin_amdgcn_
 __device__ __forceinline__ int32_t rowUp1(int32_t first, int32_t v) {
     return __builtin_amdgcn_update_dpp(first, v, 0x111, 0xf, 0xf, false);
 }
-/* within each 16-lane DPP row: the value of lane + 1 (row_ror:15, lane 15 taking lane 0's) */
-__device__ __forceinline__ int32_t rowNext(int32_t v) { return __builtin_amdgcn_update_dpp(v, v, 0x12f, 0xf, 0xf, false); }
+/* c ? a : b as one v_cndmask on the condition's lane mask (kept from becoming a branch) */
+__device__ __forceinline__ int32_t selectI(bool c, int32_t a, int32_t b) {
+    int32_t r;
+    const uint64_t m = __builtin_amdgcn_ballot_w64(c);
+    asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(b), "v"(a), "s"(m));
+    return r;
+}
+/* acc * 2 + c: one decision bit shifted in at the bottom, by an add with carry whose carry-in
+ * is the condition's lane mask (one instruction per bit) */
+__device__ __forceinline__ uint32_t pushBit(uint32_t acc, bool c) {
+    uint32_t r;
+    uint64_t co;
+    const uint64_t m = __builtin_amdgcn_ballot_w64(c);
+    asm("v_addc_co_u32_e64 %0, %1, %2, %2, %3" : "=v"(r), "=s"(co) : "v"(acc), "s"(m));
+    return r;
+}
 
 /* Backtrace decisions of a rectangle are packed per row and per sweep step: the fill sweeps
  * a strip of `rows` (<= 64) rows and C columns in C + rows - 1 anti-diagonal steps, and the
- * lane of row l decides cell (l, jj) at step t = jj + l.  Each lane packs the decisions of
- * `spw` consecutive steps into one 64-bit word (GLOBAL: 16 steps x 4 bits; LOCAL: 8 steps x
- * 8 bits, bit 4 being M > 0), and the lanes store their words side by side, so that every
- * store of the strip is one contiguous run of rows x 8 bytes: cell (l, jj) is in word
- * ((jj + l) / spw) * rows + l at bit ((jj + l) % spw) * bits.  Strips of H rows follow each
- * other (H = 64 for jobs filled one per wavefront, 16 for jobs filled 16 lanes each); a strip
- * of `rows` rows takes ceil((C + rows - 1) / spw) * rows words. */
-__device__ __forceinline__ int decShift(int mode) { return mode == 0 ? 4 : 3; }   /* log2 of steps per word */
-__device__ __forceinline__ int decBits(int mode) { return mode == 0 ? 4 : 8; }
-__device__ __forceinline__ int64_t stripWords(int64_t rows, int64_t cols, int mode) {
-    const int sh = decShift(mode);
-    return ((cols + rows - 1 + (1 << sh) - 1) >> sh) * rows;
+ * lane of row l decides cell (l, jj) at step t = jj + l.  Each lane packs the decisions of 8
+ * consecutive steps into one word, and the lanes store their words side by side, so that
+ * every store of the strip is one contiguous run of rows words: cell (l, jj) is in word
+ * ((jj + l) / 8) * rows + l.  GLOBAL: 32-bit words of 4-bit decisions, the first step of a
+ * word in its top nibble (step s at bit 4 * (7 - s % 8)), as the fill shifts each decision bit
+ * in at the bottom; LOCAL: 64-bit words of 8-bit decisions (bit 4 being M > 0), step s at bit
+ * 8 * (s % 8).  Strips of H rows follow each other (H = 64 for jobs filled one per wavefront,
+ * 16 for jobs filled 16 lanes each); a strip of `rows` rows takes ceil((C + rows - 1) / 8) *
+ * rows words. */
+constexpr int DEC_SH = 3;   /* log2 of steps per word */
+__device__ __forceinline__ int decBytes(int mode) { return mode == 0 ? 4 : 8; }   /* bytes per word */
+__device__ __forceinline__ int decShiftOf(int mode, int32_t t) {
+    return mode == 0 ? 4 * (7 - (t & 7)) : 8 * (t & 7);
+}
+template <int MODE> using DecWord = typename std::conditional<MODE == 0, uint32_t, uint64_t>::type;
+__device__ __forceinline__ int64_t stripWords(int64_t rows, int64_t cols) {
+    return ((cols + rows - 1 + 7) >> DEC_SH) * rows;
 }
 /* hs = log2 of the strip height H */
 __device__ __forceinline__ int64_t stripBytes(int64_t rows, int64_t cols, int mode, int hs) {
     const int64_t full = rows >> hs, rem = rows & ((1 << hs) - 1);
-    return 8 * (full * stripWords(1 << hs, cols, mode) + (rem ? stripWords(rem, cols, mode) : 0));
+    return decBytes(mode) * (full * stripWords(1 << hs, cols) + (rem ? stripWords(rem, cols) : 0));
 }
 
 
@@ -106,18 +127,17 @@ struct Rc {
     int64_t off;               /* first cell in the job's decision (or tile) region */
     int64_t boff;              /* first cell in the job's border region */
     bool ok;
-    __device__ int64_t idx(int32_t i, int32_t j) const { return off + (int64_t)(j - c0) * (r1 - r0 + 1) + (i - r0); }
-    __device__ bool hasCol(int32_t j) const { return j >= c0 && j <= c1; }
-    __device__ int64_t area() const { return ok ? (int64_t)(r1 - r0 + 1) * (c1 - c0 + 1) : 0; }
+    __device__ __forceinline__ int64_t idx(int32_t i, int32_t j) const { return off + (int64_t)(j - c0) * (r1 - r0 + 1) + (i - r0); }
+    __device__ __forceinline__ bool hasCol(int32_t j) const { return j >= c0 && j <= c1; }
+    __device__ __forceinline__ int64_t area() const { return ok ? (int64_t)(r1 - r0 + 1) * (c1 - c0 + 1) : 0; }
     /* the word holding the decisions of cell (i, j) and their bit position (see stripBytes) */
-    __device__ int64_t decWord(int32_t i, int32_t j, int mode, int hs, int& shift) const {
+    __device__ __forceinline__ int64_t decWord(int32_t i, int32_t j, int mode, int hs, int& shift) const {
         const int32_t H = 1 << hs;
         const int32_t ri = i - r0, jj = j - c0, s0 = ri & ~(H - 1), l = ri & (H - 1);
         const int32_t rows = min(H, r1 - r0 + 1 - s0), C = c1 - c0 + 1;
-        const int sh = decShift(mode);
         const int32_t t = jj + l;
-        shift = (t & ((1 << sh) - 1)) * decBits(mode);
-        return (off >> 3) + (int64_t)(s0 >> hs) * stripWords(H, C, mode) + (int64_t)(t >> sh) * rows + l;
+        shift = decShiftOf(mode, t);
+        return off / decBytes(mode) + (int64_t)(s0 >> hs) * stripWords(H, C) + (int64_t)(t >> DEC_SH) * rows + l;
     }
 };
 
@@ -365,6 +385,15 @@ __global__ void __launch_bounds__(256) dp_setup_kernel(DevIndex ix, DpArgs a) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         a.tileNeed[a.nJobs] = 0; a.pathNeed[a.nJobs] = 0; a.borderNeed[a.nJobs] = 0; a.serialNeed[a.nJobs] = 0;
     }
+    /* shape counters of the 16-lane jobs' rectangles (when a.dstat is set) */
+    unsigned long long nRect = 0, nCell = 0, nLaneStep = 0, nStrip = 0;
+    auto shape = [&](int64_t rows, int64_t cols) {
+        const int64_t strips = (rows + 15) >> 4;
+        nRect += 1;
+        nCell += (unsigned long long)(rows * cols);
+        nLaneStep += (unsigned long long)(16 * (strips * (cols - 1) + rows));
+        nStrip += (unsigned long long)strips;
+    };
     for (int64_t job = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; job < a.nJobs;
          job += (int64_t)gridDim.x * blockDim.x) {
         const int32_t rid = a.jobRead[job];
@@ -448,6 +477,15 @@ __global__ void __launch_bounds__(256) dp_setup_kernel(DevIndex ix, DpArgs a) {
             atomicAdd(&a.dstat[1], (unsigned long long)cellsAll);
             atomicMax(&a.dstat[2], (unsigned long long)cellsAll);
         }
+        if (a.dstat && g.cls == CLS_WAVE16) {
+            if (g.r5r1 >= 1 && g.r5c1 >= 1) shape(g.r5r1, g.r5c1);
+            for (int k = 0; k + 1 < ns; ++k) {
+                const int64_t rows = seeds[idx[k + 1]].from - seeds[idx[k]].to;
+                const int64_t cols = seeds[idx[k + 1]].start - seeds[idx[k]].end;
+                if (rows > 0 && cols > 0) shape(rows, cols);
+            }
+            if (g.r3r0 <= g.L) shape(g.L - g.r3r0 + 1, g.tLen - g.r3c0 + 1);
+        }
         a.geo[job] = g;
         a.jobCls[job] = g.cls;
         const bool wave = g.cls == CLS_WAVE || g.cls == CLS_WAVE16;
@@ -455,6 +493,20 @@ __global__ void __launch_bounds__(256) dp_setup_kernel(DevIndex ix, DpArgs a) {
         a.pathNeed[job] = L + g.tLen + 3;
         a.borderNeed[job] = wave ? bord : 0;
         a.serialNeed[job] = cellsAll;
+    }
+    if (a.dstat) {
+        for (int off = 32; off >= 1; off >>= 1) {
+            nRect += __shfl_xor(nRect, off, 64);
+            nCell += __shfl_xor(nCell, off, 64);
+            nLaneStep += __shfl_xor(nLaneStep, off, 64);
+            nStrip += __shfl_xor(nStrip, off, 64);
+        }
+        if ((threadIdx.x & 63) == 0 && nRect) {
+            atomicAdd(&a.dstat[4], nRect);
+            atomicAdd(&a.dstat[5], nCell);
+            atomicAdd(&a.dstat[6], nLaneStep);
+            atomicAdd(&a.dstat[7], nStrip);
+        }
     }
 }
 
@@ -511,10 +563,9 @@ __global__ void __launch_bounds__(64) dp_fill_kernel(DevIndex ix, DpArgs a) {
                 int32_t pM = NEG, pI = NEG, pD = NEG;
                 const int32_t steps = C + rows - 1;
                 /* this strip's decision words, packed per lane over spw steps */
-                uint64_t* const dw = (uint64_t*)G.bits + (R.off >> 3) + (int64_t)(s0 >> 6) * stripWords(64, C, MODE);
-                constexpr int dsh = MODE == 0 ? 4 : 3, dbits = MODE == 0 ? 4 : 8;
-                const int32_t dmask = (1 << dsh) - 1;
-                uint64_t acc = 0;
+                DecWord<MODE>* const dw = (DecWord<MODE>*)G.bits + R.off / (int64_t)sizeof(DecWord<MODE>) +
+                                          (int64_t)(s0 >> 6) * stripWords(64, C);
+                DecWord<MODE> acc = 0;
                 for (int32_t t = 0; t < steps; ++t) {
                     /* up = (i-1, j): row above, one step ago; diagonal = (i-1, j-1): two steps ago */
                     int32_t uM = laneUp1(cM), uI = laneUp1(cI), uD = laneUp1(cD);
@@ -548,7 +599,7 @@ __global__ void __launch_bounds__(64) dp_fill_kernel(DevIndex ix, DpArgs a) {
                                              ((max(cM + G.o, NEG) > max(cD - G.ge, NEG)) ? 0u : 8u) |
                                              ((MODE == 1 && m > 0) ? 16u : 0u);
                         cM = m; cI = nI; cD = nD;
-                        acc |= (uint64_t)dec << ((t & dmask) * dbits);
+                        acc |= (DecWord<MODE>)dec << decShiftOf(MODE, t);
                         if (i == R.r1) G.border[R.boff + jj] = make_int4(m, nI, nD, 0);
                         if (jj == C - 1) G.border[R.boff + C + (i - R.r0)] = make_int4(m, nI, nD, 0);
                         if (more && lane == rows - 1) { sTM[jj + 1] = m; sTI[jj + 1] = nI; sTD[jj + 1] = nD; }
@@ -559,10 +610,10 @@ __global__ void __launch_bounds__(64) dp_fill_kernel(DevIndex ix, DpArgs a) {
                             if (nI > nv) { nv = nI; nc = j; nst = OP_I; }
                         }
                     }
-                    /* every spw steps (and at the strip's end) the lanes store their words side
-                     * by side: one contiguous run of rows x 8 bytes */
-                    if ((t & dmask) == dmask || t == steps - 1) {
-                        if (act) dw[(int64_t)(t >> dsh) * rows + lane] = acc;
+                    /* every 8 steps (and at the strip's end) the lanes store their words side
+                     * by side: one contiguous run of rows words */
+                    if ((t & 7) == 7 || t == steps - 1) {
+                        if (act) dw[(int64_t)(t >> DEC_SH) * rows + lane] = acc;
                         acc = 0;
                     }
                 }
@@ -624,16 +675,15 @@ template __global__ void dp_fill_kernel<1>(DevIndex, DpArgs);
  * side by side.  Strips are 16 rows (lane = row), swept along anti-diagonals as in
  * dp_fill_kernel; a 16-row strip leaves fewer lanes idle on its ramp diagonals and on partial
  * strips than a 64-row one.  Per 16-lane job, in LDS: the seeds and the row above the current
- * strip (M, I, D and the target bases).  The row lane 0 reads, one entry per step, is kept in a
- * register chunk of 16 entries that rotates by one lane each step (reloaded every 16 steps),
- * and lane 0 takes it as the out-of-row value of the row_shr shift that brings the other lanes
- * their upper neighbours; the target base reaches lane l one step after lane l - 1 the same
- * way.  Each group runs a small state machine (next job, next rectangle, next strip, step) in
+ * strip (M, I, D and the target bases).  Lane 0 reads the row's entry for its column from
+ * LDS every step and takes it as the out-of-row value of the row_shr shift that brings the
+ * other lanes their upper neighbours; the target base reaches lane l one step after lane l - 1
+ * the same way.  Each group runs a small state machine (next job, next rectangle, next strip, step) in
  * one loop, so the groups need not be in the same phase.  The values, decisions and summaries
  * are those of dp_fill_kernel. */
 template <int MODE>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) dp_fill16_kernel(DevIndex ix, DpArgs a) {
-    constexpr int W = COLS16 + 32;   /* chunk reads reach 31 entries past a row of COLS16 */
+    constexpr int W = COLS16 + 32;   /* the row is read up to entry C + rows (<= COLS16 + 16) */
     __shared__ int4 sSG[4][NS16];
     __shared__ int32_t sTM[4][W], sTI[4][W], sTD[4][W];
     __shared__ uint8_t sTg[4][W];
@@ -643,9 +693,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) dp
     int32_t* const TI = sTI[grp];
     int32_t* const TD = sTD[grp];
     uint8_t* const TG = sTg[grp];
-    constexpr int dsh = MODE == 0 ? 4 : 3, dbits = MODE == 0 ? 4 : 8;
-    constexpr int32_t dmask = (1 << dsh) - 1;
     const int32_t match = a.match, mis = a.mismatch, o = -(a.gapO + a.gapE), ge = a.gapE;
+    const uint32_t mm = (uint32_t)(match + mis);
 
     enum { FETCH, RECT, STRIP, STEP, DONE };
     int phase = FETCH;
@@ -660,10 +709,11 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) dp
     Rc R{};
     int32_t Rn = 0, C = 0, s0 = 0, rows = 0, t = 0, steps = 0, i = 0;
     int32_t cM = NEG, cI = NEG, cD = NEG, pM = NEG, pI = NEG, pD = NEG;
-    int32_t chM = NEG, chI = NEG, chD = NEG, chT = 0, tbv = 0, qm = 0;
+    int32_t tbv = 0;
+    uint32_t mq = 0;   /* bit b: the lane's read base matches target code b (b & base != 0) */
     int32_t bM0 = NEG, bI0 = NEG, bD0 = NEG, cm = NEG, ci = NEG, cd = NEG;
-    uint64_t acc = 0;
-    uint64_t* dw = nullptr;
+    DecWord<MODE> acc = 0;
+    DecWord<MODE>* dw = nullptr;
     int32_t bv = NEG, bi = 0, bj = 0, sbv = NEG, sbj = 0;   /* LOCAL: best cell (lane's rows) */
     int32_t nv = NEG, nc = 0, nst = OP_M;                   /* GLOBAL: scan of R3's last row */
 
@@ -756,13 +806,14 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) dp
                 }
                 bM0 = cM; bI0 = cI; bD0 = cD;
                 const uint8_t qb = act ? q[i - 1] : (uint8_t)0;
-                qm = qb == 15 ? 0 : qb;
+                const uint32_t qm = qb == 15 ? 0u : qb;
+                mq = ((qm & 1u) ? 0xAAAAu : 0u) | ((qm & 2u) ? 0xCCCCu : 0u) | ((qm & 4u) ? 0xF0F0u : 0u) |
+                     ((qm & 8u) ? 0xFF00u : 0u);
                 pM = TM[0]; pI = TI[0]; pD = TD[0];
-                chM = TM[1 + l]; chI = TI[1 + l]; chD = TD[1 + l]; chT = TG[l];
                 tbv = 0;
                 t = 0;
                 steps = C + rows - 1;
-                dw = (uint64_t*)G.bits + (R.off >> 3) + (int64_t)(s0 >> 4) * stripWords(16, C, MODE);
+                dw = (DecWord<MODE>*)G.bits + R.off / (int64_t)sizeof(DecWord<MODE>) + (int64_t)(s0 >> 4) * stripWords(16, C);
                 acc = 0;
                 sbv = NEG; sbj = 0;
                 phase = STEP;
@@ -772,48 +823,68 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) dp
          * that no group changes phase inside the loop ---- */
         int32_t rem = phase == STEP ? steps - t : 0x7fffffff;
         rem = min(min(__shfl(rem, 0, 64), __shfl(rem, 16, 64)), min(__shfl(rem, 32, 64), __shfl(rem, 48, 64)));
+        rem = __builtin_amdgcn_readfirstlane(rem);   /* the same in every lane: scalar loop control */
         if (rem == 0x7fffffff) continue;   /* no group stepping: all are out of jobs */
         if (phase == STEP) {
-            for (int32_t r = 0; r < rem; ++r) {
-                if ((t & 15) == 0 && t > 0) {
-                    /* lane k of the chunk: row entry t + 1 + k (M, I, D), target entry t + k */
-                    chM = TM[t + 1 + l]; chI = TI[t + 1 + l]; chD = TD[t + 1 + l]; chT = TG[t + l];
-                }
+            /* lane 0's entries of the row above the strip, from LDS one step ahead: entry t + 1
+             * (and target base t) for step t.  The bottom lane has written the strip's own row
+             * only up to entry t - rows + 1 when step t reads entry t + 2 */
+            int32_t aM = TM[t + 1], aI = TI[t + 1], aD = TD[t + 1], aT = TG[t];
+            auto step = [&]() {
                 /* up = (i - 1, j): row above, one step ago (lane 0: the row above the strip);
                  * diagonal = (i - 1, j - 1): the up value of the step before */
-                const int32_t uM = rowUp1(chM, cM), uI = rowUp1(chI, cI), uD = rowUp1(chD, cD);
-                tbv = rowUp1(chT, tbv);
+                const int32_t uM = rowUp1(aM, cM), uI = rowUp1(aI, cI), uD = rowUp1(aD, cD);
+                tbv = rowUp1(aT, tbv);
+                aM = TM[t + 2]; aI = TI[t + 2]; aD = TD[t + 2]; aT = TG[t + 1];
                 const int32_t dM = pM, dI = pI, dD = pD;
                 pM = uM; pI = uI; pD = uD;
                 const int32_t jj = t - l;
                 const bool valid = act && (uint32_t)jj < (uint32_t)C;
-                const int32_t sc = (qm & tbv) ? match : -mis;
                 /* the recurrences of dp_fill_kernel: -inf (NEG) absorbs, gaps are non-negative,
                  * and every finite value lies far above NEG, so the comparisons of the
-                 * backtrace's decisions need no clamping */
+                 * backtrace's decisions need no clamping.  Score: -mis, plus match + mis where
+                 * the bases match (bit tbv of mq) */
                 const int32_t mx = max3(dM, dD, dI);
-                int32_t m = mx == NEG ? NEG : mx + sc;
+                const int32_t sv = mx - mis + (int32_t)__umul24(__builtin_amdgcn_ubfe(mq, (uint32_t)tbv, 1), mm);
+                int32_t m = selectI(mx == NEG, NEG, sv);
                 if (MODE == 1) m = max(m, 0);
                 const int32_t a1 = uM + o, b1 = uI - ge, a2 = cM + o, b2 = cD - ge;
                 const int32_t nI = max3(a1, b1, NEG), nD = max3(a2, b2, NEG);
                 /* matchMax with its quirk: I wins over M, otherwise D if above M */
-                uint32_t dec = dI > dM ? 1u : (dD > dM ? 2u : 0u);
-                dec |= (a1 > b1 ? 0u : 4u) | (a2 > b2 ? 0u : 8u);
-                if (MODE == 1) dec |= m > 0 ? 16u : 0u;
-                /* decisions enter the word from the top; a full word has step t & ~dmask in
-                 * its low bits, and older steps have been shifted out */
-                acc = (acc >> dbits) | ((uint64_t)dec << (64 - dbits));
+                if constexpr (MODE == 0) {
+                    /* the nibble D-ext, I-ext, M source (2 bits) shifted in bit by bit: after 8
+                     * steps the word holds 8 nibbles, the first step's on top */
+                    const bool iw = dI > dM;
+                    acc = pushBit(acc, a2 <= b2);
+                    acc = pushBit(acc, a1 <= b1);
+                    acc = pushBit(acc, dD > selectI(iw, INT32_MAX, dM));   /* D above M, I not */
+                    acc = pushBit(acc, iw);
+                } else {
+                    uint32_t dec = dI > dM ? 1u : (dD > dM ? 2u : 0u);
+                    dec |= (a1 > b1 ? 0u : 4u) | (a2 > b2 ? 0u : 8u) | (m > 0 ? 16u : 0u);
+                    /* bytes enter the word from the top: after 8 steps the first step's is at
+                     * the bottom */
+                    acc = (acc >> 8) | ((uint64_t)dec << 56);
+                }
                 cM = valid ? m : cM;
                 cI = valid ? nI : cI;
                 cD = valid ? nD : cD;
                 if (MODE == 1 && valid && m > sbv) { sbv = m; sbj = R.c0 + jj; }
                 if (valid && l == rows - 1) { TM[jj + 1] = m; TI[jj + 1] = nI; TD[jj + 1] = nD; }
-                chM = rowNext(chM); chI = rowNext(chI); chD = rowNext(chD); chT = rowNext(chT);
-                const int32_t pos = t & dmask;
-                if (act && (pos == dmask || t == steps - 1))
-                    dw[(int64_t)(t >> dsh) * rows + l] = acc >> ((dmask - pos) * dbits);
+                /* every 8 steps and at the strip's end the lanes store their words side by side
+                 * (a partial word aligned as a full one) */
+                const int32_t pos = t & 7;
+                if (act && (pos == 7 || t == steps - 1)) {
+                    if constexpr (MODE == 0) dw[(int64_t)(t >> DEC_SH) * rows + l] = acc << (4 * (7 - pos));
+                    else dw[(int64_t)(t >> DEC_SH) * rows + l] = acc >> (8 * (7 - pos));
+                }
                 ++t;
-            }
+            };
+            /* two steps per iteration: the registers of the values carried from one step to
+             * the next alternate instead of being copied */
+            int32_t r = 0;
+            for (; r + 2 <= rem; r += 2) { step(); step(); }
+            if (r < rem) step();
             if (t == steps) {
                 /* strip end: the lanes' last-column cells, the corner of the next strip's top
                  * row, the LOCAL best of each lane */
@@ -860,9 +931,10 @@ template __global__ void dp_fill16_kernel<1>(DevIndex, DpArgs);
 /* ------------------------------------------------------------------ trace */
 
 /* at least 4 waves per SIMD (at most 128 VGPRs).  The band lookups (vM / vI / vD, cellRect,
- * rowSeg, ...) are inlined, so the job geometry G stays in registers instead of private
- * memory that every backtrace step would wait on; 4 waves per SIMD leave room for it
- * (5 spill, 3 hide less latency) */
+ * rowSeg, Rc's methods, ...) are inlined, so the job geometry G stays in registers instead of
+ * private memory that every backtrace step would wait on.  At 4 waves 36 VGPRs per lane
+ * spill, and the kernel still runs faster than at 3 waves without spills (2.35 vs 2.64 ms
+ * per 256k-read batch of the 4 Gbp workload): more waves hide more of the memory waits */
 __global__ void __launch_bounds__(256, 4) dp_trace_kernel(DevIndex ix, DpArgs a) {
     /* the quality terms of evaluate, by quality value, in LDS (one read per base) */
     __shared__ double sQM[256], sQX[256], sQG[256];
@@ -1031,7 +1103,10 @@ __global__ void __launch_bounds__(256, 4) dp_trace_kernel(DevIndex ix, DpArgs a)
             if (inR) {
                 int sft;
                 const int64_t wi = Rb.decWord(i, j, G.mode, G.hs, sft);
-                if (wi != decWi) { decWi = wi; decW = ((const uint64_t*)G.bits)[wi]; }
+                if (wi != decWi) {
+                    decWi = wi;
+                    decW = G.mode == 0 ? (uint64_t)((const uint32_t*)G.bits)[wi] : ((const uint64_t*)G.bits)[wi];
+                }
                 dec = (uint32_t)(decW >> sft) & (G.mode == 0 ? 15u : 255u);
             }
             const bool go_on = G.mode == 0 ? i > 0 : (s != OP_M || (inR ? (dec & 16u) != 0 : vM(G, i, j, T) > 0));

@@ -211,7 +211,9 @@ struct DpArgs {
     double gapE_d;
     int32_t forceSerial;       /* every job filled serially (context option dp_serial) */
     unsigned long long* dstat; /* optional counters (NULL: none): serially filled jobs, their cells,
-                                * the largest one's cells, full last-row scans */
+                                * the largest one's cells, full last-row scans; of the jobs filled 16
+                                * lanes each: rectangles, their cells, lane-steps of their strips
+                                * (16 x (C + rows - 1) per strip), strips */
     int64_t* tileNeed;         /* rectangle cells */
     int64_t* pathNeed;
     int64_t* borderNeed;       /* border cells */

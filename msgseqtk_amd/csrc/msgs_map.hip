@@ -1282,9 +1282,9 @@ extern "C" int msgs_batch_submit(msgs_ctx* c, int32_t slot, const msgs_params* p
 static int batchGpu(msgs_ctx* c, Slot& S, std::chrono::steady_clock::time_point t0) {
     const msgs_params* p = &S.params;
     hipError_t e = hipSuccess;
-    unsigned long long* dStats = c->x().bufs.get<unsigned long long>("stats", 16, e);
+    unsigned long long* dStats = c->x().bufs.get<unsigned long long>("stats", 24, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "stats");
-    CK(hipMemsetAsync(dStats, 0, 16 * 8, c->x().stream), "stats");
+    CK(hipMemsetAsync(dStats, 0, 24 * 8, c->x().stream), "stats");
     std::shared_ptr<BatchResult> b = S.res;
     double seedUs = 0, smemUs = 0, dpUs = 0, filterUs = 0, ingestUs = 0;
     if (c->ingest && S.dev.total) {
@@ -1331,7 +1331,7 @@ static int batchGpu(msgs_ctx* c, Slot& S, std::chrono::steady_clock::time_point 
         dpUs += k;
         wall[0] = us(s0, s1); wall[1] = us(s1, s2); wall[2] = us(s2, s3); wall[3] = us(s3, s4);
     }
-    unsigned long long hs[16];
+    unsigned long long hs[24];
     CK(hipMemcpyAsync(hs, dStats, sizeof(hs), hipMemcpyDeviceToHost, c->x().stream), "stats");
     CK(hipStreamSynchronize(c->x().stream), "stats");
     if (c->ingest && S.dev.total) {
@@ -1361,6 +1361,7 @@ static int batchGpu(msgs_ctx* c, Slot& S, std::chrono::steady_clock::time_point 
     for (int k = 0; k < 4; ++k) st[32 + k] = (double)hs[10 + k];
     st[36] = c->ingest ? 2.0 * (double)S.dev.total : 0.0;
     st[37] = ingestUs;
+    for (int k = 0; k < 4; ++k) st[38 + k] = (double)hs[14 + k];
     return MSGS_OK;
 }
 
