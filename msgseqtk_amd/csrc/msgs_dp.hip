@@ -222,10 +222,15 @@ __device__ __forceinline__ int4 rectVal(const Geo& G, const Rc& R, int32_t i, in
 }
 
 /* M at (i, j) after events 1..e; (i, j) inside the matrix, k = rowSeg(i) */
+/* RECTS = false: the caller knows that the rectangle after band k is not filled at event e
+ * (it is the one about to be filled), so no rectangle value is looked up (no global reads) */
+template <bool RECTS = true>
 __device__ __forceinline__ int32_t vMk(const Geo& G, int32_t i, int32_t j, int e, int k) {
     if (k < 0) {
-        const Rc R = rectR5(G);
-        if (i >= 1 && R.ok && e >= 1 && R.hasCol(j)) return rectVal(G, R, i, j).x;
+        if constexpr (RECTS) {
+            const Rc R = rectR5(G);
+            if (i >= 1 && R.ok && e >= 1 && R.hasCol(j)) return rectVal(G, R, i, j).x;
+        }
         return initM(G, i, j);
     }
     const int4 s = G.sg[k];
@@ -237,25 +242,30 @@ __device__ __forceinline__ int32_t vMk(const Geo& G, int32_t i, int32_t j, int e
         }
         return initM(G, i, j);
     }
-    const Rc R = rectAfter(G, k, s);
-    if (R.ok && e >= R.ev && R.hasCol(j)) return rectVal(G, R, i, j).x;
+    if constexpr (RECTS) {
+        const Rc R = rectAfter(G, k, s);
+        if (R.ok && e >= R.ev && R.hasCol(j)) return rectVal(G, R, i, j).x;
+    }
     if (e >= 2 + 2 * k && i == s.x + s.z && j == s.y + s.z) return s.w + s.z * G.match;   /* step c */
     return initM(G, i, j);
 }
 
+template <bool RECTS = true>
 __device__ __forceinline__ int32_t vIk(const Geo& G, int32_t i, int32_t j, int e, int k) {
     if (k >= 0) {
         const int4 s = G.sg[k];
         if (i >= s.x + s.z) {
-            const Rc R = rectAfter(G, k, s);
-            if (R.ok && e >= R.ev && R.hasCol(j)) return rectVal(G, R, i, j).y;
+            if constexpr (RECTS) {
+                const Rc R = rectAfter(G, k, s);
+                if (R.ok && e >= R.ev && R.hasCol(j)) return rectVal(G, R, i, j).y;
+            }
             /* I row to_k + 1 := M row to_k + o at seed k's event */
-            if (i == s.x + s.z && e >= 2 + 2 * k) return addn(vMk(G, i - 1, j, 2 + 2 * k, k), G.o);
+            if (i == s.x + s.z && e >= 2 + 2 * k) return addn(vMk<RECTS>(G, i - 1, j, 2 + 2 * k, k), G.o);
         } else if (k >= 1 && e >= 2 * k) {
             const int4 p = G.sg[k - 1];
-            if (i == p.x + p.z) return addn(vMk(G, i - 1, j, 2 * k, k - 1), G.o);
+            if (i == p.x + p.z) return addn(vMk<RECTS>(G, i - 1, j, 2 * k, k - 1), G.o);
         }
-    } else {
+    } else if constexpr (RECTS) {
         const Rc R = rectR5(G);
         if (i >= 1 && R.ok && e >= 1 && R.hasCol(j)) return rectVal(G, R, i, j).y;
     }
@@ -263,24 +273,27 @@ __device__ __forceinline__ int32_t vIk(const Geo& G, int32_t i, int32_t j, int e
 }
 
 /* D at (i, j) after events 1..e; k = rowSeg(i), m = colSeg(j) */
+template <bool RECTS = true>
 __device__ __forceinline__ int32_t vDkm(const Geo& G, int32_t i, int32_t j, int e, int k, int m) {
-    if (k < 0) {
-        const Rc R = rectR5(G);
-        if (i >= 1 && R.ok && e >= 1 && R.hasCol(j)) return rectVal(G, R, i, j).z;
-    } else {
-        const int4 s = G.sg[k];
-        if (i >= s.x + s.z) {
-            const Rc R = rectAfter(G, k, s);
-            if (R.ok && e >= R.ev && R.hasCol(j)) return rectVal(G, R, i, j).z;
+    if constexpr (RECTS) {
+        if (k < 0) {
+            const Rc R = rectR5(G);
+            if (i >= 1 && R.ok && e >= 1 && R.hasCol(j)) return rectVal(G, R, i, j).z;
+        } else {
+            const int4 s = G.sg[k];
+            if (i >= s.x + s.z) {
+                const Rc R = rectAfter(G, k, s);
+                if (R.ok && e >= R.ev && R.hasCol(j)) return rectVal(G, R, i, j).z;
+            }
         }
     }
     /* D col end_m + 1 := M col end_m + o at seed m's event; m is colSeg(j) or the one before */
     if (m >= 0) {
         const int4 s = G.sg[m];
-        if (j == s.y + s.z && e >= 2 + 2 * m) return addn(vMk(G, i, j - 1, 2 + 2 * m, k), G.o);
+        if (j == s.y + s.z && e >= 2 + 2 * m) return addn(vMk<RECTS>(G, i, j - 1, 2 + 2 * m, k), G.o);
         if (m >= 1) {
             const int4 p = G.sg[m - 1];
-            if (j == p.y + p.z && e >= 2 * m) return addn(vMk(G, i, j - 1, 2 * m, k), G.o);
+            if (j == p.y + p.z && e >= 2 * m) return addn(vMk<RECTS>(G, i, j - 1, 2 * m, k), G.o);
         }
     }
     return NEG;
@@ -683,7 +696,7 @@ template __global__ void dp_fill_kernel<1>(DevIndex, DpArgs);
  * are those of dp_fill_kernel. */
 template <int MODE>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) dp_fill16_kernel(DevIndex ix, DpArgs a) {
-    constexpr int W = COLS16 + 32;   /* the row is read up to entry C + rows (<= COLS16 + 16) */
+    constexpr int W = COLS16 + 17;   /* the row is read up to entry C + rows (<= COLS16 + 16) */
     __shared__ int4 sSG[4][NS16];
     __shared__ int32_t sTM[4][W], sTI[4][W], sTD[4][W];
     __shared__ uint8_t sTg[4][W];
@@ -710,6 +723,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) dp
     int32_t Rn = 0, C = 0, s0 = 0, rows = 0, t = 0, steps = 0, i = 0;
     int32_t cM = NEG, cI = NEG, cD = NEG, pM = NEG, pI = NEG, pD = NEG;
     int32_t tbv = 0;
+    uint8_t qNext = 0;   /* the read base of the lane's row in the next strip */
     uint32_t mq = 0;   /* bit b: the lane's read base matches target code b (b & base != 0) */
     int32_t bM0 = NEG, bI0 = NEG, bD0 = NEG, cm = NEG, ci = NEG, cd = NEG;
     DecWord<MODE> acc = 0;
@@ -779,15 +793,28 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) dp
                     C = R.c1 - R.c0 + 1;
                     Rn = R.r1 - R.r0 + 1;
                     const int e = R.ev, kb = ri - 1;
-                    for (int x = l; x <= C; x += 16) {
-                        const int32_t j = R.c0 - 1 + x;
-                        TM[x] = vMk(G, R.r0 - 1, j, e - 1, kb);
-                        TI[x] = vIk(G, R.r0 - 1, j, e - 1, kb);
-                        TD[x] = vDkm(G, R.r0 - 1, j, e - 1, kb, kb);
-                        if (x < C) {
+                    /* the first strip's read bases, fetched with the target bases */
+                    qNext = l < min(16, Rn) ? q[R.r0 + l - 1] : (uint8_t)0;
+                    for (int x0 = 0; x0 <= C; x0 += 64) {
+                        /* four target bases per lane in flight, then the row entries (the row is
+                         * in seed kb's rows: no rectangle is read) */
+                        uint32_t tb4[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const int32_t x = x0 + l + 16 * u;
                             const int64_t tp = g.tStart + (R.c0 + x) - 1;
-                            const uint8_t b = (tp >= 0 && tp <= chromLen) ? tg[tp] : (uint8_t)0;
-                            TG[x] = b == 15 ? (uint8_t)0 : b;
+                            tb4[u] = (x < C && tp >= 0 && tp <= chromLen) ? tg[tp] : 0u;
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const int32_t x = x0 + l + 16 * u;
+                            if (x <= C) {
+                                const int32_t j = R.c0 - 1 + x;
+                                TM[x] = vMk<false>(G, R.r0 - 1, j, e - 1, kb);
+                                TI[x] = vIk<false>(G, R.r0 - 1, j, e - 1, kb);
+                                TD[x] = vDkm<false>(G, R.r0 - 1, j, e - 1, kb, kb);
+                                if (x < C) TG[x] = tb4[u] == 15u ? (uint8_t)0 : (uint8_t)tb4[u];
+                            }
                         }
                     }
                     s0 = 0;
@@ -800,12 +827,15 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) dp
                 const int e = R.ev, kb = ri - 1;
                 cM = NEG; cI = NEG; cD = NEG;
                 if (act) {
-                    cM = vMk(G, i, R.c0 - 1, e - 1, kb);
-                    cI = vIk(G, i, R.c0 - 1, e - 1, kb);
-                    cD = vDkm(G, i, R.c0 - 1, e - 1, kb, kb);
+                    /* the strip's left column, at the event before R's (R itself unfilled) */
+                    cM = vMk<false>(G, i, R.c0 - 1, e - 1, kb);
+                    cI = vIk<false>(G, i, R.c0 - 1, e - 1, kb);
+                    cD = vDkm<false>(G, i, R.c0 - 1, e - 1, kb, kb);
                 }
                 bM0 = cM; bI0 = cI; bD0 = cD;
-                const uint8_t qb = act ? q[i - 1] : (uint8_t)0;
+                /* this strip's read base (fetched a strip ahead), the next strip's in flight */
+                const uint32_t qb = act ? qNext : 0u;
+                qNext = s0 + 16 + l < Rn ? q[i + 16 - 1] : (uint8_t)0;
                 const uint32_t qm = qb == 15 ? 0u : qb;
                 mq = ((qm & 1u) ? 0xAAAAu : 0u) | ((qm & 2u) ? 0xCCCCu : 0u) | ((qm & 4u) ? 0xF0F0u : 0u) |
                      ((qm & 8u) ? 0xFF00u : 0u);
