@@ -610,8 +610,9 @@ int runSmem(msgs_ctx* c, const DevReads& r, const msgs_params* p, SmemStage& out
     hipError_t e = hipSuccess;
     const int64_t nCand = candCap * std::max(lanes, split ? lanesSplit / split : 0);
     const bool wide = c->db.wide;
-    a.cand = wide ? (void*)c->x().bufs.get<SmemCandT<int64_t>>("sm.cand", nCand, e)
-                  : (void*)c->x().bufs.get<SmemCandT<int32_t>>("sm.cand", nCand, e);
+    if (wide && r.maxL >= SMEM_WIDE_MAX_READ - 2)
+        return fail(c, MSGS_E_UNSUPPORTED, "reads of 2^26 bases or more on a 64-bit-row index");
+    a.cand = c->x().bufs.get<uint8_t>("sm.cand", nCand * SMEM_CAND_BYTES, e);
     a.candCap = candCap;
     a.nLanes = lanes;
     a.queue = c->x().bufs.get<unsigned long long>("sm.q", 2, e);

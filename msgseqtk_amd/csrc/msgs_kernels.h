@@ -18,14 +18,20 @@ struct SmemRec {
 };
 
 /* a candidate of a search's lock-step list: bi-interval, loglik, its moving side as a
- * text position (X, -1 when the row values are exact) and its fixed read end (pos);
- * rows of the index's width: 32 bytes for narrow indexes (Row = int32_t), 48 for wide */
+ * text position (X, -1 when the row values are exact) and its fixed read end (pos); rows
+ * of the index's width.  In the kernels' global scratch a candidate is packed into 24 or
+ * 32 bytes (SMEM_CAND_BYTES at most per entry). */
 template <class Row>
 struct SmemCandT {
     Row F, R, S, X;
     double LL;
     int32_t pos, pad;
 };
+
+constexpr int64_t SMEM_CAND_BYTES = 32;
+/* reads of at least this many bases are refused on 64-bit-row indexes (the packed
+ * candidate keeps 26 bits of read position) */
+constexpr int64_t SMEM_WIDE_MAX_READ = (int64_t)1 << 26;
 
 struct SmemArgs {
     const uint8_t* seqs;
@@ -37,7 +43,7 @@ struct SmemArgs {
     int32_t cap;               /* SMEM slots per read */
     int32_t* cnt;              /* SMEMs emitted per read (may exceed cap) */
     SmemRec* out;              /* [nReads][cap] */
-    void* cand;                /* candidate scratch, [nLanes][candCap] SmemCandT of the index's width */
+    void* cand;                /* candidate scratch, nLanes x candCap entries of SMEM_CAND_BYTES */
     int64_t candCap;
     int64_t nLanes;
     const uint8_t* need;       /* per read (may be NULL), bits 0-1: 0 = the read cannot emit an SMEM,

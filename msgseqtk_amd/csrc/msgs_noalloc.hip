@@ -131,8 +131,9 @@ extern "C" int msgs_smem_batch_noalloc(msgs_ctx* c, const uint8_t* dSeq, const i
     if (kHeader + seqBytes + prefBytes >= kNoAllocScratch)
         return fail(c, MSGS_E_CAP, "reads exceed the allocation-free scratch (read copy and loglik prefix sums)");
     const int64_t candCap = maxL + 2;
-    const bool wide = c->db.wide;
-    const size_t laneBytes = (size_t)candCap * (wide ? sizeof(SmemCandT<int64_t>) : sizeof(SmemCandT<int32_t>));
+    if (c->db.wide && maxL >= SMEM_WIDE_MAX_READ - 2)
+        return fail(c, MSGS_E_UNSUPPORTED, "reads of 2^26 bases or more on a 64-bit-row index");
+    const size_t laneBytes = (size_t)candCap * SMEM_CAND_BYTES;
     const int64_t lanesFit = (int64_t)((kNoAllocScratch - kHeader - seqBytes - prefBytes) / laneBytes) / 256 * 256;
     const int64_t lanesMax = (int64_t)std::max(1, c->nCU) * c->lanesPerCU;
     const int64_t lanes = std::min<int64_t>({std::max<int64_t>(256, (n + 255) / 256 * 256), lanesMax / 256 * 256, lanesFit});
@@ -159,7 +160,7 @@ extern "C" int msgs_smem_batch_noalloc(msgs_ctx* c, const uint8_t* dSeq, const i
         smem_prefix_kernel<<<(unsigned)std::min<int64_t>((n + 255) / 256, 4096), 256, 0, st>>>(c->db.view(), a, pref);
         a.prefLL = pref;
     }
-    if (wide) smem_kernel_t<true, int64_t><<<(unsigned)(lanes / 256), 256, 0, st>>>(c->db.view(), a);
+    if (c->db.wide) smem_kernel_t<true, int64_t><<<(unsigned)(lanes / 256), 256, 0, st>>>(c->db.view(), a);
     else smem_kernel_t<true, int32_t><<<(unsigned)(lanes / 256), 256, 0, st>>>(c->db.view(), a);
     k_na_maxcount<<<g, 256, 0, st>>>(dCounts, n, hdr);
     unsigned long long mx = 0;

@@ -95,16 +95,52 @@ __device__ __forceinline__ size_t cidx(const Ctx<Row>& c, int k) {
     return c.grouped ? (size_t)c.lane * c.a.candCap + k : (size_t)k * c.a.nLanes + c.lane;
 }
 
-template <class Row>
-__device__ __forceinline__ SmemCandT<Row>& candAt(const Ctx<Row>& c, int k) {
-    return static_cast<SmemCandT<Row>*>(c.a.cand)[cidx(c, k)];
+/* A candidate in the global scratch: three 64-bit words, four with X (HX: the kernels whose
+ * lists carry X).  Word 0 and 1 hold F, R, S and pos: 32 bits each for 32-bit rows; for 64-bit
+ * rows (all below 2^34) F in bits 0-33 of word 0, R in bits 34-63 of word 0 and 0-3 of word 1,
+ * S in bits 4-37 and pos in bits 38-63 of word 1 (reads are below 2^26 bases).  Word 2 is LL's
+ * bit pattern, word 3 X + 1. */
+template <bool HX>
+constexpr int candWords() { return HX ? 4 : 3; }
+
+template <bool HX, class Row>
+__device__ __forceinline__ void candStore(const Ctx<Row>& c, int k, int pos, Row F, Row R, Row S, double ll, Row X) {
+    uint64_t* const p = static_cast<uint64_t*>(c.a.cand) + cidx(c, k) * candWords<HX>();
+    uint64_t w0, w1;
+    if constexpr (sizeof(Row) == 8) {
+        w0 = (uint64_t)F | ((uint64_t)R << 34);
+        w1 = ((uint64_t)R >> 30) | ((uint64_t)S << 4) | ((uint64_t)(uint32_t)pos << 38);
+    } else {
+        w0 = (uint64_t)(uint32_t)F | ((uint64_t)(uint32_t)R << 32);
+        w1 = (uint64_t)(uint32_t)S | ((uint64_t)(uint32_t)pos << 32);
+    }
+    p[0] = w0;
+    p[1] = w1;
+    p[2] = (uint64_t)__double_as_longlong(ll);
+    if constexpr (HX) p[3] = (uint64_t)((int64_t)X + 1);
 }
 
-template <class Row>
-__device__ __forceinline__ void candStore(const Ctx<Row>& c, int k, int pos, Row F, Row R, Row S, double ll, Row X) {
+template <bool HX, class Row>
+__device__ __forceinline__ SmemCandT<Row> candAt(const Ctx<Row>& c, int k) {
+    const uint64_t* const p = static_cast<const uint64_t*>(c.a.cand) + cidx(c, k) * candWords<HX>();
+    const uint64_t w0 = p[0], w1 = p[1], w2 = p[2];
     SmemCandT<Row> v;
-    v.F = F; v.R = R; v.S = S; v.X = X; v.LL = ll; v.pos = pos; v.pad = 0;
-    candAt(c, k) = v;
+    if constexpr (sizeof(Row) == 8) {
+        constexpr uint64_t M = (1ull << 34) - 1;
+        v.F = (Row)(w0 & M);
+        v.R = (Row)((w0 >> 34) | ((w1 & 15u) << 30));
+        v.S = (Row)((w1 >> 4) & M);
+        v.pos = (int32_t)(w1 >> 38);
+    } else {
+        v.F = (Row)(uint32_t)w0;
+        v.R = (Row)(uint32_t)(w0 >> 32);
+        v.S = (Row)(uint32_t)w1;
+        v.pos = (int32_t)(w1 >> 32);
+    }
+    v.LL = __longlong_as_double((long long)w2);
+    v.X = HX ? (Row)((int64_t)p[3] - 1) : (Row)-1;
+    v.pad = 0;
+    return v;
 }
 
 /* The per-lane kernels keep the top W entries of each lane's list (indices nCand - W ..
@@ -138,7 +174,7 @@ template <bool FAST, class Row>
 __device__ __forceinline__ SmemCandT<Row> laneLoad(const Ctx<Row>& c, const CandSlots<FAST, Row>& s, int k, int nCand) {
     constexpr int W = CandSlots<FAST, Row>::W;
     if (k >= nCand - W) return slotLoad(s, k % W);
-    return candAt(c, k);
+    return candAt<FAST>(c, k);
 }
 
 template <bool FAST, class Row>
@@ -152,7 +188,7 @@ __device__ __forceinline__ void laneStore(const Ctx<Row>& c, CandSlots<FAST, Row
         s.LL[q][t] = ll; s.pos[q][t] = pos;
         return;
     }
-    candStore(c, k, pos, F, R, S, ll, X);
+    candStore<FAST>(c, k, pos, F, R, S, ll, X);
 }
 
 /* push entry k = nCand at the top: the entry k - W leaves the window for the global scratch */
@@ -162,7 +198,7 @@ __device__ __forceinline__ void lanePush(const Ctx<Row>& c, CandSlots<FAST, Row>
     constexpr int W = CandSlots<FAST, Row>::W;
     if (k >= W) {
         const SmemCandT<Row> o = slotLoad(s, k % W);
-        candStore(c, k - W, o.pos, o.F, o.R, o.S, o.LL, o.X);
+        candStore<FAST>(c, k - W, o.pos, o.F, o.R, o.S, o.LL, o.X);
     }
     laneStore(c, s, k, k + 1, pos, F, R, S, ll, X);
 }
@@ -950,7 +986,7 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
                 if (l.nCand < G) {
                     if (gl == l.nCand) { rPos = pp; rP = F0; rQ = R0; rS = S0; rX = -1; rLL = LL0; }
                 } else if (gl == 0) {
-                    candStore(c, l.nCand, pp, F0, R0, S0, LL0, (Row)-1);
+                    candStore<true>(c, l.nCand, pp, F0, R0, S0, LL0, (Row)-1);
                 }
                 l.nCand++;
             }
@@ -974,7 +1010,7 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
                         l.nValid = 1;
                     } else {
                         /* scratch: the first G pushes join the rest */
-                        candStore(c, gl, rPos, rP, rQ, rS, rLL, rX);
+                        candStore<true>(c, gl, rPos, rP, rQ, rS, rLL, rX);
                         l.nValid = 0;
                     }
                 }
@@ -992,7 +1028,7 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
             if (regs) {
                 if (act) { pos = rPos; P = rP; Q = rQ; S = rS; X = rX; LL = rLL; }
             } else if (act) {
-                const SmemCandT<Row> cd = candAt(c, l.lo - i);
+                const SmemCandT<Row> cd = candAt<true>(c, l.lo - i);
                 pos = cd.pos; P = cd.F; Q = cd.R; S = cd.S; X = cd.X; LL = cd.LL;
             }
             /* a lone unique candidate walks the text; the whole group follows its boundary */
@@ -1055,7 +1091,7 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
                 rP = permuteRow(dst, P); rQ = permuteRow(dst, Q); rS = permuteRow(dst, S); rX = permuteRow(dst, X);
                 rLL = permuteF64(dst, LL);
             } else if (keep) {
-                candStore(c, l.lo - (l.w + __popcll(kMask & below)), pos, P, Q, S, LL, X);
+                candStore<true>(c, l.lo - (l.w + __popcll(kMask & below)), pos, P, Q, S, LL, X);
             }
             l.nOut += __popcll(eMask);
             l.w += nKeep;
@@ -1073,7 +1109,7 @@ __global__ void __launch_bounds__(256) smem_group_kernel(DevIndex ix, SmemArgs a
                     if (!regs && l.nCand <= G) {
                         /* the list now fits the group: load it once */
                         if (gl < l.nCand) {
-                            const SmemCandT<Row> cd = candAt(c, l.lo - gl);
+                            const SmemCandT<Row> cd = candAt<true>(c, l.lo - gl);
                             rPos = cd.pos; rP = cd.F; rQ = cd.R; rS = cd.S; rX = cd.X; rLL = cd.LL;
                         }
                         l.nValid = 1;
