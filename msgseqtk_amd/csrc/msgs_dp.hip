@@ -692,15 +692,20 @@ template __global__ void dp_fill_kernel<1>(DevIndex, DpArgs);
  * LDS every step and takes it as the out-of-row value of the row_shr shift that brings the
  * other lanes their upper neighbours; the target base reaches lane l one step after lane l - 1
  * the same way.  Each group runs a small state machine (next job, next rectangle, next strip, step) in
- * one loop, so the groups need not be in the same phase.  The values, decisions and summaries
- * are those of dp_fill_kernel. */
+ * one loop, so the groups need not be in the same phase.  A group claims its next job when it
+ * starts one and fetches that job's number and geometry (into LDS) one step per strip end, so
+ * that a job start rarely waits on that chain of dependent reads.  The values, decisions and
+ * summaries are those of dp_fill_kernel. */
 template <int MODE>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) dp_fill16_kernel(DevIndex ix, DpArgs a) {
     constexpr int W = COLS16 + 17;   /* the row is read up to entry C + rows (<= COLS16 + 16) */
     __shared__ int4 sSG[4][NS16];
     __shared__ int32_t sTM[4][W], sTI[4][W], sTD[4][W];
     __shared__ uint8_t sTg[4][W];
+    __shared__ JobGeo sNextG[4];   /* the next job's geometry, fetched ahead */
     const int lane = threadIdx.x, grp = lane >> 4, l = lane & 15;
+    constexpr int GEO_WORDS = (int)(sizeof(JobGeo) / 4);
+    static_assert(GEO_WORDS <= 32, "a job's geometry is read as two words per lane");
     int4* const SG = sSG[grp];
     int32_t* const TM = sTM[grp];
     int32_t* const TI = sTI[grp];
@@ -730,19 +735,59 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) dp
     DecWord<MODE>* dw = nullptr;
     int32_t bv = NEG, bi = 0, bj = 0, sbv = NEG, sbj = 0;   /* LOCAL: best cell (lane's rows) */
     int32_t nv = NEG, nc = 0, nst = OP_M;                   /* GLOBAL: scan of R3's last row */
+    /* the next job's fetch: stage 0 nothing claimed, 1 queue index claimed (nkRaw, in the
+     * group's lane 0), 2 job number read (nJob, -1: queue exhausted), 3 geometry words read
+     * (gw0, gw1: words l and l + 16), 4 geometry in sNextG */
+    int pst = 0;
+    unsigned int nkRaw = 0;
+    int64_t nJob = -1;
+    uint32_t gw0 = 0, gw1 = 0;
+    const uint32_t* const geoWords = reinterpret_cast<const uint32_t*>(a.geo);
+    uint32_t* const nextWords = reinterpret_cast<uint32_t*>(&sNextG[grp]);
 
     while (__any(phase != DONE)) {
+        /* ---- the next job's fetch advances one stage at each strip end ---- */
+        if (phase == STRIP || phase == RECT) {
+            if (pst == 1) {
+                const unsigned int k = __shfl(nkRaw, grp * 16, 64);
+                nJob = (int64_t)k < a.nWave16Jobs ? a.wave16Jobs[k] : -1;
+                pst = 2;
+            } else if (pst == 2) {
+                if (nJob >= 0) {
+                    gw0 = l < GEO_WORDS ? geoWords[nJob * GEO_WORDS + l] : 0u;
+                    gw1 = l + 16 < GEO_WORDS ? geoWords[nJob * GEO_WORDS + l + 16] : 0u;
+                }
+                pst = 3;
+            } else if (pst == 3) {
+                if (nJob >= 0) {
+                    if (l < GEO_WORDS) nextWords[l] = gw0;
+                    if (l + 16 < GEO_WORDS) nextWords[l + 16] = gw1;
+                }
+                pst = 4;
+            }
+        }
         /* ---- control, per group: until it is stepping a strip or out of jobs ---- */
         while (phase == FETCH || phase == RECT || phase == STRIP) {
             if (phase == FETCH) {
-                unsigned int k = 0;
-                if (l == 0) k = atomicAdd(a.fill16Next, 1u);
-                k = __shfl(k, grp * 16, 64);
-                if ((int64_t)k >= a.nWave16Jobs) {
+                /* this job: from the fetch ahead, finished here from the stage it reached */
+                if (pst == 0) {
+                    if (l == 0) nkRaw = atomicAdd(a.fill16Next, 1u);
+                    pst = 1;
+                }
+                if (pst == 1) {
+                    const unsigned int k = __shfl(nkRaw, grp * 16, 64);
+                    nJob = (int64_t)k < a.nWave16Jobs ? a.wave16Jobs[k] : -1;
+                }
+                const int64_t jb = nJob;
+                if (jb >= 0) g = pst == 4 ? sNextG[grp] : a.geo[jb];
+                pst = 0;
+                if (jb < 0) {
                     phase = DONE;
                 } else {
-                    job = a.wave16Jobs[k];
-                    g = a.geo[job];
+                    job = jb;
+                    /* claim the job after this one; its index is read at the next strip end */
+                    if (l == 0) nkRaw = atomicAdd(a.fill16Next, 1u);
+                    pst = 1;
                     for (int x = l; x < g.ns; x += 16) SG[x] = a.auxSG[g.cs + x];
                     makeGeo(G, a, g, SG, job);
                     q = ((g.strand == FWD) ? a.seqs : a.rcSeqs) + a.offs[g.rid];
