@@ -400,12 +400,13 @@ __global__ void __launch_bounds__(256) dp_setup_kernel(DevIndex ix, DpArgs a) {
     }
     /* shape counters of the 16-lane jobs' rectangles (when a.dstat is set) */
     unsigned long long nRect = 0, nCell = 0, nLaneStep = 0, nStrip = 0;
+    unsigned long long jRect = 0, jCell = 0, jLaneStep = 0, jStrip = 0;   /* of the current job */
     auto shape = [&](int64_t rows, int64_t cols) {
         const int64_t strips = (rows + 15) >> 4;
-        nRect += 1;
-        nCell += (unsigned long long)(rows * cols);
-        nLaneStep += (unsigned long long)(16 * (strips * (cols - 1) + rows));
-        nStrip += (unsigned long long)strips;
+        jRect += 1;
+        jCell += (unsigned long long)(rows * cols);
+        jLaneStep += (unsigned long long)(16 * (strips * (cols - 1) + rows));
+        jStrip += (unsigned long long)strips;
     };
     for (int64_t job = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; job < a.nJobs;
          job += (int64_t)gridDim.x * blockDim.x) {
@@ -435,13 +436,15 @@ __global__ void __launch_bounds__(256) dp_setup_kernel(DevIndex ix, DpArgs a) {
          * height of the decision layout */
         bool anyRect = g.r5r1 >= 1 && g.r5c1 >= 1;
         int32_t maxCols = anyRect ? g.r5c1 : 0;
+        jRect = 0; jCell = 0; jLaneStep = 0; jStrip = 0;
+        if (anyRect) shape(g.r5r1, g.r5c1);
         for (int k = 0; k + 1 < ns; ++k) {
             const SeedRec& s = seeds[idx[k]];
             const SeedRec& t = seeds[idx[k + 1]];
             const int64_t rows = t.from - s.to, cols = t.start - s.end;
-            if (rows > 0 && cols > 0) { anyRect = true; maxCols = max(maxCols, (int32_t)cols); }
+            if (rows > 0 && cols > 0) { anyRect = true; maxCols = max(maxCols, (int32_t)cols); shape(rows, cols); }
         }
-        if (g.r3r0 <= g.L) { anyRect = true; maxCols = max(maxCols, g.tLen - g.r3c0 + 1); }
+        if (g.r3r0 <= g.L) { anyRect = true; maxCols = max(maxCols, g.tLen - g.r3c0 + 1); shape(g.L - g.r3r0 + 1, g.tLen - g.r3c0 + 1); }
         g.cls = !anyRect ? CLS_NONE
                 : a.forceSerial ? CLS_SERIAL
                 : (ns <= NS16 && maxCols <= COLS16) ? CLS_WAVE16
@@ -490,15 +493,7 @@ __global__ void __launch_bounds__(256) dp_setup_kernel(DevIndex ix, DpArgs a) {
             atomicAdd(&a.dstat[1], (unsigned long long)cellsAll);
             atomicMax(&a.dstat[2], (unsigned long long)cellsAll);
         }
-        if (a.dstat && g.cls == CLS_WAVE16) {
-            if (g.r5r1 >= 1 && g.r5c1 >= 1) shape(g.r5r1, g.r5c1);
-            for (int k = 0; k + 1 < ns; ++k) {
-                const int64_t rows = seeds[idx[k + 1]].from - seeds[idx[k]].to;
-                const int64_t cols = seeds[idx[k + 1]].start - seeds[idx[k]].end;
-                if (rows > 0 && cols > 0) shape(rows, cols);
-            }
-            if (g.r3r0 <= g.L) shape(g.L - g.r3r0 + 1, g.tLen - g.r3c0 + 1);
-        }
+        if (g.cls == CLS_WAVE16) { nRect += jRect; nCell += jCell; nLaneStep += jLaneStep; nStrip += jStrip; }
         a.geo[job] = g;
         a.jobCls[job] = g.cls;
         const bool wave = g.cls == CLS_WAVE || g.cls == CLS_WAVE16;
