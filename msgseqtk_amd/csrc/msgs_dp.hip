@@ -503,17 +503,21 @@ __global__ void __launch_bounds__(256) dp_setup_kernel(DevIndex ix, DpArgs a) {
         a.serialNeed[job] = cellsAll;
     }
     if (a.dstat) {
+        /* per wavefront, then per block: one set of atomics per block */
+        __shared__ unsigned long long red[4][4];
         for (int off = 32; off >= 1; off >>= 1) {
             nRect += __shfl_xor(nRect, off, 64);
             nCell += __shfl_xor(nCell, off, 64);
             nLaneStep += __shfl_xor(nLaneStep, off, 64);
             nStrip += __shfl_xor(nStrip, off, 64);
         }
-        if ((threadIdx.x & 63) == 0 && nRect) {
-            atomicAdd(&a.dstat[4], nRect);
-            atomicAdd(&a.dstat[5], nCell);
-            atomicAdd(&a.dstat[6], nLaneStep);
-            atomicAdd(&a.dstat[7], nStrip);
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) { red[w][0] = nRect; red[w][1] = nCell; red[w][2] = nLaneStep; red[w][3] = nStrip; }
+        __syncthreads();
+        if (threadIdx.x < 4) {
+            unsigned long long v = 0;
+            for (int k = 0; k < (int)(blockDim.x >> 6); ++k) v += red[k][threadIdx.x];
+            if (v) atomicAdd(&a.dstat[4 + threadIdx.x], v);
         }
     }
 }
