@@ -1018,8 +1018,18 @@ __global__ void __launch_bounds__(256, 4) dp_trace_kernel(DevIndex ix, DpArgs a)
         sQG[t] = a.qGapOpen[t];
     }
     __syncthreads();
-    for (int64_t job = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; job < a.nJobs;
-         job += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < a.nJobs;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        /* jobs by class and area, largest first: serial, one-wavefront, 16-lane, none, so that
+         * the lanes of a wavefront trace jobs of similar shape */
+        int64_t job = idx;
+        if (a.serialJobs) {
+            int64_t x = idx;
+            if (x < a.nSerialJobs) job = a.serialJobs[x];
+            else if ((x -= a.nSerialJobs) < a.nWaveJobs) job = a.waveJobs[x];
+            else if ((x -= a.nWaveJobs) < a.nWave16Jobs) job = a.wave16Jobs[x];
+            else job = a.noneJobs[x - a.nWave16Jobs];
+        }
         const JobGeo g = a.geo[job];
         int4* sgw = a.auxSG + g.cs;
         Geo G;

@@ -192,6 +192,11 @@ struct DpArgs {
     int64_t nWaveJobs;
     const int64_t* wave16Jobs; /* jobs filled 16 lanes each, largest first */
     int64_t nWave16Jobs;
+    const int64_t* serialJobs; /* serially filled jobs, largest first (NULL: the trace takes jobs
+                                * in job order) */
+    int64_t nSerialJobs;
+    const int64_t* noneJobs;   /* jobs without a rectangle */
+    int64_t nNoneJobs;
     unsigned int* fill16Next;  /* work counter of the 16-lane fill (zero before its launch) */
     const int64_t* tileOff;    /* per job byte offset into the decision pool */
     uint8_t* bits;             /* decisions of wavefront-filled jobs, per 64-row strip of a rectangle:

@@ -350,11 +350,13 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     int64_t* dSerialOff = c->x().bufs.get<int64_t>("dp.soff", nJobs + 1, e);
     int64_t* dWave = c->x().bufs.get<int64_t>("dp.wave", nJobs + 1, e);
     int64_t* dWave16 = c->x().bufs.get<int64_t>("dp.wave16", nJobs + 1, e);
-    unsigned long long* dHist = c->x().bufs.get<unsigned long long>("dp.whist", 256, e);
+    int64_t* dSerialJobs = c->x().bufs.get<int64_t>("dp.serjobs", nJobs + 1, e);
+    int64_t* dNoneJobs = c->x().bufs.get<int64_t>("dp.nonejobs", nJobs + 1, e);
+    unsigned long long* dHist = c->x().bufs.get<unsigned long long>("dp.whist", 512, e);
     int64_t* dTot = c->x().bufs.get<int64_t>("dp.tot", 8, e);
     unsigned int* dNext = c->x().bufs.get<unsigned int>("dp.next16", 1, e);
     if (e != hipSuccess) return fail(c, MSGS_E_NOMEM, "dp scans");
-    CK(hipMemsetAsync(dHist, 0, 256 * 8, st), "hist");
+    CK(hipMemsetAsync(dHist, 0, 512 * 8, st), "hist");
     CK(exclusiveSum(scanTmp, scanBytes, a.tileNeed, dTileOff, nJobs + 1, st), "tile scan");
     CK(exclusiveSum(scanTmp, scanBytes, a.pathNeed, dPathOff, nJobs + 1, st), "path scan");
     CK(exclusiveSum(scanTmp, scanBytes, a.borderNeed, dBorderOff, nJobs + 1, st), "border scan");
@@ -365,12 +367,19 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     k_wave_hist<<<nb, 256, 0, st>>>(a.jobCls, a.tileNeed, nJobs, dHist + 128, CLS_WAVE16);
     k_wave_base<<<1, 64, 0, st>>>(dHist + 128, dHist + 192, dTot + 5);
     k_wave_scatter<<<nb, 256, 0, st>>>(a.jobCls, a.tileNeed, nJobs, dHist + 192, dWave16, CLS_WAVE16);
+    /* the trace's order: the other two classes by area as well */
+    k_wave_hist<<<nb, 256, 0, st>>>(a.jobCls, a.serialNeed, nJobs, dHist + 256, CLS_SERIAL);
+    k_wave_base<<<1, 64, 0, st>>>(dHist + 256, dHist + 320, dTot + 6);
+    k_wave_scatter<<<nb, 256, 0, st>>>(a.jobCls, a.serialNeed, nJobs, dHist + 320, dSerialJobs, CLS_SERIAL);
+    k_wave_hist<<<nb, 256, 0, st>>>(a.jobCls, a.tileNeed, nJobs, dHist + 384, CLS_NONE);
+    k_wave_base<<<1, 64, 0, st>>>(dHist + 384, dHist + 448, dTot + 7);
+    k_wave_scatter<<<nb, 256, 0, st>>>(a.jobCls, a.tileNeed, nJobs, dHist + 448, dNoneJobs, CLS_NONE);
     k_copy_i64<<<1, 64, 0, st>>>(dTileOff + nJobs, dTot + 0);
     k_copy_i64<<<1, 64, 0, st>>>(dPathOff + nJobs, dTot + 1);
     k_copy_i64<<<1, 64, 0, st>>>(dBorderOff + nJobs, dTot + 3);
     k_copy_i64<<<1, 64, 0, st>>>(dSerialOff + nJobs, dTot + 4);
     out.tot.resize(8);
-    CK(hipMemcpyAsync(out.tot.data(), dTot, 6 * 8, hipMemcpyDeviceToHost, st), "dp totals");
+    CK(hipMemcpyAsync(out.tot.data(), dTot, 8 * 8, hipMemcpyDeviceToHost, st), "dp totals");
     CK(hipStreamSynchronize(st), "dp setup");
     const int64_t tiles = out.tot[0], paths = out.tot[1], nWave = out.tot[2], borders = out.tot[3],
                   serialCells = out.tot[4], nWave16 = out.tot[5];
@@ -391,6 +400,10 @@ static int runDp(msgs_ctx* c, const DevReads& r, const uint8_t* dQual, const msg
     a.tileOff = dTileOff; a.pathOff = dPathOff; a.borderOff = dBorderOff; a.serialOff = dSerialOff;
     a.waveJobs = dWave; a.nWaveJobs = nWave;
     a.wave16Jobs = dWave16; a.nWave16Jobs = nWave16; a.fill16Next = dNext;
+    if (out.tot[6] + nWave + nWave16 + out.tot[7] == nJobs) {
+        a.serialJobs = dSerialJobs; a.nSerialJobs = out.tot[6];
+        a.noneJobs = dNoneJobs; a.nNoneJobs = out.tot[7];
+    }
     CK(hipEventRecord(c->x().ev2, st), "event");
     if (nWave) {
         /* persistent: as many one-wavefront blocks per CU as are resident at once (blocks
