@@ -74,7 +74,14 @@ __global__ void k_rec_counts(const AlnRec* recs, int64_t nJobs, int64_t* nc, int
  * length of their area (64 buckets, largest bucket first); the order inside a bucket
  * is whatever the atomics give, which changes no result (jobs write disjoint
  * outputs).  hist[64] counts, cur[64] cursors, tot[1] the list length. */
-__device__ __forceinline__ int areaBucket(int64_t area) { return area > 0 ? 64 - __clzll((unsigned long long)area) : 0; }
+/* two buckets per octave of area (the top bit and the one below it), non-decreasing in area,
+ * areas of 2^31 and more sharing the last */
+__device__ __forceinline__ int areaBucket(int64_t area) {
+    if (area <= 0) return 0;
+    const int top = 63 - __clzll((unsigned long long)area);
+    const int half = top > 0 ? (int)((area >> (top - 1)) & 1) : 1;
+    return min(63, 2 * top + half + 1);
+}
 
 __global__ void __launch_bounds__(256) k_wave_hist(const int32_t* cls, const int64_t* area, int64_t nJobs,
                                                     unsigned long long* hist, int32_t want) {
