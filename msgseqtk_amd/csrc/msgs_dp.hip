@@ -896,8 +896,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) dp
         /* ---- steps: as many as the stepping group nearest to its strip end has left, so
          * that no group changes phase inside the loop ---- */
         int32_t rem = phase == STEP ? steps - t : 0x7fffffff;
-        rem = min(min(__shfl(rem, 0, 64), __shfl(rem, 16, 64)), min(__shfl(rem, 32, 64), __shfl(rem, 48, 64)));
-        rem = __builtin_amdgcn_readfirstlane(rem);   /* the same in every lane: scalar loop control */
+        /* each group's value is uniform in its 16 lanes: read lanes 0, 16, 32, 48 into scalars */
+        rem = min(min(__builtin_amdgcn_readlane(rem, 0), __builtin_amdgcn_readlane(rem, 16)),
+                  min(__builtin_amdgcn_readlane(rem, 32), __builtin_amdgcn_readlane(rem, 48)));
         if (rem == 0x7fffffff) continue;   /* no group stepping: all are out of jobs */
         if (phase == STEP) {
             /* lane 0's entries of the row above the strip, from LDS one step ahead: entry t + 1
