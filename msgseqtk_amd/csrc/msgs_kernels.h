@@ -199,9 +199,9 @@ struct DpArgs {
     int64_t nNoneJobs;
     unsigned int* fill16Next;  /* work counter of the 16-lane fill (zero before its launch) */
     const int64_t* tileOff;    /* per job byte offset into the decision pool */
-    uint8_t* bits;             /* decisions of wavefront-filled jobs, per 64-row strip of a rectangle:
-                                * one 64-bit word per row and 16 sweep steps (GLOBAL, 4 bits per cell)
-                                * or 8 (LOCAL, 8 bits per cell); per cell the backtrace's next state
+    uint8_t* bits;             /* decisions of wavefront-filled jobs, per strip (64 or 16 rows) of a
+                                * rectangle: one word per row and 8 sweep steps (GLOBAL: 32-bit words,
+                                * 4 bits per cell; LOCAL: 64-bit words, 8 bits); per cell the backtrace's next state
                                 * from M (bits 0-1), I (bit 2: 1 = stay), D (bit 3: 1 = stay) and, in
                                 * LOCAL mode, M > 0 (bit 4) */
     const int64_t* borderOff;  /* per job offset into the border pool */
