@@ -163,7 +163,9 @@ int msgs_db_chrom_seq(msgs_ctx* ctx, int32_t tid, uint8_t* out, int64_t cap, int
 /* ---- seeding ----
  * SMEM_LIST::findAllSMEMS (src/SMEM.cpp:186-215), one read per lane.  Per read i,
  * counts[i] SMEMs in emission order; rows5 = (from, to, fwdStart, revStart, size).
- * Returns MSGS_E_CAP with *needed set when cap rows are not enough. */
+ * Returns MSGS_E_CAP with *needed set when cap rows are not enough.  On a database with the
+ * wide index layout (index_wide 1), a read of 2^26 - 2 bases or more makes this and every
+ * entry that searches it (map, align, batch runs) return MSGS_E_UNSUPPORTED. */
 int msgs_smem_batch(msgs_ctx* ctx, const msgs_reads* reads, const msgs_params* p, int64_t* counts,
                     int64_t* rows5, double* logP, int64_t cap, int64_t* needed);
 /* SMEM_LIST::findMEMS (src/SMEM.cpp:174-184, SMEM::findMEM :42-62) with p->minLen and
